@@ -1,0 +1,348 @@
+"""ctypes front end of the FP64 oracle (oracle/lpmd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product package never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblpmd_oracle.so")
+
+LJ, FASTLJ, MORSE, BUCKINGHAM, SUTTONCHEN, FINNISSINCLAIR, GUPTA = range(1, 8)
+KIND = {
+    "lennardjones": LJ,
+    "fastlj": FASTLJ,
+    "morse": MORSE,
+    "buckingham": BUCKINGHAM,
+    "suttonchen": SUTTONCHEN,
+    "finnissinclair": FINNISSINCLAIR,
+    "gupta": GUPTA,
+}
+# keyword order of each plugin (plugins/<name>.cc constructors)
+KEYWORDS = {
+    "lennardjones": ("sigma", "epsilon", "cutoff"),
+    "fastlj": ("sigma", "epsilon", "cutoff", "bins"),
+    "morse": ("depth", "a", "re"),
+    "buckingham": ("B1", "B2", "Ro", "cutoff"),
+    "suttonchen": ("e", "a", "n", "m", "c", "cutoff"),
+    "finnissinclair": ("c0", "c1", "c2", "A", "B", "c", "d"),
+    "gupta": ("A", "r0", "p", "B", "q", "cutoff"),
+}
+DEFAULTS = {"fastlj": {"bins": 500.0}}
+
+FORCEFACTOR = 0.00964853
+KIN2EV = 103.64269
+KBOLTZMANN = 8.6173422e-05
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (-O2, no FMA contraction)."""
+    src = os.path.join(_HERE, "lpmd_oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        import subprocess
+
+        subprocess.check_call(
+            ["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-shared", "-o", LIB_PATH, src, "-lm"]
+        )
+    return LIB_PATH
+
+
+class _Conf(C.Structure):
+    _fields_ = [
+        ("n", C.c_long),
+        ("L", C.c_double * 3),
+        ("per", C.c_int * 3),
+        ("x", C.c_void_p),
+        ("v", C.c_void_p),
+        ("a", C.c_void_p),
+        ("z", C.c_void_p),
+        ("m", C.c_void_p),
+    ]
+
+
+class _Pot(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("spc_sum", C.c_int),
+        ("cutoff", C.c_double),
+        ("p", C.c_double * 8),
+        ("bins", C.c_long),
+        ("etable", C.c_void_p),
+        ("fftable", C.c_void_p),
+        ("an", C.c_double),
+        ("am", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, d, l, i = C.c_void_p, C.c_double, C.c_long, C.c_int
+        L.orc_cm_linkedcell.restype = vp
+        L.orc_cm_linkedcell.argtypes = [d, i, i, i, i]
+        L.orc_cm_verletlist.restype = vp
+        L.orc_cm_verletlist.argtypes = [d, l, l]
+        L.orc_cm_minimage.restype = vp
+        L.orc_cm_minimage.argtypes = [d]
+        L.orc_cm_free.argtypes = [vp]
+        L.orc_cm_update.argtypes = [vp, vp]
+        L.orc_cm_grid.argtypes = [vp, vp]
+        L.orc_cm_all_lists.restype = l
+        L.orc_cm_all_lists.argtypes = [vp, vp, i, d, vp, l, vp, vp, vp]
+        L.orc_pot_init.argtypes = [vp, i, vp, i, i]
+        L.orc_pot_free.argtypes = [vp]
+        L.orc_pair_energy.restype = d
+        L.orc_pair_energy.argtypes = [vp, d]
+        L.orc_pair_force.argtypes = [vp, vp, vp]
+        L.orc_rhoij.restype = d
+        L.orc_rhoij.argtypes = [vp, d]
+        L.orc_embed.restype = d
+        L.orc_embed.argtypes = [vp, d]
+        L.orc_metal_pairforce.argtypes = [vp, vp, d, vp]
+        L.orc_metal_manybodies.argtypes = [vp, vp, d, d, d, vp]
+        L.orc_update_forces.argtypes = [vp, vp, i, vp, vp, vp, vp]
+        L.orc_md_run.argtypes = [vp, vp, i, vp, d, l, vp, vp, vp]
+        L.orc_vv_position.argtypes = [vp, d, vp]
+        L.orc_vv_velocity.argtypes = [vp, d, vp]
+        L.orc_displacement.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_fitted_inside.argtypes = [vp, vp, vp]
+        L.orc_kinetic_energy.restype = d
+        L.orc_kinetic_energy.argtypes = [vp]
+        L.orc_temperature.restype = d
+        L.orc_temperature.argtypes = [vp]
+        L.orc_momentum.argtypes = [vp, vp]
+        L.orc_lc_auto_n.restype = i
+        L.orc_lc_auto_n.argtypes = [d, d]
+        L.orc_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(lib().orc_last_error().decode())
+
+
+class Conf:
+    """Atoms in a periodic orthogonal box, FixedSizeParticleSet-style AoS FP64 arrays."""
+
+    def __init__(self, x, L, z=18, mass=None, v=None, per=(1, 1, 1)):
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3).copy()
+        n = self.x.shape[0]
+        self.L = np.asarray(L, dtype=np.float64).reshape(3).copy()
+        self.v = np.zeros((n, 3)) if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3).copy()
+        self.a = np.zeros((n, 3))
+        self.z = np.broadcast_to(np.asarray(z, dtype=np.int32), (n,)).copy()
+        if mass is None:
+            from oracle import elements
+
+            mass = elements.mass_of(self.z)
+        self.m = np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,)).copy()
+        self.per = tuple(per)
+        self._s = _Conf()
+
+    @property
+    def n(self):
+        return self.x.shape[0]
+
+    def struct(self):
+        s = self._s
+        s.n = self.n
+        for q in range(3):
+            s.L[q] = self.L[q]
+            s.per[q] = int(self.per[q])
+        s.x, s.v, s.a, s.z, s.m = _p(self.x), _p(self.v), _p(self.a), _p(self.z), _p(self.m)
+        return C.byref(s)
+
+
+class CellManager:
+    """linkedcell / verletlist / minimum-image neighbour provider of the oracle."""
+
+    def __init__(self, kind: str, cutoff: float, nx=0, ny=0, nz=0, mode="auto", maxneighbors=80, each=5):
+        L = lib()
+        self.kind = kind
+        if kind == "linkedcell":
+            self.h = L.orc_cm_linkedcell(cutoff, nx, ny, nz, 1 if mode in ("auto", "AUTO") else 0)
+        elif kind == "verletlist":
+            self.h = L.orc_cm_verletlist(cutoff, maxneighbors, each)
+        elif kind == "minimumimage":
+            self.h = L.orc_cm_minimage(cutoff)
+        else:
+            raise ValueError(kind)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_cm_free(self.h)
+            self.h = None
+
+    def update(self, conf: Conf):
+        _check(lib().orc_cm_update(self.h, conf.struct()))
+
+    def grid(self):
+        g = np.zeros(5, dtype=np.int64)
+        lib().orc_cm_grid(self.h, _p(g))
+        return g
+
+    def lists(self, conf: Conf, full: bool, rcut: float, cap: int | None = None):
+        """CSR lists (counts, j, r2, rij) in each atom's own list order."""
+        n = conf.n
+        counts = np.zeros(n, dtype=np.int64)
+        cap = cap or max(1024, n * 64)
+        while True:
+            js = np.zeros(cap, dtype=np.int64)
+            r2 = np.zeros(cap)
+            rij = np.zeros((cap, 3))
+            tot = lib().orc_cm_all_lists(self.h, conf.struct(), int(full), rcut, _p(counts), cap, _p(js), _p(r2), _p(rij))
+            if tot == -1:
+                raise OracleError(lib().orc_last_error().decode())
+            if tot >= 0:
+                return counts, js[:tot], r2[:tot], rij[:tot]
+            cap = -tot
+
+
+class Pot:
+    def __init__(self, name: str, s1: int = -1, s2: int = -1, **kw):
+        vals = dict(DEFAULTS.get(name, {}))
+        vals.update(kw)
+        params = np.zeros(8)
+        for q, k in enumerate(KEYWORDS[name]):
+            params[q] = float(vals[k])
+        self.name = name
+        self.s = _Pot()
+        _check(lib().orc_pot_init(C.byref(self.s), KIND[name], _p(params), s1, s2))
+
+    def __del__(self):
+        if getattr(self, "s", None) is not None:
+            lib().orc_pot_free(C.byref(self.s))
+
+    @property
+    def cutoff(self):
+        return self.s.cutoff
+
+    @property
+    def metal(self):
+        return self.s.kind >= SUTTONCHEN
+
+    def pair_energy(self, r):
+        return np.array([lib().orc_pair_energy(C.byref(self.s), float(x)) for x in np.atleast_1d(r)])
+
+    def pair_force(self, rij):
+        rij = np.ascontiguousarray(np.atleast_2d(rij), dtype=np.float64)
+        out = np.zeros_like(rij)
+        for k in range(rij.shape[0]):
+            lib().orc_pair_force(C.byref(self.s), _p(rij[k]), _p(out[k]))
+        return out
+
+    def rhoij(self, r):
+        return np.array([lib().orc_rhoij(C.byref(self.s), float(x)) for x in np.atleast_1d(r)])
+
+    def embed(self, rho):
+        return np.array([lib().orc_embed(C.byref(self.s), float(x)) for x in np.atleast_1d(rho)])
+
+    def metal_forces(self, rij, rhoi, rhoj):
+        """(PairForce, ManyBodies) for each displacement, with r and r-hat formed as
+        metalpotential.cc:146-149 forms them."""
+        rij = np.ascontiguousarray(np.atleast_2d(rij), dtype=np.float64)
+        pf = np.zeros_like(rij)
+        mb = np.zeros_like(rij)
+        for k in range(rij.shape[0]):
+            v = rij[k]
+            r = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+            nrm = np.ascontiguousarray(v / r)
+            lib().orc_metal_pairforce(C.byref(self.s), _p(nrm), r, _p(pf[k]))
+            lib().orc_metal_manybodies(C.byref(self.s), _p(nrm), float(np.atleast_1d(rhoi)[k]), float(np.atleast_1d(rhoj)[k]), r, _p(mb[k]))
+        return pf, mb
+
+
+def _pot_array(pots):
+    arr = (_Pot * max(1, len(pots)))()
+    for k, p in enumerate(pots):
+        arr[k] = p.s
+    return arr
+
+
+def update_forces(conf: Conf, cm: CellManager, pots):
+    """CombinedPotential::UpdateForces: fills conf.a; returns (epot, virial, stress[3,3])."""
+    arr = _pot_array(pots)
+    ep = np.zeros(1)
+    vir = np.zeros(1)
+    S = np.zeros(9)
+    _check(lib().orc_update_forces(conf.struct(), cm.h, len(pots), arr, _p(ep), _p(vir), _p(S)))
+    return float(ep[0]), float(vir[0]), S.reshape(3, 3)
+
+
+def md_run(conf: Conf, cm: CellManager, pots, dt: float, steps: int):
+    """The lpmd application loop with velocityverlet; returns per-step (epot, ekin, virial)."""
+    arr = _pot_array(pots)
+    ep = np.zeros(steps)
+    ek = np.zeros(steps)
+    vr = np.zeros(steps)
+    _check(lib().orc_md_run(conf.struct(), cm.h, len(pots), arr, dt, steps, _p(ep), _p(ek), _p(vr)))
+    return ep, ek, vr
+
+
+def vv_position(conf: Conf, dt: float):
+    aold = np.zeros((conf.n, 3))
+    lib().orc_vv_position(conf.struct(), dt, _p(aold))
+    return aold
+
+
+def vv_velocity(conf: Conf, dt: float, aold: np.ndarray):
+    aold = np.ascontiguousarray(aold, dtype=np.float64)
+    lib().orc_vv_velocity(conf.struct(), dt, _p(aold))
+
+
+def displacement(L, a, b, per=(1, 1, 1)):
+    L = np.asarray(L, dtype=np.float64)
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    per = np.asarray(per, dtype=np.int32)
+    d = np.zeros(3)
+    lib().orc_displacement(_p(L), _p(per), _p(a), _p(b), _p(d))
+    return d
+
+
+def fitted_inside(L, x, per=(1, 1, 1)):
+    L = np.asarray(L, dtype=np.float64)
+    x = np.array(x, dtype=np.float64)
+    per = np.asarray(per, dtype=np.int32)
+    lib().orc_fitted_inside(_p(L), _p(per), _p(x))
+    return x
+
+
+def kinetic_energy(conf: Conf):
+    return lib().orc_kinetic_energy(conf.struct())
+
+
+def temperature(conf: Conf):
+    return lib().orc_temperature(conf.struct())
+
+
+def momentum(conf: Conf):
+    p = np.zeros(3)
+    lib().orc_momentum(conf.struct(), _p(p))
+    return p
+
+
+def lc_auto_n(edge: float, cutoff: float) -> int:
+    return lib().orc_lc_auto_n(edge, cutoff)

@@ -1,0 +1,174 @@
+"""ctypes front end of oracle/_ref/libref_harness.so: the reference's own
+linkedcell, verletlist, potential, crystal3d and velocityverlet plugin code,
+compiled from its sources by oracle/ref.mk.  Test infrastructure only; absent
+unless that recipe has been run (it needs the reference sources)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(_HERE, "_ref")
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "libref_harness.so"))
+
+
+def so(name: str) -> bytes:
+    return os.path.join(REF_DIR, name + ".so").encode()
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(os.path.join(REF_DIR, "libref_harness.so"))
+        vp, d, l, i = C.c_void_p, C.c_double, C.c_long, C.c_int
+        L.ref_last_error.restype = C.c_char_p
+        L.ref_conf_new.restype = vp
+        L.ref_conf_new.argtypes = [l, vp, vp, vp]
+        L.ref_conf_set_positions.argtypes = [vp, vp]
+        L.ref_conf_set_velocities.argtypes = [vp, vp]
+        L.ref_conf_get.argtypes = [vp, vp, vp, vp]
+        L.ref_conf_free.argtypes = [vp]
+        L.ref_cm_open.restype = vp
+        L.ref_cm_open.argtypes = [C.c_char_p, C.c_char_p]
+        L.ref_verlet_open.restype = vp
+        L.ref_verlet_open.argtypes = [C.c_char_p]
+        L.ref_cm_update.argtypes = [vp, vp, i]
+        L.ref_cm_lists.argtypes = [vp, vp, i, d, l, vp, vp, vp, vp, vp]
+        L.ref_cm_close.argtypes = [vp]
+        L.ref_pot_open.restype = vp
+        L.ref_pot_open.argtypes = [C.c_char_p, C.c_char_p]
+        L.ref_pot_cutoff.restype = d
+        L.ref_pot_cutoff.argtypes = [vp]
+        L.ref_pair_eval.argtypes = [vp, l, vp, vp, vp]
+        L.ref_metal_eval.argtypes = [vp, l, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.ref_pot_close.argtypes = [vp]
+        L.ref_crystal.argtypes = [C.c_char_p, C.c_char_p, l, vp, vp]
+        L.ref_vv_run.argtypes = [C.c_char_p, C.c_char_p, vp, vp, l, l, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().ref_last_error().decode())
+
+
+class RefConf:
+    def __init__(self, x, L, z=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        self.n = self.x.shape[0]
+        self.L = np.ascontiguousarray(L, dtype=np.float64)
+        zz = None if z is None else np.ascontiguousarray(np.broadcast_to(z, (self.n,)), dtype=np.int32)
+        self.h = lib().ref_conf_new(self.n, _p(self.L), _p(self.x), None if zz is None else _p(zz))
+        if not self.h:
+            raise RuntimeError(lib().ref_last_error().decode())
+
+    def set_positions(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _check(lib().ref_conf_set_positions(self.h, _p(x)))
+
+    def set_velocities(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(lib().ref_conf_set_velocities(self.h, _p(v)))
+
+    def get(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        a = np.zeros((self.n, 3))
+        _check(lib().ref_conf_get(self.h, _p(x), _p(v), _p(a)))
+        return x, v, a
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_conf_free(self.h)
+            self.h = None
+
+
+class RefCellManager:
+    def __init__(self, name: str, args: str):
+        if name == "verletlist":
+            self.h = lib().ref_verlet_open((args + " debug none").encode())
+        else:
+            self.h = lib().ref_cm_open(so(name), (args + " debug none").encode())
+        if not self.h:
+            raise RuntimeError(lib().ref_last_error().decode())
+
+    def update(self, conf: RefConf, attach=False):
+        _check(lib().ref_cm_update(self.h, conf.h, int(attach)))
+
+    def lists(self, conf: RefConf, full: bool, rcut: float, cap=None):
+        n = conf.n
+        cap = cap or n * 400
+        counts = np.zeros(n, dtype=np.int64)
+        js = np.zeros(cap, dtype=np.int64)
+        r2 = np.zeros(cap)
+        rij = np.zeros((cap, 3))
+        tot = np.zeros(1, dtype=np.int64)
+        _check(lib().ref_cm_lists(self.h, conf.h, int(full), rcut, cap, _p(counts), _p(js), _p(r2), _p(rij), _p(tot)))
+        t = int(tot[0])
+        if t > cap:
+            return self.lists(conf, full, rcut, cap=t)
+        return counts, js[:t], r2[:t], rij[:t]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_cm_close(self.h)
+            self.h = None
+
+
+class RefPot:
+    def __init__(self, name: str, args: str):
+        self.name = name
+        self.h = lib().ref_pot_open(so(name), (args + " debug none").encode())
+        if not self.h:
+            raise RuntimeError(lib().ref_last_error().decode())
+
+    @property
+    def cutoff(self):
+        return lib().ref_pot_cutoff(self.h)
+
+    def pair(self, rij):
+        rij = np.ascontiguousarray(np.atleast_2d(rij), dtype=np.float64)
+        m = rij.shape[0]
+        e = np.zeros(m)
+        f = np.zeros((m, 3))
+        _check(lib().ref_pair_eval(self.h, m, _p(rij), _p(e), _p(f)))
+        return e, f
+
+    def metal(self, rij, rhoi, rhoj):
+        rij = np.ascontiguousarray(np.atleast_2d(rij), dtype=np.float64)
+        m = rij.shape[0]
+        rhoi = np.ascontiguousarray(np.broadcast_to(rhoi, (m,)), dtype=np.float64)
+        rhoj = np.ascontiguousarray(np.broadcast_to(rhoj, (m,)), dtype=np.float64)
+        ep, rho, fr = np.zeros(m), np.zeros(m), np.zeros(m)
+        pf, mb = np.zeros((m, 3)), np.zeros((m, 3))
+        _check(lib().ref_metal_eval(self.h, m, _p(rij), _p(rhoi), _p(rhoj), _p(ep), _p(rho), _p(fr), _p(pf), _p(mb)))
+        return ep, rho, fr, pf, mb
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_pot_close(self.h)
+            self.h = None
+
+
+def crystal(args: str, n: int, L):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    pos = np.zeros((n, 3))
+    _check(lib().ref_crystal(so("crystal3d"), (args + " debug none").encode(), n, _p(L), _p(pos)))
+    return pos
+
+
+def vv_run(args: str, conf: RefConf, cm: RefCellManager, steps: int, acc_tables):
+    acc = np.ascontiguousarray(acc_tables, dtype=np.float64)
+    ncalls = acc.shape[0]
+    _check(lib().ref_vv_run(so("velocityverlet"), (args + " debug none").encode(), conf.h, cm.h, steps, ncalls, _p(acc)))

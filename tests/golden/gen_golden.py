@@ -1,0 +1,158 @@
+"""Write the golden fixtures under tests/golden/ from the reference's own code.
+
+Runs in a container that has the reference sources: `make -f oracle/ref.mk`
+first (a partial build of the reference's plugins, see oracle/ref.mk), then
+
+    python tests/golden/gen_golden.py
+
+Every fixture holds inputs and the reference's outputs for them: neighbour
+lists from plugins/linkedcell.cc and plugins/verletlist.cc, per-pair values of
+the seven potential plugins, crystal3d lattices and velocityverlet
+trajectories.  Large lists are stored as SHA-256 digests of their exact bytes.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import refcheck as R  # noqa: E402
+
+POTS = {
+    # Ar, the workload's LJ parameters
+    "lennardjones": dict(sigma=3.41, epsilon=0.0103, cutoff=8.5),
+    "fastlj": dict(sigma=3.41, epsilon=0.0103, cutoff=8.5, bins=500),
+    # Cu Morse (Girifalco-Weizer style numbers), no cutoff keyword
+    "morse": dict(depth=0.3429, a=1.3588, re=2.866),
+    # Na-Cl short-range Buckingham
+    "buckingham": dict(B1=1254.5, B2=0.0, Ro=0.3249, cutoff=8.0),
+    # Sutton-Chen Cu
+    "suttonchen": dict(e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=7.22),
+    # Finnis-Sinclair (Fe-like numbers)
+    "finnissinclair": dict(c0=1.2371147, c1=-0.3592185, c2=-0.0385607, A=1.828905, B=1.8, c=3.4, d=3.569745),
+    # Gupta Cu (Cleri-Rosato)
+    "gupta": dict(A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=7.22),
+}
+
+
+def args_of(kw):
+    return " ".join(f"{k} {v!r}" if isinstance(v, float) else f"{k} {v}" for k, v in kw.items())
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def fcc(nx, a, symbol="Ar"):
+    n = 4 * nx**3
+    L = np.array([nx * a] * 3)
+    return R.crystal(f"type fcc symbol {symbol} nx {nx} ny {nx} nz {nx}", n, L), L
+
+
+def jitter(x, L, sigma, seed):
+    rng = np.random.default_rng(seed)
+    return np.mod(x + rng.normal(0.0, sigma, x.shape), L)
+
+
+def list_case(name, x, L, cm_name, cm_args, rcut, store_full):
+    rc = R.RefConf(x, L)
+    cm = R.RefCellManager(cm_name, cm_args)
+    cm.update(rc, attach=True)
+    out = {"x": x, "L": L}
+    meta = {"cm": cm_name, "cm_args": cm_args, "rcut": rcut}
+    for full in (1, 0):
+        counts, js, r2, rij = cm.lists(rc, bool(full), rcut)
+        tag = "full" if full else "half"
+        out[f"{tag}_counts"] = counts.astype(np.int32)
+        if store_full:
+            out[f"{tag}_j"] = js.astype(np.int32)
+            out[f"{tag}_r2"] = r2
+        meta[f"{tag}_digest"] = digest(counts.astype(np.int64), js.astype(np.int64), r2, rij)
+    np.savez_compressed(os.path.join(HERE, f"lists_{name}.npz"), **out)
+    return meta
+
+
+def main():
+    meta = {"lists": {}, "pots": {}, "crystal": {}, "vv": {}}
+
+    # ---- neighbour lists ------------------------------------------------
+    x, L = fcc(8, 5.26)  # 2048 Ar, the LJ argon workload's lattice
+    meta["lists"]["ar2048_lc_auto"] = list_case("ar2048_lc_auto", x, L, "linkedcell", "cutoff 8.5 mode auto", 8.5, False)
+    xj = jitter(x, L, 0.05, 11)
+    meta["lists"]["ar2048j_lc_auto"] = list_case("ar2048j_lc_auto", xj, L, "linkedcell", "cutoff 8.5 mode auto", 8.5, False)
+    meta["lists"]["ar2048j_lc_n4"] = list_case("ar2048j_lc_n4", xj, L, "linkedcell", "cutoff 8.5 nx 4 ny 4 nz 4", 8.5, False)
+    meta["lists"]["ar2048j_vl"] = list_case("ar2048j_vl", xj, L, "verletlist", "cutoff 8.5 maxneighbors 200 each 5", 8.5, False)
+    # 256-atom Cu: auto n = 4 with side 2, the wrapped stencil visits cells twice
+    x, L = fcc(4, 3.61)
+    xj = jitter(x, L, 0.01, 12)
+    meta["lists"]["cu256j_lc_dup"] = list_case("cu256j_lc_dup", xj, L, "linkedcell", "cutoff 7.22 mode auto", 7.22, True)
+    # cutoff exactly on the 8th fcc shell (rc = 2a): membership decided by FP64 rounding
+    meta["lists"]["cu256_shell_vl"] = list_case("cu256_shell_vl", x, L, "verletlist", "cutoff 7.22 maxneighbors 200 each 5", 7.22, True)
+    x6, L6 = fcc(6, 3.61)
+    meta["lists"]["cu864_shell_lc"] = list_case("cu864_shell_lc", x6, L6, "linkedcell", "cutoff 7.22 mode auto", 7.22, False)
+    # a ragged random box (different edges) with explicit subdivisions
+    rng = np.random.default_rng(13)
+    Lr = np.array([23.7, 31.1, 19.3])
+    xr = rng.uniform(0, 1, (300, 3)) * Lr
+    meta["lists"]["rand300_lc_explicit"] = list_case("rand300_lc_explicit", xr, Lr, "linkedcell", "cutoff 6.0 nx 5 ny 6 nz 4", 6.0, True)
+    meta["lists"]["rand300_vl"] = list_case("rand300_vl", xr, Lr, "verletlist", "cutoff 6.0 maxneighbors 120 each 5", 6.0, True)
+
+    # ---- potentials ---------------------------------------------------------
+    rng = np.random.default_rng(21)
+    for name, kw in POTS.items():
+        rp = R.RefPot(name, args_of(kw))
+        rmax = min(rp.cutoff, 9.0)
+        rij = rng.normal(0, 1, (256, 3))
+        rij *= (rng.uniform(1.9, rmax, 256) / np.linalg.norm(rij, axis=1))[:, None]
+        rec = {"rij": rij}
+        if name in ("suttonchen", "finnissinclair", "gupta"):
+            rhoi = rng.uniform(2.0, 80.0, 256)
+            rhoj = rng.uniform(2.0, 80.0, 256)
+            ep, rho, fr, pf, mb = rp.metal(rij, rhoi, rhoj)
+            rec.update(rhoi=rhoi, rhoj=rhoj, epair=ep, rho=rho, frho=fr, pf=pf, mb=mb)
+        else:
+            e, f = rp.pair(rij)
+            rec.update(e=e, f=f)
+        np.savez_compressed(os.path.join(HERE, f"pot_{name}.npz"), **rec)
+        meta["pots"][name] = {"params": kw, "cutoff": rp.cutoff}
+
+    # ---- crystal3d -------------------------------------------------------------
+    for nx, a in ((3, 5.26), (8, 5.26), (4, 3.61)):
+        x, L = fcc(nx, a)
+        key = f"fcc{nx}_{a}"
+        np.savez_compressed(os.path.join(HERE, f"crystal_{key}.npz"), x=x, L=L)
+        meta["crystal"][key] = {"nx": nx, "a": a}
+
+    # ---- velocityverlet with replayed accelerations ----------------------------
+    x, L = fcc(3, 5.26)
+    rng = np.random.default_rng(31)
+    n = x.shape[0]
+    v = rng.normal(0, 2e-3, (n, 3))
+    steps = 12
+    acc = rng.normal(0, 1e-3, (2 * steps, n, 3))
+    # one atom pushed across the box edge to exercise FittedInside
+    v[0] = [-0.5, 0.7, -40.0]
+    rc = R.RefConf(x, L)
+    rc.set_velocities(v)
+    cm = R.RefCellManager("linkedcell", "cutoff 8.5 nx 4 ny 4 nz 4")
+    R.vv_run("dt 1.0", rc, cm, steps, acc.reshape(2 * steps, -1))
+    xf, vf, af = rc.get()
+    np.savez_compressed(os.path.join(HERE, "vv_ar108.npz"), x0=x, v0=v, L=L, acc=acc, xf=xf, vf=vf, af=af)
+    meta["vv"]["ar108"] = {"dt": 1.0, "steps": steps}
+
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("wrote", sorted(os.listdir(HERE)))
+
+
+if __name__ == "__main__":
+    main()
