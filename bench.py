@@ -1,0 +1,218 @@
+"""Benchmark of the MI355X LPMD engine.
+
+Default workload (one GPU): Sutton-Chen Cu bulk, fcc 32x32x32 cells = 131072
+atoms, a = 3.61 A, n 9 m 6 e 1.2382e-2 eV c 39.432, rc = 2a = 7.22 A, seeded
+300 K velocities plus 0.01 A Gaussian position noise, Verlet list with a
+0.3 A skin, velocity Verlet dt = 2 fs.  A step is one velocity-Verlet step of
+the whole system (both EAM passes, list rebuilds when the skin is exceeded),
+with the state resident in HBM.
+
+Prints one JSON line (value = atom-steps/s) with the dominant kernel's
+roofline (HIP events on the engine's stream) and a CPU baseline (the FP64
+oracle, a restatement of the reference path, on one host core, timed on a
+bounded sample and extrapolated to one reference step).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "sc_cu_131k": dict(
+        desc="Sutton-Chen Cu fcc 32^3 = 131072 atoms, rc 7.22, skin 0.3, dt 2 fs, 300 K",
+        nx=32, a=3.61, z=29, T=300.0, noise=0.01, dt=2.0, seed=2024,
+        pot=("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22"),
+        cm=("verletlist", "cutoff 7.22 maxneighbors 400 each 5 skin 0.3"),
+        metal=True,
+    ),
+    "lj_4m": dict(
+        desc="LJ liquid fcc 100^3 = 4,000,000 atoms, rho* 0.8442, T* 0.72, rc 2.5 sigma, skin 0.3 sigma, dt* 0.005",
+        nx=100, a=1.679596 * 3.41, z=18, T=0.72 * 0.0103 / 8.6173422e-05, noise=0.0, dt=0.005 * 2161.98, seed=7,
+        pot=("lennardjones", f"sigma 3.41 epsilon 0.0103 cutoff {2.5 * 3.41}"),
+        cm=("verletlist", f"cutoff {2.5 * 3.41} maxneighbors 400 each 5 skin {0.3 * 3.41}"),
+        metal=False,
+    ),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_system(w):
+    import lpmd2_amd as E
+
+    x, L = E.fcc(w["nx"], w["a"])
+    n = x.shape[0]
+    rng = np.random.default_rng(w["seed"])
+    if w["noise"] > 0:
+        x = np.mod(x + rng.normal(0.0, w["noise"], x.shape), L)
+    m = np.full(n, E.ELEMENT_MASS[w["z"]])
+    v = E.maxwell_boltzmann(m, w["T"], seed=w["seed"] + 1)
+    s = E.System(x, L, w["z"], v=v)
+    s.set_cellmanager(*w["cm"])
+    s.append_potential(*w["pot"])
+    return s, x, L, v
+
+
+def cpu_baseline(w, budget_s: float):
+    """The oracle (FP64 restatement of the reference path) on one core, on a
+    bounded sample of the same workload: the fcc box with (nx/2)^3 cells (1/8 of
+    the atoms, same density and parameters).  Timed there: one O(N^2)
+    verletlist rebuild and force calls; extrapolated to the full system (rebuild
+    x N^2, force call x N) and to the reference's step: 2 force calls and a
+    rebuild every `each` = 5 UpdateCell calls, i.e. 0.4 rebuilds per step."""
+    from oracle import oracle as O
+    import lpmd2_amd as E
+
+    nxs = max(2, w["nx"] // 2)
+    x, L = E.fcc(nxs, w["a"])
+    rng = np.random.default_rng(w["seed"])
+    if w["noise"] > 0:
+        x = np.mod(x + rng.normal(0.0, w["noise"], x.shape), L)
+    ns = x.shape[0]
+    n = 4 * w["nx"] ** 3
+    conf = O.Conf(x, L, z=w["z"])
+    kw = dict(zip(w["pot"][1].split()[0::2], (float(v) for v in w["pot"][1].split()[1::2])))
+    pot = O.Pot(w["pot"][0], **kw)
+    cm = O.CellManager("verletlist", pot.cutoff, maxneighbors=400, each=5)
+    t0 = time.perf_counter()
+    cm.update(conf)  # the O(N^2) rebuild
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ncalls = 0
+    while True:
+        O.update_forces(conf, cm, [pot])  # UpdateCell calls 2..: no rebuild under each 5
+        ncalls += 1
+        if time.perf_counter() - t0 > min(10.0, budget_s) or ncalls >= 3:
+            break
+    t_force = (time.perf_counter() - t0) / ncalls
+    scale = n / ns
+    t_step = 2 * t_force * scale + 0.4 * t_build * scale * scale
+    return {
+        "value": n / t_step,
+        "unit": "atom-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle, verletlist each 5, on the {ns}-atom box of the same workload: 1 O(N^2) rebuild "
+                  f"({t_build:.2f} s, scaled x{scale * scale:.0f}) + {ncalls} force calls ({t_force:.3f} s each, "
+                  f"scaled x{scale:.0f}); step = 2 force calls + 0.4 rebuild",
+        "cpu": cpu_model(),
+    }
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="sc_cu_131k", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sampling")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: each box has one MI355X and multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import lpmd2_amd as E
+
+    if E.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    w = WORKLOADS[args.workload]
+    log(f"workload {args.workload}: {w['desc']}")
+    s, x, L, v = make_system(w)
+    n = s.n
+    t0 = time.perf_counter()
+    s.velocityverlet(w["dt"], args.warmup)
+    log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.3f} s; stats {s.stats()}")
+
+    s.profile(True)
+    st0 = s.stats()
+    t0 = time.perf_counter()
+    s.velocityverlet(w["dt"], args.steps)
+    dt_wall = time.perf_counter() - t0  # velocityverlet returns after the stream is synchronised
+    st1 = s.stats()
+    prof = s.profile_read()
+    s.profile(False)
+    value = n * args.steps / dt_wall
+    ms_per_step = 1e3 * dt_wall / args.steps
+    log(f"timed {args.steps} steps: {dt_wall:.4f} s, {value:.4e} atom-steps/s, profile {prof}, stats {st1}")
+
+    # dominant kernel: the force pass (EAM pass 2 or the pair kernel)
+    nbar = st1["list_entries"] / n
+    f_ms, f_n = prof["force"]
+    r_ms, r_n = prof["rho"]
+    if w["metal"]:
+        bytes_force = n * (156 + 4 * nbar)
+        kname = "k_eam_force"
+    else:
+        bytes_force = n * (148 + 4 * nbar)
+        kname = "k_pair_force"
+    avg_force_s = f_ms / max(f_n, 1) / 1e3
+    achieved = bytes_force / avg_force_s / 1e9
+    roofline = {
+        "kernel": kname,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(avg_force_s * 1e6, 2),
+        "launches": f_n,
+        "bytes_per_launch": int(bytes_force),
+        "mean_list_len": round(nbar, 2),
+    }
+    if w["metal"] and r_n:
+        roofline["rho_pass_avg_us"] = round(r_ms / r_n * 1e3, 2)
+    step_bytes = (160 + 8 * nbar) if w["metal"] else (144 + 4 * nbar)
+    out = {
+        "metric": "atom-steps/s plain vs batched/served at <=1e-5 rel force err; p50/p99 latency",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "atoms": n, "path": "plain single-system", "desc": w["desc"]},
+        "roofline": roofline,
+        "step_hbm_fraction": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
+        "list_builds": st1["builds"] - st0["builds"],
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+/*
+ * lpmdx.h -- C ABI of the MI355X force-and-integration engine for LPMD.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Every entry point returns an
+ * int status (LPMDX_OK = 0, negative on error) and leaves a message for
+ * lpmdx_last_error(), following the reference's only FFI precedent
+ * (liblpmd/rust/src/lib.rs:26-73: null checks -> -1, non-finite input -> -2).
+ *
+ * Host arrays are AoS FP64 [3*n] in the reference's FixedSizeParticleSet
+ * layout (liblpmd/lpmd/fixedsizeparticleset.h:33-43); atoms keep the caller's
+ * index order at this boundary whatever order the device keeps internally.
+ *
+ * Each entry point names the reference interface a maintainer's plugin would
+ * forward to it (file:line).  A system is one Configuration (atoms + orthogonal
+ * periodic cell) resident in HBM; calls on one system are serialised on its own
+ * HIP stream, calls on different systems may overlap.
+ */
+#ifndef LPMDX_H
+#define LPMDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LPMDX_ABI_VERSION 1
+
+#if defined(__GNUC__)
+#define LPMDX_API __attribute__((visibility("default")))
+#else
+#define LPMDX_API
+#endif
+
+/* status codes */
+#define LPMDX_OK 0
+#define LPMDX_EINVAL (-1)     /* null pointer, bad size or unknown keyword value */
+#define LPMDX_ENONFINITE (-2) /* non-finite input value */
+#define LPMDX_EDEVICE (-3)    /* no HIP device, or a HIP call failed */
+#define LPMDX_ECAPACITY (-4)  /* a neighbour list reached maxneighbors (verletlist.cc:96-97) */
+#define LPMDX_ESTATE (-5)     /* missing cell manager / potential (configuration.h:59-63) */
+#define LPMDX_ENOMEM (-6)     /* device allocation or HBM budget exceeded */
+
+/* potential kinds (plugins/<name>.cc) */
+#define LPMDX_LENNARDJONES 1
+#define LPMDX_FASTLJ 2
+#define LPMDX_MORSE 3
+#define LPMDX_BUCKINGHAM 4
+#define LPMDX_SUTTONCHEN 5
+#define LPMDX_FINNISSINCLAIR 6
+#define LPMDX_GUPTA 7
+
+/* observables written by lpmdx_observables, SimulationEngine::SetSimulationTags
+ * (liblpmd/simulation/simulationbuilder.cc:84-124) */
+#define LPMDX_OBS_KINETIC 0   /* kinetic-energy (eV) */
+#define LPMDX_OBS_TEMP 1      /* temperature (K) */
+#define LPMDX_OBS_PX 2        /* px, py, pz */
+#define LPMDX_OBS_POTENTIAL 5 /* potential-energy (eV) */
+#define LPMDX_OBS_VIRIAL 6    /* virial (eV) */
+#define LPMDX_OBS_PRESSURE 7  /* pressure (MPa) */
+#define LPMDX_OBS_STRESS 8    /* 9 entries, s_ab = pressfactor/V * stress[a][b] */
+#define LPMDX_NOBS 17
+
+typedef struct lpmdx_system lpmdx_system;
+
+/* Message of the last failed call on this thread. */
+LPMDX_API const char* lpmdx_last_error(void);
+LPMDX_API int lpmdx_abi_version(void);
+/* Number of visible HIP devices (0 on a host without one). */
+LPMDX_API int lpmdx_device_count(int* count);
+
+/* Configuration: SimulationBuilder::CreateFixedOrthogonal
+ * (liblpmd/simulation/simulationbuilder.cc:321-326) with the cell set to
+ * box[q]*e_q.  z[n] are atomic numbers; mass may be NULL, then ElemMass[z]
+ * (liblpmd/support/elements.cc:20-35, BasicAtom::operator=, basicatom.h:57-67). */
+LPMDX_API int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const int* z,
+                        const double* mass);
+LPMDX_API int lpmdx_system_destroy(lpmdx_system* sys);
+LPMDX_API long lpmdx_system_size(const lpmdx_system* sys);
+
+/* Positions / velocities / accelerations, AoS [3n] FP64; any pointer may be
+ * NULL to leave that field.  BasicAtom::Position()/Velocity()/Acceleration()
+ * (liblpmd/lpmd/basicatom.h:37-52). */
+LPMDX_API int lpmdx_system_set_state(lpmdx_system* sys, const double* x, const double* v, const double* a);
+LPMDX_API int lpmdx_system_get_state(lpmdx_system* sys, double* x, double* v, double* a);
+
+/* Cell manager, as Configuration::SetCellManager (configuration.h:54-57) after
+ * PluginManager::LoadPluginAs<CellManager>(name, args) (pluginmanager.h:60):
+ *   "linkedcell": keywords cutoff nx ny nz mode warn-outside
+ *                 (plugins/linkedcell.cc:17-41; same defaults);
+ *   "verletlist": keywords cutoff extcutoff maxneighbors each skin
+ *                 (plugins/verletlist.cc:12-29; `skin`, when given, makes the
+ *                 list radius cutoff+skin and rebuilds only when the skin is
+ *                 exceeded).
+ * Runs one UpdateCell, like SetCellManager does. */
+LPMDX_API int lpmdx_set_cellmanager(lpmdx_system* sys, const char* name, const char* args);
+
+/* CellManager::UpdateCell (liblpmd/lpmd/cellmanager.h:22). */
+LPMDX_API int lpmdx_update_cell(lpmdx_system* sys);
+
+/* CellManager::BuildNeighborList(conf, i, nlist, full, rcut) for every atom i
+ * (cellmanager.h:24-25): counts[n]; while room remains (cap entries) the
+ * neighbour indices js and r2 values, in the order the reference's manager
+ * lists them.  *total receives the number of entries. */
+LPMDX_API int lpmdx_neighbor_lists(lpmdx_system* sys, int full, double rcut, long* counts, long cap,
+                         long* js, double* r2, long* total);
+
+/* CombinedPotential::Append + Potential::SetValidSpecies(s1, s2)
+ * (liblpmd/lpmd/combinedpotential.h, liblpmd/simulation/potential.cc:23), the
+ * potential being LoadPluginAs<Potential>(name, args): lennardjones fastlj
+ * morse buckingham suttonchen finnissinclair gupta with their plugin keywords
+ * (plugins/<name>.cc constructors).  s1 = s2 = -1 is the wildcard. */
+LPMDX_API int lpmdx_potential_append(lpmdx_system* sys, const char* name, const char* args, int s1, int s2);
+LPMDX_API int lpmdx_potential_clear(lpmdx_system* sys);
+
+/* CombinedPotential::UpdateForces (combinedpotential.h:39-52): zeroes the
+ * accelerations, virial and stress, runs UpdateCell and every potential's
+ * UpdateForces (pairpotential.cc:35-84, metalpotential.cc:62-180); returns
+ * the potential energy, virial and 3x3 stress (row-major stress[a][b]). */
+LPMDX_API int lpmdx_update_forces(lpmdx_system* sys, double* epot, double* virial, double stress[9]);
+
+/* The application loop with the velocityverlet integrator (lpmd/lpmd.cc:70-103,
+ * plugins/velocityverlet.cc:55-75, twostepintegrator.cc:12-68): `steps`
+ * DoSteps of length dt (fs).  State stays device-resident between calls.
+ * Every obs_every steps (0 = never) the potential and kinetic energy are
+ * written to epot_t[k], ekin_t[k] (k = step/obs_every - 1; either may be NULL). */
+LPMDX_API int lpmdx_velocityverlet(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
+                         double* ekin_t);
+
+/* Device-side SetSimulationTags (simulationbuilder.cc:84-124, properties.h:22-81):
+ * out[LPMDX_NOBS], see LPMDX_OBS_*.  The potential terms are those of the
+ * last force evaluation. */
+LPMDX_API int lpmdx_observables(lpmdx_system* sys, double out[LPMDX_NOBS]);
+
+/* Counters of the system's engine: [0] list builds, [1] force evaluations,
+ * [2] steps, [3] list capacity per atom, [4] largest list, [5] lanes per atom,
+ * [6] cells along x, [7] total list entries of the last build. */
+LPMDX_API int lpmdx_system_stats(const lpmdx_system* sys, long out[8]);
+
+/* Per-kernel timing with HIP events recorded on the system's own stream:
+ * enable (1) or disable (0); enabling clears the totals.  read returns the
+ * summed milliseconds and launch counts of [0] force pass (pair or EAM),
+ * [1] EAM rho pass, [2] list build (sort + list), [3] other. */
+LPMDX_API int lpmdx_system_profile(lpmdx_system* sys, int enable);
+LPMDX_API int lpmdx_system_profile_read(lpmdx_system* sys, double ms[4], long launches[4]);
+
+/* Lanes per atom used by the force kernels (1, 2, 4, 8 or 16); 0 = automatic. */
+LPMDX_API int lpmdx_system_set_lanes(lpmdx_system* sys, int lanes_per_atom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

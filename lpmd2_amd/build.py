@@ -1,0 +1,42 @@
+"""In-tree build of liblpmdx.so for gfx950 with hipcc (no CMake, no JIT cache)."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(_HERE)
+SOURCES = [os.path.join(_HERE, "csrc", "engine.hip")]
+HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp")] + [
+    os.path.join(ROOT, "include", "lpmdx.h")
+]
+OUT = os.path.join(_HERE, "liblpmdx.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # FP64 geometry must round like the reference (no contraction, no reassociation)
+    "-ffp-contract=off",
+    "-fvisibility=hidden",
+    # no packed FP32 instructions
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+]
+
+
+def build_library(force: bool = True) -> str:
+    if not force and os.path.exists(OUT):
+        newest = max(os.path.getmtime(p) for p in SOURCES + HEADERS)
+        if os.path.getmtime(OUT) >= newest:
+            return OUT
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *SOURCES]
+    subprocess.check_call(cmd)
+    os.replace(tmp, OUT)
+    return OUT

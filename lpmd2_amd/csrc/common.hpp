@@ -1,0 +1,127 @@
+// Shared device helpers for the LPMD engine (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace lpmdx {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+// Unit constants as the reference's potentials and properties see them:
+// forcefactor re-parsed from its 8-decimal session string
+// (liblpmd/runtime/units.cc:6, liblpmd/runtime/session.cc:17-23,31),
+// KIN2EV/KBOLTZMANN macros (liblpmd/lpmd/properties.h:10-11).
+constexpr double kForceFactor = 0.00964853;
+constexpr double kKin2eV = 103.64269;
+constexpr double kBoltzmann = 8.6173422E-05;
+constexpr double kPressFactor = 160217.65;
+
+struct Box {
+  double L[3];
+  double half[3]; // 0.5 * L, as OrthogonalCell::Displacement forms it
+};
+
+// OrthogonalCell::Displacement (liblpmd/lpmd/orthogonalcell.h:131-149): one
+// wrap per axis, >= on the positive side.  Compiled without FP contraction.
+__device__ __forceinline__ void min_image(const Box& b, double& dx, double& dy, double& dz) {
+  if (dx >= b.half[0])
+    dx -= b.L[0];
+  else if (dx < -b.half[0])
+    dx += b.L[0];
+  if (dy >= b.half[1])
+    dy -= b.L[1];
+  else if (dy < -b.half[1])
+    dy += b.L[1];
+  if (dz >= b.half[2])
+    dz -= b.L[2];
+  else if (dz < -b.half[2])
+    dz += b.L[2];
+}
+
+// GenericVector::SquareModule (vector.h:110): (x*x + y*y) + z*z.
+__device__ __forceinline__ double sqmod(double x, double y, double z) {
+  return x * x + y * y + z * z;
+}
+
+// OrthogonalCell::FittedInside (orthogonalcell.h:102-114); x == L stays.
+__device__ __forceinline__ double fitted_inside(double x, double ll) {
+  if (x < 0.0)
+    x -= ll * floor(x / ll);
+  else if (x > ll)
+    x -= ll * floor(x / ll);
+  return x;
+}
+
+// Compensated FP32 accumulator for per-atom force sums (Knuth's branch-free
+// TwoSum; needs the no-reassociation, no-contraction build flags).
+struct CompF {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float v) {
+    float t = s + v;
+    float bp = t - s;
+    c += (s - (t - bp)) + (v - bp);
+    s = t;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+  __device__ __forceinline__ void merge(const CompF& o) {
+    add(o.s);
+    c += o.c;
+  }
+};
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, kWave);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// Sum of NV doubles over a 256-thread block in a fixed order; the result is
+// valid in thread 0.  `sh` needs NV * (kBlock / kWave) doubles.
+template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      sh[w * NV + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < kBlock / kWave; ++q)
+        s += sh[q * NV + k];
+      v[k] = s;
+    }
+  }
+}
+
+// Per-launch control block in device memory.
+constexpr int kMaxChunk = 4096;
+struct Ctrl {
+  unsigned int maxd2[kMaxChunk];  // per step: max |dx|^2 as float bits
+  unsigned int max_count;         // largest list length of the last build
+  unsigned int pad0;
+  unsigned long long sum_count;   // total list entries of the last build (8-byte aligned)
+  unsigned int pad[12];
+};
+static_assert(offsetof(Ctrl, sum_count) % 8 == 0, "64-bit counter must be 8-byte aligned");
+static_assert(offsetof(Ctrl, sum_count) == offsetof(Ctrl, max_count) + 8, "counter block layout");
+
+} // namespace lpmdx

@@ -1,0 +1,1475 @@
+// Host engine and C ABI of the MI355X LPMD force-and-integration path.
+//
+// One lpmdx_system is one Configuration resident in HBM (positions, velocities,
+// accelerations in AoS FP64, internally kept in cell-sorted order), with its
+// cell manager, its potentials and its own HIP stream.  The entry points are
+// declared in include/lpmdx.h, each next to the reference interface it serves.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/lpmdx.h"
+#include "cells.hpp"
+#include "common.hpp"
+#include "forces.hpp"
+#include "sort.hpp"
+
+using namespace lpmdx;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hipchk(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    fail(e == hipErrorOutOfMemory ? LPMDX_ENOMEM : LPMDX_EDEVICE,
+         std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F> int guard(F&& f) {
+  try {
+    f();
+    return LPMDX_OK;
+  } catch (const Error& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return LPMDX_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return LPMDX_EINVAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+
+template <typename T> struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t m) {
+    if (m <= n && p)
+      return;
+    release();
+    if (m == 0)
+      m = 1;
+    hipchk(hipMalloc(&p, m * sizeof(T)), "hipMalloc");
+    n = m;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// plugin keyword strings (Module::ProcessArguments, liblpmd/plugins/module.cc:77-98)
+
+struct Keywords {
+  std::map<std::string, std::string> kv;
+  Keywords(const char* args, std::initializer_list<std::pair<const char*, const char*>> defaults) {
+    for (auto& d : defaults)
+      kv[d.first] = d.second;
+    std::istringstream in(args ? args : "");
+    std::vector<std::string> w;
+    std::string s;
+    while (in >> s)
+      w.push_back(s);
+    // words are taken as key value pairs; a dangling key is the reference's
+    // "Unexpected end of input while reading module arguments" (module.cc:101-104)
+    if (w.size() % 2)
+      fail(LPMDX_EINVAL, "Unexpected end of input while reading module arguments");
+    for (size_t k = 0; k < w.size(); k += 2)
+      kv[w[k]] = w[k + 1];
+  }
+  // Parameter conversions, liblpmd/lpmd/parameter.h:26-28: atof / atoi / "true";
+  // a keyword without default and without value reads as "".
+  double d(const char* k) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? atof("") : atof(it->second.c_str());
+  }
+  int i(const char* k) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? atoi("") : atoi(it->second.c_str());
+  }
+  bool b(const char* k) const {
+    auto it = kv.find(k);
+    return it != kv.end() && it->second == "true";
+  }
+  std::string s(const char* k) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? std::string() : it->second;
+  }
+  bool has(const char* k) const { return kv.count(k) > 0; }
+};
+
+// ---------------------------------------------------------------------------
+// potentials
+
+struct PotParams {
+  int kind = 0;
+  int spc_sum = -2;
+  double cutoff = HUGE_VAL; // Potential::rcutoff default (potential.cc:13)
+  double p[8] = {0};
+  long bins = 0;
+  DBuf<float> tab_e, tab_f;
+  bool metal() const { return kind >= kSuttonChen; }
+};
+
+std::unique_ptr<PotParams> make_potential(const std::string& name, const char* args, int s1, int s2) {
+  auto P = std::make_unique<PotParams>();
+  P->spc_sum = s1 + s2;
+  if (name == "lennardjones") { // plugins/lennardjones.cc:11-22
+    Keywords k(args, {});
+    P->kind = kLJ;
+    P->p[0] = k.d("sigma");
+    P->p[1] = k.d("epsilon");
+    P->cutoff = P->p[2] = k.d("cutoff");
+  } else if (name == "fastlj") { // plugins/fastlj.cc:11-26,64-80
+    Keywords k(args, {{"bins", "500"}});
+    P->kind = kFastLJ;
+    P->p[0] = k.d("sigma");
+    P->p[1] = k.d("epsilon");
+    P->cutoff = P->p[2] = k.d("cutoff");
+    P->bins = k.i("bins");
+    if (P->bins <= 0)
+      fail(LPMDX_EINVAL, "fastlj: bins must be positive");
+    std::vector<float> e(P->bins), f(P->bins);
+    const double sigma = P->p[0], epsilon = P->p[1], cutoff = P->p[2];
+    for (long i = 0; i < P->bins; ++i) {
+      double r = cutoff * double(i) / double(P->bins);
+      double r6 = pow(sigma / r, 6.0e0);
+      double r12 = r6 * r6;
+      e[i] = (float)(4.0e0 * epsilon * (r12 - r6));
+      f[i] = (float)(-48.0e0 * (epsilon / (r * r)) * (r12 - 0.50e0 * r6));
+    }
+    P->tab_e.alloc(P->bins);
+    P->tab_f.alloc(P->bins);
+    hipchk(hipMemcpy(P->tab_e.p, e.data(), P->bins * sizeof(float), hipMemcpyHostToDevice), "table");
+    hipchk(hipMemcpy(P->tab_f.p, f.data(), P->bins * sizeof(float), hipMemcpyHostToDevice), "table");
+  } else if (name == "morse") { // plugins/morse.cc:11-21 (no cutoff keyword)
+    Keywords k(args, {});
+    P->kind = kMorse;
+    P->p[0] = k.d("depth");
+    P->p[1] = k.d("a");
+    P->p[2] = k.d("re");
+  } else if (name == "buckingham") { // plugins/buckingham.cc:11-24
+    Keywords k(args, {});
+    P->kind = kBuckingham;
+    P->p[0] = k.d("B1");
+    P->p[1] = k.d("B2");
+    P->p[2] = k.d("Ro");
+    P->cutoff = P->p[3] = k.d("cutoff");
+  } else if (name == "suttonchen") { // plugins/suttonchen.cc:11-32
+    Keywords k(args, {{"corrections", "false"}});
+    P->kind = kSuttonChen;
+    P->p[0] = k.d("e");
+    P->p[1] = k.d("a");
+    P->p[2] = k.d("n");
+    P->p[3] = k.d("m");
+    P->p[4] = k.d("c");
+    P->cutoff = P->p[5] = k.d("cutoff");
+  } else if (name == "finnissinclair") { // plugins/finnissinclair.cc:10-31
+    Keywords k(args, {});
+    P->kind = kFinnisSinclair;
+    const char* keys[7] = {"c0", "c1", "c2", "A", "B", "c", "d"};
+    for (int q = 0; q < 7; ++q)
+      P->p[q] = k.d(keys[q]);
+    P->cutoff = (P->p[6] >= P->p[5]) ? P->p[6] : P->p[5];
+  } else if (name == "gupta") { // plugins/gupta.cc:10-27
+    Keywords k(args, {});
+    P->kind = kGupta;
+    const char* keys[5] = {"A", "r0", "p", "B", "q"};
+    for (int q = 0; q < 5; ++q)
+      P->p[q] = k.d(keys[q]);
+    P->cutoff = P->p[5] = k.d("cutoff");
+  } else {
+    fail(LPMDX_EINVAL, "unknown potential plugin '" + name + "'");
+  }
+  for (int q = 0; q < 8; ++q)
+    if (!std::isfinite(P->p[q]))
+      fail(LPMDX_ENONFINITE, name + ": non-finite parameter");
+  return P;
+}
+
+int int_exponent(double v) {
+  double r = std::nearbyint(v);
+  return (r == v && r >= 1 && r <= 64) ? (int)r : 0;
+}
+
+DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) {
+  DevPot d{};
+  d.kind = P.kind;
+  d.spc_sum = P.spc_sum;
+  d.rc2 = rc2_force;
+  const double* p = P.p;
+  switch (P.kind) {
+  case kLJ:
+    d.f[0] = (float)(p[0] * p[0]);
+    d.f[1] = (float)(48.0 * p[1]);
+    d.f[2] = (float)(4.0 * p[1]);
+    break;
+  case kFastLJ:
+    d.tab_e = P.tab_e.p;
+    d.tab_f = P.tab_f.p;
+    d.tab_cut = p[2];
+    d.bins = (int)P.bins;
+    break;
+  case kMorse:
+    d.f[0] = (float)p[1];
+    d.f[1] = (float)p[2];
+    d.f[2] = (float)p[0];
+    d.f[3] = (float)(2.0 * p[1] * p[0]);
+    break;
+  case kBuckingham:
+    d.f[0] = (float)p[0];
+    d.f[1] = (float)p[1];
+    d.f[2] = (float)p[2];
+    d.f[3] = (float)(1.0 / p[2]);
+    d.f[4] = (float)(6.0 * p[1]);
+    break;
+  case kSuttonChen:
+    for (int q = 0; q < 5; ++q)
+      d.f[q] = (float)p[q];
+    d.ni = int_exponent(p[2]);
+    d.mi = int_exponent(p[3]);
+    d.f[11] = (float)(p[4] * p[0]);
+    break;
+  case kGupta:
+    for (int q = 0; q < 5; ++q)
+      d.f[q] = (float)p[q];
+    d.f[11] = (float)p[3];
+    break;
+  case kFinnisSinclair:
+    for (int q = 0; q < 7; ++q)
+      d.f[q] = (float)p[q];
+    d.f[11] = (float)p[3];
+    break;
+  }
+  d.tab_cut = P.kind == kFastLJ ? p[2] : 0.0;
+  // rho pass filter travels in the (otherwise unused) FP64 slot below
+  (void)rc2_rho;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// cell managers
+
+enum CmKind { kNone = 0, kCmLinkedCell = 1, kCmVerlet = 2 };
+
+struct CellMgr {
+  int kind = kNone;
+  double cutoff = 0.0;
+  // linkedcell (plugins/linkedcell.cc:17-41)
+  int nx = 0, ny = 0, nz = 0, side = 0;
+  bool mode_auto = false, warn_outside = true, grid_fixed = false;
+  // verletlist (plugins/verletlist.cc:12-29)
+  double extcut = 0.0, skin = -1.0;
+  long maxneighbors = 80, each = 5;
+  long ucalls = 0;
+};
+
+// linkedcell.cc:146-165
+int lc_auto_n(double edge, double cutoff) {
+  double fnn = (cutoff / 2.5);
+  int n = 1;
+  while (true) {
+    double previo = fabs(edge / n - fnn);
+    n++;
+    double actual = fabs(edge / n - fnn);
+    if (actual < 1E-3)
+      break;
+    else if (actual > previo) {
+      n--;
+      break;
+    }
+  }
+  if ((n % 2) != 0)
+    n--;
+  return n;
+}
+
+double edge_module(double L) { return sqrt(L * L + 0.0 * 0.0 + 0.0 * 0.0); }
+
+double ElemMass(int z) {
+  // liblpmd/support/elements.cc:20-35 for the species the engine is used with
+  switch (z) {
+  case 1: return 1.00794;
+  case 11: return 22.989768;
+  case 13: return 26.981539;
+  case 17: return 35.4527;
+  case 18: return 39.948;
+  case 26: return 55.847;
+  case 28: return 58.6934;
+  case 29: return 63.546;
+  case 47: return 107.8682;
+  case 79: return 196.96654;
+  default: return -1.0;
+  }
+}
+
+long blocks_for(long threads) { return (threads + kBlock - 1) / kBlock; }
+
+// timing slots of lpmdx_system_profile_read
+enum ProfSlot { kProfForce = 0, kProfRho = 1, kProfBuild = 2, kProfOther = 3, kProfSlots = 4 };
+
+// max |x - xb|^2 over atoms through the minimum image (xb: positions at the build)
+__global__ __launch_bounds__(kBlock) void k_disp(ForceArgs A) {
+  long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  float d2 = 0.0f;
+  if (i < A.n) {
+    double dx = A.x[3 * i] - A.xn[3 * i];
+    double dy = A.x[3 * i + 1] - A.xn[3 * i + 1];
+    double dz = A.x[3 * i + 2] - A.xn[3 * i + 2];
+    min_image(A.b, dx, dy, dz);
+    d2 = (float)sqmod(dx, dy, dz);
+  }
+  float m = wave_max(d2);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m > 0.0f)
+    atomicMax(A.maxd2, __float_as_uint(m));
+}
+
+__global__ void k_fill_double(double* p, long m, double v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m)
+    p[i] = v;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// the system
+
+struct lpmdx_system {
+  long n = 0;
+  Box box{};
+  double ll[3] = {0, 0, 0};
+  hipStream_t st = nullptr;
+  // state, internal order
+  DBuf<double> x, xn, v, a, ffm, mass, asum, xb;
+  DBuf<int> z, perm, iperm;
+  // scratch for re-ordering
+  DBuf<double> x2, v2, a2, ffm2, mass2;
+  DBuf<int> z2, perm2, iperm2;
+  // snapshot for the skin check roll-back
+  DBuf<double> xs, vs, as;
+  // sort
+  DBuf<uint32_t> keys, vals, keys2, vals2, hist;
+  // cells
+  DBuf<int> cstart, cend;
+  Grid grid{};
+  // lists
+  DBuf<int> nbr, cnt;
+  long stride = 0;
+  int cap = 0;
+  int lpa = 1, lpa_req = 0;
+  double rl = 0.0; // list radius
+  // EAM
+  DBuf<double> w_rho, rho;
+  // reductions
+  DBuf<double> partials, partials2, red;
+  DBuf<Ctrl> ctrl;
+  Ctrl* hctrl = nullptr; // pinned
+  double* hred = nullptr;
+  // host-side model
+  CellMgr cm;
+  std::vector<std::unique_ptr<PotParams>> pots;
+  bool list_built = false;
+  bool forces_ok = false; // a = F(x) for the user-visible positions
+  bool drifted = false;   // user positions are in xn; x = drift of them
+  double disp_acc = 0.0;  // displacement bound since the list build
+  double rate = 0.0;      // largest per-step displacement seen
+  double last_e[12] = {0};
+  bool last_e_valid = false;
+  long n_builds = 0, n_forces = 0, n_steps = 0, max_list = 0, sum_list = 0;
+  // optional per-kernel timing with HIP events on this system's stream
+  bool profiling = false;
+  struct Pending {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double prof_ms[kProfSlots] = {0};
+  long prof_n[kProfSlots] = {0};
+
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hipchk(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  // brackets work enqueued between begin() and end() on the stream
+  hipEvent_t prof_begin() {
+    if (!profiling)
+      return nullptr;
+    hipEvent_t e = take_event();
+    hipchk(hipEventRecord(e, st), "hipEventRecord");
+    return e;
+  }
+  void prof_end(int slot, hipEvent_t a) {
+    if (!profiling || !a)
+      return;
+    hipEvent_t e = take_event();
+    hipchk(hipEventRecord(e, st), "hipEventRecord");
+    pending.push_back({slot, a, e});
+  }
+  void prof_collect() {
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      hipchk(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      prof_ms[p.slot] += ms;
+      prof_n[p.slot] += 1;
+      event_pool.push_back(p.a);
+      event_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+
+  ~lpmdx_system() {
+    if (st)
+      (void)hipStreamSynchronize(st);
+    for (auto& p : pending) {
+      event_pool.push_back(p.a);
+      event_pool.push_back(p.b);
+    }
+    for (auto e : event_pool)
+      (void)hipEventDestroy(e);
+    if (hctrl)
+      (void)hipHostFree(hctrl);
+    if (hred)
+      (void)hipHostFree(hred);
+    pots.clear();
+    if (st)
+      (void)hipStreamDestroy(st);
+  }
+
+  void sync() {
+    hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!pending.empty())
+      prof_collect();
+  }
+
+  void launch_check() { hipchk(hipGetLastError(), "kernel launch"); }
+
+  // ------------------------------------------------------------------ lists
+  // Effective filter of each potential, following how the reference's managers
+  // serve a potential: linkedcell caches each atom's half (full) list with the
+  // rcut of the first request after UpdateCell (linkedcell.cc:315-354);
+  // verletlist filters its stored list with the caller's rcut (verletlist.cc:151,163).
+  double half_list_rcut() const { return pots.empty() ? cm.cutoff : pots[0]->cutoff; }
+  double full_list_rcut() const {
+    for (auto& p : pots)
+      if (p->metal())
+        return p->cutoff;
+    return half_list_rcut();
+  }
+
+  double list_radius() const {
+    if (cm.kind == kCmLinkedCell)
+      return std::max(half_list_rcut(), full_list_rcut());
+    return cm.skin >= 0.0 ? cm.cutoff + cm.skin : cm.extcut;
+  }
+
+  // Exactness budget of the Verlet list: half the gap between the list radius
+  // and the largest finite filter radius below it.
+  double budget() const {
+    double rmax = 0.0;
+    for (auto& p : pots)
+      if (p->cutoff < rl)
+        rmax = std::max(rmax, p->cutoff);
+    if (pots.empty())
+      rmax = cm.cutoff;
+    return 0.5 * (rl - rmax);
+  }
+
+  void setup_grid() {
+    if (cm.kind == kCmLinkedCell) {
+      if (!cm.grid_fixed) {
+        // UpdateCell's first call fixes nx ny nz and the stencil (linkedcell.cc:146-241)
+        if (cm.mode_auto) {
+          cm.nx = lc_auto_n(edge_module(box.L[0]), cm.cutoff);
+          cm.ny = lc_auto_n(edge_module(box.L[1]), cm.cutoff);
+          cm.nz = lc_auto_n(edge_module(box.L[2]), cm.cutoff);
+        }
+        if (cm.nx <= 0 || cm.ny <= 0 || cm.nz <= 0)
+          fail(LPMDX_EINVAL, "linkedcell: nx, ny and nz must be positive (use mode auto or set them)");
+        double d = edge_module(box.L[0]) / double(cm.nx);
+        if (edge_module(box.L[1]) / double(cm.ny) < d)
+          d = edge_module(box.L[1]) / double(cm.ny);
+        if (edge_module(box.L[2]) / double(cm.nz) < d)
+          d = edge_module(box.L[2]) / double(cm.nz);
+        cm.side = int(ceil((cm.cutoff / d) - 0.5));
+        cm.grid_fixed = true;
+      }
+      grid.mode = kLinkedCell;
+      grid.nx = cm.nx;
+      grid.ny = cm.ny;
+      grid.nz = cm.nz;
+      grid.side = cm.side;
+    } else {
+      // cells no smaller than the list radius (with a margin for rounding)
+      double r = rl * (1.0 + 1e-9);
+      int nq[3];
+      for (int q = 0; q < 3; ++q) {
+        double m = std::isfinite(r) && r > 0 ? std::floor(box.L[q] / r) : 1.0;
+        nq[q] = (int)std::max(1.0, std::min(m, 1024.0));
+      }
+      grid.mode = kExact;
+      grid.nx = nq[0];
+      grid.ny = nq[1];
+      grid.nz = nq[2];
+      grid.side = 1;
+    }
+    long nc = (long)grid.nx * grid.ny * grid.nz;
+    if (nc > (1L << 28))
+      fail(LPMDX_EINVAL, "cell grid too large");
+    grid.ncell = (int)nc;
+  }
+
+  int auto_lanes() const {
+    if (lpa_req)
+      return lpa_req;
+    // keep >= ~16 waves per CU on 256 CUs
+    if (n >= 2000000)
+      return 1;
+    if (n >= 500000)
+      return 2;
+    if (n >= 120000)
+      return 4;
+    return 8;
+  }
+
+  void sort_atoms() {
+    const long nb = blocks_for(n);
+    k_cell_keys<<<nb, kBlock, 0, st>>>(x.p, perm.p, n, grid, box, keys.p, vals.p);
+    launch_check();
+    int bits = 0;
+    while (bits < 32 && (uint32_t)grid.ncell >> bits)
+      ++bits;
+    const int passes = std::max(1, (bits + 3) / 4);
+    const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
+    hist.alloc((size_t)kDigits * ntiles);
+    uint32_t *ka = keys.p, *va = vals.p, *kb = keys2.p, *vb = vals2.p;
+    for (int ps = 0; ps < passes; ++ps) {
+      const int shift = 4 * ps;
+      k_sort_count<<<ntiles, kBlock, 0, st>>>(ka, n, shift, hist.p, ntiles);
+      k_scan_excl<<<1, 1024, 0, st>>>(hist.p, (long)kDigits * ntiles);
+      k_sort_scatter<<<ntiles, kBlock, 0, st>>>(ka, va, n, shift, hist.p, ntiles, kb, vb);
+      launch_check();
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
+    // sorted keys/order now in (ka, va)
+    cstart.alloc(grid.ncell + 1);
+    cend.alloc(grid.ncell + 1);
+    k_fill_int<<<blocks_for(grid.ncell + 1), kBlock, 0, st>>>(cstart.p, grid.ncell + 1, 0);
+    k_fill_int<<<blocks_for(grid.ncell + 1), kBlock, 0, st>>>(cend.p, grid.ncell + 1, 0);
+    k_cell_ranges<<<nb, kBlock, 0, st>>>(ka, n, cstart.p, cend.p);
+    PermuteArgs P{va, iperm.p, x.p, v.p, a.p, z.p, ffm.p, mass.p, x2.p, v2.p, a2.p, z2.p,
+                  ffm2.p, mass2.p, perm2.p, iperm2.p, n};
+    k_permute<<<nb, kBlock, 0, st>>>(P);
+    launch_check();
+    std::swap(x.p, x2.p);
+    std::swap(v.p, v2.p);
+    std::swap(a.p, a2.p);
+    std::swap(z.p, z2.p);
+    std::swap(ffm.p, ffm2.p);
+    std::swap(mass.p, mass2.p);
+    std::swap(perm.p, perm2.p);
+    std::swap(iperm.p, iperm2.p);
+    // keep the sorted keys where the build kernel reads them
+    if (ka != keys.p) {
+      std::swap(keys.p, keys2.p);
+      std::swap(vals.p, vals2.p);
+    }
+  }
+
+  void build_list() {
+    // list storage: cap entries per atom, rounded to the lane count
+    lpa = auto_lanes();
+    stride = ((n + kWave - 1) / kWave) * kWave;
+    partials.alloc((size_t)blocks_for(n * 16) * 12);
+    partials2.alloc((size_t)blocks_for(n * 16) * 12);
+    if (cap == 0) {
+      // initial capacity from the density inside the list radius
+      double vol = box.L[0] * box.L[1] * box.L[2];
+      double rr = std::isfinite(rl) ? rl : std::cbrt(vol);
+      double est = (double)n / vol * 4.18879 * rr * rr * rr;
+      if (cm.kind == kCmLinkedCell)
+        est = std::max(est, (double)n / vol * pow((2 * grid.side + 1) * box.L[0] / grid.nx, 3));
+      cap = (int)std::min<double>(n * 8.0 + 64, est * 1.3 + 32);
+    }
+    for (int attempt = 0; attempt < 3; ++attempt) {
+      cap = ((cap + lpa - 1) / lpa) * lpa;
+      nbr.alloc((size_t)cap * stride);
+      cnt.alloc(n);
+      // max_count, pad0, sum_count: 16 contiguous bytes
+      hipchk(hipMemsetAsync(&ctrl.p->max_count, 0, 16, st), "memset");
+      BuildArgs B{x.p, keys.p, cstart.p, cend.p, nbr.p, cnt.p, &ctrl.p->max_count,
+                  &ctrl.p->sum_count, grid, box, rl * rl, n, stride, cap, lpa};
+      hipEvent_t e0 = prof_begin();
+      k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
+      prof_end(kProfBuild, e0);
+      launch_check();
+      hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 16, hipMemcpyDeviceToHost, st),
+             "memcpy");
+      sync();
+      const long mc = hctrl->max_count;
+      max_list = mc;
+      sum_list = (long)hctrl->sum_count;
+      if (mc <= cap)
+        return;
+      cap = (int)(mc * 1.2 + 16);
+    }
+    fail(LPMDX_ECAPACITY, "neighbour list capacity could not be established");
+  }
+
+  void rebuild() {
+    if (n == 0) {
+      list_built = true;
+      return;
+    }
+    rl = list_radius();
+    setup_grid();
+    hipEvent_t e0 = prof_begin();
+    sort_atoms();
+    prof_end(kProfBuild, e0);
+    build_list();
+    // verletlist.cc:96-97: a list may hold at most maxneighbors-1 entries
+    if (cm.kind == kCmVerlet && max_list >= cm.maxneighbors)
+      fail(LPMDX_ECAPACITY, "verletlist: maxneighbors is too small!");
+    hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    list_built = true;
+    disp_acc = 0.0;
+    n_builds++;
+  }
+
+  // exact displacement since the build, for calls through the plugin interface
+  double displacement_since_build() {
+    // |x - xb| max over atoms via a throw-away kernel
+    hipchk(hipMemsetAsync(&ctrl.p->maxd2[0], 0, sizeof(unsigned), st), "memset");
+    ForceArgs A{};
+    A.x = x.p;
+    A.xn = xb.p;
+    A.n = n;
+    A.b = box;
+    A.maxd2 = &ctrl.p->maxd2[0];
+    k_disp<<<blocks_for(n), kBlock, 0, st>>>(A);
+    launch_check();
+    hipchk(hipMemcpyAsync(&hctrl->maxd2[0], &ctrl.p->maxd2[0], sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+    float f;
+    memcpy(&f, &hctrl->maxd2[0], sizeof f);
+    return std::sqrt((double)f);
+  }
+
+
+  // CellManager::UpdateCell
+  void update_cell() {
+    if (cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    undrift();
+    if (cm.kind == kCmLinkedCell) {
+      rebuild();
+    } else {
+      // rebuilt whenever the list could miss a pair inside the filter radius;
+      // while the reference's own list is valid both deliver the same set
+      bool must = !list_built || list_radius() != rl;
+      if (!must) {
+        double b = budget();
+        must = displacement_since_build() > b;
+      }
+      if (must)
+        rebuild();
+    }
+    cm.ucalls++;
+  }
+
+  // user positions back into x (undo the look-ahead drift)
+  void undrift() {
+    if (drifted) {
+      std::swap(x.p, xn.p);
+      drifted = false;
+    }
+  }
+
+  // ------------------------------------------------------------------ forces
+  template <int KIND, bool EVAL> void launch_pair_k(ForceArgs& A, long nb) {
+    hipEvent_t e0 = prof_begin();
+    switch (lpa) {
+    case 1: k_pair_force<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 2: k_pair_force<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 4: k_pair_force<KIND, 4, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 8: k_pair_force<KIND, 8, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    default: k_pair_force<KIND, 16, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    }
+    prof_end(kProfForce, e0);
+  }
+  template <int KIND, bool EVAL> void launch_eam_k(ForceArgs& A, long nb, double rc2_rho) {
+    ForceArgs R = A;
+    R.P.rc2 = rc2_rho;
+    R.partials = partials2.p;
+    hipEvent_t e0 = prof_begin();
+    switch (lpa) {
+    case 1: k_eam_rho<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
+    case 2: k_eam_rho<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
+    case 4: k_eam_rho<KIND, 4, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
+    case 8: k_eam_rho<KIND, 8, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
+    default: k_eam_rho<KIND, 16, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
+    }
+    prof_end(kProfRho, e0);
+    A.w_rho = w_rho.p;
+    hipEvent_t e1 = prof_begin();
+    switch (lpa) {
+    case 1: k_eam_force<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 2: k_eam_force<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 4: k_eam_force<KIND, 4, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 8: k_eam_force<KIND, 8, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    default: k_eam_force<KIND, 16, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    }
+    prof_end(kProfForce, e1);
+  }
+  template <bool EVAL> void launch_pot(int kind, ForceArgs& A, long nb, double rc2_rho) {
+    switch (kind) {
+    case kLJ: launch_pair_k<kLJ, EVAL>(A, nb); break;
+    case kFastLJ: launch_pair_k<kFastLJ, EVAL>(A, nb); break;
+    case kMorse: launch_pair_k<kMorse, EVAL>(A, nb); break;
+    case kBuckingham: launch_pair_k<kBuckingham, EVAL>(A, nb); break;
+    case kSuttonChen: launch_eam_k<kSuttonChen, EVAL>(A, nb, rc2_rho); break;
+    case kFinnisSinclair: launch_eam_k<kFinnisSinclair, EVAL>(A, nb, rc2_rho); break;
+    case kGupta: launch_eam_k<kGupta, EVAL>(A, nb, rc2_rho); break;
+    }
+    launch_check();
+  }
+
+  // Filters as the reference's managers serve each potential (see half_list_rcut).
+  void filters(const PotParams& P, double& rc2_force, double& rc2_rho) const {
+    const double own = P.cutoff * P.cutoff;
+    if (cm.kind == kCmLinkedCell) {
+      const double h = half_list_rcut() * half_list_rcut();
+      const double f = full_list_rcut() * full_list_rcut();
+      // pair potentials also test their own cutoff (pairpotential.cc:56);
+      // metal potentials take the list as it comes (metalpotential.cc:140-145)
+      rc2_force = P.metal() ? h : std::min(h, own);
+      rc2_rho = f;
+    } else {
+      rc2_force = own;
+      rc2_rho = own;
+    }
+  }
+
+  // Evaluate all potentials at x; mode kForcesOnly / kInitDrift / kStep.
+  // With eval, energies land in partials and are reduced into `red`.
+  void forces(int mode, bool eval, double dt, unsigned int* maxd2) {
+    if (pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    const long nb = blocks_for(n * lpa);
+    const bool fused = pots.size() == 1;
+    for (size_t q = 0; q < pots.size(); ++q) {
+      const PotParams& P = *pots[q];
+      double rcf, rcr;
+      filters(P, rcf, rcr);
+      ForceArgs A{};
+      A.x = x.p;
+      A.xn = xn.p;
+      A.v = v.p;
+      A.a = fused ? a.p : asum.p;
+      A.z = z.p;
+      A.ffm = ffm.p;
+      A.nbr = nbr.p;
+      A.cnt = cnt.p;
+      A.partials = partials.p;
+      A.maxd2 = maxd2;
+      A.P = device_view(P, rcf, rcr);
+      A.b = box;
+      for (int k = 0; k < 3; ++k)
+        A.ll[k] = ll[k];
+      A.dt = dt;
+      A.n = n;
+      A.stride = stride;
+      A.mode = fused ? mode : kForcesOnly;
+      A.accumulate = q > 0;
+      if (P.metal()) {
+        w_rho.alloc(n);
+        rho.alloc(n);
+      }
+      if (eval)
+        launch_pot<true>(P.kind, A, nb, rcr);
+      else
+        launch_pot<false>(P.kind, A, nb, rcr);
+      if (eval) {
+        // per-potential totals accumulate into red[0..11]
+        k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 12, red.p + 12 * (1 + 2 * q));
+        if (P.metal())
+          k_reduce_partials<<<1, kBlock, 0, st>>>(partials2.p, nb, 12, red.p + 12 * (2 + 2 * q));
+        else
+          k_fill_double<<<1, 32, 0, st>>>(red.p + 12 * (2 + 2 * q), 12, 0.0);
+        launch_check();
+      }
+    }
+    if (!fused && mode != kForcesOnly) {
+      ForceArgs A{};
+      A.x = x.p;
+      A.xn = xn.p;
+      A.v = v.p;
+      A.a = a.p;
+      A.maxd2 = maxd2;
+      A.b = box;
+      for (int k = 0; k < 3; ++k)
+        A.ll[k] = ll[k];
+      A.dt = dt;
+      A.n = n;
+      A.mode = mode;
+      k_integrate<<<blocks_for(n), kBlock, 0, st>>>(A, asum.p);
+      launch_check();
+    } else if (!fused) {
+      hipchk(hipMemcpyAsync(a.p, asum.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    }
+    n_forces++;
+  }
+
+
+  // energies of the last evaluated forces: E, W, S[9]
+  void read_energies(double* e, double* w, double* S) {
+    const size_t np = pots.size();
+    hipchk(hipMemcpyAsync(hred, red.p + 12, 24 * np * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+    double E = 0.0, W = 0.0, T[9] = {0};
+    for (size_t q = 0; q < np; ++q) {
+      const double* pr = hred + 24 * q;
+      const double* em = pr + 12;
+      // CombinedPotential::energy sums each potential's energycache
+      double ep = pr[0] + em[1];
+      E += ep;
+      W += pr[2];
+      for (int k = 0; k < 9; ++k)
+        T[k] += pr[3 + k];
+    }
+    if (e)
+      *e = E;
+    if (w)
+      *w = W;
+    if (S)
+      for (int k = 0; k < 9; ++k)
+        S[k] = T[k];
+    last_e[0] = E;
+    last_e[1] = W;
+    for (int k = 0; k < 9; ++k)
+      last_e[2 + k] = T[k];
+    last_e_valid = true;
+  }
+
+  double kinetic(double p[3]) {
+    const long nb = blocks_for(n);
+    partials.alloc(std::max<size_t>((size_t)nb * 4, partials.n));
+    k_kinetic<<<nb, kBlock, 0, st>>>(v.p, mass.p, n, partials.p);
+    k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
+    launch_check();
+    hipchk(hipMemcpyAsync(hred, red.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+    if (p)
+      for (int q = 0; q < 3; ++q)
+        p[q] = hred[1 + q];
+    return hred[0] * kKin2eV;
+  }
+
+  // ------------------------------------------------------------------ VV
+  void vv(double dt, long steps, long obs_every, double* ep_t, double* ek_t) {
+    if (pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    if (cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (!(dt == dt) || !std::isfinite(dt))
+      fail(LPMDX_ENONFINITE, "dt is not finite");
+    if (n == 0 || steps <= 0)
+      return;
+    unsigned int* md = ctrl.p->maxd2;
+    // reach the look-ahead state: x = next positions, (v, a) of the last step
+    if (!drifted) {
+      if (!forces_ok) {
+        update_cell();
+        hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
+        forces(kInitDrift, false, dt, md);
+      } else {
+        hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
+        ForceArgs A{};
+        A.x = x.p;
+        A.xn = xn.p;
+        A.v = v.p;
+        A.a = a.p;
+        A.maxd2 = md;
+        A.n = n;
+        A.dt = dt;
+        for (int k = 0; k < 3; ++k)
+          A.ll[k] = ll[k];
+        k_drift<<<blocks_for(n), kBlock, 0, st>>>(A);
+        launch_check();
+      }
+      std::swap(x.p, xn.p);
+      drifted = true;
+      forces_ok = true;
+      hipchk(hipMemcpyAsync(&hctrl->maxd2[0], md, sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
+      sync();
+      float f;
+      memcpy(&f, &hctrl->maxd2[0], sizeof f);
+      disp_acc += std::sqrt((double)f);
+      rate = std::max(rate, std::sqrt((double)f));
+    }
+    long done = 0;
+    long rollbacks = 0;
+    while (done < steps) {
+      long m = steps - done;
+      if (obs_every > 0)
+        m = std::min(m, obs_every - (done % obs_every));
+      m = std::min<long>(m, kMaxChunk);
+      bool need = false;
+      if (cm.kind == kCmLinkedCell) {
+        m = 1;
+        need = true; // UpdateCell rebuilds every call
+      } else {
+        const double b = budget();
+        if (!list_built || list_radius() != rl || disp_acc > b)
+          need = true;
+        double avail = need ? b : b - disp_acc;
+        double r = rate > 0 ? 1.5 * rate : 0.0;
+        long allowed = r > 0 ? (long)std::floor(avail / r) : 4;
+        allowed = std::max(1L, allowed);
+        m = std::min(m, allowed);
+      }
+      // positions x are where the next force evaluation happens
+      drifted = false;
+      if (need)
+        rebuild();
+      const bool snapshot = cm.kind != kCmLinkedCell;
+      if (snapshot) {
+        xs.alloc(3 * n);
+        vs.alloc(3 * n);
+        as.alloc(3 * n);
+        hipchk(hipMemcpyAsync(xs.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(vs.p, v.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(as.p, a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+      }
+      hipchk(hipMemsetAsync(md, 0, m * sizeof(unsigned), st), "memset");
+      for (long s = 0; s < m; ++s) {
+        const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
+        forces(kStep, obs, dt, md + s);
+        std::swap(x.p, xn.p);
+      }
+      hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
+      sync();
+      // validity: the force at launch s saw positions displaced by
+      // disp_acc + sum_{k<s} d_k from the build positions
+      const double b = budget();
+      double acc = disp_acc;
+      long bad = -1;
+      double rmax = 0.0;
+      for (long s = 0; s < m; ++s) {
+        if (cm.kind != kCmLinkedCell && acc > b) {
+          bad = s;
+          break;
+        }
+        float f;
+        memcpy(&f, &hctrl->maxd2[s], sizeof f);
+        double d = std::sqrt((double)f);
+        acc += d;
+        rmax = std::max(rmax, d);
+      }
+      if (bad >= 0) {
+        // roll back to the chunk start and rebuild there
+        hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(a.p, as.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        rate = std::max(rate * 2.0, rmax);
+        list_built = false;
+        drifted = true;
+        if (++rollbacks > 64)
+          fail(LPMDX_EDEVICE, "velocityverlet: list validity could not be kept");
+        continue;
+      }
+      disp_acc = acc;
+      rate = std::max(0.5 * rate, rmax);
+      drifted = true;
+      cm.ucalls += 2 * m;
+      done += m;
+      n_steps += m;
+      if (obs_every > 0 && (done % obs_every) == 0) {
+        double e;
+        read_energies(&e, nullptr, nullptr);
+        // the step's state: v in v, positions in xn
+        double ek = kinetic(nullptr);
+        long idx = done / obs_every - 1;
+        if (ep_t)
+          ep_t[idx] = e;
+        if (ek_t)
+          ek_t[idx] = ek;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+namespace {
+
+void need(const void* p, const char* what) {
+  if (!p)
+    fail(LPMDX_EINVAL, std::string(what) + " is NULL");
+}
+
+void finite(const double* p, long m, const char* what) {
+  for (long k = 0; k < m; ++k)
+    if (!std::isfinite(p[k]))
+      fail(LPMDX_ENONFINITE, std::string(what) + " holds a non-finite value");
+}
+
+void ensure_device() {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c == 0)
+    fail(LPMDX_EDEVICE, "no HIP device available");
+}
+
+} // namespace
+
+extern "C" {
+
+const char* lpmdx_last_error(void) { return g_err.c_str(); }
+
+int lpmdx_abi_version(void) { return LPMDX_ABI_VERSION; }
+
+int lpmdx_device_count(int* count) {
+  return guard([&] {
+    need(count, "count");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess)
+      c = 0;
+    *count = c;
+  });
+}
+
+int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const int* z,
+                        const double* mass) {
+  return guard([&] {
+    need(out, "out");
+    need(box, "box");
+    *out = nullptr;
+    if (n < 0 || n > (1L << 30))
+      fail(LPMDX_EINVAL, "atom count out of range");
+    if (n > 0)
+      need(z, "z");
+    finite(box, 3, "box");
+    for (int q = 0; q < 3; ++q)
+      if (!(box[q] > 0.0))
+        fail(LPMDX_EINVAL, "box edges must be positive");
+    ensure_device();
+    auto S = std::make_unique<lpmdx_system>();
+    S->n = n;
+    for (int q = 0; q < 3; ++q) {
+      S->box.L[q] = box[q];
+      S->box.half[q] = 0.5 * box[q];
+      S->ll[q] = edge_module(box[q]);
+    }
+    hipchk(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking), "hipStreamCreate");
+    hipchk(hipHostMalloc(&S->hctrl, sizeof(Ctrl)), "hipHostMalloc");
+    hipchk(hipHostMalloc(&S->hred, 64 * 12 * sizeof(double)), "hipHostMalloc");
+    const size_t m = n > 0 ? n : 1;
+    for (DBuf<double>* b : {&S->x, &S->xn, &S->v, &S->a, &S->asum, &S->xb, &S->x2, &S->v2, &S->a2})
+      b->alloc(3 * m);
+    for (DBuf<double>* b : {&S->ffm, &S->mass, &S->ffm2, &S->mass2})
+      b->alloc(m);
+    for (DBuf<int>* b : {&S->z, &S->perm, &S->iperm, &S->z2, &S->perm2, &S->iperm2})
+      b->alloc(m);
+    for (DBuf<uint32_t>* b : {&S->keys, &S->vals, &S->keys2, &S->vals2})
+      b->alloc(m);
+    S->ctrl.alloc(1);
+    S->red.alloc(12 * 64);
+    hipchk(hipMemsetAsync(S->ctrl.p, 0, sizeof(Ctrl), S->st), "memset");
+    std::vector<double> ms(m, 1.0), ff(m, 0.0);
+    std::vector<int> zz(m, 0), pm(m);
+    for (long i = 0; i < n; ++i) {
+      zz[i] = z[i];
+      double mi = mass ? mass[i] : ElemMass(z[i]);
+      if (!(mi > 0.0) || !std::isfinite(mi))
+        fail(LPMDX_EINVAL, "no mass for atomic number " + std::to_string(z[i]));
+      ms[i] = mi;
+      ff[i] = kForceFactor / mi; // forcefactor / Mass(), pairpotential.cc:62
+      pm[i] = (int)i;
+    }
+    hipchk(hipMemcpyAsync(S->mass.p, ms.data(), m * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(S->ffm.p, ff.data(), m * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(S->z.p, zz.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(S->perm.p, pm.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(S->iperm.p, pm.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
+    for (DBuf<double>* b : {&S->x, &S->v, &S->a})
+      hipchk(hipMemsetAsync(b->p, 0, 3 * m * sizeof(double), S->st), "memset");
+    S->sync();
+    *out = S.release();
+  });
+}
+
+int lpmdx_system_destroy(lpmdx_system* sys) {
+  return guard([&] { delete sys; });
+}
+
+long lpmdx_system_size(const lpmdx_system* sys) { return sys ? sys->n : -1; }
+
+}
+
+namespace {
+
+// caller order <-> internal order through perm (internal -> caller)
+void to_internal(lpmdx_system* S, const double* h, double* d) {
+  const long n = S->n;
+  std::vector<int> perm(n);
+  hipchk(hipMemcpyAsync(perm.data(), S->perm.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
+  S->sync();
+  std::vector<double> tmp(3 * n);
+  for (long i = 0; i < n; ++i)
+    for (int q = 0; q < 3; ++q)
+      tmp[3 * i + q] = h[3 * perm[i] + q];
+  hipchk(hipMemcpyAsync(d, tmp.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
+  S->sync();
+}
+
+void to_caller(lpmdx_system* S, const double* d, double* h) {
+  const long n = S->n;
+  std::vector<int> perm(n);
+  std::vector<double> tmp(3 * n);
+  hipchk(hipMemcpyAsync(perm.data(), S->perm.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
+  hipchk(hipMemcpyAsync(tmp.data(), d, 3 * n * sizeof(double), hipMemcpyDeviceToHost, S->st), "memcpy");
+  S->sync();
+  for (long i = 0; i < n; ++i)
+    for (int q = 0; q < 3; ++q)
+      h[3 * perm[i] + q] = tmp[3 * i + q];
+}
+
+} // namespace
+
+extern "C" {
+
+int lpmdx_system_set_state(lpmdx_system* S, const double* x, const double* v, const double* a) {
+  return guard([&] {
+    need(S, "system");
+    if (S->n == 0)
+      return;
+    if (x)
+      finite(x, 3 * S->n, "positions");
+    if (v)
+      finite(v, 3 * S->n, "velocities");
+    if (a)
+      finite(a, 3 * S->n, "accelerations");
+    S->undrift();
+    if (x) {
+      to_internal(S, x, S->x.p);
+      S->forces_ok = false;
+    }
+    if (v)
+      to_internal(S, v, S->v.p);
+    if (a) {
+      to_internal(S, a, S->a.p);
+      S->forces_ok = false;
+    }
+  });
+}
+
+int lpmdx_system_get_state(lpmdx_system* S, double* x, double* v, double* a) {
+  return guard([&] {
+    need(S, "system");
+    if (S->n == 0)
+      return;
+    const double* xu = S->drifted ? S->xn.p : S->x.p;
+    if (x)
+      to_caller(S, xu, x);
+    if (v)
+      to_caller(S, S->v.p, v);
+    if (a)
+      to_caller(S, S->a.p, a);
+  });
+}
+
+int lpmdx_set_cellmanager(lpmdx_system* S, const char* name, const char* args) {
+  return guard([&] {
+    need(S, "system");
+    need(name, "name");
+    std::string nm(name);
+    CellMgr c;
+    if (nm == "linkedcell") {
+      // plugins/linkedcell.cc:17-41 (the mode keyword alone decides auto)
+      Keywords k(args, {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"},
+                        {"warn-outside", "true"}});
+      c.kind = kCmLinkedCell;
+      c.cutoff = k.d("cutoff");
+      c.nx = k.i("nx");
+      c.ny = k.i("ny");
+      c.nz = k.i("nz");
+      std::string mode = k.s("mode");
+      c.mode_auto = (mode == "auto" || mode == "AUTO");
+      c.warn_outside = k.b("warn-outside");
+    } else if (nm == "verletlist") {
+      // plugins/verletlist.cc:12-29; extcut = 1.08*cutoff (the unread extcutoff
+      // keyword stays unread), `skin` switches to a skin-triggered list
+      Keywords k(args, {{"cutoff", "0.0"}, {"extcutoff", "0.0"}, {"maxneighbors", "80"}, {"each", "5"}});
+      c.kind = kCmVerlet;
+      c.cutoff = k.d("cutoff");
+      c.maxneighbors = k.i("maxneighbors");
+      c.each = std::max(1, k.i("each"));
+      c.extcut = 1.08 * c.cutoff;
+      if (k.has("skin")) {
+        c.skin = k.d("skin");
+        if (!(c.skin >= 0.0))
+          fail(LPMDX_EINVAL, "verletlist: skin must be non-negative");
+      }
+      if (fabs(c.cutoff) < 1E-10)
+        fail(LPMDX_EINVAL, "verletlist: No cutoff was defined");
+    } else {
+      fail(LPMDX_EINVAL, "unknown cell manager '" + nm + "'");
+    }
+    S->undrift();
+    S->cm = c;
+    S->list_built = false;
+    S->forces_ok = false;
+    S->update_cell(); // Configuration::SetCellManager calls UpdateCell
+  });
+}
+
+int lpmdx_update_cell(lpmdx_system* S) {
+  return guard([&] {
+    need(S, "system");
+    S->update_cell();
+  });
+}
+
+int lpmdx_potential_append(lpmdx_system* S, const char* name, const char* args, int s1, int s2) {
+  return guard([&] {
+    need(S, "system");
+    need(name, "name");
+    S->pots.push_back(make_potential(name, args, s1, s2));
+    S->forces_ok = false;
+    S->undrift();
+  });
+}
+
+int lpmdx_potential_clear(lpmdx_system* S) {
+  return guard([&] {
+    need(S, "system");
+    S->pots.clear();
+    S->forces_ok = false;
+    S->undrift();
+  });
+}
+
+int lpmdx_update_forces(lpmdx_system* S, double* epot, double* virial, double stress[9]) {
+  return guard([&] {
+    need(S, "system");
+    if (S->cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (S->pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    if (S->n == 0) {
+      if (epot)
+        *epot = 0.0;
+      if (virial)
+        *virial = 0.0;
+      if (stress)
+        for (int k = 0; k < 9; ++k)
+          stress[k] = 0.0;
+      return;
+    }
+    S->update_cell();
+    S->forces(kForcesOnly, true, 0.0, S->ctrl.p->maxd2);
+    S->read_energies(epot, virial, stress);
+    S->forces_ok = true;
+  });
+}
+
+int lpmdx_velocityverlet(lpmdx_system* S, double dt, long steps, long obs_every, double* epot_t,
+                         double* ekin_t) {
+  return guard([&] {
+    need(S, "system");
+    if (steps < 0 || obs_every < 0)
+      fail(LPMDX_EINVAL, "negative step count");
+    S->vv(dt, steps, obs_every, epot_t, ekin_t);
+  });
+}
+
+int lpmdx_observables(lpmdx_system* S, double out[LPMDX_NOBS]) {
+  return guard([&] {
+    need(S, "system");
+    need(out, "out");
+    for (int k = 0; k < LPMDX_NOBS; ++k)
+      out[k] = 0.0;
+    if (S->n == 0)
+      return;
+    double p[3];
+    const double ke = S->kinetic(p);
+    const double vol = S->ll[0] * S->ll[1] * S->ll[2];
+    out[LPMDX_OBS_KINETIC] = ke;
+    out[LPMDX_OBS_TEMP] = (2.0 / 3.0) * ke / (kBoltzmann * double(S->n));
+    for (int q = 0; q < 3; ++q)
+      out[LPMDX_OBS_PX + q] = p[q];
+    if (S->last_e_valid) {
+      out[LPMDX_OBS_POTENTIAL] = S->last_e[0];
+      out[LPMDX_OBS_VIRIAL] = S->last_e[1];
+      out[LPMDX_OBS_PRESSURE] = (kPressFactor / vol) * ((2.0 / 3.0) * ke + (1.0 / 3.0) * S->last_e[1]);
+      for (int k = 0; k < 9; ++k)
+        out[LPMDX_OBS_STRESS + k] = (kPressFactor / vol) * S->last_e[2 + k];
+    }
+  });
+}
+
+int lpmdx_system_stats(const lpmdx_system* S, long out[8]) {
+  return guard([&] {
+    need(S, "system");
+    need(out, "out");
+    out[0] = S->n_builds;
+    out[1] = S->n_forces;
+    out[2] = S->n_steps;
+    out[3] = S->cap;
+    out[4] = S->max_list;
+    out[5] = S->lpa;
+    out[6] = S->grid.nx;
+    out[7] = S->sum_list;
+  });
+}
+
+int lpmdx_system_profile(lpmdx_system* S, int enable) {
+  return guard([&] {
+    need(S, "system");
+    S->sync();
+    S->profiling = enable != 0;
+    for (int k = 0; k < kProfSlots; ++k) {
+      S->prof_ms[k] = 0.0;
+      S->prof_n[k] = 0;
+    }
+  });
+}
+
+int lpmdx_system_profile_read(lpmdx_system* S, double ms[4], long launches[4]) {
+  return guard([&] {
+    need(S, "system");
+    need(ms, "ms");
+    need(launches, "launches");
+    S->sync();
+    for (int k = 0; k < kProfSlots; ++k) {
+      ms[k] = S->prof_ms[k];
+      launches[k] = S->prof_n[k];
+    }
+  });
+}
+
+int lpmdx_system_set_lanes(lpmdx_system* S, int lanes) {
+  return guard([&] {
+    need(S, "system");
+    if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16))
+      fail(LPMDX_EINVAL, "lanes per atom must be 0, 1, 2, 4, 8 or 16");
+    S->lpa_req = lanes;
+    S->list_built = false;
+    S->cap = 0;
+  });
+}
+
+int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, long cap, long* js,
+                         double* r2, long* total) {
+  return guard([&] {
+    need(S, "system");
+    need(counts, "counts");
+    need(total, "total");
+    if (S->cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (!S->list_built)
+      S->update_cell();
+    S->undrift();
+    const long n = S->n;
+    if (n == 0) {
+      *total = 0;
+      return;
+    }
+    if (S->cm.kind == kCmLinkedCell && rcut > S->rl) {
+      // the cached lists are built on the first request's rcut (linkedcell.cc:315-354)
+      S->rl = rcut;
+      S->build_list();
+    }
+    std::vector<int> perm(n), cntv(n), nb((size_t)S->cap * S->stride);
+    std::vector<double> x(3 * n);
+    hipchk(hipMemcpyAsync(perm.data(), S->perm.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(cntv.data(), S->cnt.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(nb.data(), S->nbr.p, nb.size() * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
+    hipchk(hipMemcpyAsync(x.data(), S->x.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, S->st), "memcpy");
+    S->sync();
+    std::vector<long> iperm(n);
+    for (long i = 0; i < n; ++i)
+      iperm[perm[i]] = i;
+    const long rowlen = S->stride * S->lpa;
+    const double rc2 = rcut * rcut;
+    long k = 0;
+    std::vector<std::pair<long, double>> row;
+    for (long o = 0; o < n; ++o) {
+      const long i = iperm[o];
+      row.clear();
+      for (int c = 0; c < cntv[i]; ++c) {
+        const long jj = nb[(long)(c / S->lpa) * rowlen + i * S->lpa + (c % S->lpa)];
+        const long oj = perm[jj];
+        if (!full && S->cm.kind == kCmLinkedCell && oj > o)
+          continue; // linkedcell half list keeps j < i (linkedcell.cc:323)
+        if (!full && S->cm.kind == kCmVerlet && oj < o)
+          continue; // verletlist half list keeps j > i (verletlist.cc:157)
+        double d[3];
+        for (int q = 0; q < 3; ++q) {
+          d[q] = x[3 * jj + q] - x[3 * i + q];
+          if (d[q] >= S->box.half[q])
+            d[q] -= S->box.L[q];
+          else if (d[q] < -S->box.half[q])
+            d[q] += S->box.L[q];
+        }
+        const double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (rr < rc2)
+          row.emplace_back(oj, rr);
+      }
+      if (S->cm.kind == kCmVerlet)
+        std::stable_sort(row.begin(), row.end(),
+                         [](const std::pair<long, double>& p, const std::pair<long, double>& q) {
+                           return p.first < q.first;
+                         });
+      counts[o] = (long)row.size();
+      for (auto& e : row) {
+        if (k < cap) {
+          if (js)
+            js[k] = e.first;
+          if (r2)
+            r2[k] = e.second;
+        }
+        ++k;
+      }
+    }
+    *total = k;
+  });
+}
+
+} // extern "C"

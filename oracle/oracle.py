@@ -48,7 +48,8 @@ def build(force: bool = False) -> str:
         import subprocess
 
         subprocess.check_call(
-            ["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-shared", "-o", LIB_PATH, src, "-lm"]
+            ["gcc", "-O2", "-std=c11", "-fPIC", "-fno-semantic-interposition", "-ffp-contract=off", "-shared",
+             "-o", LIB_PATH, src, "-lm"]
         )
     return LIB_PATH
 

@@ -1,0 +1,219 @@
+"""GPU parity of the HIP engine against the FP64 oracle and the golden fixtures,
+through the C ABI.
+
+Bars: neighbour lists bit-exact (indices, order, r2); forces within 1e-5
+relative RMS of FP64 (‖F - F64‖2 / ‖F64‖2, the tolerance the workload
+states), potential energy within 1e-6 relative; integrator bit-exact when the
+forces are exactly zero.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lpmd2_amd as E
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+META = json.load(open(os.path.join(GOLDEN, "golden.json")))
+FORCE_TOL = 1e-5
+ENERGY_TOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def jittered(nx, a, sigma, seed):
+    x, L = E.fcc(nx, a)
+    return np.mod(x + np.random.default_rng(seed).normal(0.0, sigma, x.shape), L), L
+
+
+def rel_rms(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# --------------------------------------------------------------- neighbour lists
+@pytest.mark.parametrize("case", sorted(META["lists"]))
+def test_lists_match_golden(case):
+    m = META["lists"][case]
+    say("lists", case)
+    d = np.load(os.path.join(GOLDEN, f"lists_{case}.npz"))
+    x, L = d["x"], d["L"]
+    s = E.System(x, L, 18)
+    s.set_cellmanager(m["cm"], m["cm_args"])
+    oc = O.Conf(x, L)
+    args = dict(zip(m["cm_args"].split()[0::2], m["cm_args"].split()[1::2]))
+    if m["cm"] == "linkedcell":
+        om = O.CellManager("linkedcell", float(args["cutoff"]), int(args.get("nx", 0)), int(args.get("ny", 0)),
+                           int(args.get("nz", 0)), mode=args.get("mode", "noauto"))
+    else:
+        om = O.CellManager("verletlist", float(args["cutoff"]), maxneighbors=int(args["maxneighbors"]),
+                           each=int(args["each"]))
+    om.update(oc)
+    for full, tag in ((True, "full"), (False, "half")):
+        counts, js, r2 = s.neighbor_lists(full, m["rcut"])
+        assert np.array_equal(counts, d[f"{tag}_counts"])
+        oc_counts, ojs, or2, _ = om.lists(oc, full, m["rcut"])
+        assert np.array_equal(js, ojs)
+        assert np.array_equal(r2, or2)
+        if f"{tag}_j" in d:
+            assert np.array_equal(js, d[f"{tag}_j"])
+
+
+# --------------------------------------------------------------- forces
+POTS = [
+    ("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5", dict(sigma=3.41, epsilon=0.0103, cutoff=8.5), 18, 5.26, 0.1),
+    ("fastlj", "sigma 3.41 epsilon 0.0103 cutoff 8.5", dict(sigma=3.41, epsilon=0.0103, cutoff=8.5, bins=500), 18, 5.26, 0.1),
+    ("morse", "depth 0.3429 a 1.3588 re 2.866", dict(depth=0.3429, a=1.3588, re=2.866), 29, 3.61, 0.05),
+    ("buckingham", "B1 1254.5 B2 2.0 Ro 0.3249 cutoff 7.0", dict(B1=1254.5, B2=2.0, Ro=0.3249, cutoff=7.0), 11, 3.61, 0.05),
+    ("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22",
+     dict(e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=7.22), 29, 3.61, 0.05),
+    ("finnissinclair", "c0 1.2371147 c1 -0.3592185 c2 -0.0385607 A 1.828905 B 1.8 c 3.4 d 3.569745",
+     dict(c0=1.2371147, c1=-0.3592185, c2=-0.0385607, A=1.828905, B=1.8, c=3.4, d=3.569745), 26, 2.87 * 1.26, 0.05),
+    ("gupta", "A 0.0855 r0 2.556 p 10.960 B 1.2240 q 2.278 cutoff 7.22",
+     dict(A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=7.22), 29, 3.61, 0.05),
+]
+
+
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
+@pytest.mark.parametrize("name,args,kw,z,a,jit", POTS, ids=[p[0] for p in POTS])
+def test_forces_match_oracle(name, args, kw, z, a, jit, cm):
+    say("forces", name, cm)
+    x, L = jittered(5, a, jit, 17)
+    pot = O.Pot(name, **kw)
+    rc = pot.cutoff if np.isfinite(pot.cutoff) else 7.0
+    if cm == "linkedcell":
+        cm_args = f"cutoff {rc} mode auto"
+        om = O.CellManager("linkedcell", rc, mode="auto")
+    else:
+        cm_args = f"cutoff {rc} maxneighbors 400 each 5"
+        om = O.CellManager("verletlist", rc, maxneighbors=400, each=5)
+    s = E.System(x, L, z)
+    s.set_cellmanager(cm, cm_args)
+    s.append_potential(name, args)
+    e, w, S = s.update_forces()
+    _, _, acc = s.get_state()
+    conf = O.Conf(x, L, z=z)
+    e0, w0, S0 = O.update_forces(conf, om, [pot])
+    err = rel_rms(acc, conf.a)
+    say(f"  rel force rms {err:.2e}  E {e:.10g} vs {e0:.10g}  W {w:.6g} vs {w0:.6g}")
+    assert err <= FORCE_TOL
+    assert abs(e - e0) <= ENERGY_TOL * abs(e0)
+    assert abs(w - w0) <= 1e-5 * max(abs(w0), 1e-3)
+    assert np.abs(S - S0).max() <= 1e-5 * max(np.abs(S0).max(), 1e-3)
+
+
+def test_species_pairs_buckingham_rocksalt():
+    """Three species-pair potentials (Na-Na, Cl-Cl, Na-Cl) with SetValidSpecies."""
+    say("species rocksalt")
+    x, L = jittered(4, 5.64, 0.05, 3)
+    z = np.where(np.arange(x.shape[0]) % 2 == 0, 11, 17).astype(np.int32)
+    pots = [("buckingham", "B1 424.1 B2 1.05 Ro 0.3170 cutoff 7.0", dict(B1=424.1, B2=1.05, Ro=0.3170, cutoff=7.0), 11, 11),
+            ("buckingham", "B1 1256.3 B2 2.0 Ro 0.3321 cutoff 7.0", dict(B1=1256.3, B2=2.0, Ro=0.3321, cutoff=7.0), 17, 17),
+            ("buckingham", "B1 1254.5 B2 0.0 Ro 0.3249 cutoff 7.0", dict(B1=1254.5, B2=0.0, Ro=0.3249, cutoff=7.0), 11, 17)]
+    s = E.System(x, L, z)
+    s.set_cellmanager("verletlist", "cutoff 7.0 maxneighbors 300 each 5")
+    op = []
+    for name, args, kw, s1, s2 in pots:
+        s.append_potential(name, args, s1, s2)
+        op.append(O.Pot(name, s1, s2, **kw))
+    e, w, S = s.update_forces()
+    _, _, acc = s.get_state()
+    conf = O.Conf(x, L, z=z)
+    e0, w0, S0 = O.update_forces(conf, O.CellManager("verletlist", 7.0, maxneighbors=300, each=5), op)
+    err = rel_rms(acc, conf.a)
+    say(f"  rel force rms {err:.2e}  E {e} vs {e0}")
+    assert err <= FORCE_TOL
+    assert abs(e - e0) <= ENERGY_TOL * abs(e0)
+
+
+# --------------------------------------------------------------- integrator
+def test_free_flight_is_bit_exact():
+    """epsilon = 0 makes every force exactly zero: the trajectory is the
+    reference's AdvancePosition/FittedInside arithmetic alone."""
+    say("free flight")
+    x, L = jittered(3, 5.26, 0.3, 5)
+    v = np.random.default_rng(2).normal(0, 0.05, x.shape)
+    s = E.System(x, L, 18, v=v)
+    s.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 300 each 5 skin 1.0")
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0 cutoff 8.5")
+    s.velocityverlet(2.0, 37)
+    xg, vg, ag = s.get_state()
+    conf = O.Conf(x, L, v=v)
+    for _ in range(37):
+        conf.a[:] = 0.0
+        aold = O.vv_position(conf, 2.0)
+        conf.a[:] = 0.0
+        O.vv_velocity(conf, 2.0, aold)
+    assert np.array_equal(xg, conf.x)
+    assert np.array_equal(vg, conf.v)
+
+
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
+def test_short_trajectory_matches_oracle(cm):
+    """20 steps of LJ argon at 100 K: per-atom positions stay within FP32
+    force noise of the FP64 trajectory."""
+    say("trajectory", cm)
+    x, L = E.fcc(4, 5.26)
+    m = np.full(x.shape[0], 39.948)
+    v = E.maxwell_boltzmann(m, 100.0, seed=7)
+    s = E.System(x, L, 18, v=v)
+    if cm == "linkedcell":
+        s.set_cellmanager("linkedcell", "cutoff 8.5 mode auto")
+        om = O.CellManager("linkedcell", 8.5, mode="auto")
+    else:
+        s.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 200 each 5")
+        om = O.CellManager("verletlist", 8.5, maxneighbors=200, each=5)
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    ep, ek = s.velocityverlet(1.0, 20, obs_every=5)
+    xg, vg, _ = s.get_state()
+    conf = O.Conf(x, L, v=v)
+    e0, k0, _ = O.md_run(conf, om, [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)], 1.0, 20)
+    dx = xg - conf.x
+    dx -= L * np.round(dx / L)
+    say(f"  max |dx| {np.abs(dx).max():.2e}  E {ep} vs {e0[4::5]}")
+    assert np.abs(dx).max() < 1e-8
+    np.testing.assert_allclose(ep, e0[4::5], rtol=1e-6)
+    np.testing.assert_allclose(ek, k0[4::5], rtol=1e-6)
+
+
+def test_nve_energy_conservation_suttonchen():
+    say("nve sc")
+    x, L = jittered(5, 3.61, 0.01, 9)
+    m = np.full(x.shape[0], 63.546)
+    v = E.maxwell_boltzmann(m, 300.0, seed=9)
+    s = E.System(x, L, 29, v=v)
+    s.set_cellmanager("verletlist", "cutoff 7.22 maxneighbors 400 each 5 skin 0.3")
+    s.append_potential("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22")
+    ep, ek = s.velocityverlet(2.0, 400, obs_every=20)
+    et = ep + ek
+    say(f"  E_tot drift {et.max() - et.min():.3e} eV over {len(et)} samples, builds {s.stats()['builds']}")
+    assert (et.max() - et.min()) < 2e-3 * abs(ek.mean())
+
+
+# --------------------------------------------------------------- error behaviour
+def test_error_paths():
+    say("errors")
+    x, L = jittered(3, 3.61, 0.01, 1)
+    s = E.System(x, L, 29)
+    with pytest.raises(E.LpmdxError) as e:
+        s.update_forces()
+    assert e.value.code == E.ESTATE
+    with pytest.raises(E.LpmdxError) as e:
+        s.set_cellmanager("verletlist", "cutoff 7.22 maxneighbors 100 each 5")
+    assert e.value.code == E.ECAPACITY and "maxneighbors" in str(e.value)
+    with pytest.raises(E.LpmdxError) as e:
+        s.set_cellmanager("linkedcell", "cutoff 7.22 mode")
+    assert e.value.code == E.EINVAL
+    with pytest.raises(E.LpmdxError) as e:
+        s.set_state(x=np.full_like(x, np.nan))
+    assert e.value.code == E.ENONFINITE
+    with pytest.raises(E.LpmdxError) as e:
+        s.append_potential("nosuchpotential", "")
+    assert e.value.code == E.EINVAL
