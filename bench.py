@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--workload", default="sc_cu_131k", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sampling")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per atom (0 = automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: each box has one MI355X and multi-GPU runs are not part of this build",
@@ -145,6 +146,8 @@ def main():
     w = WORKLOADS[args.workload]
     log(f"workload {args.workload}: {w['desc']}")
     s, x, L, v = make_system(w)
+    if args.lanes:
+        s.set_lanes(args.lanes)
     n = s.n
     t0 = time.perf_counter()
     s.velocityverlet(w["dt"], args.warmup)

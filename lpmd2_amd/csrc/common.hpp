@@ -56,23 +56,6 @@ __device__ __forceinline__ double fitted_inside(double x, double ll) {
   return x;
 }
 
-// Compensated FP32 accumulator for per-atom force sums (Knuth's branch-free
-// TwoSum; needs the no-reassociation, no-contraction build flags).
-struct CompF {
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float v) {
-    float t = s + v;
-    float bp = t - s;
-    c += (s - (t - bp)) + (v - bp);
-    s = t;
-  }
-  __device__ __forceinline__ float value() const { return s + c; }
-  __device__ __forceinline__ void merge(const CompF& o) {
-    add(o.s);
-    c += o.c;
-  }
-};
-
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
   for (int o = kWave / 2; o > 0; o >>= 1)
     v += __shfl_xor(v, o, kWave);

@@ -251,20 +251,35 @@ DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) {
     d.f[4] = (float)(6.0 * p[1]);
     break;
   case kSuttonChen:
+    // {e, a, n, m, c, -, -, -, -n e, m c e / 2, -, K = c e}
     for (int q = 0; q < 5; ++q)
       d.f[q] = (float)p[q];
+    d.f[8] = (float)(-p[2] * p[0]);
+    d.f[9] = (float)(0.5 * p[3] * p[4] * p[0]);
     d.ni = int_exponent(p[2]);
     d.mi = int_exponent(p[3]);
     d.f[11] = (float)(p[4] * p[0]);
     break;
   case kGupta:
+    // {A, r0, p, B, q, p, 2q, 2q/r0, p/r0, -A p/r0, B q/r0, K = B}:
+    // e^{-p(r-r0)/r0} = e^{p - (p/r0) r}, e^{-2q(r-r0)/r0} = e^{2q - (2q/r0) r}
     for (int q = 0; q < 5; ++q)
       d.f[q] = (float)p[q];
+    d.f[5] = (float)p[2];
+    d.f[6] = (float)(2.0 * p[4]);
+    d.f[7] = (float)(2.0 * p[4] / p[1]);
+    d.f[8] = (float)(p[2] / p[1]);
+    d.f[9] = (float)(-p[0] * p[2] / p[1]);
+    d.f[10] = (float)(p[3] * p[4] / p[1]);
     d.f[11] = (float)p[3];
     break;
   case kFinnisSinclair:
+    // {c0, c1, c2, A, B, c, d, -, B/d, 3B/d, -A/2, K = A}
     for (int q = 0; q < 7; ++q)
       d.f[q] = (float)p[q];
+    d.f[8] = (float)(p[4] / p[6]);
+    d.f[9] = (float)(3.0 * p[4] / p[6]);
+    d.f[10] = (float)(-0.5 * p[3]);
     d.f[11] = (float)p[3];
     break;
   }
@@ -387,7 +402,8 @@ struct lpmdx_system {
   int lpa = 1, lpa_req = 0;
   double rl = 0.0; // list radius
   // EAM
-  DBuf<double> w_rho, rho;
+  DBuf<float> w_rho;
+  DBuf<double> rho;
   // reductions
   DBuf<double> partials, partials2, red;
   DBuf<Ctrl> ctrl;

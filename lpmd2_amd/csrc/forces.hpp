@@ -3,9 +3,11 @@
 // Precision split: the displacement d = x_j - x_i, its minimum image and
 // r2 = |d|^2 are formed in FP64 exactly as OrthogonalCell::Displacement and
 // SquareModule form them, so cutoff decisions match the reference bit for bit;
-// d and r2 are then cast to FP32 for the pair and embedding arithmetic; the
-// per-atom force sums are compensated FP32; accelerations, the position and
-// velocity update, rho and the global energy/virial/stress sums are FP64.
+// r2 is then cast to FP32 for the pair and embedding arithmetic; each pair's
+// FP32 force scalar multiplies the FP64 displacement and the per-atom sums
+// accumulate in FP64 (this bounds the summation error below what compensated
+// FP32 sums reach, in fewer instructions on CDNA4); accelerations, the position
+// and velocity update, rho and the global energy/virial/stress sums are FP64.
 //
 // Full neighbour lists: every atom accumulates its own force, so there is no
 // scatter and no atomics; each pair's energy, virial and stress enter half per
@@ -39,29 +41,33 @@ struct DevPot {
   int bins;
 };
 
-__device__ __forceinline__ float ipow(float x, int n) {
-  float r = 1.0f;
-  float b = x;
-  while (n) {
-    if (n & 1)
-      r *= b;
-    b *= b;
-    n >>= 1;
-  }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// t^k for 0 <= k <= 31 without branches (square-and-multiply over the bits of k)
+__device__ __forceinline__ float powu(float t, int k) {
+  const float t2 = t * t, t4 = t2 * t2, t8 = t4 * t4, t16 = t8 * t8;
+  float r = (k & 1) ? t : 1.0f;
+  r *= (k & 2) ? t2 : 1.0f;
+  r *= (k & 4) ? t4 : 1.0f;
+  r *= (k & 8) ? t8 : 1.0f;
+  r *= (k & 16) ? t16 : 1.0f;
   return r;
 }
 
-__device__ __forceinline__ float powi_or_powf(float x, int ni, float nf) {
-  return ni > 0 ? ipow(x, ni) : powf(x, nf);
+__device__ __forceinline__ float powk(float t, int ki, float kf) {
+  return ki > 0 ? powu(t, ki) : powf(t, kf);
 }
 
 // Pair potentials: force on i is fscal * rij (rij = x_j - x_i), pair energy e.
+// Pair forms follow plugins/{lennardjones,fastlj,morse,buckingham}.cc; the
+// FP32 evaluation uses hardware reciprocal / reciprocal square root (1 ulp).
 template <int KIND>
 __device__ __forceinline__ void pair_eval(const DevPot& P, double r2d, float r2, float& fscal,
                                           float& e) {
   if constexpr (KIND == kLJ) {
     // lennardjones.cc:53-67: E = 4 eps (s12 - s6), ff = -48 eps/r2 (s12 - s6/2)
-    const float inv = 1.0f / r2;
+    const float inv = rcp(r2);
     const float s2 = P.f[0] * inv;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
@@ -77,63 +83,69 @@ __device__ __forceinline__ void pair_eval(const DevPot& P, double r2d, float r2,
     e = P.tab_e[k];
   } else if constexpr (KIND == kMorse) {
     // morse.cc:53-62: E = D (1 - e^{-a(r-re)})^2, ff = 2 a D / r (1 - ex) ex
-    const float r = sqrtf(r2);
+    const float ir = rsq(r2);
+    const float r = r2 * ir;
     const float ex = expf(-P.f[0] * (r - P.f[1]));
     const float om = 1.0f - ex;
-    fscal = P.f[3] / r * om * ex;
+    fscal = P.f[3] * ir * om * ex;
     e = P.f[2] * om * om;
   } else if constexpr (KIND == kBuckingham) {
     // buckingham.cc:58-75: E = B1 e^{-r/Ro} - B2/r^6, ff = 6 B2/r^8 - B1 e^{-r/Ro}/(Ro r)
-    const float r = sqrtf(r2);
-    const float ir2 = 1.0f / r2;
+    const float ir = rsq(r2);
+    const float r = r2 * ir;
+    const float ir2 = ir * ir;
     const float ir6 = ir2 * ir2 * ir2;
     const float ex = P.f[0] * expf(-r * P.f[3]);
-    fscal = P.f[4] * ir6 * ir2 - ex * P.f[3] / r;
+    fscal = P.f[4] * ir6 * ir2 - ex * P.f[3] * ir;
     e = ex - P.f[1] * ir6;
   }
 }
 
-// EAM pieces in terms of r: pair energy V, dV/dr, rho(r), and G(r) such that
-// the many-body force on i along r-hat is (w_i + w_j) G(r), w = rho^{-1/2}.
-template <int KIND>
-__device__ __forceinline__ float eam_rho(const DevPot& P, float r) {
+// EAM: rho_ij(r) and, for the force pass, the pair energy V and the scalar
+// fscal with force on i = fscal * rij:
+//   fscal = (dV/dr + (w_i + w_j) G(r)) / r,  w = rho^{-1/2},
+// where (w_i + w_j) G(r) r-hat is the reference's ManyBodies term.
+template <int KIND> __device__ __forceinline__ float eam_rho(const DevPot& P, float r2) {
   if constexpr (KIND == kSuttonChen) { // suttonchen.cc:78-81: a^m r^-m
-    return powi_or_powf(P.f[1] / r, P.mi, P.f[3]);
+    return powk(P.f[1] * rsq(r2), P.mi, P.f[3]);
   } else if constexpr (KIND == kGupta) { // gupta.cc:69
-    return expf(-2.0f * P.f[4] * (r - P.f[1]) / P.f[1]);
+    const float r = r2 * rsq(r2);
+    return expf(P.f[6] - P.f[7] * r);
   } else { // finnissinclair.cc:82-87
+    const float r = r2 * rsq(r2);
     const float d = P.f[6];
     if (r <= d) {
       const float t = r - d;
-      return (t * t) * (1.0f + P.f[4] * t / d);
+      return (t * t) * (1.0f + P.f[8] * t);
     }
     return 0.0f;
   }
 }
 
 template <int KIND>
-__device__ __forceinline__ void eam_pair(const DevPot& P, float r, float& V, float& dV, float& G) {
+__device__ __forceinline__ void eam_pair(const DevPot& P, float r2, float wsum, float& fscal,
+                                         float& V) {
+  const float ir = rsq(r2);
   if constexpr (KIND == kSuttonChen) {
-    // suttonchen.cc:73-95 with f = {e, a, n, m, c}
-    const float t = P.f[1] / r;
-    const float tn = powi_or_powf(t, P.ni, P.f[2]);
-    const float tm = powi_or_powf(t, P.mi, P.f[3]);
+    // suttonchen.cc:73-95, f = {e, a, n, m, c, -, -, -, -n e, m c e / 2}
+    const float t = P.f[1] * ir;
+    const float tn = powk(t, P.ni, P.f[2]);
+    const float tm = powk(t, P.mi, P.f[3]);
     V = P.f[0] * tn;
-    dV = -P.f[2] * V / r;
-    G = 0.5f * P.f[3] * P.f[4] * P.f[0] * tm / r;
+    fscal = (ir * ir) * (P.f[8] * tn + P.f[9] * wsum * tm);
   } else if constexpr (KIND == kGupta) {
-    // gupta.cc:67-82 with f = {A, r0, p, B, q}
-    const float ep = expf(-P.f[2] * (r - P.f[1]) / P.f[1]);
-    const float eq = expf(-2.0f * P.f[4] * (r - P.f[1]) / P.f[1]);
+    // gupta.cc:67-82, f = {A, r0, p, B, q, p, 2q, 2q/r0, p/r0, -A p/r0, B q/r0}
+    const float r = r2 * ir;
+    const float ep = expf(P.f[5] - P.f[8] * r);
+    const float eq = expf(P.f[6] - P.f[7] * r);
     V = P.f[0] * ep;
-    dV = -(P.f[0] * P.f[2] / P.f[1]) * ep;
-    G = (P.f[3] * P.f[4] / P.f[1]) * eq;
+    fscal = ir * (P.f[9] * ep + P.f[10] * wsum * eq);
   } else {
-    // finnissinclair.cc:75-108 with f = {c0, c1, c2, A, B, c, d}
+    // finnissinclair.cc:75-108, f = {c0, c1, c2, A, B, c, d, -, B/d, 3B/d, -A/2}
+    const float r = r2 * ir;
     const float c = P.f[5], d = P.f[6];
+    float dV = 0.0f, G = 0.0f;
     V = 0.0f;
-    dV = 0.0f;
-    G = 0.0f;
     if (r <= c) {
       const float t = r - c;
       const float poly = P.f[0] + P.f[1] * r + P.f[2] * r * r;
@@ -142,8 +154,9 @@ __device__ __forceinline__ void eam_pair(const DevPot& P, float r, float& V, flo
     }
     if (r <= d) {
       const float t = r - d;
-      G = -0.5f * P.f[3] * (2.0f * t + 3.0f * P.f[4] * t * t / d);
+      G = P.f[10] * (2.0f * t + P.f[9] * t * t);
     }
+    fscal = (dV + wsum * G) * ir;
   }
 }
 
@@ -163,7 +176,7 @@ struct ForceArgs {
   const double* ffm; // [n] forcefactor / m
   const int* nbr;
   const int* cnt;
-  const double* w_rho;  // EAM: [n] rho^{-1/2} (FP64)
+  const float* w_rho;   // EAM: [n] rho^{-1/2}
   double* partials;     // [nblocks * 12] when evaluating energies
   unsigned int* maxd2;  // per-step max displacement^2 (float bits)
   DevPot P;
@@ -244,7 +257,7 @@ struct Tally {
   }
 };
 
-template <int LPA> __device__ __forceinline__ float group_sum(float v) {
+template <int LPA> __device__ __forceinline__ float group_sum_f(float v) {
 #pragma unroll
   for (int o = LPA / 2; o > 0; o >>= 1)
     v += __shfl_xor(v, o, kWave);
@@ -279,45 +292,85 @@ __device__ __forceinline__ void write_partials(const ForceArgs& A, Tally& t, dou
   }
 }
 
+// Neighbour loop shared by the force kernels: lane l of atom i visits entries
+// k = l, l + LPA, ... of its list in batches of kUnroll, issuing the index and
+// position loads of a batch before any arithmetic, so the gathers overlap.
+// body(j, dx, dy, dz, r2) sees the FP64 minimum-image displacement.
+constexpr int kUnroll = 4;
+
+template <int LPA, typename Body>
+__device__ __forceinline__ void for_neighbours(const ForceArgs& A, long i, int l, double xi,
+                                               double yi, double zi, Body&& body) {
+  const int c = A.cnt[i];
+  const long rowlen = A.stride * LPA;
+  const int* __restrict__ nb = A.nbr + i * LPA + l;
+  for (int k0 = l; k0 < c; k0 += kUnroll * LPA) {
+    int js[kUnroll];
+    double px[kUnroll], py[kUnroll], pz[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = k0 + u * LPA;
+      js[u] = k < c ? nb[(long)(k / LPA) * rowlen] : (int)i;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const double* p = A.x + 3 * (long)js[u];
+      px[u] = p[0];
+      py[u] = p[1];
+      pz[u] = p[2];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (k0 + u * LPA < c) {
+        double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
+        min_image(A.b, dx, dy, dz);
+        body(js[u], dx, dy, dz, sqmod(dx, dy, dz));
+      }
+    }
+  }
+}
+
+// Per-atom force: FP32 pair scalar times the FP64 displacement, summed in FP64
+// (a product of an FP32 and an FP64 value plus an FP64 add per component).
+struct ForceSum {
+  double x = 0.0, y = 0.0, z = 0.0;
+  __device__ __forceinline__ void add(float fs, double dx, double dy, double dz) {
+    const double f = (double)fs;
+    x = fma(f, dx, x);
+    y = fma(f, dy, y);
+    z = fma(f, dz, z);
+  }
+};
+
 template <int KIND, int LPA, bool EVAL>
 __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   const long i = t / LPA;
   const int l = (int)(t % LPA);
   const bool active = i < A.n;
-  CompF fx, fy, fz;
+  ForceSum F;
   Tally tal;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
-    const int c = A.cnt[i];
     const int si = A.z[i];
     const bool all = A.P.spc_sum < 0;
-    const long rowlen = A.stride * LPA;
-    for (int k = l; k < c; k += LPA) {
-      const int j = A.nbr[(long)(k / LPA) * rowlen + i * LPA + l];
-      double dx = A.x[3 * (long)j] - xi;
-      double dy = A.x[3 * (long)j + 1] - yi;
-      double dz = A.x[3 * (long)j + 2] - zi;
-      min_image(A.b, dx, dy, dz);
-      const double r2 = sqmod(dx, dy, dz);
-      if (r2 < A.P.rc2 && (all || si + A.z[j] == A.P.spc_sum)) {
-        const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz, fr2 = (float)r2;
+    const double rc2 = A.P.rc2;
+    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](int j, double dx, double dy, double dz, double r2) {
+      if (r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum)) {
+        const float fr2 = (float)r2;
         float fs, e;
         pair_eval<KIND>(A.P, r2, fr2, fs, e);
-        fx.add(fs * fdx);
-        fy.add(fs * fdy);
-        fz.add(fs * fdz);
+        F.add(fs, dx, dy, dz);
         if constexpr (EVAL)
-          tal.add(fs, fdx, fdy, fdz, fr2, e);
+          tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, e);
       }
-    }
+    });
   }
-  float Fx = group_sum<LPA>(fx.value()), Fy = group_sum<LPA>(fy.value()),
-        Fz = group_sum<LPA>(fz.value());
+  const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
   float d2 = 0.0f;
   if (active && l == 0) {
     const double m = A.ffm[i];
-    vv_tail(A, i, (double)Fx * m, (double)Fy * m, (double)Fz * m, d2);
+    vv_tail(A, i, Fx * m, Fy * m, Fz * m, d2);
   }
   flush_maxd2(A, d2);
   if constexpr (EVAL)
@@ -325,10 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
 }
 
 // EAM pass 1: rho_i over the full list within the list cutoff (no species
-// check, metalpotential.cc:121-132); writes rho_i^{-1/2} and, when evaluating,
-// the embedding energy F(rho_i) = -K sqrt(rho_i) into the block partials.
+// check, metalpotential.cc:121-132), accumulated in FP64; writes
+// w_i = rho_i^{-1/2} (FP32) and rho_i, and, when evaluating, the embedding
+// energy F(rho_i) = -K sqrt(rho_i) into the block partials.
 template <int KIND, int LPA, bool EVAL>
-__global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, double* w_out, double* rho_out) {
+__global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   const long i = t / LPA;
   const int l = (int)(t % LPA);
@@ -336,23 +390,16 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, double* w_out, 
   double rho = 0.0;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
-    const int c = A.cnt[i];
-    const long rowlen = A.stride * LPA;
-    for (int k = l; k < c; k += LPA) {
-      const int j = A.nbr[(long)(k / LPA) * rowlen + i * LPA + l];
-      double dx = A.x[3 * (long)j] - xi;
-      double dy = A.x[3 * (long)j + 1] - yi;
-      double dz = A.x[3 * (long)j + 2] - zi;
-      min_image(A.b, dx, dy, dz);
-      const double r2 = sqmod(dx, dy, dz);
-      if (r2 < A.P.rc2)
-        rho += (double)eam_rho<KIND>(A.P, sqrtf((float)r2));
-    }
+    const double rc2 = A.P.rc2;
+    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](int, double, double, double, double r2) {
+      if (r2 < rc2)
+        rho += (double)eam_rho<KIND>(A.P, (float)r2);
+    });
   }
   rho = group_sum<LPA>(rho);
   double emb = 0.0;
   if (active && l == 0) {
-    w_out[i] = 1.0 / sqrt(rho);
+    w_out[i] = (float)(1.0 / sqrt(rho));
     if (rho_out)
       rho_out[i] = rho;
     emb = -(double)A.P.f[11] * sqrt(rho);
@@ -371,42 +418,30 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
   const long i = t / LPA;
   const int l = (int)(t % LPA);
   const bool active = i < A.n;
-  CompF fx, fy, fz;
+  ForceSum F;
   Tally tal;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
-    const int c = A.cnt[i];
     const int si = A.z[i];
-    const float wi = (float)A.w_rho[i];
+    const float wi = A.w_rho[i];
     const bool all = A.P.spc_sum < 0;
-    const long rowlen = A.stride * LPA;
-    for (int k = l; k < c; k += LPA) {
-      const int j = A.nbr[(long)(k / LPA) * rowlen + i * LPA + l];
-      double dx = A.x[3 * (long)j] - xi;
-      double dy = A.x[3 * (long)j + 1] - yi;
-      double dz = A.x[3 * (long)j + 2] - zi;
-      min_image(A.b, dx, dy, dz);
-      const double r2 = sqmod(dx, dy, dz);
-      if (r2 < A.P.rc2 && (all || si + A.z[j] == A.P.spc_sum)) {
-        const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz, fr2 = (float)r2;
-        const float r = sqrtf(fr2);
-        float V, dV, G;
-        eam_pair<KIND>(A.P, r, V, dV, G);
-        const float s = (dV + (wi + (float)A.w_rho[j]) * G) / r;
-        fx.add(s * fdx);
-        fy.add(s * fdy);
-        fz.add(s * fdz);
+    const double rc2 = A.P.rc2;
+    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](int j, double dx, double dy, double dz, double r2) {
+      if (r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum)) {
+        const float fr2 = (float)r2;
+        float fs, V;
+        eam_pair<KIND>(A.P, fr2, wi + A.w_rho[j], fs, V);
+        F.add(fs, dx, dy, dz);
         if constexpr (EVAL)
-          tal.add(s, fdx, fdy, fdz, fr2, V);
+          tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, V);
       }
-    }
+    });
   }
-  float Fx = group_sum<LPA>(fx.value()), Fy = group_sum<LPA>(fy.value()),
-        Fz = group_sum<LPA>(fz.value());
+  const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
   float d2 = 0.0f;
   if (active && l == 0) {
     const double m = A.ffm[i];
-    vv_tail(A, i, (double)Fx * m, (double)Fy * m, (double)Fz * m, d2);
+    vv_tail(A, i, Fx * m, Fy * m, Fz * m, d2);
   }
   flush_maxd2(A, d2);
   if constexpr (EVAL)

@@ -9,6 +9,7 @@ MASS = {
     13: 26.981539,   # Al
     17: 35.4527,     # Cl
     18: 39.948,      # Ar
+    26: 55.847,      # Fe
     28: 58.6934,     # Ni
     29: 63.546,      # Cu
     47: 107.8682,    # Ag
