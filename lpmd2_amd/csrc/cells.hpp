@@ -10,9 +10,13 @@
 //    distinct cell of the 27-cell neighbourhood once, so that the list holds
 //    exactly the minimum-image pairs inside the radius (the set
 //    plugins/verletlist.cc and plugins/minimumimage.cc deliver).
-// Lists are full lists in a lane-interleaved transposed layout:
-//   entry k of atom i at  (k / lpa) * (stride * lpa) + i * lpa + (k % lpa)
-// so that the lanes of a wavefront read consecutive words.
+// Lists are full lists in a tiled, chunked layout.  A tile holds the T = 64/lpa
+// atoms one wavefront serves; each atom's entries are stored in chunks of 8
+// consecutive words (32 bytes, one L2 sector):
+//   entry k of atom i at  tile(i) * T * cap + (k / 8) * T * 8 + (i mod T) * 8 + (k mod 8).
+// The build fills a chunk in registers and writes it whole, so no sector is
+// ever written partially; the lpa lanes serving an atom read consecutive words
+// of its chunk, and a tile's list is one contiguous block.
 #pragma once
 
 #include "common.hpp"
@@ -145,8 +149,17 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   A.iperm_new[o] = (int)k;
 }
 
+constexpr int kChunk = 8;
+
+__host__ __device__ __forceinline__ long list_index(long i, int k, int lpa, int cap) {
+  const int T = kWave / lpa;
+  return (i / T) * (long)T * cap + (long)(k / kChunk) * (T * kChunk) + (i % T) * kChunk +
+         (k % kChunk);
+}
+
 struct BuildArgs {
   const double* x;
+  const float4* xf; // FP32 copy of x (w unused), written by k_to_f32 before the build
   const uint32_t* skeys;
   const int* cstart;
   const int* cend;
@@ -156,41 +169,130 @@ struct BuildArgs {
   unsigned long long* sum_count;
   Grid g;
   Box b;
-  double rl2; // list radius squared (FP64 r2 < rl2, as the managers test)
+  float Lf[3], halff[3];
+  double rl2; // list radius squared
+  float rl2f; // admission radius^2 in FP32: the list radius widened by the FP32 rounding bound
   long n;
-  long stride; // >= n, row length of the transposed list
+  long stride; // n rounded up to a whole number of tiles
   int cap;     // entries per atom
   int lpa;     // lanes per atom of the consuming kernels
+  int path;    // kPathExact64 / kPathMin32 / kPathShift32
 };
 
+// Build paths.  The FP32 paths admit candidates whose FP32 distance is inside a
+// radius widened by the rounding bound of FP32 positions, so the list is a
+// superset of the pairs inside the list radius; every consumer re-tests
+// r2 < rc^2 in FP64 exactly as the reference does.
+enum BuildPath {
+  kPathExact64 = 0, // FP64 distance with the reference's minimum image (small boxes)
+  kPathMin32 = 1,   // FP32 distance with a per-pair minimum image (linkedcell stencils)
+  kPathShift32 = 2  // FP32 distance, one periodic shift per stencil cell (>= 3 cells per axis)
+};
+
+// FP32 copy of the positions folded into [0, L) (in FP64 first), the frame the
+// periodic binning uses: an atom at x == L (FittedInside keeps it) or outside
+// the box is binned by floor(n x / L) mod n and compared at its folded place.
+__global__ __launch_bounds__(kBlock) void k_to_f32(const double* __restrict__ x, long n, Box b,
+                                                   float4* __restrict__ xf) {
+  long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  float w[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double v = x[3 * i + q];
+    v -= b.L[q] * floor(v / b.L[q]);
+    w[q] = (float)v;
+  }
+  xf[i] = make_float4(w[0], w[1], w[2], 0.0f);
+}
+
+// One atom's pending chunk of list entries, held in registers until it is full.
+struct Chunk {
+  int v[kChunk];
+};
+
+__device__ __forceinline__ void flush_chunk(const BuildArgs& A, long i, int c, const Chunk& b) {
+  // c entries so far; the chunk holding entry c-1 starts at entry (c-1) & ~7
+  const int k0 = (c - 1) & ~(kChunk - 1);
+  if (k0 < A.cap) {
+    int4* dst = reinterpret_cast<int4*>(A.nbr + list_index(i, k0, A.lpa, A.cap));
+    dst[0] = make_int4(b.v[0], b.v[1], b.v[2], b.v[3]);
+    dst[1] = make_int4(b.v[4], b.v[5], b.v[6], b.v[7]);
+  }
+}
+
+__device__ __forceinline__ void push(const BuildArgs& A, long i, int k, int& c, Chunk& b) {
+  const int slot = c & (kChunk - 1);
+#pragma unroll
+  for (int u = 0; u < kChunk; ++u)
+    b.v[u] = (slot == u) ? k : b.v[u];
+  ++c;
+  if (slot == kChunk - 1)
+    flush_chunk(A, i, c, b);
+}
+
+__device__ __forceinline__ float wrap1f(float d, float L, float h) {
+  const float w = (d >= h) ? -L : ((d < -h) ? L : 0.0f);
+  return d + w;
+}
+
+// candidates of cell p against atom i
 __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, const double* xi,
-                                          int& c) {
+                                          float sx, float sy, float sz, int& c, Chunk& b) {
   const int lo = A.cstart[p], hi = A.cend[p];
-  const long rowlen = A.stride * A.lpa;
-  for (int k = lo; k < hi; ++k) {
-    if (k == i)
-      continue;
-    double dx = A.x[3 * k] - xi[0];
-    double dy = A.x[3 * k + 1] - xi[1];
-    double dz = A.x[3 * k + 2] - xi[2];
-    min_image(A.b, dx, dy, dz);
-    double r2 = sqmod(dx, dy, dz);
-    if (r2 < A.rl2) {
-      if (c < A.cap)
-        A.nbr[(long)(c / A.lpa) * rowlen + i * A.lpa + (c % A.lpa)] = k;
-      ++c;
+  if (A.path == kPathExact64) {
+    for (int k = lo; k < hi; ++k) {
+      double dx = A.x[3 * (long)k] - xi[0], dy = A.x[3 * (long)k + 1] - xi[1],
+             dz = A.x[3 * (long)k + 2] - xi[2];
+      min_image(A.b, dx, dy, dz);
+      if (k != i && sqmod(dx, dy, dz) < A.rl2)
+        push(A, i, k, c, b);
+    }
+  } else if (A.path == kPathMin32) {
+    for (int k = lo; k < hi; ++k) {
+      const float4 q = A.xf[k];
+      const float dx = wrap1f(q.x - sx, A.Lf[0], A.halff[0]);
+      const float dy = wrap1f(q.y - sy, A.Lf[1], A.halff[1]);
+      const float dz = wrap1f(q.z - sz, A.Lf[2], A.halff[2]);
+      if (k != i && dx * dx + dy * dy + dz * dz < A.rl2f)
+        push(A, i, k, c, b);
+    }
+  } else {
+    // (sx, sy, sz) is atom i moved by minus this cell's periodic image shift
+    int k = lo;
+    for (; k + 1 < hi; k += 2) {
+      const float4 q0 = A.xf[k], q1 = A.xf[k + 1];
+      const float ax = q0.x - sx, ay = q0.y - sy, az = q0.z - sz;
+      const float bx = q1.x - sx, by = q1.y - sy, bz = q1.z - sz;
+      const float ra = ax * ax + ay * ay + az * az, rb = bx * bx + by * by + bz * bz;
+      if (k != i && ra < A.rl2f)
+        push(A, i, k, c, b);
+      if (k + 1 != i && rb < A.rl2f)
+        push(A, i, k + 1, c, b);
+    }
+    if (k < hi) {
+      const float4 q0 = A.xf[k];
+      const float ax = q0.x - sx, ay = q0.y - sy, az = q0.z - sz;
+      if (k != i && ax * ax + ay * ay + az * az < A.rl2f)
+        push(A, i, k, c, b);
     }
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
-  long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  long i = xcd_block() * kBlock + threadIdx.x;
   int c = 0;
+  Chunk b;
+#pragma unroll
+  for (int u = 0; u < kChunk; ++u)
+    b.v[u] = 0;
   if (i < A.n) {
     const Grid& g = A.g;
     uint32_t key = A.skeys[i];
     if (key < (uint32_t)g.ncell) {
       const double xi[3] = {A.x[3 * i], A.x[3 * i + 1], A.x[3 * i + 2]};
+      const float4 fi = A.xf[i];
       const int ci = key % g.nx, cj = (key / g.nx) % g.ny, ck = key / (g.nx * g.ny);
       if (g.mode == kLinkedCell) {
         const int s = g.side;
@@ -204,7 +306,26 @@ __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
               // p == ncell only when side >= n (WrapAround's -n case): the
               // reference reads past its cell table there; treated as empty
               if (p < g.ncell)
-                scan_cell(A, (int)p, i, xi, c);
+                scan_cell(A, (int)p, i, xi, fi.x, fi.y, fi.z, c, b);
+            }
+          }
+        }
+      } else if (A.path == kPathShift32) {
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int rz = ck + dz;
+          const float shz = rz < 0 ? -A.Lf[2] : (rz >= g.nz ? A.Lf[2] : 0.0f);
+          const int z2 = pmod(rz, g.nz);
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int ry = cj + dy;
+            const float shy = ry < 0 ? -A.Lf[1] : (ry >= g.ny ? A.Lf[1] : 0.0f);
+            const int z1 = pmod(ry, g.ny);
+            for (int dx = -1; dx <= 1; ++dx) {
+              const int rx = ci + dx;
+              const float shx = rx < 0 ? -A.Lf[0] : (rx >= g.nx ? A.Lf[0] : 0.0f);
+              const int z0 = pmod(rx, g.nx);
+              // image of cell (z0,z1,z2) next to atom i sits at x_j + sh: compare
+              // x_j against x_i - sh
+              scan_cell(A, (z2 * g.ny + z1) * g.nx + z0, i, xi, fi.x - shx, fi.y - shy, fi.z - shz, c, b);
             }
           }
         }
@@ -218,12 +339,14 @@ __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
             const int z1 = pmod(cj + dy, g.ny);
             for (int dx = ox0; dx <= ox1; ++dx) {
               const int z0 = pmod(ci + dx, g.nx);
-              scan_cell(A, (z2 * g.ny + z1) * g.nx + z0, i, xi, c);
+              scan_cell(A, (z2 * g.ny + z1) * g.nx + z0, i, xi, fi.x, fi.y, fi.z, c, b);
             }
           }
         }
       }
     }
+    if (c & (kChunk - 1))
+      flush_chunk(A, i, c, b); // the last, partly filled chunk
     A.cnt[i] = c;
   }
   unsigned int m = wave_max((unsigned int)c);

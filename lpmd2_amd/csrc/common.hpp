@@ -26,20 +26,19 @@ struct Box {
 };
 
 // OrthogonalCell::Displacement (liblpmd/lpmd/orthogonalcell.h:131-149): one
-// wrap per axis, >= on the positive side.  Compiled without FP contraction.
+// wrap per axis, >= on the positive side.  Written with selects instead of
+// branches: d + (-L) and d + (+L) round exactly like d - L and d + L, and
+// d + 0.0 equals d (a -0.0 component may come out as +0.0, which no squared
+// distance or force component can tell apart).  Compiled without contraction.
+__device__ __forceinline__ double wrap1(double d, double L, double h) {
+  const double w = (d >= h) ? -L : ((d < -h) ? L : 0.0);
+  return d + w;
+}
+
 __device__ __forceinline__ void min_image(const Box& b, double& dx, double& dy, double& dz) {
-  if (dx >= b.half[0])
-    dx -= b.L[0];
-  else if (dx < -b.half[0])
-    dx += b.L[0];
-  if (dy >= b.half[1])
-    dy -= b.L[1];
-  else if (dy < -b.half[1])
-    dy += b.L[1];
-  if (dz >= b.half[2])
-    dz -= b.L[2];
-  else if (dz < -b.half[2])
-    dz += b.L[2];
+  dx = wrap1(dx, b.L[0], b.half[0]);
+  dy = wrap1(dy, b.L[1], b.half[1]);
+  dz = wrap1(dz, b.L[2], b.half[2]);
 }
 
 // GenericVector::SquareModule (vector.h:110): (x*x + y*y) + z*z.
@@ -54,6 +53,19 @@ __device__ __forceinline__ double fitted_inside(double x, double ll) {
   else if (x > ll)
     x -= ll * floor(x / ll);
   return x;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Remap the block index so that XCD x works through one contiguous range of
+// logical blocks (a slab of the cell-sorted atoms): the neighbour cells a
+// block gathers from are then in flight on the same XCD and hit its L2.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ long xcd_block() {
+  const long nb = gridDim.x;
+  const long b = blockIdx.x;
+  const long q = nb / kNumXcd, r = nb % kNumXcd;
+  const long x = b % kNumXcd;
+  return x * q + (x < r ? x : r) + b / kNumXcd;
 }
 
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {

@@ -20,332 +20,17 @@
 #include "cells.hpp"
 #include "common.hpp"
 #include "forces.hpp"
+#include "host_util.hpp"
 #include "sort.hpp"
 
 using namespace lpmdx;
 
+namespace lpmdx {
+thread_local std::string g_err;
+}
+
 namespace {
 
-thread_local std::string g_err;
-
-struct Error {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
-
-void hipchk(hipError_t e, const char* what) {
-  if (e != hipSuccess)
-    fail(e == hipErrorOutOfMemory ? LPMDX_ENOMEM : LPMDX_EDEVICE,
-         std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F> int guard(F&& f) {
-  try {
-    f();
-    return LPMDX_OK;
-  } catch (const Error& e) {
-    g_err = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host allocation failed";
-    return LPMDX_ENOMEM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return LPMDX_EINVAL;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// device buffers
-
-template <typename T> struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p)
-      (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void alloc(size_t m) {
-    if (m <= n && p)
-      return;
-    release();
-    if (m == 0)
-      m = 1;
-    hipchk(hipMalloc(&p, m * sizeof(T)), "hipMalloc");
-    n = m;
-  }
-};
-
-// ---------------------------------------------------------------------------
-// plugin keyword strings (Module::ProcessArguments, liblpmd/plugins/module.cc:77-98)
-
-struct Keywords {
-  std::map<std::string, std::string> kv;
-  Keywords(const char* args, std::initializer_list<std::pair<const char*, const char*>> defaults) {
-    for (auto& d : defaults)
-      kv[d.first] = d.second;
-    std::istringstream in(args ? args : "");
-    std::vector<std::string> w;
-    std::string s;
-    while (in >> s)
-      w.push_back(s);
-    // words are taken as key value pairs; a dangling key is the reference's
-    // "Unexpected end of input while reading module arguments" (module.cc:101-104)
-    if (w.size() % 2)
-      fail(LPMDX_EINVAL, "Unexpected end of input while reading module arguments");
-    for (size_t k = 0; k < w.size(); k += 2)
-      kv[w[k]] = w[k + 1];
-  }
-  // Parameter conversions, liblpmd/lpmd/parameter.h:26-28: atof / atoi / "true";
-  // a keyword without default and without value reads as "".
-  double d(const char* k) const {
-    auto it = kv.find(k);
-    return it == kv.end() ? atof("") : atof(it->second.c_str());
-  }
-  int i(const char* k) const {
-    auto it = kv.find(k);
-    return it == kv.end() ? atoi("") : atoi(it->second.c_str());
-  }
-  bool b(const char* k) const {
-    auto it = kv.find(k);
-    return it != kv.end() && it->second == "true";
-  }
-  std::string s(const char* k) const {
-    auto it = kv.find(k);
-    return it == kv.end() ? std::string() : it->second;
-  }
-  bool has(const char* k) const { return kv.count(k) > 0; }
-};
-
-// ---------------------------------------------------------------------------
-// potentials
-
-struct PotParams {
-  int kind = 0;
-  int spc_sum = -2;
-  double cutoff = HUGE_VAL; // Potential::rcutoff default (potential.cc:13)
-  double p[8] = {0};
-  long bins = 0;
-  DBuf<float> tab_e, tab_f;
-  bool metal() const { return kind >= kSuttonChen; }
-};
-
-std::unique_ptr<PotParams> make_potential(const std::string& name, const char* args, int s1, int s2) {
-  auto P = std::make_unique<PotParams>();
-  P->spc_sum = s1 + s2;
-  if (name == "lennardjones") { // plugins/lennardjones.cc:11-22
-    Keywords k(args, {});
-    P->kind = kLJ;
-    P->p[0] = k.d("sigma");
-    P->p[1] = k.d("epsilon");
-    P->cutoff = P->p[2] = k.d("cutoff");
-  } else if (name == "fastlj") { // plugins/fastlj.cc:11-26,64-80
-    Keywords k(args, {{"bins", "500"}});
-    P->kind = kFastLJ;
-    P->p[0] = k.d("sigma");
-    P->p[1] = k.d("epsilon");
-    P->cutoff = P->p[2] = k.d("cutoff");
-    P->bins = k.i("bins");
-    if (P->bins <= 0)
-      fail(LPMDX_EINVAL, "fastlj: bins must be positive");
-    std::vector<float> e(P->bins), f(P->bins);
-    const double sigma = P->p[0], epsilon = P->p[1], cutoff = P->p[2];
-    for (long i = 0; i < P->bins; ++i) {
-      double r = cutoff * double(i) / double(P->bins);
-      double r6 = pow(sigma / r, 6.0e0);
-      double r12 = r6 * r6;
-      e[i] = (float)(4.0e0 * epsilon * (r12 - r6));
-      f[i] = (float)(-48.0e0 * (epsilon / (r * r)) * (r12 - 0.50e0 * r6));
-    }
-    P->tab_e.alloc(P->bins);
-    P->tab_f.alloc(P->bins);
-    hipchk(hipMemcpy(P->tab_e.p, e.data(), P->bins * sizeof(float), hipMemcpyHostToDevice), "table");
-    hipchk(hipMemcpy(P->tab_f.p, f.data(), P->bins * sizeof(float), hipMemcpyHostToDevice), "table");
-  } else if (name == "morse") { // plugins/morse.cc:11-21 (no cutoff keyword)
-    Keywords k(args, {});
-    P->kind = kMorse;
-    P->p[0] = k.d("depth");
-    P->p[1] = k.d("a");
-    P->p[2] = k.d("re");
-  } else if (name == "buckingham") { // plugins/buckingham.cc:11-24
-    Keywords k(args, {});
-    P->kind = kBuckingham;
-    P->p[0] = k.d("B1");
-    P->p[1] = k.d("B2");
-    P->p[2] = k.d("Ro");
-    P->cutoff = P->p[3] = k.d("cutoff");
-  } else if (name == "suttonchen") { // plugins/suttonchen.cc:11-32
-    Keywords k(args, {{"corrections", "false"}});
-    P->kind = kSuttonChen;
-    P->p[0] = k.d("e");
-    P->p[1] = k.d("a");
-    P->p[2] = k.d("n");
-    P->p[3] = k.d("m");
-    P->p[4] = k.d("c");
-    P->cutoff = P->p[5] = k.d("cutoff");
-  } else if (name == "finnissinclair") { // plugins/finnissinclair.cc:10-31
-    Keywords k(args, {});
-    P->kind = kFinnisSinclair;
-    const char* keys[7] = {"c0", "c1", "c2", "A", "B", "c", "d"};
-    for (int q = 0; q < 7; ++q)
-      P->p[q] = k.d(keys[q]);
-    P->cutoff = (P->p[6] >= P->p[5]) ? P->p[6] : P->p[5];
-  } else if (name == "gupta") { // plugins/gupta.cc:10-27
-    Keywords k(args, {});
-    P->kind = kGupta;
-    const char* keys[5] = {"A", "r0", "p", "B", "q"};
-    for (int q = 0; q < 5; ++q)
-      P->p[q] = k.d(keys[q]);
-    P->cutoff = P->p[5] = k.d("cutoff");
-  } else {
-    fail(LPMDX_EINVAL, "unknown potential plugin '" + name + "'");
-  }
-  for (int q = 0; q < 8; ++q)
-    if (!std::isfinite(P->p[q]))
-      fail(LPMDX_ENONFINITE, name + ": non-finite parameter");
-  return P;
-}
-
-int int_exponent(double v) {
-  double r = std::nearbyint(v);
-  return (r == v && r >= 1 && r <= 64) ? (int)r : 0;
-}
-
-DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) {
-  DevPot d{};
-  d.kind = P.kind;
-  d.spc_sum = P.spc_sum;
-  d.rc2 = rc2_force;
-  const double* p = P.p;
-  switch (P.kind) {
-  case kLJ:
-    d.f[0] = (float)(p[0] * p[0]);
-    d.f[1] = (float)(48.0 * p[1]);
-    d.f[2] = (float)(4.0 * p[1]);
-    break;
-  case kFastLJ:
-    d.tab_e = P.tab_e.p;
-    d.tab_f = P.tab_f.p;
-    d.tab_cut = p[2];
-    d.bins = (int)P.bins;
-    break;
-  case kMorse:
-    d.f[0] = (float)p[1];
-    d.f[1] = (float)p[2];
-    d.f[2] = (float)p[0];
-    d.f[3] = (float)(2.0 * p[1] * p[0]);
-    break;
-  case kBuckingham:
-    d.f[0] = (float)p[0];
-    d.f[1] = (float)p[1];
-    d.f[2] = (float)p[2];
-    d.f[3] = (float)(1.0 / p[2]);
-    d.f[4] = (float)(6.0 * p[1]);
-    break;
-  case kSuttonChen:
-    // {e, a, n, m, c, -, -, -, -n e, m c e / 2, -, K = c e}
-    for (int q = 0; q < 5; ++q)
-      d.f[q] = (float)p[q];
-    d.f[8] = (float)(-p[2] * p[0]);
-    d.f[9] = (float)(0.5 * p[3] * p[4] * p[0]);
-    d.ni = int_exponent(p[2]);
-    d.mi = int_exponent(p[3]);
-    d.f[11] = (float)(p[4] * p[0]);
-    break;
-  case kGupta:
-    // {A, r0, p, B, q, p, 2q, 2q/r0, p/r0, -A p/r0, B q/r0, K = B}:
-    // e^{-p(r-r0)/r0} = e^{p - (p/r0) r}, e^{-2q(r-r0)/r0} = e^{2q - (2q/r0) r}
-    for (int q = 0; q < 5; ++q)
-      d.f[q] = (float)p[q];
-    d.f[5] = (float)p[2];
-    d.f[6] = (float)(2.0 * p[4]);
-    d.f[7] = (float)(2.0 * p[4] / p[1]);
-    d.f[8] = (float)(p[2] / p[1]);
-    d.f[9] = (float)(-p[0] * p[2] / p[1]);
-    d.f[10] = (float)(p[3] * p[4] / p[1]);
-    d.f[11] = (float)p[3];
-    break;
-  case kFinnisSinclair:
-    // {c0, c1, c2, A, B, c, d, -, B/d, 3B/d, -A/2, K = A}
-    for (int q = 0; q < 7; ++q)
-      d.f[q] = (float)p[q];
-    d.f[8] = (float)(p[4] / p[6]);
-    d.f[9] = (float)(3.0 * p[4] / p[6]);
-    d.f[10] = (float)(-0.5 * p[3]);
-    d.f[11] = (float)p[3];
-    break;
-  }
-  d.tab_cut = P.kind == kFastLJ ? p[2] : 0.0;
-  // rho pass filter travels in the (otherwise unused) FP64 slot below
-  (void)rc2_rho;
-  return d;
-}
-
-// ---------------------------------------------------------------------------
-// cell managers
-
-enum CmKind { kNone = 0, kCmLinkedCell = 1, kCmVerlet = 2 };
-
-struct CellMgr {
-  int kind = kNone;
-  double cutoff = 0.0;
-  // linkedcell (plugins/linkedcell.cc:17-41)
-  int nx = 0, ny = 0, nz = 0, side = 0;
-  bool mode_auto = false, warn_outside = true, grid_fixed = false;
-  // verletlist (plugins/verletlist.cc:12-29)
-  double extcut = 0.0, skin = -1.0;
-  long maxneighbors = 80, each = 5;
-  long ucalls = 0;
-};
-
-// linkedcell.cc:146-165
-int lc_auto_n(double edge, double cutoff) {
-  double fnn = (cutoff / 2.5);
-  int n = 1;
-  while (true) {
-    double previo = fabs(edge / n - fnn);
-    n++;
-    double actual = fabs(edge / n - fnn);
-    if (actual < 1E-3)
-      break;
-    else if (actual > previo) {
-      n--;
-      break;
-    }
-  }
-  if ((n % 2) != 0)
-    n--;
-  return n;
-}
-
-double edge_module(double L) { return sqrt(L * L + 0.0 * 0.0 + 0.0 * 0.0); }
-
-double ElemMass(int z) {
-  // liblpmd/support/elements.cc:20-35 for the species the engine is used with
-  switch (z) {
-  case 1: return 1.00794;
-  case 11: return 22.989768;
-  case 13: return 26.981539;
-  case 17: return 35.4527;
-  case 18: return 39.948;
-  case 26: return 55.847;
-  case 28: return 58.6934;
-  case 29: return 63.546;
-  case 47: return 107.8682;
-  case 79: return 196.96654;
-  default: return -1.0;
-  }
-}
-
-long blocks_for(long threads) { return (threads + kBlock - 1) / kBlock; }
 
 // timing slots of lpmdx_system_profile_read
 enum ProfSlot { kProfForce = 0, kProfRho = 1, kProfBuild = 2, kProfOther = 3, kProfSlots = 4 };
@@ -397,6 +82,7 @@ struct lpmdx_system {
   Grid grid{};
   // lists
   DBuf<int> nbr, cnt;
+  DBuf<float4> xf;
   long stride = 0;
   int cap = 0;
   int lpa = 1, lpa_req = 0;
@@ -630,7 +316,7 @@ struct lpmdx_system {
   void build_list() {
     // list storage: cap entries per atom, rounded to the lane count
     lpa = auto_lanes();
-    stride = ((n + kWave - 1) / kWave) * kWave;
+    stride = ((n + kWave - 1) / kWave) * kWave; // whole tiles for every lane count
     partials.alloc((size_t)blocks_for(n * 16) * 12);
     partials2.alloc((size_t)blocks_for(n * 16) * 12);
     if (cap == 0) {
@@ -643,13 +329,51 @@ struct lpmdx_system {
       cap = (int)std::min<double>(n * 8.0 + 64, est * 1.3 + 32);
     }
     for (int attempt = 0; attempt < 3; ++attempt) {
-      cap = ((cap + lpa - 1) / lpa) * lpa;
+      cap = ((cap + kChunk - 1) / kChunk) * kChunk; // whole 8-entry chunks
       nbr.alloc((size_t)cap * stride);
       cnt.alloc(n);
       // max_count, pad0, sum_count: 16 contiguous bytes
       hipchk(hipMemsetAsync(&ctrl.p->max_count, 0, 16, st), "memset");
-      BuildArgs B{x.p, keys.p, cstart.p, cend.p, nbr.p, cnt.p, &ctrl.p->max_count,
-                  &ctrl.p->sum_count, grid, box, rl * rl, n, stride, cap, lpa};
+      const double rl2 = rl * rl;
+      // FP32 positions are off by at most ulp(Lmax)/2; two of them in a distance
+      const double lmax = std::max(box.L[0], std::max(box.L[1], box.L[2]));
+      const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
+      const float rl2f = std::isfinite(rl2) ? (float)((rl + 2.0 * pos_err) * (rl + 2.0 * pos_err) * (1.0 + 1e-6))
+                                            : INFINITY;
+      // FP32 paths need the minimum image to be unambiguous within the list radius
+      const double hmin = std::min(box.half[0], std::min(box.half[1], box.half[2]));
+      int path = kPathExact64;
+      if (std::isfinite(rl2) && hmin > rl + 4.0 * pos_err + 1e-9 * hmin) {
+        if (grid.mode == kExact && grid.nx >= 3 && grid.ny >= 3 && grid.nz >= 3)
+          path = kPathShift32;
+        else
+          path = kPathMin32;
+      }
+      xf.alloc(n);
+      k_to_f32<<<blocks_for(n), kBlock, 0, st>>>(x.p, n, box, xf.p);
+      BuildArgs B{};
+      B.x = x.p;
+      B.xf = xf.p;
+      B.skeys = keys.p;
+      B.cstart = cstart.p;
+      B.cend = cend.p;
+      B.nbr = nbr.p;
+      B.cnt = cnt.p;
+      B.max_count = &ctrl.p->max_count;
+      B.sum_count = &ctrl.p->sum_count;
+      B.g = grid;
+      B.b = box;
+      for (int q = 0; q < 3; ++q) {
+        B.Lf[q] = (float)box.L[q];
+        B.halff[q] = (float)box.half[q];
+      }
+      B.rl2 = rl2;
+      B.rl2f = rl2f;
+      B.n = n;
+      B.stride = stride;
+      B.cap = cap;
+      B.lpa = lpa;
+      B.path = path;
       hipEvent_t e0 = prof_begin();
       k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
       prof_end(kProfBuild, e0);
@@ -796,8 +520,11 @@ struct lpmdx_system {
       rc2_force = P.metal() ? h : std::min(h, own);
       rc2_rho = f;
     } else {
-      rc2_force = own;
-      rc2_rho = own;
+      // the stored list may hold a few pairs just beyond its radius (the build
+      // admits with a small FP32 margin); a potential without a finite cutoff
+      // takes the list radius itself
+      rc2_force = std::min(own, rl * rl);
+      rc2_rho = rc2_force;
     }
   }
 
@@ -830,6 +557,7 @@ struct lpmdx_system {
       A.dt = dt;
       A.n = n;
       A.stride = stride;
+      A.cap = cap;
       A.mode = fused ? mode : kForcesOnly;
       A.accumulate = q > 0;
       if (P.metal()) {
@@ -1442,7 +1170,6 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
     std::vector<long> iperm(n);
     for (long i = 0; i < n; ++i)
       iperm[perm[i]] = i;
-    const long rowlen = S->stride * S->lpa;
     const double rc2 = rcut * rcut;
     long k = 0;
     std::vector<std::pair<long, double>> row;
@@ -1450,7 +1177,7 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
       const long i = iperm[o];
       row.clear();
       for (int c = 0; c < cntv[i]; ++c) {
-        const long jj = nb[(long)(c / S->lpa) * rowlen + i * S->lpa + (c % S->lpa)];
+        const long jj = nb[list_index(i, c, S->lpa, S->cap)];
         const long oj = perm[jj];
         if (!full && S->cm.kind == kCmLinkedCell && oj > o)
           continue; // linkedcell half list keeps j < i (linkedcell.cc:323)

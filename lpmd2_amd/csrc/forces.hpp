@@ -14,6 +14,7 @@
 // direction.
 #pragma once
 
+#include "cells.hpp"
 #include "common.hpp"
 
 namespace lpmdx {
@@ -185,6 +186,7 @@ struct ForceArgs {
   double dt;
   long n;
   long stride;
+  int cap; // list entries per atom (a multiple of the lanes per atom)
   int mode;
   int accumulate; // add to a instead of overwriting (several potentials)
 };
@@ -302,15 +304,26 @@ template <int LPA, typename Body>
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, long i, int l, double xi,
                                                double yi, double zi, Body&& body) {
   const int c = A.cnt[i];
-  const long rowlen = A.stride * LPA;
-  const int* __restrict__ nb = A.nbr + i * LPA + l;
+  constexpr int T = kWave / LPA;
+  // entry k of this atom: chunk k/8 of the tile row, word k%8 (see list_index)
+  const int* __restrict__ nb = A.nbr + (i / T) * (long)T * A.cap + (i % T) * kChunk;
   for (int k0 = l; k0 < c; k0 += kUnroll * LPA) {
     int js[kUnroll];
     double px[kUnroll], py[kUnroll], pz[kUnroll];
+    if constexpr (LPA == 1 && kUnroll == 4) {
+      // k0 is a multiple of 4: entries k0..k0+3 are one aligned 16-byte word
+      // group of the chunk; slots past the list end are replaced by the atom itself
+      const int4 q = *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
+      js[0] = q.x;
+      js[1] = k0 + 1 < c ? q.y : (int)i;
+      js[2] = k0 + 2 < c ? q.z : (int)i;
+      js[3] = k0 + 3 < c ? q.w : (int)i;
+    } else {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int k = k0 + u * LPA;
-      js[u] = k < c ? nb[(long)(k / LPA) * rowlen] : (int)i;
+      for (int u = 0; u < kUnroll; ++u) {
+        const int k = k0 + u * LPA;
+        js[u] = k < c ? nb[(long)(k / kChunk) * (T * kChunk) + (k % kChunk)] : (int)i;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -321,11 +334,11 @@ __device__ __forceinline__ void for_neighbours(const ForceArgs& A, long i, int l
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      if (k0 + u * LPA < c) {
-        double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
-        min_image(A.b, dx, dy, dz);
-        body(js[u], dx, dy, dz, sqmod(dx, dy, dz));
-      }
+      double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
+      min_image(A.b, dx, dy, dz);
+      // entries past the list end point at the atom itself (r2 = 0); the body
+      // evaluates every slot and masks its contribution with `valid`
+      body(k0 + u * LPA < c, js[u], dx, dy, dz, sqmod(dx, dy, dz));
     }
   }
 }
@@ -344,7 +357,7 @@ struct ForceSum {
 
 template <int KIND, int LPA, bool EVAL>
 __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = xcd_block() * kBlock + threadIdx.x;
   const long i = t / LPA;
   const int l = (int)(t % LPA);
   const bool active = i < A.n;
@@ -355,15 +368,15 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
     const int si = A.z[i];
     const bool all = A.P.spc_sum < 0;
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](int j, double dx, double dy, double dz, double r2) {
-      if (r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum)) {
-        const float fr2 = (float)r2;
-        float fs, e;
-        pair_eval<KIND>(A.P, r2, fr2, fs, e);
-        F.add(fs, dx, dy, dz);
-        if constexpr (EVAL)
-          tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, e);
-      }
+    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int j, double dx, double dy, double dz, double r2) {
+      const bool in = valid && r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum);
+      const float fr2 = in ? (float)r2 : 1.0f;
+      float fs, e;
+      pair_eval<KIND>(A.P, r2, fr2, fs, e);
+      fs = in ? fs : 0.0f;
+      F.add(fs, dx, dy, dz);
+      if constexpr (EVAL)
+        tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, in ? e : 0.0f);
     });
   }
   const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
@@ -383,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
 // energy F(rho_i) = -K sqrt(rho_i) into the block partials.
 template <int KIND, int LPA, bool EVAL>
 __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = xcd_block() * kBlock + threadIdx.x;
   const long i = t / LPA;
   const int l = (int)(t % LPA);
   const bool active = i < A.n;
@@ -391,9 +404,10 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, d
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](int, double, double, double, double r2) {
-      if (r2 < rc2)
-        rho += (double)eam_rho<KIND>(A.P, (float)r2);
+    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int, double, double, double, double r2) {
+      const bool in = valid && r2 < rc2;
+      const float v = eam_rho<KIND>(A.P, in ? (float)r2 : 1.0f);
+      rho += in ? (double)v : 0.0;
     });
   }
   rho = group_sum<LPA>(rho);
@@ -414,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, d
 // (metalpotential.cc:140-169), fused with the integrator tail.
 template <int KIND, int LPA, bool EVAL>
 __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long t = xcd_block() * kBlock + threadIdx.x;
   const long i = t / LPA;
   const int l = (int)(t % LPA);
   const bool active = i < A.n;
@@ -426,15 +440,15 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
     const float wi = A.w_rho[i];
     const bool all = A.P.spc_sum < 0;
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](int j, double dx, double dy, double dz, double r2) {
-      if (r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum)) {
-        const float fr2 = (float)r2;
-        float fs, V;
-        eam_pair<KIND>(A.P, fr2, wi + A.w_rho[j], fs, V);
-        F.add(fs, dx, dy, dz);
-        if constexpr (EVAL)
-          tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, V);
-      }
+    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int j, double dx, double dy, double dz, double r2) {
+      const bool in = valid && r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum);
+      const float fr2 = in ? (float)r2 : 1.0f;
+      float fs, V;
+      eam_pair<KIND>(A.P, fr2, wi + A.w_rho[j], fs, V);
+      fs = in ? fs : 0.0f;
+      F.add(fs, dx, dy, dz);
+      if constexpr (EVAL)
+        tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, in ? V : 0.0f);
     });
   }
   const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
