@@ -145,6 +145,49 @@ LPMDX_API int lpmdx_system_profile_read(lpmdx_system* sys, double ms[4], long la
 /* Lanes per atom used by the force kernels (1, 2, 4, 8 or 16); 0 = automatic. */
 LPMDX_API int lpmdx_system_set_lanes(lpmdx_system* sys, int lanes_per_atom);
 
+/* ------------------------------------------------------------------------
+ * Batched engine: many independent periodic systems of uneven size in shared
+ * HBM arenas, advanced together (one HIP graph per step; every per-system
+ * decision is taken on the device).  No reference counterpart: it is the
+ * engine behind running many lpmd jobs (lpmd/lpmd.cc:58-108, one Configuration
+ * each) at once.  Each system uses verletlist semantics with a skin (the exact
+ * minimum-image set, plugins/verletlist.cc:135-167) and the velocityverlet
+ * integrator; nothing a system computes depends on the others.
+ * ---------------------------------------------------------------------- */
+typedef struct lpmdx_batch lpmdx_batch;
+
+/* Arenas: atom slots, neighbour-list words and cell slots. */
+LPMDX_API int lpmdx_batch_create(lpmdx_batch** out, long atom_capacity, long list_words, long cell_slots);
+LPMDX_API int lpmdx_batch_destroy(lpmdx_batch* b);
+/* Adds one system (AoS FP64 x, v; v may be NULL) with 1-4 potentials of one
+ * kind (names[k], args[k] as lpmdx_potential_append, species s1[k], s2[k] or
+ * NULL for wildcards) and cell-manager keywords "cutoff C skin S
+ * [maxneighbors M]"; it will run `steps` velocity-Verlet steps of dt (fs).
+ * *sid receives its id.  LPMDX_ENOMEM when an arena has no room. */
+LPMDX_API int lpmdx_batch_add(lpmdx_batch* b, long n, const double box[3], const int* z, const double* mass,
+                              const double* x, const double* v, const char* cm_args, int npot,
+                              const char** names, const char** args, const int* s1, const int* s2,
+                              double dt, long steps, long* sid);
+LPMDX_API int lpmdx_batch_remove(lpmdx_batch* b, long sid);
+/* Advances every system by up to `steps` steps (a new system's first step is
+ * its initial force evaluation); returns when the device is done. */
+LPMDX_API int lpmdx_batch_run(lpmdx_batch* b, long steps);
+/* State of one system in its caller's atom order; info[8] = steps done, steps
+ * left, list builds, started, longest list, list capacity, lanes, overflow. */
+LPMDX_API int lpmdx_batch_get(lpmdx_batch* b, long sid, double* x, double* v, double* a, long info[8]);
+/* Potential energy, virial and kinetic energy of one system at its last step. */
+LPMDX_API int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]);
+/* Times trial_steps steps per candidate lane count (1, 2, 4) for each
+ * potential kind on a snapshot of the batch and keeps the fastest; lanes_out[kind]. */
+LPMDX_API int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[8]);
+LPMDX_API int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes);
+/* [0] live systems, [1] atoms, [2] atom capacity, [3] largest free atom run,
+ * [4] largest free list run (words), [5] largest free cell run, [6] steps run,
+ * [7] system slots. */
+LPMDX_API int lpmdx_batch_info(lpmdx_batch* b, long out[8]);
+/* HBM bytes one atom slot takes in the per-atom arenas (for admission budgets). */
+LPMDX_API long lpmdx_batch_bytes_per_atom(void);
+
 #ifdef __cplusplus
 }
 #endif

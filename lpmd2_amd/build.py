@@ -6,8 +6,9 @@ import subprocess
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
-SOURCES = [os.path.join(_HERE, "csrc", "engine.hip")]
-HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp")] + [
+SOURCES = [os.path.join(_HERE, "csrc", f) for f in ("engine.hip", "batch.hip")]
+HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp",
+                                                     "batch.hpp", "host_util.hpp")] + [
     os.path.join(ROOT, "include", "lpmdx.h")
 ]
 OUT = os.path.join(_HERE, "liblpmdx.so")

@@ -1,0 +1,858 @@
+// Host side of the batched engine (lpmdx_batch_* in include/lpmdx.h): many
+// independent periodic systems in shared HBM arenas, advanced together by a
+// HIP graph that replays one step of fixed launches (see batch.hpp).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+
+#include "batch.hpp"
+#include "host_util.hpp"
+
+using namespace lpmdx;
+
+namespace {
+
+// First-fit allocator over [0, cap) in whole units.
+struct Arena {
+  long cap = 0;
+  std::map<long, long> free; // start -> length
+  void init(long c) {
+    cap = c;
+    free.clear();
+    if (c > 0)
+      free[0] = c;
+  }
+  long alloc(long len) {
+    if (len <= 0)
+      return 0;
+    for (auto it = free.begin(); it != free.end(); ++it) {
+      if (it->second >= len) {
+        const long s = it->first, l = it->second;
+        free.erase(it);
+        if (l > len)
+          free[s + len] = l - len;
+        return s;
+      }
+    }
+    return -1;
+  }
+  void release(long s, long len) {
+    if (len <= 0)
+      return;
+    auto it = free.emplace(s, len).first;
+    auto nx = std::next(it);
+    if (nx != free.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free.erase(nx);
+    }
+    if (it != free.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free.erase(it);
+      }
+    }
+  }
+  long largest() const {
+    long m = 0;
+    for (auto& f : free)
+      m = std::max(m, f.second);
+    return m;
+  }
+};
+
+struct HostSys {
+  bool live = false;
+  long n = 0, tiles = 0, atom0 = 0, list0 = 0, listw = 0, cell0 = 0, cells = 0;
+  std::vector<std::unique_ptr<PotParams>> pots;
+  int kind = 0;
+};
+
+constexpr long kBytesPerAtom = 240; // per atom slot over all per-atom arenas
+
+} // namespace
+
+struct lpmdx_batch {
+  hipStream_t st = nullptr;
+  long acap = 0, lcap = 0, ccap = 0;
+  Arena atoms, lists, cells; // atoms counted in tiles
+  // per-atom arenas
+  DBuf<double> x0, x1, v, a, sv, sa, ffm, mass, sffm, smass, rho;
+  DBuf<int> z, sz, perm, sperm, cnt, nbr, cstart, cend, tsys;
+  DBuf<float4> xf;
+  DBuf<float> w;
+  DBuf<uint32_t> k0, k1, v0, v1;
+  // tables
+  std::vector<BSys> hsys;
+  std::vector<HostSys> meta;
+  DBuf<BSys> dsys;
+  DBuf<BState> dst;
+  BState* hst = nullptr; // pinned mirror
+  // tile lists
+  DBuf<int> tiles_all;
+  long n_all = 0;
+  struct Group {
+    int kind, lpa;
+    bool metal;
+    DBuf<int> tiles;
+    long n = 0;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  int lpa_of_kind[8] = {0, 1, 1, 1, 1, 1, 1, 1};
+  // graph of one step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool dirty = true;
+  // evaluation scratch
+  DBuf<double> evala, partials, red;
+  double* hred = nullptr;
+  long steps_run = 0;
+
+  ~lpmdx_batch() {
+    if (st)
+      (void)hipStreamSynchronize(st);
+    if (gexec)
+      (void)hipGraphExecDestroy(gexec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    if (hst)
+      (void)hipHostFree(hst);
+    if (hred)
+      (void)hipHostFree(hred);
+    meta.clear();
+    if (st)
+      (void)hipStreamDestroy(st);
+  }
+
+  void sync() { hipchk(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+
+  BArena arena() {
+    BArena R{};
+    R.x[0] = x0.p;
+    R.x[1] = x1.p;
+    R.v = v.p;
+    R.a = a.p;
+    R.z = z.p;
+    R.ffm = ffm.p;
+    R.mass = mass.p;
+    R.perm = perm.p;
+    R.xf = xf.p;
+    R.w = w.p;
+    R.rho = rho.p;
+    R.cnt = cnt.p;
+    R.nbr = nbr.p;
+    R.cstart = cstart.p;
+    R.cend = cend.p;
+    R.keys[0] = k0.p;
+    R.keys[1] = k1.p;
+    R.vals[0] = v0.p;
+    R.vals[1] = v1.p;
+    R.sv = sv.p;
+    R.sa = sa.p;
+    R.sz = sz.p;
+    R.sffm = sffm.p;
+    R.smass = smass.p;
+    R.sperm = sperm.p;
+    return R;
+  }
+
+  BArgs args(const int* tiles, long ntiles) {
+    BArgs B{};
+    B.sys = dsys.p;
+    B.st = dst.p;
+    B.R = arena();
+    B.tiles = tiles;
+    B.tsys = tsys.p;
+    B.ntiles = ntiles;
+    B.nsys = (int)hsys.size();
+    return B;
+  }
+
+  void create(long atom_cap, long list_words, long cell_slots) {
+    acap = ((atom_cap + kTile - 1) / kTile) * kTile;
+    lcap = list_words;
+    ccap = cell_slots;
+    atoms.init(acap / kTile);
+    lists.init(lcap / kTile);
+    cells.init(ccap);
+    const size_t A = std::max<long>(acap, kTile);
+    for (DBuf<double>* b : {&x0, &x1, &v, &a, &sv, &sa})
+      b->alloc(3 * A);
+    for (DBuf<double>* b : {&ffm, &mass, &sffm, &smass, &rho})
+      b->alloc(A);
+    for (DBuf<int>* b : {&z, &sz, &perm, &sperm, &cnt})
+      b->alloc(A);
+    for (DBuf<uint32_t>* b : {&k0, &k1, &v0, &v1})
+      b->alloc(A);
+    xf.alloc(A);
+    w.alloc(A);
+    tsys.alloc(A / kTile);
+    nbr.alloc(std::max<long>(lcap, kTile));
+    cstart.alloc(std::max<long>(ccap, 1));
+    cend.alloc(std::max<long>(ccap, 1));
+    hipchk(hipMemsetAsync(x0.p, 0, 3 * A * sizeof(double), st), "memset");
+    hipchk(hipMemsetAsync(x1.p, 0, 3 * A * sizeof(double), st), "memset");
+    hipchk(hipMemsetAsync(nbr.p, 0, std::max<long>(lcap, kTile) * sizeof(int), st), "memset");
+    hipchk(hipHostMalloc(&hred, 16 * sizeof(double)), "hipHostMalloc");
+    red.alloc(16);
+    sync();
+  }
+
+  // ---------------------------------------------------------------- systems
+  long add(long n, const double box[3], const int* zz, const double* mm, const double* x,
+           const double* vel, const char* cm_args, int npot, const char** names, const char** pargs,
+           const int* s1, const int* s2, double dt, long steps) {
+    if (n <= 0)
+      fail(LPMDX_EINVAL, "a batched system needs at least one atom");
+    if (npot < 1 || npot > kMaxTerms)
+      fail(LPMDX_EINVAL, "a batched system takes 1 to 4 potentials");
+    need(box, "box");
+    need(zz, "z");
+    need(x, "x");
+    finite(box, 3, "box");
+    finite(x, 3 * n, "positions");
+    if (vel)
+      finite(vel, 3 * n, "velocities");
+    if (!std::isfinite(dt) || steps < 0)
+      fail(LPMDX_EINVAL, "bad dt or step count");
+    HostSys H;
+    BSys S{};
+    for (int q = 0; q < npot; ++q) {
+      need(names[q], "potential name");
+      H.pots.push_back(make_potential(names[q], pargs ? pargs[q] : "", s1 ? s1[q] : -1, s2 ? s2[q] : -1));
+      if (H.pots.back()->kind != H.pots[0]->kind)
+        fail(LPMDX_EINVAL, "the potentials of one batched system must be of one kind");
+    }
+    H.kind = H.pots[0]->kind;
+    const bool metal = H.pots[0]->metal();
+    if (metal && npot > 1)
+      fail(LPMDX_EINVAL, "a batched EAM system takes one potential");
+    // the cell manager keywords of the plain path's verletlist with a skin
+    Keywords k(cm_args, {{"cutoff", "0.0"}, {"skin", "0.3"}, {"maxneighbors", "0"}});
+    double cutoff = k.d("cutoff");
+    if (!(cutoff > 0.0)) {
+      cutoff = 0.0;
+      for (auto& p : H.pots)
+        cutoff = std::max(cutoff, std::isfinite(p->cutoff) ? p->cutoff : 0.0);
+    }
+    if (!(cutoff > 0.0))
+      fail(LPMDX_EINVAL, "batched system needs a finite cutoff");
+    const double skin = k.d("skin");
+    if (!(skin >= 0.0))
+      fail(LPMDX_EINVAL, "skin must be non-negative");
+    const double rl = cutoff + skin;
+    for (int q = 0; q < 3; ++q) {
+      if (!(box[q] > 0.0))
+        fail(LPMDX_EINVAL, "box edges must be positive");
+      S.b.L[q] = box[q];
+      S.b.half[q] = 0.5 * box[q];
+      S.ll[q] = edge_module(box[q]);
+      S.Lf[q] = (float)box[q];
+      S.halff[q] = (float)S.b.half[q];
+    }
+    int nq[3];
+    for (int q = 0; q < 3; ++q)
+      nq[q] = (int)std::max(1.0, std::min(std::floor(box[q] / (rl * (1.0 + 1e-9))), 512.0));
+    S.g.nx = nq[0];
+    S.g.ny = nq[1];
+    S.g.nz = nq[2];
+    S.g.side = 1;
+    S.g.mode = kExact;
+    S.g.ncell = nq[0] * nq[1] * nq[2];
+    const double lmax = std::max(box[0], std::max(box[1], box[2]));
+    const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
+    const double hmin = std::min(S.b.half[0], std::min(S.b.half[1], S.b.half[2]));
+    S.path = kPathExact64;
+    if (hmin > rl + 4.0 * pos_err + 1e-9 * hmin)
+      S.path = (nq[0] >= 3 && nq[1] >= 3 && nq[2] >= 3) ? kPathShift32 : kPathMin32;
+    S.rl2 = rl * rl;
+    S.rl2f = (float)((rl + 2.0 * pos_err) * (rl + 2.0 * pos_err) * (1.0 + 1e-6));
+    S.budget = 0.5 * skin;
+    S.dt = dt;
+    S.kind = H.kind;
+    S.metal = metal;
+    S.nterm = npot;
+    for (int q = 0; q < npot; ++q) {
+      const double own = H.pots[q]->cutoff * H.pots[q]->cutoff;
+      const double f = std::min(own, rl * rl);
+      S.P[q] = device_view(*H.pots[q], f, f);
+    }
+    const double vol = box[0] * box[1] * box[2];
+    const double est = (double)n / vol * 4.18879 * rl * rl * rl;
+    long cap = (long)std::min<double>((double)n, est * 1.5 + 32);
+    const long maxn = k.i("maxneighbors");
+    if (maxn > 0)
+      cap = std::max(cap, maxn);
+    cap = ((cap + kChunk - 1) / kChunk) * kChunk;
+    const long tiles = (n + kTile - 1) / kTile;
+    const long t0 = atoms.alloc(tiles);
+    if (t0 < 0)
+      fail(LPMDX_ENOMEM, "atom arena full");
+    const long lw = tiles * kTile * cap;
+    const long l0 = lists.alloc((lw + kTile - 1) / kTile);
+    if (l0 < 0) {
+      atoms.release(t0, tiles);
+      fail(LPMDX_ENOMEM, "list arena full");
+    }
+    const long c0 = cells.alloc(S.g.ncell + 1);
+    if (c0 < 0) {
+      atoms.release(t0, tiles);
+      lists.release(l0, (lw + kTile - 1) / kTile);
+      fail(LPMDX_ENOMEM, "cell arena full");
+    }
+    S.live = 1;
+    S.a0 = t0 * kTile;
+    S.n = (int)n;
+    S.ntile = (int)tiles;
+    S.c0 = c0;
+    S.l0 = l0 * kTile;
+    S.cap = (int)cap;
+    S.lpa = lpa_of_kind[H.kind];
+    H.live = true;
+    H.n = n;
+    H.tiles = tiles;
+    H.atom0 = S.a0;
+    H.list0 = l0;
+    H.listw = (lw + kTile - 1) / kTile;
+    H.cell0 = c0;
+    H.cells = S.g.ncell + 1;
+    // upload the atoms
+    std::vector<double> ff(n), ms(n);
+    std::vector<int> pm(n), zz2(n);
+    for (long i = 0; i < n; ++i) {
+      const double mi = mm ? mm[i] : ElemMass(zz[i]);
+      if (!(mi > 0.0) || !std::isfinite(mi))
+        fail(LPMDX_EINVAL, "no mass for atomic number " + std::to_string(zz[i]));
+      ff[i] = kForceFactor / mi;
+      ms[i] = mi;
+      pm[i] = (int)i;
+      zz2[i] = zz[i];
+    }
+    std::vector<double> zero(3 * n, 0.0);
+    const long a0 = S.a0;
+    hipchk(hipMemcpyAsync(x0.p + 3 * a0, x, 3 * n * sizeof(double), hipMemcpyHostToDevice, st), "memcpy");
+    hipchk(hipMemcpyAsync(v.p + 3 * a0, vel ? vel : zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, st),
+           "memcpy");
+    hipchk(hipMemcpyAsync(a.p + 3 * a0, zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, st), "memcpy");
+    hipchk(hipMemcpyAsync(ffm.p + a0, ff.data(), n * sizeof(double), hipMemcpyHostToDevice, st), "memcpy");
+    hipchk(hipMemcpyAsync(mass.p + a0, ms.data(), n * sizeof(double), hipMemcpyHostToDevice, st), "memcpy");
+    hipchk(hipMemcpyAsync(z.p + a0, zz2.data(), n * sizeof(int), hipMemcpyHostToDevice, st), "memcpy");
+    hipchk(hipMemcpyAsync(perm.p + a0, pm.data(), n * sizeof(int), hipMemcpyHostToDevice, st), "memcpy");
+    // slot
+    long sid = -1;
+    for (size_t q = 0; q < hsys.size(); ++q)
+      if (!meta[q].live) {
+        sid = (long)q;
+        break;
+      }
+    if (sid < 0) {
+      sid = (long)hsys.size();
+      hsys.push_back(S);
+      meta.push_back(std::move(H));
+    } else {
+      hsys[sid] = S;
+      meta[sid] = std::move(H);
+    }
+    BState s0{};
+    s0.left = steps;
+    s0.init = 1;
+    grow_state();
+    hst[sid] = s0;
+    hipchk(hipMemcpyAsync(dst.p + sid, &hst[sid], sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+    sync();
+    dirty = true;
+    return sid;
+  }
+
+  void grow_state() {
+    const size_t m = std::max<size_t>(hsys.size(), 1);
+    if (dst.n >= m && hst)
+      return;
+    const size_t cap2 = std::max<size_t>(m, 2 * dst.n);
+    DBuf<BState> nd;
+    nd.alloc(cap2);
+    BState* nh = nullptr;
+    hipchk(hipHostMalloc(&nh, cap2 * sizeof(BState)), "hipHostMalloc");
+    memset(nh, 0, cap2 * sizeof(BState));
+    if (dst.p && hst) {
+      hipchk(hipMemcpyAsync(nd.p, dst.p, dst.n * sizeof(BState), hipMemcpyDeviceToDevice, st), "memcpy");
+      memcpy(nh, hst, dst.n * sizeof(BState));
+    } else {
+      hipchk(hipMemsetAsync(nd.p, 0, cap2 * sizeof(BState), st), "memset");
+    }
+    sync();
+    std::swap(dst.p, nd.p);
+    std::swap(dst.n, nd.n);
+    if (hst)
+      (void)hipHostFree(hst);
+    hst = nh;
+  }
+
+  void remove(long sid) {
+    check_sid(sid);
+    HostSys& H = meta[sid];
+    atoms.release(H.atom0 / kTile, H.tiles);
+    lists.release(H.list0, H.listw);
+    cells.release(H.cell0, H.cells);
+    H = HostSys();
+    hsys[sid].live = 0;
+    hst[sid] = BState{};
+    hipchk(hipMemcpyAsync(dst.p + sid, &hst[sid], sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+    sync();
+    dirty = true;
+  }
+
+  void check_sid(long sid) const {
+    if (sid < 0 || sid >= (long)meta.size() || !meta[sid].live)
+      fail(LPMDX_EINVAL, "no such system in the batch");
+  }
+
+  // ---------------------------------------------------------------- launch plan
+  void plan() {
+    // system table
+    for (size_t s = 0; s < hsys.size(); ++s)
+      if (meta[s].live)
+        hsys[s].lpa = lpa_of_kind[hsys[s].kind];
+    dsys.alloc(std::max<size_t>(hsys.size(), 1));
+    if (!hsys.empty())
+      hipchk(hipMemcpyAsync(dsys.p, hsys.data(), hsys.size() * sizeof(BSys), hipMemcpyHostToDevice, st), "memcpy");
+    // tiles
+    std::vector<int> ts(acap / kTile, 0), all;
+    std::map<int, std::vector<int>> by_kind;
+    for (size_t s = 0; s < hsys.size(); ++s) {
+      if (!meta[s].live)
+        continue;
+      const long t0 = hsys[s].a0 / kTile;
+      for (long t = 0; t < meta[s].tiles; ++t) {
+        ts[t0 + t] = (int)s;
+        all.push_back((int)(t0 + t));
+        by_kind[hsys[s].kind].push_back((int)(t0 + t));
+      }
+    }
+    hipchk(hipMemcpyAsync(tsys.p, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, st), "memcpy");
+    tiles_all.alloc(std::max<size_t>(all.size(), 1));
+    n_all = (long)all.size();
+    if (!all.empty())
+      hipchk(hipMemcpyAsync(tiles_all.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, st), "memcpy");
+    groups.clear();
+    for (auto& kv : by_kind) {
+      auto G = std::make_unique<Group>();
+      G->kind = kv.first;
+      G->lpa = lpa_of_kind[kv.first];
+      G->metal = kv.first >= kSuttonChen;
+      G->n = (long)kv.second.size();
+      G->tiles.alloc(kv.second.size());
+      hipchk(hipMemcpyAsync(G->tiles.p, kv.second.data(), kv.second.size() * sizeof(int), hipMemcpyHostToDevice, st),
+             "memcpy");
+      groups.push_back(std::move(G));
+    }
+    sync();
+    capture();
+    dirty = false;
+  }
+
+  template <int KIND> void launch_group(Group& G) {
+    BArgs B = args(G.tiles.p, G.n);
+    const long thr = G.n * kTile * G.lpa;
+    const long nb = blocks_for(thr);
+    switch (G.lpa) {
+    case 1:
+      if (G.metal)
+        k_b_rho<KIND, 1><<<nb, kBlock, 0, st>>>(B);
+      k_b_force<KIND, 1><<<nb, kBlock, 0, st>>>(B);
+      break;
+    case 2:
+      if (G.metal)
+        k_b_rho<KIND, 2><<<nb, kBlock, 0, st>>>(B);
+      k_b_force<KIND, 2><<<nb, kBlock, 0, st>>>(B);
+      break;
+    default:
+      if (G.metal)
+        k_b_rho<KIND, 4><<<nb, kBlock, 0, st>>>(B);
+      k_b_force<KIND, 4><<<nb, kBlock, 0, st>>>(B);
+      break;
+    }
+  }
+
+  void enqueue_step() {
+    const int ns = (int)hsys.size();
+    BArgs B = args(tiles_all.p, n_all);
+    k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B);
+    k_b_sort<<<ns, kSortThreads, 0, st>>>(B);
+    if (n_all)
+      k_b_build<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
+    for (auto& G : groups) {
+      switch (G->kind) {
+      case kLJ: launch_group<kLJ>(*G); break;
+      case kFastLJ: launch_group<kFastLJ>(*G); break;
+      case kMorse: launch_group<kMorse>(*G); break;
+      case kBuckingham: launch_group<kBuckingham>(*G); break;
+      case kSuttonChen: launch_group<kSuttonChen>(*G); break;
+      case kFinnisSinclair: launch_group<kFinnisSinclair>(*G); break;
+      case kGupta: launch_group<kGupta>(*G); break;
+      }
+    }
+    hipchk(hipGetLastError(), "kernel launch");
+  }
+
+  void capture() {
+    if (gexec)
+      (void)hipGraphExecDestroy(gexec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    hipchk(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    enqueue_step();
+    hipchk(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+    hipchk(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+
+  // ---------------------------------------------------------------- running
+  void pull_state() {
+    if (!hsys.empty())
+      hipchk(hipMemcpyAsync(hst, dst.p, hsys.size() * sizeof(BState), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+  }
+
+  long pending_steps() {
+    long m = 0;
+    for (size_t s = 0; s < hsys.size(); ++s)
+      if (meta[s].live)
+        m = std::max(m, hst[s].left + (hst[s].init ? 1 : 0));
+    return m;
+  }
+
+  // advances every live system by up to `steps` steps (the first step of a new
+  // system is its initial force evaluation)
+  void run(long steps) {
+    if (dirty)
+      plan();
+    if (hsys.empty())
+      return;
+    pull_state();
+    long todo = std::min(steps, pending_steps());
+    for (long k = 0; k < todo; ++k)
+      hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+    steps_run += todo;
+    pull_state();
+    std::string bad;
+    for (size_t s = 0; s < hsys.size(); ++s)
+      if (meta[s].live && hst[s].overflow)
+        bad += " " + std::to_string(s);
+    if (!bad.empty())
+      fail(LPMDX_ECAPACITY, "neighbour list capacity exceeded in system(s):" + bad);
+  }
+
+  // ---------------------------------------------------------------- results
+  void get(long sid, double* xo, double* vo, double* ao, long* info) {
+    check_sid(sid);
+    pull_state();
+    const long n = meta[sid].n, a0 = hsys[sid].a0;
+    const BState& sst = hst[sid];
+    std::vector<int> pm(n);
+    std::vector<double> tx(3 * n), tv(3 * n), ta(3 * n);
+    const double* xs = (sst.upar ? x1.p : x0.p) + 3 * a0;
+    hipchk(hipMemcpyAsync(pm.data(), perm.p + a0, n * sizeof(int), hipMemcpyDeviceToHost, st), "memcpy");
+    hipchk(hipMemcpyAsync(tx.data(), xs, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    hipchk(hipMemcpyAsync(tv.data(), v.p + 3 * a0, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    hipchk(hipMemcpyAsync(ta.data(), a.p + 3 * a0, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+    for (long i = 0; i < n; ++i)
+      for (int q = 0; q < 3; ++q) {
+        if (xo)
+          xo[3 * pm[i] + q] = tx[3 * i + q];
+        if (vo)
+          vo[3 * pm[i] + q] = tv[3 * i + q];
+        if (ao)
+          ao[3 * pm[i] + q] = ta[3 * i + q];
+      }
+    if (info) {
+      info[0] = sst.done;
+      info[1] = sst.left;
+      info[2] = sst.builds;
+      info[3] = sst.init == 1 ? 0 : 1; // 2: the first force evaluation has run
+      info[4] = (long)sst.maxcnt;
+      info[5] = hsys[sid].cap;
+      info[6] = hsys[sid].lpa;
+      info[7] = sst.overflow;
+    }
+  }
+
+  template <int KIND> void eval_k(long sid, int lpa, long nb, double* pa, int rho_pass) {
+    BArgs B = args(nullptr, 0);
+    switch (lpa) {
+    case 1: k_b_eval<KIND, 1><<<nb, kBlock, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
+    case 2: k_b_eval<KIND, 2><<<nb, kBlock, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
+    default: k_b_eval<KIND, 4><<<nb, kBlock, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
+    }
+  }
+
+  // potential energy, virial, kinetic energy of one system at its last step
+  void energies(long sid, double out[3]) {
+    check_sid(sid);
+    if (dirty)
+      plan();
+    pull_state();
+    if (hst[sid].init == 1)
+      fail(LPMDX_ESTATE, "the system has not run its first force evaluation yet");
+    const BSys& S = hsys[sid];
+    const long n = meta[sid].n;
+    const int lpa = S.lpa;
+    const long nb = blocks_for((long)S.ntile * kTile * lpa);
+    evala.alloc(3 * std::max<long>(n, 1));
+    partials.alloc(nb * 12 * 2);
+    double* p1 = partials.p;
+    double* p2 = partials.p + nb * 12;
+    auto launch = [&](double* pa, int rho_pass) {
+      switch (S.kind) {
+      case kLJ: eval_k<kLJ>(sid, lpa, nb, pa, rho_pass); break;
+      case kFastLJ: eval_k<kFastLJ>(sid, lpa, nb, pa, rho_pass); break;
+      case kMorse: eval_k<kMorse>(sid, lpa, nb, pa, rho_pass); break;
+      case kBuckingham: eval_k<kBuckingham>(sid, lpa, nb, pa, rho_pass); break;
+      case kSuttonChen: eval_k<kSuttonChen>(sid, lpa, nb, pa, rho_pass); break;
+      case kFinnisSinclair: eval_k<kFinnisSinclair>(sid, lpa, nb, pa, rho_pass); break;
+      case kGupta: eval_k<kGupta>(sid, lpa, nb, pa, rho_pass); break;
+      }
+    };
+    double e_emb = 0.0;
+    if (S.metal) {
+      launch(p2, 1);
+      k_reduce_partials<<<1, kBlock, 0, st>>>(p2, nb, 12, red.p);
+      hipchk(hipMemcpyAsync(hred, red.p, 12 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+      sync();
+      e_emb = hred[1];
+    }
+    launch(p1, 0);
+    k_reduce_partials<<<1, kBlock, 0, st>>>(p1, nb, 12, red.p);
+    // kinetic energy of the system's slice
+    k_kinetic<<<blocks_for(n), kBlock, 0, st>>>(v.p + 3 * S.a0, mass.p + S.a0, n, p2);
+    k_reduce_partials<<<1, kBlock, 0, st>>>(p2, blocks_for(n), 4, red.p + 12);
+    hipchk(hipGetLastError(), "kernel launch");
+    hipchk(hipMemcpyAsync(hred, red.p, 16 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+    out[0] = hred[0] + e_emb;
+    out[1] = hred[2];
+    out[2] = hred[12] * kKin2eV;
+  }
+
+  // ---------------------------------------------------------------- autotuner
+  // Lanes per atom of each potential kind's force launch, chosen by timing a
+  // few steps of this batch per candidate on a snapshot of its state; the
+  // snapshot is restored afterwards, so tuning leaves the trajectories alone.
+  void autotune(long trial_steps, int out_lpa[8]) {
+    if (dirty)
+      plan();
+    pull_state();
+    const size_t A = acap;
+    DBuf<double> bx0, bx1, bv, ba, bffm, bmass;
+    DBuf<int> bz, bperm;
+    DBuf<BState> bst;
+    for (DBuf<double>* b : {&bx0, &bx1, &bv, &ba})
+      b->alloc(3 * A);
+    for (DBuf<double>* b : {&bffm, &bmass})
+      b->alloc(A);
+    bz.alloc(A);
+    bperm.alloc(A);
+    bst.alloc(std::max<size_t>(hsys.size(), 1));
+    auto copy = [&](bool save) {
+      auto cp = [&](void* d, void* s, size_t bytes) {
+        hipchk(hipMemcpyAsync(save ? d : s, save ? s : d, bytes, hipMemcpyDeviceToDevice, st), "memcpy");
+      };
+      cp(bx0.p, x0.p, 3 * A * sizeof(double));
+      cp(bx1.p, x1.p, 3 * A * sizeof(double));
+      cp(bv.p, v.p, 3 * A * sizeof(double));
+      cp(ba.p, a.p, 3 * A * sizeof(double));
+      cp(bffm.p, ffm.p, A * sizeof(double));
+      cp(bmass.p, mass.p, A * sizeof(double));
+      cp(bz.p, z.p, A * sizeof(int));
+      cp(bperm.p, perm.p, A * sizeof(int));
+      cp(bst.p, dst.p, hsys.size() * sizeof(BState));
+      sync();
+    };
+    copy(true);
+    std::vector<BState> saved(hst, hst + hsys.size());
+    // every trial starts from the saved state with its lists invalidated,
+    // so each candidate pays for one build, as a steady run would
+    auto reset = [&]() {
+      copy(false);
+      for (size_t s = 0; s < hsys.size(); ++s) {
+        hst[s] = saved[s];
+        hst[s].built = 0;
+        hst[s].left = std::max<long>(hst[s].left, trial_steps + 1);
+      }
+      hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+      sync();
+    };
+    std::vector<int> kinds;
+    for (auto& G : groups)
+      kinds.push_back(G->kind);
+    for (int kind : kinds) {
+      double best = 1e300;
+      int best_lpa = lpa_of_kind[kind];
+      for (int cand : {1, 2, 4}) {
+        lpa_of_kind[kind] = cand;
+        plan();
+        reset();
+        // warm up (includes the build), then time
+        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+        sync();
+        auto t0 = std::chrono::steady_clock::now();
+        for (long k = 0; k < trial_steps; ++k)
+          hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+        sync();
+        double dt_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt_s < best) {
+          best = dt_s;
+          best_lpa = cand;
+        }
+      }
+      lpa_of_kind[kind] = best_lpa;
+    }
+    plan();
+    copy(false);
+    for (size_t s = 0; s < hsys.size(); ++s) {
+      hst[s] = saved[s];
+      hst[s].built = 0; // lists are rebuilt for the chosen layout
+    }
+    hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+    sync();
+    if (out_lpa)
+      for (int q = 0; q < 8; ++q)
+        out_lpa[q] = lpa_of_kind[q];
+  }
+
+  static void need(const void* p, const char* what) {
+    if (!p)
+      fail(LPMDX_EINVAL, std::string(what) + " is NULL");
+  }
+  static void finite(const double* p, long m, const char* what) {
+    for (long q = 0; q < m; ++q)
+      if (!std::isfinite(p[q]))
+        fail(LPMDX_ENONFINITE, std::string(what) + " holds a non-finite value");
+  }
+};
+
+extern "C" {
+
+int lpmdx_batch_create(lpmdx_batch** out, long atom_capacity, long list_words, long cell_slots) {
+  return guard([&] {
+    if (!out)
+      fail(LPMDX_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (atom_capacity <= 0 || list_words <= 0 || cell_slots <= 0)
+      fail(LPMDX_EINVAL, "arena capacities must be positive");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0)
+      fail(LPMDX_EDEVICE, "no HIP device available");
+    auto B = std::make_unique<lpmdx_batch>();
+    hipchk(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking), "hipStreamCreate");
+    B->create(atom_capacity, list_words, cell_slots);
+    *out = B.release();
+  });
+}
+
+int lpmdx_batch_destroy(lpmdx_batch* b) {
+  return guard([&] { delete b; });
+}
+
+int lpmdx_batch_add(lpmdx_batch* b, long n, const double box[3], const int* z, const double* mass,
+                    const double* x, const double* v, const char* cm_args, int npot,
+                    const char** names, const char** args, const int* s1, const int* s2, double dt,
+                    long steps, long* sid) {
+  return guard([&] {
+    if (!b || !sid)
+      fail(LPMDX_EINVAL, "batch or sid is NULL");
+    if (!names)
+      fail(LPMDX_EINVAL, "names is NULL");
+    *sid = b->add(n, box, z, mass, x, v, cm_args, npot, names, args, s1, s2, dt, steps);
+  });
+}
+
+int lpmdx_batch_remove(lpmdx_batch* b, long sid) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    b->remove(sid);
+  });
+}
+
+int lpmdx_batch_run(lpmdx_batch* b, long steps) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    if (steps < 0)
+      fail(LPMDX_EINVAL, "negative step count");
+    b->run(steps);
+  });
+}
+
+int lpmdx_batch_get(lpmdx_batch* b, long sid, double* x, double* v, double* a, long info[8]) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    b->get(sid, x, v, a, info);
+  });
+}
+
+int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]) {
+  return guard([&] {
+    if (!b || !out)
+      fail(LPMDX_EINVAL, "batch or out is NULL");
+    b->energies(sid, out);
+  });
+}
+
+int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[8]) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    if (trial_steps <= 0)
+      fail(LPMDX_EINVAL, "trial steps must be positive");
+    b->autotune(trial_steps, lanes_out);
+  });
+}
+
+int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    if (kind < 1 || kind > 7 || !(lanes == 1 || lanes == 2 || lanes == 4))
+      fail(LPMDX_EINVAL, "kind 1..7 and lanes 1, 2 or 4");
+    b->lpa_of_kind[kind] = lanes;
+    b->pull_state();
+    for (size_t s = 0; s < b->hsys.size(); ++s)
+      if (b->meta[s].live && b->hsys[s].kind == kind)
+        b->hst[s].built = 0;
+    hipchk(hipMemcpyAsync(b->dst.p, b->hst, b->hsys.size() * sizeof(BState), hipMemcpyHostToDevice, b->st),
+           "memcpy");
+    b->sync();
+    b->dirty = true;
+  });
+}
+
+int lpmdx_batch_info(lpmdx_batch* b, long out[8]) {
+  return guard([&] {
+    if (!b || !out)
+      fail(LPMDX_EINVAL, "batch or out is NULL");
+    long live = 0, atoms_used = 0;
+    for (auto& m : b->meta)
+      if (m.live) {
+        ++live;
+        atoms_used += m.n;
+      }
+    out[0] = live;
+    out[1] = atoms_used;
+    out[2] = b->acap;
+    out[3] = b->atoms.largest() * kTile;
+    out[4] = b->lists.largest() * kTile;
+    out[5] = b->cells.largest();
+    out[6] = b->steps_run;
+    out[7] = (long)b->hsys.size();
+  });
+}
+
+long lpmdx_batch_bytes_per_atom(void) { return kBytesPerAtom; }
+
+} // extern "C"

@@ -76,7 +76,7 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const Box& b, const 
 
 // keys[perm[i]] = cell of atom i; vals[o] = o.  Presenting atoms to the
 // stable sort in caller order keeps ascending caller index inside each cell.
-__global__ __launch_bounds__(kBlock) void k_cell_keys(const double* __restrict__ x,
+static __global__ __launch_bounds__(kBlock) void k_cell_keys(const double* __restrict__ x,
                                                       const int* __restrict__ perm, long n, Grid g,
                                                       Box b, uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ vals) {
@@ -88,14 +88,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(const double* __restrict__
   vals[i] = (uint32_t)i;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill_int(int* __restrict__ p, long m, int v) {
+static __global__ __launch_bounds__(kBlock) void k_fill_int(int* __restrict__ p, long m, int v) {
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i < m)
     p[i] = v;
 }
 
 // [cstart[c], cend[c]) = the sorted positions of cell c (c == ncell: outside).
-__global__ __launch_bounds__(kBlock) void k_cell_ranges(const uint32_t* __restrict__ skeys, long n,
+static __global__ __launch_bounds__(kBlock) void k_cell_ranges(const uint32_t* __restrict__ skeys, long n,
                                                         int* __restrict__ cstart,
                                                         int* __restrict__ cend) {
   long k = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -130,7 +130,7 @@ struct PermuteArgs {
   long n;
 };
 
-__global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
+static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   long k = (long)blockIdx.x * kBlock + threadIdx.x;
   if (k >= A.n)
     return;
@@ -192,7 +192,7 @@ enum BuildPath {
 // FP32 copy of the positions folded into [0, L) (in FP64 first), the frame the
 // periodic binning uses: an atom at x == L (FittedInside keeps it) or outside
 // the box is binned by floor(n x / L) mod n and compared at its folded place.
-__global__ __launch_bounds__(kBlock) void k_to_f32(const double* __restrict__ x, long n, Box b,
+static __global__ __launch_bounds__(kBlock) void k_to_f32(const double* __restrict__ x, long n, Box b,
                                                    float4* __restrict__ xf) {
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
@@ -280,14 +280,15 @@ __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, con
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
-  long i = xcd_block() * kBlock + threadIdx.x;
+// List of atom i (every lane of a wavefront must call it: it ends with
+// wavefront reductions into max_count / sum_count).
+__device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool active) {
   int c = 0;
   Chunk b;
 #pragma unroll
   for (int u = 0; u < kChunk; ++u)
     b.v[u] = 0;
-  if (i < A.n) {
+  if (active) {
     const Grid& g = A.g;
     uint32_t key = A.skeys[i];
     if (key < (uint32_t)g.ncell) {
@@ -356,6 +357,11 @@ __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
     atomicMax(A.max_count, m);
     atomicAdd(A.sum_count, t);
   }
+}
+
+static __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
+  const long i = xcd_block() * kBlock + threadIdx.x;
+  build_atom(A, i, i < A.n);
 }
 
 } // namespace lpmdx

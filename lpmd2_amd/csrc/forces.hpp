@@ -181,6 +181,8 @@ struct ForceArgs {
   double* partials;     // [nblocks * 12] when evaluating energies
   unsigned int* maxd2;  // per-step max displacement^2 (float bits)
   DevPot P;
+  const DevPot* terms; // several potentials of one kind on one list (species pairs), or null
+  int nterm;
   Box b;
   double ll[3]; // |cell vector q| as FittedInside reads it
   double dt;
@@ -303,7 +305,9 @@ constexpr int kUnroll = 4;
 template <int LPA, typename Body>
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, long i, int l, double xi,
                                                double yi, double zi, Body&& body) {
-  const int c = A.cnt[i];
+  // a list that outgrew its capacity (reported as an error) is read no further
+  // than the capacity, so no entry outside the atom's chunks is ever used
+  const int c = min(A.cnt[i], A.cap);
   constexpr int T = kWave / LPA;
   // entry k of this atom: chunk k/8 of the tile row, word k%8 (see list_index)
   const int* __restrict__ nb = A.nbr + (i / T) * (long)T * A.cap + (i % T) * kChunk;
@@ -355,28 +359,47 @@ struct ForceSum {
   }
 };
 
+// Sum of the pair terms of one kind that apply to a pair (CombinedPotential
+// adds every potential whose species rule matches, combinedpotential.h:48-49).
+template <int KIND>
+__device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int si, int sj, double r2,
+                                           float& fs, float& e) {
+  if (A.nterm <= 1) {
+    const bool in = valid && r2 < A.P.rc2 && (A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
+    float f, en;
+    pair_eval<KIND>(A.P, r2, in ? (float)r2 : 1.0f, f, en);
+    fs = in ? f : 0.0f;
+    e = in ? en : 0.0f;
+  } else {
+    fs = 0.0f;
+    e = 0.0f;
+    for (int q = 0; q < A.nterm; ++q) {
+      const DevPot& P = A.terms[q];
+      const bool in = valid && r2 < P.rc2 && (P.spc_sum < 0 || si + sj == P.spc_sum);
+      float f, en;
+      pair_eval<KIND>(P, r2, in ? (float)r2 : 1.0f, f, en);
+      fs += in ? f : 0.0f;
+      e += in ? en : 0.0f;
+    }
+  }
+}
+
+// Per-atom bodies of the force kernels; lane l of the LPA lanes serving atom i.
+// Every lane of a wavefront must call them (they end with wavefront reductions).
 template <int KIND, int LPA, bool EVAL>
-__global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
-  const long t = xcd_block() * kBlock + threadIdx.x;
-  const long i = t / LPA;
-  const int l = (int)(t % LPA);
-  const bool active = i < A.n;
+__device__ __forceinline__ void pair_atom(const ForceArgs& A, long i, int l, bool active) {
   ForceSum F;
   Tally tal;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const int si = A.z[i];
-    const bool all = A.P.spc_sum < 0;
-    const double rc2 = A.P.rc2;
+    const bool need_z = A.nterm > 1 || A.P.spc_sum >= 0;
     for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int j, double dx, double dy, double dz, double r2) {
-      const bool in = valid && r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum);
-      const float fr2 = in ? (float)r2 : 1.0f;
       float fs, e;
-      pair_eval<KIND>(A.P, r2, fr2, fs, e);
-      fs = in ? fs : 0.0f;
+      pair_terms<KIND>(A, valid, si, need_z ? A.z[j] : 0, r2, fs, e);
       F.add(fs, dx, dy, dz);
       if constexpr (EVAL)
-        tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, in ? e : 0.0f);
+        tal.add(fs, (float)dx, (float)dy, (float)dz, (float)r2, e);
     });
   }
   const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
@@ -395,11 +418,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
 // w_i = rho_i^{-1/2} (FP32) and rho_i, and, when evaluating, the embedding
 // energy F(rho_i) = -K sqrt(rho_i) into the block partials.
 template <int KIND, int LPA, bool EVAL>
-__global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
-  const long t = xcd_block() * kBlock + threadIdx.x;
-  const long i = t / LPA;
-  const int l = (int)(t % LPA);
-  const bool active = i < A.n;
+__device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, long i, int l, bool active,
+                                             float* w_out, double* rho_out) {
   double rho = 0.0;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
@@ -427,11 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, d
 // EAM pass 2: pair + many-body forces over the full list with AppliesTo
 // (metalpotential.cc:140-169), fused with the integrator tail.
 template <int KIND, int LPA, bool EVAL>
-__global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
-  const long t = xcd_block() * kBlock + threadIdx.x;
-  const long i = t / LPA;
-  const int l = (int)(t % LPA);
-  const bool active = i < A.n;
+__device__ __forceinline__ void eam_force_atom(const ForceArgs& A, long i, int l, bool active) {
   ForceSum F;
   Tally tal;
   if (active) {
@@ -462,9 +478,27 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
     write_partials(A, tal, 0.0);
 }
 
+template <int KIND, int LPA, bool EVAL>
+static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
+  const long t = xcd_block() * kBlock + threadIdx.x;
+  pair_atom<KIND, LPA, EVAL>(A, t / LPA, (int)(t % LPA), t / LPA < A.n);
+}
+
+template <int KIND, int LPA, bool EVAL>
+static __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
+  const long t = xcd_block() * kBlock + threadIdx.x;
+  eam_rho_atom<KIND, LPA, EVAL>(A, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out, rho_out);
+}
+
+template <int KIND, int LPA, bool EVAL>
+static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
+  const long t = xcd_block() * kBlock + threadIdx.x;
+  eam_force_atom<KIND, LPA, EVAL>(A, t / LPA, (int)(t % LPA), t / LPA < A.n);
+}
+
 // Velocity Verlet with precomputed accelerations (several potentials):
 // a_new holds the summed accelerations.
-__global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const double* __restrict__ anew) {
+static __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const double* __restrict__ anew) {
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   float d2 = 0.0f;
   if (i < A.n)
@@ -473,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const double*
 }
 
 // Drift only: xn = FittedInside((x + v dt) + a/2 dt^2) (velocityverlet.cc:60-67).
-__global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
+static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= A.n)
     return;
@@ -485,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
 }
 
 // Kinetic energy and momentum partials (properties.h:22-81).
-__global__ __launch_bounds__(kBlock) void k_kinetic(const double* __restrict__ v,
+static __global__ __launch_bounds__(kBlock) void k_kinetic(const double* __restrict__ v,
                                                     const double* __restrict__ mass, long n,
                                                     double* __restrict__ partials) {
   __shared__ double sh[4 * (kBlock / kWave)];
@@ -506,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_kinetic(const double* __restrict__ v
 }
 
 // Fixed-order sum of nblk rows of `width` partials: out[q] = sum_b p[b*width+q].
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ p, long nblk,
+static __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ p, long nblk,
                                                             int width, double* __restrict__ out) {
   __shared__ double sh[kBlock];
   for (int q = 0; q < width; ++q) {

@@ -32,7 +32,7 @@ __device__ __forceinline__ void digit_counts(const uint32_t* keys, long n, long 
 }
 
 // hist[b * ntiles + tile] = number of keys with digit b in the tile.
-__global__ __launch_bounds__(kBlock) void k_sort_count(const uint32_t* __restrict__ keys, long n,
+static __global__ __launch_bounds__(kBlock) void k_sort_count(const uint32_t* __restrict__ keys, long n,
                                                        int shift, uint32_t* __restrict__ hist,
                                                        int ntiles) {
   __shared__ uint32_t sh[kDigits][kBlock / kWave];
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_count(const uint32_t* __restric
 }
 
 // In-place exclusive scan of m uint32 values by one 1024-thread block.
-__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ data, long m) {
+static __global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ data, long m) {
   __shared__ uint32_t sh[1024];
   const long per = (m + 1023) / 1024;
   const long lo = threadIdx.x * per;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ data,
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* __restrict__ keys,
+static __global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ vals, long n,
                                                          int shift,
                                                          const uint32_t* __restrict__ offs,

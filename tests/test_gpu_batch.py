@@ -1,0 +1,136 @@
+"""GPU checks of the batched engine through the C ABI: forces against the FP64
+oracle, results identical alone and in a mix, agreement with the plain
+single-system path, and an autotuner that leaves trajectories alone."""
+import sys
+
+import numpy as np
+import pytest
+
+import lpmd2_amd as E
+from lpmd2_amd.batch import Batch
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LJ = ("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+SC = ("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22")
+GUPTA = ("gupta", "A 0.0855 r0 2.556 p 10.960 B 1.2240 q 2.278 cutoff 7.22")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def system(kind, nx, seed, T):
+    if kind == "lj":
+        x, L = E.fcc(nx, 5.26)
+        z, pot, cm, dt = 18, LJ, "cutoff 8.5 skin 1.0", 2.0
+    elif kind == "sc":
+        x, L = E.fcc(nx, 3.61)
+        z, pot, cm, dt = 29, SC, "cutoff 7.22 skin 0.3", 2.0
+    else:
+        x, L = E.fcc(nx, 3.61)
+        z, pot, cm, dt = 29, GUPTA, "cutoff 7.22 skin 0.3", 2.0
+    rng = np.random.default_rng(seed)
+    x = np.mod(x + rng.normal(0, 0.02, x.shape), L)
+    m = np.full(x.shape[0], E.ELEMENT_MASS[z])
+    v = E.maxwell_boltzmann(m, T, seed=seed + 100)
+    return dict(x=x, box=L, z=z, potentials=[pot], cm=cm, v=v, dt=dt)
+
+
+def test_batch_forces_match_oracle():
+    say("batch forces")
+    specs = [system("lj", 4, 1, 100.0), system("sc", 5, 2, 300.0), system("gupta", 4, 3, 500.0)]
+    b = Batch(atom_capacity=4096)
+    sids = [b.add(steps=0, **s) for s in specs]
+    b.run(1)  # the initial force evaluation
+    for sid, s in zip(sids, specs):
+        x, v, a, info = b.get(sid)
+        assert info["started"] == 1 and info["overflow"] == 0
+        name, args = s["potentials"][0]
+        kw = dict(zip(args.split()[0::2], (float(q) for q in args.split()[1::2])))
+        pot = O.Pot(name, **kw)
+        conf = O.Conf(s["x"], s["box"], z=s["z"])
+        cm = O.CellManager("verletlist", pot.cutoff, maxneighbors=400, each=5)
+        e0, w0, _ = O.update_forces(conf, cm, [pot])
+        err = np.linalg.norm(a - conf.a) / np.linalg.norm(conf.a)
+        ep, vir, ek = b.energies(sid)
+        say(f"  {name}: rel force rms {err:.2e}, E {ep:.9g} vs {e0:.9g}")
+        assert err <= 1e-5
+        assert abs(ep - e0) <= 1e-6 * abs(e0)
+        np.testing.assert_array_equal(x, s["x"])
+
+
+def test_alone_and_mixed_are_identical():
+    say("alone vs mixed")
+    a_spec = system("sc", 4, 7, 600.0)
+    others = [system("lj", 3, 8, 200.0), system("gupta", 5, 9, 900.0), system("lj", 5, 10, 120.0)]
+    b1 = Batch(atom_capacity=2048)
+    s1 = b1.add(steps=60, **a_spec)
+    b1.run(100)
+    x1, v1, a1, i1 = b1.get(s1)
+    b2 = Batch(atom_capacity=8192)
+    b2.add(steps=80, **others[0])
+    s2 = b2.add(steps=60, **a_spec)
+    b2.add(steps=30, **others[1])
+    b2.add(steps=90, **others[2])
+    b2.run(100)
+    x2, v2, a2, i2 = b2.get(s2)
+    say(f"  builds alone {i1['builds']} mixed {i2['builds']}, done {i1['done']} {i2['done']}")
+    assert i1["done"] == i2["done"] == 60
+    np.testing.assert_array_equal(x1, x2)
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(a1, a2)
+
+
+def test_batch_matches_plain_path():
+    say("batch vs plain")
+    s = system("lj", 4, 11, 100.0)
+    b = Batch(atom_capacity=1024)
+    sid = b.add(steps=40, **s)
+    b.run(50)
+    xb, vb, ab, info = b.get(sid)
+    p = E.System(s["x"], s["box"], s["z"], v=s["v"])
+    p.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 400 each 5 skin 1.0")
+    p.append_potential(*s["potentials"][0])
+    p.velocityverlet(s["dt"], 40)
+    xp, vp, ap = p.get_state()
+    dx = xb - xp
+    dx -= s["box"] * np.round(dx / s["box"])
+    say(f"  max |dx| {np.abs(dx).max():.2e}, builds {info['builds']}")
+    assert np.abs(dx).max() < 1e-9
+    np.testing.assert_allclose(vb, vp, rtol=0, atol=1e-11)
+
+
+def test_autotune_keeps_state():
+    say("autotune")
+    specs = [system("lj", 4, 21, 100.0), system("sc", 4, 22, 300.0)]
+    b = Batch(atom_capacity=4096)
+    sids = [b.add(steps=30, **s) for s in specs]
+    b.run(5)
+    before = [b.get(s)[:2] for s in sids]
+    lanes = b.autotune(trial_steps=5)
+    after = [b.get(s)[:2] for s in sids]
+    for (x0, v0), (x1, v1) in zip(before, after):
+        np.testing.assert_array_equal(x0, x1)
+        np.testing.assert_array_equal(v0, v1)
+    say(f"  lanes {lanes}")
+    assert lanes["lennardjones"] in (1, 2, 4) and lanes["suttonchen"] in (1, 2, 4)
+    b.run(40)
+    for s in sids:
+        assert b.get(s)[3]["done"] == 30
+
+
+def test_arena_admission_and_reuse():
+    say("arena")
+    b = Batch(atom_capacity=600, list_words=600 * 160)
+    s1 = b.add(steps=5, **system("lj", 3, 31, 100.0))   # 108 atoms
+    s2 = b.add(steps=5, **system("lj", 4, 32, 100.0))   # 256 atoms
+    with pytest.raises(E.LpmdxError) as e:
+        b.add(steps=5, **system("lj", 5, 33, 100.0))    # 500 atoms: no room
+    assert e.value.code == E.ENOMEM
+    b.remove(s2)
+    s3 = b.add(steps=5, **system("lj", 4, 34, 100.0))
+    b.run(10)
+    assert b.get(s1)[3]["done"] == 5 and b.get(s3)[3]["done"] == 5
