@@ -11,6 +11,17 @@ Prints one JSON line (value = atom-steps/s) with the dominant kernel's
 roofline (HIP events on the engine's stream) and a CPU baseline (the FP64
 oracle, a restatement of the reference path, on one host core, timed on a
 bounded sample and extrapolated to one reference step).
+
+The same line carries the two other legs of the metric:
+  "batched": 512 periodic systems, sizes log-uniform in [256, 16384] atoms,
+      half LJ argon, half Sutton-Chen Cu/Ni/Ag, 1000 steps each in one batch
+      (graph-replayed steps, autotuned lanes), beside the plain path running
+      the same systems one at a time (timed over --plain-steps steps each);
+  "served": Poisson arrivals of 500-65536-atom Gupta / Finnis-Sinclair /
+      Buckingham rocksalt jobs of 100-5000 steps at a few rates, p50/p99
+      completion latency and atom-steps/s, beside the same jobs run one at a
+      time on the plain path (service times measured, FIFO queue replayed on
+      the same arrival times).
 """
 from __future__ import annotations
 
@@ -122,6 +133,144 @@ def cpu_model():
     return "unknown"
 
 
+def plain_run(job, steps=None, warmup=0):
+    """One job on the plain single-system path: (seconds to create and upload,
+    seconds of `steps` velocity-Verlet steps after `warmup` steps)."""
+    import lpmd2_amd as E
+
+    t0 = time.perf_counter()
+    s = E.System(job["x"], job["box"], job["z"], v=job["v"])
+    s.set_cellmanager("verletlist", job["cm"] + " maxneighbors 4096")
+    for p in job["potentials"]:
+        s.append_potential(*p)
+    t_create = time.perf_counter() - t0
+    if warmup:
+        s.velocityverlet(job["dt"], warmup)
+    t0 = time.perf_counter()
+    s.velocityverlet(job["dt"], job["steps"] if steps is None else steps)
+    t_run = time.perf_counter() - t0
+    return t_create, t_run
+
+
+def batched_leg(args):
+    """512 systems in one batch against the same systems one at a time."""
+    from lpmd2_amd import workloads as W
+    from lpmd2_amd.batch import Batch
+
+    steps = args.batch_steps
+    warm = 10
+    ens = W.batched_ensemble(seed=3, n_systems=args.batch_systems, steps=steps + warm)
+    atoms = sum(j["x"].shape[0] for j in ens)
+    padded = sum(-(-j["x"].shape[0] // 64) * 64 for j in ens)
+    log(f"batched: {len(ens)} systems, {atoms} atoms")
+    b = Batch(atom_capacity=padded + 64, list_words=padded * 320, cell_slots=max(4096, padded // 4))
+    t0 = time.perf_counter()
+    sids = [b.add(x=j["x"], box=j["box"], z=j["z"], potentials=j["potentials"], cm=j["cm"], v=j["v"], dt=j["dt"],
+                  steps=j["steps"]) for j in ens]
+    log(f"  added in {time.perf_counter() - t0:.2f} s")
+    t0 = time.perf_counter()
+    lanes = b.autotune(trial_steps=5)
+    log(f"  autotune {time.perf_counter() - t0:.2f} s: {lanes}")
+    b.run(warm + 1)  # initial force evaluation + warmup steps
+    t0 = time.perf_counter()
+    b.run(steps)
+    t_b = time.perf_counter() - t0
+    info = b.info()
+    done = [b.get(sid)[3] for sid in sids[:4]]
+    assert all(d["left"] == 0 and d["done"] == steps + warm for d in done), done
+    batched = atoms * steps / t_b
+    log(f"  batched {steps} steps: {t_b:.3f} s, {batched:.4e} atom-steps/s, {info}")
+    b.close()
+    # the plain path, one system at a time
+    t_plain = 0.0
+    t0 = time.perf_counter()
+    for j in ens:
+        _, t_run = plain_run(j, steps=args.plain_steps, warmup=2)
+        t_plain += t_run
+    plain = atoms * args.plain_steps / t_plain
+    log(f"  plain one at a time ({args.plain_steps} steps each): {t_plain:.3f} s of steps "
+        f"({time.perf_counter() - t0:.1f} s wall), {plain:.4e} atom-steps/s")
+    return {
+        "value": batched,
+        "unit": "atom-steps/s",
+        "systems": len(ens),
+        "atoms": atoms,
+        "steps": steps,
+        "ms_per_step": 1e3 * t_b / steps,
+        "lanes": {k: v for k, v in lanes.items() if k in ("lennardjones", "suttonchen")},
+        "plain_value": plain,
+        "plain_steps_per_system": args.plain_steps,
+        "speedup_vs_plain": batched / plain,
+        "workload": "512 systems, N log-uniform [256, 16384], half LJ Ar rho* 0.8-1.0 (skin 1.0), "
+                    "half Sutton-Chen Cu/Ni/Ag rc 2a (skin 0.3), 100-900 K, seed 3",
+    }
+
+
+def served_leg(args):
+    """Poisson arrivals into the server at a few rates, and the same jobs run
+    one at a time on the plain path."""
+    from lpmd2_amd import workloads as W
+    from lpmd2_amd.server import Server
+
+    out = {"rates": [], "workload": "Poisson arrivals, N log-uniform [500, 65536], Gupta/Finnis-Sinclair Cu/Al/Ni "
+                                    "fcc or Buckingham rocksalt NaCl, 300-1200 K, 100-5000 steps, seed 5"}
+    plain_service = None
+    for rate in args.serve_rates:
+        jobs = W.served_jobs(seed=5, n_jobs=args.serve_jobs, rate_hz=rate)
+        work = sum(j["x"].shape[0] * j["steps"] for _, j in jobs)
+        with Server(args.serve_budget << 30, max_round_steps=256) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for t_arr, j in jobs:
+                d = t_arr - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                ids.append(srv.submit_job(j))
+            assert srv.wait(timeout=600.0)
+            st = srv.stats()
+            lat = []
+            failed = 0
+            for i in ids:
+                try:
+                    lat.append(srv.result(i)[3]["latency_s"])
+                except Exception as e:  # a failed job is reported, not hidden
+                    failed += 1
+                    log(f"  job {i} failed: {e}")
+        rec = {
+            "rate_hz": rate,
+            "jobs": len(jobs),
+            "failed": failed,
+            "p50_s": st["p50_s"],
+            "p99_s": st["p99_s"],
+            "value": st["atom_steps"] / st["span_s"],
+            "span_s": st["span_s"],
+            "max_resident": int(st["max_resident"]),
+            "rounds": int(st["rounds"]),
+        }
+        log(f"served at {rate} jobs/s: {rec}")
+        if plain_service is None:
+            plain_service = []
+            for _, j in jobs:
+                tc, tr = plain_run(j)
+                plain_service.append(tc + tr)
+            plain_rate = work / sum(plain_service)
+            log(f"  plain one at a time: {sum(plain_service):.3f} s, {plain_rate:.4e} atom-steps/s")
+        # the same arrivals served first-come first-served by the plain path
+        done = 0.0
+        plat = []
+        for (t_arr, _), ts in zip(jobs, plain_service):
+            done = max(done, t_arr) + ts
+            plat.append(done - t_arr)
+        rec["plain_fifo_p50_s"] = float(np.percentile(plat, 50))
+        rec["plain_fifo_p99_s"] = float(np.percentile(plat, 99))
+        rec["speedup_vs_plain"] = rec["value"] / plain_rate
+        out["rates"].append(rec)
+    out["plain_value"] = plain_rate
+    out["value"] = max(r["value"] for r in out["rates"])
+    out["unit"] = "atom-steps/s"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +280,14 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sampling")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per atom (0 = automatic)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--plain-steps", type=int, default=100, help="steps per system of the plain comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
+    ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 1e6])
+    ap.add_argument("--serve-budget", type=int, default=16, help="server HBM budget in GiB")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: each box has one MI355X and multi-GPU runs are not part of this build",
@@ -211,6 +368,11 @@ def main():
         "step_hbm_fraction": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
         "list_builds": st1["builds"] - st0["builds"],
     }
+    s.close()
+    if not args.no_batch:
+        out["batched"] = batched_leg(args)
+    if not args.no_serve:
+        out["served"] = served_leg(args)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)

@@ -187,6 +187,53 @@ LPMDX_API int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes);
 LPMDX_API int lpmdx_batch_info(lpmdx_batch* b, long out[8]);
 /* HBM bytes one atom slot takes in the per-atom arenas (for admission budgets). */
 LPMDX_API long lpmdx_batch_bytes_per_atom(void);
+/* Per system slot s < nslots: left[s] = graph steps still to run (its steps
+ * plus the initial force evaluation while that is pending), flags[s] bit 0
+ * live, bit 1 list overflow, bit 2 initial force evaluation pending. */
+LPMDX_API int lpmdx_batch_progress(lpmdx_batch* b, long nslots, long* left, int* flags);
+
+/* ------------------------------------------------------------------------
+ * Serving loop: MD jobs (one system, its potentials, dt and step count each)
+ * are submitted at any time; a worker thread admits queued jobs into a batch
+ * whose arenas are sized from an HBM budget whenever they fit, advances all
+ * admitted jobs together in rounds of graph-replayed steps, and retires each
+ * job when its steps are done, so arrivals join work already on the device
+ * (continuous batching).  Job state stays in HBM from admission to
+ * completion.  No reference counterpart: it runs many lpmd invocations
+ * (lpmd/lpmd.cc:58-108) as requests.  Results of a job do not depend on the
+ * other jobs it ran beside.
+ * ---------------------------------------------------------------------- */
+typedef struct lpmdx_server lpmdx_server;
+
+/* hbm_budget: bytes for the atom, list and cell arenas; max_round_steps:
+ * most steps between two admission points. */
+LPMDX_API int lpmdx_server_create(lpmdx_server** out, long hbm_budget, long max_round_steps);
+/* Stops the worker (jobs still queued or running are dropped) and frees. */
+LPMDX_API int lpmdx_server_destroy(lpmdx_server* s);
+/* Queues one job (arguments as lpmdx_batch_add; inputs are copied). */
+LPMDX_API int lpmdx_server_submit(lpmdx_server* s, long n, const double box[3], const int* z,
+                                  const double* mass, const double* x, const double* v,
+                                  const char* cm_args, int npot, const char** names,
+                                  const char** args, const int* s1, const int* s2, double dt,
+                                  long steps, long* job);
+/* Waits up to timeout_s seconds (< 0: no limit) for one job (job >= 0) or for
+ * every job submitted so far (job < 0); *done = 1 when it finished. */
+LPMDX_API int lpmdx_server_wait(lpmdx_server* s, long job, double timeout_s, int* done);
+/* Final x, v, a of a finished job in its atom order (each may be NULL) and
+ * out[8] = potential energy, virial, kinetic energy, seconds queued (submit to
+ * admission), seconds to completion (submit to done), list builds, steps
+ * run, status (0 done, else the LPMDX_E* code the job failed with).  Returns
+ * LPMDX_ESTATE while the job is unfinished. */
+LPMDX_API int lpmdx_server_result(lpmdx_server* s, long job, double* x, double* v, double* a, double out[8]);
+/* Frees the stored result of a finished job. */
+LPMDX_API int lpmdx_server_release(lpmdx_server* s, long job);
+/* out[12] = jobs done, jobs failed, p50 and p99 completion latency (s), mean
+ * latency (s), atom-steps done, seconds from the first submission to the last
+ * completion, rounds, most jobs resident at once, jobs queued, jobs running,
+ * graph steps replayed. */
+LPMDX_API int lpmdx_server_stats(lpmdx_server* s, double out[12]);
+/* Lanes per atom of one potential kind's force launch (1, 2 or 4). */
+LPMDX_API int lpmdx_server_set_lanes(lpmdx_server* s, int kind, int lanes);
 
 #ifdef __cplusplus
 }

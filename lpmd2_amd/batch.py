@@ -31,6 +31,7 @@ def _bind():
     L.lpmdx_batch_set_lanes.argtypes = [vp, i, i]
     L.lpmdx_batch_info.argtypes = [vp, vp]
     L.lpmdx_batch_bytes_per_atom.restype = l
+    L.lpmdx_batch_progress.argtypes = [vp, l, vp, vp]
     _bound = True
     return L
 
@@ -108,6 +109,14 @@ class Batch:
 
     def set_lanes(self, kind: str, lanes: int):
         _check(lib().lpmdx_batch_set_lanes(self._h, KINDS[kind], lanes))
+
+    def progress(self, nslots: int):
+        """(left, flags) per slot: graph steps still to run; bit 0 live, bit 1
+        overflow, bit 2 initial force evaluation pending."""
+        left = np.zeros(nslots, dtype=np.int64)
+        flags = np.zeros(nslots, dtype=np.int32)
+        _check(lib().lpmdx_batch_progress(self._h, nslots, _p(left), _p(flags)))
+        return left, flags
 
     def info(self):
         out = np.zeros(8, dtype=np.int64)

@@ -6,7 +6,7 @@ import subprocess
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
-SOURCES = [os.path.join(_HERE, "csrc", f) for f in ("engine.hip", "batch.hip")]
+SOURCES = [os.path.join(_HERE, "csrc", f) for f in ("engine.hip", "batch.hip", "server.cpp")]
 HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp",
                                                      "batch.hpp", "host_util.hpp")] + [
     os.path.join(ROOT, "include", "lpmdx.h")
@@ -32,12 +32,26 @@ FLAGS = [
 
 
 def build_library(force: bool = True) -> str:
+    """Compiles every source to an object in parallel, then links the shared
+    library next to this file (written to a temporary name, then renamed)."""
     if not force and os.path.exists(OUT):
         newest = max(os.path.getmtime(p) for p in SOURCES + HEADERS)
         if os.path.getmtime(OUT) >= newest:
             return OUT
+    objdir = os.path.join(_HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    procs = []
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        procs.append((src, subprocess.Popen([HIPCC, *compile_flags, *lang, "-c", "-o", obj, src])))
+        objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, f"hipcc {' '.join(failed)}")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *SOURCES]
-    subprocess.check_call(cmd)
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, OUT)
     return OUT

@@ -91,6 +91,10 @@ struct lpmdx_batch {
   // tile lists
   DBuf<int> tiles_all;
   long n_all = 0;
+  // sort chunks and their digit counts
+  DBuf<int2> chunks;
+  DBuf<uint32_t> hist;
+  int nchunks = 0, passes = 1;
   struct Group {
     int kind, lpa;
     bool metal;
@@ -165,6 +169,9 @@ struct lpmdx_batch {
     B.tsys = tsys.p;
     B.ntiles = ntiles;
     B.nsys = (int)hsys.size();
+    B.chunks = chunks.p;
+    B.nchunks = nchunks;
+    B.hist = hist.p;
     return B;
   }
 
@@ -253,12 +260,24 @@ struct lpmdx_batch {
     int nq[3];
     for (int q = 0; q < 3; ++q)
       nq[q] = (int)std::max(1.0, std::min(std::floor(box[q] / (rl * (1.0 + 1e-9))), 512.0));
+    // at most kMaxSortCells cells (two digit passes of the sort): coarser
+    // cells only add candidates, the list is the same
+    while ((long)nq[0] * nq[1] * nq[2] > kMaxSortCells) {
+      const int q = (nq[0] >= nq[1] && nq[0] >= nq[2]) ? 0 : (nq[1] >= nq[2] ? 1 : 2);
+      nq[q] -= 1;
+    }
     S.g.nx = nq[0];
     S.g.ny = nq[1];
     S.g.nz = nq[2];
     S.g.side = 1;
     S.g.mode = kExact;
     S.g.ncell = nq[0] * nq[1] * nq[2];
+    {
+      int bits = 0;
+      while (bits < 31 && ((uint32_t)(S.g.ncell - 1) >> bits))
+        ++bits;
+      S.passes = std::max(1, (bits + kRadixBits - 1) / kRadixBits);
+    }
     const double lmax = std::max(box[0], std::max(box[1], box[2]));
     const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
     const double hmin = std::min(S.b.half[0], std::min(S.b.half[1], S.b.half[2]));
@@ -409,10 +428,25 @@ struct lpmdx_batch {
 
   // ---------------------------------------------------------------- launch plan
   void plan() {
-    // system table
-    for (size_t s = 0; s < hsys.size(); ++s)
-      if (meta[s].live)
-        hsys[s].lpa = lpa_of_kind[hsys[s].kind];
+    // system table and sort chunks
+    std::vector<int2> ch;
+    passes = 1;
+    for (size_t s = 0; s < hsys.size(); ++s) {
+      if (!meta[s].live)
+        continue;
+      BSys& S = hsys[s];
+      S.lpa = lpa_of_kind[S.kind];
+      S.ch0 = (int)ch.size();
+      S.nch = (S.n + kSortChunk - 1) / kSortChunk;
+      for (int j = 0; j < S.nch; ++j)
+        ch.push_back(make_int2((int)s, j));
+      passes = std::max(passes, S.passes);
+    }
+    nchunks = (int)ch.size();
+    chunks.alloc(std::max<size_t>(ch.size(), 1));
+    hist.alloc(std::max<size_t>(ch.size(), 1) * kRadix);
+    if (!ch.empty())
+      hipchk(hipMemcpyAsync(chunks.p, ch.data(), ch.size() * sizeof(int2), hipMemcpyHostToDevice, st), "memcpy");
     dsys.alloc(std::max<size_t>(hsys.size(), 1));
     if (!hsys.empty())
       hipchk(hipMemcpyAsync(dsys.p, hsys.data(), hsys.size() * sizeof(BSys), hipMemcpyHostToDevice, st), "memcpy");
@@ -478,7 +512,18 @@ struct lpmdx_batch {
     const int ns = (int)hsys.size();
     BArgs B = args(tiles_all.p, n_all);
     k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B);
-    k_b_sort<<<ns, kSortThreads, 0, st>>>(B);
+    if (nchunks) {
+      for (int p = 0; p < passes; ++p) {
+        k_b_count<<<nchunks, kBlock, 0, st>>>(B, p);
+        k_b_scan<<<ns, kScanThreads, 0, st>>>(B, p);
+        k_b_scatter<<<nchunks, kBlock, 0, st>>>(B, p);
+      }
+    }
+    if (n_all) {
+      k_b_gather<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
+      k_b_commit<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
+    }
+    k_b_sorted<<<blocks_for(ns), kBlock, 0, st>>>(B);
     if (n_all)
       k_b_build<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
     for (auto& G : groups) {
@@ -519,7 +564,7 @@ struct lpmdx_batch {
     long m = 0;
     for (size_t s = 0; s < hsys.size(); ++s)
       if (meta[s].live)
-        m = std::max(m, hst[s].left + (hst[s].init ? 1 : 0));
+        m = std::max(m, hst[s].left + (hst[s].init == 1 ? 1 : 0));
     return m;
   }
 
@@ -854,5 +899,23 @@ int lpmdx_batch_info(lpmdx_batch* b, long out[8]) {
 }
 
 long lpmdx_batch_bytes_per_atom(void) { return kBytesPerAtom; }
+
+int lpmdx_batch_progress(lpmdx_batch* b, long nslots, long* left, int* flags) {
+  return guard([&] {
+    if (!b || (nslots > 0 && (!left || !flags)))
+      fail(LPMDX_EINVAL, "batch, left or flags is NULL");
+    b->pull_state();
+    for (long s = 0; s < nslots; ++s) {
+      left[s] = 0;
+      flags[s] = 0;
+      if (s >= (long)b->hsys.size() || !b->meta[s].live)
+        continue;
+      const BState& q = b->hst[s];
+      const int pending = q.init == 1 ? 1 : 0;
+      left[s] = q.left + pending;
+      flags[s] = 1 | (q.overflow ? 2 : 0) | (pending ? 4 : 0);
+    }
+  });
+}
 
 } // extern "C"

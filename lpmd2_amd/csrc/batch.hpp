@@ -2,8 +2,12 @@
 // uneven size advanced together.  Every step is the same fixed sequence of
 // launches (captured once in a HIP graph and replayed):
 //   k_b_control  one thread per system: step bookkeeping, skin check
-//   k_b_sort     one workgroup per system that needs a new list: stable
-//                in-workgroup radix sort of its atoms into cells, reorder
+//   k_b_count / k_b_scan / k_b_scatter (one triple per 8-bit digit pass)
+//                stable radix sort of the atoms of every system that needs a
+//                new list by cell: 4096-atom chunks, one workgroup each, so a
+//                large system is sorted by many workgroups at once
+//   k_b_gather / k_b_commit / k_b_sorted
+//                cell ranges and the reorder of the per-atom state
 //   k_b_build    every tile of those systems: neighbour lists
 //   k_b_rho      EAM systems: rho pass            (one launch per kind)
 //   k_b_force    all active systems: forces + velocity Verlet (per kind)
@@ -18,13 +22,18 @@
 #include "cells.hpp"
 #include "common.hpp"
 #include "forces.hpp"
-#include "sort.hpp"
 
 namespace lpmdx {
 
 constexpr int kMaxTerms = 4;
 constexpr int kTile = kWave; // atoms per tile
-constexpr int kSortThreads = 512;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortItems8 = 16;                     // atoms per thread of a sort chunk
+constexpr int kSortChunk = kBlock * kSortItems8;    // atoms per sort chunk (one workgroup)
+constexpr int kMaxSortCells = 1 << (2 * kRadixBits); // cells per system: at most two digit passes
+constexpr int kScanThreads = 1024;
+static_assert(kBlock == kRadix, "one thread per digit in the sort kernels");
 
 // Read-only description of one system for a round.
 struct BSys {
@@ -46,6 +55,8 @@ struct BSys {
   double budget; // half the skin: rebuild once the displacement bound passes it
   double dt;
   int kind, nterm, metal;
+  int passes;   // digit passes its cell ids need (1 or 2)
+  int ch0, nch; // its first sort chunk and number of chunks
   DevPot P[kMaxTerms];
 };
 
@@ -102,6 +113,9 @@ struct BArgs {
   const int* tsys;    // system of every global tile
   long ntiles;        // entries of `tiles`
   int nsys;
+  const int2* chunks; // sort chunks: (system, chunk index within it)
+  int nchunks;
+  uint32_t* hist;     // per system: kRadix x nch digit counts, digit-major
 };
 
 // ---------------------------------------------------------------- control
@@ -153,167 +167,261 @@ static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B) {
 }
 
 // ---------------------------------------------------------------- sort
-// Stable LSD radix sort (4-bit digits) of one system's atoms by cell, inside
-// one workgroup; thread t owns the contiguous items [t*ipt, (t+1)*ipt), so
-// equal keys keep their order.  Then cell ranges, the reorder of the per-atom
-// state, and the FP32 folded positions for the list build.
-static __global__ __launch_bounds__(kSortThreads) void k_b_sort(BArgs B) {
+// Stable LSD radix sort of each rebuilding system's atoms by cell id, 8-bit
+// digits.  Pass p: k_b_count histograms digit p per chunk (the first pass
+// also computes the cell ids from the positions), k_b_scan turns one
+// system's digit-major (digit, chunk) counts into scatter offsets, and
+// k_b_scatter writes every key to offset + its rank among the equal digits
+// before it in the chunk (wave ballots give the rank inside a wavefront,
+// LDS prefixes across the wavefronts), so equal cell ids keep ascending
+// atom order.  Keys and values ping-pong between keys[0]/vals[0] and
+// keys[1]/vals[1]; a system's sorted order ends in buffer (passes & 1).
+
+__device__ __forceinline__ bool sort_chunk(const BArgs& B, int pass, int& s, int& j) {
+  const int c = blockIdx.x;
+  if (c >= B.nchunks)
+    return false;
+  const int2 cj = B.chunks[c];
+  s = cj.x;
+  j = cj.y;
+  const BState& st = B.st[s];
+  return B.sys[s].live && st.act && st.flag && pass < B.sys[s].passes;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_b_count(BArgs B, int pass) {
+  int s, j;
+  if (!sort_chunk(B, pass, s, j))
+    return; // uniform over the workgroup
+  const BSys& S = B.sys[s];
+  const BState& st = B.st[s];
+  __shared__ uint32_t h[kRadix];
+  const int t = threadIdx.x;
+  h[t] = 0u;
+  __syncthreads();
+  const long a0 = S.a0;
+  const int n = S.n;
+  const int shift = kRadixBits * pass;
+  uint32_t* kin = B.R.keys[pass & 1] + a0;
+  uint32_t* vin = B.R.vals[pass & 1] + a0;
+  const double* xc = B.R.x[st.par] + 3 * a0;
+  for (int u = 0; u < kSortItems8; ++u) {
+    const int i = j * kSortChunk + u * kBlock + t;
+    if (i < n) {
+      uint32_t key;
+      if (pass == 0) {
+        key = cell_key(S.g, S.b, xc + 3 * (long)i);
+        kin[i] = key;
+        vin[i] = (uint32_t)i;
+      } else {
+        key = kin[i];
+      }
+      atomicAdd(&h[(key >> shift) & (kRadix - 1)], 1u); // a count: order-free
+    }
+  }
+  __syncthreads();
+  B.hist[(long)S.ch0 * kRadix + (long)t * S.nch + j] = h[t];
+}
+
+static __global__ __launch_bounds__(kScanThreads) void k_b_scan(BArgs B, int pass) {
   const int s = blockIdx.x;
   if (s >= B.nsys)
     return;
   const BSys& S = B.sys[s];
-  BState& st = B.st[s];
-  if (!(st.act && st.flag))
+  const BState& st = B.st[s];
+  if (!S.live || !(st.act && st.flag) || pass >= S.passes)
     return;
-  __shared__ uint32_t table[kDigits * kSortThreads];
-  __shared__ uint32_t part[kSortThreads];
+  __shared__ uint32_t part[kScanThreads];
   const int t = threadIdx.x;
+  uint32_t* h = B.hist + (long)S.ch0 * kRadix;
+  const long m = (long)kRadix * S.nch;
+  const long per = (m + kScanThreads - 1) / kScanThreads;
+  const long lo = min(m, (long)t * per), hi = min(m, lo + per);
+  uint32_t sum = 0;
+  for (long k = lo; k < hi; ++k)
+    sum += h[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < kScanThreads; o <<= 1) {
+    const uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = t ? part[t - 1] : 0u;
+  for (long k = lo; k < hi; ++k) {
+    const uint32_t v = h[k];
+    h[k] = run;
+    run += v;
+  }
+  if (pass == 0) {
+    // cell ranges of empty cells stay [0, 0)
+    int* cs = B.R.cstart + S.c0;
+    int* ce = B.R.cend + S.c0;
+    for (int q = t; q <= S.g.ncell; q += kScanThreads) {
+      cs[q] = 0;
+      ce[q] = 0;
+    }
+  }
+}
+
+static __global__ __launch_bounds__(kBlock) void k_b_scatter(BArgs B, int pass) {
+  int s, j;
+  if (!sort_chunk(B, pass, s, j))
+    return; // uniform over the workgroup
+  const BSys& S = B.sys[s];
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ uint32_t run[kRadix];
+  __shared__ uint32_t wcnt[kWaves][kRadix];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  run[t] = B.hist[(long)S.ch0 * kRadix + (long)t * S.nch + j];
   const long a0 = S.a0;
   const int n = S.n;
-  const int ipt = (n + kSortThreads - 1) / kSortThreads;
-  const int lo = min(n, t * ipt), hi = min(n, lo + ipt);
-  const Grid& g = S.g;
-  const double* xc = B.R.x[st.par] + 3 * a0;
-  uint32_t* ka = B.R.keys[0] + a0;
-  uint32_t* va = B.R.vals[0] + a0;
-  uint32_t* kb = B.R.keys[1] + a0;
-  uint32_t* vb = B.R.vals[1] + a0;
-  for (int k = lo; k < hi; ++k) {
-    ka[k] = cell_key(g, S.b, xc + 3 * k);
-    va[k] = (uint32_t)k;
-  }
-  int bits = 0;
-  while (bits < 32 && ((uint32_t)g.ncell >> bits))
-    ++bits;
-  const int passes = max(1, (bits + 3) / 4);
-  for (int ps = 0; ps < passes; ++ps) {
-    const int shift = 4 * ps;
-    __syncthreads();
-    uint32_t c[kDigits];
+  const int shift = kRadixBits * pass;
+  const uint32_t* kin = B.R.keys[pass & 1] + a0;
+  const uint32_t* vin = B.R.vals[pass & 1] + a0;
+  uint32_t* kout = B.R.keys[(pass + 1) & 1] + a0;
+  uint32_t* vout = B.R.vals[(pass + 1) & 1] + a0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int u = 0; u < kSortItems8; ++u) {
+    const int base = j * kSortChunk + u * kBlock;
+    if (base >= n)
+      break; // uniform over the workgroup
+    const int i = base + t;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t val = valid ? vin[i] : 0u;
+    const uint32_t d = (key >> shift) & (kRadix - 1);
+    // lanes of this wavefront holding the same digit
+    unsigned long long peers = __ballot(valid);
 #pragma unroll
-    for (int d = 0; d < kDigits; ++d)
-      c[d] = 0;
-    for (int k = lo; k < hi; ++k) {
-      const uint32_t d = (ka[k] >> shift) & (kDigits - 1);
-#pragma unroll
-      for (int e = 0; e < kDigits; ++e)
-        c[e] += (d == (uint32_t)e) ? 1u : 0u;
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
     }
 #pragma unroll
-    for (int d = 0; d < kDigits; ++d)
-      table[d * kSortThreads + t] = c[d];
+    for (int q = 0; q < kWaves; ++q)
+      wcnt[q][t] = 0u;
     __syncthreads();
-    // exclusive scan of the digit-major table: thread t scans 16 entries
+    if (valid && __ffsll((unsigned long long)peers) - 1 == lane)
+      wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
     {
-      uint32_t sum = 0;
-      for (int q = 0; q < kDigits; ++q)
-        sum += table[t * kDigits + q];
-      part[t] = sum;
-      __syncthreads();
-      for (int o = 1; o < kSortThreads; o <<= 1) {
-        uint32_t v = t >= o ? part[t - o] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+      // digit t: offsets of each wavefront's run, then the running base
+      uint32_t pre = run[t];
+#pragma unroll
+      for (int q = 0; q < kWaves; ++q) {
+        const uint32_t c = wcnt[q][t];
+        wcnt[q][t] = pre;
+        pre += c;
       }
-      uint32_t run = t ? part[t - 1] : 0u;
-      for (int q = 0; q < kDigits; ++q) {
-        const uint32_t v = table[t * kDigits + q];
-        table[t * kDigits + q] = run;
-        run += v;
+      run[t] = pre;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = wcnt[w][d] + (uint32_t)__popcll(peers & below);
+      if (pos < (uint32_t)n) {
+        kout[pos] = key;
+        vout[pos] = val;
       }
     }
     __syncthreads();
-    uint32_t pos[kDigits];
+  }
+}
+
+// Cell ranges from the sorted ids and the reorder of the per-atom state:
+// positions into the other position buffer, the rest through scratch.
+static __global__ __launch_bounds__(kBlock) void k_b_gather(BArgs B) {
+  const long t = xcd_block() * kBlock + threadIdx.x;
+  const long tl = t / kTile;
+  if (tl >= B.ntiles)
+    return;
+  const int tile = B.tiles[tl];
+  const int s = B.tsys[tile];
+  const BSys& S = B.sys[s];
+  const BState& st = B.st[s];
+  if (!(st.act && st.flag))
+    return;
+  const long a0 = S.a0;
+  const long k = (long)tile * kTile - a0 + (t % kTile);
+  const int n = S.n;
+  if (k >= n)
+    return;
+  const uint32_t* K = B.R.keys[S.passes & 1] + a0;
+  const uint32_t* V = B.R.vals[S.passes & 1] + a0;
+  const uint32_t key = K[k];
+  if (k == 0 || K[k - 1] != key)
+    B.R.cstart[S.c0 + key] = (int)k;
+  if (k == n - 1 || K[k + 1] != key)
+    B.R.cend[S.c0 + key] = (int)k + 1;
+  long o = V[k];
+  if (o >= n)
+    o = k; // unreachable for a complete permutation; keeps every read inside the system
+  const double* xc = B.R.x[st.par] + 3 * a0;
+  double* xo = B.R.x[st.par ^ 1] + 3 * a0;
 #pragma unroll
-    for (int d = 0; d < kDigits; ++d)
-      pos[d] = table[d * kSortThreads + t];
-    for (int k = lo; k < hi; ++k) {
-      const uint32_t key = ka[k];
-      const uint32_t d = (key >> shift) & (kDigits - 1);
-      uint32_t p = 0;
+  for (int q = 0; q < 3; ++q) {
+    xo[3 * k + q] = xc[3 * o + q];
+    B.R.sv[3 * (a0 + k) + q] = B.R.v[3 * (a0 + o) + q];
+    B.R.sa[3 * (a0 + k) + q] = B.R.a[3 * (a0 + o) + q];
+  }
+  B.R.sz[a0 + k] = B.R.z[a0 + o];
+  B.R.sffm[a0 + k] = B.R.ffm[a0 + o];
+  B.R.smass[a0 + k] = B.R.mass[a0 + o];
+  B.R.sperm[a0 + k] = B.R.perm[a0 + o];
+}
+
+static __global__ __launch_bounds__(kBlock) void k_b_commit(BArgs B) {
+  const long t = xcd_block() * kBlock + threadIdx.x;
+  const long tl = t / kTile;
+  if (tl >= B.ntiles)
+    return;
+  const int tile = B.tiles[tl];
+  const int s = B.tsys[tile];
+  const BSys& S = B.sys[s];
+  const BState& st = B.st[s];
+  if (!(st.act && st.flag))
+    return;
+  const long a0 = S.a0;
+  const long k = (long)tile * kTile - a0 + (t % kTile);
+  if (k >= S.n)
+    return;
+  const long g = a0 + k;
 #pragma unroll
-      for (int e = 0; e < kDigits; ++e)
-        if (d == (uint32_t)e) {
-          p = pos[e];
-          pos[e] = p + 1;
-        }
-      kb[p] = key;
-      vb[p] = va[k];
-    }
-    uint32_t* tk = ka;
-    ka = kb;
-    kb = tk;
-    uint32_t* tv = va;
-    va = vb;
-    vb = tv;
+  for (int q = 0; q < 3; ++q) {
+    B.R.v[3 * g + q] = B.R.sv[3 * g + q];
+    B.R.a[3 * g + q] = B.R.sa[3 * g + q];
   }
-  __syncthreads();
-  // sorted keys where the list build reads them (keys[0])
-  uint32_t* k0 = B.R.keys[0] + a0;
-  if (ka != k0)
-    for (int k = lo; k < hi; ++k)
-      k0[k] = ka[k];
-  // cell ranges
-  int* cs = B.R.cstart + S.c0;
-  int* ce = B.R.cend + S.c0;
-  for (int q = t; q <= g.ncell; q += kSortThreads) {
-    cs[q] = 0;
-    ce[q] = 0;
+  B.R.z[g] = B.R.sz[g];
+  B.R.ffm[g] = B.R.sffm[g];
+  B.R.mass[g] = B.R.smass[g];
+  B.R.perm[g] = B.R.sperm[g];
+  // FP32 positions folded into [0, L) for the list build
+  const double* xo = B.R.x[st.par ^ 1] + 3 * a0;
+  float w[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double v = xo[3 * k + q];
+    v -= S.b.L[q] * floor(v / S.b.L[q]);
+    w[q] = (float)v;
   }
-  __syncthreads();
-  for (int k = lo; k < hi; ++k) {
-    const uint32_t key = ka[k];
-    if (k == 0 || ka[k - 1] != key)
-      cs[key] = k;
-    if (k == n - 1 || ka[k + 1] != key)
-      ce[key] = k + 1;
-  }
-  // reorder: positions into the other buffer, the rest through scratch
+  B.R.xf[g] = make_float4(w[0], w[1], w[2], 0.0f);
+}
+
+// The sorted positions become the current ones.
+static __global__ __launch_bounds__(kBlock) void k_b_sorted(BArgs B) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= B.nsys)
+    return;
+  BState& st = B.st[s];
+  if (!B.sys[s].live || !(st.act && st.flag))
+    return;
   const int np = st.par ^ 1;
-  double* xo = B.R.x[np] + 3 * a0;
-  for (int k = lo; k < hi; ++k) {
-    const long o = va[k];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      xo[3 * k + q] = xc[3 * o + q];
-      B.R.sv[3 * (a0 + k) + q] = B.R.v[3 * (a0 + o) + q];
-      B.R.sa[3 * (a0 + k) + q] = B.R.a[3 * (a0 + o) + q];
-    }
-    B.R.sz[a0 + k] = B.R.z[a0 + o];
-    B.R.sffm[a0 + k] = B.R.ffm[a0 + o];
-    B.R.smass[a0 + k] = B.R.mass[a0 + o];
-    B.R.sperm[a0 + k] = B.R.perm[a0 + o];
-  }
-  __syncthreads();
-  for (int k = lo; k < hi; ++k) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      B.R.v[3 * (a0 + k) + q] = B.R.sv[3 * (a0 + k) + q];
-      B.R.a[3 * (a0 + k) + q] = B.R.sa[3 * (a0 + k) + q];
-    }
-    B.R.z[a0 + k] = B.R.sz[a0 + k];
-    B.R.ffm[a0 + k] = B.R.sffm[a0 + k];
-    B.R.mass[a0 + k] = B.R.smass[a0 + k];
-    B.R.perm[a0 + k] = B.R.sperm[a0 + k];
-    // FP32 positions folded into [0, L) for the list build
-    float w[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      double v = xo[3 * k + q];
-      v -= S.b.L[q] * floor(v / S.b.L[q]);
-      w[q] = (float)v;
-    }
-    B.R.xf[a0 + k] = make_float4(w[0], w[1], w[2], 0.0f);
-  }
-  __syncthreads();
-  if (t == 0) {
-    st.par = np;
-    st.upar = np;
-    st.maxcnt = 0u;
-    st.sumcnt = 0ull;
-    st.builds += 1;
-  }
+  st.par = np;
+  st.upar = np;
+  st.maxcnt = 0u;
+  st.sumcnt = 0ull;
+  st.builds += 1;
 }
 
 // ---------------------------------------------------------------- per-tile views
@@ -384,7 +492,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
   BuildArgs A{};
   A.x = B.R.x[st.par] + 3 * a0;
   A.xf = B.R.xf + a0;
-  A.skeys = B.R.keys[0] + a0;
+  A.skeys = B.R.keys[S.passes & 1] + a0;
   A.cstart = B.R.cstart + S.c0;
   A.cend = B.R.cend + S.c0;
   A.nbr = B.R.nbr + S.l0;

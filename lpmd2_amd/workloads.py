@@ -23,7 +23,8 @@ GUPTA = {
     "Al": dict(z=13, A=0.1221, r0=2.864, p=8.612, B=1.3160, q=2.516, a=4.05),
     "Ni": dict(z=28, A=0.0376, r0=2.491, p=16.999, B=1.0700, q=1.189, a=3.524),
 }
-# Finnis-Sinclair-type parameter sets used for the fcc metals of the served mix
+# Finnis-Sinclair parameter sets for the served mix: synthetic, not fitted to the
+# metals (they bind the fcc lattice at these lattice constants, nothing more)
 FINNIS_SINCLAIR = {
     "Cu": dict(z=29, c0=0.6, c1=-0.3, c2=0.04, A=1.2, B=0.5, c=4.9, d=4.6, a=3.61),
     "Al": dict(z=13, c0=0.5, c1=-0.25, c2=0.035, A=1.1, B=0.4, c=5.5, d=5.2, a=4.05),

@@ -39,6 +39,15 @@ def test_no_device_fails_loudly():
     with pytest.raises(lpmd2_amd.LpmdxError) as e:
         lpmd2_amd.System(np.zeros((4, 3)), [10.0] * 3, 18)
     assert e.value.code == lpmd2_amd.EDEVICE
+    from lpmd2_amd.batch import Batch
+    from lpmd2_amd.server import Server
+
+    with pytest.raises(lpmd2_amd.LpmdxError) as e:
+        Batch(1024)
+    assert e.value.code == lpmd2_amd.EDEVICE
+    with pytest.raises(lpmd2_amd.LpmdxError) as e:
+        Server(1 << 30)
+    assert e.value.code == lpmd2_amd.EDEVICE
 
 
 def test_null_arguments_are_rejected():
@@ -48,6 +57,13 @@ def test_null_arguments_are_rejected():
     assert lib.lpmdx_system_create(None, 4, None, None, None) == lpmd2_amd.EINVAL
     assert lib.lpmdx_update_forces(None, None, None, None) == lpmd2_amd.EINVAL
     assert b"NULL" in lib.lpmdx_last_error()
+    from lpmd2_amd.server import _bind
+
+    L = _bind()
+    assert L.lpmdx_server_submit(None, 4, None, None, None, None, None, None, 1, None, None, None, None, 1.0, 1,
+                                 None) == lpmd2_amd.EINVAL
+    assert L.lpmdx_server_stats(None, None) == lpmd2_amd.EINVAL
+    assert L.lpmdx_server_create(None, 1 << 30, 16) == lpmd2_amd.EINVAL
 
 
 def test_lattice_generator_matches_crystal3d(golden_dir):
