@@ -133,6 +133,35 @@ def cpu_model():
     return "unknown"
 
 
+def copy_probe(nbytes: int):
+    """One device-to-device hipMemcpy of nbytes (reads and writes nbytes once)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy probe: {nbytes} bytes device to device")
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary profiles/<kernel>_<workload>_traffic.json (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counter on a copy of known size), or
+    None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def plain_run(job, steps=None, warmup=0):
     """One job on the plain single-system path: (seconds to create and upload,
     seconds of `steps` velocity-Verlet steps after `warmup` steps)."""
@@ -181,6 +210,19 @@ def batched_leg(args):
     batched = atoms * steps / t_b
     log(f"  batched {steps} steps: {t_b:.3f} s, {batched:.4e} atom-steps/s, {info}")
     b.close()
+    out = {
+        "value": batched,
+        "unit": "atom-steps/s",
+        "systems": len(ens),
+        "atoms": atoms,
+        "steps": steps,
+        "ms_per_step": 1e3 * t_b / steps,
+        "lanes": {k: v for k, v in lanes.items() if k in ("lennardjones", "suttonchen")},
+        "workload": "512 systems, N log-uniform [256, 16384], half LJ Ar rho* 0.8-1.0 (skin 1.0), "
+                    "half Sutton-Chen Cu/Ni/Ag rc 2a (skin 0.3), 100-900 K, seed 3",
+    }
+    if args.no_plain:
+        return out
     # the plain path, one system at a time
     t_plain = 0.0
     t0 = time.perf_counter()
@@ -190,20 +232,10 @@ def batched_leg(args):
     plain = atoms * args.plain_steps / t_plain
     log(f"  plain one at a time ({args.plain_steps} steps each): {t_plain:.3f} s of steps "
         f"({time.perf_counter() - t0:.1f} s wall), {plain:.4e} atom-steps/s")
-    return {
-        "value": batched,
-        "unit": "atom-steps/s",
-        "systems": len(ens),
-        "atoms": atoms,
-        "steps": steps,
-        "ms_per_step": 1e3 * t_b / steps,
-        "lanes": {k: v for k, v in lanes.items() if k in ("lennardjones", "suttonchen")},
-        "plain_value": plain,
-        "plain_steps_per_system": args.plain_steps,
-        "speedup_vs_plain": batched / plain,
-        "workload": "512 systems, N log-uniform [256, 16384], half LJ Ar rho* 0.8-1.0 (skin 1.0), "
-                    "half Sutton-Chen Cu/Ni/Ag rc 2a (skin 0.3), 100-900 K, seed 3",
-    }
+    out["plain_value"] = plain
+    out["plain_steps_per_system"] = args.plain_steps
+    out["speedup_vs_plain"] = batched / plain
+    return out
 
 
 def served_leg(args):
@@ -248,6 +280,9 @@ def served_leg(args):
             "rounds": int(st["rounds"]),
         }
         log(f"served at {rate} jobs/s: {rec}")
+        out["rates"].append(rec)
+        if args.no_plain:
+            continue
         if plain_service is None:
             plain_service = []
             for _, j in jobs:
@@ -264,8 +299,8 @@ def served_leg(args):
         rec["plain_fifo_p50_s"] = float(np.percentile(plat, 50))
         rec["plain_fifo_p99_s"] = float(np.percentile(plat, 99))
         rec["speedup_vs_plain"] = rec["value"] / plain_rate
-        out["rates"].append(rec)
-    out["plain_value"] = plain_rate
+    if plain_service is not None:
+        out["plain_value"] = plain_rate
     out["value"] = max(r["value"] for r in out["rates"])
     out["unit"] = "atom-steps/s"
     return out
@@ -288,6 +323,11 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 1e6])
     ap.add_argument("--serve-budget", type=int, default=16, help="server HBM budget in GiB")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a copy of known size that calibrates "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the one-at-a-time plain comparisons of the batched and served legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: each box has one MI355X and multi-GPU runs are not part of this build",
@@ -300,6 +340,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
     w = WORKLOADS[args.workload]
     log(f"workload {args.workload}: {w['desc']}")
     s, x, L, v = make_system(w)
@@ -341,7 +383,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(args.workload, kname),
         "avg_launch_us": round(avg_force_s * 1e6, 2),
         "launches": f_n,
         "bytes_per_launch": int(bytes_force),

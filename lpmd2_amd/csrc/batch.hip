@@ -68,7 +68,7 @@ struct HostSys {
   int kind = 0;
 };
 
-constexpr long kBytesPerAtom = 240; // per atom slot over all per-atom arenas
+constexpr long kBytesPerAtom = 264; // per atom slot over all per-atom arenas
 
 } // namespace
 
@@ -77,7 +77,7 @@ struct lpmdx_batch {
   long acap = 0, lcap = 0, ccap = 0;
   Arena atoms, lists, cells; // atoms counted in tiles
   // per-atom arenas
-  DBuf<double> x0, x1, v, a, sv, sa, ffm, mass, sffm, smass, rho;
+  DBuf<double> x0, x1, xb, v, a, sv, sa, ffm, mass, sffm, smass, rho;
   DBuf<int> z, sz, perm, sperm, cnt, nbr, cstart, cend, tsys;
   DBuf<float4> xf;
   DBuf<float> w;
@@ -134,6 +134,7 @@ struct lpmdx_batch {
     BArena R{};
     R.x[0] = x0.p;
     R.x[1] = x1.p;
+    R.xb = xb.p;
     R.v = v.p;
     R.a = a.p;
     R.z = z.p;
@@ -183,7 +184,7 @@ struct lpmdx_batch {
     lists.init(lcap / kTile);
     cells.init(ccap);
     const size_t A = std::max<long>(acap, kTile);
-    for (DBuf<double>* b : {&x0, &x1, &v, &a, &sv, &sa})
+    for (DBuf<double>* b : {&x0, &x1, &xb, &v, &a, &sv, &sa})
       b->alloc(3 * A);
     for (DBuf<double>* b : {&ffm, &mass, &sffm, &smass, &rho})
       b->alloc(A);

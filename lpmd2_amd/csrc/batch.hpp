@@ -62,7 +62,7 @@ struct BSys {
 
 // Mutable per-system state on the device.
 struct BState {
-  double disp;        // displacement bound since the list build
+  double disp;        // largest displacement from the build positions
   long left;          // steps still to run
   long done;          // steps run
   unsigned int maxd2; // largest |dx|^2 of the last step (float bits)
@@ -82,6 +82,7 @@ struct BState {
 // Arena pointers (one entry per atom slot unless noted).
 struct BArena {
   double* x[2]; // [3A] position buffers
+  double* xb;   // [3A] positions at the last list build
   double* v;    // [3A]
   double* a;    // [3A]
   int* z;
@@ -157,7 +158,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B) {
   st.act = 1;
   st.left -= 1;
   st.done += 1;
-  st.disp += sqrt((double)m);
+  st.disp = sqrt((double)m); // largest |x - xb| of the positions about to be used
   st.flag = (!st.built) || (st.disp > S.budget);
   if (st.flag) {
     st.disp = 0.0;
@@ -359,9 +360,11 @@ static __global__ __launch_bounds__(kBlock) void k_b_gather(BArgs B) {
     o = k; // unreachable for a complete permutation; keeps every read inside the system
   const double* xc = B.R.x[st.par] + 3 * a0;
   double* xo = B.R.x[st.par ^ 1] + 3 * a0;
+  double* xb = B.R.xb + 3 * a0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     xo[3 * k + q] = xc[3 * o + q];
+    xb[3 * k + q] = xc[3 * o + q];
     B.R.sv[3 * (a0 + k) + q] = B.R.v[3 * (a0 + o) + q];
     B.R.sa[3 * (a0 + k) + q] = B.R.a[3 * (a0 + o) + q];
   }
@@ -465,6 +468,7 @@ __device__ __forceinline__ ForceArgs force_view(const BArgs& B, int s, double* x
   A.cnt = B.R.cnt + a0;
   A.w_rho = B.R.w + a0;
   A.maxd2 = &B.st[s].maxd2;
+  A.xb = B.R.xb + 3 * a0;
   A.P = S.P[0];
   A.terms = S.nterm > 1 ? S.P : nullptr;
   A.nterm = S.nterm;

@@ -101,8 +101,9 @@ struct lpmdx_system {
   bool list_built = false;
   bool forces_ok = false; // a = F(x) for the user-visible positions
   bool drifted = false;   // user positions are in xn; x = drift of them
-  double disp_acc = 0.0;  // displacement bound since the list build
-  double rate = 0.0;      // largest per-step displacement seen
+  double disp_cur = 0.0; // largest |x - xb| of the positions of the next force evaluation
+  long chunk = 8;        // steps of the next velocity-Verlet chunk
+  long n_rollbacks = 0;
   double last_e[12] = {0};
   bool last_e_valid = false;
   long n_builds = 0, n_forces = 0, n_steps = 0, max_list = 0, sum_list = 0;
@@ -407,7 +408,6 @@ struct lpmdx_system {
       fail(LPMDX_ECAPACITY, "verletlist: maxneighbors is too small!");
     hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
     list_built = true;
-    disp_acc = 0.0;
     n_builds++;
   }
 
@@ -550,6 +550,7 @@ struct lpmdx_system {
       A.cnt = cnt.p;
       A.partials = partials.p;
       A.maxd2 = maxd2;
+      A.xb = xb.p;
       A.P = device_view(P, rcf, rcr);
       A.b = box;
       for (int k = 0; k < 3; ++k)
@@ -585,6 +586,7 @@ struct lpmdx_system {
       A.v = v.p;
       A.a = a.p;
       A.maxd2 = maxd2;
+      A.xb = xb.p;
       A.b = box;
       for (int k = 0; k < 3; ++k)
         A.ll[k] = ll[k];
@@ -655,22 +657,30 @@ struct lpmdx_system {
     if (n == 0 || steps <= 0)
       return;
     unsigned int* md = ctrl.p->maxd2;
+    auto read_disp = [&](long s) {
+      float f;
+      memcpy(&f, &hctrl->maxd2[s], sizeof f);
+      return std::sqrt((double)f);
+    };
     // reach the look-ahead state: x = next positions, (v, a) of the last step
     if (!drifted) {
-      if (!forces_ok) {
+      if (!forces_ok)
         update_cell();
-        hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
+      hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
+      if (!forces_ok) {
         forces(kInitDrift, false, dt, md);
       } else {
-        hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
         ForceArgs A{};
         A.x = x.p;
         A.xn = xn.p;
         A.v = v.p;
         A.a = a.p;
         A.maxd2 = md;
+        A.xb = xb.p;
+        A.b = box;
         A.n = n;
         A.dt = dt;
+        A.mode = kStep;
         for (int k = 0; k < 3; ++k)
           A.ll[k] = ll[k];
         k_drift<<<blocks_for(n), kBlock, 0, st>>>(A);
@@ -681,13 +691,15 @@ struct lpmdx_system {
       forces_ok = true;
       hipchk(hipMemcpyAsync(&hctrl->maxd2[0], md, sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       sync();
-      float f;
-      memcpy(&f, &hctrl->maxd2[0], sizeof f);
-      disp_acc += std::sqrt((double)f);
-      rate = std::max(rate, std::sqrt((double)f));
+      disp_cur = read_disp(0);
     }
+    // Steps run in chunks without a host round trip; after a chunk the exact
+    // displacement of every step's positions from the build positions says
+    // whether the chunk's lists stayed valid (every atom within half the skin
+    // of its build position).  A chunk that went past it is rolled back and
+    // re-run up to its last valid step; the list is rebuilt there.
     long done = 0;
-    long rollbacks = 0;
+    long forced = 0; // steps known to be valid after a roll-back
     while (done < steps) {
       long m = steps - done;
       if (obs_every > 0)
@@ -699,18 +711,19 @@ struct lpmdx_system {
         need = true; // UpdateCell rebuilds every call
       } else {
         const double b = budget();
-        if (!list_built || list_radius() != rl || disp_acc > b)
-          need = true;
-        double avail = need ? b : b - disp_acc;
-        double r = rate > 0 ? 1.5 * rate : 0.0;
-        long allowed = r > 0 ? (long)std::floor(avail / r) : 4;
-        allowed = std::max(1L, allowed);
-        m = std::min(m, allowed);
+        need = !list_built || list_radius() != rl || disp_cur > b;
+        if (forced > 0)
+          m = std::min(m, forced);
+        else
+          m = std::min(m, chunk);
       }
+      forced = 0;
       // positions x are where the next force evaluation happens
       drifted = false;
-      if (need)
+      if (need) {
         rebuild();
+        disp_cur = 0.0;
+      }
       const bool snapshot = cm.kind != kCmLinkedCell;
       if (snapshot) {
         xs.alloc(3 * n);
@@ -728,37 +741,37 @@ struct lpmdx_system {
       }
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       sync();
-      // validity: the force at launch s saw positions displaced by
-      // disp_acc + sum_{k<s} d_k from the build positions
+      // the force of step s saw the positions drifted by step s-1
       const double b = budget();
-      double acc = disp_acc;
+      double d = disp_cur, growth = 0.0;
       long bad = -1;
-      double rmax = 0.0;
       for (long s = 0; s < m; ++s) {
-        if (cm.kind != kCmLinkedCell && acc > b) {
+        if (cm.kind != kCmLinkedCell && d > b) {
           bad = s;
           break;
         }
-        float f;
-        memcpy(&f, &hctrl->maxd2[s], sizeof f);
-        double d = std::sqrt((double)f);
-        acc += d;
-        rmax = std::max(rmax, d);
+        const double dn = read_disp(s);
+        growth = std::max(growth, dn - d);
+        d = dn;
       }
       if (bad >= 0) {
-        // roll back to the chunk start and rebuild there
+        // back to the chunk start; re-run the valid steps, then rebuild
         hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
         hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
         hipchk(hipMemcpyAsync(a.p, as.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        rate = std::max(rate * 2.0, rmax);
-        list_built = false;
         drifted = true;
-        if (++rollbacks > 64)
-          fail(LPMDX_EDEVICE, "velocityverlet: list validity could not be kept");
+        forced = bad;
+        chunk = std::max<long>(1, bad);
+        n_rollbacks++;
         continue;
       }
-      disp_acc = acc;
-      rate = std::max(0.5 * rate, rmax);
+      disp_cur = d;
+      // next chunk: double while the displacement stays well inside the
+      // budget, but no longer than its growth rate allows
+      long next = std::min<long>(2 * chunk, kMaxChunk);
+      if (growth > 0.0)
+        next = std::min<long>(next, std::max<long>(1, (long)(0.9 * (b - disp_cur) / growth)));
+      chunk = std::max<long>(1, next);
       drifted = true;
       cm.ucalls += 2 * m;
       done += m;

@@ -179,7 +179,8 @@ struct ForceArgs {
   const int* cnt;
   const float* w_rho;   // EAM: [n] rho^{-1/2}
   double* partials;     // [nblocks * 12] when evaluating energies
-  unsigned int* maxd2;  // per-step max displacement^2 (float bits)
+  unsigned int* maxd2;  // max over atoms of |x_next - xb|^2 (float bits), or null
+  const double* xb;     // [3n] positions at the last list build
   DevPot P;
   const DevPot* terms; // several potentials of one kind on one list (species pairs), or null
   int nterm;
@@ -192,6 +193,19 @@ struct ForceArgs {
   int mode;
   int accumulate; // add to a instead of overwriting (several potentials)
 };
+
+// |p - xb_i|^2 through the minimum image: how far atom i has moved since the
+// list was built (the list stays valid while every atom is within half the
+// skin of its build position).
+__device__ __forceinline__ float build_displacement2(const ForceArgs& A, long i, double px, double py,
+                                                     double pz) {
+  if (!A.xb)
+    return 0.0f;
+  const double* q = A.xb + 3 * i;
+  double dx = px - q[0], dy = py - q[1], dz = pz - q[2];
+  min_image(A.b, dx, dy, dz);
+  return (float)sqmod(dx, dy, dz);
+}
 
 // Velocity Verlet tail for atom i given its new acceleration (FP64):
 // plugins/velocityverlet.cc:60-75, in the reference's order of operations.
@@ -230,8 +244,7 @@ __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, d
   double px = (x[0] + vx * dt) + ax * 0.5 * dt * dt;
   double py = (x[1] + vy * dt) + ay * 0.5 * dt * dt;
   double pz = (x[2] + vz * dt) + az * 0.5 * dt * dt;
-  const float ddx = (float)(px - x[0]), ddy = (float)(py - x[1]), ddz = (float)(pz - x[2]);
-  d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+  d2 = build_displacement2(A, i, px, py, pz);
   double* xn = A.xn + 3 * i;
   xn[0] = fitted_inside(px, A.ll[0]);
   xn[1] = fitted_inside(py, A.ll[1]);
@@ -509,13 +522,17 @@ static __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const 
 // Drift only: xn = FittedInside((x + v dt) + a/2 dt^2) (velocityverlet.cc:60-67).
 static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= A.n)
-    return;
-  const double dt = A.dt;
-  for (int q = 0; q < 3; ++q) {
-    double p = (A.x[3 * i + q] + A.v[3 * i + q] * dt) + A.a[3 * i + q] * 0.5 * dt * dt;
-    A.xn[3 * i + q] = fitted_inside(p, A.ll[q]);
+  float d2 = 0.0f;
+  if (i < A.n) {
+    const double dt = A.dt;
+    double p[3];
+    for (int q = 0; q < 3; ++q) {
+      p[q] = (A.x[3 * i + q] + A.v[3 * i + q] * dt) + A.a[3 * i + q] * 0.5 * dt * dt;
+      A.xn[3 * i + q] = fitted_inside(p[q], A.ll[q]);
+    }
+    d2 = build_displacement2(A, i, p[0], p[1], p[2]);
   }
+  flush_maxd2(A, d2);
 }
 
 // Kinetic energy and momentum partials (properties.h:22-81).
