@@ -117,6 +117,18 @@ LPMDX_API int lpmdx_potential_clear(lpmdx_system* sys);
  * the potential energy, virial and 3x3 stress (row-major stress[a][b]). */
 LPMDX_API int lpmdx_update_forces(lpmdx_system* sys, double* epot, double* virial, double stress[9]);
 
+/* One potential's UpdateForces (pairpotential.cc:35-84 for pair potentials,
+ * metalpotential.cc:62-180 for metals) on its own: potential k (the k-th
+ * appended, from 0) evaluated on the lists of the last UpdateCell (run first
+ * if the positions changed since).  acc[3n] receives that potential's
+ * accelerations F_k * forcefactor / m in caller order (what the reference adds
+ * to each atom's acceleration); its energy, virial and stress go to epot,
+ * virial, stress (each may be NULL).  The system's own accelerations are left
+ * as they are.  This is the call a plugin forwards one Potential::UpdateForces
+ * of a CombinedPotential to. */
+LPMDX_API int lpmdx_potential_forces(lpmdx_system* sys, int k, double* acc, double* epot, double* virial,
+                                     double stress[9]);
+
 /* The application loop with the velocityverlet integrator (lpmd/lpmd.cc:70-103,
  * plugins/velocityverlet.cc:55-75, twostepintegrator.cc:12-68): `steps`
  * DoSteps of length dt (fs).  State stays device-resident between calls.

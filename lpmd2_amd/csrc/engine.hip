@@ -101,6 +101,8 @@ struct lpmdx_system {
   bool list_built = false;
   bool forces_ok = false; // a = F(x) for the user-visible positions
   bool drifted = false;   // user positions are in xn; x = drift of them
+  long pos_version = 0;   // bumped whenever the user positions change
+  long cell_version = -1; // pos_version at the last UpdateCell
   double disp_cur = 0.0; // largest |x - xb| of the positions of the next force evaluation
   long chunk = 8;        // steps of the next velocity-Verlet chunk
   long n_rollbacks = 0;
@@ -450,6 +452,7 @@ struct lpmdx_system {
         rebuild();
     }
     cm.ucalls++;
+    cell_version = pos_version;
   }
 
   // user positions back into x (undo the look-ahead drift)
@@ -632,6 +635,53 @@ struct lpmdx_system {
     last_e_valid = true;
   }
 
+  // One potential's UpdateForces (pairpotential.cc:35-84 / metalpotential.cc:62-180
+  // for potential k alone): its accelerations into asum (internal order),
+  // its energy, virial and stress into red.
+  void forces_term(int k) {
+    const PotParams& P = *pots[k];
+    double rcf, rcr;
+    filters(P, rcf, rcr);
+    const long nb = blocks_for(n * lpa);
+    asum.alloc(3 * n);
+    ForceArgs A{};
+    A.x = x.p;
+    A.xn = xn.p;
+    A.v = v.p;
+    A.a = asum.p;
+    A.z = z.p;
+    A.ffm = ffm.p;
+    A.nbr = nbr.p;
+    A.cnt = cnt.p;
+    A.partials = partials.p;
+    A.maxd2 = ctrl.p->maxd2;
+    A.xb = xb.p;
+    A.P = device_view(P, rcf, rcr);
+    A.b = box;
+    for (int q = 0; q < 3; ++q)
+      A.ll[q] = ll[q];
+    A.n = n;
+    A.stride = stride;
+    A.cap = cap;
+    A.mode = kForcesOnly;
+    A.accumulate = 0;
+    if (P.metal()) {
+      w_rho.alloc(n);
+      rho.alloc(n);
+    }
+    launch_pot<true>(P.kind, A, nb, rcr);
+    k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 12, red.p + 12 * (1 + 2 * k));
+    if (P.metal())
+      k_reduce_partials<<<1, kBlock, 0, st>>>(partials2.p, nb, 12, red.p + 12 * (2 + 2 * k));
+    else
+      k_fill_double<<<1, 32, 0, st>>>(red.p + 12 * (2 + 2 * k), 12, 0.0);
+    launch_check();
+    hipchk(hipMemcpyAsync(hred, red.p + 12 * (1 + 2 * k), 24 * sizeof(double), hipMemcpyDeviceToHost, st),
+           "memcpy");
+    sync();
+    n_forces++;
+  }
+
   double kinetic(double p[3]) {
     const long nb = blocks_for(n);
     partials.alloc(std::max<size_t>((size_t)nb * 4, partials.n));
@@ -766,6 +816,7 @@ struct lpmdx_system {
         continue;
       }
       disp_cur = d;
+      pos_version++;
       // next chunk: double while the displacement stays well inside the
       // budget, but no longer than its growth rate allows
       long next = std::min<long>(2 * chunk, kMaxChunk);
@@ -946,6 +997,7 @@ int lpmdx_system_set_state(lpmdx_system* S, const double* x, const double* v, co
     if (x) {
       to_internal(S, x, S->x.p);
       S->forces_ok = false;
+      S->pos_version++;
     }
     if (v)
       to_internal(S, v, S->v.p);
@@ -1063,6 +1115,37 @@ int lpmdx_update_forces(lpmdx_system* S, double* epot, double* virial, double st
     S->forces(kForcesOnly, true, 0.0, S->ctrl.p->maxd2);
     S->read_energies(epot, virial, stress);
     S->forces_ok = true;
+  });
+}
+
+int lpmdx_potential_forces(lpmdx_system* S, int k, double* acc, double* epot, double* virial, double stress[9]) {
+  return guard([&] {
+    need(S, "system");
+    if (S->cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (k < 0 || k >= (int)S->pots.size())
+      fail(LPMDX_EINVAL, "no potential with index " + std::to_string(k));
+    double e = 0.0, w = 0.0, t[9] = {0};
+    if (S->n > 0) {
+      S->undrift();
+      if (!S->list_built || S->cell_version != S->pos_version)
+        S->update_cell();
+      S->forces_term(k);
+      const double* pr = S->hred;
+      e = pr[0] + pr[12 + 1];
+      w = pr[2];
+      for (int q = 0; q < 9; ++q)
+        t[q] = pr[3 + q];
+      if (acc)
+        to_caller(S, S->asum.p, acc);
+    }
+    if (epot)
+      *epot = e;
+    if (virial)
+      *virial = w;
+    if (stress)
+      for (int q = 0; q < 9; ++q)
+        stress[q] = t[q];
   });
 }
 

@@ -1,0 +1,410 @@
+// Implementation of the LPMD plugin classes over the engine's C ABI.
+#include "lpmdx_lpmd.hpp"
+
+#include <lpmd/basicparticleset.h>
+#include <lpmd/combinedpotential.h>
+#include <lpmd/error.h>
+#include <lpmd/simulation.h>
+
+#include <cmath>
+#include <iostream>
+
+namespace lpmdx_lpmd {
+
+namespace {
+
+void check(int rc, const std::string& what) {
+  if (rc != LPMDX_OK)
+    throw lpmd::RuntimeError(what + ": " + lpmdx_last_error());
+}
+
+struct Keyword {
+  const char* name;
+  const char* def; // nullptr: the reference defines it without a default
+};
+
+// Plugin keywords with the reference's names and defaults
+// (plugins/<name>.cc constructors), so control files carry over unchanged;
+// "skin" is this engine's verletlist extension.
+const std::map<std::string, std::vector<Keyword>>& Keywords() {
+  static const std::map<std::string, std::vector<Keyword>> m = {
+      {"lennardjones", {{"sigma", nullptr}, {"epsilon", nullptr}, {"cutoff", nullptr}}},
+      {"fastlj", {{"sigma", nullptr}, {"epsilon", nullptr}, {"cutoff", nullptr}, {"bins", "500"}}},
+      {"morse", {{"depth", nullptr}, {"a", nullptr}, {"re", nullptr}}},
+      {"buckingham", {{"B1", nullptr}, {"B2", nullptr}, {"Ro", nullptr}, {"cutoff", nullptr}}},
+      {"suttonchen",
+       {{"e", nullptr}, {"a", nullptr}, {"n", nullptr}, {"m", nullptr}, {"c", nullptr}, {"cutoff", nullptr},
+        {"corrections", "false"}}},
+      {"finnissinclair",
+       {{"c0", nullptr}, {"c1", nullptr}, {"c2", nullptr}, {"A", nullptr}, {"B", nullptr}, {"c", nullptr},
+        {"d", nullptr}}},
+      {"gupta", {{"A", nullptr}, {"r0", nullptr}, {"p", nullptr}, {"B", nullptr}, {"q", nullptr}, {"cutoff", nullptr}}},
+      {"linkedcell",
+       {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}, {"warn-outside", "true"}}},
+      {"verletlist", {{"cutoff", "0.0"}, {"extcutoff", "0.0"}, {"maxneighbors", "80"}, {"each", "5"}, {"skin", "0"}}},
+      {"velocityverlet", {{"dt", "1.0"}, {"start", "1"}}},
+  };
+  return m;
+}
+
+void Define(lpmd::Module& m, const std::string& name) {
+  for (const Keyword& k : Keywords().at(name)) {
+    if (k.def)
+      m.DefineKeyword(k.name, k.def);
+    else
+      m.DefineKeyword(k.name);
+  }
+}
+
+const char* kVersion = "2.1-mi355x";
+
+void Positions(lpmd::Configuration& conf, std::vector<double>& x) {
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const long n = atoms.Size();
+  x.resize(3 * n);
+  for (long i = 0; i < n; ++i) {
+    const lpmd::Vector& p = atoms[i].Position();
+    x[3 * i] = p[0];
+    x[3 * i + 1] = p[1];
+    x[3 * i + 2] = p[2];
+  }
+}
+
+void Velocities(lpmd::Configuration& conf, std::vector<double>& v) {
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const long n = atoms.Size();
+  v.resize(3 * n);
+  for (long i = 0; i < n; ++i) {
+    const lpmd::Vector& p = atoms[i].Velocity();
+    v[3 * i] = p[0];
+    v[3 * i + 1] = p[1];
+    v[3 * i + 2] = p[2];
+  }
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ cell manager
+
+DeviceCellManager::DeviceCellManager(const std::string& name, std::string args)
+    : lpmd::Plugin(name, kVersion), kind_(name), args_(args) {
+  Define(*this, name);
+  ProcessArguments(args);
+  cutoff_ = double((*this)["cutoff"]);
+}
+
+DeviceCellManager::~DeviceCellManager() {
+  if (sys_)
+    lpmdx_system_destroy(sys_);
+}
+
+void DeviceCellManager::ShowHelp() const {
+  std::cout << " Module Name        = " << kind_ << " (MI355X engine)\n"
+            << "      Cell binning and neighbour lists on the GPU; keywords as the CPU plugin"
+            << (kind_ == "verletlist" ? ", plus skin (list radius cutoff+skin, rebuilt when exceeded)" : "")
+            << ".\n";
+}
+
+double DeviceCellManager::Cutoff() const { return cutoff_; }
+
+void DeviceCellManager::Reset() {
+  if (sys_)
+    lpmdx_system_destroy(sys_);
+  sys_ = nullptr;
+  n_ = -1;
+  order_.clear();
+  lists_ok_ = false;
+}
+
+void DeviceCellManager::Create(lpmd::Configuration& conf) {
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const lpmd::BasicCell& cell = conf.Cell();
+  if (!cell.IsOrthogonal())
+    throw lpmd::RuntimeError(kind_ + " (MI355X): the cell must be orthogonal");
+  for (int q = 0; q < 3; ++q)
+    if (!cell.Periodicity(q))
+      throw lpmd::RuntimeError(kind_ + " (MI355X): the cell must be periodic along every axis");
+  const long n = atoms.Size();
+  std::vector<int> z(n);
+  std::vector<double> m(n);
+  for (long i = 0; i < n; ++i) {
+    z[i] = atoms[i].Z();
+    m[i] = atoms[i].Mass();
+  }
+  double box[3];
+  for (int q = 0; q < 3; ++q)
+    box[q] = cell[q][q];
+  Reset();
+  check(lpmdx_system_create(&sys_, n, box, z.data(), m.data()), kind_);
+  n_ = n;
+  for (int q = 0; q < 3; ++q)
+    box_[q] = box[q];
+  Positions(conf, xbuf_);
+  check(lpmdx_system_set_state(sys_, xbuf_.data(), nullptr, nullptr), kind_);
+  check(lpmdx_set_cellmanager(sys_, kind_.c_str(), args_.c_str()), kind_); // runs one UpdateCell
+}
+
+lpmdx_system* DeviceCellManager::Bind(lpmd::Configuration& conf, bool upload) {
+  const lpmd::BasicCell& cell = conf.Cell();
+  bool same = sys_ && n_ == conf.Atoms().Size();
+  for (int q = 0; q < 3 && same; ++q)
+    same = box_[q] == cell[q][q];
+  if (!same) {
+    Create(conf);
+    return sys_;
+  }
+  if (upload) {
+    std::vector<double> x;
+    Positions(conf, x);
+    if (x != xbuf_) {
+      xbuf_.swap(x);
+      check(lpmdx_system_set_state(sys_, xbuf_.data(), nullptr, nullptr), kind_);
+      lists_ok_ = false;
+    }
+  }
+  return sys_;
+}
+
+void DeviceCellManager::UpdateCell(lpmd::Configuration& conf) {
+  const bool existed = sys_ != nullptr;
+  Bind(conf, true);
+  if (existed)
+    check(lpmdx_update_cell(sys_), kind_);
+  lists_ok_ = false;
+}
+
+void DeviceCellManager::UpdateAtom(lpmd::Configuration&, long) {
+  // positions are sent with the next UpdateCell or force call
+  lists_ok_ = false;
+}
+
+void DeviceCellManager::BuildNeighborList(lpmd::Configuration& conf, long i, lpmd::NeighborList& nlist, bool full,
+                                          double rcut) {
+  if (!sys_)
+    UpdateCell(conf);
+  if (!lists_ok_ || full != lists_full_ || rcut != lists_rcut_) {
+    const long n = n_;
+    counts_.assign(n, 0);
+    long total = 0;
+    check(lpmdx_neighbor_lists(sys_, full ? 1 : 0, rcut, counts_.data(), 0, nullptr, nullptr, &total), kind_);
+    js_.assign(total > 0 ? total : 1, 0);
+    r2_.assign(total > 0 ? total : 1, 0.0);
+    check(lpmdx_neighbor_lists(sys_, full ? 1 : 0, rcut, counts_.data(), total, js_.data(), r2_.data(), &total),
+          kind_);
+    offs_.assign(n + 1, 0);
+    for (long k = 0; k < n; ++k)
+      offs_[k + 1] = offs_[k] + counts_[k];
+    lists_ok_ = true;
+    lists_full_ = full;
+    lists_rcut_ = rcut;
+  }
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const lpmd::BasicCell& cell = conf.Cell();
+  nlist.Clear();
+  for (long k = offs_[i]; k < offs_[i + 1]; ++k) {
+    const long j = js_[k];
+    lpmd::AtomPair nn;
+    nn.i = &atoms[i];
+    nn.j = &atoms[j];
+    nn.i_index = i;
+    nn.j_index = j;
+    nn.rij = cell.Displacement(atoms[i].Position(), atoms[j].Position());
+    nn.r2 = r2_[k];
+    nlist.Append(nn);
+  }
+}
+
+int DeviceCellManager::IndexOf(const DevicePotential& p) {
+  for (size_t k = 0; k < order_.size(); ++k)
+    if (order_[k] == &p)
+      return (int)k;
+  int s1, s2;
+  p.Species(s1, s2);
+  check(lpmdx_potential_append(sys_, p.Kind().c_str(), p.Args().c_str(), s1, s2), p.Kind());
+  order_.push_back(&p);
+  return (int)order_.size() - 1;
+}
+
+void DeviceCellManager::UsePotentials(const std::vector<const DevicePotential*>& pots) {
+  if (pots == order_)
+    return;
+  check(lpmdx_potential_clear(sys_), kind_);
+  order_.clear();
+  for (const DevicePotential* p : pots)
+    IndexOf(*p);
+}
+
+// ------------------------------------------------------------------ potentials
+
+DevicePotential::DevicePotential(const std::string& name, std::string args)
+    : lpmd::Plugin(name, kVersion), kind_(name), args_(args) {
+  Define(*this, name);
+  ProcessArguments(args);
+  if (name == "finnissinclair") {
+    const double c = double((*this)["c"]), d = double((*this)["d"]);
+    SetCutoff(d >= c ? d : c);
+  } else if (name != "morse") { // morse keeps Potential's HUGE_VAL (plugins/morse.cc:11-21)
+    SetCutoff(double((*this)["cutoff"]));
+  }
+}
+
+void DevicePotential::ShowHelp() const {
+  std::cout << " Module Name        = " << kind_ << " (MI355X engine)\n"
+            << "      Forces on the GPU through the lpmdx engine; keywords as the CPU plugin.\n";
+}
+
+void DevicePotential::Species(int& s1, int& s2) const {
+  const int far = 1 << 20;
+  if (AppliesTo(0, far) && AppliesTo(0, far + 1)) {
+    s1 = s2 = -1; // wildcard
+    return;
+  }
+  s1 = 0;
+  s2 = far;
+  for (int v = 0; v < 4096; ++v)
+    if (AppliesTo(0, v)) {
+      s2 = v;
+      return;
+    }
+}
+
+void DevicePotential::UpdateForces(lpmd::Configuration& conf) {
+  auto* cm = dynamic_cast<DeviceCellManager*>(&conf.GetCellManager());
+  if (!cm)
+    throw lpmd::RuntimeError(kind_ +
+                             " (MI355X): the cell manager must be the MI355X linkedcell or verletlist plugin");
+  lpmdx_system* sys = cm->Bind(conf, true);
+  const int k = cm->IndexOf(*this);
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const long n = atoms.Size();
+  acc_.resize(3 * (n > 0 ? n : 1));
+  double e = 0.0, w = 0.0, s[9];
+  check(lpmdx_potential_forces(sys, k, acc_.data(), &e, &w, s), kind_);
+  for (long i = 0; i < n; ++i)
+    atoms[i].Acceleration() += lpmd::Vector(acc_[3 * i], acc_[3 * i + 1], acc_[3 * i + 2]);
+  energycache = e;
+  conf.Virial() += w;
+  lpmd::Matrix& st = conf.StressTensor();
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      st.Set(a, b, st.Get(a, b) + s[3 * a + b]);
+}
+
+double DevicePotential::energy(lpmd::Configuration&) { return energycache; }
+
+double DevicePotential::AtomEnergy(lpmd::Configuration&, long) {
+  throw lpmd::RuntimeError(kind_ + " (MI355X): per-atom energies are not computed by the GPU engine");
+}
+
+// ------------------------------------------------------------------ integrator
+
+DeviceVelocityVerlet::DeviceVelocityVerlet(std::string args) : lpmd::Plugin("velocityverlet", kVersion) {
+  Define(*this, "velocityverlet");
+  ProcessArguments(args);
+  dt = double((*this)["dt"]);
+  start_ = int((*this)["start"]);
+}
+
+void DeviceVelocityVerlet::ShowHelp() const {
+  std::cout << " Module Name        = velocityverlet (MI355X engine)\n"
+            << "      Velocity Verlet with dt (fs); on the GPU when every potential and the cell\n"
+            << "      manager are MI355X plugins.\n";
+}
+
+void DeviceVelocityVerlet::Initialize(lpmd::Simulation&, lpmd::Potential&) {
+  last_ = nullptr; // the next step sends the host state
+}
+
+void DeviceVelocityVerlet::HostStep(lpmd::Simulation& sim, lpmd::Potential& p) {
+  // TwoStepIntegrator::Advance with VelocityVerlet's position and velocity
+  // updates, in the reference's order of operations
+  p.UpdateForces(sim);
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  lpmd::BasicCell& cell = sim.Cell();
+  const long n = atoms.Size();
+  std::vector<lpmd::Vector> aold(n);
+  for (long i = 0; i < n; ++i) {
+    const lpmd::Vector np =
+        atoms[i].Position() + atoms[i].Velocity() * dt + 0.5 * atoms[i].Acceleration() * dt * dt;
+    aold[i] = atoms[i].Acceleration();
+    atoms[i].Position() = cell.FittedInside(np);
+  }
+  p.UpdateForces(sim);
+  for (long i = 0; i < n; ++i)
+    atoms[i].Velocity() = atoms[i].Velocity() + 0.5 * dt * (aold[i] + atoms[i].Acceleration());
+  last_ = nullptr;
+}
+
+void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
+  auto* cp = dynamic_cast<lpmd::CombinedPotential*>(&p);
+  auto* cm = dynamic_cast<DeviceCellManager*>(&sim.GetCellManager());
+  std::vector<const DevicePotential*> dev;
+  bool device = cm && cp && cp->Size() > 0;
+  for (long k = 0; device && k < cp->Size(); ++k) {
+    auto* d = dynamic_cast<DevicePotential*>(&(*cp)[k]);
+    if (!d)
+      device = false;
+    else
+      dev.push_back(d);
+  }
+  if (!device) {
+    HostStep(sim, p);
+    return;
+  }
+  lpmdx_system* sys = cm->Bind(sim, false);
+  cm->UsePotentials(dev);
+  std::vector<double> x, v;
+  Positions(sim, x);
+  Velocities(sim, v);
+  if (sys != last_ || x != hx_ || v != hv_) {
+    // the host changed the state (or this is the first step): send it; the
+    // engine then starts from a fresh force evaluation
+    check(lpmdx_system_set_state(sys, x.data(), v.data(), nullptr), "velocityverlet");
+    hx_.swap(x);
+    hv_.swap(v);
+    last_ = sys;
+  }
+  double ep = 0.0, ek = 0.0;
+  check(lpmdx_velocityverlet(sys, dt, 1, 1, &ep, &ek), "velocityverlet");
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  hx_.resize(3 * n);
+  hv_.resize(3 * n);
+  ha_.resize(3 * n);
+  check(lpmdx_system_get_state(sys, hx_.data(), hv_.data(), ha_.data()), "velocityverlet");
+  for (long i = 0; i < n; ++i) {
+    atoms[i].Position() = lpmd::Vector(hx_[3 * i], hx_[3 * i + 1], hx_[3 * i + 2]);
+    atoms[i].Velocity() = lpmd::Vector(hv_[3 * i], hv_[3 * i + 1], hv_[3 * i + 2]);
+    atoms[i].Acceleration() = lpmd::Vector(ha_[3 * i], ha_[3 * i + 1], ha_[3 * i + 2]);
+  }
+  cm->Mirror(hx_); // host and device hold the same positions
+  // what CombinedPotential::UpdateForces leaves behind (combinedpotential.h:39-52)
+  double obs[LPMDX_NOBS];
+  check(lpmdx_observables(sys, obs), "velocityverlet");
+  sim.Virial() = obs[LPMDX_OBS_VIRIAL];
+  const lpmd::BasicCell& cell = sim.Cell();
+  const double vol = cell[0][0] * cell[1][1] * cell[2][2];
+  lpmd::Matrix& st = sim.StressTensor();
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      st.Set(a, b, obs[LPMDX_OBS_STRESS + 3 * a + b] * vol / 160217.65); // units.cc:9 pressfactor
+  lpmd::Tag tag("potential-energy");
+  sim.SetTag(sim, tag, ep);
+}
+
+// ------------------------------------------------------------------ factory
+
+lpmd::Plugin* Create(const std::string& name, const std::string& args) {
+  static const char* pots[] = {"lennardjones", "fastlj", "morse", "buckingham", "suttonchen", "finnissinclair",
+                               "gupta"};
+  for (const char* p : pots)
+    if (name == p)
+      return new DevicePotential(name, args);
+  if (name == "linkedcell" || name == "verletlist")
+    return new DeviceCellManager(name, args);
+  if (name == "velocityverlet")
+    return new DeviceVelocityVerlet(args);
+  throw lpmd::RuntimeError("no MI355X plugin named " + name);
+}
+
+} // namespace lpmdx_lpmd

@@ -1,0 +1,11 @@
+// The create/destroy pair of one drop-in plugin object (liblpmd/lpmd/plugin.h:42-46).
+// Compiled once per reference plugin name with -DLPMDX_PLUGIN_NAME='"<name>"'
+// into <name>.so, each linked against liblpmdx_lpmd.so.
+#include "lpmdx_lpmd.hpp"
+
+#ifndef LPMDX_PLUGIN_NAME
+#error "build with -DLPMDX_PLUGIN_NAME='\"<reference plugin name>\"'"
+#endif
+
+lpmd::Plugin* create(std::string args) { return lpmdx_lpmd::Create(LPMDX_PLUGIN_NAME, args); }
+void destroy(lpmd::Plugin* m) { delete m; }

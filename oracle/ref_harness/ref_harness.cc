@@ -30,6 +30,7 @@
 #include <lpmd/cellmanager.h>
 #include <lpmd/combinedpotential.h>
 #include <lpmd/configuration.h>
+#include <lpmd/elements.h>
 #include <lpmd/fixedsizeparticleset.h>
 #include <lpmd/integrator.h>
 #include <lpmd/metalpotential.h>
@@ -157,8 +158,10 @@ void* ref_conf_new(long n, const double* L, const double* pos, const int* z) {
         SetBox(*c, L);
         for (long i = 0; i < n; ++i) {
           c->Atoms()[i].Position() = Vector(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-          if (z)
+          if (z) {
             c->Atoms()[i].Z() = z[i];
+            c->Atoms()[i].Mass() = ElemMass[z[i]];
+          }
         }
       }) != 0)
     return nullptr;
@@ -372,6 +375,84 @@ int ref_vv_run(const char* so_path, const char* args, void* conf, void* cm, long
       vv->Advance(*c, ft);
     // keep the plugin object alive with its old-config copy until here
     delete g.plugin;
+  });
+}
+
+// ---- any plugin, and the reference's composition over loaded plugins -----
+// These drive plugins that are not the reference's own (the MI355X drop-in
+// plugins) through the reference's interfaces: CombinedPotential
+// (combinedpotential.h:39-52), Configuration::SetCellManager and
+// Integrator::Advance.
+void* ref_plugin_open(const char* so_path, const char* args) {
+  Loaded* L = new Loaded();
+  if (Guard([&] { *L = Load(so_path, args); }) != 0) {
+    delete L;
+    return nullptr;
+  }
+  return L;
+}
+
+void ref_plugin_close(void* p) {
+  Loaded* L = static_cast<Loaded*>(p);
+  if (L)
+    delete L->plugin;
+  delete L;
+}
+
+// CombinedPotential::UpdateForces on conf (its cell manager attached before)
+// with the given potentials and species pairs (s = -1: any); acc[3n] and
+// out[11] = energy, virial, stress[a][b] (row-major, StressTensor().Get(a, b)).
+int ref_combined_forces(void* conf, int npot, void** pots, const int* s1, const int* s2, double* acc,
+                        double* out) {
+  return Guard([&] {
+    HarnessConf* c = static_cast<HarnessConf*>(conf);
+    CombinedPotential cp;
+    for (int k = 0; k < npot; ++k) {
+      Potential* p = dynamic_cast<Potential*>(static_cast<Loaded*>(pots[k])->plugin);
+      if (!p)
+        throw std::runtime_error("not a Potential");
+      p->SetValidSpecies(s1 ? s1[k] : -1, s2 ? s2[k] : -1);
+      cp.Append(*p);
+    }
+    cp.UpdateForces(*c);
+    const long n = c->Atoms().Size();
+    for (long i = 0; i < n; ++i)
+      for (int q = 0; q < 3; ++q)
+        acc[3 * i + q] = c->Atoms()[i].Acceleration()[q];
+    out[0] = cp.energy(*c);
+    out[1] = c->Virial();
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        out[2 + 3 * a + b] = c->StressTensor().Get(a, b);
+  });
+}
+
+// Integrator::Initialize then `steps` Integrator::Advance calls on conf with
+// the given potentials (appended to the configuration's CombinedPotential);
+// epot[s] = the potential-energy tag after step s, when the integrator sets it.
+int ref_integrate(void* integrator, void* conf, int npot, void** pots, const int* s1, const int* s2, long steps,
+                  double* epot) {
+  return Guard([&] {
+    lpmd::Integrator* itg = dynamic_cast<lpmd::Integrator*>(static_cast<Loaded*>(integrator)->plugin);
+    if (!itg)
+      throw std::runtime_error("not an Integrator");
+    HarnessConf* c = static_cast<HarnessConf*>(conf);
+    CombinedPotential& cp = c->Potentials();
+    cp.Clear();
+    for (int k = 0; k < npot; ++k) {
+      Potential* p = dynamic_cast<Potential*>(static_cast<Loaded*>(pots[k])->plugin);
+      if (!p)
+        throw std::runtime_error("not a Potential");
+      p->SetValidSpecies(s1 ? s1[k] : -1, s2 ? s2[k] : -1);
+      cp.Append(*p);
+    }
+    itg->Initialize(*c, cp);
+    Tag tag("potential-energy");
+    for (long s = 0; s < steps; ++s) {
+      itg->Advance(*c, cp);
+      if (epot)
+        epot[s] = c->HaveAny(tag) ? atof(c->GetTag(*c, tag).c_str()) : 0.0;
+    }
   });
 }
 

@@ -50,6 +50,11 @@ def lib():
         L.ref_pot_close.argtypes = [vp]
         L.ref_crystal.argtypes = [C.c_char_p, C.c_char_p, l, vp, vp]
         L.ref_vv_run.argtypes = [C.c_char_p, C.c_char_p, vp, vp, l, l, vp]
+        L.ref_plugin_open.restype = vp
+        L.ref_plugin_open.argtypes = [C.c_char_p, C.c_char_p]
+        L.ref_plugin_close.argtypes = [vp]
+        L.ref_combined_forces.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.ref_integrate.argtypes = [vp, vp, i, vp, vp, vp, l, vp]
         _lib = L
     return _lib
 
@@ -172,3 +177,77 @@ def vv_run(args: str, conf: RefConf, cm: RefCellManager, steps: int, acc_tables)
     acc = np.ascontiguousarray(acc_tables, dtype=np.float64)
     ncalls = acc.shape[0]
     _check(lib().ref_vv_run(so("velocityverlet"), (args + " debug none").encode(), conf.h, cm.h, steps, ncalls, _p(acc)))
+
+
+# ---- plugins loaded through the reference's create(args), and the reference's
+# ---- composition driving them (the MI355X drop-in plugins among them)
+DROPIN_DIR = os.path.join(os.path.dirname(_HERE), "lpmd2_amd", "plugins", "_build")
+
+
+def dropin_available() -> bool:
+    return available() and os.path.exists(os.path.join(DROPIN_DIR, "liblpmdx_lpmd.so"))
+
+
+class RefPlugin:
+    """A plugin object made by the create() of a .so file (the reference's
+    own from oracle/_ref, or a drop-in from lpmd2_amd/plugins/_build)."""
+
+    def __init__(self, path: str, args: str):
+        self.path = path
+        self.h = lib().ref_plugin_open(path.encode(), (args + " debug none").encode())
+        if not self.h:
+            raise RuntimeError(lib().ref_last_error().decode())
+
+    @classmethod
+    def reference(cls, name: str, args: str):
+        return cls(so(name).decode(), args)
+
+    @classmethod
+    def dropin(cls, name: str, args: str):
+        return cls(os.path.join(DROPIN_DIR, name + ".so"), args)
+
+    def attach(self, conf: RefConf):
+        """Configuration::SetCellManager(this) (runs its UpdateCell)."""
+        _check(lib().ref_cm_update(self.h, conf.h, 1))
+
+    def update(self, conf: RefConf):
+        _check(lib().ref_cm_update(self.h, conf.h, 0))
+
+    def lists(self, conf: RefConf, full: bool, rcut: float, cap=None):
+        return RefCellManager.lists(self, conf, full, rcut, cap)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ref_plugin_close(self.h)
+            self.h = None
+
+
+def _handles(pots):
+    arr = (C.c_void_p * len(pots))(*[p.h for p in pots])
+    return C.cast(arr, C.c_void_p), arr
+
+
+def _species(pots, species):
+    s1 = np.array([s[0] for s in species] if species else [-1] * len(pots), dtype=np.int32)
+    s2 = np.array([s[1] for s in species] if species else [-1] * len(pots), dtype=np.int32)
+    return s1, s2
+
+
+def combined_forces(conf: RefConf, pots, species=None):
+    """CombinedPotential::UpdateForces over loaded potential plugins on conf
+    (whose cell manager is attached): (acc, energy, virial, stress[3x3])."""
+    hp, keep = _handles(pots)
+    s1, s2 = _species(pots, species)
+    acc = np.zeros((conf.n, 3))
+    out = np.zeros(11)
+    _check(lib().ref_combined_forces(conf.h, len(pots), hp, _p(s1), _p(s2), _p(acc), _p(out)))
+    return acc, float(out[0]), float(out[1]), out[2:].reshape(3, 3)
+
+
+def integrate(integrator: RefPlugin, conf: RefConf, pots, steps: int, species=None):
+    """Integrator::Initialize + `steps` Advance calls; the potential-energy tag per step."""
+    hp, keep = _handles(pots)
+    s1, s2 = _species(pots, species)
+    ep = np.zeros(max(steps, 1))
+    _check(lib().ref_integrate(integrator.h, conf.h, len(pots), hp, _p(s1), _p(s2), steps, _p(ep)))
+    return ep[:steps]

@@ -1,0 +1,57 @@
+"""CPU checks of the drop-in plugins (lpmd2_amd/plugins): every plugin object
+loads through the reference's create(args) entry point (liblpmd/lpmd/plugin.h:42-46),
+parses its keywords like the reference plugin of the same name, reports the
+same cutoff, and fails loudly when no GPU is visible (no CPU fallback).
+Skipped where the plugins are not built (they need the reference headers)."""
+import numpy as np
+import pytest
+
+import lpmd2_amd as E
+from oracle import refcheck as R
+
+pytestmark = pytest.mark.skipif(not R.dropin_available(), reason="drop-in plugins not built")
+
+POTS = [
+    ("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5"),
+    ("fastlj", "sigma 3.41 epsilon 0.0103 cutoff 8.5"),
+    ("morse", "depth 0.3429 a 1.3588 re 2.866"),
+    ("buckingham", "B1 1254.5 B2 2.0 Ro 0.3249 cutoff 7.0"),
+    ("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22"),
+    ("finnissinclair", "c0 1.2371147 c1 -0.3592185 c2 -0.0385607 A 1.828905 B 1.8 c 3.4 d 3.569745"),
+    ("gupta", "A 0.0855 r0 2.556 p 10.960 B 1.2240 q 2.278 cutoff 7.22"),
+]
+
+
+@pytest.mark.parametrize("name,args", POTS, ids=[p[0] for p in POTS])
+def test_potential_plugins_match_reference_keywords(name, args):
+    ours = R.RefPlugin.dropin(name, args)
+    ref = R.RefPlugin.reference(name, args)
+    # Potential::GetCutoff through the reference's interface
+    assert R.lib().ref_pot_cutoff(ours.h) == R.lib().ref_pot_cutoff(ref.h)
+    # a dangling keyword is the same error in both
+    msgs = []
+    for make in (R.RefPlugin.dropin, R.RefPlugin.reference):
+        with pytest.raises(RuntimeError) as e:
+            make(name, args + " cutoff")
+        msgs.append(str(e.value))
+    assert msgs[0] == msgs[1]
+    ours.close()
+
+
+@pytest.mark.parametrize("name,args", [("linkedcell", "cutoff 8.5 mode auto"),
+                                       ("verletlist", "cutoff 8.5 maxneighbors 200 each 5"),
+                                       ("verletlist", "cutoff 8.5 maxneighbors 200 skin 1.0"),
+                                       ("velocityverlet", "dt 2.0")])
+def test_other_plugins_load(name, args):
+    p = R.RefPlugin.dropin(name, args)
+    p.close()
+
+
+def test_without_gpu_the_plugins_fail_loudly():
+    if E.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    x, L = E.fcc(3, 5.26)
+    conf = R.RefConf(x, L, z=18)
+    cm = R.RefPlugin.dropin("linkedcell", "cutoff 8.5 mode auto")
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        cm.attach(conf)
