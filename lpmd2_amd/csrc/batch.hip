@@ -78,7 +78,8 @@ struct lpmdx_batch {
   Arena atoms, lists, cells; // atoms counted in tiles
   // per-atom arenas
   DBuf<double> x0, x1, xb, v, a, sv, sa, ffm, mass, sffm, smass, rho;
-  DBuf<int> z, sz, perm, sperm, cnt, nbr, cstart, cend, tsys;
+  DBuf<int> z, sz, perm, sperm, cnt, nbr, cb, tsys;
+  DBuf<HaloTile> halo; // per atom tile
   DBuf<float4> xf;
   DBuf<float> w;
   DBuf<uint32_t> k0, k1, v0, v1;
@@ -146,8 +147,8 @@ struct lpmdx_batch {
     R.rho = rho.p;
     R.cnt = cnt.p;
     R.nbr = nbr.p;
-    R.cstart = cstart.p;
-    R.cend = cend.p;
+    R.cb = cb.p;
+    R.halo = halo.p;
     R.keys[0] = k0.p;
     R.keys[1] = k1.p;
     R.vals[0] = v0.p;
@@ -173,6 +174,7 @@ struct lpmdx_batch {
     B.chunks = chunks.p;
     B.nchunks = nchunks;
     B.hist = hist.p;
+    B.halo_on = halo_cap() > 0;
     return B;
   }
 
@@ -196,8 +198,8 @@ struct lpmdx_batch {
     w.alloc(A);
     tsys.alloc(A / kTile);
     nbr.alloc(std::max<long>(lcap, kTile));
-    cstart.alloc(std::max<long>(ccap, 1));
-    cend.alloc(std::max<long>(ccap, 1));
+    cb.alloc(std::max<long>(ccap, 2));
+    halo.alloc(std::max<long>(acap / kTile, 1));
     hipchk(hipMemsetAsync(x0.p, 0, 3 * A * sizeof(double), st), "memset");
     hipchk(hipMemsetAsync(x1.p, 0, 3 * A * sizeof(double), st), "memset");
     hipchk(hipMemsetAsync(nbr.p, 0, std::max<long>(lcap, kTile) * sizeof(int), st), "memset");
@@ -288,6 +290,10 @@ struct lpmdx_batch {
     S.rl2 = rl * rl;
     S.rl2f = (float)((rl + 2.0 * pos_err) * (rl + 2.0 * pos_err) * (1.0 + 1e-6));
     S.budget = 0.5 * skin;
+    {
+      const double m = std::sqrt((double)S.rl2f) * (1.0 + 1e-6) + 2.0 * S.budget + 1e-9 * lmax;
+      S.margin = m < hmin ? m : 0.0;
+    }
     S.dt = dt;
     S.kind = H.kind;
     S.metal = metal;
@@ -314,7 +320,7 @@ struct lpmdx_batch {
       atoms.release(t0, tiles);
       fail(LPMDX_ENOMEM, "list arena full");
     }
-    const long c0 = cells.alloc(S.g.ncell + 1);
+    const long c0 = cells.alloc(S.g.ncell + 2);
     if (c0 < 0) {
       atoms.release(t0, tiles);
       lists.release(l0, (lw + kTile - 1) / kTile);
@@ -335,7 +341,7 @@ struct lpmdx_batch {
     H.list0 = l0;
     H.listw = (lw + kTile - 1) / kTile;
     H.cell0 = c0;
-    H.cells = S.g.ncell + 1;
+    H.cells = S.g.ncell + 2;
     // upload the atoms
     std::vector<double> ff(n), ms(n);
     std::vector<int> pm(n), zz2(n);
@@ -486,25 +492,29 @@ struct lpmdx_batch {
     dirty = false;
   }
 
+  // halo cap of the staged kernels; 0 (nothing staged) unless staging is enabled
+  static int halo_cap() { return stage_enabled() ? kHaloCap : 0; }
+
   template <int KIND> void launch_group(Group& G) {
     BArgs B = args(G.tiles.p, G.n);
-    const long thr = G.n * kTile * G.lpa;
-    const long nb = blocks_for(thr);
+    const int fparts = G.metal ? (kStageW | kStageS) : kStageS;
+    const int kHaloCap = halo_cap();
+    const size_t lr = stage_bytes(kHaloCap, 0), lf = stage_bytes(kHaloCap, fparts);
     switch (G.lpa) {
     case 1:
       if (G.metal)
-        k_b_rho<KIND, 1><<<nb, kBlock, 0, st>>>(B);
-      k_b_force<KIND, 1><<<nb, kBlock, 0, st>>>(B);
+        k_b_rho<KIND, 1><<<G.n, kWave, lr, st>>>(B, kHaloCap);
+      k_b_force<KIND, 1><<<G.n, kWave, lf, st>>>(B, kHaloCap, fparts);
       break;
     case 2:
       if (G.metal)
-        k_b_rho<KIND, 2><<<nb, kBlock, 0, st>>>(B);
-      k_b_force<KIND, 2><<<nb, kBlock, 0, st>>>(B);
+        k_b_rho<KIND, 2><<<G.n, 2 * kWave, lr, st>>>(B, kHaloCap);
+      k_b_force<KIND, 2><<<G.n, 2 * kWave, lf, st>>>(B, kHaloCap, fparts);
       break;
     default:
       if (G.metal)
-        k_b_rho<KIND, 4><<<nb, kBlock, 0, st>>>(B);
-      k_b_force<KIND, 4><<<nb, kBlock, 0, st>>>(B);
+        k_b_rho<KIND, 4><<<G.n, 4 * kWave, lr, st>>>(B, kHaloCap);
+      k_b_force<KIND, 4><<<G.n, 4 * kWave, lf, st>>>(B, kHaloCap, fparts);
       break;
     }
   }
@@ -525,8 +535,11 @@ struct lpmdx_batch {
       k_b_commit<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
     }
     k_b_sorted<<<blocks_for(ns), kBlock, 0, st>>>(B);
-    if (n_all)
+    if (n_all) {
+      if (halo_cap() > 0)
+        k_b_halo<<<blocks_for(n_all), kBlock, 0, st>>>(B, halo_cap());
       k_b_build<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
+    }
     for (auto& G : groups) {
       switch (G->kind) {
       case kLJ: launch_group<kLJ>(*G); break;
@@ -583,9 +596,12 @@ struct lpmdx_batch {
     steps_run += todo;
     pull_state();
     std::string bad;
-    for (size_t s = 0; s < hsys.size(); ++s)
+    for (size_t s = 0; s < hsys.size(); ++s) {
+      if (meta[s].live && hst[s].halo_miss)
+        fail(LPMDX_EDEVICE, "neighbour cell outside its tile halo (internal error) in system " + std::to_string(s));
       if (meta[s].live && hst[s].overflow)
         bad += " " + std::to_string(s);
+    }
     if (!bad.empty())
       fail(LPMDX_ECAPACITY, "neighbour list capacity exceeded in system(s):" + bad);
   }
@@ -627,10 +643,17 @@ struct lpmdx_batch {
 
   template <int KIND> void eval_k(long sid, int lpa, long nb, double* pa, int rho_pass) {
     BArgs B = args(nullptr, 0);
+    const int parts = rho_pass ? 0 : (hsys[sid].metal ? (kStageW | kStageS) : kStageS);
+    const int kHaloCap = halo_cap();
+    const size_t lds = stage_bytes(kHaloCap, parts);
     switch (lpa) {
-    case 1: k_b_eval<KIND, 1><<<nb, kBlock, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
-    case 2: k_b_eval<KIND, 2><<<nb, kBlock, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
-    default: k_b_eval<KIND, 4><<<nb, kBlock, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
+    case 1: k_b_eval<KIND, 1><<<nb, kWave, lds, st>>>(B, (int)sid, evala.p, pa, rho_pass, kHaloCap, parts); break;
+    case 2:
+      k_b_eval<KIND, 2><<<nb, 2 * kWave, lds, st>>>(B, (int)sid, evala.p, pa, rho_pass, kHaloCap, parts);
+      break;
+    default:
+      k_b_eval<KIND, 4><<<nb, 4 * kWave, lds, st>>>(B, (int)sid, evala.p, pa, rho_pass, kHaloCap, parts);
+      break;
     }
   }
 
@@ -645,7 +668,7 @@ struct lpmdx_batch {
     const BSys& S = hsys[sid];
     const long n = meta[sid].n;
     const int lpa = S.lpa;
-    const long nb = blocks_for((long)S.ntile * kTile * lpa);
+    const long nb = S.ntile; // one workgroup per tile
     evala.alloc(3 * std::max<long>(n, 1));
     partials.alloc(nb * 12 * 2);
     double* p1 = partials.p;

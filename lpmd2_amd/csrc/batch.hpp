@@ -7,10 +7,14 @@
 //                new list by cell: 4096-atom chunks, one workgroup each, so a
 //                large system is sorted by many workgroups at once
 //   k_b_gather / k_b_commit / k_b_sorted
-//                cell ranges and the reorder of the per-atom state
-//   k_b_build    every tile of those systems: neighbour lists
+//                cell starts and the reorder of the per-atom state
+//   k_b_halo     every tile of those systems: its halo (the cells its
+//                atoms' neighbours can be in, as ranges of sorted atoms)
+//   k_b_build    every tile of those systems: neighbour lists (halo slots)
 //   k_b_rho      EAM systems: rho pass            (one launch per kind)
-//   k_b_force    all active systems: forces + velocity Verlet (per kind)
+//   k_b_force    all active systems: forces + velocity Verlet (per kind);
+//                one workgroup per tile, neighbour positions read from the
+//                tile's halo copied into LDS
 // The force and build launches sweep a static list of 64-atom tiles covering
 // all systems, so a launch is load balanced across the whole GPU whatever the
 // system sizes.  Each system's atoms, cells and lists live in its own slices
@@ -53,6 +57,7 @@ struct BSys {
   double rl2;
   float rl2f;
   double budget; // half the skin: rebuild once the displacement bound passes it
+  double margin; // largest separation of a listed pair (0: always take the minimum image)
   double dt;
   int kind, nterm, metal;
   int passes;   // digit passes its cell ids need (1 or 2)
@@ -74,7 +79,7 @@ struct BState {
   int init;           // 1: the first force (and drift) is pending, 2: running
   int overflow;       // a list outgrew cap (sticky)
   unsigned int maxcnt;        // longest list of the last build
-  unsigned int pad;
+  unsigned int halo_miss;     // a neighbour cell fell outside its tile's halo (sticky)
   unsigned long long sumcnt;  // list entries of the last build
   long builds;                // list builds so far
 };
@@ -94,8 +99,8 @@ struct BArena {
   double* rho;
   int* cnt;
   int* nbr;     // list arena (words)
-  int* cstart;  // cell arena
-  int* cend;
+  int* cb;       // cell arena: cell starts of each system (ncell + 2)
+  HaloTile* halo; // per global tile
   uint32_t* keys[2];
   uint32_t* vals[2];
   double* sv;   // [3A] reorder scratch
@@ -117,6 +122,7 @@ struct BArgs {
   const int2* chunks; // sort chunks: (system, chunk index within it)
   int nchunks;
   uint32_t* hist;     // per system: kRadix x nch digit counts, digit-major
+  int halo_on;        // tile halos are built (staging enabled)
 };
 
 // ---------------------------------------------------------------- control
@@ -254,15 +260,6 @@ static __global__ __launch_bounds__(kScanThreads) void k_b_scan(BArgs B, int pas
     h[k] = run;
     run += v;
   }
-  if (pass == 0) {
-    // cell ranges of empty cells stay [0, 0)
-    int* cs = B.R.cstart + S.c0;
-    int* ce = B.R.cend + S.c0;
-    for (int q = t; q <= S.g.ncell; q += kScanThreads) {
-      cs[q] = 0;
-      ce[q] = 0;
-    }
-  }
 }
 
 static __global__ __launch_bounds__(kBlock) void k_b_scatter(BArgs B, int pass) {
@@ -350,11 +347,17 @@ static __global__ __launch_bounds__(kBlock) void k_b_gather(BArgs B) {
     return;
   const uint32_t* K = B.R.keys[S.passes & 1] + a0;
   const uint32_t* V = B.R.vals[S.passes & 1] + a0;
-  const uint32_t key = K[k];
-  if (k == 0 || K[k - 1] != key)
-    B.R.cstart[S.c0 + key] = (int)k;
-  if (k == n - 1 || K[k + 1] != key)
-    B.R.cend[S.c0 + key] = (int)k + 1;
+  // cell starts: atom k writes the cells after the previous atom's cell up to
+  // its own; the last atom also the cells after its own
+  {
+    int* cb = B.R.cb + S.c0;
+    const long lo = k == 0 ? 0 : (long)K[k - 1] + 1;
+    for (long c = lo; c <= (long)K[k]; ++c)
+      cb[c] = (int)k;
+    if (k == n - 1)
+      for (long c = (long)K[k] + 1; c <= (long)S.g.ncell + 1; ++c)
+        cb[c] = n;
+  }
   long o = V[k];
   if (o >= n)
     o = k; // unreachable for a complete permutation; keeps every read inside the system
@@ -469,6 +472,7 @@ __device__ __forceinline__ ForceArgs force_view(const BArgs& B, int s, double* x
   A.w_rho = B.R.w + a0;
   A.maxd2 = &B.st[s].maxd2;
   A.xb = B.R.xb + 3 * a0;
+  A.margin = S.margin;
   A.P = S.P[0];
   A.terms = S.nterm > 1 ? S.P : nullptr;
   A.nterm = S.nterm;
@@ -481,6 +485,21 @@ __device__ __forceinline__ ForceArgs force_view(const BArgs& B, int s, double* x
   A.cap = S.cap;
   A.mode = st.init == 2 ? kInitDrift : kStep;
   return A;
+}
+
+// ---------------------------------------------------------------- halo
+// One thread per tile of every rebuilding system.
+static __global__ __launch_bounds__(kBlock) void k_b_halo(BArgs B, int cap) {
+  const long tl = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (tl >= B.ntiles)
+    return;
+  const int tile = B.tiles[tl];
+  const int s = B.tsys[tile];
+  const BSys& S = B.sys[s];
+  const BState& st = B.st[s];
+  if (!(st.act && st.flag))
+    return;
+  make_halo(B.R.keys[S.passes & 1] + S.a0, B.R.cb + S.c0, S.g, S.n, tile - S.a0 / kTile, cap, B.R.halo + tile);
 }
 
 // ---------------------------------------------------------------- build
@@ -497,8 +516,9 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
   A.x = B.R.x[st.par] + 3 * a0;
   A.xf = B.R.xf + a0;
   A.skeys = B.R.keys[S.passes & 1] + a0;
-  A.cstart = B.R.cstart + S.c0;
-  A.cend = B.R.cend + S.c0;
+  A.cb = B.R.cb + S.c0;
+  A.halo = B.halo_on ? B.R.halo + S.a0 / kTile : nullptr;
+  A.halo_miss = &st.halo_miss;
   A.nbr = B.R.nbr + S.l0;
   A.cnt = B.R.cnt + a0;
   A.max_count = &st.maxcnt;
@@ -523,51 +543,95 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
 }
 
 // ---------------------------------------------------------------- forces
+// One workgroup of 64 * LPA threads per tile of the launch's tile list; the
+// tile's halo (halo slots in its list) is staged in LDS (cap atoms at most).
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kBlock) void k_b_rho(BArgs B) {
-  const TileSlot ts = tile_slot<LPA>(B);
-  if (!ts.ok || !B.st[ts.s].act)
-    return; // uniform over the wavefront
-  const ForceArgs A = force_view(B, ts.s, nullptr);
-  eam_rho_atom<KIND, LPA, false>(A, ts.i, ts.l, ts.i < A.n, B.R.w + B.sys[ts.s].a0, nullptr);
+static __global__ __launch_bounds__(kWave * LPA) void k_b_rho(BArgs B, int cap) {
+  const long tl = xcd_block();
+  if (tl >= B.ntiles)
+    return;
+  const int tile = B.tiles[tl];
+  const int s = B.tsys[tile];
+  if (!B.st[s].act)
+    return; // uniform over the workgroup
+  const ForceArgs A = force_view(B, s, nullptr);
+  const long i = (long)tile * kTile - B.sys[s].a0 + threadIdx.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
+  float* w = B.R.w + B.sys[s].a0;
+  with_tile(A, B.R.halo, tile, cap, 0, nullptr, [&](bool staged, const Stage& H) {
+    if (staged)
+      eam_rho_atom<KIND, LPA, false, true>(A, H, i, l, i < A.n, w, nullptr);
+    else
+      eam_rho_atom<KIND, LPA, false, false>(A, H, i, l, i < A.n, w, nullptr);
+  });
 }
 
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kBlock) void k_b_force(BArgs B) {
-  const TileSlot ts = tile_slot<LPA>(B);
-  if (!ts.ok || !B.st[ts.s].act)
-    return; // uniform over the wavefront
-  const ForceArgs A = force_view(B, ts.s, nullptr);
-  if (B.sys[ts.s].metal)
-    eam_force_atom<KIND, LPA, false>(A, ts.i, ts.l, ts.i < A.n);
-  else
-    pair_atom<KIND, LPA, false>(A, ts.i, ts.l, ts.i < A.n);
+static __global__ __launch_bounds__(kWave * LPA) void k_b_force(BArgs B, int cap, int parts) {
+  const long tl = xcd_block();
+  if (tl >= B.ntiles)
+    return;
+  const int tile = B.tiles[tl];
+  const int s = B.tsys[tile];
+  if (!B.st[s].act)
+    return; // uniform over the workgroup
+  const ForceArgs A = force_view(B, s, nullptr);
+  const long i = (long)tile * kTile - B.sys[s].a0 + threadIdx.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
+  const bool metal = B.sys[s].metal;
+  with_tile(A, B.R.halo, tile, cap, parts, metal ? A.w_rho : nullptr, [&](bool staged, const Stage& H) {
+    if (metal) {
+      if (staged)
+        eam_force_atom<KIND, LPA, false, true>(A, H, i, l, i < A.n);
+      else
+        eam_force_atom<KIND, LPA, false, false>(A, H, i, l, i < A.n);
+    } else {
+      if (staged)
+        pair_atom<KIND, LPA, false, true>(A, H, i, l, i < A.n);
+      else
+        pair_atom<KIND, LPA, false, false>(A, H, i, l, i < A.n);
+    }
+  });
 }
 
-// Energy of one system at its user positions x[upar] (no state change): the
-// launch covers exactly that system's tiles, so every block belongs to it and
-// its block partials sum, in a fixed order, to the system's totals.
+// Energy of one system at its user positions x[upar] (no state change): one
+// workgroup per tile of that system, block partials summed in a fixed order.
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kBlock) void k_b_eval(BArgs B, int s, double* scratch_a, double* partials,
-                                                   int rho_pass) {
-  const long t = xcd_block() * kBlock + threadIdx.x;
+static __global__ __launch_bounds__(kWave * LPA) void k_b_eval(BArgs B, int s, double* scratch_a,
+                                                               double* partials, int rho_pass, int cap,
+                                                               int parts) {
   const BSys& S = B.sys[s];
-  const long i = t / LPA;
-  const int l = (int)(t % LPA);
+  const long tile = S.a0 / kTile + xcd_block();
+  const long i = (long)xcd_block() * kTile + threadIdx.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
   ForceArgs A = force_view(B, s, nullptr);
   A.x = B.R.x[B.st[s].upar] + 3 * S.a0;
   A.a = scratch_a;
   A.mode = kForcesOnly;
   A.partials = partials;
   A.maxd2 = nullptr;
-  if (S.metal) {
-    if (rho_pass)
-      eam_rho_atom<KIND, LPA, true>(A, i, l, i < A.n, B.R.w + S.a0, nullptr);
-    else
-      eam_force_atom<KIND, LPA, true>(A, i, l, i < A.n);
-  } else {
-    pair_atom<KIND, LPA, true>(A, i, l, i < A.n);
-  }
+  const int p = rho_pass ? 0 : parts;
+  with_tile(A, B.R.halo, tile, cap, p, (S.metal && !rho_pass) ? A.w_rho : nullptr,
+            [&](bool staged, const Stage& H) {
+    if (S.metal) {
+      if (rho_pass) {
+        if (staged)
+          eam_rho_atom<KIND, LPA, true, true>(A, H, i, l, i < A.n, B.R.w + S.a0, nullptr);
+        else
+          eam_rho_atom<KIND, LPA, true, false>(A, H, i, l, i < A.n, B.R.w + S.a0, nullptr);
+      } else {
+        if (staged)
+          eam_force_atom<KIND, LPA, true, true>(A, H, i, l, i < A.n);
+        else
+          eam_force_atom<KIND, LPA, true, false>(A, H, i, l, i < A.n);
+      }
+    } else {
+      if (staged)
+        pair_atom<KIND, LPA, true, true>(A, H, i, l, i < A.n);
+      else
+        pair_atom<KIND, LPA, true, false>(A, H, i, l, i < A.n);
+    }
+  });
 }
 
 } // namespace lpmdx

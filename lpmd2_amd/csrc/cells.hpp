@@ -108,6 +108,164 @@ static __global__ __launch_bounds__(kBlock) void k_cell_ranges(const uint32_t* _
     cend[key] = (int)(k + 1);
 }
 
+// cb[c] = first sorted position of cell c, for c = 0 .. ncell + 1 (ncell + 2
+// entries; cb[ncell + 1] = n): the atoms of cell c are [cb[c], cb[c + 1]),
+// empty cells included.  n + 1 threads; thread k writes the cells after the
+// previous atom's cell up to its own.
+static __global__ __launch_bounds__(kBlock) void k_cell_begin(const uint32_t* __restrict__ skeys, long n,
+                                                       int ncell, int* __restrict__ cb) {
+  const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (k > n)
+    return;
+  const long lo = k == 0 ? 0 : (long)skeys[k - 1] + 1;
+  const long hi = k == n ? (long)ncell + 1 : (long)skeys[k];
+  for (long c = lo; c <= hi; ++c)
+    cb[c] = (int)k;
+}
+
+// ---------------------------------------------------------------------------
+// Tile halos.  The atoms of a 64-atom tile (one wavefront of the build) have
+// their neighbours in the 27-cell neighbourhoods of the tile's cells; the
+// union of those cells, merged into runs of consecutive cell ids, is a few
+// contiguous ranges of the cell-sorted atoms: the tile's halo.  The force
+// kernels copy a tile's halo into LDS with coalesced loads and read neighbour
+// positions there; the list stores each neighbour as its slot in the halo
+// (slots run through the ranges in order).  Tiles whose halo is too large,
+// spans too many cell rows, or that use the reference linkedcell stencil keep
+// global atom indices (total = -1).
+constexpr int kHaloTile = 64;    // atoms per tile
+constexpr int kHaloRanges = 32;  // most ranges of one halo
+constexpr int kHaloCap = 2000;   // most atoms of one staged halo: 32 B each in LDS, with the
+                                 // kernels' static reduction scratch, within 64 KiB
+constexpr int kHaloRows = 4;     // most cell rows one staged tile may span
+
+// Tile staging is off unless LPMDX_STAGE=1 is set in the environment: on the
+// measured workloads the LDS a 64-atom tile's FP64 halo needs (1000-2000
+// atoms, 24-32 B each) costs more occupancy than the L1 gathers it saves.
+inline bool stage_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("LPMDX_STAGE");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+struct HaloTile {
+  int total; // halo atoms, or -1: global indices
+  int nr;    // ranges
+  int pad[2];
+  int4 r[kHaloRanges]; // (first cell, last cell, first sorted atom, first slot)
+};
+
+// Range of cell p in D (the first cell run holding p), or -1.
+__device__ __forceinline__ int halo_range(const HaloTile* D, int nr, int p) {
+  for (int r = 0; r < nr; ++r) {
+    const int4 q = D->r[r];
+    if (p >= q.x && p <= q.y)
+      return r;
+  }
+  return -1;
+}
+
+// Halo of tile t (atoms [64 t, 64 t + 64) of a system with n atoms sorted by
+// cell, sorted keys skeys, cell starts cb) on an exact grid.
+__device__ inline void make_halo(const uint32_t* __restrict__ skeys, const int* __restrict__ cb, const Grid& g,
+                                 long n, long t, int cap, HaloTile* __restrict__ D) {
+  const long a0 = t * kHaloTile, a1 = min(n, a0 + kHaloTile);
+  int total = -1, nr = 0;
+  int2 iv[kHaloRows * 9 * 2];
+  int m = 0;
+  bool ok = g.mode == kExact && a0 < a1;
+  int cf = 0, cl = 0;
+  if (ok) {
+    cf = (int)skeys[a0];
+    cl = (int)skeys[a1 - 1];
+    ok = cl < g.ncell && cl / g.nx - cf / g.nx + 1 <= kHaloRows;
+  }
+  if (ok) {
+    // neighbour offsets per axis that name distinct cells: {-1,0,1}, {0,1} or {0}
+    const int zy0 = g.nz >= 3 ? -1 : 0, zy1 = g.nz >= 2 ? 1 : 0;
+    const int yy0 = g.ny >= 3 ? -1 : 0, yy1 = g.ny >= 2 ? 1 : 0;
+    for (int row = cf / g.nx; row <= cl / g.nx; ++row) {
+      const int y = row % g.ny, z = row / g.ny;
+      const int xa = row == cf / g.nx ? cf % g.nx : 0;
+      const int xb = row == cl / g.nx ? cl % g.nx : g.nx - 1;
+      // x pieces of the neighbour rows: [xa-1, xb+1] around the ring, or the whole row
+      int p0 = 0, p1 = g.nx - 1, q0 = 1, q1 = 0; // second piece empty unless wrapping
+      if (g.nx >= 3 && xb - xa + 3 < g.nx) {
+        p0 = xa - 1;
+        p1 = xb + 1;
+        if (p0 < 0) {
+          q0 = 0;
+          q1 = p1;
+          p0 += g.nx;
+          p1 = g.nx - 1;
+        } else if (p1 >= g.nx) {
+          q0 = 0;
+          q1 = p1 - g.nx;
+          p1 = g.nx - 1;
+        }
+      }
+      for (int dz = zy0; dz <= zy1; ++dz) {
+        const int z2 = pmod(z + dz, g.nz);
+        for (int dy = yy0; dy <= yy1; ++dy) {
+          const int base = (z2 * g.ny + pmod(y + dy, g.ny)) * g.nx;
+          iv[m++] = make_int2(base + p0, base + p1);
+          if (q0 <= q1)
+            iv[m++] = make_int2(base + q0, base + q1);
+        }
+      }
+    }
+    // sort the cell intervals by first cell, then merge overlapping / adjacent ones
+    for (int a = 1; a < m; ++a) {
+      const int2 v = iv[a];
+      int b = a - 1;
+      while (b >= 0 && iv[b].x > v.x) {
+        iv[b + 1] = iv[b];
+        --b;
+      }
+      iv[b + 1] = v;
+    }
+    total = 0;
+    int k = 0;
+    while (k < m && ok) {
+      int lo = iv[k].x, hi = iv[k].y;
+      ++k;
+      while (k < m && iv[k].x <= hi + 1) {
+        hi = max(hi, iv[k].y);
+        ++k;
+      }
+      const int s0 = cb[lo], s1 = cb[hi + 1];
+      if (s1 > s0) {
+        if (nr == kHaloRanges) {
+          ok = false;
+          break;
+        }
+        D->r[nr++] = make_int4(lo, hi, s0, total);
+        total += s1 - s0;
+      }
+    }
+    if (!ok || total > cap)
+      total = -1;
+  }
+  D->total = ok ? total : -1;
+  D->nr = ok ? nr : 0;
+}
+
+// Halos of tiles [0, ntile) of one system; halo_max = the largest staged halo.
+static __global__ __launch_bounds__(kBlock) void k_halo(const uint32_t* __restrict__ skeys,
+                                                 const int* __restrict__ cb, Grid g, long n, long ntile,
+                                                 int cap, HaloTile* __restrict__ out,
+                                                 unsigned int* __restrict__ halo_max) {
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ntile)
+    return;
+  make_halo(skeys, cb, g, n, t, cap, out + t);
+  const int total = out[t].total;
+  if (total > 0)
+    atomicMax(halo_max, (unsigned int)total);
+}
+
 // Reorders the per-atom state into the new (cell-sorted) order.  order[k] is
 // the caller index of the atom placed at k; iperm maps caller -> old position.
 struct PermuteArgs {
@@ -161,8 +319,9 @@ struct BuildArgs {
   const double* x;
   const float4* xf; // FP32 copy of x (w unused), written by k_to_f32 before the build
   const uint32_t* skeys;
-  const int* cstart;
-  const int* cend;
+  const int* cb; // cell starts (k_cell_begin)
+  const HaloTile* halo; // per 64-atom tile, or null: global indices everywhere
+  unsigned int* halo_miss; // set when a neighbour cell lies outside its tile's halo
   int* nbr;
   int* cnt;
   unsigned int* max_count;
@@ -237,17 +396,18 @@ __device__ __forceinline__ float wrap1f(float d, float L, float h) {
   return d + w;
 }
 
-// candidates of cell p against atom i
+// candidates of cell p against atom i; entries are written as k + shift (the
+// halo slot of sorted atom k, or k itself for global indices)
 __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, const double* xi,
-                                          float sx, float sy, float sz, int& c, Chunk& b) {
-  const int lo = A.cstart[p], hi = A.cend[p];
+                                          float sx, float sy, float sz, int& c, Chunk& b, int shift) {
+  const int lo = A.cb[p], hi = A.cb[p + 1];
   if (A.path == kPathExact64) {
     for (int k = lo; k < hi; ++k) {
       double dx = A.x[3 * (long)k] - xi[0], dy = A.x[3 * (long)k + 1] - xi[1],
              dz = A.x[3 * (long)k + 2] - xi[2];
       min_image(A.b, dx, dy, dz);
       if (k != i && sqmod(dx, dy, dz) < A.rl2)
-        push(A, i, k, c, b);
+        push(A, i, k + shift, c, b);
     }
   } else if (A.path == kPathMin32) {
     for (int k = lo; k < hi; ++k) {
@@ -256,7 +416,7 @@ __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, con
       const float dy = wrap1f(q.y - sy, A.Lf[1], A.halff[1]);
       const float dz = wrap1f(q.z - sz, A.Lf[2], A.halff[2]);
       if (k != i && dx * dx + dy * dy + dz * dz < A.rl2f)
-        push(A, i, k, c, b);
+        push(A, i, k + shift, c, b);
     }
   } else {
     // (sx, sy, sz) is atom i moved by minus this cell's periodic image shift
@@ -267,17 +427,36 @@ __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, con
       const float bx = q1.x - sx, by = q1.y - sy, bz = q1.z - sz;
       const float ra = ax * ax + ay * ay + az * az, rb = bx * bx + by * by + bz * bz;
       if (k != i && ra < A.rl2f)
-        push(A, i, k, c, b);
+        push(A, i, k + shift, c, b);
       if (k + 1 != i && rb < A.rl2f)
-        push(A, i, k + 1, c, b);
+        push(A, i, k + 1 + shift, c, b);
     }
     if (k < hi) {
       const float4 q0 = A.xf[k];
       const float ax = q0.x - sx, ay = q0.y - sy, az = q0.z - sz;
       if (k != i && ax * ax + ay * ay + az * az < A.rl2f)
-        push(A, i, k, c, b);
+        push(A, i, k + shift, c, b);
     }
   }
+}
+
+// Slot shift of cell p for tile halo D (nr ranges): slot = sorted index + shift.
+// A non-empty cell outside the halo cannot happen for an exact grid; it is
+// reported (halo_miss) and the cell skipped rather than read wrongly.
+__device__ __forceinline__ bool cell_shift(const BuildArgs& A, const HaloTile* D, int nr, int p, int& shift) {
+  shift = 0;
+  if (!D)
+    return true;
+  if (A.cb[p + 1] == A.cb[p])
+    return true;
+  const int r = halo_range(D, nr, p);
+  if (r < 0) {
+    atomicOr(A.halo_miss, 1u);
+    return false;
+  }
+  const int4 q = D->r[r];
+  shift = q.w - q.z;
+  return true;
 }
 
 // List of atom i (every lane of a wavefront must call it: it ends with
@@ -288,6 +467,17 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
 #pragma unroll
   for (int u = 0; u < kChunk; ++u)
     b.v[u] = 0;
+  // the tile's halo (the wavefront serves one tile: a uniform descriptor)
+  const HaloTile* D = nullptr;
+  int nr = 0;
+  if (A.halo) {
+    const long t = __builtin_amdgcn_readfirstlane((int)(i / kHaloTile));
+    // (a wavefront past the last tile has no descriptor)
+    if (t < A.stride / kHaloTile && A.halo[t].total >= 0) {
+      D = A.halo + t;
+      nr = D->nr;
+    }
+  }
   if (active) {
     const Grid& g = A.g;
     uint32_t key = A.skeys[i];
@@ -307,7 +497,7 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
               // p == ncell only when side >= n (WrapAround's -n case): the
               // reference reads past its cell table there; treated as empty
               if (p < g.ncell)
-                scan_cell(A, (int)p, i, xi, fi.x, fi.y, fi.z, c, b);
+                scan_cell(A, (int)p, i, xi, fi.x, fi.y, fi.z, c, b, 0);
             }
           }
         }
@@ -326,7 +516,10 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
               const int z0 = pmod(rx, g.nx);
               // image of cell (z0,z1,z2) next to atom i sits at x_j + sh: compare
               // x_j against x_i - sh
-              scan_cell(A, (z2 * g.ny + z1) * g.nx + z0, i, xi, fi.x - shx, fi.y - shy, fi.z - shz, c, b);
+              const int p = (z2 * g.ny + z1) * g.nx + z0;
+              int shift;
+              if (cell_shift(A, D, nr, p, shift))
+                scan_cell(A, p, i, xi, fi.x - shx, fi.y - shy, fi.z - shz, c, b, shift);
             }
           }
         }
@@ -340,7 +533,10 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
             const int z1 = pmod(cj + dy, g.ny);
             for (int dx = ox0; dx <= ox1; ++dx) {
               const int z0 = pmod(ci + dx, g.nx);
-              scan_cell(A, (z2 * g.ny + z1) * g.nx + z0, i, xi, fi.x, fi.y, fi.z, c, b);
+              const int p = (z2 * g.ny + z1) * g.nx + z0;
+              int shift;
+              if (cell_shift(A, D, nr, p, shift))
+                scan_cell(A, p, i, xi, fi.x, fi.y, fi.z, c, b, shift);
             }
           }
         }

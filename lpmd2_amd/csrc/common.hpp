@@ -10,6 +10,7 @@ namespace lpmdx {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+constexpr int kMaxWaves = 16; // wavefronts of the largest workgroup (1024 threads)
 
 // Unit constants as the reference's potentials and properties see them:
 // forcefactor re-parsed from its 8-decimal session string
@@ -100,7 +101,7 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], dou
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int q = 0; q < kBlock / kWave; ++q)
+      for (int q = 0; q < (int)(blockDim.x / kWave); ++q)
         s += sh[q * NV + k];
       v[k] = s;
     }
@@ -114,7 +115,9 @@ struct Ctrl {
   unsigned int max_count;         // largest list length of the last build
   unsigned int pad0;
   unsigned long long sum_count;   // total list entries of the last build (8-byte aligned)
-  unsigned int pad[12];
+  unsigned int halo_max;          // largest staged tile halo of the last build
+  unsigned int halo_miss;         // a neighbour cell fell outside its tile's halo
+  unsigned int pad[10];
 };
 static_assert(offsetof(Ctrl, sum_count) % 8 == 0, "64-bit counter must be 8-byte aligned");
 static_assert(offsetof(Ctrl, sum_count) == offsetof(Ctrl, max_count) + 8, "counter block layout");

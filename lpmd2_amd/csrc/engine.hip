@@ -78,7 +78,11 @@ struct lpmdx_system {
   // sort
   DBuf<uint32_t> keys, vals, keys2, vals2, hist;
   // cells
-  DBuf<int> cstart, cend;
+  DBuf<int> cb;           // cell starts of the sorted atoms (ncell + 2)
+  DBuf<HaloTile> halo;    // per 64-atom tile: the halo its list slots refer to
+  bool staged = false;    // lists hold halo slots (tile-staged force kernels)
+  int halo_cap = 0;       // largest staged halo of the current lists
+  long ntile = 0;
   Grid grid{};
   // lists
   DBuf<int> nbr, cnt;
@@ -202,6 +206,17 @@ struct lpmdx_system {
     return cm.skin >= 0.0 ? cm.cutoff + cm.skin : cm.extcut;
   }
 
+  // Largest separation a listed pair can have at a force evaluation: the
+  // list radius padded as the FP32 build pads it, plus twice the displacement
+  // the skin test allows since the build; 0 when that reaches half an edge.
+  double wrap_margin() const {
+    const double lmax = std::max(box.L[0], std::max(box.L[1], box.L[2]));
+    const double hmin = std::min(box.half[0], std::min(box.half[1], box.half[2]));
+    const double pos_err = 2.0 * std::ldexp(lmax, -24);
+    const double m = rl * (1.0 + 1e-6) + 4.0 * pos_err + 2.0 * budget() + 1e-9 * lmax;
+    return m < hmin ? m : 0.0;
+  }
+
   // Exactness budget of the Verlet list: half the gap between the list radius
   // and the largest finite filter radius below it.
   double budget() const {
@@ -258,9 +273,18 @@ struct lpmdx_system {
     grid.ncell = (int)nc;
   }
 
+  // Tile-staged lists (halo slots, LDS-staged force kernels): exact grids,
+  // when enabled (stage_enabled).
+  bool stage_ok(int lanes) const { return stage_enabled() && grid.mode == kExact && lanes <= 16; }
+
+  long force_blocks() const { return staged ? ntile : blocks_for(n * lpa); }
+
   int auto_lanes() const {
     if (lpa_req)
       return lpa_req;
+    if (stage_ok(4))
+      return n >= 2000000 ? 4 : 8; // one 64 * lanes workgroup per tile
+
     // keep >= ~16 waves per CU on 256 CUs
     if (n >= 2000000)
       return 1;
@@ -292,11 +316,8 @@ struct lpmdx_system {
       std::swap(va, vb);
     }
     // sorted keys/order now in (ka, va)
-    cstart.alloc(grid.ncell + 1);
-    cend.alloc(grid.ncell + 1);
-    k_fill_int<<<blocks_for(grid.ncell + 1), kBlock, 0, st>>>(cstart.p, grid.ncell + 1, 0);
-    k_fill_int<<<blocks_for(grid.ncell + 1), kBlock, 0, st>>>(cend.p, grid.ncell + 1, 0);
-    k_cell_ranges<<<nb, kBlock, 0, st>>>(ka, n, cstart.p, cend.p);
+    cb.alloc(grid.ncell + 2);
+    k_cell_begin<<<blocks_for(n + 1), kBlock, 0, st>>>(ka, n, grid.ncell, cb.p);
     PermuteArgs P{va, iperm.p, x.p, v.p, a.p, z.p, ffm.p, mass.p, x2.p, v2.p, a2.p, z2.p,
                   ffm2.p, mass2.p, perm2.p, iperm2.p, n};
     k_permute<<<nb, kBlock, 0, st>>>(P);
@@ -320,8 +341,23 @@ struct lpmdx_system {
     // list storage: cap entries per atom, rounded to the lane count
     lpa = auto_lanes();
     stride = ((n + kWave - 1) / kWave) * kWave; // whole tiles for every lane count
-    partials.alloc((size_t)blocks_for(n * 16) * 12);
-    partials2.alloc((size_t)blocks_for(n * 16) * 12);
+    ntile = stride / kHaloTile;
+    staged = stage_ok(lpa);
+    const size_t np = (size_t)std::max<long>(blocks_for(n * 16), ntile) * 12;
+    partials.alloc(np);
+    partials2.alloc(np);
+    // halo_max, halo_miss
+    hipchk(hipMemsetAsync(&ctrl.p->halo_max, 0, 8, st), "memset");
+    if (staged) {
+      halo.alloc(ntile);
+      static const int cap_env = [] {
+        const char* e = getenv("LPMDX_HALO_CAP");
+        return e ? std::max(64, std::min(atoi(e), kHaloCap)) : kHaloCap;
+      }();
+      k_halo<<<blocks_for(ntile), kBlock, 0, st>>>(keys.p, cb.p, grid, n, ntile, cap_env, halo.p,
+                                                   &ctrl.p->halo_max);
+      launch_check();
+    }
     if (cap == 0) {
       // initial capacity from the density inside the list radius
       double vol = box.L[0] * box.L[1] * box.L[2];
@@ -358,8 +394,9 @@ struct lpmdx_system {
       B.x = x.p;
       B.xf = xf.p;
       B.skeys = keys.p;
-      B.cstart = cstart.p;
-      B.cend = cend.p;
+      B.cb = cb.p;
+      B.halo = staged ? halo.p : nullptr;
+      B.halo_miss = &ctrl.p->halo_miss;
       B.nbr = nbr.p;
       B.cnt = cnt.p;
       B.max_count = &ctrl.p->max_count;
@@ -381,9 +418,13 @@ struct lpmdx_system {
       k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
       prof_end(kProfBuild, e0);
       launch_check();
-      hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 16, hipMemcpyDeviceToHost, st),
+      // max_count, pad0, sum_count, halo_max, halo_miss: 24 contiguous bytes
+      hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 24, hipMemcpyDeviceToHost, st),
              "memcpy");
       sync();
+      if (hctrl->halo_miss)
+        fail(LPMDX_EDEVICE, "neighbour cell outside its tile halo (internal error)");
+      halo_cap = staged ? std::min<int>((int)(((hctrl->halo_max + 7) / 8) * 8), kHaloCap) : 0;
       const long mc = hctrl->max_count;
       max_list = mc;
       sum_list = (long)hctrl->sum_count;
@@ -466,6 +507,21 @@ struct lpmdx_system {
   // ------------------------------------------------------------------ forces
   template <int KIND, bool EVAL> void launch_pair_k(ForceArgs& A, long nb) {
     hipEvent_t e0 = prof_begin();
+    if (staged) {
+      const int parts = (A.nterm > 1 || A.P.spc_sum >= 0) ? kStageS : 0;
+      const size_t lds = stage_bytes(halo_cap, parts);
+      switch (lpa) {
+      case 1: k_pair_force_st<KIND, 1, EVAL><<<ntile, kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
+      case 2: k_pair_force_st<KIND, 2, EVAL><<<ntile, 2 * kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
+      case 4: k_pair_force_st<KIND, 4, EVAL><<<ntile, 4 * kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
+      case 8: k_pair_force_st<KIND, 8, EVAL><<<ntile, 8 * kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
+      default:
+        k_pair_force_st<KIND, 16, EVAL><<<ntile, 16 * kWave, lds, st>>>(A, halo.p, halo_cap, parts);
+        break;
+      }
+      prof_end(kProfForce, e0);
+      return;
+    }
     switch (lpa) {
     case 1: k_pair_force<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(A); break;
     case 2: k_pair_force<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(A); break;
@@ -480,6 +536,39 @@ struct lpmdx_system {
     R.P.rc2 = rc2_rho;
     R.partials = partials2.p;
     hipEvent_t e0 = prof_begin();
+    if (staged) {
+      const int parts = kStageW | (A.P.spc_sum >= 0 ? kStageS : 0);
+      const size_t lr = stage_bytes(halo_cap, 0), lf = stage_bytes(halo_cap, parts);
+      switch (lpa) {
+      case 1: k_eam_rho_st<KIND, 1, EVAL><<<ntile, kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p); break;
+      case 2:
+        k_eam_rho_st<KIND, 2, EVAL><<<ntile, 2 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
+        break;
+      case 4:
+        k_eam_rho_st<KIND, 4, EVAL><<<ntile, 4 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
+        break;
+      case 8:
+        k_eam_rho_st<KIND, 8, EVAL><<<ntile, 8 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
+        break;
+      default:
+        k_eam_rho_st<KIND, 16, EVAL><<<ntile, 16 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
+        break;
+      }
+      prof_end(kProfRho, e0);
+      A.w_rho = w_rho.p;
+      hipEvent_t e1 = prof_begin();
+      switch (lpa) {
+      case 1: k_eam_force_st<KIND, 1, EVAL><<<ntile, kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
+      case 2: k_eam_force_st<KIND, 2, EVAL><<<ntile, 2 * kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
+      case 4: k_eam_force_st<KIND, 4, EVAL><<<ntile, 4 * kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
+      case 8: k_eam_force_st<KIND, 8, EVAL><<<ntile, 8 * kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
+      default:
+        k_eam_force_st<KIND, 16, EVAL><<<ntile, 16 * kWave, lf, st>>>(A, halo.p, halo_cap, parts);
+        break;
+      }
+      prof_end(kProfForce, e1);
+      return;
+    }
     switch (lpa) {
     case 1: k_eam_rho<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
     case 2: k_eam_rho<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
@@ -536,7 +625,7 @@ struct lpmdx_system {
   void forces(int mode, bool eval, double dt, unsigned int* maxd2) {
     if (pots.empty())
       fail(LPMDX_ESTATE, "no potential set");
-    const long nb = blocks_for(n * lpa);
+    const long nb = force_blocks();
     const bool fused = pots.size() == 1;
     for (size_t q = 0; q < pots.size(); ++q) {
       const PotParams& P = *pots[q];
@@ -554,6 +643,7 @@ struct lpmdx_system {
       A.partials = partials.p;
       A.maxd2 = maxd2;
       A.xb = xb.p;
+      A.margin = wrap_margin();
       A.P = device_view(P, rcf, rcr);
       A.b = box;
       for (int k = 0; k < 3; ++k)
@@ -642,7 +732,7 @@ struct lpmdx_system {
     const PotParams& P = *pots[k];
     double rcf, rcr;
     filters(P, rcf, rcr);
-    const long nb = blocks_for(n * lpa);
+    const long nb = force_blocks();
     asum.alloc(3 * n);
     ForceArgs A{};
     A.x = x.p;
@@ -656,6 +746,7 @@ struct lpmdx_system {
     A.partials = partials.p;
     A.maxd2 = ctrl.p->maxd2;
     A.xb = xb.p;
+    A.margin = wrap_margin();
     A.P = device_view(P, rcf, rcr);
     A.b = box;
     for (int q = 0; q < 3; ++q)
@@ -1262,7 +1353,21 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
     hipchk(hipMemcpyAsync(cntv.data(), S->cnt.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
     hipchk(hipMemcpyAsync(nb.data(), S->nbr.p, nb.size() * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
     hipchk(hipMemcpyAsync(x.data(), S->x.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, S->st), "memcpy");
+    std::vector<HaloTile> ht(S->staged ? S->ntile : 0);
+    if (S->staged)
+      hipchk(hipMemcpyAsync(ht.data(), S->halo.p, ht.size() * sizeof(HaloTile), hipMemcpyDeviceToHost, S->st),
+             "memcpy");
     S->sync();
+    // list entries of staged tiles are halo slots: back to sorted atom indices
+    auto atom_of = [&](long i, long e) -> long {
+      if (!S->staged || ht[i / kHaloTile].total < 0)
+        return e;
+      const HaloTile& D = ht[i / kHaloTile];
+      int r = 0;
+      while (r + 1 < D.nr && D.r[r + 1].w <= e)
+        ++r;
+      return (long)D.r[r].z + (e - D.r[r].w);
+    };
     std::vector<long> iperm(n);
     for (long i = 0; i < n; ++i)
       iperm[perm[i]] = i;
@@ -1273,7 +1378,7 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
       const long i = iperm[o];
       row.clear();
       for (int c = 0; c < cntv[i]; ++c) {
-        const long jj = nb[list_index(i, c, S->lpa, S->cap)];
+        const long jj = atom_of(i, nb[list_index(i, c, S->lpa, S->cap)]);
         const long oj = perm[jj];
         if (!full && S->cm.kind == kCmLinkedCell && oj > o)
           continue; // linkedcell half list keeps j < i (linkedcell.cc:323)

@@ -181,6 +181,10 @@ struct ForceArgs {
   double* partials;     // [nblocks * 12] when evaluating energies
   unsigned int* maxd2;  // max over atoms of |x_next - xb|^2 (float bits), or null
   const double* xb;     // [3n] positions at the last list build
+  // Largest separation a listed pair can have now (list radius + padding +
+  // twice the displacement allowed since the build), or 0.  An atom at least
+  // this far from every face has no listed neighbour across the boundary.
+  double margin;
   DevPot P;
   const DevPot* terms; // several potentials of one kind on one list (species pairs), or null
   int nterm;
@@ -288,7 +292,7 @@ template <int LPA> __device__ __forceinline__ double group_sum(double v) {
 }
 
 __device__ __forceinline__ void write_partials(const ForceArgs& A, Tally& t, double extra) {
-  __shared__ double sh[9 * (kBlock / kWave)];
+  __shared__ double sh[9 * kMaxWaves]; // up to 1024-thread workgroups
   double v[9] = {t.e, extra, t.w, t.s[0], t.s[1], t.s[2], t.s[3], t.s[4], t.s[5]};
   block_sum<9>(v, sh);
   if (threadIdx.x == 0) {
@@ -315,49 +319,166 @@ __device__ __forceinline__ void write_partials(const ForceArgs& A, Tally& t, dou
 // body(j, dx, dy, dz, r2) sees the FP64 minimum-image displacement.
 constexpr int kUnroll = 4;
 
-template <int LPA, typename Body>
-__device__ __forceinline__ void for_neighbours(const ForceArgs& A, long i, int l, double xi,
-                                               double yi, double zi, Body&& body) {
+// LDS copy of a tile's halo (slots as in the tile's list); W and S are null
+// unless the kernel reads neighbour rho^{-1/2} or species.
+struct Stage {
+  const double* X;
+  const double* Y;
+  const double* Z;
+  const float* W;
+  const int* S;
+};
+
+// Copies tile D's halo into LDS.  The descriptor's ranges go to LDS first;
+// then every thread takes slots threadIdx.x + u * blockDim.x, finds each
+// slot's range there and issues the loads of up to kStageBatch slots before
+// storing any, so one memory latency covers the whole halo.
+constexpr int kStageBatch = 8;
+
+__device__ __forceinline__ void stage_halo(const ForceArgs& A, const HaloTile* D, double* X, double* Y, double* Z,
+                                           float* W, const float* wsrc, int* S) {
+  __shared__ int4 sr[kHaloRanges];
+  const int nr = D->nr, total = D->total;
+  if ((int)threadIdx.x < nr)
+    sr[threadIdx.x] = D->r[threadIdx.x];
+  __syncthreads();
+  int r = 0; // ranges only move forward as the slot grows
+  for (int base = threadIdx.x; base < total; base += kStageBatch * blockDim.x) {
+    long k[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int slot = base + u * blockDim.x;
+      k[u] = -1;
+      if (slot < total) {
+        while (r + 1 < nr && sr[r + 1].w <= slot)
+          ++r;
+        k[u] = (long)sr[r].z + (slot - sr[r].w);
+      }
+    }
+    double vx[kStageBatch], vy[kStageBatch], vz[kStageBatch];
+    float vw[kStageBatch];
+    int vs[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      if (k[u] >= 0) {
+        vx[u] = A.x[3 * k[u]];
+        vy[u] = A.x[3 * k[u] + 1];
+        vz[u] = A.x[3 * k[u] + 2];
+        vw[u] = W ? wsrc[k[u]] : 0.0f;
+        vs[u] = S ? A.z[k[u]] : 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int slot = base + u * blockDim.x;
+      if (k[u] >= 0) {
+        X[slot] = vx[u];
+        Y[slot] = vy[u];
+        Z[slot] = vz[u];
+        if (W)
+          W[slot] = vw[u];
+        if (S)
+          S[slot] = vs[u];
+      }
+    }
+  }
+}
+
+template <int LPA>
+__device__ __forceinline__ void list_batch(const int* __restrict__ nb, int k0, int c, int pad,
+                                           int (&js)[kUnroll]) {
+  constexpr int T = kWave / LPA;
+  if constexpr (LPA == 1 && kUnroll == 4) {
+    // k0 is a multiple of 4: entries k0..k0+3 are one aligned 16-byte word
+    // group of the chunk
+    const int4 q = *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
+    js[0] = k0 < c ? q.x : pad;
+    js[1] = k0 + 1 < c ? q.y : pad;
+    js[2] = k0 + 2 < c ? q.z : pad;
+    js[3] = k0 + 3 < c ? q.w : pad;
+  } else {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = k0 + u * LPA;
+      js[u] = k < c ? nb[(long)(k / kChunk) * (T * kChunk) + (k % kChunk)] : pad;
+    }
+  }
+}
+
+template <int LPA, bool WRAP, bool STAGED, typename Body>
+__device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& H, long i, int l, double xi,
+                                               double yi, double zi, bool need_s, bool need_w, Body& body) {
   // a list that outgrew its capacity (reported as an error) is read no further
   // than the capacity, so no entry outside the atom's chunks is ever used
   const int c = min(A.cnt[i], A.cap);
   constexpr int T = kWave / LPA;
+  constexpr int step = kUnroll * LPA;
   // entry k of this atom: chunk k/8 of the tile row, word k%8 (see list_index)
   const int* __restrict__ nb = A.nbr + (i / T) * (long)T * A.cap + (i % T) * kChunk;
-  for (int k0 = l; k0 < c; k0 += kUnroll * LPA) {
+  const int pad = STAGED ? 0 : (int)i;
+  // staged: the next batch's entries are loaded while this batch is
+  // evaluated (few workgroups per CU hide little latency otherwise)
+  int jn[kUnroll];
+  if (STAGED && l < c)
+    list_batch<LPA>(nb, l, c, pad, jn);
+  for (int k0 = l; k0 < c; k0 += step) {
     int js[kUnroll];
-    double px[kUnroll], py[kUnroll], pz[kUnroll];
-    if constexpr (LPA == 1 && kUnroll == 4) {
-      // k0 is a multiple of 4: entries k0..k0+3 are one aligned 16-byte word
-      // group of the chunk; slots past the list end are replaced by the atom itself
-      const int4 q = *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
-      js[0] = q.x;
-      js[1] = k0 + 1 < c ? q.y : (int)i;
-      js[2] = k0 + 2 < c ? q.z : (int)i;
-      js[3] = k0 + 3 < c ? q.w : (int)i;
-    } else {
+    if constexpr (STAGED) {
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int k = k0 + u * LPA;
-        js[u] = k < c ? nb[(long)(k / kChunk) * (T * kChunk) + (k % kChunk)] : (int)i;
+      for (int u = 0; u < kUnroll; ++u)
+        js[u] = jn[u];
+      if (k0 + step < c)
+        list_batch<LPA>(nb, k0 + step, c, pad, jn);
+    } else {
+      list_batch<LPA>(nb, k0, c, pad, js);
+    }
+    double px[kUnroll], py[kUnroll], pz[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if constexpr (STAGED) {
+        px[u] = H.X[js[u]];
+        py[u] = H.Y[js[u]];
+        pz[u] = H.Z[js[u]];
+      } else {
+        const double* p = A.x + 3 * (long)js[u];
+        px[u] = p[0];
+        py[u] = p[1];
+        pz[u] = p[2];
       }
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const double* p = A.x + 3 * (long)js[u];
-      px[u] = p[0];
-      py[u] = p[1];
-      pz[u] = p[2];
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+      const int j = js[u];
+      int sj = 0;
+      float wj = 0.0f;
+      if (need_s)
+        sj = STAGED ? H.S[j] : A.z[j];
+      if (need_w)
+        wj = STAGED ? H.W[j] : A.w_rho[j];
       double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
-      min_image(A.b, dx, dy, dz);
-      // entries past the list end point at the atom itself (r2 = 0); the body
-      // evaluates every slot and masks its contribution with `valid`
-      body(k0 + u * LPA < c, js[u], dx, dy, dz, sqmod(dx, dy, dz));
+      if constexpr (WRAP)
+        min_image(A.b, dx, dy, dz);
+      body(k0 + u * LPA < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
     }
   }
+}
+
+// Runs body(valid, s_j, w_j, dx, dy, dz, r2) over atom i's list, with
+// neighbour species s_j (need_s) and rho^{-1/2} w_j (need_w).  Where no atom
+// of the wavefront is within A.margin of a face, no displacement can reach
+// half a box edge, the minimum image is the identity
+// (OrthogonalCell::Displacement would not wrap) and the loop skips it; the
+// wavefront takes one variant, so the branch is uniform.
+template <int LPA, bool STAGED, typename Body>
+__device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& H, long i, int l, double xi,
+                                               double yi, double zi, bool need_s, bool need_w, Body&& body) {
+  const double m = A.margin;
+  const bool inner = m > 0.0 && xi >= m && xi <= A.b.L[0] - m && yi >= m && yi <= A.b.L[1] - m && zi >= m &&
+                     zi <= A.b.L[2] - m;
+  if (__all(inner))
+    neighbour_loop<LPA, false, STAGED>(A, H, i, l, xi, yi, zi, need_s, need_w, body);
+  else
+    neighbour_loop<LPA, true, STAGED>(A, H, i, l, xi, yi, zi, need_s, need_w, body);
 }
 
 // Per-atom force: FP32 pair scalar times the FP64 displacement, summed in FP64
@@ -399,17 +520,18 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
 
 // Per-atom bodies of the force kernels; lane l of the LPA lanes serving atom i.
 // Every lane of a wavefront must call them (they end with wavefront reductions).
-template <int KIND, int LPA, bool EVAL>
-__device__ __forceinline__ void pair_atom(const ForceArgs& A, long i, int l, bool active) {
+template <int KIND, int LPA, bool EVAL, bool STAGED = false>
+__device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active) {
   ForceSum F;
   Tally tal;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const int si = A.z[i];
     const bool need_z = A.nterm > 1 || A.P.spc_sum >= 0;
-    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int j, double dx, double dy, double dz, double r2) {
+    for_neighbours<LPA, STAGED>(A, H, i, l, xi, yi, zi, need_z, false,
+                                [&](bool valid, int sj, float, double dx, double dy, double dz, double r2) {
       float fs, e;
-      pair_terms<KIND>(A, valid, si, need_z ? A.z[j] : 0, r2, fs, e);
+      pair_terms<KIND>(A, valid, si, sj, r2, fs, e);
       F.add(fs, dx, dy, dz);
       if constexpr (EVAL)
         tal.add(fs, (float)dx, (float)dy, (float)dz, (float)r2, e);
@@ -430,14 +552,15 @@ __device__ __forceinline__ void pair_atom(const ForceArgs& A, long i, int l, boo
 // check, metalpotential.cc:121-132), accumulated in FP64; writes
 // w_i = rho_i^{-1/2} (FP32) and rho_i, and, when evaluating, the embedding
 // energy F(rho_i) = -K sqrt(rho_i) into the block partials.
-template <int KIND, int LPA, bool EVAL>
-__device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, long i, int l, bool active,
+template <int KIND, int LPA, bool EVAL, bool STAGED = false>
+__device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                              float* w_out, double* rho_out) {
   double rho = 0.0;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int, double, double, double, double r2) {
+    for_neighbours<LPA, STAGED>(A, H, i, l, xi, yi, zi, false, false,
+                                [&](bool valid, int, float, double, double, double, double r2) {
       const bool in = valid && r2 < rc2;
       const float v = eam_rho<KIND>(A.P, in ? (float)r2 : 1.0f);
       rho += in ? (double)v : 0.0;
@@ -459,8 +582,8 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, long i, int l, 
 
 // EAM pass 2: pair + many-body forces over the full list with AppliesTo
 // (metalpotential.cc:140-169), fused with the integrator tail.
-template <int KIND, int LPA, bool EVAL>
-__device__ __forceinline__ void eam_force_atom(const ForceArgs& A, long i, int l, bool active) {
+template <int KIND, int LPA, bool EVAL, bool STAGED = false>
+__device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active) {
   ForceSum F;
   Tally tal;
   if (active) {
@@ -469,11 +592,12 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, long i, int l
     const float wi = A.w_rho[i];
     const bool all = A.P.spc_sum < 0;
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA>(A, i, l, xi, yi, zi, [&](bool valid, int j, double dx, double dy, double dz, double r2) {
-      const bool in = valid && r2 < rc2 && (all || si + A.z[j] == A.P.spc_sum);
+    for_neighbours<LPA, STAGED>(A, H, i, l, xi, yi, zi, !all, true,
+                                [&](bool valid, int sj, float wj, double dx, double dy, double dz, double r2) {
+      const bool in = valid && r2 < rc2 && (all || si + sj == A.P.spc_sum);
       const float fr2 = in ? (float)r2 : 1.0f;
       float fs, V;
-      eam_pair<KIND>(A.P, fr2, wi + A.w_rho[j], fs, V);
+      eam_pair<KIND>(A.P, fr2, wi + wj, fs, V);
       fs = in ? fs : 0.0f;
       F.add(fs, dx, dy, dz);
       if constexpr (EVAL)
@@ -494,19 +618,110 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, long i, int l
 template <int KIND, int LPA, bool EVAL>
 static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
   const long t = xcd_block() * kBlock + threadIdx.x;
-  pair_atom<KIND, LPA, EVAL>(A, t / LPA, (int)(t % LPA), t / LPA < A.n);
+  pair_atom<KIND, LPA, EVAL>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n);
 }
 
 template <int KIND, int LPA, bool EVAL>
 static __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
   const long t = xcd_block() * kBlock + threadIdx.x;
-  eam_rho_atom<KIND, LPA, EVAL>(A, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out, rho_out);
+  eam_rho_atom<KIND, LPA, EVAL>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out, rho_out);
 }
 
 template <int KIND, int LPA, bool EVAL>
 static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
   const long t = xcd_block() * kBlock + threadIdx.x;
-  eam_force_atom<KIND, LPA, EVAL>(A, t / LPA, (int)(t % LPA), t / LPA < A.n);
+  eam_force_atom<KIND, LPA, EVAL>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n);
+}
+
+// ---- tile-staged kernels: one workgroup of 64 * LPA threads per 64-atom
+// tile; a tile whose list holds halo slots first copies its halo into LDS
+// (cap = the largest halo of the launch; dynamic LDS of cap * (24 + 4 * [W]
+// + 4 * [S]) bytes), a tile with global indices reads global memory.
+enum StageParts { kStageW = 1, kStageS = 2 };
+
+struct StageLds {
+  double* X;
+  double* Y;
+  double* Z;
+  float* W;
+  int* S;
+};
+
+__device__ __forceinline__ StageLds stage_lds(int cap, int parts) {
+  extern __shared__ double lds_dyn[];
+  StageLds L;
+  L.X = lds_dyn;
+  L.Y = L.X + cap;
+  L.Z = L.Y + cap;
+  char* rest = reinterpret_cast<char*>(L.Z + cap);
+  L.W = (parts & kStageW) ? reinterpret_cast<float*>(rest) : nullptr;
+  if (parts & kStageW)
+    rest += sizeof(float) * (size_t)cap;
+  L.S = (parts & kStageS) ? reinterpret_cast<int*>(rest) : nullptr;
+  return L;
+}
+
+__host__ __device__ inline size_t stage_bytes(int cap, int parts) {
+  return (size_t)cap * (3 * sizeof(double) + ((parts & kStageW) ? sizeof(float) : 0) +
+                        ((parts & kStageS) ? sizeof(int) : 0));
+}
+
+// Runs fn(staged, Stage) for this block's tile after staging its halo.
+template <typename F>
+__device__ __forceinline__ void with_tile(const ForceArgs& A, const HaloTile* halo, long tile, int cap, int parts,
+                                          const float* wsrc, F&& fn) {
+  const HaloTile* D = halo + tile;
+  const bool staged = cap > 0 && D->total >= 0; // uniform over the block (cap 0: staging off)
+  if (staged) {
+    StageLds L = stage_lds(cap, parts);
+    stage_halo(A, D, L.X, L.Y, L.Z, L.W, wsrc, L.S);
+    __syncthreads();
+    fn(true, Stage{L.X, L.Y, L.Z, L.W, L.S});
+  } else {
+    fn(false, Stage{});
+  }
+}
+
+template <int KIND, int LPA, bool EVAL>
+static __global__ __launch_bounds__(kWave * LPA) void k_pair_force_st(ForceArgs A, const HaloTile* halo, int cap,
+                                                                      int parts) {
+  const long tile = xcd_block();
+  const long i = tile * kHaloTile + threadIdx.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
+  with_tile(A, halo, tile, cap, parts, nullptr, [&](bool staged, const Stage& H) {
+    if (staged)
+      pair_atom<KIND, LPA, EVAL, true>(A, H, i, l, i < A.n);
+    else
+      pair_atom<KIND, LPA, EVAL, false>(A, H, i, l, i < A.n);
+  });
+}
+
+template <int KIND, int LPA, bool EVAL>
+static __global__ __launch_bounds__(kWave * LPA) void k_eam_rho_st(ForceArgs A, const HaloTile* halo, int cap,
+                                                                   float* w_out, double* rho_out) {
+  const long tile = xcd_block();
+  const long i = tile * kHaloTile + threadIdx.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
+  with_tile(A, halo, tile, cap, 0, nullptr, [&](bool staged, const Stage& H) {
+    if (staged)
+      eam_rho_atom<KIND, LPA, EVAL, true>(A, H, i, l, i < A.n, w_out, rho_out);
+    else
+      eam_rho_atom<KIND, LPA, EVAL, false>(A, H, i, l, i < A.n, w_out, rho_out);
+  });
+}
+
+template <int KIND, int LPA, bool EVAL>
+static __global__ __launch_bounds__(kWave * LPA) void k_eam_force_st(ForceArgs A, const HaloTile* halo, int cap,
+                                                                     int parts) {
+  const long tile = xcd_block();
+  const long i = tile * kHaloTile + threadIdx.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
+  with_tile(A, halo, tile, cap, parts, A.w_rho, [&](bool staged, const Stage& H) {
+    if (staged)
+      eam_force_atom<KIND, LPA, EVAL, true>(A, H, i, l, i < A.n);
+    else
+      eam_force_atom<KIND, LPA, EVAL, false>(A, H, i, l, i < A.n);
+  });
 }
 
 // Velocity Verlet with precomputed accelerations (several potentials):
