@@ -285,12 +285,13 @@ struct lpmdx_system {
     if (stage_ok(4))
       return n >= 2000000 ? 4 : 8; // one 64 * lanes workgroup per tile
 
-    // keep >= ~16 waves per CU on 256 CUs
-    if (n >= 2000000)
-      return 1;
+    // fewer lanes per atom issue fewer instructions per neighbour; small
+    // systems need more lanes to fill 256 CUs
     if (n >= 500000)
+      return 1;
+    if (n >= 60000)
       return 2;
-    if (n >= 120000)
+    if (n >= 15000)
       return 4;
     return 8;
   }

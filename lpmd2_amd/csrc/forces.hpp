@@ -313,9 +313,9 @@ __device__ __forceinline__ void write_partials(const ForceArgs& A, Tally& t, dou
   }
 }
 
-// Neighbour loop shared by the force kernels: lane l of atom i visits entries
-// k = l, l + LPA, ... of its list in batches of kUnroll, issuing the index and
-// position loads of a batch before any arithmetic, so the gathers overlap.
+// Neighbour loop shared by the force kernels: lane l of atom i visits the
+// 4-entry word groups l, l + LPA, ... of its list, issuing the index and
+// position loads of a group before any arithmetic, so the gathers overlap.
 // body(j, dx, dy, dz, r2) sees the FP64 minimum-image displacement.
 constexpr int kUnroll = 4;
 
@@ -384,25 +384,20 @@ __device__ __forceinline__ void stage_halo(const ForceArgs& A, const HaloTile* D
   }
 }
 
+// The list entries of one batch: kUnroll consecutive entries k0 .. k0+3 of
+// this lane (k0 a multiple of 4, so they are one aligned 16-byte word group
+// of a chunk); slots past the list end read a valid place (the atom itself,
+// or halo slot 0) and are masked by `valid`.
 template <int LPA>
 __device__ __forceinline__ void list_batch(const int* __restrict__ nb, int k0, int c, int pad,
                                            int (&js)[kUnroll]) {
+  static_assert(kUnroll == 4 && kChunk % kUnroll == 0, "one int4 per batch");
   constexpr int T = kWave / LPA;
-  if constexpr (LPA == 1 && kUnroll == 4) {
-    // k0 is a multiple of 4: entries k0..k0+3 are one aligned 16-byte word
-    // group of the chunk
-    const int4 q = *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
-    js[0] = k0 < c ? q.x : pad;
-    js[1] = k0 + 1 < c ? q.y : pad;
-    js[2] = k0 + 2 < c ? q.z : pad;
-    js[3] = k0 + 3 < c ? q.w : pad;
-  } else {
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int k = k0 + u * LPA;
-      js[u] = k < c ? nb[(long)(k / kChunk) * (T * kChunk) + (k % kChunk)] : pad;
-    }
-  }
+  const int4 q = *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
+  js[0] = k0 < c ? q.x : pad;
+  js[1] = k0 + 1 < c ? q.y : pad;
+  js[2] = k0 + 2 < c ? q.z : pad;
+  js[3] = k0 + 3 < c ? q.w : pad;
 }
 
 template <int LPA, bool WRAP, bool STAGED, typename Body>
@@ -412,6 +407,7 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
   // than the capacity, so no entry outside the atom's chunks is ever used
   const int c = min(A.cnt[i], A.cap);
   constexpr int T = kWave / LPA;
+  // lane l takes the 4-entry word groups l, l + LPA, l + 2 LPA, ... of the list
   constexpr int step = kUnroll * LPA;
   // entry k of this atom: chunk k/8 of the tile row, word k%8 (see list_index)
   const int* __restrict__ nb = A.nbr + (i / T) * (long)T * A.cap + (i % T) * kChunk;
@@ -419,9 +415,9 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
   // staged: the next batch's entries are loaded while this batch is
   // evaluated (few workgroups per CU hide little latency otherwise)
   int jn[kUnroll];
-  if (STAGED && l < c)
-    list_batch<LPA>(nb, l, c, pad, jn);
-  for (int k0 = l; k0 < c; k0 += step) {
+  if (STAGED && kUnroll * l < c)
+    list_batch<LPA>(nb, kUnroll * l, c, pad, jn);
+  for (int k0 = kUnroll * l; k0 < c; k0 += step) {
     int js[kUnroll];
     if constexpr (STAGED) {
 #pragma unroll
@@ -458,7 +454,7 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
       double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
       if constexpr (WRAP)
         min_image(A.b, dx, dy, dz);
-      body(k0 + u * LPA < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
+      body(k0 + u < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
     }
   }
 }
