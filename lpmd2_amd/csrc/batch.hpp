@@ -504,6 +504,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_halo(BArgs B, int cap) {
 
 // ---------------------------------------------------------------- build
 static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
+  __shared__ int ring[kRing * kBlock];
   const TileSlot ts = tile_slot<1>(B);
   if (!ts.ok)
     return;
@@ -536,7 +537,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
   A.cap = S.cap;
   A.lpa = S.lpa;
   A.path = S.path;
-  build_atom(A, ts.i, ts.i < S.n);
+  build_atom(A, ts.i, ts.i < S.n, ring);
   // each lane checks the count it has just written itself
   if (ts.i < S.n && A.cnt[ts.i] > S.cap)
     st.overflow = 1;

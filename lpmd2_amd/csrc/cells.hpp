@@ -14,8 +14,8 @@
 // atoms one wavefront serves; each atom's entries are stored in chunks of 8
 // consecutive words (32 bytes, one L2 sector):
 //   entry k of atom i at  tile(i) * T * cap + (k / 8) * T * 8 + (i mod T) * 8 + (k mod 8).
-// The build fills a chunk in registers and writes it whole, so no sector is
-// ever written partially; the lpa lanes serving an atom read consecutive words
+// The build gathers a chunk in LDS and writes it whole, so no sector is ever
+// written partially; the lpa lanes serving an atom read consecutive words
 // of its chunk, and a tile's list is one contiguous block.
 #pragma once
 
@@ -366,29 +366,56 @@ static __global__ __launch_bounds__(kBlock) void k_to_f32(const double* __restri
   xf[i] = make_float4(w[0], w[1], w[2], 0.0f);
 }
 
-// One atom's pending chunk of list entries, held in registers until it is full.
-struct Chunk {
-  int v[kChunk];
+// One atom's list being written.  Accepted entries wait in a ring of kRing
+// slots in LDS (the thread's own column of a kRing x kBlock array, so a
+// wavefront's writes hit 64 distinct banks) and leave for HBM a whole chunk
+// (two 16-byte stores) at a time.  A candidate is written to the ring slot
+// after the last entry whether or not it is accepted (acceptance only advances
+// the count), so the candidate loop has no branch per candidate.  drain() runs
+// after every kGroup candidates: with fewer than kChunk entries pending after
+// it and at most kGroup accepted before the next, at most kRing - 1 entries are
+// ever pending and the ring never overwrites one.
+constexpr int kRing = 16;
+constexpr int kGroup = 8;
+static_assert(kRing >= kChunk + kGroup - 1 && (kRing & (kRing - 1)) == 0, "ring too small");
+
+struct Pending {
+  int* col;    // entry slot u at col[u * kBlock]
+  long base;   // list_index(i, 0, lpa, cap): the atom's first chunk
+  long cstep;  // distance between the atom's consecutive chunks
+  int c;       // entries so far
+  int f;       // entries already stored (a multiple of kChunk)
 };
 
-__device__ __forceinline__ void flush_chunk(const BuildArgs& A, long i, int c, const Chunk& b) {
-  // c entries so far; the chunk holding entry c-1 starts at entry (c-1) & ~7
-  const int k0 = (c - 1) & ~(kChunk - 1);
-  if (k0 < A.cap) {
-    int4* dst = reinterpret_cast<int4*>(A.nbr + list_index(i, k0, A.lpa, A.cap));
-    dst[0] = make_int4(b.v[0], b.v[1], b.v[2], b.v[3]);
-    dst[1] = make_int4(b.v[4], b.v[5], b.v[6], b.v[7]);
-  }
+__device__ __forceinline__ Pending make_pending(const BuildArgs& A, long i, int* ring) {
+  Pending P;
+  P.col = ring + threadIdx.x;
+  // lpa is a power of two, so T = kWave / lpa and i / T is a shift
+  const int sh = __builtin_ctz(kWave / A.lpa);
+  const long T = 1L << sh;
+  P.base = (i >> sh) * T * A.cap + (i & (T - 1)) * kChunk;
+  P.cstep = T * kChunk;
+  P.c = 0;
+  P.f = 0;
+  return P;
 }
 
-__device__ __forceinline__ void push(const BuildArgs& A, long i, int k, int& c, Chunk& b) {
-  const int slot = c & (kChunk - 1);
-#pragma unroll
-  for (int u = 0; u < kChunk; ++u)
-    b.v[u] = (slot == u) ? k : b.v[u];
-  ++c;
-  if (slot == kChunk - 1)
-    flush_chunk(A, i, c, b);
+__device__ __forceinline__ void push(Pending& P, int k, bool take) {
+  P.col[(P.c & (kRing - 1)) * kBlock] = k;
+  P.c += take ? 1 : 0;
+}
+
+// store the oldest pending chunk once it is whole (or, at the end, partly filled)
+__device__ __forceinline__ void drain(const BuildArgs& A, Pending& P, bool last) {
+  if (P.c - P.f >= kChunk || (last && P.c > P.f)) {
+    if (P.f < A.cap) {
+      const int* s = P.col + (P.f & (kRing - 1)) * kBlock;
+      int4* dst = reinterpret_cast<int4*>(A.nbr + P.base + (long)(P.f / kChunk) * P.cstep);
+      dst[0] = make_int4(s[0], s[kBlock], s[2 * kBlock], s[3 * kBlock]);
+      dst[1] = make_int4(s[4 * kBlock], s[5 * kBlock], s[6 * kBlock], s[7 * kBlock]);
+    }
+    P.f += kChunk;
+  }
 }
 
 __device__ __forceinline__ float wrap1f(float d, float L, float h) {
@@ -398,44 +425,48 @@ __device__ __forceinline__ float wrap1f(float d, float L, float h) {
 
 // candidates of cell p against atom i; entries are written as k + shift (the
 // halo slot of sorted atom k, or k itself for global indices)
+// Candidates are taken kGroup at a time; the index is clamped to the cell's
+// last atom so every load stays inside the cell, and only k < hi is accepted.
 __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, const double* xi,
-                                          float sx, float sy, float sz, int& c, Chunk& b, int shift) {
+                                          float sx, float sy, float sz, Pending& P, int shift) {
   const int lo = A.cb[p], hi = A.cb[p + 1];
+  const int ii = (int)i;
   if (A.path == kPathExact64) {
-    for (int k = lo; k < hi; ++k) {
-      double dx = A.x[3 * (long)k] - xi[0], dy = A.x[3 * (long)k + 1] - xi[1],
-             dz = A.x[3 * (long)k + 2] - xi[2];
-      min_image(A.b, dx, dy, dz);
-      if (k != i && sqmod(dx, dy, dz) < A.rl2)
-        push(A, i, k + shift, c, b);
-    }
-  } else if (A.path == kPathMin32) {
-    for (int k = lo; k < hi; ++k) {
-      const float4 q = A.xf[k];
-      const float dx = wrap1f(q.x - sx, A.Lf[0], A.halff[0]);
-      const float dy = wrap1f(q.y - sy, A.Lf[1], A.halff[1]);
-      const float dz = wrap1f(q.z - sz, A.Lf[2], A.halff[2]);
-      if (k != i && dx * dx + dy * dy + dz * dz < A.rl2f)
-        push(A, i, k + shift, c, b);
+    for (int k0 = lo; k0 < hi; k0 += kGroup) {
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int k = min(k0 + u, hi - 1);
+        double dx = A.x[3 * (long)k] - xi[0], dy = A.x[3 * (long)k + 1] - xi[1],
+               dz = A.x[3 * (long)k + 2] - xi[2];
+        min_image(A.b, dx, dy, dz);
+        push(P, k + shift, k0 + u < hi && k != ii && sqmod(dx, dy, dz) < A.rl2);
+      }
+      drain(A, P, false);
     }
   } else {
-    // (sx, sy, sz) is atom i moved by minus this cell's periodic image shift
-    int k = lo;
-    for (; k + 1 < hi; k += 2) {
-      const float4 q0 = A.xf[k], q1 = A.xf[k + 1];
-      const float ax = q0.x - sx, ay = q0.y - sy, az = q0.z - sz;
-      const float bx = q1.x - sx, by = q1.y - sy, bz = q1.z - sz;
-      const float ra = ax * ax + ay * ay + az * az, rb = bx * bx + by * by + bz * bz;
-      if (k != i && ra < A.rl2f)
-        push(A, i, k + shift, c, b);
-      if (k + 1 != i && rb < A.rl2f)
-        push(A, i, k + 1 + shift, c, b);
-    }
-    if (k < hi) {
-      const float4 q0 = A.xf[k];
-      const float ax = q0.x - sx, ay = q0.y - sy, az = q0.z - sz;
-      if (k != i && ax * ax + ay * ay + az * az < A.rl2f)
-        push(A, i, k + shift, c, b);
+    // the group's positions are loaded before any is tested, and the tests
+    // combine with & (no short circuit), so the loads are in flight together
+    const bool wrap = A.path == kPathMin32;
+    for (int k0 = lo; k0 < hi; k0 += kGroup) {
+      float4 q[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u)
+        q[u] = A.xf[min(k0 + u, hi - 1)];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int k = min(k0 + u, hi - 1);
+        // kPathShift32: (sx, sy, sz) is atom i moved by minus this cell's
+        // periodic image shift; kPathMin32: a per-pair minimum image
+        float dx = q[u].x - sx, dy = q[u].y - sy, dz = q[u].z - sz;
+        if (wrap) {
+          dx = wrap1f(dx, A.Lf[0], A.halff[0]);
+          dy = wrap1f(dy, A.Lf[1], A.halff[1]);
+          dz = wrap1f(dz, A.Lf[2], A.halff[2]);
+        }
+        const bool in = dx * dx + dy * dy + dz * dz < A.rl2f;
+        push(P, k + shift, (k0 + u < hi) & (k != ii) & in);
+      }
+      drain(A, P, false);
     }
   }
 }
@@ -461,12 +492,9 @@ __device__ __forceinline__ bool cell_shift(const BuildArgs& A, const HaloTile* D
 
 // List of atom i (every lane of a wavefront must call it: it ends with
 // wavefront reductions into max_count / sum_count).
-__device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool active) {
-  int c = 0;
-  Chunk b;
-#pragma unroll
-  for (int u = 0; u < kChunk; ++u)
-    b.v[u] = 0;
+// ring: the block's kRing x kBlock LDS array for pending entries.
+__device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool active, int* ring) {
+  Pending P = make_pending(A, i, ring);
   // the tile's halo (the wavefront serves one tile: a uniform descriptor)
   const HaloTile* D = nullptr;
   int nr = 0;
@@ -497,7 +525,7 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
               // p == ncell only when side >= n (WrapAround's -n case): the
               // reference reads past its cell table there; treated as empty
               if (p < g.ncell)
-                scan_cell(A, (int)p, i, xi, fi.x, fi.y, fi.z, c, b, 0);
+                scan_cell(A, (int)p, i, xi, fi.x, fi.y, fi.z, P, 0);
             }
           }
         }
@@ -519,7 +547,7 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
               const int p = (z2 * g.ny + z1) * g.nx + z0;
               int shift;
               if (cell_shift(A, D, nr, p, shift))
-                scan_cell(A, p, i, xi, fi.x - shx, fi.y - shy, fi.z - shz, c, b, shift);
+                scan_cell(A, p, i, xi, fi.x - shx, fi.y - shy, fi.z - shz, P, shift);
             }
           }
         }
@@ -536,16 +564,16 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
               const int p = (z2 * g.ny + z1) * g.nx + z0;
               int shift;
               if (cell_shift(A, D, nr, p, shift))
-                scan_cell(A, p, i, xi, fi.x, fi.y, fi.z, c, b, shift);
+                scan_cell(A, p, i, xi, fi.x, fi.y, fi.z, P, shift);
             }
           }
         }
       }
     }
-    if (c & (kChunk - 1))
-      flush_chunk(A, i, c, b); // the last, partly filled chunk
-    A.cnt[i] = c;
+    drain(A, P, true); // the last, partly filled chunk
+    A.cnt[i] = P.c;
   }
+  const int c = active ? P.c : 0;
   unsigned int m = wave_max((unsigned int)c);
   unsigned long long t = wave_sum((unsigned long long)c);
   if ((threadIdx.x & (kWave - 1)) == 0 && m > 0) {
@@ -556,8 +584,9 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
 }
 
 static __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
+  __shared__ int ring[kRing * kBlock];
   const long i = xcd_block() * kBlock + threadIdx.x;
-  build_atom(A, i, i < A.n);
+  build_atom(A, i, i < A.n, ring);
 }
 
 } // namespace lpmdx
