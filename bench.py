@@ -374,6 +374,8 @@ def main():
     else:
         bytes_force = n * (148 + 4 * nbar)
         kname = "k_pair_force"
+    if st1["brick"]:
+        kname += "_bk"  # the LDS-staged variant, one workgroup per brick of cells
     avg_force_s = f_ms / max(f_n, 1) / 1e3
     achieved = bytes_force / avg_force_s / 1e9
     roofline = {

@@ -144,7 +144,8 @@ LPMDX_API int lpmdx_observables(lpmdx_system* sys, double out[LPMDX_NOBS]);
 
 /* Counters of the system's engine: [0] list builds, [1] force evaluations,
  * [2] steps, [3] list capacity per atom, [4] largest list, [5] lanes per atom,
- * [6] cells along x, [7] total list entries of the last build. */
+ * [6] brick edge in cells of the LDS-staged force kernels (0: unstaged),
+ * [7] total list entries of the last build. */
 LPMDX_API int lpmdx_system_stats(const lpmdx_system* sys, long out[8]);
 
 /* Per-kernel timing with HIP events recorded on the system's own stream:

@@ -174,7 +174,7 @@ class System:
     def stats(self):
         out = np.zeros(8, dtype=np.int64)
         _check(lib().lpmdx_system_stats(self._h, _p(out)))
-        keys = ("builds", "force_evals", "steps", "list_cap", "max_list", "lanes", "grid_nx", "list_entries")
+        keys = ("builds", "force_evals", "steps", "list_cap", "max_list", "lanes", "brick", "list_entries")
         return dict(zip(keys, (int(v) for v in out)))
 
     def profile(self, enable: bool = True):

@@ -79,7 +79,6 @@ struct lpmdx_batch {
   // per-atom arenas
   DBuf<double> x0, x1, xb, v, a, sv, sa, ffm, mass, sffm, smass, rho;
   DBuf<int> z, sz, perm, sperm, cnt, nbr, cb, tsys;
-  DBuf<HaloTile> halo; // per atom tile
   DBuf<float4> xf;
   DBuf<float> w;
   DBuf<uint32_t> k0, k1, v0, v1;
@@ -148,7 +147,6 @@ struct lpmdx_batch {
     R.cnt = cnt.p;
     R.nbr = nbr.p;
     R.cb = cb.p;
-    R.halo = halo.p;
     R.keys[0] = k0.p;
     R.keys[1] = k1.p;
     R.vals[0] = v0.p;
@@ -174,7 +172,6 @@ struct lpmdx_batch {
     B.chunks = chunks.p;
     B.nchunks = nchunks;
     B.hist = hist.p;
-    B.halo_on = halo_cap() > 0;
     return B;
   }
 
@@ -199,7 +196,6 @@ struct lpmdx_batch {
     tsys.alloc(A / kTile);
     nbr.alloc(std::max<long>(lcap, kTile));
     cb.alloc(std::max<long>(ccap, 2));
-    halo.alloc(std::max<long>(acap / kTile, 1));
     hipchk(hipMemsetAsync(x0.p, 0, 3 * A * sizeof(double), st), "memset");
     hipchk(hipMemsetAsync(x1.p, 0, 3 * A * sizeof(double), st), "memset");
     hipchk(hipMemsetAsync(nbr.p, 0, std::max<long>(lcap, kTile) * sizeof(int), st), "memset");
@@ -492,29 +488,23 @@ struct lpmdx_batch {
     dirty = false;
   }
 
-  // halo cap of the staged kernels; 0 (nothing staged) unless staging is enabled
-  static int halo_cap() { return stage_enabled() ? kHaloCap : 0; }
-
   template <int KIND> void launch_group(Group& G) {
     BArgs B = args(G.tiles.p, G.n);
-    const int fparts = G.metal ? (kStageW | kStageS) : kStageS;
-    const int kHaloCap = halo_cap();
-    const size_t lr = stage_bytes(kHaloCap, 0), lf = stage_bytes(kHaloCap, fparts);
     switch (G.lpa) {
     case 1:
       if (G.metal)
-        k_b_rho<KIND, 1><<<G.n, kWave, lr, st>>>(B, kHaloCap);
-      k_b_force<KIND, 1><<<G.n, kWave, lf, st>>>(B, kHaloCap, fparts);
+        k_b_rho<KIND, 1><<<G.n, kWave, 0, st>>>(B);
+      k_b_force<KIND, 1><<<G.n, kWave, 0, st>>>(B);
       break;
     case 2:
       if (G.metal)
-        k_b_rho<KIND, 2><<<G.n, 2 * kWave, lr, st>>>(B, kHaloCap);
-      k_b_force<KIND, 2><<<G.n, 2 * kWave, lf, st>>>(B, kHaloCap, fparts);
+        k_b_rho<KIND, 2><<<G.n, 2 * kWave, 0, st>>>(B);
+      k_b_force<KIND, 2><<<G.n, 2 * kWave, 0, st>>>(B);
       break;
     default:
       if (G.metal)
-        k_b_rho<KIND, 4><<<G.n, 4 * kWave, lr, st>>>(B, kHaloCap);
-      k_b_force<KIND, 4><<<G.n, 4 * kWave, lf, st>>>(B, kHaloCap, fparts);
+        k_b_rho<KIND, 4><<<G.n, 4 * kWave, 0, st>>>(B);
+      k_b_force<KIND, 4><<<G.n, 4 * kWave, 0, st>>>(B);
       break;
     }
   }
@@ -536,8 +526,6 @@ struct lpmdx_batch {
     }
     k_b_sorted<<<blocks_for(ns), kBlock, 0, st>>>(B);
     if (n_all) {
-      if (halo_cap() > 0)
-        k_b_halo<<<blocks_for(n_all), kBlock, 0, st>>>(B, halo_cap());
       k_b_build<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
     }
     for (auto& G : groups) {
@@ -597,8 +585,6 @@ struct lpmdx_batch {
     pull_state();
     std::string bad;
     for (size_t s = 0; s < hsys.size(); ++s) {
-      if (meta[s].live && hst[s].halo_miss)
-        fail(LPMDX_EDEVICE, "neighbour cell outside its tile halo (internal error) in system " + std::to_string(s));
       if (meta[s].live && hst[s].overflow)
         bad += " " + std::to_string(s);
     }
@@ -643,17 +629,10 @@ struct lpmdx_batch {
 
   template <int KIND> void eval_k(long sid, int lpa, long nb, double* pa, int rho_pass) {
     BArgs B = args(nullptr, 0);
-    const int parts = rho_pass ? 0 : (hsys[sid].metal ? (kStageW | kStageS) : kStageS);
-    const int kHaloCap = halo_cap();
-    const size_t lds = stage_bytes(kHaloCap, parts);
     switch (lpa) {
-    case 1: k_b_eval<KIND, 1><<<nb, kWave, lds, st>>>(B, (int)sid, evala.p, pa, rho_pass, kHaloCap, parts); break;
-    case 2:
-      k_b_eval<KIND, 2><<<nb, 2 * kWave, lds, st>>>(B, (int)sid, evala.p, pa, rho_pass, kHaloCap, parts);
-      break;
-    default:
-      k_b_eval<KIND, 4><<<nb, 4 * kWave, lds, st>>>(B, (int)sid, evala.p, pa, rho_pass, kHaloCap, parts);
-      break;
+    case 1: k_b_eval<KIND, 1><<<nb, kWave, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
+    case 2: k_b_eval<KIND, 2><<<nb, 2 * kWave, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
+    default: k_b_eval<KIND, 4><<<nb, 4 * kWave, 0, st>>>(B, (int)sid, evala.p, pa, rho_pass); break;
     }
   }
 

@@ -8,13 +8,10 @@
 //                large system is sorted by many workgroups at once
 //   k_b_gather / k_b_commit / k_b_sorted
 //                cell starts and the reorder of the per-atom state
-//   k_b_halo     every tile of those systems: its halo (the cells its
-//                atoms' neighbours can be in, as ranges of sorted atoms)
-//   k_b_build    every tile of those systems: neighbour lists (halo slots)
+//   k_b_build    every tile of those systems: neighbour lists
 //   k_b_rho      EAM systems: rho pass            (one launch per kind)
 //   k_b_force    all active systems: forces + velocity Verlet (per kind);
-//                one workgroup per tile, neighbour positions read from the
-//                tile's halo copied into LDS
+//                one workgroup per tile
 // The force and build launches sweep a static list of 64-atom tiles covering
 // all systems, so a launch is load balanced across the whole GPU whatever the
 // system sizes.  Each system's atoms, cells and lists live in its own slices
@@ -79,7 +76,7 @@ struct BState {
   int init;           // 1: the first force (and drift) is pending, 2: running
   int overflow;       // a list outgrew cap (sticky)
   unsigned int maxcnt;        // longest list of the last build
-  unsigned int halo_miss;     // a neighbour cell fell outside its tile's halo (sticky)
+  unsigned int pad;
   unsigned long long sumcnt;  // list entries of the last build
   long builds;                // list builds so far
 };
@@ -100,7 +97,6 @@ struct BArena {
   int* cnt;
   int* nbr;     // list arena (words)
   int* cb;       // cell arena: cell starts of each system (ncell + 2)
-  HaloTile* halo; // per global tile
   uint32_t* keys[2];
   uint32_t* vals[2];
   double* sv;   // [3A] reorder scratch
@@ -122,7 +118,6 @@ struct BArgs {
   const int2* chunks; // sort chunks: (system, chunk index within it)
   int nchunks;
   uint32_t* hist;     // per system: kRadix x nch digit counts, digit-major
-  int halo_on;        // tile halos are built (staging enabled)
 };
 
 // ---------------------------------------------------------------- control
@@ -487,21 +482,6 @@ __device__ __forceinline__ ForceArgs force_view(const BArgs& B, int s, double* x
   return A;
 }
 
-// ---------------------------------------------------------------- halo
-// One thread per tile of every rebuilding system.
-static __global__ __launch_bounds__(kBlock) void k_b_halo(BArgs B, int cap) {
-  const long tl = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (tl >= B.ntiles)
-    return;
-  const int tile = B.tiles[tl];
-  const int s = B.tsys[tile];
-  const BSys& S = B.sys[s];
-  const BState& st = B.st[s];
-  if (!(st.act && st.flag))
-    return;
-  make_halo(B.R.keys[S.passes & 1] + S.a0, B.R.cb + S.c0, S.g, S.n, tile - S.a0 / kTile, cap, B.R.halo + tile);
-}
-
 // ---------------------------------------------------------------- build
 static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
   __shared__ int ring[kRing * kBlock];
@@ -518,8 +498,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
   A.xf = B.R.xf + a0;
   A.skeys = B.R.keys[S.passes & 1] + a0;
   A.cb = B.R.cb + S.c0;
-  A.halo = B.halo_on ? B.R.halo + S.a0 / kTile : nullptr;
-  A.halo_miss = &st.halo_miss;
+  A.bhalo = nullptr;
   A.nbr = B.R.nbr + S.l0;
   A.cnt = B.R.cnt + a0;
   A.max_count = &st.maxcnt;
@@ -544,10 +523,11 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
 }
 
 // ---------------------------------------------------------------- forces
-// One workgroup of 64 * LPA threads per tile of the launch's tile list; the
-// tile's halo (halo slots in its list) is staged in LDS (cap atoms at most).
+// One workgroup of 64 * LPA threads per tile of the launch's tile list.  The
+// EAM passes load each list batch when it starts (no prefetch): measured
+// faster for these one-tile workgroups.
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kWave * LPA) void k_b_rho(BArgs B, int cap) {
+static __global__ __launch_bounds__(kWave * LPA) void k_b_rho(BArgs B) {
   const long tl = xcd_block();
   if (tl >= B.ntiles)
     return;
@@ -558,17 +538,12 @@ static __global__ __launch_bounds__(kWave * LPA) void k_b_rho(BArgs B, int cap) 
   const ForceArgs A = force_view(B, s, nullptr);
   const long i = (long)tile * kTile - B.sys[s].a0 + threadIdx.x / LPA;
   const int l = (int)(threadIdx.x % LPA);
-  float* w = B.R.w + B.sys[s].a0;
-  with_tile(A, B.R.halo, tile, cap, 0, nullptr, [&](bool staged, const Stage& H) {
-    if (staged)
-      eam_rho_atom<KIND, LPA, false, true>(A, H, i, l, i < A.n, w, nullptr);
-    else
-      eam_rho_atom<KIND, LPA, false, false>(A, H, i, l, i < A.n, w, nullptr);
-  });
+  double emb = 0.0;
+  eam_rho_atom<KIND, LPA, false, false, false>(A, Stage{}, i, l, i < A.n, B.R.w + B.sys[s].a0, nullptr, emb);
 }
 
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kWave * LPA) void k_b_force(BArgs B, int cap, int parts) {
+static __global__ __launch_bounds__(kWave * LPA) void k_b_force(BArgs B) {
   const long tl = xcd_block();
   if (tl >= B.ntiles)
     return;
@@ -579,30 +554,28 @@ static __global__ __launch_bounds__(kWave * LPA) void k_b_force(BArgs B, int cap
   const ForceArgs A = force_view(B, s, nullptr);
   const long i = (long)tile * kTile - B.sys[s].a0 + threadIdx.x / LPA;
   const int l = (int)(threadIdx.x % LPA);
-  const bool metal = B.sys[s].metal;
-  with_tile(A, B.R.halo, tile, cap, parts, metal ? A.w_rho : nullptr, [&](bool staged, const Stage& H) {
-    if (metal) {
-      if (staged)
-        eam_force_atom<KIND, LPA, false, true>(A, H, i, l, i < A.n);
-      else
-        eam_force_atom<KIND, LPA, false, false>(A, H, i, l, i < A.n);
-    } else {
-      if (staged)
-        pair_atom<KIND, LPA, false, true>(A, H, i, l, i < A.n);
-      else
-        pair_atom<KIND, LPA, false, false>(A, H, i, l, i < A.n);
-    }
-  });
+  Tally tal;
+  // species are read only by the systems whose potentials need them (the
+  // branch is uniform: one workgroup, one system)
+  if (B.sys[s].metal) {
+    if (A.P.spc_sum >= 0)
+      eam_force_atom<KIND, LPA, false, false, true, false>(A, Stage{}, i, l, i < A.n, tal);
+    else
+      eam_force_atom<KIND, LPA, false, false, false, false>(A, Stage{}, i, l, i < A.n, tal);
+  } else {
+    if (A.nterm > 1 || A.P.spc_sum >= 0)
+      pair_atom<KIND, LPA, false, false, true>(A, Stage{}, i, l, i < A.n, tal);
+    else
+      pair_atom<KIND, LPA, false, false, false>(A, Stage{}, i, l, i < A.n, tal);
+  }
 }
 
 // Energy of one system at its user positions x[upar] (no state change): one
 // workgroup per tile of that system, block partials summed in a fixed order.
 template <int KIND, int LPA>
 static __global__ __launch_bounds__(kWave * LPA) void k_b_eval(BArgs B, int s, double* scratch_a,
-                                                               double* partials, int rho_pass, int cap,
-                                                               int parts) {
+                                                               double* partials, int rho_pass) {
   const BSys& S = B.sys[s];
-  const long tile = S.a0 / kTile + xcd_block();
   const long i = (long)xcd_block() * kTile + threadIdx.x / LPA;
   const int l = (int)(threadIdx.x % LPA);
   ForceArgs A = force_view(B, s, nullptr);
@@ -611,28 +584,17 @@ static __global__ __launch_bounds__(kWave * LPA) void k_b_eval(BArgs B, int s, d
   A.mode = kForcesOnly;
   A.partials = partials;
   A.maxd2 = nullptr;
-  const int p = rho_pass ? 0 : parts;
-  with_tile(A, B.R.halo, tile, cap, p, (S.metal && !rho_pass) ? A.w_rho : nullptr,
-            [&](bool staged, const Stage& H) {
-    if (S.metal) {
-      if (rho_pass) {
-        if (staged)
-          eam_rho_atom<KIND, LPA, true, true>(A, H, i, l, i < A.n, B.R.w + S.a0, nullptr);
-        else
-          eam_rho_atom<KIND, LPA, true, false>(A, H, i, l, i < A.n, B.R.w + S.a0, nullptr);
-      } else {
-        if (staged)
-          eam_force_atom<KIND, LPA, true, true>(A, H, i, l, i < A.n);
-        else
-          eam_force_atom<KIND, LPA, true, false>(A, H, i, l, i < A.n);
-      }
-    } else {
-      if (staged)
-        pair_atom<KIND, LPA, true, true>(A, H, i, l, i < A.n);
-      else
-        pair_atom<KIND, LPA, true, false>(A, H, i, l, i < A.n);
-    }
-  });
+  Tally tal;
+  double emb = 0.0;
+  if (S.metal) {
+    if (rho_pass)
+      eam_rho_atom<KIND, LPA, true, false>(A, Stage{}, i, l, i < A.n, B.R.w + S.a0, nullptr, emb);
+    else
+      eam_force_atom<KIND, LPA, true, false, true>(A, Stage{}, i, l, i < A.n, tal);
+  } else {
+    pair_atom<KIND, LPA, true, false, true>(A, Stage{}, i, l, i < A.n, tal);
+  }
+  write_partials(A, tal, emb);
 }
 
 } // namespace lpmdx

@@ -29,8 +29,39 @@ struct Grid {
   int nx, ny, nz;
   int side;  // linkedcell stencil half-width
   int mode;  // GridMode
-  int ncell; // nx*ny*nz; key == ncell marks an atom outside every cell
+  int ncell; // cell ids: nx*ny*nz, or nbx*nby*nbz*brick^3 on a bricked grid;
+             // key == ncell marks an atom outside every cell (linkedcell)
+  int brick; // cells per brick edge on a bricked exact grid, 0 otherwise
+  int nbx, nby, nbz; // bricks per axis (bricked grids)
 };
+
+// Cell id of cell (i, j, k) (0 <= i < nx, ...).  The plain order is x-fastest.
+// A bricked grid numbers cells brick by brick (bricks x-fastest, the cells of
+// a brick x-fastest), so the atoms of one brick are one contiguous range of
+// the cell-sorted order; ids of cells past the grid edge in a partial brick
+// hold no atoms.
+__host__ __device__ __forceinline__ int cell_id(const Grid& g, int i, int j, int k) {
+  if (g.brick <= 0)
+    return (k * g.ny + j) * g.nx + i;
+  const int B = g.brick;
+  const int bi = i / B, bj = j / B, bk = k / B;
+  const int b = (bk * g.nby + bj) * g.nbx + bi;
+  return ((b * B + (k - bk * B)) * B + (j - bj * B)) * B + (i - bi * B);
+}
+
+__host__ __device__ __forceinline__ void cell_coords(const Grid& g, int id, int& i, int& j, int& k) {
+  if (g.brick <= 0) {
+    i = id % g.nx;
+    j = (id / g.nx) % g.ny;
+    k = id / (g.nx * g.ny);
+    return;
+  }
+  const int B = g.brick, B3 = B * B * B;
+  const int b = id / B3, r = id - b * B3;
+  i = (b % g.nbx) * B + r % B;
+  j = ((b / g.nbx) % g.nby) * B + (r / B) % B;
+  k = (b / (g.nbx * g.nby)) * B + r / (B * B);
+}
 
 // WrapAround, plugins/linkedcell.cc:9-15 (i == -n maps to n, as there).
 __device__ __forceinline__ int wrap_around(int i, int n) {
@@ -71,7 +102,7 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const Box& b, const 
   i = pmod(i, g.nx);
   j = pmod(j, g.ny);
   k = pmod(k, g.nz);
-  return (uint32_t)(((long)k * g.ny + j) * g.nx + i);
+  return (uint32_t)cell_id(g, i, j, k);
 }
 
 // keys[perm[i]] = cell of atom i; vals[o] = o.  Presenting atoms to the
@@ -94,20 +125,6 @@ static __global__ __launch_bounds__(kBlock) void k_fill_int(int* __restrict__ p,
     p[i] = v;
 }
 
-// [cstart[c], cend[c]) = the sorted positions of cell c (c == ncell: outside).
-static __global__ __launch_bounds__(kBlock) void k_cell_ranges(const uint32_t* __restrict__ skeys, long n,
-                                                        int* __restrict__ cstart,
-                                                        int* __restrict__ cend) {
-  long k = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n)
-    return;
-  uint32_t key = skeys[k];
-  if (k == 0 || skeys[k - 1] != key)
-    cstart[key] = (int)k;
-  if (k == n - 1 || skeys[k + 1] != key)
-    cend[key] = (int)(k + 1);
-}
-
 // cb[c] = first sorted position of cell c, for c = 0 .. ncell + 1 (ncell + 2
 // entries; cb[ncell + 1] = n): the atoms of cell c are [cb[c], cb[c + 1]),
 // empty cells included.  n + 1 threads; thread k writes the cells after the
@@ -124,146 +141,69 @@ static __global__ __launch_bounds__(kBlock) void k_cell_begin(const uint32_t* __
 }
 
 // ---------------------------------------------------------------------------
-// Tile halos.  The atoms of a 64-atom tile (one wavefront of the build) have
-// their neighbours in the 27-cell neighbourhoods of the tile's cells; the
-// union of those cells, merged into runs of consecutive cell ids, is a few
-// contiguous ranges of the cell-sorted atoms: the tile's halo.  The force
-// kernels copy a tile's halo into LDS with coalesced loads and read neighbour
-// positions there; the list stores each neighbour as its slot in the halo
-// (slots run through the ranges in order).  Tiles whose halo is too large,
-// spans too many cell rows, or that use the reference linkedcell stencil keep
-// global atom indices (total = -1).
-constexpr int kHaloTile = 64;    // atoms per tile
-constexpr int kHaloRanges = 32;  // most ranges of one halo
-constexpr int kHaloCap = 2000;   // most atoms of one staged halo: 32 B each in LDS, with the
-                                 // kernels' static reduction scratch, within 64 KiB
-constexpr int kHaloRows = 4;     // most cell rows one staged tile may span
+// Brick halos.  On a bricked grid a force workgroup serves one brick of
+// brick^3 cells.  Its atoms' neighbours lie in the (brick + 2)^3 block of
+// cells around it (with periodic wrap; on a small grid one cell may stand at
+// two places of the block): the brick's halo.  The workgroup copies the halo's
+// atoms into LDS once and the neighbour lists store each neighbour as its slot
+// there: halo cells in block order (x fastest), a cell's atoms in sorted order.
+constexpr int kMaxBrick = 3;
+constexpr int kMaxHaloCells = (kMaxBrick + 2) * (kMaxBrick + 2) * (kMaxBrick + 2);
+constexpr int kHaloThreads = 128; // k_brick_halo: one thread per halo cell
+static_assert(kMaxHaloCells <= kHaloThreads, "one thread per halo cell");
 
-// Tile staging is off unless LPMDX_STAGE=1 is set in the environment: on the
-// measured workloads the LDS a 64-atom tile's FP64 halo needs (1000-2000
-// atoms, 24-32 B each) costs more occupancy than the L1 gathers it saves.
-inline bool stage_enabled() {
-  static const bool on = [] {
-    const char* e = getenv("LPMDX_STAGE");
-    return e && e[0] == '1';
-  }();
-  return on;
+__host__ __device__ __forceinline__ int halo_cells(int brick) { return (brick + 2) * (brick + 2) * (brick + 2); }
+
+// Cell id of halo cell h of brick b.
+__device__ __forceinline__ int halo_cell(const Grid& g, int b, int h) {
+  const int E = g.brick + 2;
+  const int hx = h % E, hy = (h / E) % E, hz = h / (E * E);
+  const int i = pmod((b % g.nbx) * g.brick + hx - 1, g.nx);
+  const int j = pmod(((b / g.nbx) % g.nby) * g.brick + hy - 1, g.ny);
+  const int k = pmod((b / (g.nbx * g.nby)) * g.brick + hz - 1, g.nz);
+  return cell_id(g, i, j, k);
 }
 
-struct HaloTile {
-  int total; // halo atoms, or -1: global indices
-  int nr;    // ranges
-  int pad[2];
-  int4 r[kHaloRanges]; // (first cell, last cell, first sorted atom, first slot)
-};
-
-// Range of cell p in D (the first cell run holding p), or -1.
-__device__ __forceinline__ int halo_range(const HaloTile* D, int nr, int p) {
-  for (int r = 0; r < nr; ++r) {
-    const int4 q = D->r[r];
-    if (p >= q.x && p <= q.y)
-      return r;
+// Halo table: per brick, halo_cells + 1 entries of int2: (first slot, first
+// sorted atom) of every halo cell, then (halo atoms, atoms of the brick).
+// One workgroup of kHaloThreads per brick; the largest halo and the largest
+// brick go to halo_max and brick_max.
+static __global__ __launch_bounds__(kHaloThreads) void k_brick_halo(const int* __restrict__ cb, Grid g,
+                                                              int2* __restrict__ out,
+                                                              unsigned int* __restrict__ halo_max,
+                                                              unsigned int* __restrict__ brick_max) {
+  __shared__ int wsum[kHaloThreads / kWave];
+  const int b = blockIdx.x, h = threadIdx.x, H = halo_cells(g.brick);
+  int first = 0, count = 0;
+  if (h < H) {
+    const int p = halo_cell(g, b, h);
+    first = cb[p];
+    count = cb[p + 1] - first;
   }
-  return -1;
-}
-
-// Halo of tile t (atoms [64 t, 64 t + 64) of a system with n atoms sorted by
-// cell, sorted keys skeys, cell starts cb) on an exact grid.
-__device__ inline void make_halo(const uint32_t* __restrict__ skeys, const int* __restrict__ cb, const Grid& g,
-                                 long n, long t, int cap, HaloTile* __restrict__ D) {
-  const long a0 = t * kHaloTile, a1 = min(n, a0 + kHaloTile);
-  int total = -1, nr = 0;
-  int2 iv[kHaloRows * 9 * 2];
-  int m = 0;
-  bool ok = g.mode == kExact && a0 < a1;
-  int cf = 0, cl = 0;
-  if (ok) {
-    cf = (int)skeys[a0];
-    cl = (int)skeys[a1 - 1];
-    ok = cl < g.ncell && cl / g.nx - cf / g.nx + 1 <= kHaloRows;
+  // inclusive scan of count: within each wavefront, then across the two
+  const int lane = h & (kWave - 1);
+  int incl = count;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int t = __shfl_up(incl, o, kWave);
+    if (lane >= o)
+      incl += t;
   }
-  if (ok) {
-    // neighbour offsets per axis that name distinct cells: {-1,0,1}, {0,1} or {0}
-    const int zy0 = g.nz >= 3 ? -1 : 0, zy1 = g.nz >= 2 ? 1 : 0;
-    const int yy0 = g.ny >= 3 ? -1 : 0, yy1 = g.ny >= 2 ? 1 : 0;
-    for (int row = cf / g.nx; row <= cl / g.nx; ++row) {
-      const int y = row % g.ny, z = row / g.ny;
-      const int xa = row == cf / g.nx ? cf % g.nx : 0;
-      const int xb = row == cl / g.nx ? cl % g.nx : g.nx - 1;
-      // x pieces of the neighbour rows: [xa-1, xb+1] around the ring, or the whole row
-      int p0 = 0, p1 = g.nx - 1, q0 = 1, q1 = 0; // second piece empty unless wrapping
-      if (g.nx >= 3 && xb - xa + 3 < g.nx) {
-        p0 = xa - 1;
-        p1 = xb + 1;
-        if (p0 < 0) {
-          q0 = 0;
-          q1 = p1;
-          p0 += g.nx;
-          p1 = g.nx - 1;
-        } else if (p1 >= g.nx) {
-          q0 = 0;
-          q1 = p1 - g.nx;
-          p1 = g.nx - 1;
-        }
-      }
-      for (int dz = zy0; dz <= zy1; ++dz) {
-        const int z2 = pmod(z + dz, g.nz);
-        for (int dy = yy0; dy <= yy1; ++dy) {
-          const int base = (z2 * g.ny + pmod(y + dy, g.ny)) * g.nx;
-          iv[m++] = make_int2(base + p0, base + p1);
-          if (q0 <= q1)
-            iv[m++] = make_int2(base + q0, base + q1);
-        }
-      }
-    }
-    // sort the cell intervals by first cell, then merge overlapping / adjacent ones
-    for (int a = 1; a < m; ++a) {
-      const int2 v = iv[a];
-      int b = a - 1;
-      while (b >= 0 && iv[b].x > v.x) {
-        iv[b + 1] = iv[b];
-        --b;
-      }
-      iv[b + 1] = v;
-    }
-    total = 0;
-    int k = 0;
-    while (k < m && ok) {
-      int lo = iv[k].x, hi = iv[k].y;
-      ++k;
-      while (k < m && iv[k].x <= hi + 1) {
-        hi = max(hi, iv[k].y);
-        ++k;
-      }
-      const int s0 = cb[lo], s1 = cb[hi + 1];
-      if (s1 > s0) {
-        if (nr == kHaloRanges) {
-          ok = false;
-          break;
-        }
-        D->r[nr++] = make_int4(lo, hi, s0, total);
-        total += s1 - s0;
-      }
-    }
-    if (!ok || total > cap)
-      total = -1;
+  if (lane == kWave - 1)
+    wsum[h / kWave] = incl;
+  __syncthreads();
+  for (int w = 0; w < h / kWave; ++w)
+    incl += wsum[w];
+  int2* row = out + (long)b * (H + 1);
+  if (h < H)
+    row[h] = make_int2(incl - count, first);
+  if (h == H - 1) {
+    const int B3 = g.brick * g.brick * g.brick;
+    const int atoms = cb[(b + 1) * B3] - cb[b * B3];
+    row[H] = make_int2(incl, atoms);
+    atomicMax(halo_max, (unsigned int)incl);
+    atomicMax(brick_max, (unsigned int)atoms);
   }
-  D->total = ok ? total : -1;
-  D->nr = ok ? nr : 0;
-}
-
-// Halos of tiles [0, ntile) of one system; halo_max = the largest staged halo.
-static __global__ __launch_bounds__(kBlock) void k_halo(const uint32_t* __restrict__ skeys,
-                                                 const int* __restrict__ cb, Grid g, long n, long ntile,
-                                                 int cap, HaloTile* __restrict__ out,
-                                                 unsigned int* __restrict__ halo_max) {
-  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= ntile)
-    return;
-  make_halo(skeys, cb, g, n, t, cap, out + t);
-  const int total = out[t].total;
-  if (total > 0)
-    atomicMax(halo_max, (unsigned int)total);
 }
 
 // Reorders the per-atom state into the new (cell-sorted) order.  order[k] is
@@ -320,8 +260,7 @@ struct BuildArgs {
   const float4* xf; // FP32 copy of x (w unused), written by k_to_f32 before the build
   const uint32_t* skeys;
   const int* cb; // cell starts (k_cell_begin)
-  const HaloTile* halo; // per 64-atom tile, or null: global indices everywhere
-  unsigned int* halo_miss; // set when a neighbour cell lies outside its tile's halo
+  const int2* bhalo; // brick halo table (k_brick_halo), or null: global indices
   int* nbr;
   int* cnt;
   unsigned int* max_count;
@@ -471,48 +410,43 @@ __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, con
   }
 }
 
-// Slot shift of cell p for tile halo D (nr ranges): slot = sorted index + shift.
-// A non-empty cell outside the halo cannot happen for an exact grid; it is
-// reported (halo_miss) and the cell skipped rather than read wrongly.
-__device__ __forceinline__ bool cell_shift(const BuildArgs& A, const HaloTile* D, int nr, int p, int& shift) {
-  shift = 0;
-  if (!D)
-    return true;
-  if (A.cb[p + 1] == A.cb[p])
-    return true;
-  const int r = halo_range(D, nr, p);
-  if (r < 0) {
-    atomicOr(A.halo_miss, 1u);
-    return false;
+// Where atom i's neighbours go in the list: on a bricked grid (A.bhalo) the
+// halo slot of its brick, slot = sorted index + shift of the neighbour cell,
+// else the sorted index itself.
+struct HaloRow {
+  const int2* row = nullptr; // the brick's halo table row, or null
+  int li = 0, lj = 0, lk = 0; // atom i's cell within its brick
+  // shift of the halo cell at offset (dx, dy, dz) from atom i's cell
+  __device__ __forceinline__ int shift(int E, int dx, int dy, int dz) const {
+    if (!row)
+      return 0;
+    const int2 q = row[((lk + 1 + dz) * E + (lj + 1 + dy)) * E + (li + 1 + dx)];
+    return q.x - q.y;
   }
-  const int4 q = D->r[r];
-  shift = q.w - q.z;
-  return true;
-}
+};
 
 // List of atom i (every lane of a wavefront must call it: it ends with
 // wavefront reductions into max_count / sum_count).
 // ring: the block's kRing x kBlock LDS array for pending entries.
 __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool active, int* ring) {
   Pending P = make_pending(A, i, ring);
-  // the tile's halo (the wavefront serves one tile: a uniform descriptor)
-  const HaloTile* D = nullptr;
-  int nr = 0;
-  if (A.halo) {
-    const long t = __builtin_amdgcn_readfirstlane((int)(i / kHaloTile));
-    // (a wavefront past the last tile has no descriptor)
-    if (t < A.stride / kHaloTile && A.halo[t].total >= 0) {
-      D = A.halo + t;
-      nr = D->nr;
-    }
-  }
   if (active) {
     const Grid& g = A.g;
     uint32_t key = A.skeys[i];
     if (key < (uint32_t)g.ncell) {
       const double xi[3] = {A.x[3 * i], A.x[3 * i + 1], A.x[3 * i + 2]};
       const float4 fi = A.xf[i];
-      const int ci = key % g.nx, cj = (key / g.nx) % g.ny, ck = key / (g.nx * g.ny);
+      int ci, cj, ck;
+      cell_coords(g, (int)key, ci, cj, ck);
+      HaloRow hr;
+      const int E = g.brick + 2;
+      if (A.bhalo) {
+        const int B = g.brick;
+        hr.row = A.bhalo + (long)(key / (B * B * B)) * (halo_cells(B) + 1);
+        hr.li = ci % B;
+        hr.lj = cj % B;
+        hr.lk = ck % B;
+      }
       if (g.mode == kLinkedCell) {
         const int s = g.side;
         for (int dz = -s; dz <= s; ++dz) {
@@ -544,10 +478,8 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
               const int z0 = pmod(rx, g.nx);
               // image of cell (z0,z1,z2) next to atom i sits at x_j + sh: compare
               // x_j against x_i - sh
-              const int p = (z2 * g.ny + z1) * g.nx + z0;
-              int shift;
-              if (cell_shift(A, D, nr, p, shift))
-                scan_cell(A, p, i, xi, fi.x - shx, fi.y - shy, fi.z - shz, P, shift);
+              scan_cell(A, cell_id(g, z0, z1, z2), i, xi, fi.x - shx, fi.y - shy, fi.z - shz, P,
+                        hr.shift(E, dx, dy, dz));
             }
           }
         }
@@ -561,10 +493,7 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
             const int z1 = pmod(cj + dy, g.ny);
             for (int dx = ox0; dx <= ox1; ++dx) {
               const int z0 = pmod(ci + dx, g.nx);
-              const int p = (z2 * g.ny + z1) * g.nx + z0;
-              int shift;
-              if (cell_shift(A, D, nr, p, shift))
-                scan_cell(A, p, i, xi, fi.x, fi.y, fi.z, P, shift);
+              scan_cell(A, cell_id(g, z0, z1, z2), i, xi, fi.x, fi.y, fi.z, P, hr.shift(E, dx, dy, dz));
             }
           }
         }

@@ -61,6 +61,7 @@ __device__ __forceinline__ double fitted_inside(double x, double ll) {
 // logical blocks (a slab of the cell-sorted atoms): the neighbour cells a
 // block gathers from are then in flight on the same XCD and hit its L2.
 constexpr int kNumXcd = 8;
+constexpr int kNumCu = 256;
 __device__ __forceinline__ long xcd_block() {
   const long nb = gridDim.x;
   const long b = blockIdx.x;
@@ -115,8 +116,8 @@ struct Ctrl {
   unsigned int max_count;         // largest list length of the last build
   unsigned int pad0;
   unsigned long long sum_count;   // total list entries of the last build (8-byte aligned)
-  unsigned int halo_max;          // largest staged tile halo of the last build
-  unsigned int halo_miss;         // a neighbour cell fell outside its tile's halo
+  unsigned int halo_max;          // largest brick halo of the last build (atoms)
+  unsigned int brick_max;         // most atoms in one brick of the last build
   unsigned int pad[10];
 };
 static_assert(offsetof(Ctrl, sum_count) % 8 == 0, "64-bit counter must be 8-byte aligned");

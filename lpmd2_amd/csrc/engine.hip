@@ -60,6 +60,43 @@ __global__ void k_fill_double(double* p, long m, double v) {
 } // namespace
 
 // ---------------------------------------------------------------------------
+// Launches of the brick-staged kernels: one workgroup per brick.  Dynamic LDS
+// above 64 KiB has to be allowed per kernel; each instantiation does so once.
+struct BrickLaunch {
+  long bricks;
+  int threads;
+  size_t lds;
+  hipStream_t st;
+  const int2* bhalo;
+  int brick;
+  int cap;
+};
+
+inline bool allow_stage_lds(const void* kernel) {
+  hipchk(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kStageLdsLimit),
+         "hipFuncSetAttribute");
+  return true;
+}
+
+template <int KIND, int LPA, bool EVAL> struct BrickKernels {
+  template <bool SPC> static void pair(const BrickLaunch& L, const ForceArgs& A, int parts) {
+    static const bool ok = allow_stage_lds((const void*)k_pair_force_bk<KIND, LPA, EVAL, SPC>);
+    (void)ok;
+    k_pair_force_bk<KIND, LPA, EVAL, SPC><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick, L.cap, parts);
+  }
+  static void rho(const BrickLaunch& L, const ForceArgs& A, float* w, double* rho_out) {
+    static const bool ok = allow_stage_lds((const void*)k_eam_rho_bk<KIND, LPA, EVAL>);
+    (void)ok;
+    k_eam_rho_bk<KIND, LPA, EVAL><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick, L.cap, w, rho_out);
+  }
+  template <bool SPC> static void eam(const BrickLaunch& L, const ForceArgs& A, int parts) {
+    static const bool ok = allow_stage_lds((const void*)k_eam_force_bk<KIND, LPA, EVAL, SPC>);
+    (void)ok;
+    k_eam_force_bk<KIND, LPA, EVAL, SPC><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick, L.cap, parts);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // the system
 
 struct lpmdx_system {
@@ -79,10 +116,11 @@ struct lpmdx_system {
   DBuf<uint32_t> keys, vals, keys2, vals2, hist;
   // cells
   DBuf<int> cb;           // cell starts of the sorted atoms (ncell + 2)
-  DBuf<HaloTile> halo;    // per 64-atom tile: the halo its list slots refer to
-  bool staged = false;    // lists hold halo slots (tile-staged force kernels)
-  int halo_cap = 0;       // largest staged halo of the current lists
-  long ntile = 0;
+  DBuf<int2> bhalo;       // brick halo table (bricked grids)
+  bool staged = false;    // lists hold brick halo slots (brick-staged force kernels)
+  int halo_cap = 0;       // LDS slots of the staged kernels: >= the largest halo
+  int brick_threads = 0;  // workgroup size of the staged kernels
+  long nbrick = 0;
   Grid grid{};
   // lists
   DBuf<int> nbr, cnt;
@@ -271,20 +309,92 @@ struct lpmdx_system {
     if (nc > (1L << 28))
       fail(LPMDX_EINVAL, "cell grid too large");
     grid.ncell = (int)nc;
+    grid.brick = grid.mode == kExact ? choose_brick() : 0;
+    grid.nbx = grid.nby = grid.nbz = 0;
+    nbrick = 0;
+    if (grid.brick > 0) {
+      const int B = grid.brick;
+      grid.nbx = (grid.nx + B - 1) / B;
+      grid.nby = (grid.ny + B - 1) / B;
+      grid.nbz = (grid.nz + B - 1) / B;
+      nbrick = (long)grid.nbx * grid.nby * grid.nbz;
+      grid.ncell = (int)(nbrick * B * B * B);
+    }
   }
 
-  // Tile-staged lists (halo slots, LDS-staged force kernels): exact grids,
-  // when enabled (stage_enabled).
-  bool stage_ok(int lanes) const { return stage_enabled() && grid.mode == kExact && lanes <= 16; }
+  // Brick edge (cells) for the staged force kernels, or 0 for global-index
+  // lists.  Staging pays for pair potentials, whose unstaged kernels are
+  // bound by the neighbour-position gathers; the EAM kernels do enough
+  // arithmetic per neighbour that the unstaged ones measured faster.  A
+  // brick's halo must fit the LDS of one workgroup with room for two
+  // workgroups per CU, and there must be enough bricks to fill 256 CUs;
+  // larger bricks stage fewer halo atoms per atom.  LPMDX_BRICK=0/2/3 forces
+  // the choice.
+  int choose_brick() const {
+    static const int forced = [] {
+      const char* e = getenv("LPMDX_BRICK");
+      return e ? atoi(e) : -1;
+    }();
+    const long cells = (long)grid.nx * grid.ny * grid.nz;
+    if (forced == 0 || cells == 0)
+      return 0;
+    if (forced < 0)
+      for (auto& p : pots)
+        if (p->metal())
+          return 0;
+    const double per_cell = (double)n / (double)cells;
+    for (int B = kMaxBrick; B >= 2; --B) {
+      if (forced > 0 && B != forced)
+        continue;
+      if (grid.nx < B + 2 || grid.ny < B + 2 || grid.nz < B + 2)
+        continue;
+      const long nb = (long)((grid.nx + B - 1) / B) * ((grid.ny + B - 1) / B) * ((grid.nz + B - 1) / B);
+      // the mean halo with 10 % for density fluctuations
+      const double bytes = (double)stage_bytes((int)(1.1 * per_cell * halo_cells(B)), stage_parts());
+      if (forced > 0 || (bytes <= kStageLdsLimit / 2 && nb >= 2 * kNumCu))
+        return B;
+    }
+    return 0;
+  }
 
-  long force_blocks() const { return staged ? ntile : blocks_for(n * lpa); }
+  // The brick choice and the staged LDS size depend on the potentials: a new
+  // set of potentials that would choose another brick edge, or stage more
+  // (rho^{-1/2}, species) than fits, gets new lists.
+  void lists_follow_potentials() {
+    if (!list_built || cm.kind == kCmLinkedCell)
+      return;
+    if (choose_brick() != grid.brick || (staged && stage_bytes(halo_cap, stage_parts()) > (size_t)kStageLdsLimit))
+      list_built = false;
+  }
+
+  // What the staged kernels of the current potentials copy into LDS besides
+  // positions: rho^{-1/2} for EAM, species when a species rule or several
+  // potentials share the list.
+  int stage_parts() const {
+    int parts = 0;
+    for (auto& p : pots) {
+      if (p->metal())
+        parts |= kStageW;
+      if (p->spc_sum >= 0 || pots.size() > 1)
+        parts |= kStageS;
+    }
+    return parts;
+  }
+
+  long force_blocks() const { return staged ? nbrick : blocks_for(n * lpa); }
 
   int auto_lanes() const {
     if (lpa_req)
       return lpa_req;
-    if (stage_ok(4))
-      return n >= 2000000 ? 4 : 8; // one 64 * lanes workgroup per tile
-
+    if (grid.brick > 0) {
+      // one workgroup per brick, up to 1024 threads: lanes that keep a brick's
+      // atoms within one pass
+      const double per_brick = (double)n / (double)nbrick;
+      int l = 1;
+      while (l < 4 && 1.3 * per_brick * (2 * l) <= 1024.0)
+        l *= 2;
+      return l;
+    }
     // fewer lanes per atom issue fewer instructions per neighbour; small
     // systems need more lanes to fill 256 CUs
     if (n >= 500000)
@@ -342,21 +452,16 @@ struct lpmdx_system {
     // list storage: cap entries per atom, rounded to the lane count
     lpa = auto_lanes();
     stride = ((n + kWave - 1) / kWave) * kWave; // whole tiles for every lane count
-    ntile = stride / kHaloTile;
-    staged = stage_ok(lpa);
-    const size_t np = (size_t)std::max<long>(blocks_for(n * 16), ntile) * 12;
+    // staged kernels serve up to 4 lanes per atom
+    staged = grid.brick > 0 && lpa <= 4;
+    const size_t np = (size_t)std::max<long>(blocks_for(n * 16), nbrick) * 12;
     partials.alloc(np);
     partials2.alloc(np);
-    // halo_max, halo_miss
+    // halo_max, brick_max
     hipchk(hipMemsetAsync(&ctrl.p->halo_max, 0, 8, st), "memset");
     if (staged) {
-      halo.alloc(ntile);
-      static const int cap_env = [] {
-        const char* e = getenv("LPMDX_HALO_CAP");
-        return e ? std::max(64, std::min(atoi(e), kHaloCap)) : kHaloCap;
-      }();
-      k_halo<<<blocks_for(ntile), kBlock, 0, st>>>(keys.p, cb.p, grid, n, ntile, cap_env, halo.p,
-                                                   &ctrl.p->halo_max);
+      bhalo.alloc((size_t)nbrick * (halo_cells(grid.brick) + 1));
+      k_brick_halo<<<nbrick, kHaloThreads, 0, st>>>(cb.p, grid, bhalo.p, &ctrl.p->halo_max, &ctrl.p->brick_max);
       launch_check();
     }
     if (cap == 0) {
@@ -368,7 +473,7 @@ struct lpmdx_system {
         est = std::max(est, (double)n / vol * pow((2 * grid.side + 1) * box.L[0] / grid.nx, 3));
       cap = (int)std::min<double>(n * 8.0 + 64, est * 1.3 + 32);
     }
-    for (int attempt = 0; attempt < 3; ++attempt) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
       cap = ((cap + kChunk - 1) / kChunk) * kChunk; // whole 8-entry chunks
       nbr.alloc((size_t)cap * stride);
       cnt.alloc(n);
@@ -396,8 +501,7 @@ struct lpmdx_system {
       B.xf = xf.p;
       B.skeys = keys.p;
       B.cb = cb.p;
-      B.halo = staged ? halo.p : nullptr;
-      B.halo_miss = &ctrl.p->halo_miss;
+      B.bhalo = staged ? bhalo.p : nullptr;
       B.nbr = nbr.p;
       B.cnt = cnt.p;
       B.max_count = &ctrl.p->max_count;
@@ -419,13 +523,22 @@ struct lpmdx_system {
       k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
       prof_end(kProfBuild, e0);
       launch_check();
-      // max_count, pad0, sum_count, halo_max, halo_miss: 24 contiguous bytes
+      // max_count, pad0, sum_count, halo_max, brick_max: 24 contiguous bytes
       hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 24, hipMemcpyDeviceToHost, st),
              "memcpy");
       sync();
-      if (hctrl->halo_miss)
-        fail(LPMDX_EDEVICE, "neighbour cell outside its tile halo (internal error)");
-      halo_cap = staged ? std::min<int>((int)(((hctrl->halo_max + 7) / 8) * 8), kHaloCap) : 0;
+      if (staged) {
+        halo_cap = (int)(((hctrl->halo_max + 63) / 64) * 64);
+        const int lanes_atoms = (int)hctrl->brick_max * lpa;
+        brick_threads = std::max(kWave, std::min(1024, ((lanes_atoms + kWave - 1) / kWave) * kWave));
+        if (stage_bytes(halo_cap, stage_parts()) > (size_t)kStageLdsLimit) {
+          // a halo too large for LDS (a dense spot): global indices this time
+          // (the grid keeps its brick order; the unstaged kernels take any order)
+          staged = false;
+          halo_cap = 0;
+          continue;
+        }
+      }
       const long mc = hctrl->max_count;
       max_list = mc;
       sum_list = (long)hctrl->sum_count;
@@ -506,66 +619,65 @@ struct lpmdx_system {
   }
 
   // ------------------------------------------------------------------ forces
+  // SPC: kernels that read species (a species rule or several potentials of
+  // one kind); the others run a branch-free neighbour loop
+  static bool reads_species(const ForceArgs& A) { return A.nterm > 1 || A.P.spc_sum >= 0; }
+
   template <int KIND, bool EVAL> void launch_pair_k(ForceArgs& A, long nb) {
+    if (reads_species(A))
+      launch_pair_s<KIND, EVAL, true>(A, nb);
+    else
+      launch_pair_s<KIND, EVAL, false>(A, nb);
+  }
+  template <int KIND, bool EVAL, bool SPC> void launch_pair_s(ForceArgs& A, long nb) {
     hipEvent_t e0 = prof_begin();
     if (staged) {
-      const int parts = (A.nterm > 1 || A.P.spc_sum >= 0) ? kStageS : 0;
-      const size_t lds = stage_bytes(halo_cap, parts);
+      const int parts = SPC ? kStageS : 0;
+      const BrickLaunch L{nbrick, brick_threads, stage_bytes(halo_cap, parts), st, bhalo.p, grid.brick, halo_cap};
       switch (lpa) {
-      case 1: k_pair_force_st<KIND, 1, EVAL><<<ntile, kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
-      case 2: k_pair_force_st<KIND, 2, EVAL><<<ntile, 2 * kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
-      case 4: k_pair_force_st<KIND, 4, EVAL><<<ntile, 4 * kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
-      case 8: k_pair_force_st<KIND, 8, EVAL><<<ntile, 8 * kWave, lds, st>>>(A, halo.p, halo_cap, parts); break;
-      default:
-        k_pair_force_st<KIND, 16, EVAL><<<ntile, 16 * kWave, lds, st>>>(A, halo.p, halo_cap, parts);
-        break;
+      case 1: BrickKernels<KIND, 1, EVAL>::template pair<SPC>(L, A, parts); break;
+      case 2: BrickKernels<KIND, 2, EVAL>::template pair<SPC>(L, A, parts); break;
+      default: BrickKernels<KIND, 4, EVAL>::template pair<SPC>(L, A, parts); break;
       }
       prof_end(kProfForce, e0);
       return;
     }
     switch (lpa) {
-    case 1: k_pair_force<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    case 2: k_pair_force<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    case 4: k_pair_force<KIND, 4, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    case 8: k_pair_force<KIND, 8, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    default: k_pair_force<KIND, 16, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 1: k_pair_force<KIND, 1, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    case 2: k_pair_force<KIND, 2, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    case 4: k_pair_force<KIND, 4, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    case 8: k_pair_force<KIND, 8, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    default: k_pair_force<KIND, 16, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
     }
     prof_end(kProfForce, e0);
   }
   template <int KIND, bool EVAL> void launch_eam_k(ForceArgs& A, long nb, double rc2_rho) {
+    if (A.P.spc_sum >= 0)
+      launch_eam_s<KIND, EVAL, true>(A, nb, rc2_rho);
+    else
+      launch_eam_s<KIND, EVAL, false>(A, nb, rc2_rho);
+  }
+  template <int KIND, bool EVAL, bool SPC> void launch_eam_s(ForceArgs& A, long nb, double rc2_rho) {
     ForceArgs R = A;
     R.P.rc2 = rc2_rho;
     R.partials = partials2.p;
     hipEvent_t e0 = prof_begin();
     if (staged) {
-      const int parts = kStageW | (A.P.spc_sum >= 0 ? kStageS : 0);
-      const size_t lr = stage_bytes(halo_cap, 0), lf = stage_bytes(halo_cap, parts);
+      const int parts = kStageW | (SPC ? kStageS : 0);
+      const BrickLaunch Lr{nbrick, brick_threads, stage_bytes(halo_cap, 0), st, bhalo.p, grid.brick, halo_cap};
+      const BrickLaunch Lf{nbrick, brick_threads, stage_bytes(halo_cap, parts), st, bhalo.p, grid.brick, halo_cap};
       switch (lpa) {
-      case 1: k_eam_rho_st<KIND, 1, EVAL><<<ntile, kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p); break;
-      case 2:
-        k_eam_rho_st<KIND, 2, EVAL><<<ntile, 2 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
-        break;
-      case 4:
-        k_eam_rho_st<KIND, 4, EVAL><<<ntile, 4 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
-        break;
-      case 8:
-        k_eam_rho_st<KIND, 8, EVAL><<<ntile, 8 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
-        break;
-      default:
-        k_eam_rho_st<KIND, 16, EVAL><<<ntile, 16 * kWave, lr, st>>>(R, halo.p, halo_cap, w_rho.p, rho.p);
-        break;
+      case 1: BrickKernels<KIND, 1, EVAL>::rho(Lr, R, w_rho.p, rho.p); break;
+      case 2: BrickKernels<KIND, 2, EVAL>::rho(Lr, R, w_rho.p, rho.p); break;
+      default: BrickKernels<KIND, 4, EVAL>::rho(Lr, R, w_rho.p, rho.p); break;
       }
       prof_end(kProfRho, e0);
       A.w_rho = w_rho.p;
       hipEvent_t e1 = prof_begin();
       switch (lpa) {
-      case 1: k_eam_force_st<KIND, 1, EVAL><<<ntile, kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
-      case 2: k_eam_force_st<KIND, 2, EVAL><<<ntile, 2 * kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
-      case 4: k_eam_force_st<KIND, 4, EVAL><<<ntile, 4 * kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
-      case 8: k_eam_force_st<KIND, 8, EVAL><<<ntile, 8 * kWave, lf, st>>>(A, halo.p, halo_cap, parts); break;
-      default:
-        k_eam_force_st<KIND, 16, EVAL><<<ntile, 16 * kWave, lf, st>>>(A, halo.p, halo_cap, parts);
-        break;
+      case 1: BrickKernels<KIND, 1, EVAL>::template eam<SPC>(Lf, A, parts); break;
+      case 2: BrickKernels<KIND, 2, EVAL>::template eam<SPC>(Lf, A, parts); break;
+      default: BrickKernels<KIND, 4, EVAL>::template eam<SPC>(Lf, A, parts); break;
       }
       prof_end(kProfForce, e1);
       return;
@@ -581,11 +693,11 @@ struct lpmdx_system {
     A.w_rho = w_rho.p;
     hipEvent_t e1 = prof_begin();
     switch (lpa) {
-    case 1: k_eam_force<KIND, 1, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    case 2: k_eam_force<KIND, 2, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    case 4: k_eam_force<KIND, 4, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    case 8: k_eam_force<KIND, 8, EVAL><<<nb, kBlock, 0, st>>>(A); break;
-    default: k_eam_force<KIND, 16, EVAL><<<nb, kBlock, 0, st>>>(A); break;
+    case 1: k_eam_force<KIND, 1, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    case 2: k_eam_force<KIND, 2, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    case 4: k_eam_force<KIND, 4, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    case 8: k_eam_force<KIND, 8, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
+    default: k_eam_force<KIND, 16, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
     }
     prof_end(kProfForce, e1);
   }
@@ -1173,6 +1285,7 @@ int lpmdx_potential_append(lpmdx_system* S, const char* name, const char* args, 
     need(name, "name");
     S->pots.push_back(make_potential(name, args, s1, s2));
     S->forces_ok = false;
+    S->lists_follow_potentials();
     S->undrift();
   });
 }
@@ -1182,6 +1295,7 @@ int lpmdx_potential_clear(lpmdx_system* S) {
     need(S, "system");
     S->pots.clear();
     S->forces_ok = false;
+    S->lists_follow_potentials();
     S->undrift();
   });
 }
@@ -1286,7 +1400,7 @@ int lpmdx_system_stats(const lpmdx_system* S, long out[8]) {
     out[3] = S->cap;
     out[4] = S->max_list;
     out[5] = S->lpa;
-    out[6] = S->grid.nx;
+    out[6] = S->staged ? S->grid.brick : 0;
     out[7] = S->sum_list;
   });
 }
@@ -1354,20 +1468,25 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
     hipchk(hipMemcpyAsync(cntv.data(), S->cnt.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
     hipchk(hipMemcpyAsync(nb.data(), S->nbr.p, nb.size() * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
     hipchk(hipMemcpyAsync(x.data(), S->x.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, S->st), "memcpy");
-    std::vector<HaloTile> ht(S->staged ? S->ntile : 0);
-    if (S->staged)
-      hipchk(hipMemcpyAsync(ht.data(), S->halo.p, ht.size() * sizeof(HaloTile), hipMemcpyDeviceToHost, S->st),
+    const int H = S->staged ? halo_cells(S->grid.brick) : 0;
+    std::vector<int2> bh(S->staged ? (size_t)S->nbrick * (H + 1) : 0);
+    std::vector<uint32_t> keys(S->staged ? n : 0);
+    if (S->staged) {
+      hipchk(hipMemcpyAsync(bh.data(), S->bhalo.p, bh.size() * sizeof(int2), hipMemcpyDeviceToHost, S->st),
              "memcpy");
+      hipchk(hipMemcpyAsync(keys.data(), S->keys.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, S->st), "memcpy");
+    }
     S->sync();
-    // list entries of staged tiles are halo slots: back to sorted atom indices
+    // staged lists hold slots of the atom's brick halo: back to sorted indices
+    const int B3 = S->grid.brick * S->grid.brick * S->grid.brick;
     auto atom_of = [&](long i, long e) -> long {
-      if (!S->staged || ht[i / kHaloTile].total < 0)
+      if (!S->staged)
         return e;
-      const HaloTile& D = ht[i / kHaloTile];
-      int r = 0;
-      while (r + 1 < D.nr && D.r[r + 1].w <= e)
-        ++r;
-      return (long)D.r[r].z + (e - D.r[r].w);
+      const int2* row = bh.data() + (size_t)(keys[i] / B3) * (H + 1);
+      int h = 0; // the last halo cell whose first slot is <= e
+      while (h + 1 < H && row[h + 1].x <= e)
+        ++h;
+      return (long)row[h].y + (e - row[h].x);
     };
     std::vector<long> iperm(n);
     for (long i = 0; i < n; ++i)

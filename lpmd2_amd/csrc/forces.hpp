@@ -319,8 +319,8 @@ __device__ __forceinline__ void write_partials(const ForceArgs& A, Tally& t, dou
 // body(j, dx, dy, dz, r2) sees the FP64 minimum-image displacement.
 constexpr int kUnroll = 4;
 
-// LDS copy of a tile's halo (slots as in the tile's list); W and S are null
-// unless the kernel reads neighbour rho^{-1/2} or species.
+// LDS copy of a brick's halo (slots as in the brick's lists); W and S are
+// null unless the kernel reads neighbour rho^{-1/2} or species.
 struct Stage {
   const double* X;
   const double* Y;
@@ -329,80 +329,18 @@ struct Stage {
   const int* S;
 };
 
-// Copies tile D's halo into LDS.  The descriptor's ranges go to LDS first;
-// then every thread takes slots threadIdx.x + u * blockDim.x, finds each
-// slot's range there and issues the loads of up to kStageBatch slots before
-// storing any, so one memory latency covers the whole halo.
-constexpr int kStageBatch = 8;
-
-__device__ __forceinline__ void stage_halo(const ForceArgs& A, const HaloTile* D, double* X, double* Y, double* Z,
-                                           float* W, const float* wsrc, int* S) {
-  __shared__ int4 sr[kHaloRanges];
-  const int nr = D->nr, total = D->total;
-  if ((int)threadIdx.x < nr)
-    sr[threadIdx.x] = D->r[threadIdx.x];
-  __syncthreads();
-  int r = 0; // ranges only move forward as the slot grows
-  for (int base = threadIdx.x; base < total; base += kStageBatch * blockDim.x) {
-    long k[kStageBatch];
-#pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
-      const int slot = base + u * blockDim.x;
-      k[u] = -1;
-      if (slot < total) {
-        while (r + 1 < nr && sr[r + 1].w <= slot)
-          ++r;
-        k[u] = (long)sr[r].z + (slot - sr[r].w);
-      }
-    }
-    double vx[kStageBatch], vy[kStageBatch], vz[kStageBatch];
-    float vw[kStageBatch];
-    int vs[kStageBatch];
-#pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
-      if (k[u] >= 0) {
-        vx[u] = A.x[3 * k[u]];
-        vy[u] = A.x[3 * k[u] + 1];
-        vz[u] = A.x[3 * k[u] + 2];
-        vw[u] = W ? wsrc[k[u]] : 0.0f;
-        vs[u] = S ? A.z[k[u]] : 0;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
-      const int slot = base + u * blockDim.x;
-      if (k[u] >= 0) {
-        X[slot] = vx[u];
-        Y[slot] = vy[u];
-        Z[slot] = vz[u];
-        if (W)
-          W[slot] = vw[u];
-        if (S)
-          S[slot] = vs[u];
-      }
-    }
-  }
-}
-
 // The list entries of one batch: kUnroll consecutive entries k0 .. k0+3 of
 // this lane (k0 a multiple of 4, so they are one aligned 16-byte word group
-// of a chunk); slots past the list end read a valid place (the atom itself,
-// or halo slot 0) and are masked by `valid`.
-template <int LPA>
-__device__ __forceinline__ void list_batch(const int* __restrict__ nb, int k0, int c, int pad,
-                                           int (&js)[kUnroll]) {
+// of a chunk), as loaded.
+template <int LPA> __device__ __forceinline__ int4 list_batch(const int* __restrict__ nb, int k0) {
   static_assert(kUnroll == 4 && kChunk % kUnroll == 0, "one int4 per batch");
   constexpr int T = kWave / LPA;
-  const int4 q = *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
-  js[0] = k0 < c ? q.x : pad;
-  js[1] = k0 + 1 < c ? q.y : pad;
-  js[2] = k0 + 2 < c ? q.z : pad;
-  js[3] = k0 + 3 < c ? q.w : pad;
+  return *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
 }
 
-template <int LPA, bool WRAP, bool STAGED, typename Body>
+template <int LPA, bool WRAP, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, typename Body>
 __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& H, long i, int l, double xi,
-                                               double yi, double zi, bool need_s, bool need_w, Body& body) {
+                                               double yi, double zi, Body& body) {
   // a list that outgrew its capacity (reported as an error) is read no further
   // than the capacity, so no entry outside the atom's chunks is ever used
   const int c = min(A.cnt[i], A.cap);
@@ -411,23 +349,29 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
   constexpr int step = kUnroll * LPA;
   // entry k of this atom: chunk k/8 of the tile row, word k%8 (see list_index)
   const int* __restrict__ nb = A.nbr + (i / T) * (long)T * A.cap + (i % T) * kChunk;
+  // entries past the list end read a valid place (the atom itself, or halo
+  // slot 0) and are masked by `valid`
   const int pad = STAGED ? 0 : (int)i;
-  // staged: the next batch's entries are loaded while this batch is
-  // evaluated (few workgroups per CU hide little latency otherwise)
-  int jn[kUnroll];
-  if (STAGED && kUnroll * l < c)
-    list_batch<LPA>(nb, kUnroll * l, c, pad, jn);
+  // PREFETCH: the next batch's entries are loaded one batch ahead and read
+  // only when that batch starts, so the list load is in flight while this
+  // batch runs.  Otherwise the batch's entries and then its position gathers
+  // are loaded at its start (measured faster for the unstaged pair kernels,
+  // where the gathers, not the list, hold a batch up and the arithmetic per
+  // batch is too short to cover a list load).
+  int4 qn = make_int4(pad, pad, pad, pad);
+  if (PREFETCH && kUnroll * l < c)
+    qn = list_batch<LPA>(nb, kUnroll * l);
   for (int k0 = kUnroll * l; k0 < c; k0 += step) {
-    int js[kUnroll];
-    if constexpr (STAGED) {
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        js[u] = jn[u];
+    int4 q;
+    if constexpr (PREFETCH) {
+      q = qn;
       if (k0 + step < c)
-        list_batch<LPA>(nb, k0 + step, c, pad, jn);
+        qn = list_batch<LPA>(nb, k0 + step);
     } else {
-      list_batch<LPA>(nb, k0, c, pad, js);
+      q = list_batch<LPA>(nb, k0);
     }
+    const int js[kUnroll] = {k0 < c ? q.x : pad, k0 + 1 < c ? q.y : pad, k0 + 2 < c ? q.z : pad,
+                             k0 + 3 < c ? q.w : pad};
     double px[kUnroll], py[kUnroll], pz[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -447,9 +391,9 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
       const int j = js[u];
       int sj = 0;
       float wj = 0.0f;
-      if (need_s)
+      if constexpr (NEED_S)
         sj = STAGED ? H.S[j] : A.z[j];
-      if (need_w)
+      if constexpr (NEED_W)
         wj = STAGED ? H.W[j] : A.w_rho[j];
       double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
       if constexpr (WRAP)
@@ -460,21 +404,21 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
 }
 
 // Runs body(valid, s_j, w_j, dx, dy, dz, r2) over atom i's list, with
-// neighbour species s_j (need_s) and rho^{-1/2} w_j (need_w).  Where no atom
+// neighbour species s_j (NEED_S) and rho^{-1/2} w_j (NEED_W).  Where no atom
 // of the wavefront is within A.margin of a face, no displacement can reach
 // half a box edge, the minimum image is the identity
 // (OrthogonalCell::Displacement would not wrap) and the loop skips it; the
 // wavefront takes one variant, so the branch is uniform.
-template <int LPA, bool STAGED, typename Body>
+template <int LPA, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, typename Body>
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& H, long i, int l, double xi,
-                                               double yi, double zi, bool need_s, bool need_w, Body&& body) {
+                                               double yi, double zi, Body&& body) {
   const double m = A.margin;
   const bool inner = m > 0.0 && xi >= m && xi <= A.b.L[0] - m && yi >= m && yi <= A.b.L[1] - m && zi >= m &&
                      zi <= A.b.L[2] - m;
   if (__all(inner))
-    neighbour_loop<LPA, false, STAGED>(A, H, i, l, xi, yi, zi, need_s, need_w, body);
+    neighbour_loop<LPA, false, STAGED, NEED_S, NEED_W, PREFETCH>(A, H, i, l, xi, yi, zi, body);
   else
-    neighbour_loop<LPA, true, STAGED>(A, H, i, l, xi, yi, zi, need_s, need_w, body);
+    neighbour_loop<LPA, true, STAGED, NEED_S, NEED_W, PREFETCH>(A, H, i, l, xi, yi, zi, body);
 }
 
 // Per-atom force: FP32 pair scalar times the FP64 displacement, summed in FP64
@@ -491,10 +435,17 @@ struct ForceSum {
 
 // Sum of the pair terms of one kind that apply to a pair (CombinedPotential
 // adds every potential whose species rule matches, combinedpotential.h:48-49).
-template <int KIND>
+// SPC = false: one potential for every pair (no species rule), branch free.
+template <int KIND, bool SPC>
 __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int si, int sj, double r2,
                                            float& fs, float& e) {
-  if (A.nterm <= 1) {
+  if (!SPC) {
+    const bool in = valid && r2 < A.P.rc2;
+    float f, en;
+    pair_eval<KIND>(A.P, r2, in ? (float)r2 : 1.0f, f, en);
+    fs = in ? f : 0.0f;
+    e = in ? en : 0.0f;
+  } else if (A.nterm <= 1) {
     const bool in = valid && r2 < A.P.rc2 && (A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
     float f, en;
     pair_eval<KIND>(A.P, r2, in ? (float)r2 : 1.0f, f, en);
@@ -516,18 +467,21 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
 
 // Per-atom bodies of the force kernels; lane l of the LPA lanes serving atom i.
 // Every lane of a wavefront must call them (they end with wavefront reductions).
-template <int KIND, int LPA, bool EVAL, bool STAGED = false>
-__device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active) {
+// Energies, virial and stress (EVAL) add into the caller's tal.  SPC: the
+// potentials read species (a species rule, or several potentials of the kind
+// on one list); without it the neighbour loop has no branch.
+template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC>
+__device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
+                                          Tally& tal) {
   ForceSum F;
-  Tally tal;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
-    const int si = A.z[i];
-    const bool need_z = A.nterm > 1 || A.P.spc_sum >= 0;
-    for_neighbours<LPA, STAGED>(A, H, i, l, xi, yi, zi, need_z, false,
-                                [&](bool valid, int sj, float, double dx, double dy, double dz, double r2) {
+    const int si = SPC ? A.z[i] : 0;
+    for_neighbours<LPA, STAGED, SPC, false, STAGED>(A, H, i, l, xi, yi, zi,
+                                            [&](bool valid, int sj, float, double dx, double dy, double dz,
+                                                double r2) {
       float fs, e;
-      pair_terms<KIND>(A, valid, si, sj, r2, fs, e);
+      pair_terms<KIND, SPC>(A, valid, si, sj, r2, fs, e);
       F.add(fs, dx, dy, dz);
       if constexpr (EVAL)
         tal.add(fs, (float)dx, (float)dy, (float)dz, (float)r2, e);
@@ -540,57 +494,51 @@ __device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, lo
     vv_tail(A, i, Fx * m, Fy * m, Fz * m, d2);
   }
   flush_maxd2(A, d2);
-  if constexpr (EVAL)
-    write_partials(A, tal, 0.0);
 }
 
 // EAM pass 1: rho_i over the full list within the list cutoff (no species
 // check, metalpotential.cc:121-132), accumulated in FP64; writes
 // w_i = rho_i^{-1/2} (FP32) and rho_i, and, when evaluating, the embedding
-// energy F(rho_i) = -K sqrt(rho_i) into the block partials.
-template <int KIND, int LPA, bool EVAL, bool STAGED = false>
+// energy F(rho_i) = -K sqrt(rho_i) into the caller's emb.
+template <int KIND, int LPA, bool EVAL, bool STAGED, bool PREFETCH = true>
 __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
-                                             float* w_out, double* rho_out) {
+                                             float* w_out, double* rho_out, double& emb) {
   double rho = 0.0;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA, STAGED>(A, H, i, l, xi, yi, zi, false, false,
-                                [&](bool valid, int, float, double, double, double, double r2) {
+    for_neighbours<LPA, STAGED, false, false, PREFETCH>(A, H, i, l, xi, yi, zi,
+                                              [&](bool valid, int, float, double, double, double, double r2) {
       const bool in = valid && r2 < rc2;
       const float v = eam_rho<KIND>(A.P, in ? (float)r2 : 1.0f);
       rho += in ? (double)v : 0.0;
     });
   }
   rho = group_sum<LPA>(rho);
-  double emb = 0.0;
   if (active && l == 0) {
     w_out[i] = (float)(1.0 / sqrt(rho));
     if (rho_out)
       rho_out[i] = rho;
-    emb = -(double)A.P.f[11] * sqrt(rho);
-  }
-  if constexpr (EVAL) {
-    Tally tal;
-    write_partials(A, tal, emb);
+    if constexpr (EVAL)
+      emb += -(double)A.P.f[11] * sqrt(rho);
   }
 }
 
 // EAM pass 2: pair + many-body forces over the full list with AppliesTo
 // (metalpotential.cc:140-169), fused with the integrator tail.
-template <int KIND, int LPA, bool EVAL, bool STAGED = false>
-__device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active) {
+template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC, bool PREFETCH = true>
+__device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
+                                               Tally& tal) {
   ForceSum F;
-  Tally tal;
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
-    const int si = A.z[i];
+    const int si = SPC ? A.z[i] : 0;
     const float wi = A.w_rho[i];
-    const bool all = A.P.spc_sum < 0;
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA, STAGED>(A, H, i, l, xi, yi, zi, !all, true,
-                                [&](bool valid, int sj, float wj, double dx, double dy, double dz, double r2) {
-      const bool in = valid && r2 < rc2 && (all || si + sj == A.P.spc_sum);
+    for_neighbours<LPA, STAGED, SPC, true, PREFETCH>(A, H, i, l, xi, yi, zi,
+                                           [&](bool valid, int sj, float wj, double dx, double dy, double dz,
+                                               double r2) {
+      const bool in = valid && r2 < rc2 && (!SPC || A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
       const float fr2 = in ? (float)r2 : 1.0f;
       float fs, V;
       eam_pair<KIND>(A.P, fr2, wi + wj, fs, V);
@@ -607,33 +555,46 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& 
     vv_tail(A, i, Fx * m, Fy * m, Fz * m, d2);
   }
   flush_maxd2(A, d2);
+}
+
+template <int KIND, int LPA, bool EVAL, bool SPC>
+static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
+  const long t = xcd_block() * kBlock + threadIdx.x;
+  Tally tal;
+  pair_atom<KIND, LPA, EVAL, false, SPC>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, tal);
   if constexpr (EVAL)
     write_partials(A, tal, 0.0);
 }
 
 template <int KIND, int LPA, bool EVAL>
-static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
-  const long t = xcd_block() * kBlock + threadIdx.x;
-  pair_atom<KIND, LPA, EVAL>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n);
-}
-
-template <int KIND, int LPA, bool EVAL>
 static __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
   const long t = xcd_block() * kBlock + threadIdx.x;
-  eam_rho_atom<KIND, LPA, EVAL>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out, rho_out);
+  double emb = 0.0;
+  eam_rho_atom<KIND, LPA, EVAL, false>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out, rho_out, emb);
+  if constexpr (EVAL) {
+    Tally tal;
+    write_partials(A, tal, emb);
+  }
 }
 
-template <int KIND, int LPA, bool EVAL>
+template <int KIND, int LPA, bool EVAL, bool SPC>
 static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
   const long t = xcd_block() * kBlock + threadIdx.x;
-  eam_force_atom<KIND, LPA, EVAL>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n);
+  Tally tal;
+  eam_force_atom<KIND, LPA, EVAL, false, SPC>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, tal);
+  if constexpr (EVAL)
+    write_partials(A, tal, 0.0);
 }
 
-// ---- tile-staged kernels: one workgroup of 64 * LPA threads per 64-atom
-// tile; a tile whose list holds halo slots first copies its halo into LDS
-// (cap = the largest halo of the launch; dynamic LDS of cap * (24 + 4 * [W]
-// + 4 * [S]) bytes), a tile with global indices reads global memory.
+// ---- brick-staged kernels: one workgroup per brick of a bricked grid.  The
+// workgroup copies its brick's halo into LDS (positions as three FP64 arrays,
+// plus rho^{-1/2} and species when the kernel reads them: dynamic LDS of
+// cap * (24 + 4 [W] + 4 [S]) bytes, cap >= the largest halo of the launch),
+// then its threads take the brick's atoms, LPA lanes each, blockDim / LPA
+// atoms per pass; the neighbour lists hold halo slots.
 enum StageParts { kStageW = 1, kStageS = 2 };
+// dynamic LDS of one staged workgroup: 160 KiB less the kernels' static LDS
+constexpr int kStageLdsLimit = 160 * 1024 - 4096;
 
 struct StageLds {
   double* X;
@@ -662,62 +623,145 @@ __host__ __device__ inline size_t stage_bytes(int cap, int parts) {
                         ((parts & kStageS) ? sizeof(int) : 0));
 }
 
-// Runs fn(staged, Stage) for this block's tile after staging its halo.
-template <typename F>
-__device__ __forceinline__ void with_tile(const ForceArgs& A, const HaloTile* halo, long tile, int cap, int parts,
-                                          const float* wsrc, F&& fn) {
-  const HaloTile* D = halo + tile;
-  const bool staged = cap > 0 && D->total >= 0; // uniform over the block (cap 0: staging off)
-  if (staged) {
-    StageLds L = stage_lds(cap, parts);
-    stage_halo(A, D, L.X, L.Y, L.Z, L.W, wsrc, L.S);
-    __syncthreads();
-    fn(true, Stage{L.X, L.Y, L.Z, L.W, L.S});
-  } else {
-    fn(false, Stage{});
+// Copies the halo of one brick (its row of the k_brick_halo table; bricks of
+// `brick` cells per edge) into L and returns the brick's first sorted atom and
+// atom count.
+// Every thread takes slots threadIdx.x + u * blockDim.x, finds each slot's
+// halo cell by binary search over the cells' first slots, and issues the
+// loads of kStageBatch slots before storing any.  Slots past the halo's end
+// load the last slot's atom (no branch around a load, which would make the
+// compiler wait for each load before the next) and store nothing.
+constexpr int kStageBatch = 8;
+
+__device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __restrict__ row, int brick,
+                                            const StageLds& L, const float* __restrict__ wsrc, int& a0,
+                                            int& na) {
+  __shared__ int2 sr[kMaxHaloCells + 1];
+  const int E = brick + 2, H = E * E * E;
+  for (int h = threadIdx.x; h <= H; h += blockDim.x)
+    sr[h] = row[h];
+  __syncthreads();
+  const int total = sr[H].x;
+  for (int base = threadIdx.x; base < total; base += kStageBatch * blockDim.x) {
+    int k[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int slot = min(base + u * (int)blockDim.x, total - 1);
+      // the last halo cell whose first slot is <= slot (sr[0].x == 0)
+      int lo = 0, hi = H - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sr[mid].x <= slot)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      k[u] = sr[lo].y + (slot - sr[lo].x);
+    }
+    double vx[kStageBatch], vy[kStageBatch], vz[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const double* p = A.x + 3 * (long)k[u];
+      vx[u] = p[0];
+      vy[u] = p[1];
+      vz[u] = p[2];
+    }
+    float vw[kStageBatch];
+    int vs[kStageBatch];
+    if (L.W) {
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u)
+        vw[u] = wsrc[k[u]];
+    }
+    if (L.S) {
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u)
+        vs[u] = A.z[k[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int slot = base + u * blockDim.x;
+      if (slot < total) {
+        L.X[slot] = vx[u];
+        L.Y[slot] = vy[u];
+        L.Z[slot] = vz[u];
+      }
+    }
+    if (L.W) {
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u) {
+        const int slot = base + u * blockDim.x;
+        if (slot < total)
+          L.W[slot] = vw[u];
+      }
+    }
+    if (L.S) {
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u) {
+        const int slot = base + u * blockDim.x;
+        if (slot < total)
+          L.S[slot] = vs[u];
+      }
+    }
+  }
+  // the brick's own first cell is halo cell (1, 1, 1)
+  a0 = sr[(E + 1) * E + 1].y;
+  na = sr[H].y;
+  __syncthreads();
+}
+
+// Runs body(i, l, active) over the brick's atoms after staging its halo.
+template <int LPA, typename Body>
+__device__ __forceinline__ void for_brick_atoms(const ForceArgs& A, const int2* bhalo, int brick, int cap,
+                                                int parts, const float* wsrc, Stage& S, Body&& body) {
+  const StageLds L = stage_lds(cap, parts);
+  int a0, na;
+  stage_brick(A, bhalo + xcd_block() * (halo_cells(brick) + 1), brick, L, wsrc, a0, na);
+  S = Stage{L.X, L.Y, L.Z, L.W, L.S};
+  const int per = blockDim.x / LPA;
+  const int l = (int)(threadIdx.x % LPA);
+  for (int base = 0; base < na; base += per) {
+    const int k = base + (int)(threadIdx.x / LPA);
+    body((long)a0 + k, l, k < na);
   }
 }
 
-template <int KIND, int LPA, bool EVAL>
-static __global__ __launch_bounds__(kWave * LPA) void k_pair_force_st(ForceArgs A, const HaloTile* halo, int cap,
-                                                                      int parts) {
-  const long tile = xcd_block();
-  const long i = tile * kHaloTile + threadIdx.x / LPA;
-  const int l = (int)(threadIdx.x % LPA);
-  with_tile(A, halo, tile, cap, parts, nullptr, [&](bool staged, const Stage& H) {
-    if (staged)
-      pair_atom<KIND, LPA, EVAL, true>(A, H, i, l, i < A.n);
-    else
-      pair_atom<KIND, LPA, EVAL, false>(A, H, i, l, i < A.n);
+template <int KIND, int LPA, bool EVAL, bool SPC>
+static __global__ __launch_bounds__(1024) void k_pair_force_bk(ForceArgs A, const int2* bhalo, int brick, int cap,
+                                                               int parts) {
+  Tally tal;
+  Stage S;
+  for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, nullptr, S, [&](long i, int l, bool active) {
+    pair_atom<KIND, LPA, EVAL, true, SPC>(A, S, i, l, active, tal);
   });
+  if constexpr (EVAL)
+    write_partials(A, tal, 0.0);
 }
 
 template <int KIND, int LPA, bool EVAL>
-static __global__ __launch_bounds__(kWave * LPA) void k_eam_rho_st(ForceArgs A, const HaloTile* halo, int cap,
-                                                                   float* w_out, double* rho_out) {
-  const long tile = xcd_block();
-  const long i = tile * kHaloTile + threadIdx.x / LPA;
-  const int l = (int)(threadIdx.x % LPA);
-  with_tile(A, halo, tile, cap, 0, nullptr, [&](bool staged, const Stage& H) {
-    if (staged)
-      eam_rho_atom<KIND, LPA, EVAL, true>(A, H, i, l, i < A.n, w_out, rho_out);
-    else
-      eam_rho_atom<KIND, LPA, EVAL, false>(A, H, i, l, i < A.n, w_out, rho_out);
+static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const int2* bhalo, int brick, int cap,
+                                                            float* w_out, double* rho_out) {
+  double emb = 0.0;
+  Stage S;
+  for_brick_atoms<LPA>(A, bhalo, brick, cap, 0, nullptr, S, [&](long i, int l, bool active) {
+    eam_rho_atom<KIND, LPA, EVAL, true>(A, S, i, l, active, w_out, rho_out, emb);
   });
+  if constexpr (EVAL) {
+    Tally tal;
+    write_partials(A, tal, emb);
+  }
 }
 
-template <int KIND, int LPA, bool EVAL>
-static __global__ __launch_bounds__(kWave * LPA) void k_eam_force_st(ForceArgs A, const HaloTile* halo, int cap,
-                                                                     int parts) {
-  const long tile = xcd_block();
-  const long i = tile * kHaloTile + threadIdx.x / LPA;
-  const int l = (int)(threadIdx.x % LPA);
-  with_tile(A, halo, tile, cap, parts, A.w_rho, [&](bool staged, const Stage& H) {
-    if (staged)
-      eam_force_atom<KIND, LPA, EVAL, true>(A, H, i, l, i < A.n);
-    else
-      eam_force_atom<KIND, LPA, EVAL, false>(A, H, i, l, i < A.n);
+template <int KIND, int LPA, bool EVAL, bool SPC>
+static __global__ __launch_bounds__(1024) void k_eam_force_bk(ForceArgs A, const int2* bhalo, int brick, int cap,
+                                                              int parts) {
+  Tally tal;
+  Stage S;
+  for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, A.w_rho, S, [&](long i, int l, bool active) {
+    eam_force_atom<KIND, LPA, EVAL, true, SPC>(A, S, i, l, active, tal);
   });
+  if constexpr (EVAL)
+    write_partials(A, tal, 0.0);
 }
 
 // Velocity Verlet with precomputed accelerations (several potentials):
