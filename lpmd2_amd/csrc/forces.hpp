@@ -60,6 +60,43 @@ __device__ __forceinline__ float powk(float t, int ki, float kf) {
   return ki > 0 ? powu(t, ki) : powf(t, kf);
 }
 
+// t^K for a compile-time 0 < K < 32: the same squares multiplied in the same
+// order as powu (by 1.0 first, which is exact), so the result is bit-identical.
+template <int K> __device__ __forceinline__ float powc(float t) {
+  static_assert(K > 0 && K < 32, "exponent out of range");
+  const float t2 = t * t, t4 = t2 * t2, t8 = t4 * t4, t16 = t8 * t8;
+  float r = 1.0f;
+  if constexpr ((K & 1) != 0)
+    r = t;
+  if constexpr ((K & 2) != 0)
+    r *= t2;
+  if constexpr ((K & 4) != 0)
+    r *= t4;
+  if constexpr ((K & 8) != 0)
+    r *= t8;
+  if constexpr ((K & 16) != 0)
+    r *= t16;
+  return r;
+}
+
+// Sutton-Chen exponent pair fixed at compile time: EXP = 32 n + m, 0 = read
+// them at run time.
+constexpr int sc_exp(int n, int m) { return 32 * n + m; }
+
+template <int EXP> __device__ __forceinline__ float sc_pow_n(const DevPot& P, float t) {
+  if constexpr (EXP != 0)
+    return powc<EXP / 32>(t);
+  else
+    return powk(t, P.ni, P.f[2]);
+}
+
+template <int EXP> __device__ __forceinline__ float sc_pow_m(const DevPot& P, float t) {
+  if constexpr (EXP != 0)
+    return powc<EXP % 32>(t);
+  else
+    return powk(t, P.mi, P.f[3]);
+}
+
 // Pair potentials: force on i is fscal * rij (rij = x_j - x_i), pair energy e.
 // Pair forms follow plugins/{lennardjones,fastlj,morse,buckingham}.cc; the
 // FP32 evaluation uses hardware reciprocal / reciprocal square root (1 ulp).
@@ -106,9 +143,9 @@ __device__ __forceinline__ void pair_eval(const DevPot& P, double r2d, float r2,
 // fscal with force on i = fscal * rij:
 //   fscal = (dV/dr + (w_i + w_j) G(r)) / r,  w = rho^{-1/2},
 // where (w_i + w_j) G(r) r-hat is the reference's ManyBodies term.
-template <int KIND> __device__ __forceinline__ float eam_rho(const DevPot& P, float r2) {
+template <int KIND, int EXP = 0> __device__ __forceinline__ float eam_rho(const DevPot& P, float r2) {
   if constexpr (KIND == kSuttonChen) { // suttonchen.cc:78-81: a^m r^-m
-    return powk(P.f[1] * rsq(r2), P.mi, P.f[3]);
+    return sc_pow_m<EXP>(P, P.f[1] * rsq(r2));
   } else if constexpr (KIND == kGupta) { // gupta.cc:69
     const float r = r2 * rsq(r2);
     return expf(P.f[6] - P.f[7] * r);
@@ -123,15 +160,15 @@ template <int KIND> __device__ __forceinline__ float eam_rho(const DevPot& P, fl
   }
 }
 
-template <int KIND>
+template <int KIND, int EXP = 0>
 __device__ __forceinline__ void eam_pair(const DevPot& P, float r2, float wsum, float& fscal,
                                          float& V) {
   const float ir = rsq(r2);
   if constexpr (KIND == kSuttonChen) {
     // suttonchen.cc:73-95, f = {e, a, n, m, c, -, -, -, -n e, m c e / 2}
     const float t = P.f[1] * ir;
-    const float tn = powk(t, P.ni, P.f[2]);
-    const float tm = powk(t, P.mi, P.f[3]);
+    const float tn = sc_pow_n<EXP>(P, t);
+    const float tm = sc_pow_m<EXP>(P, t);
     V = P.f[0] * tn;
     fscal = (ir * ir) * (P.f[8] * tn + P.f[9] * wsum * tm);
   } else if constexpr (KIND == kGupta) {
@@ -500,7 +537,7 @@ __device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, lo
 // check, metalpotential.cc:121-132), accumulated in FP64; writes
 // w_i = rho_i^{-1/2} (FP32) and rho_i, and, when evaluating, the embedding
 // energy F(rho_i) = -K sqrt(rho_i) into the caller's emb.
-template <int KIND, int LPA, bool EVAL, bool STAGED, bool PREFETCH = true>
+template <int KIND, int LPA, bool EVAL, bool STAGED, bool PREFETCH = true, int EXP = 0>
 __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                              float* w_out, double* rho_out, double& emb) {
   double rho = 0.0;
@@ -510,7 +547,7 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H,
     for_neighbours<LPA, STAGED, false, false, PREFETCH>(A, H, i, l, xi, yi, zi,
                                               [&](bool valid, int, float, double, double, double, double r2) {
       const bool in = valid && r2 < rc2;
-      const float v = eam_rho<KIND>(A.P, in ? (float)r2 : 1.0f);
+      const float v = eam_rho<KIND, EXP>(A.P, in ? (float)r2 : 1.0f);
       rho += in ? (double)v : 0.0;
     });
   }
@@ -526,7 +563,7 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H,
 
 // EAM pass 2: pair + many-body forces over the full list with AppliesTo
 // (metalpotential.cc:140-169), fused with the integrator tail.
-template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC, bool PREFETCH = true>
+template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC, bool PREFETCH = true, int EXP = 0>
 __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                                Tally& tal) {
   ForceSum F;
@@ -541,7 +578,7 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& 
       const bool in = valid && r2 < rc2 && (!SPC || A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
       const float fr2 = in ? (float)r2 : 1.0f;
       float fs, V;
-      eam_pair<KIND>(A.P, fr2, wi + wj, fs, V);
+      eam_pair<KIND, EXP>(A.P, fr2, wi + wj, fs, V);
       fs = in ? fs : 0.0f;
       F.add(fs, dx, dy, dz);
       if constexpr (EVAL)
@@ -566,22 +603,23 @@ static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
     write_partials(A, tal, 0.0);
 }
 
-template <int KIND, int LPA, bool EVAL>
+template <int KIND, int LPA, bool EVAL, int EXP = 0>
 static __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
   const long t = xcd_block() * kBlock + threadIdx.x;
   double emb = 0.0;
-  eam_rho_atom<KIND, LPA, EVAL, false>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out, rho_out, emb);
+  eam_rho_atom<KIND, LPA, EVAL, false, true, EXP>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out,
+                                                  rho_out, emb);
   if constexpr (EVAL) {
     Tally tal;
     write_partials(A, tal, emb);
   }
 }
 
-template <int KIND, int LPA, bool EVAL, bool SPC>
+template <int KIND, int LPA, bool EVAL, bool SPC, int EXP = 0>
 static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
   const long t = xcd_block() * kBlock + threadIdx.x;
   Tally tal;
-  eam_force_atom<KIND, LPA, EVAL, false, SPC>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, tal);
+  eam_force_atom<KIND, LPA, EVAL, false, SPC, true, EXP>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, tal);
   if constexpr (EVAL)
     write_partials(A, tal, 0.0);
 }
@@ -727,8 +765,8 @@ __device__ __forceinline__ void for_brick_atoms(const ForceArgs& A, const int2* 
 }
 
 template <int KIND, int LPA, bool EVAL, bool SPC>
-static __global__ __launch_bounds__(1024) void k_pair_force_bk(ForceArgs A, const int2* bhalo, int brick, int cap,
-                                                               int parts) {
+static __global__ __launch_bounds__(1024) void k_pair_force_bk(ForceArgs A, const int2* bhalo,
+                                                                                  int brick, int cap, int parts) {
   Tally tal;
   Stage S;
   for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, nullptr, S, [&](long i, int l, bool active) {
@@ -738,13 +776,14 @@ static __global__ __launch_bounds__(1024) void k_pair_force_bk(ForceArgs A, cons
     write_partials(A, tal, 0.0);
 }
 
-template <int KIND, int LPA, bool EVAL>
-static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const int2* bhalo, int brick, int cap,
-                                                            float* w_out, double* rho_out) {
+template <int KIND, int LPA, bool EVAL, int EXP = 0>
+static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const int2* bhalo,
+                                                                               int brick, int cap, float* w_out,
+                                                                               double* rho_out) {
   double emb = 0.0;
   Stage S;
   for_brick_atoms<LPA>(A, bhalo, brick, cap, 0, nullptr, S, [&](long i, int l, bool active) {
-    eam_rho_atom<KIND, LPA, EVAL, true>(A, S, i, l, active, w_out, rho_out, emb);
+    eam_rho_atom<KIND, LPA, EVAL, true, true, EXP>(A, S, i, l, active, w_out, rho_out, emb);
   });
   if constexpr (EVAL) {
     Tally tal;
@@ -752,13 +791,13 @@ static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const i
   }
 }
 
-template <int KIND, int LPA, bool EVAL, bool SPC>
-static __global__ __launch_bounds__(1024) void k_eam_force_bk(ForceArgs A, const int2* bhalo, int brick, int cap,
-                                                              int parts) {
+template <int KIND, int LPA, bool EVAL, bool SPC, int EXP = 0>
+static __global__ __launch_bounds__(1024) void k_eam_force_bk(ForceArgs A, const int2* bhalo,
+                                                                                 int brick, int cap, int parts) {
   Tally tal;
   Stage S;
   for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, A.w_rho, S, [&](long i, int l, bool active) {
-    eam_force_atom<KIND, LPA, EVAL, true, SPC>(A, S, i, l, active, tal);
+    eam_force_atom<KIND, LPA, EVAL, true, SPC, true, EXP>(A, S, i, l, active, tal);
   });
   if constexpr (EVAL)
     write_partials(A, tal, 0.0);
