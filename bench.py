@@ -38,6 +38,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+
+# Algorithmic flops of the force pass: per listed pair the FP64 displacement and
+# r^2 (3 sub, 3 mul, 2 add); per pair inside the cutoff the FP64 force sum
+# (3 FMA = 6 flops) and the FP32 pair function (LJ: rcp, 3 mul for s6, 1 for
+# s12, 4 for the force scalar = 9; Sutton-Chen force pass: rsq, 1 mul for t,
+# 5 for t^9 and t^6 at the common exponents, 6 for the force scalar = 13).
+FP64_PER_LISTED_PAIR = 8
+FP64_PER_CUT_PAIR = 6
+FP32_PER_CUT_PAIR = {"lennardjones": 9, "suttonchen": 13}
 
 WORKLOADS = {
     "sc_cu_131k": dict(
@@ -45,6 +56,7 @@ WORKLOADS = {
         nx=32, a=3.61, z=29, T=300.0, noise=0.01, dt=2.0, seed=2024,
         pot=("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22"),
         cm=("verletlist", "cutoff 7.22 maxneighbors 400 each 5 skin 0.3"),
+        rc=7.22, skin=0.3,
         metal=True,
     ),
     "lj_4m": dict(
@@ -52,6 +64,7 @@ WORKLOADS = {
         nx=100, a=1.679596 * 3.41, z=18, T=0.72 * 0.0103 / 8.6173422e-05, noise=0.0, dt=0.005 * 2161.98, seed=7,
         pot=("lennardjones", f"sigma 3.41 epsilon 0.0103 cutoff {2.5 * 3.41}"),
         cm=("verletlist", f"cutoff {2.5 * 3.41} maxneighbors 400 each 5 skin {0.3 * 3.41}"),
+        rc=2.5 * 3.41, skin=0.3 * 3.41,
         metal=False,
     ),
 }
@@ -393,6 +406,19 @@ def main():
     }
     if w["metal"] and r_n:
         roofline["rho_pass_avg_us"] = round(r_ms / r_n * 1e3, 2)
+    # VALU rates of the same launch from the algorithmic flop counts above; the
+    # pairs inside the cutoff are the listed ones scaled by (rc / r_list)^3
+    rc, rlist = w["rc"], w["rc"] + w["skin"]
+    ncut = nbar * (rc / rlist) ** 3
+    f64 = n * (nbar * FP64_PER_LISTED_PAIR + ncut * FP64_PER_CUT_PAIR) / avg_force_s / 1e12
+    f32 = n * ncut * FP32_PER_CUT_PAIR[w["pot"][0]] / avg_force_s / 1e12
+    valu = {
+        "fp32_tflops": round(f32, 3),
+        "fp32_frac": round(f32 / FP32_PEAK_TFLOPS, 4),
+        "fp64_tflops": round(f64, 3),
+        "fp64_frac": round(f64 / FP64_PEAK_TFLOPS, 4),
+        "pairs_in_cutoff_per_atom": round(ncut, 2),
+    }
     step_bytes = (160 + 8 * nbar) if w["metal"] else (144 + 4 * nbar)
     out = {
         "metric": "atom-steps/s plain vs batched/served at <=1e-5 rel force err; p50/p99 latency",
@@ -410,6 +436,7 @@ def main():
         "config": {"workload": args.workload, "atoms": n, "path": "plain single-system", "desc": w["desc"]},
         "roofline": roofline,
         "step_hbm_fraction": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
+        "valu": valu,
         "list_builds": st1["builds"] - st0["builds"],
     }
     s.close()

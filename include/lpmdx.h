@@ -89,7 +89,13 @@ LPMDX_API int lpmdx_system_get_state(lpmdx_system* sys, double* x, double* v, do
  *   "verletlist": keywords cutoff extcutoff maxneighbors each skin
  *                 (plugins/verletlist.cc:12-29; `skin`, when given, makes the
  *                 list radius cutoff+skin and rebuilds only when the skin is
- *                 exceeded).
+ *                 exceeded);
+ *   "minimumimage": keyword cutoff (plugins/minimumimage.cc:12-19; 0 means the
+ *                 largest half cell edge), each potential listed with its own
+ *                 cutoff, one image per pair; `skin` as for verletlist;
+ *   "lcbinary":   keywords cutoff nx ny nz mode (plugins/lcbinary.cc:9-33):
+ *                 linked subcells of at most one atom, grown by two per axis
+ *                 while two atoms share one.
  * Runs one UpdateCell, like SetCellManager does. */
 LPMDX_API int lpmdx_set_cellmanager(lpmdx_system* sys, const char* name, const char* args);
 

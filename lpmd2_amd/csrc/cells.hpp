@@ -140,6 +140,16 @@ static __global__ __launch_bounds__(kBlock) void k_cell_begin(const uint32_t* __
     cb[c] = (int)k;
 }
 
+// Largest number of atoms in one cell (lcbinary wants at most one).
+static __global__ __launch_bounds__(kBlock) void k_cell_occupancy(const int* __restrict__ cb, int ncell,
+                                                           unsigned int* __restrict__ out) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  unsigned int m = c < ncell ? (unsigned int)(cb[c + 1] - cb[c]) : 0u;
+  m = wave_max(m);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m > 0)
+    atomicMax(out, m);
+}
+
 // ---------------------------------------------------------------------------
 // Brick halos.  On a bricked grid a force workgroup serves one brick of
 // brick^3 cells.  Its atoms' neighbours lie in the (brick + 2)^3 block of

@@ -118,7 +118,8 @@ struct Ctrl {
   unsigned long long sum_count;   // total list entries of the last build (8-byte aligned)
   unsigned int halo_max;          // largest brick halo of the last build (atoms)
   unsigned int brick_max;         // most atoms in one brick of the last build
-  unsigned int pad[10];
+  unsigned int cell_max;          // most atoms in one cell (lcbinary)
+  unsigned int pad[9];
 };
 static_assert(offsetof(Ctrl, sum_count) % 8 == 0, "64-bit counter must be 8-byte aligned");
 static_assert(offsetof(Ctrl, sum_count) == offsetof(Ctrl, max_count) + 8, "counter block layout");

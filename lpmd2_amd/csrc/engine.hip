@@ -130,6 +130,7 @@ struct lpmdx_system {
   int cap = 0;
   int lpa = 1, lpa_req = 0;
   double rl = 0.0; // list radius
+  double rl_min = 0.0; // a list radius an export asked for (minimumimage, lcbinary)
   // EAM
   DBuf<float> w_rho;
   DBuf<double> rho;
@@ -239,9 +240,27 @@ struct lpmdx_system {
     return half_list_rcut();
   }
 
+  // Cell managers whose lists come from the reference's linked-cell stencil.
+  bool stencil_grid() const { return cm.kind == kCmLinkedCell || cm.kind == kCmLcBinary; }
+  // Managers that rebuild on every UpdateCell: linkedcell and lcbinary (their
+  // UpdateCell re-bins), minimumimage without a skin (it lists afresh on every
+  // request).
+  bool rebuilds_every_update() const {
+    return stencil_grid() || (cm.kind == kCmMinImage && cm.skin < 0.0);
+  }
+
   double list_radius() const {
     if (cm.kind == kCmLinkedCell)
       return std::max(half_list_rcut(), full_list_rcut());
+    if (cm.kind == kCmMinImage || cm.kind == kCmLcBinary) {
+      // both serve each potential with its own cutoff (minimumimage.cc:74-100,
+      // lcbinary.cc:264-306); an export may ask for a wider radius (rl_min)
+      double r = pots.empty() ? cm.cutoff : 0.0;
+      for (auto& p : pots)
+        r = std::max(r, p->cutoff);
+      r = std::max(r, rl_min);
+      return (cm.kind == kCmMinImage && cm.skin >= 0.0) ? r + cm.skin : r;
+    }
     return cm.skin >= 0.0 ? cm.cutoff + cm.skin : cm.extcut;
   }
 
@@ -292,6 +311,25 @@ struct lpmdx_system {
       grid.ny = cm.ny;
       grid.nz = cm.nz;
       grid.side = cm.side;
+    } else if (cm.kind == kCmLcBinary) {
+      if (!cm.grid_fixed) {
+        // lcbinary.cc:88-145 (auto mode picks n once; the later growth keeps it)
+        if (cm.mode_auto) {
+          cm.nx = lcb_auto_n(edge_module(box.L[0]), cm.cutoff);
+          cm.ny = lcb_auto_n(edge_module(box.L[1]), cm.cutoff);
+          cm.nz = lcb_auto_n(edge_module(box.L[2]), cm.cutoff);
+          cm.mode_auto = false;
+        }
+        if (cm.nx <= 0 || cm.ny <= 0 || cm.nz <= 0)
+          fail(LPMDX_EINVAL, "lcbinary: nx, ny and nz must be positive");
+        lcb_side();
+        cm.grid_fixed = true;
+      }
+      grid.mode = kLinkedCell;
+      grid.nx = cm.nx;
+      grid.ny = cm.ny;
+      grid.nz = cm.nz;
+      grid.side = cm.side;
     } else {
       // cells no smaller than the list radius (with a margin for rounding)
       double r = rl * (1.0 + 1e-9);
@@ -321,6 +359,20 @@ struct lpmdx_system {
       nbrick = (long)grid.nbx * grid.nby * grid.nbz;
       grid.ncell = (int)(nbrick * B * B * B);
     }
+  }
+
+  // lcbinary.cc:150-157: stencil half-width from the smallest subcell edge.
+  // The reference wraps a stencil offset once, so a stencil wider than the
+  // grid would index outside it there; such a grid is refused.
+  void lcb_side() {
+    double d = edge_module(box.L[0]) / double(cm.nx);
+    if (edge_module(box.L[1]) / double(cm.ny) < d)
+      d = edge_module(box.L[1]) / double(cm.ny);
+    if (edge_module(box.L[2]) / double(cm.nz) < d)
+      d = edge_module(box.L[2]) / double(cm.nz);
+    cm.side = int(ceil((cm.cutoff / d) - 0.5));
+    if (cm.side >= cm.nx || cm.side >= cm.ny || cm.side >= cm.nz)
+      fail(LPMDX_EINVAL, "lcbinary: the stencil is wider than the subcell grid");
   }
 
   // Brick edge (cells) for the staged force kernels, or 0 for global-index
@@ -362,7 +414,7 @@ struct lpmdx_system {
   // set of potentials that would choose another brick edge, or stage more
   // (rho^{-1/2}, species) than fits, gets new lists.
   void lists_follow_potentials() {
-    if (!list_built || cm.kind == kCmLinkedCell)
+    if (!list_built || stencil_grid())
       return;
     if (choose_brick() != grid.brick || (staged && stage_bytes(halo_cap, stage_parts()) > (size_t)kStageLdsLimit))
       list_built = false;
@@ -470,7 +522,7 @@ struct lpmdx_system {
       double vol = box.L[0] * box.L[1] * box.L[2];
       double rr = std::isfinite(rl) ? rl : std::cbrt(vol);
       double est = (double)n / vol * 4.18879 * rr * rr * rr;
-      if (cm.kind == kCmLinkedCell)
+      if (stencil_grid())
         est = std::max(est, (double)n / vol * pow((2 * grid.side + 1) * box.L[0] / grid.nx, 3));
       cap = (int)std::min<double>(n * 8.0 + 64, est * 1.3 + 32);
     }
@@ -559,6 +611,16 @@ struct lpmdx_system {
     setup_grid();
     hipEvent_t e0 = prof_begin();
     sort_atoms();
+    // lcbinary.cc:244-276: a subcell that holds two atoms grows the grid by two
+    // subcells per axis, and the atoms are binned again
+    while (cm.kind == kCmLcBinary && max_cell_occupancy() > 1) {
+      cm.nx += 2;
+      cm.ny += 2;
+      cm.nz += 2;
+      lcb_side();
+      setup_grid();
+      sort_atoms();
+    }
     prof_end(kProfBuild, e0);
     build_list();
     // verletlist.cc:96-97: a list may hold at most maxneighbors-1 entries
@@ -567,6 +629,16 @@ struct lpmdx_system {
     hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
     list_built = true;
     n_builds++;
+  }
+
+  unsigned int max_cell_occupancy() {
+    hipchk(hipMemsetAsync(&ctrl.p->cell_max, 0, sizeof(unsigned), st), "memset");
+    k_cell_occupancy<<<blocks_for(grid.ncell), kBlock, 0, st>>>(cb.p, grid.ncell, &ctrl.p->cell_max);
+    launch_check();
+    hipchk(hipMemcpyAsync(&hctrl->cell_max, &ctrl.p->cell_max, sizeof(unsigned), hipMemcpyDeviceToHost, st),
+           "memcpy");
+    sync();
+    return hctrl->cell_max;
   }
 
   // exact displacement since the build, for calls through the plugin interface
@@ -594,7 +666,13 @@ struct lpmdx_system {
     if (cm.kind == kNone)
       fail(LPMDX_ESTATE, "cellmanager not set");
     undrift();
-    if (cm.kind == kCmLinkedCell) {
+    if (cm.kind == kCmMinImage && fabs(cm.cutoff) < 1E-10) {
+      // minimumimage.cc:63-69: no cutoff given: the largest half cell edge
+      for (int q = 0; q < 3; ++q)
+        if (0.5 * edge_module(box.L[q]) > cm.cutoff)
+          cm.cutoff = 0.5 * edge_module(box.L[q]);
+    }
+    if (rebuilds_every_update()) {
       rebuild();
     } else {
       // rebuilt whenever the list could miss a pair inside the filter radius;
@@ -782,6 +860,10 @@ struct lpmdx_system {
       // metal potentials take the list as it comes (metalpotential.cc:140-145)
       rc2_force = P.metal() ? h : std::min(h, own);
       rc2_rho = f;
+    } else if (cm.kind == kCmMinImage || cm.kind == kCmLcBinary) {
+      // every request lists with the potential's own cutoff
+      rc2_force = own;
+      rc2_rho = own;
     } else {
       // the stored list may hold a few pairs just beyond its radius (the build
       // admits with a small FP32 margin); a potential without a finite cutoff
@@ -1018,7 +1100,7 @@ struct lpmdx_system {
         m = std::min(m, obs_every - (done % obs_every));
       m = std::min<long>(m, kMaxChunk);
       bool need = false;
-      if (cm.kind == kCmLinkedCell) {
+      if (rebuilds_every_update()) {
         m = 1;
         need = true; // UpdateCell rebuilds every call
       } else {
@@ -1036,7 +1118,7 @@ struct lpmdx_system {
         rebuild();
         disp_cur = 0.0;
       }
-      const bool snapshot = cm.kind != kCmLinkedCell;
+      const bool snapshot = !rebuilds_every_update();
       if (snapshot) {
         xs.alloc(3 * n);
         vs.alloc(3 * n);
@@ -1058,7 +1140,7 @@ struct lpmdx_system {
       double d = disp_cur, growth = 0.0;
       long bad = -1;
       for (long s = 0; s < m; ++s) {
-        if (cm.kind != kCmLinkedCell && d > b) {
+        if (!rebuilds_every_update() && d > b) {
           bad = s;
           break;
         }
@@ -1319,6 +1401,27 @@ int lpmdx_set_cellmanager(lpmdx_system* S, const char* name, const char* args) {
       }
       if (fabs(c.cutoff) < 1E-10)
         fail(LPMDX_EINVAL, "verletlist: No cutoff was defined");
+    } else if (nm == "minimumimage") {
+      // plugins/minimumimage.cc:12-19; `skin` is this engine's extension, as
+      // for verletlist (without it the lists are rebuilt on every UpdateCell)
+      Keywords k(args, {{"cutoff", "0.0"}});
+      c.kind = kCmMinImage;
+      c.cutoff = k.d("cutoff");
+      if (k.has("skin")) {
+        c.skin = k.d("skin");
+        if (!(c.skin >= 0.0))
+          fail(LPMDX_EINVAL, "minimumimage: skin must be non-negative");
+      }
+    } else if (nm == "lcbinary") {
+      // plugins/lcbinary.cc:9-33: automatic subdivisions when one is 0 or with mode auto
+      Keywords k(args, {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}});
+      c.kind = kCmLcBinary;
+      c.cutoff = k.d("cutoff");
+      c.nx = k.i("nx");
+      c.ny = k.i("ny");
+      c.nz = k.i("nz");
+      const std::string mode = k.s("mode");
+      c.mode_auto = c.nx == 0 || c.ny == 0 || c.nz == 0 || mode == "auto" || mode == "AUTO";
     } else {
       fail(LPMDX_EINVAL, "unknown cell manager '" + nm + "'");
     }
@@ -1515,10 +1618,19 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
       *total = 0;
       return;
     }
+    if (S->cm.kind == kCmMinImage && rcut < 1.0E-10)
+      fail(LPMDX_EINVAL, "MinimumImage cutoff equals zero..."); // minimumimage.cc:76-77
     if (S->cm.kind == kCmLinkedCell && rcut > S->rl) {
       // the cached lists are built on the first request's rcut (linkedcell.cc:315-354)
       S->rl = rcut;
       S->build_list();
+    }
+    if ((S->cm.kind == kCmMinImage || S->cm.kind == kCmLcBinary) && rcut > S->rl) {
+      // these list afresh with the caller's radius: lists of that radius (the
+      // next UpdateCell returns to the potentials' radius)
+      S->rl_min = rcut;
+      S->rebuild();
+      S->rl_min = 0.0;
     }
     std::vector<int> perm(n), cntv(n), nb((size_t)S->cap * S->stride);
     std::vector<double> x(3 * n);
@@ -1558,10 +1670,10 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
       for (int c = 0; c < cntv[i]; ++c) {
         const long jj = atom_of(i, nb[list_index(i, c, S->lpa, S->cap)]);
         const long oj = perm[jj];
-        if (!full && S->cm.kind == kCmLinkedCell && oj > o)
-          continue; // linkedcell half list keeps j < i (linkedcell.cc:323)
-        if (!full && S->cm.kind == kCmVerlet && oj < o)
-          continue; // verletlist half list keeps j > i (verletlist.cc:157)
+        if (!full && S->stencil_grid() && oj > o)
+          continue; // linkedcell / lcbinary half lists keep j < i (linkedcell.cc:323, lcbinary.cc:291)
+        if (!full && (S->cm.kind == kCmVerlet || S->cm.kind == kCmMinImage) && oj < o)
+          continue; // verletlist / minimumimage half lists keep j > i (verletlist.cc:157, minimumimage.cc:90)
         double d[3];
         for (int q = 0; q < 3; ++q) {
           d[q] = x[3 * jj + q] - x[3 * i + q];
@@ -1574,7 +1686,7 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
         if (rr < rc2)
           row.emplace_back(oj, rr);
       }
-      if (S->cm.kind == kCmVerlet)
+      if (S->cm.kind == kCmVerlet || S->cm.kind == kCmMinImage)
         std::stable_sort(row.begin(), row.end(),
                          [](const std::pair<long, double>& p, const std::pair<long, double>& q) {
                            return p.first < q.first;

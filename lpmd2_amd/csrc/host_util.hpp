@@ -290,12 +290,12 @@ inline DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) 
 // ---------------------------------------------------------------------------
 // cell managers
 
-enum CmKind { kNone = 0, kCmLinkedCell = 1, kCmVerlet = 2 };
+enum CmKind { kNone = 0, kCmLinkedCell = 1, kCmVerlet = 2, kCmMinImage = 3, kCmLcBinary = 4 };
 
 struct CellMgr {
   int kind = kNone;
   double cutoff = 0.0;
-  // linkedcell (plugins/linkedcell.cc:17-41)
+  // linkedcell (plugins/linkedcell.cc:17-41), lcbinary (plugins/lcbinary.cc:9-33)
   int nx = 0, ny = 0, nz = 0, side = 0;
   bool mode_auto = false, warn_outside = true, grid_fixed = false;
   // verletlist (plugins/verletlist.cc:12-29)
@@ -321,6 +321,26 @@ inline int lc_auto_n(double edge, double cutoff) {
   }
   if ((n % 2) != 0)
     n--;
+  return n;
+}
+
+// lcbinary.cc:88-145: the same stepping toward cutoff/5, an odd n rounded up
+inline int lcb_auto_n(double edge, double cutoff) {
+  double fnn = cutoff / 5.0;
+  int n = 1;
+  while (true) {
+    double previo = fabs(edge / n - fnn);
+    n++;
+    double actual = fabs(edge / n - fnn);
+    if (actual < 1E-3)
+      break;
+    else if (actual > previo) {
+      n--;
+      break;
+    }
+  }
+  if ((n % 2) != 0)
+    n++;
   return n;
 }
 

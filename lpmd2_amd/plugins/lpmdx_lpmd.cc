@@ -42,6 +42,8 @@ const std::map<std::string, std::vector<Keyword>>& Keywords() {
       {"linkedcell",
        {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}, {"warn-outside", "true"}}},
       {"verletlist", {{"cutoff", "0.0"}, {"extcutoff", "0.0"}, {"maxneighbors", "80"}, {"each", "5"}, {"skin", "0"}}},
+      {"minimumimage", {{"cutoff", "0.0"}, {"skin", "0"}}},
+      {"lcbinary", {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}}},
       {"velocityverlet", {{"dt", "1.0"}, {"start", "1"}}},
   };
   return m;
@@ -101,7 +103,9 @@ DeviceCellManager::~DeviceCellManager() {
 void DeviceCellManager::ShowHelp() const {
   std::cout << " Module Name        = " << kind_ << " (MI355X engine)\n"
             << "      Cell binning and neighbour lists on the GPU; keywords as the CPU plugin"
-            << (kind_ == "verletlist" ? ", plus skin (list radius cutoff+skin, rebuilt when exceeded)" : "")
+            << ((kind_ == "verletlist" || kind_ == "minimumimage")
+                    ? ", plus skin (list radius cutoff+skin, rebuilt when exceeded)"
+                    : "")
             << ".\n";
 }
 
@@ -171,6 +175,13 @@ void DeviceCellManager::UpdateCell(lpmd::Configuration& conf) {
   if (existed)
     check(lpmdx_update_cell(sys_), kind_);
   lists_ok_ = false;
+  if (kind_ == "minimumimage" && fabs(cutoff_) < 1E-10) {
+    // minimumimage.cc:63-69: without a cutoff, the largest half cell edge
+    const lpmd::BasicCell& cell = conf.Cell();
+    for (int q = 0; q < 3; ++q)
+      if (0.5 * cell[q].Module() > cutoff_)
+        cutoff_ = 0.5 * cell[q].Module();
+  }
 }
 
 void DeviceCellManager::UpdateAtom(lpmd::Configuration&, long) {
@@ -272,7 +283,7 @@ void DevicePotential::UpdateForces(lpmd::Configuration& conf) {
   auto* cm = dynamic_cast<DeviceCellManager*>(&conf.GetCellManager());
   if (!cm)
     throw lpmd::RuntimeError(kind_ +
-                             " (MI355X): the cell manager must be the MI355X linkedcell or verletlist plugin");
+                             " (MI355X): the cell manager must be an MI355X cell manager plugin (linkedcell, verletlist, minimumimage, lcbinary)");
   lpmdx_system* sys = cm->Bind(conf, true);
   const int k = cm->IndexOf(*this);
   lpmd::BasicParticleSet& atoms = conf.Atoms();
@@ -400,7 +411,7 @@ lpmd::Plugin* Create(const std::string& name, const std::string& args) {
   for (const char* p : pots)
     if (name == p)
       return new DevicePotential(name, args);
-  if (name == "linkedcell" || name == "verletlist")
+  if (name == "linkedcell" || name == "verletlist" || name == "minimumimage" || name == "lcbinary")
     return new DeviceCellManager(name, args);
   if (name == "velocityverlet")
     return new DeviceVelocityVerlet(args);

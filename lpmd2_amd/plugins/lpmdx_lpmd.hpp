@@ -3,8 +3,10 @@
 // These classes implement the reference's plugin interfaces and forward to the
 // engine's C ABI (include/lpmdx.h):
 //   DeviceCellManager     lpmd::CellManager (liblpmd/lpmd/cellmanager.h:15-27) as
-//                         "linkedcell" (plugins/linkedcell.cc:17-41) and
-//                         "verletlist" (plugins/verletlist.cc:12-29)
+//                         "linkedcell" (plugins/linkedcell.cc:17-41),
+//                         "verletlist" (plugins/verletlist.cc:12-29),
+//                         "minimumimage" (plugins/minimumimage.cc:12-19) and
+//                         "lcbinary" (plugins/lcbinary.cc:9-33)
 //   DevicePotential       lpmd::Potential (liblpmd/lpmd/potential.h:11-36) as
 //                         lennardjones fastlj morse buckingham suttonchen
 //                         finnissinclair gupta (plugins/<name>.cc)

@@ -121,6 +121,8 @@ struct orc_cm {
   long* indexc;
   orc_nlist* cache_half;
   orc_nlist* cache_full;
+  /* lcbinary: the atom of each subcell, or -1 */
+  long* occ;
   /* verletlist */
   long maxneighbors, each, step_count, old_size;
   double extcut;
@@ -163,6 +165,18 @@ orc_cm* orc_cm_verletlist(double cutoff, long maxneighbors, long each) {
   return cm;
 }
 
+orc_cm* orc_cm_lcbinary(double cutoff, int nx, int ny, int nz, int mode_auto) {
+  orc_cm* cm = calloc(1, sizeof *cm);
+  cm->kind = ORC_CM_LCBINARY;
+  cm->cutoff = cutoff;
+  cm->nx = nx;
+  cm->ny = ny;
+  cm->nz = nz;
+  /* lcbinary.cc:22-28: automatic when a subdivision is 0 or with mode auto */
+  cm->mode_auto = mode_auto || nx == 0 || ny == 0 || nz == 0;
+  return cm;
+}
+
 orc_cm* orc_cm_minimage(double cutoff) {
   orc_cm* cm = calloc(1, sizeof *cm);
   cm->kind = ORC_CM_MINIMAGE;
@@ -193,6 +207,7 @@ void orc_cm_free(orc_cm* cm) {
   free(cm->indexc);
   free(cm->nv);
   free(cm->vlist);
+  free(cm->occ);
   free(cm);
 }
 
@@ -338,6 +353,149 @@ static int lc_list(orc_cm* cm, const orc_conf* c, long i, int full, double rcut,
   return 0;
 }
 
+/* lcbinary.cc:88-145: like linkedcell's choice but toward cutoff/5, and an odd n is rounded up. */
+int orc_lcb_auto_n(double edge, double cutoff) {
+  double fnn = cutoff / 5.0;
+  int n = 1;
+  while (1) {
+    double previo = fabs(edge / n - fnn);
+    n++;
+    double actual = fabs(edge / n - fnn);
+    if (actual < 1E-3)
+      break;
+    else if (actual > previo) {
+      n--;
+      break;
+    }
+  }
+  if ((n % 2) != 0)
+    n++;
+  return n;
+}
+
+/* Subcell of position x as lcbinary.cc:198-212 bins it: floor(n * fraction),
+ * one wrap of n - 1 < i.  A negative index (an atom below the box) indexes
+ * outside the reference's tables; it is an error here. */
+static long lcb_cell(const orc_cm* cm, const orc_conf* c, const double* x) {
+  int i = (int)floor(cm->nx * (x[0] / c->L[0]));
+  int j = (int)floor(cm->ny * (x[1] / c->L[1]));
+  int k = (int)floor(cm->nz * (x[2] / c->L[2]));
+  if (i > cm->nx - 1)
+    i -= cm->nx;
+  if (j > cm->ny - 1)
+    j -= cm->ny;
+  if (k > cm->nz - 1)
+    k -= cm->nz;
+  if (i < 0 || j < 0 || k < 0 || i > cm->nx - 1 || j > cm->ny - 1 || k > cm->nz - 1)
+    return -1;
+  return (long)k * (cm->nx * cm->ny) + (long)j * cm->nx + i;
+}
+
+/* LCBinary::UpdateCell, plugins/lcbinary.cc:85-241: the stencil table of the
+ * current subdivision (offsets wrapped once), then every atom into its
+ * subcell; a subcell reached twice grows every axis by two subcells and
+ * everything is done again. */
+static int lcb_update(orc_cm* cm, const orc_conf* c) {
+  if (cm->mode_auto) {
+    cm->nx = orc_lcb_auto_n(edge_module(c->L[0]), cm->cutoff);
+    cm->ny = orc_lcb_auto_n(edge_module(c->L[1]), cm->cutoff);
+    cm->nz = orc_lcb_auto_n(edge_module(c->L[2]), cm->cutoff);
+    cm->mode_auto = 0;
+  }
+  for (;;) {
+    if (cm->nx <= 0 || cm->ny <= 0 || cm->nz <= 0)
+      return fail("lcbinary: a subdivision count is zero");
+    const long ncell = (long)cm->nx * cm->ny * cm->nz;
+    if (cm->subcell == NULL) {
+      double d = edge_module(c->L[0]) / (double)cm->nx;
+      if (edge_module(c->L[1]) / (double)cm->ny < d)
+        d = edge_module(c->L[1]) / (double)cm->ny;
+      if (edge_module(c->L[2]) / (double)cm->nz < d)
+        d = edge_module(c->L[2]) / (double)cm->nz;
+      const int side = (int)ceil((cm->cutoff / d) - 0.5);
+      if (side >= cm->nx || side >= cm->ny || side >= cm->nz)
+        return fail("lcbinary: the stencil is wider than the subcell grid");
+      cm->side = side;
+      cm->cells_inside = (long)(2 * side + 1) * (2 * side + 1) * (2 * side + 1);
+      cm->subcell = malloc(ncell * cm->cells_inside * sizeof(int));
+      if (!cm->subcell)
+        return fail("lcbinary: stencil table allocation failed");
+      for (int k = 0; k < cm->nz; ++k)
+        for (int j = 0; j < cm->ny; ++j)
+          for (int i = 0; i < cm->nx; ++i) {
+            long r = 0;
+            long q = (long)k * (cm->nx * cm->ny) + (long)j * cm->nx + i;
+            for (int dz = -side; dz <= side; ++dz)
+              for (int dy = -side; dy <= side; ++dy)
+                for (int dx = -side; dx <= side; ++dx) {
+                  int z0 = i + dx, z1 = j + dy, z2 = k + dz;
+                  if (z0 < 0)
+                    z0 += cm->nx;
+                  else if (z0 > cm->nx - 1)
+                    z0 -= cm->nx;
+                  if (z1 < 0)
+                    z1 += cm->ny;
+                  else if (z1 > cm->ny - 1)
+                    z1 -= cm->ny;
+                  if (z2 < 0)
+                    z2 += cm->nz;
+                  else if (z2 > cm->nz - 1)
+                    z2 -= cm->nz;
+                  long p = (long)z2 * (cm->nx * cm->ny) + (long)z1 * cm->nx + z0;
+                  cm->subcell[q * cm->cells_inside + (r++)] = (int)p;
+                }
+          }
+    }
+    if (cm->occ == NULL)
+      cm->occ = malloc(ncell * sizeof(long));
+    for (long q = 0; q < ncell; ++q)
+      cm->occ[q] = -1;
+    int grow = 0;
+    for (long r = 0; r < c->n; ++r) {
+      long q = lcb_cell(cm, c, c->x + 3 * r);
+      if (q < 0)
+        return fail("lcbinary: an atom lies below the cell");
+      if (cm->occ[q] != -1)
+        grow = 1;
+      cm->occ[q] = r;
+    }
+    if (!grow)
+      return 0;
+    cm->nx += 2;
+    cm->ny += 2;
+    cm->nz += 2;
+    free(cm->subcell);
+    cm->subcell = NULL;
+    free(cm->occ);
+    cm->occ = NULL;
+  }
+}
+
+/* LCBinary::BuildNeighborList, plugins/lcbinary.cc:264-306: the occupied
+ * subcells of the stencil in stencil order (no cache); half keeps j < i. */
+static int lcb_list(orc_cm* cm, const orc_conf* c, long i, int full, double rcut, orc_nlist* out) {
+  nl_clear(out);
+  out->i = i;
+  const double* xi = c->x + 3 * i;
+  const long cind = lcb_cell(cm, c, xi);
+  if (cind < 0)
+    return fail("lcbinary: an atom lies below the cell");
+  for (long s = 0; s < cm->cells_inside; ++s) {
+    const long z = cm->occ[cm->subcell[cind * cm->cells_inside + s]];
+    if (z < 0 || z == i)
+      continue;
+    if (z > i && !full)
+      continue;
+    double rij[3];
+    orc_displacement(c->L, c->per, xi, c->x + 3 * z, rij);
+    const double r2 = sqmod(rij);
+    if (r2 < rcut * rcut)
+      if (nl_push(out, z, rij, r2))
+        return -1;
+  }
+  return 0;
+}
+
 /* VerletListCellManager::UpdateVerletList, plugins/verletlist.cc:67-104 */
 static int vl_rebuild(orc_cm* cm, const orc_conf* c) {
   if (fabs(cm->cutoff) < 1E-10)
@@ -422,6 +580,8 @@ int orc_cm_update(orc_cm* cm, const orc_conf* c) {
     return lc_update(cm, c);
   case ORC_CM_VERLETLIST:
     return vl_update(cm, c);
+  case ORC_CM_LCBINARY:
+    return lcb_update(cm, c);
   default:
     return 0;
   }
@@ -433,6 +593,8 @@ int orc_cm_list(orc_cm* cm, const orc_conf* c, long i, int full, double rcut, or
     return lc_list(cm, c, i, full, rcut, out);
   case ORC_CM_VERLETLIST:
     return vl_list(cm, c, i, full, rcut, out);
+  case ORC_CM_LCBINARY:
+    return lcb_list(cm, c, i, full, rcut, out);
   default:
     return mi_list(c, i, full, rcut, out);
   }

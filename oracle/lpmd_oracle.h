@@ -44,11 +44,13 @@ typedef struct {
   double* rij; /* 3*size */
 } orc_nlist;
 
-enum { ORC_CM_LINKEDCELL = 1, ORC_CM_VERLETLIST = 2, ORC_CM_MINIMAGE = 3 };
+enum { ORC_CM_LINKEDCELL = 1, ORC_CM_VERLETLIST = 2, ORC_CM_MINIMAGE = 3, ORC_CM_LCBINARY = 4 };
 
 typedef struct orc_cm orc_cm;
 
 orc_cm* orc_cm_linkedcell(double cutoff, int nx, int ny, int nz, int mode_auto);
+orc_cm* orc_cm_lcbinary(double cutoff, int nx, int ny, int nz, int mode_auto);
+int orc_lcb_auto_n(double edge, double cutoff);
 orc_cm* orc_cm_verletlist(double cutoff, long maxneighbors, long each);
 orc_cm* orc_cm_minimage(double cutoff);
 void orc_cm_free(orc_cm* cm);

@@ -97,6 +97,10 @@ def lib():
         L.orc_cm_verletlist.argtypes = [d, l, l]
         L.orc_cm_minimage.restype = vp
         L.orc_cm_minimage.argtypes = [d]
+        L.orc_cm_lcbinary.restype = vp
+        L.orc_cm_lcbinary.argtypes = [d, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_lcb_auto_n.restype = C.c_int
+        L.orc_lcb_auto_n.argtypes = [d, d]
         L.orc_cm_free.argtypes = [vp]
         L.orc_cm_update.argtypes = [vp, vp]
         L.orc_cm_grid.argtypes = [vp, vp]
@@ -177,7 +181,7 @@ class Conf:
 
 
 class CellManager:
-    """linkedcell / verletlist / minimum-image neighbour provider of the oracle."""
+    """linkedcell / verletlist / minimumimage / lcbinary neighbour provider of the oracle."""
 
     def __init__(self, kind: str, cutoff: float, nx=0, ny=0, nz=0, mode="auto", maxneighbors=80, each=5):
         L = lib()
@@ -188,6 +192,8 @@ class CellManager:
             self.h = L.orc_cm_verletlist(cutoff, maxneighbors, each)
         elif kind == "minimumimage":
             self.h = L.orc_cm_minimage(cutoff)
+        elif kind == "lcbinary":
+            self.h = L.orc_cm_lcbinary(cutoff, nx, ny, nz, 1 if mode in ("auto", "AUTO") else 0)
         else:
             raise ValueError(kind)
 
@@ -347,3 +353,8 @@ def momentum(conf: Conf):
 
 def lc_auto_n(edge: float, cutoff: float) -> int:
     return lib().orc_lc_auto_n(edge, cutoff)
+
+
+def lcb_auto_n(edge: float, cutoff: float) -> int:
+    """lcbinary's automatic subdivision count (plugins/lcbinary.cc:88-145)."""
+    return lib().orc_lcb_auto_n(edge, cutoff)

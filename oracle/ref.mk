@@ -24,7 +24,7 @@ CORE_SRCS := \
   liblpmd/simulation/integrator.cc liblpmd/simulation/twostepintegrator.cc \
   liblpmd/simulation/simulation.cc liblpmd/simulation/systemmodifier.cc
 
-PLUGINS := linkedcell verletlist lennardjones fastlj morse buckingham \
+PLUGINS := linkedcell verletlist minimumimage lcbinary lennardjones fastlj morse buckingham \
            suttonchen finnissinclair gupta crystal3d velocityverlet
 
 CORE_OBJS := $(patsubst %.cc,$(OUT)/obj/%.o,$(CORE_SRCS))

@@ -6,7 +6,8 @@ first (a partial build of the reference's plugins, see oracle/ref.mk), then
     python tests/golden/gen_golden.py
 
 Every fixture holds inputs and the reference's outputs for them: neighbour
-lists from plugins/linkedcell.cc and plugins/verletlist.cc, per-pair values of
+lists from plugins/linkedcell.cc, plugins/verletlist.cc, plugins/minimumimage.cc
+and plugins/lcbinary.cc, per-pair values of
 the seven potential plugins, crystal3d lattices and velocityverlet
 trajectories.  Large lists are stored as SHA-256 digests of their exact bytes.
 """
@@ -105,6 +106,20 @@ def main():
     xr = rng.uniform(0, 1, (300, 3)) * Lr
     meta["lists"]["rand300_lc_explicit"] = list_case("rand300_lc_explicit", xr, Lr, "linkedcell", "cutoff 6.0 nx 5 ny 6 nz 4", 6.0, True)
     meta["lists"]["rand300_vl"] = list_case("rand300_vl", xr, Lr, "verletlist", "cutoff 6.0 maxneighbors 120 each 5", 6.0, True)
+    # minimumimage: one image per pair by brute force; the second radius is
+    # wider than half the smallest edge (19.3 / 2)
+    meta["lists"]["rand300_mi"] = list_case("rand300_mi", xr, Lr, "minimumimage", "cutoff 0.0", 6.0, True)
+    meta["lists"]["rand300_mi_wide"] = list_case("rand300_mi_wide", xr, Lr, "minimumimage", "cutoff 12.0", 12.0, True)
+    # lcbinary: subcells of about cutoff/5 holding at most one atom; random
+    # atoms collide and grow the grid; the 256-atom Cu box has a stencil wider
+    # than the grid along no axis but wider than half of it (entries repeat)
+    meta["lists"]["rand300_lcb"] = list_case("rand300_lcb", xr, Lr, "lcbinary", "cutoff 6.0", 6.0, True)
+    x, L = fcc(8, 5.26)
+    xj = jitter(x, L, 0.05, 11)
+    meta["lists"]["ar2048j_lcb_auto"] = list_case("ar2048j_lcb_auto", xj, L, "lcbinary", "cutoff 8.5 mode auto", 8.5, False)
+    x, L = fcc(4, 3.61)
+    xj = jitter(x, L, 0.01, 12)
+    meta["lists"]["cu256j_lcb"] = list_case("cu256j_lcb", xj, L, "lcbinary", "cutoff 7.22 mode auto", 7.22, True)
 
     # ---- potentials ---------------------------------------------------------
     rng = np.random.default_rng(21)

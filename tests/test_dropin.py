@@ -41,6 +41,8 @@ def test_potential_plugins_match_reference_keywords(name, args):
 @pytest.mark.parametrize("name,args", [("linkedcell", "cutoff 8.5 mode auto"),
                                        ("verletlist", "cutoff 8.5 maxneighbors 200 each 5"),
                                        ("verletlist", "cutoff 8.5 maxneighbors 200 skin 1.0"),
+                                       ("minimumimage", "cutoff 8.5"),
+                                       ("lcbinary", "cutoff 8.5 mode auto"),
                                        ("velocityverlet", "dt 2.0")])
 def test_other_plugins_load(name, args):
     p = R.RefPlugin.dropin(name, args)

@@ -33,7 +33,8 @@ def rel_rms(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("case", ["ar2048j_lc_auto", "cu256j_lc_dup", "rand300_vl"])
+@pytest.mark.parametrize("case", ["ar2048j_lc_auto", "cu256j_lc_dup", "rand300_vl", "rand300_mi_wide", "rand300_lcb",
+                                  "cu256j_lcb"])
 def test_dropin_lists_match_golden(case):
     m = META["lists"][case]
     say("drop-in lists", case)
@@ -59,22 +60,33 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("cmname", ["linkedcell", "verletlist"])
+CM_ARGS = {"linkedcell": "cutoff {rc} mode auto", "verletlist": "cutoff {rc} maxneighbors 400 each 5",
+           "minimumimage": "cutoff {rc}", "lcbinary": "cutoff {rc} mode auto"}
+
+
+def oracle_cm(cmname, rc):
+    if cmname in ("linkedcell", "lcbinary"):
+        return O.CellManager(cmname, rc, mode="auto")
+    if cmname == "minimumimage":
+        return O.CellManager("minimumimage", rc)
+    return O.CellManager("verletlist", rc, maxneighbors=400, each=5)
+
+
+@pytest.mark.parametrize("cmname", ["linkedcell", "verletlist", "minimumimage", "lcbinary"])
 @pytest.mark.parametrize("name,args,kw,z,a", CASES, ids=[c[0] for c in CASES])
 def test_dropin_combined_forces_match_oracle(name, args, kw, z, a, cmname):
     say("drop-in forces", name, cmname)
     x, L = E.fcc(5, a)
     x = np.mod(x + np.random.default_rng(5).normal(0.0, 0.05, x.shape), L)
     rc = kw["cutoff"]
-    cm_args = f"cutoff {rc} mode auto" if cmname == "linkedcell" else f"cutoff {rc} maxneighbors 400 each 5"
+    cm_args = CM_ARGS[cmname].format(rc=rc)
     conf = R.RefConf(x, L, z=z)
     cm = R.RefPlugin.dropin(cmname, cm_args)
     cm.attach(conf)
     pot = R.RefPlugin.dropin(name, args)
     acc, e, w, S = R.combined_forces(conf, [pot])
     oc = O.Conf(x, L, z=z)
-    om = (O.CellManager("linkedcell", rc, mode="auto") if cmname == "linkedcell"
-          else O.CellManager("verletlist", rc, maxneighbors=400, each=5))
+    om = oracle_cm(cmname, rc)
     e0, w0, S0 = O.update_forces(oc, om, [O.Pot(name, **kw)])
     err = rel_rms(acc, oc.a)
     say(f"  rel force rms {err:.2e}  E {e:.10g} vs {e0:.10g}")

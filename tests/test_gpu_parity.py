@@ -38,6 +38,18 @@ def rel_rms(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def oracle_cm(cm, cm_args):
+    """The oracle's cell manager for a plugin name and keyword string."""
+    args = dict(zip(cm_args.split()[0::2], cm_args.split()[1::2]))
+    cut = float(args.get("cutoff", 0.0))
+    if cm in ("linkedcell", "lcbinary"):
+        return O.CellManager(cm, cut, int(args.get("nx", 0)), int(args.get("ny", 0)), int(args.get("nz", 0)),
+                             mode=args.get("mode", "noauto"))
+    if cm == "minimumimage":
+        return O.CellManager("minimumimage", cut)
+    return O.CellManager("verletlist", cut, maxneighbors=int(args["maxneighbors"]), each=int(args["each"]))
+
+
 # --------------------------------------------------------------- neighbour lists
 @pytest.mark.parametrize("case", sorted(META["lists"]))
 def test_lists_match_golden(case):
@@ -48,13 +60,7 @@ def test_lists_match_golden(case):
     s = E.System(x, L, 18)
     s.set_cellmanager(m["cm"], m["cm_args"])
     oc = O.Conf(x, L)
-    args = dict(zip(m["cm_args"].split()[0::2], m["cm_args"].split()[1::2]))
-    if m["cm"] == "linkedcell":
-        om = O.CellManager("linkedcell", float(args["cutoff"]), int(args.get("nx", 0)), int(args.get("ny", 0)),
-                           int(args.get("nz", 0)), mode=args.get("mode", "noauto"))
-    else:
-        om = O.CellManager("verletlist", float(args["cutoff"]), maxneighbors=int(args["maxneighbors"]),
-                           each=int(args["each"]))
+    om = oracle_cm(m["cm"], m["cm_args"])
     om.update(oc)
     for full, tag in ((True, "full"), (False, "half")):
         counts, js, r2 = s.neighbor_lists(full, m["rcut"])
@@ -81,19 +87,16 @@ POTS = [
 ]
 
 
-@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist", "minimumimage", "lcbinary"])
 @pytest.mark.parametrize("name,args,kw,z,a,jit", POTS, ids=[p[0] for p in POTS])
 def test_forces_match_oracle(name, args, kw, z, a, jit, cm):
     say("forces", name, cm)
     x, L = jittered(5, a, jit, 17)
     pot = O.Pot(name, **kw)
     rc = pot.cutoff if np.isfinite(pot.cutoff) else 7.0
-    if cm == "linkedcell":
-        cm_args = f"cutoff {rc} mode auto"
-        om = O.CellManager("linkedcell", rc, mode="auto")
-    else:
-        cm_args = f"cutoff {rc} maxneighbors 400 each 5"
-        om = O.CellManager("verletlist", rc, maxneighbors=400, each=5)
+    cm_args = {"linkedcell": f"cutoff {rc} mode auto", "verletlist": f"cutoff {rc} maxneighbors 400 each 5",
+               "minimumimage": f"cutoff {rc}", "lcbinary": f"cutoff {rc} mode auto"}[cm]
+    om = oracle_cm(cm, cm_args)
     s = E.System(x, L, z)
     s.set_cellmanager(cm, cm_args)
     s.append_potential(name, args)

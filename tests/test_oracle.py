@@ -27,9 +27,11 @@ META = json.load(open(os.path.join(GOLDEN, "golden.json")))
 def parse_cm(cm, args, ):
     kw = dict(zip(args.split()[0::2], args.split()[1::2]))
     cut = float(kw["cutoff"])
-    if cm == "linkedcell":
-        return O.CellManager("linkedcell", cut, int(kw.get("nx", 0)), int(kw.get("ny", 0)), int(kw.get("nz", 0)),
+    if cm in ("linkedcell", "lcbinary"):
+        return O.CellManager(cm, cut, int(kw.get("nx", 0)), int(kw.get("ny", 0)), int(kw.get("nz", 0)),
                              mode=kw.get("mode", "noauto"))
+    if cm == "minimumimage":
+        return O.CellManager("minimumimage", cut)
     return O.CellManager("verletlist", cut, maxneighbors=int(kw["maxneighbors"]), each=int(kw["each"]))
 
 
@@ -81,6 +83,38 @@ def test_duplicate_stencil_case_has_duplicates():
     start = np.concatenate([[0], np.cumsum(counts)])
     dup = sum(len(js[start[i]:start[i + 1]]) - len(np.unique(js[start[i]:start[i + 1]])) for i in range(len(counts)))
     assert dup > 0  # 2*side+1 = 5 > n = 4 visits cells twice
+
+
+def test_lcbinary_cases_grow_and_repeat():
+    # random atoms share subcells: the grid grows past the automatic choice
+    d = np.load(os.path.join(GOLDEN, "lists_rand300_lcb.npz"))
+    conf = O.Conf(d["x"], d["L"])
+    cm = O.CellManager("lcbinary", 6.0)
+    cm.update(conf)
+    g = cm.grid()
+    assert g[0] > O.lcb_auto_n(d["L"][0], 6.0)
+    # the 256-atom Cu box: a stencil of 11 subcells on a grid of 10 repeats entries
+    d = np.load(os.path.join(GOLDEN, "lists_cu256j_lcb.npz"))
+    counts, js = d["full_counts"], d["full_j"]
+    start = np.concatenate([[0], np.cumsum(counts)])
+    dup = sum(len(js[start[i]:start[i + 1]]) - len(np.unique(js[start[i]:start[i + 1]])) for i in range(len(counts)))
+    assert dup > 0
+
+
+def test_minimumimage_matches_brute_force_set():
+    # every pair within the radius, by its single minimum image, ascending j
+    for case in ("rand300_mi", "rand300_mi_wide"):
+        d = np.load(os.path.join(GOLDEN, f"lists_{case}.npz"))
+        x, L = d["x"], d["L"]
+        rc = META["lists"][case]["rcut"]
+        dx = x[None, :, :] - x[:, None, :]
+        dx -= L * np.where(dx >= L / 2, 1.0, np.where(dx < -L / 2, -1.0, 0.0))
+        r2 = (dx * dx).sum(axis=2)
+        np.fill_diagonal(r2, np.inf)
+        expect = [np.nonzero(r2[i] < rc * rc)[0] for i in range(len(x))]
+        start = np.concatenate([[0], np.cumsum(d["full_counts"])])
+        for i in range(len(x)):
+            assert np.array_equal(d["full_j"][start[i]:start[i + 1]], expect[i])
 
 
 def test_shell_cutoff_counts():
