@@ -449,8 +449,9 @@ struct lpmdx_system {
       return l;
     }
     // fewer lanes per atom issue fewer instructions per neighbour; small
-    // systems need more lanes to fill 256 CUs
-    if (n >= 500000)
+    // systems need more lanes to fill 256 CUs (131 072 Sutton-Chen atoms:
+    // 1 lane 7 % faster than 2)
+    if (n >= 100000)
       return 1;
     if (n >= 60000)
       return 2;

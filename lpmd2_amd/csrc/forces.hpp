@@ -369,6 +369,11 @@ struct Stage {
 // The list entries of one batch: kUnroll consecutive entries k0 .. k0+3 of
 // this lane (k0 a multiple of 4, so they are one aligned 16-byte word group
 // of a chunk), as loaded.
+// Batches of list entries in flight ahead of the one being evaluated (one
+// batch ahead left list loads exposed for the unstaged EAM kernels at one
+// lane per atom; three or four measured no better than two).
+constexpr int kListAhead = 2;
+
 template <int LPA> __device__ __forceinline__ int4 list_batch(const int* __restrict__ nb, int k0) {
   static_assert(kUnroll == 4 && kChunk % kUnroll == 0, "one int4 per batch");
   constexpr int T = kWave / LPA;
@@ -395,15 +400,22 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
   // are loaded at its start (measured faster for the unstaged pair kernels,
   // where the gathers, not the list, hold a batch up and the arithmetic per
   // batch is too short to cover a list load).
-  int4 qn = make_int4(pad, pad, pad, pad);
-  if (PREFETCH && kUnroll * l < c)
-    qn = list_batch<LPA>(nb, kUnroll * l);
+  int4 qn[kListAhead];
+#pragma unroll
+  for (int d = 0; d < kListAhead; ++d) {
+    qn[d] = make_int4(pad, pad, pad, pad);
+    if (PREFETCH && kUnroll * l + d * step < c)
+      qn[d] = list_batch<LPA>(nb, kUnroll * l + d * step);
+  }
   for (int k0 = kUnroll * l; k0 < c; k0 += step) {
     int4 q;
     if constexpr (PREFETCH) {
-      q = qn;
-      if (k0 + step < c)
-        qn = list_batch<LPA>(nb, k0 + step);
+      q = qn[0];
+#pragma unroll
+      for (int d = 0; d + 1 < kListAhead; ++d)
+        qn[d] = qn[d + 1];
+      if (k0 + kListAhead * step < c)
+        qn[kListAhead - 1] = list_batch<LPA>(nb, k0 + kListAhead * step);
     } else {
       q = list_batch<LPA>(nb, k0);
     }
