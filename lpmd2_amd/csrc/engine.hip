@@ -133,6 +133,7 @@ struct lpmdx_system {
   double rl_min = 0.0; // a list radius an export asked for (minimumimage, lcbinary)
   // EAM
   DBuf<float> w_rho;
+  DBuf<double4> xw; // EAM: packed (x, y, z, rho^{-1/2}) for the unstaged force pass
   DBuf<double> rho;
   // reductions
   DBuf<double> partials, partials2, red;
@@ -911,6 +912,8 @@ struct lpmdx_system {
       if (P.metal()) {
         w_rho.alloc(n);
         rho.alloc(n);
+        xw.alloc(n);
+        A.xw = xw.p;
       }
       if (eval)
         launch_pot<true>(P.kind, A, nb, rcr);
@@ -1013,6 +1016,8 @@ struct lpmdx_system {
     if (P.metal()) {
       w_rho.alloc(n);
       rho.alloc(n);
+      xw.alloc(n);
+      A.xw = xw.p;
     }
     launch_pot<true>(P.kind, A, nb, rcr);
     k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 12, red.p + 12 * (1 + 2 * k));

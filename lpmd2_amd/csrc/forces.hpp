@@ -215,6 +215,9 @@ struct ForceArgs {
   const int* nbr;
   const int* cnt;
   const float* w_rho;   // EAM: [n] rho^{-1/2}
+  // EAM, unstaged force pass: [n] (x, y, z, rho^{-1/2}) of each atom, written
+  // by the rho pass, so one neighbour is two 16-byte loads instead of three
+  double4* xw;
   double* partials;     // [nblocks * 12] when evaluating energies
   unsigned int* maxd2;  // max over atoms of |x_next - xb|^2 (float bits), or null
   const double* xb;     // [3n] positions at the last list build
@@ -380,7 +383,8 @@ template <int LPA> __device__ __forceinline__ int4 list_batch(const int* __restr
   return *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
 }
 
-template <int LPA, bool WRAP, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, typename Body>
+template <int LPA, bool WRAP, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED,
+          typename Body>
 __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& H, long i, int l, double xi,
                                                double yi, double zi, Body& body) {
   // a list that outgrew its capacity (reported as an error) is read no further
@@ -422,12 +426,19 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
     const int js[kUnroll] = {k0 < c ? q.x : pad, k0 + 1 < c ? q.y : pad, k0 + 2 < c ? q.z : pad,
                              k0 + 3 < c ? q.w : pad};
     double px[kUnroll], py[kUnroll], pz[kUnroll];
+    float pw[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       if constexpr (STAGED) {
         px[u] = H.X[js[u]];
         py[u] = H.Y[js[u]];
         pz[u] = H.Z[js[u]];
+      } else if constexpr (PACKED) {
+        const double4 p = A.xw[js[u]];
+        px[u] = p.x;
+        py[u] = p.y;
+        pz[u] = p.z;
+        pw[u] = (float)p.w;
       } else {
         const double* p = A.x + 3 * (long)js[u];
         px[u] = p[0];
@@ -443,7 +454,7 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
       if constexpr (NEED_S)
         sj = STAGED ? H.S[j] : A.z[j];
       if constexpr (NEED_W)
-        wj = STAGED ? H.W[j] : A.w_rho[j];
+        wj = STAGED ? H.W[j] : (PACKED ? pw[u] : A.w_rho[j]);
       double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
       if constexpr (WRAP)
         min_image(A.b, dx, dy, dz);
@@ -458,16 +469,16 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
 // half a box edge, the minimum image is the identity
 // (OrthogonalCell::Displacement would not wrap) and the loop skips it; the
 // wavefront takes one variant, so the branch is uniform.
-template <int LPA, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, typename Body>
+template <int LPA, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED = false, typename Body>
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& H, long i, int l, double xi,
                                                double yi, double zi, Body&& body) {
   const double m = A.margin;
   const bool inner = m > 0.0 && xi >= m && xi <= A.b.L[0] - m && yi >= m && yi <= A.b.L[1] - m && zi >= m &&
                      zi <= A.b.L[2] - m;
   if (__all(inner))
-    neighbour_loop<LPA, false, STAGED, NEED_S, NEED_W, PREFETCH>(A, H, i, l, xi, yi, zi, body);
+    neighbour_loop<LPA, false, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
   else
-    neighbour_loop<LPA, true, STAGED, NEED_S, NEED_W, PREFETCH>(A, H, i, l, xi, yi, zi, body);
+    neighbour_loop<LPA, true, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
 }
 
 // Per-atom force: FP32 pair scalar times the FP64 displacement, summed in FP64
@@ -549,12 +560,16 @@ __device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, lo
 // check, metalpotential.cc:121-132), accumulated in FP64; writes
 // w_i = rho_i^{-1/2} (FP32) and rho_i, and, when evaluating, the embedding
 // energy F(rho_i) = -K sqrt(rho_i) into the caller's emb.
+// With A.xw set, also writes the atom's packed (x, y, z, w) record.
 template <int KIND, int LPA, bool EVAL, bool STAGED, bool PREFETCH = true, int EXP = 0>
 __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                              float* w_out, double* rho_out, double& emb) {
   double rho = 0.0;
+  double xi = 0.0, yi = 0.0, zi = 0.0;
   if (active) {
-    const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
+    xi = A.x[3 * i];
+    yi = A.x[3 * i + 1];
+    zi = A.x[3 * i + 2];
     const double rc2 = A.P.rc2;
     for_neighbours<LPA, STAGED, false, false, PREFETCH>(A, H, i, l, xi, yi, zi,
                                               [&](bool valid, int, float, double, double, double, double r2) {
@@ -565,7 +580,10 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H,
   }
   rho = group_sum<LPA>(rho);
   if (active && l == 0) {
-    w_out[i] = (float)(1.0 / sqrt(rho));
+    const float w = (float)(1.0 / sqrt(rho));
+    w_out[i] = w;
+    if (A.xw)
+      A.xw[i] = make_double4(xi, yi, zi, (double)w);
     if (rho_out)
       rho_out[i] = rho;
     if constexpr (EVAL)
@@ -575,7 +593,9 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H,
 
 // EAM pass 2: pair + many-body forces over the full list with AppliesTo
 // (metalpotential.cc:140-169), fused with the integrator tail.
-template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC, bool PREFETCH = true, int EXP = 0>
+// PACKED: neighbour positions and rho^{-1/2} from A.xw.
+template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC, bool PREFETCH = true, int EXP = 0,
+          bool PACKED = false>
 __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                                Tally& tal) {
   ForceSum F;
@@ -584,7 +604,7 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& 
     const int si = SPC ? A.z[i] : 0;
     const float wi = A.w_rho[i];
     const double rc2 = A.P.rc2;
-    for_neighbours<LPA, STAGED, SPC, true, PREFETCH>(A, H, i, l, xi, yi, zi,
+    for_neighbours<LPA, STAGED, SPC, true, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi,
                                            [&](bool valid, int sj, float wj, double dx, double dy, double dz,
                                                double r2) {
       const bool in = valid && r2 < rc2 && (!SPC || A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
@@ -631,7 +651,8 @@ template <int KIND, int LPA, bool EVAL, bool SPC, int EXP = 0>
 static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
   const long t = xcd_block() * kBlock + threadIdx.x;
   Tally tal;
-  eam_force_atom<KIND, LPA, EVAL, false, SPC, true, EXP>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, tal);
+  eam_force_atom<KIND, LPA, EVAL, false, SPC, true, EXP, true>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n,
+                                                               tal);
   if constexpr (EVAL)
     write_partials(A, tal, 0.0);
 }
