@@ -1106,13 +1106,14 @@ struct lpmdx_system {
         m = std::min(m, obs_every - (done % obs_every));
       m = std::min<long>(m, kMaxChunk);
       bool need = false;
+      const bool replay = forced > 0; // steps that passed the test before a roll-back
       if (rebuilds_every_update()) {
         m = 1;
         need = true; // UpdateCell rebuilds every call
       } else {
         const double b = budget();
         need = !list_built || list_radius() != rl || disp_cur > b;
-        if (forced > 0)
+        if (replay)
           m = std::min(m, forced);
         else
           m = std::min(m, chunk);
@@ -1124,7 +1125,10 @@ struct lpmdx_system {
         rebuild();
         disp_cur = 0.0;
       }
-      const bool snapshot = !rebuilds_every_update();
+      // Only a chunk of two or more new steps can fail the test (its first
+      // force uses the chunk's starting positions, which passed it): the others
+      // need no snapshot to roll back to.
+      const bool snapshot = !rebuilds_every_update() && m > 1 && !replay;
       if (snapshot) {
         xs.alloc(3 * n);
         vs.alloc(3 * n);
@@ -1155,6 +1159,8 @@ struct lpmdx_system {
         d = dn;
       }
       if (bad >= 0) {
+        if (!snapshot)
+          fail(LPMDX_EDEVICE, "velocityverlet: a chunk without a snapshot failed the skin test (internal error)");
         // back to the chunk start; re-run the valid steps, then rebuild
         hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
         hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
