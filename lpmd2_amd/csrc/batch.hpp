@@ -23,18 +23,14 @@
 #include "cells.hpp"
 #include "common.hpp"
 #include "forces.hpp"
+#include "sort.hpp"
 
 namespace lpmdx {
 
 constexpr int kMaxTerms = 4;
 constexpr int kTile = kWave; // atoms per tile
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
-constexpr int kSortItems8 = 16;                     // atoms per thread of a sort chunk
-constexpr int kSortChunk = kBlock * kSortItems8;    // atoms per sort chunk (one workgroup)
 constexpr int kMaxSortCells = 1 << (2 * kRadixBits); // cells per system: at most two digit passes
 constexpr int kScanThreads = 1024;
-static_assert(kBlock == kRadix, "one thread per digit in the sort kernels");
 
 // Read-only description of one system for a round.
 struct BSys {
@@ -206,7 +202,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_count(BArgs B, int pass) {
   uint32_t* kin = B.R.keys[pass & 1] + a0;
   uint32_t* vin = B.R.vals[pass & 1] + a0;
   const double* xc = B.R.x[st.par] + 3 * a0;
-  for (int u = 0; u < kSortItems8; ++u) {
+  for (int u = 0; u < kSortItems; ++u) {
     const int i = j * kSortChunk + u * kBlock + t;
     if (i < n) {
       uint32_t key;
@@ -262,11 +258,9 @@ static __global__ __launch_bounds__(kBlock) void k_b_scatter(BArgs B, int pass) 
   if (!sort_chunk(B, pass, s, j))
     return; // uniform over the workgroup
   const BSys& S = B.sys[s];
-  constexpr int kWaves = kBlock / kWave;
-  __shared__ uint32_t run[kRadix];
-  __shared__ uint32_t wcnt[kWaves][kRadix];
-  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  run[t] = B.hist[(long)S.ch0 * kRadix + (long)t * S.nch + j];
+  __shared__ RankLds L;
+  const int t = threadIdx.x;
+  L.run[t] = B.hist[(long)S.ch0 * kRadix + (long)t * S.nch + j];
   const long a0 = S.a0;
   const int n = S.n;
   const int shift = kRadixBits * pass;
@@ -274,8 +268,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_scatter(BArgs B, int pass) 
   const uint32_t* vin = B.R.vals[pass & 1] + a0;
   uint32_t* kout = B.R.keys[(pass + 1) & 1] + a0;
   uint32_t* vout = B.R.vals[(pass + 1) & 1] + a0;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int u = 0; u < kSortItems8; ++u) {
+  for (int u = 0; u < kSortItems; ++u) {
     const int base = j * kSortChunk + u * kBlock;
     if (base >= n)
       break; // uniform over the workgroup
@@ -283,42 +276,11 @@ static __global__ __launch_bounds__(kBlock) void k_b_scatter(BArgs B, int pass) 
     const bool valid = i < n;
     const uint32_t key = valid ? kin[i] : 0u;
     const uint32_t val = valid ? vin[i] : 0u;
-    const uint32_t d = (key >> shift) & (kRadix - 1);
-    // lanes of this wavefront holding the same digit
-    unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const unsigned long long bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
+    const uint32_t pos = rank_tile(valid, (key >> shift) & (kRadix - 1), L);
+    if (valid && pos < (uint32_t)n) {
+      kout[pos] = key;
+      vout[pos] = val;
     }
-#pragma unroll
-    for (int q = 0; q < kWaves; ++q)
-      wcnt[q][t] = 0u;
-    __syncthreads();
-    if (valid && __ffsll((unsigned long long)peers) - 1 == lane)
-      wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      // digit t: offsets of each wavefront's run, then the running base
-      uint32_t pre = run[t];
-#pragma unroll
-      for (int q = 0; q < kWaves; ++q) {
-        const uint32_t c = wcnt[q][t];
-        wcnt[q][t] = pre;
-        pre += c;
-      }
-      run[t] = pre;
-    }
-    __syncthreads();
-    if (valid) {
-      const uint32_t pos = wcnt[w][d] + (uint32_t)__popcll(peers & below);
-      if (pos < (uint32_t)n) {
-        kout[pos] = key;
-        vout[pos] = val;
-      }
-    }
-    __syncthreads();
   }
 }
 

@@ -105,8 +105,9 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const Box& b, const 
   return (uint32_t)cell_id(g, i, j, k);
 }
 
-// keys[perm[i]] = cell of atom i; vals[o] = o.  Presenting atoms to the
-// stable sort in caller order keeps ascending caller index inside each cell.
+// keys[perm[i]] = cell of the atom in slot i, vals[perm[i]] = i.  Presenting
+// atoms to the stable sort in caller order keeps ascending caller index inside
+// each cell; the sorted values are then the atoms' current slots.
 static __global__ __launch_bounds__(kBlock) void k_cell_keys(const double* __restrict__ x,
                                                       const int* __restrict__ perm, long n, Grid g,
                                                       Box b, uint32_t* __restrict__ keys,
@@ -114,9 +115,9 @@ static __global__ __launch_bounds__(kBlock) void k_cell_keys(const double* __res
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
     return;
-  int o = perm[i];
+  const int o = perm[i];
   keys[o] = cell_key(g, b, x + 3 * i);
-  vals[i] = (uint32_t)i;
+  vals[o] = (uint32_t)i;
 }
 
 static __global__ __launch_bounds__(kBlock) void k_fill_int(int* __restrict__ p, long m, int v) {
@@ -147,7 +148,7 @@ static __global__ __launch_bounds__(kBlock) void k_cell_occupancy(const int* __r
   unsigned int m = c < ncell ? (unsigned int)(cb[c + 1] - cb[c]) : 0u;
   m = wave_max(m);
   if ((threadIdx.x & (kWave - 1)) == 0 && m > 0)
-    atomicMax(out, m);
+    atomic_max_sparse(out, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,16 +212,16 @@ static __global__ __launch_bounds__(kHaloThreads) void k_brick_halo(const int* _
     const int B3 = g.brick * g.brick * g.brick;
     const int atoms = cb[(b + 1) * B3] - cb[b * B3];
     row[H] = make_int2(incl, atoms);
-    atomicMax(halo_max, (unsigned int)incl);
-    atomicMax(brick_max, (unsigned int)atoms);
+    atomic_max_sparse(halo_max, (unsigned int)incl);
+    atomic_max_sparse(brick_max, (unsigned int)atoms);
   }
 }
 
 // Reorders the per-atom state into the new (cell-sorted) order.  order[k] is
-// the caller index of the atom placed at k; iperm maps caller -> old position.
+// the current slot of the atom placed at k; perm maps slots to caller indices.
 struct PermuteArgs {
   const uint32_t* order;
-  const int* iperm_old;
+  const int* perm_old;
   const double* x;
   const double* v;
   const double* a;
@@ -234,27 +235,35 @@ struct PermuteArgs {
   double* ffm2;
   double* mass2;
   int* perm_new;
-  int* iperm_new;
   long n;
 };
 
+// Every load is issued before the first store (the output arrays are distinct
+// from the inputs, which the compiler cannot know).
 static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
-  long k = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long k = (long)blockIdx.x * kBlock + threadIdx.x;
   if (k >= A.n)
     return;
-  int o = (int)A.order[k];
-  long s = A.iperm_old[o];
+  const long s = A.order[k];
+  double x[3], v[3], a[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    A.x2[3 * k + q] = A.x[3 * s + q];
-    A.v2[3 * k + q] = A.v[3 * s + q];
-    A.a2[3 * k + q] = A.a[3 * s + q];
+    x[q] = A.x[3 * s + q];
+    v[q] = A.v[3 * s + q];
+    a[q] = A.a[3 * s + q];
   }
-  A.z2[k] = A.z[s];
-  A.ffm2[k] = A.ffm[s];
-  A.mass2[k] = A.mass[s];
+  const int z = A.z[s], o = A.perm_old[s];
+  const double ffm = A.ffm[s], mass = A.mass[s];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    A.x2[3 * k + q] = x[q];
+    A.v2[3 * k + q] = v[q];
+    A.a2[3 * k + q] = a[q];
+  }
+  A.z2[k] = z;
+  A.ffm2[k] = ffm;
+  A.mass2[k] = mass;
   A.perm_new[k] = o;
-  A.iperm_new[o] = (int)k;
 }
 
 constexpr int kChunk = 8;
@@ -274,7 +283,8 @@ struct BuildArgs {
   int* nbr;
   int* cnt;
   unsigned int* max_count;
-  unsigned long long* sum_count;
+  unsigned long long* sum_count; // one counter, or kSumSlots slots (sum_spread)
+  int sum_spread;                // 0: one counter; 1: spread slots (atomic_add_spread)
   Grid g;
   Box b;
   float Lf[3], halff[3];
@@ -517,8 +527,11 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
   unsigned long long t = wave_sum((unsigned long long)c);
   if ((threadIdx.x & (kWave - 1)) == 0 && m > 0) {
     // integer max and sum: the result does not depend on the order
-    atomicMax(A.max_count, m);
-    atomicAdd(A.sum_count, t);
+    atomic_max_sparse(A.max_count, m);
+    if (A.sum_spread)
+      atomic_add_spread(A.sum_count, t);
+    else
+      atomicAdd(A.sum_count, t);
   }
 }
 

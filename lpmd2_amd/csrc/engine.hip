@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_disp(ForceArgs A) {
   }
   float m = wave_max(d2);
   if ((threadIdx.x & (kWave - 1)) == 0 && m > 0.0f)
-    atomicMax(A.maxd2, __float_as_uint(m));
+    atomic_max_sparse(A.maxd2, __float_as_uint(m));
 }
 
 __global__ void k_fill_double(double* p, long m, double v) {
@@ -107,14 +107,15 @@ struct lpmdx_system {
   hipStream_t st = nullptr;
   // state, internal order
   DBuf<double> x, xn, v, a, ffm, mass, asum, xb;
-  DBuf<int> z, perm, iperm;
+  DBuf<int> z, perm; // perm[k]: caller index of the atom in slot k
   // scratch for re-ordering
   DBuf<double> x2, v2, a2, ffm2, mass2;
-  DBuf<int> z2, perm2, iperm2;
+  DBuf<int> z2, perm2;
   // snapshot for the skin check roll-back
   DBuf<double> xs, vs, as;
   // sort
-  DBuf<uint32_t> keys, vals, keys2, vals2, hist;
+  DBuf<uint32_t> keys, vals, keys2, vals2, hist, htot;
+  DBuf<unsigned long long> sumslots; // spread list-length sum of a build
   // cells
   DBuf<int> cb;           // cell starts of the sorted atoms (ncell + 2)
   DBuf<int2> bhalo;       // brick halo table (bricked grids)
@@ -465,18 +466,16 @@ struct lpmdx_system {
     const long nb = blocks_for(n);
     k_cell_keys<<<nb, kBlock, 0, st>>>(x.p, perm.p, n, grid, box, keys.p, vals.p);
     launch_check();
-    int bits = 0;
-    while (bits < 32 && (uint32_t)grid.ncell >> bits)
-      ++bits;
-    const int passes = std::max(1, (bits + 3) / 4);
-    const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
-    hist.alloc((size_t)kDigits * ntiles);
+    const int passes = radix_passes((uint32_t)grid.ncell);
+    const int nch = (int)((n + kSortChunk - 1) / kSortChunk);
+    hist.alloc((size_t)kRadix * nch);
+    htot.alloc(kRadix);
     uint32_t *ka = keys.p, *va = vals.p, *kb = keys2.p, *vb = vals2.p;
     for (int ps = 0; ps < passes; ++ps) {
-      const int shift = 4 * ps;
-      k_sort_count<<<ntiles, kBlock, 0, st>>>(ka, n, shift, hist.p, ntiles);
-      k_scan_excl<<<1, 1024, 0, st>>>(hist.p, (long)kDigits * ntiles);
-      k_sort_scatter<<<ntiles, kBlock, 0, st>>>(ka, va, n, shift, hist.p, ntiles, kb, vb);
+      const int shift = kRadixBits * ps;
+      k_radix_count<<<nch, kBlock, 0, st>>>(ka, n, shift, hist.p, nch);
+      k_radix_rows<<<kRadix / kSortWaves, kBlock, 0, st>>>(hist.p, nch, htot.p);
+      k_radix_scatter<<<nch, kBlock, 0, st>>>(ka, va, n, shift, hist.p, htot.p, nch, kb, vb);
       launch_check();
       std::swap(ka, kb);
       std::swap(va, vb);
@@ -484,8 +483,8 @@ struct lpmdx_system {
     // sorted keys/order now in (ka, va)
     cb.alloc(grid.ncell + 2);
     k_cell_begin<<<blocks_for(n + 1), kBlock, 0, st>>>(ka, n, grid.ncell, cb.p);
-    PermuteArgs P{va, iperm.p, x.p, v.p, a.p, z.p, ffm.p, mass.p, x2.p, v2.p, a2.p, z2.p,
-                  ffm2.p, mass2.p, perm2.p, iperm2.p, n};
+    PermuteArgs P{va, perm.p, x.p, v.p, a.p, z.p, ffm.p, mass.p, x2.p, v2.p, a2.p, z2.p,
+                  ffm2.p, mass2.p, perm2.p, n};
     k_permute<<<nb, kBlock, 0, st>>>(P);
     launch_check();
     std::swap(x.p, x2.p);
@@ -495,7 +494,6 @@ struct lpmdx_system {
     std::swap(ffm.p, ffm2.p);
     std::swap(mass.p, mass2.p);
     std::swap(perm.p, perm2.p);
-    std::swap(iperm.p, iperm2.p);
     // keep the sorted keys where the build kernel reads them
     if (ka != keys.p) {
       std::swap(keys.p, keys2.p);
@@ -534,6 +532,8 @@ struct lpmdx_system {
       cnt.alloc(n);
       // max_count, pad0, sum_count: 16 contiguous bytes
       hipchk(hipMemsetAsync(&ctrl.p->max_count, 0, 16, st), "memset");
+      sumslots.alloc((size_t)kSumSlots * kSlotStride);
+      hipchk(hipMemsetAsync(sumslots.p, 0, (size_t)kSumSlots * kSlotStride * sizeof(unsigned long long), st), "memset");
       const double rl2 = rl * rl;
       // FP32 positions are off by at most ulp(Lmax)/2; two of them in a distance
       const double lmax = std::max(box.L[0], std::max(box.L[1], box.L[2]));
@@ -560,7 +560,8 @@ struct lpmdx_system {
       B.nbr = nbr.p;
       B.cnt = cnt.p;
       B.max_count = &ctrl.p->max_count;
-      B.sum_count = &ctrl.p->sum_count;
+      B.sum_count = sumslots.p;
+      B.sum_spread = 1;
       B.g = grid;
       B.b = box;
       for (int q = 0; q < 3; ++q) {
@@ -577,6 +578,7 @@ struct lpmdx_system {
       hipEvent_t e0 = prof_begin();
       k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
       prof_end(kProfBuild, e0);
+      k_sum_slots<<<1, kWave, 0, st>>>(sumslots.p, &ctrl.p->sum_count);
       launch_check();
       // max_count, pad0, sum_count, halo_max, brick_max: 24 contiguous bytes
       hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 24, hipMemcpyDeviceToHost, st),
@@ -1268,7 +1270,7 @@ int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const i
       b->alloc(3 * m);
     for (DBuf<double>* b : {&S->ffm, &S->mass, &S->ffm2, &S->mass2})
       b->alloc(m);
-    for (DBuf<int>* b : {&S->z, &S->perm, &S->iperm, &S->z2, &S->perm2, &S->iperm2})
+    for (DBuf<int>* b : {&S->z, &S->perm, &S->z2, &S->perm2})
       b->alloc(m);
     for (DBuf<uint32_t>* b : {&S->keys, &S->vals, &S->keys2, &S->vals2})
       b->alloc(m);
@@ -1290,7 +1292,6 @@ int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const i
     hipchk(hipMemcpyAsync(S->ffm.p, ff.data(), m * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
     hipchk(hipMemcpyAsync(S->z.p, zz.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
     hipchk(hipMemcpyAsync(S->perm.p, pm.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
-    hipchk(hipMemcpyAsync(S->iperm.p, pm.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
     for (DBuf<double>* b : {&S->x, &S->v, &S->a})
       hipchk(hipMemsetAsync(b->p, 0, 3 * m * sizeof(double), S->st), "memset");
     S->sync();

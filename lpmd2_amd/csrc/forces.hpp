@@ -298,7 +298,7 @@ __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, d
 __device__ __forceinline__ void flush_maxd2(const ForceArgs& A, float d2) {
   float m = wave_max(d2);
   if ((threadIdx.x & (kWave - 1)) == 0 && A.mode != kForcesOnly && m > 0.0f)
-    atomicMax(A.maxd2, __float_as_uint(m));
+    atomic_max_sparse(A.maxd2, __float_as_uint(m)); // non-negative floats order as their bits
 }
 
 // Energy/virial/stress accumulators of one lane (FP64), half per direction.
