@@ -191,7 +191,7 @@ struct lpmdx_batch {
       b->alloc(A);
     for (DBuf<uint32_t>* b : {&k0, &k1, &v0, &v1})
       b->alloc(A);
-    xf.alloc(A);
+    xf.alloc(A + kGroup); // padding read by the candidate groups of the last system
     w.alloc(A);
     tsys.alloc(A / kTile);
     nbr.alloc(std::max<long>(lcap, kTile));

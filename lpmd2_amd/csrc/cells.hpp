@@ -276,7 +276,7 @@ __host__ __device__ __forceinline__ long list_index(long i, int k, int lpa, int 
 
 struct BuildArgs {
   const double* x;
-  const float4* xf; // FP32 copy of x (w unused), written by k_to_f32 before the build
+  const float4* xf; // FP32 copy of x (w unused) by k_to_f32, then kGroup - 1 entries of padding
   const uint32_t* skeys;
   const int* cb; // cell starts (k_cell_begin)
   const int2* bhalo; // brick halo table (k_brick_halo), or null: global indices
@@ -382,15 +382,53 @@ __device__ __forceinline__ float wrap1f(float d, float L, float h) {
   return d + w;
 }
 
+// FP32 candidates [lo, hi) against atom i (the FP32 paths).  A group of
+// kGroup positions is loaded before any is tested: xf holds kGroup - 1 entries
+// of padding past the last atom, so the loads of a group read xf[k0 .. k0 + 7]
+// at fixed offsets from one address, and the entries at or past hi (the next
+// cell's atoms, or padding) are never accepted.  SELF: the cell holds atom i
+// itself, which is skipped.  The distance is formed with fused multiply-adds:
+// its rounding stays far inside the admission margin of rl2f.
+template <bool SELF>
+__device__ __forceinline__ void scan_cell_f32(const BuildArgs& A, int lo, int hi, int ii, float sx, float sy,
+                                              float sz, Pending& P, int shift, bool wrap) {
+  const int m = hi - lo;
+  for (int k0 = 0; k0 < m; k0 += kGroup) {
+    const float4* q4 = A.xf + lo + k0;
+    float4 q[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u)
+      q[u] = q4[u];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      // kPathShift32: (sx, sy, sz) is atom i moved by minus this cell's
+      // periodic image shift; kPathMin32: a per-pair minimum image
+      float dx = q[u].x - sx, dy = q[u].y - sy, dz = q[u].z - sz;
+      if (wrap) {
+        dx = wrap1f(dx, A.Lf[0], A.halff[0]);
+        dy = wrap1f(dy, A.Lf[1], A.halff[1]);
+        dz = wrap1f(dz, A.Lf[2], A.halff[2]);
+      }
+      const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      const int k = lo + k0 + u;
+      bool take = (k0 + u < m) & (d2 < A.rl2f);
+      if (SELF)
+        take &= k != ii;
+      push(P, k + shift, take);
+    }
+    drain(A, P, false);
+  }
+}
+
 // candidates of cell p against atom i; entries are written as k + shift (the
 // halo slot of sorted atom k, or k itself for global indices)
-// Candidates are taken kGroup at a time; the index is clamped to the cell's
-// last atom so every load stays inside the cell, and only k < hi is accepted.
 __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, const double* xi,
                                           float sx, float sy, float sz, Pending& P, int shift) {
   const int lo = A.cb[p], hi = A.cb[p + 1];
   const int ii = (int)i;
   if (A.path == kPathExact64) {
+    // candidates kGroup at a time; the index is clamped to the cell's last
+    // atom so every load stays inside the cell, and only k < hi is accepted
     for (int k0 = lo; k0 < hi; k0 += kGroup) {
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) {
@@ -403,30 +441,12 @@ __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, con
       drain(A, P, false);
     }
   } else {
-    // the group's positions are loaded before any is tested, and the tests
-    // combine with & (no short circuit), so the loads are in flight together
+    // atom i lies in its own cell only: [lo, hi) holds it iff lo <= i < hi
     const bool wrap = A.path == kPathMin32;
-    for (int k0 = lo; k0 < hi; k0 += kGroup) {
-      float4 q[kGroup];
-#pragma unroll
-      for (int u = 0; u < kGroup; ++u)
-        q[u] = A.xf[min(k0 + u, hi - 1)];
-#pragma unroll
-      for (int u = 0; u < kGroup; ++u) {
-        const int k = min(k0 + u, hi - 1);
-        // kPathShift32: (sx, sy, sz) is atom i moved by minus this cell's
-        // periodic image shift; kPathMin32: a per-pair minimum image
-        float dx = q[u].x - sx, dy = q[u].y - sy, dz = q[u].z - sz;
-        if (wrap) {
-          dx = wrap1f(dx, A.Lf[0], A.halff[0]);
-          dy = wrap1f(dy, A.Lf[1], A.halff[1]);
-          dz = wrap1f(dz, A.Lf[2], A.halff[2]);
-        }
-        const bool in = dx * dx + dy * dy + dz * dz < A.rl2f;
-        push(P, k + shift, (k0 + u < hi) & (k != ii) & in);
-      }
-      drain(A, P, false);
-    }
+    if (lo <= ii && ii < hi)
+      scan_cell_f32<true>(A, lo, hi, ii, sx, sy, sz, P, shift, wrap);
+    else
+      scan_cell_f32<false>(A, lo, hi, ii, sx, sy, sz, P, shift, wrap);
   }
 }
 

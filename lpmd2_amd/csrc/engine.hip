@@ -549,7 +549,7 @@ struct lpmdx_system {
         else
           path = kPathMin32;
       }
-      xf.alloc(n);
+      xf.alloc(n + kGroup); // padding read by the candidate groups
       k_to_f32<<<blocks_for(n), kBlock, 0, st>>>(x.p, n, box, xf.p);
       BuildArgs B{};
       B.x = x.p;
