@@ -252,6 +252,7 @@ struct lpmdx_batch {
         fail(LPMDX_EINVAL, "box edges must be positive");
       S.b.L[q] = box[q];
       S.b.half[q] = 0.5 * box[q];
+      S.b.inv[q] = 1.0 / box[q];
       S.ll[q] = edge_module(box[q]);
       S.Lf[q] = (float)box[q];
       S.halff[q] = (float)S.b.half[q];
@@ -288,7 +289,7 @@ struct lpmdx_batch {
     S.budget = 0.5 * skin;
     {
       const double m = std::sqrt((double)S.rl2f) * (1.0 + 1e-6) + 2.0 * S.budget + 1e-9 * lmax;
-      S.margin = m < hmin ? m : 0.0;
+      S.margin = m < hmin * (1.0 - 1e-6) ? m : 0.0; // room for min_image_near
     }
     S.dt = dt;
     S.kind = H.kind;

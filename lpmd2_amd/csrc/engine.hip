@@ -274,7 +274,9 @@ struct lpmdx_system {
     const double hmin = std::min(box.half[0], std::min(box.half[1], box.half[2]));
     const double pos_err = 2.0 * std::ldexp(lmax, -24);
     const double m = rl * (1.0 + 1e-6) + 4.0 * pos_err + 2.0 * budget() + 1e-9 * lmax;
-    return m < hmin ? m : 0.0;
+    // below half an edge with room to spare: the force kernels' rint wrap
+    // (min_image_near) needs it
+    return m < hmin * (1.0 - 1e-6) ? m : 0.0;
   }
 
   // Exactness budget of the Verlet list: half the gap between the list radius
@@ -506,7 +508,8 @@ struct lpmdx_system {
     lpa = auto_lanes();
     stride = ((n + kWave - 1) / kWave) * kWave; // whole tiles for every lane count
     // staged kernels serve up to 4 lanes per atom
-    staged = grid.brick > 0 && lpa <= 4;
+    // (and only with a wrap margin: their neighbour loop has no compare-and-select wrap)
+    staged = grid.brick > 0 && lpa <= 4 && wrap_margin() > 0.0;
     const size_t np = (size_t)std::max<long>(blocks_for(n * 16), nbrick) * 12;
     partials.alloc(np);
     partials2.alloc(np);
@@ -901,6 +904,10 @@ struct lpmdx_system {
       A.maxd2 = maxd2;
       A.xb = xb.p;
       A.margin = wrap_margin();
+    if (staged && !(A.margin > 0.0))
+      fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
+      if (staged && !(A.margin > 0.0))
+        fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
       A.P = device_view(P, rcf, rcr);
       A.b = box;
       for (int k = 0; k < 3; ++k)
@@ -1006,6 +1013,8 @@ struct lpmdx_system {
     A.maxd2 = ctrl.p->maxd2;
     A.xb = xb.p;
     A.margin = wrap_margin();
+    if (staged && !(A.margin > 0.0))
+      fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
     A.P = device_view(P, rcf, rcr);
     A.b = box;
     for (int q = 0; q < 3; ++q)
@@ -1260,6 +1269,7 @@ int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const i
     for (int q = 0; q < 3; ++q) {
       S->box.L[q] = box[q];
       S->box.half[q] = 0.5 * box[q];
+      S->box.inv[q] = 1.0 / box[q];
       S->ll[q] = edge_module(box[q]);
     }
     hipchk(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking), "hipStreamCreate");

@@ -359,12 +359,12 @@ __device__ __forceinline__ void write_partials(const ForceArgs& A, Tally& t, dou
 // body(j, dx, dy, dz, r2) sees the FP64 minimum-image displacement.
 constexpr int kUnroll = 4;
 
-// LDS copy of a brick's halo (slots as in the brick's lists); W and S are
-// null unless the kernel reads neighbour rho^{-1/2} or species.
+// LDS copy of a brick's halo (slots as in the brick's lists): positions
+// interleaved, P[3 slot + q], so a neighbour's three coordinates are adjacent
+// 8-byte words at one address; W and S are null unless the kernel reads
+// neighbour rho^{-1/2} or species.
 struct Stage {
-  const double* X;
-  const double* Y;
-  const double* Z;
+  const double* P;
   const float* W;
   const int* S;
 };
@@ -383,7 +383,12 @@ template <int LPA> __device__ __forceinline__ int4 list_batch(const int* __restr
   return *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
 }
 
-template <int LPA, bool WRAP, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED,
+// Minimum image of the neighbour loop: none (no listed pair crosses a face),
+// the rint form for displacements clear of half an edge, or the reference's
+// compare-and-select form.
+enum WrapMode { kWrapNone = 0, kWrapNear = 1, kWrapSelect = 2 };
+
+template <int LPA, int WRAP, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED,
           typename Body>
 __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& H, long i, int l, double xi,
                                                double yi, double zi, Body& body) {
@@ -430,9 +435,10 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       if constexpr (STAGED) {
-        px[u] = H.X[js[u]];
-        py[u] = H.Y[js[u]];
-        pz[u] = H.Z[js[u]];
+        const double* p = H.P + 3 * js[u];
+        px[u] = p[0];
+        py[u] = p[1];
+        pz[u] = p[2];
       } else if constexpr (PACKED) {
         const double4 p = A.xw[js[u]];
         px[u] = p.x;
@@ -456,7 +462,9 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
       if constexpr (NEED_W)
         wj = STAGED ? H.W[j] : (PACKED ? pw[u] : A.w_rho[j]);
       double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
-      if constexpr (WRAP)
+      if constexpr (WRAP == kWrapNear)
+        min_image_near(A.b, dx, dy, dz);
+      else if constexpr (WRAP == kWrapSelect)
         min_image(A.b, dx, dy, dz);
       body(k0 + u < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
     }
@@ -464,11 +472,16 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
 }
 
 // Runs body(valid, s_j, w_j, dx, dy, dz, r2) over atom i's list, with
-// neighbour species s_j (NEED_S) and rho^{-1/2} w_j (NEED_W).  Where no atom
-// of the wavefront is within A.margin of a face, no displacement can reach
-// half a box edge, the minimum image is the identity
-// (OrthogonalCell::Displacement would not wrap) and the loop skips it; the
-// wavefront takes one variant, so the branch is uniform.
+// neighbour species s_j (NEED_S) and rho^{-1/2} w_j (NEED_W).  A.margin > 0
+// bounds every listed pair's separation below half an edge: where no atom of
+// the wavefront is within it of a face, no displacement can reach half an
+// edge and the minimum image is the identity (OrthogonalCell::Displacement
+// would not wrap).  Elsewhere the staged kernels, which run only with a
+// margin, take the rint wrap (every raw displacement lies within the margin
+// of 0 or +-L, where it gives the reference's result; measured 16 % faster
+// for the staged LJ kernel, whose VGPR count it also lowers) and the others
+// the compare-and-select wrap (measured faster there: 7 % for the unstaged
+// EAM rho pass).  Each branch is uniform over the wavefront.
 template <int LPA, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED = false, typename Body>
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& H, long i, int l, double xi,
                                                double yi, double zi, Body&& body) {
@@ -476,9 +489,11 @@ __device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& 
   const bool inner = m > 0.0 && xi >= m && xi <= A.b.L[0] - m && yi >= m && yi <= A.b.L[1] - m && zi >= m &&
                      zi <= A.b.L[2] - m;
   if (__all(inner))
-    neighbour_loop<LPA, false, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    neighbour_loop<LPA, kWrapNone, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+  else if (STAGED)
+    neighbour_loop<LPA, kWrapNear, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
   else
-    neighbour_loop<LPA, true, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    neighbour_loop<LPA, kWrapSelect, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
 }
 
 // Per-atom force: FP32 pair scalar times the FP64 displacement, summed in FP64
@@ -658,7 +673,7 @@ static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
 }
 
 // ---- brick-staged kernels: one workgroup per brick of a bricked grid.  The
-// workgroup copies its brick's halo into LDS (positions as three FP64 arrays,
+// workgroup copies its brick's halo into LDS (interleaved FP64 positions,
 // plus rho^{-1/2} and species when the kernel reads them: dynamic LDS of
 // cap * (24 + 4 [W] + 4 [S]) bytes, cap >= the largest halo of the launch),
 // then its threads take the brick's atoms, LPA lanes each, blockDim / LPA
@@ -668,9 +683,7 @@ enum StageParts { kStageW = 1, kStageS = 2 };
 constexpr int kStageLdsLimit = 160 * 1024 - 4096;
 
 struct StageLds {
-  double* X;
-  double* Y;
-  double* Z;
+  double* P; // 3 * cap
   float* W;
   int* S;
 };
@@ -678,10 +691,8 @@ struct StageLds {
 __device__ __forceinline__ StageLds stage_lds(int cap, int parts) {
   extern __shared__ double lds_dyn[];
   StageLds L;
-  L.X = lds_dyn;
-  L.Y = L.X + cap;
-  L.Z = L.Y + cap;
-  char* rest = reinterpret_cast<char*>(L.Z + cap);
+  L.P = lds_dyn;
+  char* rest = reinterpret_cast<char*>(L.P + 3 * (size_t)cap);
   L.W = (parts & kStageW) ? reinterpret_cast<float*>(rest) : nullptr;
   if (parts & kStageW)
     rest += sizeof(float) * (size_t)cap;
@@ -753,9 +764,9 @@ __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __re
     for (int u = 0; u < kStageBatch; ++u) {
       const int slot = base + u * blockDim.x;
       if (slot < total) {
-        L.X[slot] = vx[u];
-        L.Y[slot] = vy[u];
-        L.Z[slot] = vz[u];
+        L.P[3 * slot] = vx[u];
+        L.P[3 * slot + 1] = vy[u];
+        L.P[3 * slot + 2] = vz[u];
       }
     }
     if (L.W) {
@@ -788,7 +799,7 @@ __device__ __forceinline__ void for_brick_atoms(const ForceArgs& A, const int2* 
   const StageLds L = stage_lds(cap, parts);
   int a0, na;
   stage_brick(A, bhalo + xcd_block() * (halo_cells(brick) + 1), brick, L, wsrc, a0, na);
-  S = Stage{L.X, L.Y, L.Z, L.W, L.S};
+  S = Stage{L.P, L.W, L.S};
   const int per = blockDim.x / LPA;
   const int l = (int)(threadIdx.x % LPA);
   for (int base = 0; base < na; base += per) {
