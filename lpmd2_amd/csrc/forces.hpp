@@ -14,6 +14,8 @@
 // direction.
 #pragma once
 
+#include <type_traits>
+
 #include "cells.hpp"
 #include "common.hpp"
 
@@ -416,7 +418,7 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
     if (PREFETCH && kUnroll * l + d * step < c)
       qn[d] = list_batch<LPA>(nb, kUnroll * l + d * step);
   }
-  for (int k0 = kUnroll * l; k0 < c; k0 += step) {
+  auto next = [&](int k0) {
     int4 q;
     if constexpr (PREFETCH) {
       q = qn[0];
@@ -428,14 +430,21 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
     } else {
       q = list_batch<LPA>(nb, k0);
     }
-    const int js[kUnroll] = {k0 < c ? q.x : pad, k0 + 1 < c ? q.y : pad, k0 + 2 < c ? q.z : pad,
-                             k0 + 3 < c ? q.w : pad};
+    return q;
+  };
+  // one batch; FULL: all kUnroll entries are in the list (no masks)
+  auto batch = [&](int k0, const int4& q, auto full) {
+    constexpr bool FULL = decltype(full)::value;
+    const int js[kUnroll] = {FULL || k0 < c ? q.x : pad, FULL || k0 + 1 < c ? q.y : pad,
+                             FULL || k0 + 2 < c ? q.z : pad, FULL || k0 + 3 < c ? q.w : pad};
     double px[kUnroll], py[kUnroll], pz[kUnroll];
     float pw[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       if constexpr (STAGED) {
-        const double* p = H.P + 3 * js[u];
+        // slot j at byte 24 j: one 24-bit multiply-add for the address
+        const double* p = reinterpret_cast<const double*>(reinterpret_cast<const char*>(H.P) +
+                                                          __mul24(js[u], 3 * (int)sizeof(double)));
         px[u] = p[0];
         py[u] = p[1];
         pz[u] = p[2];
@@ -452,6 +461,11 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
         pz[u] = p[2];
       }
     }
+    // the batch's gathers go out together, before any arithmetic uses them
+    // (left to itself the scheduler delays the last one behind the first
+    // three, which exposes its latency)
+    if constexpr (!STAGED)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int j = js[u];
@@ -466,8 +480,20 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
         min_image_near(A.b, dx, dy, dz);
       else if constexpr (WRAP == kWrapSelect)
         min_image(A.b, dx, dy, dz);
-      body(k0 + u < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
+      body(FULL || k0 + u < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
     }
+  };
+  if constexpr (STAGED) {
+    // whole batches, then this lane's partly filled last batch, if any
+    // (measured 2 % faster for the staged LJ kernel)
+    int k0 = kUnroll * l;
+    for (; k0 + kUnroll <= c; k0 += step)
+      batch(k0, next(k0), std::true_type{});
+    if (k0 < c)
+      batch(k0, next(k0), std::false_type{});
+  } else {
+    for (int k0 = kUnroll * l; k0 < c; k0 += step)
+      batch(k0, next(k0), std::false_type{});
   }
 }
 
@@ -517,13 +543,13 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
   if (!SPC) {
     const bool in = valid && r2 < A.P.rc2;
     float f, en;
-    pair_eval<KIND>(A.P, r2, in ? (float)r2 : 1.0f, f, en);
+    pair_eval<KIND>(A.P, r2, valid ? (float)r2 : 1.0f, f, en);
     fs = in ? f : 0.0f;
     e = in ? en : 0.0f;
   } else if (A.nterm <= 1) {
     const bool in = valid && r2 < A.P.rc2 && (A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
     float f, en;
-    pair_eval<KIND>(A.P, r2, in ? (float)r2 : 1.0f, f, en);
+    pair_eval<KIND>(A.P, r2, valid ? (float)r2 : 1.0f, f, en);
     fs = in ? f : 0.0f;
     e = in ? en : 0.0f;
   } else {
@@ -533,7 +559,7 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
       const DevPot& P = A.terms[q];
       const bool in = valid && r2 < P.rc2 && (P.spc_sum < 0 || si + sj == P.spc_sum);
       float f, en;
-      pair_eval<KIND>(P, r2, in ? (float)r2 : 1.0f, f, en);
+      pair_eval<KIND>(P, r2, valid ? (float)r2 : 1.0f, f, en);
       fs += in ? f : 0.0f;
       e += in ? en : 0.0f;
     }
@@ -589,7 +615,7 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H,
     for_neighbours<LPA, STAGED, false, false, PREFETCH>(A, H, i, l, xi, yi, zi,
                                               [&](bool valid, int, float, double, double, double, double r2) {
       const bool in = valid && r2 < rc2;
-      const float v = eam_rho<KIND, EXP>(A.P, in ? (float)r2 : 1.0f);
+      const float v = eam_rho<KIND, EXP>(A.P, valid ? (float)r2 : 1.0f);
       rho += in ? (double)v : 0.0;
     });
   }
@@ -623,7 +649,7 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& 
                                            [&](bool valid, int sj, float wj, double dx, double dy, double dz,
                                                double r2) {
       const bool in = valid && r2 < rc2 && (!SPC || A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
-      const float fr2 = in ? (float)r2 : 1.0f;
+      const float fr2 = valid ? (float)r2 : 1.0f;
       float fs, V;
       eam_pair<KIND, EXP>(A.P, fr2, wi + wj, fs, V);
       fs = in ? fs : 0.0f;
