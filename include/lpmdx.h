@@ -37,6 +37,11 @@ extern "C" {
 #define LPMDX_EDEVICE (-3)    /* no HIP device, or a HIP call failed */
 #define LPMDX_ECAPACITY (-4)  /* a neighbour list reached maxneighbors (verletlist.cc:96-97) */
 #define LPMDX_ESTATE (-5)     /* missing cell manager / potential (configuration.h:59-63) */
+
+/* Potential kinds 1 .. LPMDX_NKINDS, as lpmdx_batch_set_lanes / lanes_out index them:
+ * 1 lennardjones, 2 fastlj, 3 morse, 4 buckingham, 5 suttonchen, 6 finnissinclair,
+ * 7 gupta, 8 harmonic, 9 lennardjonesMod, 10 finnissinclair-ext. */
+#define LPMDX_NKINDS 10
 #define LPMDX_ENOMEM (-6)     /* device allocation or HBM budget exceeded */
 
 /* potential kinds (plugins/<name>.cc) */
@@ -197,8 +202,9 @@ LPMDX_API int lpmdx_batch_get(lpmdx_batch* b, long sid, double* x, double* v, do
 /* Potential energy, virial and kinetic energy of one system at its last step. */
 LPMDX_API int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]);
 /* Times trial_steps steps per candidate lane count (1, 2, 4) for each
- * potential kind on a snapshot of the batch and keeps the fastest; lanes_out[kind]. */
-LPMDX_API int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[8]);
+ * potential kind on a snapshot of the batch and keeps the fastest;
+ * lanes_out[kind] for kind 0 .. LPMDX_NKINDS (LPMDX_NKINDS + 1 entries). */
+LPMDX_API int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1]);
 LPMDX_API int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes);
 /* [0] live systems, [1] atoms, [2] atom capacity, [3] largest free atom run,
  * [4] largest free list run (words), [5] largest free cell run, [6] steps run,

@@ -9,7 +9,8 @@ import numpy as np
 from . import _check, _p, lib
 
 KINDS = {"lennardjones": 1, "fastlj": 2, "morse": 3, "buckingham": 4, "suttonchen": 5, "finnissinclair": 6,
-         "gupta": 7}
+         "gupta": 7, "harmonic": 8, "lennardjonesMod": 9, "finnissinclair-ext": 10}
+NKINDS = 10  # LPMDX_NKINDS
 
 _bound = False
 
@@ -103,7 +104,7 @@ class Batch:
         return float(out[0]), float(out[1]), float(out[2])
 
     def autotune(self, trial_steps: int = 10):
-        lanes = np.zeros(8, dtype=np.int32)
+        lanes = np.zeros(NKINDS + 1, dtype=np.int32)
         _check(lib().lpmdx_batch_autotune(self._h, trial_steps, _p(lanes)))
         return {name: int(lanes[k]) for name, k in KINDS.items()}
 

@@ -8,6 +8,8 @@
 #include "batch.hpp"
 #include "host_util.hpp"
 
+static_assert(lpmdx::kNumKinds == LPMDX_NKINDS, "kind count of the C ABI");
+
 using namespace lpmdx;
 
 namespace {
@@ -102,7 +104,7 @@ struct lpmdx_batch {
     long n = 0;
   };
   std::vector<std::unique_ptr<Group>> groups;
-  int lpa_of_kind[8] = {0, 1, 1, 1, 1, 1, 1, 1};
+  int lpa_of_kind[kNumKinds + 1] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
   // graph of one step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -477,7 +479,7 @@ struct lpmdx_batch {
       auto G = std::make_unique<Group>();
       G->kind = kv.first;
       G->lpa = lpa_of_kind[kv.first];
-      G->metal = kv.first >= kSuttonChen;
+      G->metal = metal_kind(kv.first);
       G->n = (long)kv.second.size();
       G->tiles.alloc(kv.second.size());
       hipchk(hipMemcpyAsync(G->tiles.p, kv.second.data(), kv.second.size() * sizeof(int), hipMemcpyHostToDevice, st),
@@ -538,6 +540,9 @@ struct lpmdx_batch {
       case kSuttonChen: launch_group<kSuttonChen>(*G); break;
       case kFinnisSinclair: launch_group<kFinnisSinclair>(*G); break;
       case kGupta: launch_group<kGupta>(*G); break;
+      case kHarmonic: launch_group<kHarmonic>(*G); break;
+      case kLJMod: launch_group<kLJMod>(*G); break;
+      case kFinnisSinclairExt: launch_group<kFinnisSinclairExt>(*G); break;
       }
     }
     hipchk(hipGetLastError(), "kernel launch");
@@ -662,6 +667,9 @@ struct lpmdx_batch {
       case kSuttonChen: eval_k<kSuttonChen>(sid, lpa, nb, pa, rho_pass); break;
       case kFinnisSinclair: eval_k<kFinnisSinclair>(sid, lpa, nb, pa, rho_pass); break;
       case kGupta: eval_k<kGupta>(sid, lpa, nb, pa, rho_pass); break;
+      case kHarmonic: eval_k<kHarmonic>(sid, lpa, nb, pa, rho_pass); break;
+      case kLJMod: eval_k<kLJMod>(sid, lpa, nb, pa, rho_pass); break;
+      case kFinnisSinclairExt: eval_k<kFinnisSinclairExt>(sid, lpa, nb, pa, rho_pass); break;
       }
     };
     double e_emb = 0.0;
@@ -689,7 +697,7 @@ struct lpmdx_batch {
   // Lanes per atom of each potential kind's force launch, chosen by timing a
   // few steps of this batch per candidate on a snapshot of its state; the
   // snapshot is restored afterwards, so tuning leaves the trajectories alone.
-  void autotune(long trial_steps, int out_lpa[8]) {
+  void autotune(long trial_steps, int out_lpa[kNumKinds + 1]) {
     if (dirty)
       plan();
     pull_state();
@@ -768,7 +776,7 @@ struct lpmdx_batch {
     hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
     sync();
     if (out_lpa)
-      for (int q = 0; q < 8; ++q)
+      for (int q = 0; q <= kNumKinds; ++q)
         out_lpa[q] = lpa_of_kind[q];
   }
 
@@ -853,7 +861,7 @@ int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]) {
   });
 }
 
-int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[8]) {
+int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1]) {
   return guard([&] {
     if (!b)
       fail(LPMDX_EINVAL, "batch is NULL");
@@ -867,8 +875,8 @@ int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes) {
   return guard([&] {
     if (!b)
       fail(LPMDX_EINVAL, "batch is NULL");
-    if (kind < 1 || kind > 7 || !(lanes == 1 || lanes == 2 || lanes == 4))
-      fail(LPMDX_EINVAL, "kind 1..7 and lanes 1, 2 or 4");
+    if (kind < 1 || kind > kNumKinds || !(lanes == 1 || lanes == 2 || lanes == 4))
+      fail(LPMDX_EINVAL, "kind 1.." + std::to_string(kNumKinds) + " and lanes 1, 2 or 4");
     b->lpa_of_kind[kind] = lanes;
     b->pull_state();
     for (size_t s = 0; s < b->hsys.size(); ++s)

@@ -853,6 +853,9 @@ struct lpmdx_system {
     case kSuttonChen: launch_eam_k<kSuttonChen, EVAL>(A, nb, rc2_rho); break;
     case kFinnisSinclair: launch_eam_k<kFinnisSinclair, EVAL>(A, nb, rc2_rho); break;
     case kGupta: launch_eam_k<kGupta, EVAL>(A, nb, rc2_rho); break;
+    case kHarmonic: launch_pair_k<kHarmonic, EVAL>(A, nb); break;
+    case kLJMod: launch_pair_k<kLJMod, EVAL>(A, nb); break;
+    case kFinnisSinclairExt: launch_eam_k<kFinnisSinclairExt, EVAL>(A, nb, rc2_rho); break;
     }
     launch_check();
   }

@@ -28,15 +28,25 @@ enum PotKind {
   kBuckingham = 4,
   kSuttonChen = 5,
   kFinnisSinclair = 6,
-  kGupta = 7
+  kGupta = 7,
+  kHarmonic = 8,
+  kLJMod = 9,
+  kFinnisSinclairExt = 10
 };
+constexpr int kNumKinds = 10; // kinds are 1 .. kNumKinds
+
+// the two-pass (embedded-atom) kinds
+__host__ __device__ constexpr bool metal_kind(int k) {
+  return k == kSuttonChen || k == kFinnisSinclair || k == kGupta || k == kFinnisSinclairExt;
+}
 
 struct DevPot {
   int kind;
   int spc_sum;  // Potential::AppliesTo: negative = all pairs
   double rc2;   // pair filter r2 < rc2 in FP64 (list cutoff and own cutoff)
   float f[12];  // kind-specific FP32 parameters (see PotParams in engine.hip)
-  int ni, mi;   // integer exponents (Sutton-Chen), 0 if not integral
+  int ni, mi;   // integer exponents (Sutton-Chen, lennardjonesMod), 0 if not integral
+  double g[6];  // FP64 parameters (finnissinclair-ext: c0 .. c4, c)
   // fastlj table (FP32 copies of the reference's FP64 tables)
   const float* tab_e;
   const float* tab_f;
@@ -138,6 +148,22 @@ __device__ __forceinline__ void pair_eval(const DevPot& P, double r2d, float r2,
     const float ex = P.f[0] * expf(-r * P.f[3]);
     fscal = P.f[4] * ir6 * ir2 - ex * P.f[3] * ir;
     e = ex - P.f[1] * ir6;
+  } else if constexpr (KIND == kHarmonic) {
+    // harmonic.cc:50-60: E = k/2 (r - a)^2, ff = k (r - a) / r; f = {k, a, k/2}
+    const float ir = rsq(r2);
+    const float t = r2 * ir - P.f[1];
+    fscal = P.f[0] * t * ir;
+    e = P.f[2] * t * t;
+  } else if constexpr (KIND == kLJMod) {
+    // lennardjonesMod.cc:65-79 with t = sigma^2 / r^2: (sigma/r)^m = t^{m/2},
+    // E = 4 eps (t^{m/2} - c t^{n/2}), ff = -4 eps / r^2 (m t^{m/2} - c n t^{n/2});
+    // f = {sigma^2, 4 eps, m, c n, c, m/2, n/2}, mi / ni = m/2, n/2 when integral
+    const float inv = rcp(r2);
+    const float t = P.f[0] * inv;
+    const float tm = powk(t, P.mi, P.f[5]);
+    const float tn = powk(t, P.ni, P.f[6]);
+    fscal = -P.f[1] * inv * (P.f[2] * tm - P.f[3] * tn);
+    e = P.f[1] * (tm - P.f[4] * tn);
   }
 }
 
@@ -151,6 +177,14 @@ template <int KIND, int EXP = 0> __device__ __forceinline__ float eam_rho(const 
   } else if constexpr (KIND == kGupta) { // gupta.cc:69
     const float r = r2 * rsq(r2);
     return expf(P.f[6] - P.f[7] * r);
+  } else if constexpr (KIND == kFinnisSinclairExt) { // finnissinclair-ext.cc:93-99
+    const float r = r2 * rsq(r2);
+    const float d = P.f[6];
+    if (r <= d) {
+      const float t = r - d;
+      return (t * t) * (1.0f + P.f[8] * t * t);
+    }
+    return 0.0f;
   } else { // finnissinclair.cc:82-87
     const float r = r2 * rsq(r2);
     const float d = P.f[6];
@@ -180,6 +214,26 @@ __device__ __forceinline__ void eam_pair(const DevPot& P, float r2, float wsum, 
     const float eq = expf(P.f[6] - P.f[7] * r);
     V = P.f[0] * ep;
     fscal = ir * (P.f[9] * ep + P.f[10] * wsum * eq);
+  } else if constexpr (KIND == kFinnisSinclairExt) {
+    // finnissinclair-ext.cc:85-123, f = {c0, c1, c2, A, c3, c, d, c4, B^2, 4 B^2, -A/2, K = A},
+    // g = {c0, c1, c2, c3, c4, c}.  The pair polynomial runs in FP64: its
+    // terms cancel (Mo: c0 48, c1 r -96 and c2 r^2 46 sum to -0.2 near the
+    // first shell), which FP32 would leave good to 1e-4 only.
+    const float r = r2 * ir;
+    const float c = P.f[5], d = P.f[6];
+    float dV = 0.0f, G = 0.0f;
+    V = 0.0f;
+    if (r <= c) {
+      const double rd = (double)r, t = rd - P.g[5], rr = rd * rd;
+      const double poly = P.g[0] + P.g[1] * rd + P.g[2] * rr + P.g[3] * rr * rd + P.g[4] * rr * rr;
+      V = (float)(t * t * poly);
+      dV = (float)(2.0 * t * poly + (P.g[1] + 2.0 * P.g[2] * rd + 3.0 * P.g[3] * rr + 4.0 * P.g[4] * rr * rd) * t * t);
+    }
+    if (r <= d) {
+      const float t = r - d;
+      G = P.f[10] * (2.0f * t + P.f[9] * t * t * t);
+    }
+    fscal = (dV + wsum * G) * ir;
   } else {
     // finnissinclair.cc:75-108, f = {c0, c1, c2, A, B, c, d, -, B/d, 3B/d, -A/2}
     const float r = r2 * ir;

@@ -130,10 +130,10 @@ struct PotParams {
   int kind = 0;
   int spc_sum = -2;
   double cutoff = HUGE_VAL; // Potential::rcutoff default (potential.cc:13)
-  double p[8] = {0};
+  double p[10] = {0};
   long bins = 0;
   DBuf<float> tab_e, tab_f;
-  bool metal() const { return kind >= kSuttonChen; }
+  bool metal() const { return metal_kind(kind); }
 };
 
 inline std::unique_ptr<PotParams> make_potential(const std::string& name, const char* args, int s1, int s2) {
@@ -203,18 +203,40 @@ inline std::unique_ptr<PotParams> make_potential(const std::string& name, const 
     for (int q = 0; q < 5; ++q)
       P->p[q] = k.d(keys[q]);
     P->cutoff = P->p[5] = k.d("cutoff");
+  } else if (name == "harmonic") { // plugins/harmonic.cc:11-19 (no cutoff keyword)
+    Keywords k(args, {});
+    P->kind = kHarmonic;
+    P->p[0] = k.d("k");
+    P->p[1] = k.d("a");
+  } else if (name == "lennardjonesMod") { // plugins/lennardjonesMod.cc:11-28
+    Keywords k(args, {{"m", "12"}, {"n", "6"}, {"cohesive", "1"}});
+    P->kind = kLJMod;
+    P->p[0] = k.d("sigma");
+    P->p[1] = k.d("epsilon");
+    P->p[2] = k.d("m");
+    P->p[3] = k.d("n");
+    P->p[4] = k.d("cohesive");
+    P->cutoff = P->p[5] = k.d("cutoff");
+  } else if (name == "finnissinclair-ext") { // plugins/finnissinclair-ext.cc:11-36
+    Keywords k(args, {});
+    P->kind = kFinnisSinclairExt;
+    const char* keys[9] = {"c0", "c1", "c2", "c3", "c4", "A", "B", "c", "d"};
+    for (int q = 0; q < 9; ++q)
+      P->p[q] = k.d(keys[q]);
+    P->cutoff = (P->p[8] >= P->p[7]) ? P->p[8] : P->p[7];
   } else {
     fail(LPMDX_EINVAL, "unknown potential plugin '" + name + "'");
   }
-  for (int q = 0; q < 8; ++q)
+  for (int q = 0; q < 10; ++q)
     if (!std::isfinite(P->p[q]))
       fail(LPMDX_ENONFINITE, name + ": non-finite parameter");
   return P;
 }
 
+// v when it is an integer that powu takes (1 .. 31), else 0 (powf at run time)
 inline int int_exponent(double v) {
   double r = std::nearbyint(v);
-  return (r == v && r >= 1 && r <= 64) ? (int)r : 0;
+  return (r == v && r >= 1 && r <= 31) ? (int)r : 0;
 }
 
 inline DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) {
@@ -279,6 +301,42 @@ inline DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) 
     d.f[9] = (float)(3.0 * p[4] / p[6]);
     d.f[10] = (float)(-0.5 * p[3]);
     d.f[11] = (float)p[3];
+    break;
+  case kHarmonic:
+    // {k, a, k/2}
+    d.f[0] = (float)p[0];
+    d.f[1] = (float)p[1];
+    d.f[2] = (float)(0.5 * p[0]);
+    break;
+  case kLJMod:
+    // {sigma^2, 4 eps, m, c n, c, m/2, n/2}; mi, ni: m/2, n/2 when integral
+    d.f[0] = (float)(p[0] * p[0]);
+    d.f[1] = (float)(4.0 * p[1]);
+    d.f[2] = (float)p[2];
+    d.f[3] = (float)(p[4] * p[3]);
+    d.f[4] = (float)p[4];
+    d.f[5] = (float)(0.5 * p[2]);
+    d.f[6] = (float)(0.5 * p[3]);
+    d.mi = int_exponent(0.5 * p[2]);
+    d.ni = int_exponent(0.5 * p[3]);
+    break;
+  case kFinnisSinclairExt:
+    // p = {c0, c1, c2, c3, c4, A, B, c, d} -> f = {c0, c1, c2, A, c3, c, d, c4, B^2, 4 B^2, -A/2, K = A}
+    d.f[0] = (float)p[0];
+    d.f[1] = (float)p[1];
+    d.f[2] = (float)p[2];
+    d.f[3] = (float)p[5];
+    d.f[4] = (float)p[3];
+    d.f[5] = (float)p[7];
+    d.f[6] = (float)p[8];
+    d.f[7] = (float)p[4];
+    d.f[8] = (float)(p[6] * p[6]);
+    d.f[9] = (float)(4.0 * p[6] * p[6]);
+    d.f[10] = (float)(-0.5 * p[5]);
+    d.f[11] = (float)p[5];
+    for (int q = 0; q < 5; ++q)
+      d.g[q] = p[q];
+    d.g[5] = p[7];
     break;
   }
   d.tab_cut = P.kind == kFastLJ ? p[2] : 0.0;
@@ -357,6 +415,7 @@ inline double ElemMass(int z) {
   case 26: return 55.847;
   case 28: return 58.6934;
   case 29: return 63.546;
+  case 42: return 95.94;
   case 47: return 107.8682;
   case 79: return 196.96654;
   default: return -1.0;

@@ -82,7 +82,7 @@ struct lpmdx_server {
   std::map<long, std::unique_ptr<Job>> jobs;
   std::deque<Job*> queue;
   std::map<long, Job*> resident; // batch slot -> job (worker-owned; changed under mu)
-  int lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int lanes[LPMDX_NKINDS + 1] = {0};
   bool lanes_dirty = false;
   // statistics (under mu)
   std::vector<double> latency;
@@ -235,7 +235,7 @@ struct lpmdx_server {
           return;
         arr = arrivals;
         if (lanes_dirty) {
-          for (int k = 1; k < 8; ++k)
+          for (int k = 1; k <= LPMDX_NKINDS; ++k)
             if (lanes[k])
               (void)lpmdx_batch_set_lanes(b, k, lanes[k]);
           lanes_dirty = false;
@@ -457,8 +457,8 @@ int lpmdx_server_set_lanes(lpmdx_server* s, int kind, int lanes) {
   return guard([&] {
     if (!s)
       fail(LPMDX_EINVAL, "server is NULL");
-    if (kind < 1 || kind > 7 || !(lanes == 1 || lanes == 2 || lanes == 4))
-      fail(LPMDX_EINVAL, "kind 1..7 and lanes 1, 2 or 4");
+    if (kind < 1 || kind > LPMDX_NKINDS || !(lanes == 1 || lanes == 2 || lanes == 4))
+      fail(LPMDX_EINVAL, "kind 1.." + std::to_string(LPMDX_NKINDS) + " and lanes 1, 2 or 4");
     std::lock_guard<std::mutex> lk(s->mu);
     s->lanes[kind] = lanes;
     s->lanes_dirty = true;

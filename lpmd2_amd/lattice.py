@@ -13,6 +13,7 @@ ELEMENT_MASS = {
     26: 55.847,
     28: 58.6934,
     29: 63.546,
+    42: 95.94,
     47: 107.8682,
     79: 196.96654,
 }

@@ -39,6 +39,12 @@ const std::map<std::string, std::vector<Keyword>>& Keywords() {
        {{"c0", nullptr}, {"c1", nullptr}, {"c2", nullptr}, {"A", nullptr}, {"B", nullptr}, {"c", nullptr},
         {"d", nullptr}}},
       {"gupta", {{"A", nullptr}, {"r0", nullptr}, {"p", nullptr}, {"B", nullptr}, {"q", nullptr}, {"cutoff", nullptr}}},
+      {"harmonic", {{"k", nullptr}, {"a", nullptr}}},
+      {"lennardjonesMod",
+       {{"sigma", nullptr}, {"epsilon", nullptr}, {"m", "12"}, {"n", "6"}, {"cohesive", "1"}, {"cutoff", nullptr}}},
+      {"finnissinclair-ext",
+       {{"c0", nullptr}, {"c1", nullptr}, {"c2", nullptr}, {"c3", nullptr}, {"c4", nullptr}, {"A", nullptr},
+        {"B", nullptr}, {"c", nullptr}, {"d", nullptr}}},
       {"linkedcell",
        {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}, {"warn-outside", "true"}}},
       {"verletlist", {{"cutoff", "0.0"}, {"extcutoff", "0.0"}, {"maxneighbors", "80"}, {"each", "5"}, {"skin", "0"}}},
@@ -251,10 +257,12 @@ DevicePotential::DevicePotential(const std::string& name, std::string args)
     : lpmd::Plugin(name, kVersion), kind_(name), args_(args) {
   Define(*this, name);
   ProcessArguments(args);
-  if (name == "finnissinclair") {
+  if (name == "finnissinclair" || name == "finnissinclair-ext") {
+    // cutoff = max(c, d) (finnissinclair.cc:10-31, finnissinclair-ext.cc:11-36)
     const double c = double((*this)["c"]), d = double((*this)["d"]);
     SetCutoff(d >= c ? d : c);
-  } else if (name != "morse") { // morse keeps Potential's HUGE_VAL (plugins/morse.cc:11-21)
+  } else if (name != "morse" && name != "harmonic") {
+    // morse and harmonic keep Potential's HUGE_VAL (morse.cc:11-21, harmonic.cc:11-19)
     SetCutoff(double((*this)["cutoff"]));
   }
 }
@@ -406,8 +414,9 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
 // ------------------------------------------------------------------ factory
 
 lpmd::Plugin* Create(const std::string& name, const std::string& args) {
-  static const char* pots[] = {"lennardjones", "fastlj", "morse", "buckingham", "suttonchen", "finnissinclair",
-                               "gupta"};
+  static const char* pots[] = {"lennardjones", "fastlj",   "morse",           "buckingham",
+                               "suttonchen",   "finnissinclair", "gupta", "harmonic",
+                               "lennardjonesMod", "finnissinclair-ext"};
   for (const char* p : pots)
     if (name == p)
       return new DevicePotential(name, args);

@@ -14,6 +14,7 @@ CORE     := oracle/_ref
 CXX      ?= g++
 CXXFLAGS := -O2 -std=c++17 -fPIC -w -I$(REF)/liblpmd -Iinclude
 NAMES    := lennardjones fastlj morse buckingham suttonchen finnissinclair gupta \
+            harmonic lennardjonesMod finnissinclair-ext \
             linkedcell verletlist minimumimage lcbinary velocityverlet
 RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..' -Wl,-rpath,'$$ORIGIN/../../../oracle/_ref'
 

@@ -12,6 +12,7 @@ MASS = {
     26: 55.847,      # Fe
     28: 58.6934,     # Ni
     29: 63.546,      # Cu
+    42: 95.94,       # Mo
     47: 107.8682,    # Ag
     79: 196.96654,   # Au
 }

@@ -636,7 +636,10 @@ long orc_cm_all_lists(orc_cm* cm, const orc_conf* c, int full, double rcut, long
 /* potentials                                                                 */
 /* ------------------------------------------------------------------------ */
 
-int orc_pot_is_metal(const orc_pot* p) { return p->kind >= ORC_SUTTONCHEN; }
+int orc_pot_is_metal(const orc_pot* p) {
+  return p->kind == ORC_SUTTONCHEN || p->kind == ORC_FINNISSINCLAIR || p->kind == ORC_GUPTA ||
+         p->kind == ORC_FINNISSINCLAIREXT;
+}
 
 /* Potential::SetValidSpecies / AppliesTo, liblpmd/simulation/potential.cc:23-27 */
 static inline int applies_to(const orc_pot* p, int s1, int s2) {
@@ -692,6 +695,17 @@ int orc_pot_init(orc_pot* p, int kind, const double* a, int s1, int s2) {
     memcpy(p->p, a, 6 * sizeof(double));
     p->cutoff = a[5];
     break;
+  case ORC_HARMONIC: /* harmonic.cc:11-19: no SetCutoff, rcutoff stays HUGE_VAL */
+    memcpy(p->p, a, 2 * sizeof(double));
+    break;
+  case ORC_LJMOD: /* lennardjonesMod.cc:11-28 */
+    memcpy(p->p, a, 6 * sizeof(double));
+    p->cutoff = a[5];
+    break;
+  case ORC_FINNISSINCLAIREXT: /* finnissinclair-ext.cc:11-36: cutoff = max(c, d) */
+    memcpy(p->p, a, 9 * sizeof(double));
+    p->cutoff = (a[8] >= a[7]) ? a[8] : a[7];
+    break;
   default:
     return fail("unknown potential kind");
   }
@@ -742,6 +756,20 @@ double orc_pair_energy(const orc_pot* p, double r) {
     return 0.0e0;
   case ORC_GUPTA: /* gupta.cc:67 */
     return a[0] * exp(-a[2] * (r - a[1]) / a[1]);
+  case ORC_HARMONIC: /* harmonic.cc:50-52 */
+    return 0.5e0 * a[0] * (fabs(r) - a[1]) * (fabs(r) - a[1]);
+  case ORC_LJMOD: { /* lennardjonesMod.cc:65-70 */
+    double rtmp = a[0] / r;
+    double rn = pow(rtmp, a[3]);
+    double rm = pow(rtmp, a[2]);
+    return 4.0e0 * a[1] * (rm - a[4] * rn);
+  }
+  case ORC_FINNISSINCLAIREXT: /* finnissinclair-ext.cc:85-91 */
+    if (r <= a[7]) {
+      double r2 = r * r;
+      return (r - a[7]) * (r - a[7]) * (a[0] + a[1] * r + a[2] * r2 + a[3] * r2 * r + a[4] * r2 * r2);
+    }
+    return 0.0e0;
   }
   return NAN;
 }
@@ -777,6 +805,18 @@ void orc_pair_force(const orc_pot* p, const double rij[3], double out[3]) {
     ff = f6 - pe;
     break;
   }
+  case ORC_HARMONIC: { /* harmonic.cc:54-60 */
+    double rr = sqrt(sqmod(rij));
+    ff = a[0] * (rr - a[1]) / rr;
+    break;
+  }
+  case ORC_LJMOD: { /* lennardjonesMod.cc:72-79 */
+    double rr2 = sqmod(rij);
+    double rn = pow(a[0] * a[0] / rr2, a[3] / 2.0);
+    double rm = pow(a[0] * a[0] / rr2, a[2] / 2.0);
+    ff = -4.0e0 * (a[1] / rr2) * (a[2] * rm - a[4] * a[3] * rn);
+    break;
+  }
   }
   for (int q = 0; q < 3; ++q)
     out[q] = rij[q] * ff;
@@ -795,6 +835,12 @@ double orc_rhoij(const orc_pot* p, double r) {
     return 0.0e0;
   case ORC_GUPTA: /* gupta.cc:69 */
     return exp(-2.0e0 * a[4] * (r - a[1]) / a[1]);
+  case ORC_FINNISSINCLAIREXT: /* finnissinclair-ext.cc:93-99 */
+    if (r <= a[8]) {
+      double rd = r - a[8];
+      return (rd * rd) * (1.0e0 + a[6] * a[6] * rd * rd);
+    }
+    return 0.0e0;
   }
   return NAN;
 }
@@ -808,6 +854,8 @@ double orc_embed(const orc_pot* p, double rho) {
     return -a[3] * sqrt(rho);
   case ORC_GUPTA: /* gupta.cc:71 */
     return -a[3] * sqrt(rho);
+  case ORC_FINNISSINCLAIREXT: /* finnissinclair-ext.cc:101 */
+    return -a[5] * sqrt(rho);
   }
   return NAN;
 }
@@ -833,6 +881,18 @@ void orc_metal_pairforce(const orc_pot* p, const double nrm[3], double r, double
     break;
   case ORC_GUPTA: /* gupta.cc:73-75 */
     s = -(a[0] * a[2] / a[1]) * exp(-a[2] * (r - a[1]) / a[1]);
+    break;
+  case ORC_FINNISSINCLAIREXT: /* finnissinclair-ext.cc:103-112 */
+    if (r <= a[7]) {
+      double r2 = r * r;
+      double rc = r - a[7];
+      double t1 = 2.0e0 * rc * (a[0] + a[1] * r + a[2] * r2 + a[3] * r2 * r + a[4] * r2 * r2);
+      double t2 = (a[1] + 2.0e0 * a[2] * r + 3.0e0 * a[3] * r2 + 4.0e0 * a[4] * r2 * r) * rc * rc;
+      s = t1 + t2;
+    } else {
+      out[0] = out[1] = out[2] = 0.0;
+      return;
+    }
     break;
   }
   for (int q = 0; q < 3; ++q)
@@ -863,6 +923,17 @@ void orc_metal_manybodies(const orc_pot* p, const double nrm[3], double rhoi, do
   case ORC_GUPTA: /* gupta.cc:77-82 */
     s = (a[3] * a[4] / a[1]) * ((1.0e0 / sqrt(rhoi)) + (1.0e0 / sqrt(rhoj))) *
         exp(-2.0e0 * a[4] * (r - a[1]) / a[1]);
+    break;
+  case ORC_FINNISSINCLAIREXT: /* finnissinclair-ext.cc:114-123 */
+    if (r <= a[8]) {
+      double rd = r - a[8];
+      double t1 = a[5] * 0.5e0 * ((1.0e0 / sqrt(rhoi)) + (1.0e0 / sqrt(rhoj)));
+      double t2 = 2.0e0 * rd + 4.0e0 * a[6] * a[6] * rd * rd * rd;
+      s = -t1 * t2;
+    } else {
+      out[0] = out[1] = out[2] = 0.0;
+      return;
+    }
     break;
   }
   for (int q = 0; q < 3; ++q)

@@ -71,14 +71,17 @@ enum {
   ORC_BUCKINGHAM = 4,
   ORC_SUTTONCHEN = 5,
   ORC_FINNISSINCLAIR = 6,
-  ORC_GUPTA = 7
+  ORC_GUPTA = 7,
+  ORC_HARMONIC = 8,
+  ORC_LJMOD = 9,
+  ORC_FINNISSINCLAIREXT = 10
 };
 
 typedef struct {
   int kind;
   int spc_sum;    /* Potential::SetValidSpecies: s1+s2, negative = wildcard */
   double cutoff;  /* Potential::rcutoff */
-  double p[8];    /* kind-specific parameters in keyword order */
+  double p[10];   /* kind-specific parameters in keyword order */
   long bins;      /* fastlj */
   double* etable; /* fastlj */
   double* fftable;
@@ -89,7 +92,9 @@ typedef struct {
  *  LJ: sigma epsilon cutoff          FASTLJ: sigma epsilon cutoff bins
  *  MORSE: depth a re                 BUCKINGHAM: B1 B2 Ro cutoff
  *  SUTTONCHEN: e a n m c cutoff      FINNISSINCLAIR: c0 c1 c2 A B c d
- *  GUPTA: A r0 p B q cutoff */
+ *  GUPTA: A r0 p B q cutoff          HARMONIC: k a
+ *  LJMOD: sigma epsilon m n cohesive cutoff
+ *  FINNISSINCLAIREXT: c0 c1 c2 c3 c4 A B c d */
 int orc_pot_init(orc_pot* p, int kind, const double* params, int s1, int s2);
 void orc_pot_free(orc_pot* p);
 int orc_pot_is_metal(const orc_pot* p);

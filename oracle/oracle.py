@@ -14,7 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblpmd_oracle.so")
 
-LJ, FASTLJ, MORSE, BUCKINGHAM, SUTTONCHEN, FINNISSINCLAIR, GUPTA = range(1, 8)
+LJ, FASTLJ, MORSE, BUCKINGHAM, SUTTONCHEN, FINNISSINCLAIR, GUPTA, HARMONIC, LJMOD, FINNISSINCLAIREXT = range(1, 11)
+METAL = (SUTTONCHEN, FINNISSINCLAIR, GUPTA, FINNISSINCLAIREXT)
 KIND = {
     "lennardjones": LJ,
     "fastlj": FASTLJ,
@@ -23,6 +24,9 @@ KIND = {
     "suttonchen": SUTTONCHEN,
     "finnissinclair": FINNISSINCLAIR,
     "gupta": GUPTA,
+    "harmonic": HARMONIC,
+    "lennardjonesMod": LJMOD,
+    "finnissinclair-ext": FINNISSINCLAIREXT,
 }
 # keyword order of each plugin (plugins/<name>.cc constructors)
 KEYWORDS = {
@@ -33,8 +37,11 @@ KEYWORDS = {
     "suttonchen": ("e", "a", "n", "m", "c", "cutoff"),
     "finnissinclair": ("c0", "c1", "c2", "A", "B", "c", "d"),
     "gupta": ("A", "r0", "p", "B", "q", "cutoff"),
+    "harmonic": ("k", "a"),
+    "lennardjonesMod": ("sigma", "epsilon", "m", "n", "cohesive", "cutoff"),
+    "finnissinclair-ext": ("c0", "c1", "c2", "c3", "c4", "A", "B", "c", "d"),
 }
-DEFAULTS = {"fastlj": {"bins": 500.0}}
+DEFAULTS = {"fastlj": {"bins": 500.0}, "lennardjonesMod": {"m": 12.0, "n": 6.0, "cohesive": 1.0}}
 
 FORCEFACTOR = 0.00964853
 KIN2EV = 103.64269
@@ -72,7 +79,7 @@ class _Pot(C.Structure):
         ("kind", C.c_int),
         ("spc_sum", C.c_int),
         ("cutoff", C.c_double),
-        ("p", C.c_double * 8),
+        ("p", C.c_double * 10),
         ("bins", C.c_long),
         ("etable", C.c_void_p),
         ("fftable", C.c_void_p),
@@ -231,7 +238,7 @@ class Pot:
     def __init__(self, name: str, s1: int = -1, s2: int = -1, **kw):
         vals = dict(DEFAULTS.get(name, {}))
         vals.update(kw)
-        params = np.zeros(8)
+        params = np.zeros(10)
         for q, k in enumerate(KEYWORDS[name]):
             params[q] = float(vals[k])
         self.name = name
@@ -248,7 +255,7 @@ class Pot:
 
     @property
     def metal(self):
-        return self.s.kind >= SUTTONCHEN
+        return self.s.kind in METAL
 
     def pair_energy(self, r):
         return np.array([lib().orc_pair_energy(C.byref(self.s), float(x)) for x in np.atleast_1d(r)])

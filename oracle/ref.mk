@@ -25,7 +25,8 @@ CORE_SRCS := \
   liblpmd/simulation/simulation.cc liblpmd/simulation/systemmodifier.cc
 
 PLUGINS := linkedcell verletlist minimumimage lcbinary lennardjones fastlj morse buckingham \
-           suttonchen finnissinclair gupta crystal3d velocityverlet
+           suttonchen finnissinclair gupta harmonic lennardjonesMod finnissinclair-ext \
+           crystal3d velocityverlet
 
 CORE_OBJS := $(patsubst %.cc,$(OUT)/obj/%.o,$(CORE_SRCS))
 PLUGIN_SOS := $(patsubst %,$(OUT)/%.so,$(PLUGINS))

@@ -8,7 +8,7 @@ first (a partial build of the reference's plugins, see oracle/ref.mk), then
 Every fixture holds inputs and the reference's outputs for them: neighbour
 lists from plugins/linkedcell.cc, plugins/verletlist.cc, plugins/minimumimage.cc
 and plugins/lcbinary.cc, per-pair values of
-the seven potential plugins, crystal3d lattices and velocityverlet
+the ten potential plugins, crystal3d lattices and velocityverlet
 trajectories.  Large lists are stored as SHA-256 digests of their exact bytes.
 """
 from __future__ import annotations
@@ -39,7 +39,15 @@ POTS = {
     "finnissinclair": dict(c0=1.2371147, c1=-0.3592185, c2=-0.0385607, A=1.828905, B=1.8, c=3.4, d=3.569745),
     # Gupta Cu (Cleri-Rosato)
     "gupta": dict(A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=7.22),
+    # the plugin's help example, no cutoff keyword
+    "harmonic": dict(k=3.4, a=3.0),
+    # Ar sigma/epsilon with exponents 10 and 5 (m/2 integral, n/2 not) and partial cohesion
+    "lennardjonesMod": dict(sigma=3.41, epsilon=0.0103, m=10.0, n=5.0, cohesive=0.8, cutoff=8.5),
+    # Mo (the plugin's help example) with B = 1.2 so that the rho term's B is exercised
+    "finnissinclair-ext": dict(c0=47.98066, c1=-34.09924, c2=5.832293, c3=0.017494, c4=0.020393,
+                               A=1.848648, B=1.2, c=3.2572, d=4.1472),
 }
+METALS = ("suttonchen", "finnissinclair", "gupta", "finnissinclair-ext")
 
 
 def args_of(kw):
@@ -129,7 +137,7 @@ def main():
         rij = rng.normal(0, 1, (256, 3))
         rij *= (rng.uniform(1.9, rmax, 256) / np.linalg.norm(rij, axis=1))[:, None]
         rec = {"rij": rij}
-        if name in ("suttonchen", "finnissinclair", "gupta"):
+        if name in METALS:
             rhoi = rng.uniform(2.0, 80.0, 256)
             rhoj = rng.uniform(2.0, 80.0, 256)
             ep, rho, fr, pf, mb = rp.metal(rij, rhoi, rhoj)

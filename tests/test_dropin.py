@@ -19,10 +19,15 @@ POTS = [
     ("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22"),
     ("finnissinclair", "c0 1.2371147 c1 -0.3592185 c2 -0.0385607 A 1.828905 B 1.8 c 3.4 d 3.569745"),
     ("gupta", "A 0.0855 r0 2.556 p 10.960 B 1.2240 q 2.278 cutoff 7.22"),
+    ("harmonic", "k 3.4 a 3.0"),
+    ("lennardjonesMod", "sigma 3.41 epsilon 0.0103 m 10 n 5 cohesive 0.8 cutoff 8.5"),
+    ("lennardjonesMod", "sigma 3.41 epsilon 0.0103 cutoff 8.5"),
+    ("finnissinclair-ext", "c0 47.98066 c1 -34.09924 c2 5.832293 c3 0.017494 c4 0.020393 A 1.848648 B 1.2 "
+                           "c 3.2572 d 4.1472"),
 ]
 
 
-@pytest.mark.parametrize("name,args", POTS, ids=[p[0] for p in POTS])
+@pytest.mark.parametrize("name,args", POTS, ids=[f"{p[0]}-{k}" for k, p in enumerate(POTS)])
 def test_potential_plugins_match_reference_keywords(name, args):
     ours = R.RefPlugin.dropin(name, args)
     ref = R.RefPlugin.reference(name, args)

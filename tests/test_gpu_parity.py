@@ -73,6 +73,10 @@ def test_lists_match_golden(case):
 
 
 # --------------------------------------------------------------- forces
+# Mo finnissinclair-ext (the plugin's help example) with B = 1.2
+FSX = dict(c0=47.98066, c1=-34.09924, c2=5.832293, c3=0.017494, c4=0.020393, A=1.848648, B=1.2, c=3.2572,
+           d=4.1472)
+FSX_ARGS = " ".join(f"{k} {v}" for k, v in FSX.items())
 POTS = [
     ("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5", dict(sigma=3.41, epsilon=0.0103, cutoff=8.5), 18, 5.26, 0.1),
     ("fastlj", "sigma 3.41 epsilon 0.0103 cutoff 8.5", dict(sigma=3.41, epsilon=0.0103, cutoff=8.5, bins=500), 18, 5.26, 0.1),
@@ -84,6 +88,10 @@ POTS = [
      dict(c0=1.2371147, c1=-0.3592185, c2=-0.0385607, A=1.828905, B=1.8, c=3.4, d=3.569745), 26, 2.87 * 1.26, 0.05),
     ("gupta", "A 0.0855 r0 2.556 p 10.960 B 1.2240 q 2.278 cutoff 7.22",
      dict(A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=7.22), 29, 3.61, 0.05),
+    ("harmonic", "k 3.4 a 3.0", dict(k=3.4, a=3.0), 29, 3.61, 0.2),
+    ("lennardjonesMod", "sigma 3.41 epsilon 0.0103 m 10 n 5 cohesive 0.8 cutoff 8.5",
+     dict(sigma=3.41, epsilon=0.0103, m=10, n=5, cohesive=0.8, cutoff=8.5), 18, 5.26, 0.1),
+    ("finnissinclair-ext", FSX_ARGS, FSX, 42, 4.0, 0.05),
 ]
 
 

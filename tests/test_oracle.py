@@ -296,6 +296,10 @@ POT_CASES = [
     ("suttonchen", dict(e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=7.0), 29, 3.61, 7.0),
     ("finnissinclair", dict(c0=1.2371147, c1=-0.3592185, c2=-0.0385607, A=1.828905, B=1.8, c=3.4, d=3.569745), 26, 2.87 * 1.26, 4.0),
     ("gupta", dict(A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=7.0), 29, 3.61, 7.0),
+    ("harmonic", dict(k=3.4, a=3.0), 29, 3.61, 5.0),
+    ("lennardjonesMod", dict(LJ, m=10, n=5, cohesive=0.8), 18, 5.26, 8.5),
+    ("finnissinclair-ext", dict(c0=47.98066, c1=-34.09924, c2=5.832293, c3=0.017494, c4=0.020393, A=1.848648, B=1.2,
+                                c=3.2572, d=4.1472), 42, 4.0, 4.2),
 ]
 
 
