@@ -48,7 +48,11 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
 # 5 for t^9 and t^6 at the common exponents, 6 for the force scalar = 13).
 FP64_PER_LISTED_PAIR = 8
 FP64_PER_CUT_PAIR = 6
-FP32_PER_CUT_PAIR = {"lennardjones": 9, "suttonchen": 13}
+FP32_PER_CUT_PAIR = {"lennardjones": 9, "suttonchen": 13,
+                     # harmonic: r from rsq, (r - a), k (r - a) / r; lennardjonesMod: t = sigma^2 / r^2,
+                     # t^5 by squaring, t^2.5 through powf (counted as 4), the force scalar;
+                     # finnissinclair-ext: r, the rho-derivative term (the pair polynomial is FP64)
+                     "harmonic": 4, "lennardjonesMod": 14, "finnissinclair-ext": 9}
 
 WORKLOADS = {
     "sc_cu_131k": dict(
@@ -66,6 +70,35 @@ WORKLOADS = {
         cm=("verletlist", f"cutoff {2.5 * 3.41} maxneighbors 400 each 5 skin {0.3 * 3.41}"),
         rc=2.5 * 3.41, skin=0.3 * 3.41,
         metal=False,
+    ),
+    # the widened potentials, one 131072-atom box each
+    "harmonic_131k": dict(
+        desc="harmonic springs k 3.4 a 3.0 on fcc 32^3 = 131072 atoms, a 4.243 (nearest neighbours at the spring "
+             "length), list cutoff 3.5, skin 0.3, dt 1 fs, 300 K",
+        nx=32, a=4.243, z=18, T=300.0, noise=0.01, dt=1.0, seed=11,
+        pot=("harmonic", "k 3.4 a 3.0"),
+        cm=("verletlist", "cutoff 3.5 maxneighbors 100 each 5 skin 0.3"),
+        rc=3.5, skin=0.3,
+        metal=False,
+    ),
+    "ljmod_ar_131k": dict(
+        desc="lennardjonesMod Ar m 10 n 5 cohesive 0.8 on fcc 32^3 = 131072 atoms, a 5.26, rc 8.5, skin 1.0, "
+             "dt 2 fs, 100 K",
+        nx=32, a=5.26, z=18, T=100.0, noise=0.01, dt=2.0, seed=12,
+        pot=("lennardjonesMod", "sigma 3.41 epsilon 0.0103 m 10 n 5 cohesive 0.8 cutoff 8.5"),
+        cm=("verletlist", "cutoff 8.5 maxneighbors 400 each 5 skin 1.0"),
+        rc=8.5, skin=1.0,
+        metal=False,
+    ),
+    "fsext_mo_131k": dict(
+        desc="finnissinclair-ext Mo (c 3.2572, d 4.1472, B 1.2) on fcc 32^3 = 131072 atoms, a 4.0, skin 0.3, "
+             "dt 2 fs, 300 K",
+        nx=32, a=4.0, z=42, T=300.0, noise=0.01, dt=2.0, seed=13,
+        pot=("finnissinclair-ext", "c0 47.98066 c1 -34.09924 c2 5.832293 c3 0.017494 c4 0.020393 A 1.848648 "
+                                   "B 1.2 c 3.2572 d 4.1472"),
+        cm=("verletlist", "cutoff 4.1472 maxneighbors 200 each 5 skin 0.3"),
+        rc=4.1472, skin=0.3,
+        metal=True,
     ),
 }
 
@@ -110,7 +143,9 @@ def cpu_baseline(w, budget_s: float):
     conf = O.Conf(x, L, z=w["z"])
     kw = dict(zip(w["pot"][1].split()[0::2], (float(v) for v in w["pot"][1].split()[1::2])))
     pot = O.Pot(w["pot"][0], **kw)
-    cm = O.CellManager("verletlist", pot.cutoff, maxneighbors=400, each=5)
+    # a potential without a cutoff (harmonic) takes the workload's list cutoff
+    cut = pot.cutoff if np.isfinite(pot.cutoff) else w["rc"]
+    cm = O.CellManager("verletlist", cut, maxneighbors=400, each=5)
     t0 = time.perf_counter()
     cm.update(conf)  # the O(N^2) rebuild
     t_build = time.perf_counter() - t0
