@@ -91,11 +91,11 @@ WORKLOADS = {
         metal=False,
     ),
     "fsext_mo_131k": dict(
-        desc="finnissinclair-ext Mo (c 3.2572, d 4.1472, B 1.2) on fcc 32^3 = 131072 atoms, a 4.0, skin 0.3, "
-             "dt 2 fs, 300 K",
-        nx=32, a=4.0, z=42, T=300.0, noise=0.01, dt=2.0, seed=13,
+        desc="finnissinclair-ext Mo (the plugin's example: c 3.2572, d 4.1472, B 0) on fcc 32^3 = 131072 atoms, "
+             "a 3.625 (zero virial of the static lattice), skin 0.3, dt 2 fs, 300 K",
+        nx=32, a=3.625, z=42, T=300.0, noise=0.01, dt=2.0, seed=13,
         pot=("finnissinclair-ext", "c0 47.98066 c1 -34.09924 c2 5.832293 c3 0.017494 c4 0.020393 A 1.848648 "
-                                   "B 1.2 c 3.2572 d 4.1472"),
+                                   "B 0.0 c 3.2572 d 4.1472"),
         cm=("verletlist", "cutoff 4.1472 maxneighbors 200 each 5 skin 0.3"),
         rc=4.1472, skin=0.3,
         metal=True,

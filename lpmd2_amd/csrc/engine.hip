@@ -380,13 +380,18 @@ struct lpmdx_system {
   }
 
   // Brick edge (cells) for the staged force kernels, or 0 for global-index
-  // lists.  Staging pays for pair potentials, whose unstaged kernels are
-  // bound by the neighbour-position gathers; the EAM kernels do enough
-  // arithmetic per neighbour that the unstaged ones measured faster.  A
-  // brick's halo must fit the LDS of one workgroup with room for two
-  // workgroups per CU, and there must be enough bricks to fill 256 CUs;
-  // larger bricks stage fewer halo atoms per atom.  LPMDX_BRICK=0/2/3 forces
-  // the choice.
+  // lists.  Staging copies halo / brick atoms per atom (4.6 at B = 3, 8 at
+  // B = 2) to save the neighbour-position gathers of the unstaged kernels, so
+  // it pays for long lists and well-filled bricks.  Measured on one MI355X:
+  // LJ 4M (B 3, 527 atoms a brick, 77 neighbours: 16.6 per staged atom)
+  // 0.61 against 0.89 ms; Sutton-Chen 131k (B 2, 310 atoms, 151 neighbours:
+  // 18.9) 5 % faster; lennardjonesMod Ar 131k (B 2, 180 atoms, 12.4), the
+  // finnissinclair-ext Mo box (B 3, 201 atoms, 6.7) and harmonic springs
+  // (B 3, 81 atoms, 2.6) 30-110 % slower.  Hence: at least 256 atoms a brick
+  // and 15 listed neighbours per staged atom.  A brick's halo must also fit
+  // the LDS of one workgroup with room for two workgroups per CU, and there
+  // must be enough bricks to fill 256 CUs.  LPMDX_BRICK=0/2/3 forces the
+  // choice.
   int choose_brick() const {
     static const int forced = [] {
       const char* e = getenv("LPMDX_BRICK");
@@ -395,11 +400,11 @@ struct lpmdx_system {
     const long cells = (long)grid.nx * grid.ny * grid.nz;
     if (forced == 0 || cells == 0)
       return 0;
-    if (forced < 0)
-      for (auto& p : pots)
-        if (p->metal())
-          return 0;
     const double per_cell = (double)n / (double)cells;
+    // mean list length: atoms of the cells' density within the list radius
+    const double side = std::min(box.L[0] / grid.nx, std::min(box.L[1] / grid.ny, box.L[2] / grid.nz));
+    const double reach = std::isfinite(rl) ? std::min(1.0, rl / side) : 1.0;
+    const double nbar = per_cell * 4.18879 * reach * reach * reach;
     for (int B = kMaxBrick; B >= 2; --B) {
       if (forced > 0 && B != forced)
         continue;
@@ -408,7 +413,10 @@ struct lpmdx_system {
       const long nb = (long)((grid.nx + B - 1) / B) * ((grid.ny + B - 1) / B) * ((grid.nz + B - 1) / B);
       // the mean halo with 10 % for density fluctuations
       const double bytes = (double)stage_bytes((int)(1.1 * per_cell * halo_cells(B)), stage_parts());
-      if (forced > 0 || (bytes <= kStageLdsLimit / 2 && nb >= 2 * kNumCu))
+      const double per_brick = per_cell * B * B * B;
+      const double staged_per_atom = (double)halo_cells(B) / (B * B * B);
+      if (forced > 0 || (bytes <= kStageLdsLimit / 2 && nb >= 2 * kNumCu && per_brick >= 256.0 &&
+                         nbar >= 15.0 * staged_per_atom))
         return B;
     }
     return 0;
