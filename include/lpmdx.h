@@ -252,11 +252,13 @@ LPMDX_API int lpmdx_server_wait(lpmdx_server* s, long job, double timeout_s, int
 LPMDX_API int lpmdx_server_result(lpmdx_server* s, long job, double* x, double* v, double* a, double out[8]);
 /* Frees the stored result of a finished job. */
 LPMDX_API int lpmdx_server_release(lpmdx_server* s, long job);
-/* out[12] = jobs done, jobs failed, p50 and p99 completion latency (s), mean
- * latency (s), atom-steps done, seconds from the first submission to the last
- * completion, rounds, most jobs resident at once, jobs queued, jobs running,
- * graph steps replayed. */
-LPMDX_API int lpmdx_server_stats(lpmdx_server* s, double out[12]);
+/* out[LPMDX_NSERVER_STATS] = jobs done, jobs failed, p50 and p99 completion
+ * latency (s), mean latency (s), atom-steps done, seconds from the first
+ * submission to the last completion, rounds, most jobs resident at once, jobs
+ * queued, jobs running, graph steps replayed, then the worker's seconds spent
+ * admitting jobs, running steps and retiring jobs, and 0. */
+#define LPMDX_NSERVER_STATS 16
+LPMDX_API int lpmdx_server_stats(lpmdx_server* s, double out[LPMDX_NSERVER_STATS]);
 /* Lanes per atom of one potential kind's force launch (1, 2 or 4). */
 LPMDX_API int lpmdx_server_set_lanes(lpmdx_server* s, int kind, int lanes);
 

@@ -76,6 +76,9 @@ constexpr long kBytesPerAtom = 264; // per atom slot over all per-atom arenas
 
 struct lpmdx_batch {
   hipStream_t st = nullptr;
+  std::vector<hipStream_t> bst; // per-kind branches of a step
+  std::vector<hipEvent_t> ev_join;
+  hipEvent_t ev_fork = nullptr;
   long acap = 0, lcap = 0, ccap = 0;
   Arena atoms, lists, cells; // atoms counted in tiles
   // per-atom arenas
@@ -104,7 +107,10 @@ struct lpmdx_batch {
     long n = 0;
   };
   std::vector<std::unique_ptr<Group>> groups;
-  int lpa_of_kind[kNumKinds + 1] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  // lanes per atom of each kind's force launch until autotuned or set: 4
+  // (served Gupta / Buckingham / Finnis-Sinclair jobs measured 18 / 31 / 3 %
+  // faster than at one lane; the tile kernels keep few atoms per workgroup)
+  int lpa_of_kind[kNumKinds + 1] = {0, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
   // graph of one step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -126,6 +132,12 @@ struct lpmdx_batch {
     if (hred)
       (void)hipHostFree(hred);
     meta.clear();
+    for (hipStream_t b : bst)
+      (void)hipStreamDestroy(b);
+    for (hipEvent_t e : ev_join)
+      (void)hipEventDestroy(e);
+    if (ev_fork)
+      (void)hipEventDestroy(ev_fork);
     if (st)
       (void)hipStreamDestroy(st);
   }
@@ -491,24 +503,54 @@ struct lpmdx_batch {
     dirty = false;
   }
 
-  template <int KIND> void launch_group(Group& G) {
+  template <int KIND> void launch_group(Group& G, hipStream_t s) {
     BArgs B = args(G.tiles.p, G.n);
     switch (G.lpa) {
     case 1:
       if (G.metal)
-        k_b_rho<KIND, 1><<<G.n, kWave, 0, st>>>(B);
-      k_b_force<KIND, 1><<<G.n, kWave, 0, st>>>(B);
+        k_b_rho<KIND, 1><<<G.n, kWave, 0, s>>>(B);
+      k_b_force<KIND, 1><<<G.n, kWave, 0, s>>>(B);
       break;
     case 2:
       if (G.metal)
-        k_b_rho<KIND, 2><<<G.n, 2 * kWave, 0, st>>>(B);
-      k_b_force<KIND, 2><<<G.n, 2 * kWave, 0, st>>>(B);
+        k_b_rho<KIND, 2><<<G.n, 2 * kWave, 0, s>>>(B);
+      k_b_force<KIND, 2><<<G.n, 2 * kWave, 0, s>>>(B);
       break;
     default:
       if (G.metal)
-        k_b_rho<KIND, 4><<<G.n, 4 * kWave, 0, st>>>(B);
-      k_b_force<KIND, 4><<<G.n, 4 * kWave, 0, st>>>(B);
+        k_b_rho<KIND, 4><<<G.n, 4 * kWave, 0, s>>>(B);
+      k_b_force<KIND, 4><<<G.n, 4 * kWave, 0, s>>>(B);
       break;
+    }
+  }
+
+  // Streams and events of the per-kind branches of a step (created once,
+  // default priority).
+  void ensure_branches(size_t k) {
+    if (!ev_fork)
+      hipchk(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+    while (bst.size() < k) {
+      hipStream_t b;
+      hipEvent_t e;
+      hipchk(hipStreamCreateWithFlags(&b, hipStreamNonBlocking), "hipStreamCreate");
+      hipchk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      bst.push_back(b);
+      ev_join.push_back(e);
+    }
+  }
+
+  void launch_kind(Group& G, hipStream_t s) {
+    switch (G.kind) {
+    case kLJ: launch_group<kLJ>(G, s); break;
+    case kFastLJ: launch_group<kFastLJ>(G, s); break;
+    case kMorse: launch_group<kMorse>(G, s); break;
+    case kBuckingham: launch_group<kBuckingham>(G, s); break;
+    case kSuttonChen: launch_group<kSuttonChen>(G, s); break;
+    case kFinnisSinclair: launch_group<kFinnisSinclair>(G, s); break;
+    case kGupta: launch_group<kGupta>(G, s); break;
+    case kHarmonic: launch_group<kHarmonic>(G, s); break;
+    case kLJMod: launch_group<kLJMod>(G, s); break;
+    case kFinnisSinclairExt: launch_group<kFinnisSinclairExt>(G, s); break;
     }
   }
 
@@ -531,18 +573,17 @@ struct lpmdx_batch {
     if (n_all) {
       k_b_build<<<blocks_for(n_all * kTile), kBlock, 0, st>>>(B);
     }
-    for (auto& G : groups) {
-      switch (G->kind) {
-      case kLJ: launch_group<kLJ>(*G); break;
-      case kFastLJ: launch_group<kFastLJ>(*G); break;
-      case kMorse: launch_group<kMorse>(*G); break;
-      case kBuckingham: launch_group<kBuckingham>(*G); break;
-      case kSuttonChen: launch_group<kSuttonChen>(*G); break;
-      case kFinnisSinclair: launch_group<kFinnisSinclair>(*G); break;
-      case kGupta: launch_group<kGupta>(*G); break;
-      case kHarmonic: launch_group<kHarmonic>(*G); break;
-      case kLJMod: launch_group<kLJMod>(*G); break;
-      case kFinnisSinclairExt: launch_group<kFinnisSinclairExt>(*G); break;
+    if (groups.size() == 1) {
+      launch_kind(*groups[0], st);
+    } else if (groups.size() > 1) {
+      // kinds touch disjoint systems: one branch each, joined before the step ends
+      ensure_branches(groups.size());
+      hipchk(hipEventRecord(ev_fork, st), "hipEventRecord");
+      for (size_t g = 0; g < groups.size(); ++g) {
+        hipchk(hipStreamWaitEvent(bst[g], ev_fork, 0), "hipStreamWaitEvent");
+        launch_kind(*groups[g], bst[g]);
+        hipchk(hipEventRecord(ev_join[g], bst[g]), "hipEventRecord");
+        hipchk(hipStreamWaitEvent(st, ev_join[g], 0), "hipStreamWaitEvent");
       }
     }
     hipchk(hipGetLastError(), "kernel launch");

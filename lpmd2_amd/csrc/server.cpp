@@ -88,6 +88,8 @@ struct lpmdx_server {
   std::vector<double> latency;
   long n_done = 0, n_failed = 0, rounds = 0, max_resident = 0, graph_steps = 0;
   double atom_steps = 0.0;
+  // worker seconds: admitting jobs, running steps, retiring jobs
+  double t_admit = 0.0, t_run = 0.0, t_retire = 0.0;
   bool any = false;
   Clock::time_point t_first, t_last;
 
@@ -241,13 +243,20 @@ struct lpmdx_server {
           lanes_dirty = false;
         }
       }
+      auto t0 = std::chrono::steady_clock::now();
       admit(arr, seen);
+      auto t1 = std::chrono::steady_clock::now();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        t_admit += std::chrono::duration<double>(t1 - t0).count();
+      }
       if (resident.empty())
         continue; // everything queued failed admission
       if (!retire(left, flags)) { // zero-step jobs finish at once; also sizes the round
         fail_all(LPMDX_EDEVICE, lpmdx_last_error());
         return;
       }
+      auto t2 = std::chrono::steady_clock::now();
       if (resident.empty())
         continue;
       long round = max_round;
@@ -267,13 +276,17 @@ struct lpmdx_server {
         if (rc == LPMDX_ECAPACITY || stop || arrivals != seen)
           break; // overflowed job to retire, or new arrivals to admit
       }
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        ++rounds;
-      }
+      auto t3 = std::chrono::steady_clock::now();
       if (!retire(left, flags)) {
         fail_all(LPMDX_EDEVICE, lpmdx_last_error());
         return;
+      }
+      auto t4 = std::chrono::steady_clock::now();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        ++rounds;
+        t_retire += std::chrono::duration<double>((t2 - t1) + (t4 - t3)).count();
+        t_run += std::chrono::duration<double>(t3 - t2).count();
       }
     }
   }
@@ -428,7 +441,7 @@ int lpmdx_server_release(lpmdx_server* s, long job) {
   });
 }
 
-int lpmdx_server_stats(lpmdx_server* s, double out[12]) {
+int lpmdx_server_stats(lpmdx_server* s, double out[LPMDX_NSERVER_STATS]) {
   return guard([&] {
     if (!s || !out)
       fail(LPMDX_EINVAL, "server or out is NULL");
@@ -450,6 +463,10 @@ int lpmdx_server_stats(lpmdx_server* s, double out[12]) {
     out[9] = (double)s->queue.size();
     out[10] = (double)s->resident.size();
     out[11] = (double)s->graph_steps;
+    out[12] = s->t_admit;
+    out[13] = s->t_run;
+    out[14] = s->t_retire;
+    out[15] = 0.0;
   });
 }
 

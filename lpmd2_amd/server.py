@@ -31,7 +31,7 @@ def _bind():
 
 
 STAT_KEYS = ("done", "failed", "p50_s", "p99_s", "mean_s", "atom_steps", "span_s", "rounds", "max_resident",
-             "queued", "running", "graph_steps")
+             "queued", "running", "graph_steps", "admit_s", "run_s", "retire_s", "reserved")
 
 
 class Server:
@@ -112,7 +112,7 @@ class Server:
         self.sizes.pop(job, None)
 
     def stats(self):
-        out = np.zeros(12)
+        out = np.zeros(len(STAT_KEYS))
         _check(lib().lpmdx_server_stats(self._h, _p(out)))
         return {k: float(q) for k, q in zip(STAT_KEYS, out)}
 
