@@ -5,6 +5,8 @@
 
 #include <chrono>
 
+#include <set>
+
 #include "batch.hpp"
 #include "host_util.hpp"
 
@@ -313,6 +315,21 @@ struct lpmdx_batch {
       const double own = H.pots[q]->cutoff * H.pots[q]->cutoff;
       const double f = std::min(own, rl * rl);
       S.P[q] = device_view(*H.pots[q], f, f);
+    }
+    {
+      // species pairs of this system that more than one potential applies to
+      // (Potential::AppliesTo, potential.cc:23-27): none lets the force kernel
+      // evaluate one selected term per pair
+      std::set<int> zs(zz, zz + n);
+      int most = 0;
+      for (int za : zs)
+        for (int zb : zs) {
+          int c = 0;
+          for (int q = 0; q < npot; ++q)
+            c += (S.P[q].spc_sum < 0 || S.P[q].spc_sum == za + zb) ? 1 : 0;
+          most = std::max(most, c);
+        }
+      S.one_term = npot > 1 && most <= 1;
     }
     const double vol = box[0] * box[1] * box[2];
     const double est = (double)n / vol * 4.18879 * rl * rl * rl;

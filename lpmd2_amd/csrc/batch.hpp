@@ -53,6 +53,7 @@ struct BSys {
   double margin; // largest separation of a listed pair (0: always take the minimum image)
   double dt;
   int kind, nterm, metal;
+  int one_term; // every species pair present meets at most one of the nterm potentials
   int passes;   // digit passes its cell ids need (1 or 2)
   int ch0, nch; // its first sort chunk and number of chunks
   DevPot P[kMaxTerms];
@@ -433,6 +434,7 @@ __device__ __forceinline__ ForceArgs force_view(const BArgs& B, int s, double* x
   A.P = S.P[0];
   A.terms = S.nterm > 1 ? S.P : nullptr;
   A.nterm = S.nterm;
+  A.one_term = S.one_term;
   A.b = S.b;
   for (int q = 0; q < 3; ++q)
     A.ll[q] = S.ll[q];

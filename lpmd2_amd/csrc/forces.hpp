@@ -283,6 +283,7 @@ struct ForceArgs {
   double margin;
   DevPot P;
   const DevPot* terms; // several potentials of one kind on one list (species pairs), or null
+  int one_term;        // with terms: each species pair meets at most one of them
   int nterm;
   Box b;
   double ll[3]; // |cell vector q| as FittedInside reads it
@@ -604,6 +605,25 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
     const bool in = valid && r2 < A.P.rc2 && (A.P.spc_sum < 0 || si + sj == A.P.spc_sum);
     float f, en;
     pair_eval<KIND>(A.P, r2, valid ? (float)r2 : 1.0f, f, en);
+    fs = in ? f : 0.0f;
+    e = in ? en : 0.0f;
+  } else if (A.one_term) {
+    // at most one term per pair: its parameters selected per lane, one evaluation
+    const int ss = si + sj;
+    DevPot Q = A.terms[0];
+    bool any = Q.spc_sum < 0 || ss == Q.spc_sum;
+    for (int q = 1; q < A.nterm; ++q) {
+      const DevPot& T = A.terms[q];
+      const bool m = T.spc_sum < 0 || ss == T.spc_sum;
+#pragma unroll
+      for (int k = 0; k < 12; ++k)
+        Q.f[k] = m ? T.f[k] : Q.f[k];
+      Q.rc2 = m ? T.rc2 : Q.rc2;
+      any = any || m;
+    }
+    const bool in = valid && any && r2 < Q.rc2;
+    float f, en;
+    pair_eval<KIND>(Q, r2, valid ? (float)r2 : 1.0f, f, en);
     fs = in ? f : 0.0f;
     e = in ? en : 0.0f;
   } else {
