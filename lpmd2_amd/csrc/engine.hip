@@ -51,6 +51,24 @@ __global__ __launch_bounds__(kBlock) void k_disp(ForceArgs A) {
     atomic_max_sparse(A.maxd2, __float_as_uint(m));
 }
 
+// Start of a velocity-Verlet chunk in one launch: the per-step maxima md[0, m)
+// cleared and, with a snapshot, x, v, a copied for a roll-back (3 n doubles
+// each; thread t takes double t of each array).
+__global__ __launch_bounds__(kBlock) void k_chunk_begin(const double* __restrict__ x, const double* __restrict__ v,
+                                                        const double* __restrict__ a, double* __restrict__ xs,
+                                                        double* __restrict__ vs, double* __restrict__ as, long n3,
+                                                        unsigned int* __restrict__ md, long m, int snapshot) {
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t < m)
+    md[t] = 0u;
+  if (snapshot && t < n3) {
+    const double px = x[t], pv = v[t], pa = a[t];
+    xs[t] = px;
+    vs[t] = pv;
+    as[t] = pa;
+  }
+}
+
 __global__ void k_fill_double(double* p, long m, double v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m)
@@ -1155,11 +1173,11 @@ struct lpmdx_system {
         xs.alloc(3 * n);
         vs.alloc(3 * n);
         as.alloc(3 * n);
-        hipchk(hipMemcpyAsync(xs.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(vs.p, v.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(as.p, a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
       }
-      hipchk(hipMemsetAsync(md, 0, m * sizeof(unsigned), st), "memset");
+      // snapshot and cleared per-step maxima in one launch (m <= kMaxChunk)
+      k_chunk_begin<<<blocks_for(std::max<long>(snapshot ? 3 * n : 0, m)), kBlock, 0, st>>>(
+          x.p, v.p, a.p, xs.p, vs.p, as.p, 3 * n, md, m, snapshot ? 1 : 0);
+      launch_check();
       for (long s = 0; s < m; ++s) {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
         forces(kStep, obs, dt, md + s);
