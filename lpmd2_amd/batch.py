@@ -9,7 +9,8 @@ import numpy as np
 from . import _check, _p, lib
 
 KINDS = {"lennardjones": 1, "fastlj": 2, "morse": 3, "buckingham": 4, "suttonchen": 5, "finnissinclair": 6,
-         "gupta": 7, "harmonic": 8, "lennardjonesMod": 9, "finnissinclair-ext": 10}
+         "gupta": 7, "harmonic": 8, "lennardjonesMod": 9, "finnissinclair-ext": 10,
+         "tabulatedpair": 2}  # tabulatedpair runs the fastlj table kernels
 NKINDS = 10  # LPMDX_NKINDS
 
 _bound = False

@@ -8,7 +8,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
 SOURCES = [os.path.join(_HERE, "csrc", f) for f in ("engine.hip", "batch.hip", "server.cpp")]
 HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp",
-                                                     "batch.hpp", "host_util.hpp")] + [
+                                                     "batch.hpp", "host_util.hpp", "table.hpp")] + [
     os.path.join(ROOT, "include", "lpmdx.h")
 ]
 OUT = os.path.join(_HERE, "liblpmdx.so")

@@ -19,6 +19,7 @@
 #include "cells.hpp"
 #include "common.hpp"
 #include "forces.hpp"
+#include "table.hpp"
 
 namespace lpmdx {
 
@@ -203,6 +204,29 @@ inline std::unique_ptr<PotParams> make_potential(const std::string& name, const 
     for (int q = 0; q < 5; ++q)
       P->p[q] = k.d(keys[q]);
     P->cutoff = P->p[5] = k.d("cutoff");
+  } else if (name == "tabulatedpair") { // plugins/tabulatedpair.cc:12-96
+    // the fastlj kernels' table lookup, k = floor(bins r / cutoff), on the file's table
+    Keywords k(args, {{"file", ""}});
+    const std::string file = k.s("file");
+    if (file.empty())
+      fail(LPMDX_EINVAL, "tabulatedpair: no table file given (keyword file)");
+    const PairTable T = read_pair_table(file);
+    if (!T.ok)
+      fail(LPMDX_EINVAL, "tabulatedpair: cannot open the table file " + file);
+    if (T.e.empty() || !(T.cutoff > 0.0))
+      fail(LPMDX_EINVAL, "tabulatedpair: no table rows (r E ff) with a positive r in " + file);
+    P->kind = kFastLJ;
+    P->bins = (long)T.e.size();
+    P->cutoff = P->p[2] = T.cutoff;
+    std::vector<float> e(P->bins), f(P->bins);
+    for (long i = 0; i < P->bins; ++i) {
+      e[i] = (float)T.e[i];
+      f[i] = (float)T.f[i];
+    }
+    P->tab_e.alloc(P->bins);
+    P->tab_f.alloc(P->bins);
+    hipchk(hipMemcpy(P->tab_e.p, e.data(), P->bins * sizeof(float), hipMemcpyHostToDevice), "table");
+    hipchk(hipMemcpy(P->tab_f.p, f.data(), P->bins * sizeof(float), hipMemcpyHostToDevice), "table");
   } else if (name == "harmonic") { // plugins/harmonic.cc:11-19 (no cutoff keyword)
     Keywords k(args, {});
     P->kind = kHarmonic;

@@ -1,6 +1,8 @@
 // Implementation of the LPMD plugin classes over the engine's C ABI.
 #include "lpmdx_lpmd.hpp"
 
+#include "../csrc/table.hpp"
+
 #include <lpmd/basicparticleset.h>
 #include <lpmd/combinedpotential.h>
 #include <lpmd/error.h>
@@ -40,6 +42,7 @@ const std::map<std::string, std::vector<Keyword>>& Keywords() {
         {"d", nullptr}}},
       {"gupta", {{"A", nullptr}, {"r0", nullptr}, {"p", nullptr}, {"B", nullptr}, {"q", nullptr}, {"cutoff", nullptr}}},
       {"harmonic", {{"k", nullptr}, {"a", nullptr}}},
+      {"tabulatedpair", {{"file", ""}}},
       {"lennardjonesMod",
        {{"sigma", nullptr}, {"epsilon", nullptr}, {"m", "12"}, {"n", "6"}, {"cohesive", "1"}, {"cutoff", nullptr}}},
       {"finnissinclair-ext",
@@ -257,7 +260,14 @@ DevicePotential::DevicePotential(const std::string& name, std::string args)
     : lpmd::Plugin(name, kVersion), kind_(name), args_(args) {
   Define(*this, name);
   ProcessArguments(args);
-  if (name == "finnissinclair" || name == "finnissinclair-ext") {
+  if (name == "tabulatedpair") {
+    // cutoff = the table's largest r (tabulatedpair.cc:34-79); the reference
+    // ends the program when the file does not open, this plugin throws
+    const lpmdx::PairTable T = lpmdx::read_pair_table(std::string((*this)["file"]));
+    if (!T.ok)
+      throw lpmd::RuntimeError("Impossible open the potential data file.");
+    SetCutoff(T.cutoff);
+  } else if (name == "finnissinclair" || name == "finnissinclair-ext") {
     // cutoff = max(c, d) (finnissinclair.cc:10-31, finnissinclair-ext.cc:11-36)
     const double c = double((*this)["c"]), d = double((*this)["d"]);
     SetCutoff(d >= c ? d : c);
@@ -416,7 +426,7 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
 lpmd::Plugin* Create(const std::string& name, const std::string& args) {
   static const char* pots[] = {"lennardjones", "fastlj",   "morse",           "buckingham",
                                "suttonchen",   "finnissinclair", "gupta", "harmonic",
-                               "lennardjonesMod", "finnissinclair-ext"};
+                               "lennardjonesMod", "finnissinclair-ext", "tabulatedpair"};
   for (const char* p : pots)
     if (name == p)
       return new DevicePotential(name, args);

@@ -10,7 +10,7 @@
 //   DevicePotential       lpmd::Potential (liblpmd/lpmd/potential.h:11-36) as
 //                         lennardjones fastlj morse buckingham suttonchen
 //                         finnissinclair gupta harmonic lennardjonesMod
-//                         finnissinclair-ext (plugins/<name>.cc)
+//                         finnissinclair-ext tabulatedpair (plugins/<name>.cc)
 //   DeviceVelocityVerlet  lpmd::Integrator (liblpmd/lpmd/integrator.h:15-40) as
 //                         "velocityverlet" (plugins/velocityverlet.cc:13-20)
 // They take the same plugin names and keyword strings, so a directory holding

@@ -14,13 +14,14 @@ CORE     := oracle/_ref
 CXX      ?= g++
 CXXFLAGS := -O2 -std=c++17 -fPIC -w -I$(REF)/liblpmd -Iinclude
 NAMES    := lennardjones fastlj morse buckingham suttonchen finnissinclair gupta \
-            harmonic lennardjonesMod finnissinclair-ext \
+            harmonic lennardjonesMod finnissinclair-ext tabulatedpair \
             linkedcell verletlist minimumimage lcbinary velocityverlet
 RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..' -Wl,-rpath,'$$ORIGIN/../../../oracle/_ref'
 
 all: $(patsubst %,$(OUT)/%.so,$(NAMES))
 
 $(OUT)/liblpmdx_lpmd.so: lpmd2_amd/plugins/lpmdx_lpmd.cc lpmd2_amd/plugins/lpmdx_lpmd.hpp include/lpmdx.h \
+                         lpmd2_amd/csrc/table.hpp \
                          lpmd2_amd/liblpmdx.so $(CORE)/liblpmdref_core.so
 	@mkdir -p $(OUT)
 	$(CXX) $(CXXFLAGS) -shared -o $@ $< -L$(CORE) -llpmdref_core -Llpmd2_amd -l:liblpmdx.so $(RPATH) \

@@ -712,6 +712,24 @@ int orc_pot_init(orc_pot* p, int kind, const double* a, int s1, int s2) {
   return 0;
 }
 
+/* tabulatedpair.cc:12-79: bins = rows, cutoff = the last increasing r; the
+ * lookup is fastlj's (tabulatedpair.cc:81-96) */
+int orc_pot_init_table(orc_pot* p, const double* e, const double* ff, long bins, double cutoff, int s1, int s2) {
+  memset(p, 0, sizeof *p);
+  p->kind = ORC_TABULATED;
+  p->spc_sum = s1 + s2;
+  if (bins <= 0 || !(cutoff > 0.0))
+    return fail("tabulatedpair: empty table");
+  p->cutoff = cutoff;
+  p->p[2] = cutoff; /* fastlj_bin reads the cutoff here */
+  p->bins = bins;
+  p->etable = malloc(bins * sizeof(double));
+  p->fftable = malloc(bins * sizeof(double));
+  memcpy(p->etable, e, bins * sizeof(double));
+  memcpy(p->fftable, ff, bins * sizeof(double));
+  return 0;
+}
+
 void orc_pot_free(orc_pot* p) {
   free(p->etable);
   free(p->fftable);
@@ -737,6 +755,7 @@ double orc_pair_energy(const orc_pot* p, double r) {
     return 4.0e0 * a[1] * (r12 - r6);
   }
   case ORC_FASTLJ: /* fastlj.cc:81-85 */
+  case ORC_TABULATED: /* tabulatedpair.cc:81-85 */
     return p->etable[fastlj_bin(p, r)];
   case ORC_MORSE: /* morse.cc:53-55 */
     return a[0] * (1.0 - exp(-a[1] * (r - a[2]))) * (1.0 - exp(-a[1] * (r - a[2])));
@@ -785,7 +804,8 @@ void orc_pair_force(const orc_pot* p, const double rij[3], double out[3]) {
     ff = -48.0e0 * (a[1] / rr2) * (r12 - 0.50e0 * r6);
     break;
   }
-  case ORC_FASTLJ: { /* fastlj.cc:87-95 */
+  case ORC_FASTLJ: /* fastlj.cc:87-95 */
+  case ORC_TABULATED: { /* tabulatedpair.cc:87-96 */
     double rr2 = sqmod(rij);
     double rr = sqrt(rr2);
     ff = p->fftable[fastlj_bin(p, rr)];

@@ -74,7 +74,8 @@ enum {
   ORC_GUPTA = 7,
   ORC_HARMONIC = 8,
   ORC_LJMOD = 9,
-  ORC_FINNISSINCLAIREXT = 10
+  ORC_FINNISSINCLAIREXT = 10,
+  ORC_TABULATED = 11
 };
 
 typedef struct {
@@ -96,6 +97,8 @@ typedef struct {
  *  LJMOD: sigma epsilon m n cohesive cutoff
  *  FINNISSINCLAIREXT: c0 c1 c2 c3 c4 A B c d */
 int orc_pot_init(orc_pot* p, int kind, const double* params, int s1, int s2);
+/* tabulatedpair: the table's rows (energies, force factors) and cutoff */
+int orc_pot_init_table(orc_pot* p, const double* e, const double* ff, long bins, double cutoff, int s1, int s2);
 void orc_pot_free(orc_pot* p);
 int orc_pot_is_metal(const orc_pot* p);
 double orc_pair_energy(const orc_pot* p, double r);

@@ -14,7 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblpmd_oracle.so")
 
-LJ, FASTLJ, MORSE, BUCKINGHAM, SUTTONCHEN, FINNISSINCLAIR, GUPTA, HARMONIC, LJMOD, FINNISSINCLAIREXT = range(1, 11)
+LJ, FASTLJ, MORSE, BUCKINGHAM, SUTTONCHEN, FINNISSINCLAIR, GUPTA, HARMONIC, LJMOD, FINNISSINCLAIREXT, TABULATED = \
+    range(1, 12)
 METAL = (SUTTONCHEN, FINNISSINCLAIR, GUPTA, FINNISSINCLAIREXT)
 KIND = {
     "lennardjones": LJ,
@@ -27,6 +28,7 @@ KIND = {
     "harmonic": HARMONIC,
     "lennardjonesMod": LJMOD,
     "finnissinclair-ext": FINNISSINCLAIREXT,
+    "tabulatedpair": TABULATED,
 }
 # keyword order of each plugin (plugins/<name>.cc constructors)
 KEYWORDS = {
@@ -114,6 +116,7 @@ def lib():
         L.orc_cm_all_lists.restype = l
         L.orc_cm_all_lists.argtypes = [vp, vp, i, d, vp, l, vp, vp, vp]
         L.orc_pot_init.argtypes = [vp, i, vp, i, i]
+        L.orc_pot_init_table.argtypes = [vp, vp, vp, l, d, i, i]
         L.orc_pot_free.argtypes = [vp]
         L.orc_pair_energy.restype = d
         L.orc_pair_energy.argtypes = [vp, d]
@@ -234,8 +237,42 @@ class CellManager:
             cap = -tot
 
 
+def read_pair_table(path: str):
+    """The table of plugins/tabulatedpair.cc:34-79: rows of exactly three
+    whitespace-separated fields (r, E, ff) not starting with '#', in file
+    order; the cutoff is the last r larger than every r before it.  Fields are
+    converted like atof (a leading number, else 0)."""
+    import re
+
+    def atof(t):
+        m = re.match(r"[+-]?(\d+\.?\d*([eE][+-]?\d+)?|\.\d+([eE][+-]?\d+)?)", t)
+        return float(m.group(0)) if m else 0.0
+
+    e, ff = [], []
+    cutoff, previo = 0.0, 0.0
+    with open(path) as f:
+        for line in f:
+            w = line.replace("\t", " ").split()
+            if len(w) != 3 or w[0].startswith("#"):
+                continue
+            r = atof(w[0])
+            if r > previo:
+                cutoff = previo = r
+            e.append(atof(w[1]))
+            ff.append(atof(w[2]))
+    return np.array(e), np.array(ff), cutoff
+
+
 class Pot:
     def __init__(self, name: str, s1: int = -1, s2: int = -1, **kw):
+        self.name = name
+        self.s = _Pot()
+        if name == "tabulatedpair":
+            e, ff, cutoff = read_pair_table(kw["file"])
+            e = np.ascontiguousarray(e, dtype=np.float64)
+            ff = np.ascontiguousarray(ff, dtype=np.float64)
+            _check(lib().orc_pot_init_table(C.byref(self.s), _p(e), _p(ff), len(e), cutoff, s1, s2))
+            return
         vals = dict(DEFAULTS.get(name, {}))
         vals.update(kw)
         params = np.zeros(10)

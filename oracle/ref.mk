@@ -24,7 +24,7 @@ CORE_SRCS := \
   liblpmd/simulation/integrator.cc liblpmd/simulation/twostepintegrator.cc \
   liblpmd/simulation/simulation.cc liblpmd/simulation/systemmodifier.cc
 
-PLUGINS := linkedcell verletlist minimumimage lcbinary lennardjones fastlj morse buckingham \
+PLUGINS := linkedcell verletlist minimumimage lcbinary lennardjones fastlj morse buckingham tabulatedpair \
            suttonchen finnissinclair gupta harmonic lennardjonesMod finnissinclair-ext \
            crystal3d velocityverlet
 

@@ -8,7 +8,7 @@ first (a partial build of the reference's plugins, see oracle/ref.mk), then
 Every fixture holds inputs and the reference's outputs for them: neighbour
 lists from plugins/linkedcell.cc, plugins/verletlist.cc, plugins/minimumimage.cc
 and plugins/lcbinary.cc, per-pair values of
-the ten potential plugins, crystal3d lattices and velocityverlet
+the eleven potential plugins (a pair table file for tabulatedpair), crystal3d lattices and velocityverlet
 trajectories.  Large lists are stored as SHA-256 digests of their exact bytes.
 """
 from __future__ import annotations
@@ -48,6 +48,21 @@ POTS = {
                                A=1.848648, B=1.2, c=3.2572, d=4.1472),
 }
 METALS = ("suttonchen", "finnissinclair", "gupta", "finnissinclair-ext")
+TABLE = "table_morse_cu.dat"
+
+
+def write_table(path, bins=400, cutoff=7.0):
+    """A Cu Morse pair table in the tabulatedpair format (r E ff per row, r up
+    to the cutoff), with a comment line, a blank line and a row of another width
+    that the reader skips."""
+    D, a, re_ = 0.3429, 1.3588, 2.866
+    with open(path, "w") as f:
+        f.write("# r  E  ff   (Morse Cu, depth 0.3429 a 1.3588 re 2.866)\n\n")
+        f.write("0.0 0.0\n")
+        for k in range(bins):
+            r = cutoff * (k + 1) / bins
+            ex = np.exp(-a * (r - re_))
+            f.write(f"{r:.10g}\t{D * (1 - ex) ** 2:.12g} {2 * a * D / r * (1 - ex) * ex:.12g}\n")
 
 
 def args_of(kw):
@@ -131,8 +146,14 @@ def main():
 
     # ---- potentials ---------------------------------------------------------
     rng = np.random.default_rng(21)
-    for name, kw in POTS.items():
-        rp = R.RefPot(name, args_of(kw))
+    write_table(os.path.join(HERE, TABLE))
+    pots = dict(POTS)
+    pots["tabulatedpair"] = {"file": TABLE}  # resolved in tests/golden/
+    for name, kw in pots.items():
+        if name == "tabulatedpair":
+            rp = R.RefPot(name, f"file {os.path.join(HERE, TABLE)}")
+        else:
+            rp = R.RefPot(name, args_of(kw))
         rmax = min(rp.cutoff, 9.0)
         rij = rng.normal(0, 1, (256, 3))
         rij *= (rng.uniform(1.9, rmax, 256) / np.linalg.norm(rij, axis=1))[:, None]

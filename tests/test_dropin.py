@@ -3,6 +3,8 @@ loads through the reference's create(args) entry point (liblpmd/lpmd/plugin.h:42
 parses its keywords like the reference plugin of the same name, reports the
 same cutoff, and fails loudly when no GPU is visible (no CPU fallback).
 Skipped where the plugins are not built (they need the reference headers)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +26,8 @@ POTS = [
     ("lennardjonesMod", "sigma 3.41 epsilon 0.0103 cutoff 8.5"),
     ("finnissinclair-ext", "c0 47.98066 c1 -34.09924 c2 5.832293 c3 0.017494 c4 0.020393 A 1.848648 B 1.2 "
                            "c 3.2572 d 4.1472"),
+    ("tabulatedpair", "file " + os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                             "table_morse_cu.dat")),
 ]
 
 

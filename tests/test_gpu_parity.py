@@ -77,6 +77,8 @@ def test_lists_match_golden(case):
 FSX = dict(c0=47.98066, c1=-34.09924, c2=5.832293, c3=0.017494, c4=0.020393, A=1.848648, B=1.2, c=3.2572,
            d=4.1472)
 FSX_ARGS = " ".join(f"{k} {v}" for k, v in FSX.items())
+# a Cu Morse pair table (r E ff rows) for tabulatedpair, a fixture in tests/golden/
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "table_morse_cu.dat")
 POTS = [
     ("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5", dict(sigma=3.41, epsilon=0.0103, cutoff=8.5), 18, 5.26, 0.1),
     ("fastlj", "sigma 3.41 epsilon 0.0103 cutoff 8.5", dict(sigma=3.41, epsilon=0.0103, cutoff=8.5, bins=500), 18, 5.26, 0.1),
@@ -92,6 +94,7 @@ POTS = [
     ("lennardjonesMod", "sigma 3.41 epsilon 0.0103 m 10 n 5 cohesive 0.8 cutoff 8.5",
      dict(sigma=3.41, epsilon=0.0103, m=10, n=5, cohesive=0.8, cutoff=8.5), 18, 5.26, 0.1),
     ("finnissinclair-ext", FSX_ARGS, FSX, 42, 4.0, 0.05),
+    ("tabulatedpair", f"file {TABLE}", dict(file=TABLE), 29, 3.61, 0.05),
 ]
 
 

@@ -128,7 +128,10 @@ def test_shell_cutoff_counts():
 @pytest.mark.parametrize("name", sorted(META["pots"]))
 def test_potentials_match_reference(name):
     d = np.load(os.path.join(GOLDEN, f"pot_{name}.npz"))
-    p = O.Pot(name, **META["pots"][name]["params"])
+    params = dict(META["pots"][name]["params"])
+    if name == "tabulatedpair":
+        params["file"] = os.path.join(GOLDEN, params["file"])  # a fixture beside the vectors
+    p = O.Pot(name, **params)
     assert p.cutoff == META["pots"][name]["cutoff"]
     rij = d["rij"]
     r = np.sqrt(rij[:, 0] * rij[:, 0] + rij[:, 1] * rij[:, 1] + rij[:, 2] * rij[:, 2])
@@ -300,6 +303,7 @@ POT_CASES = [
     ("lennardjonesMod", dict(LJ, m=10, n=5, cohesive=0.8), 18, 5.26, 8.5),
     ("finnissinclair-ext", dict(c0=47.98066, c1=-34.09924, c2=5.832293, c3=0.017494, c4=0.020393, A=1.848648, B=1.2,
                                 c=3.2572, d=4.1472), 42, 4.0, 4.2),
+    ("tabulatedpair", dict(file=os.path.join(GOLDEN, "table_morse_cu.dat")), 29, 3.61, 7.0),
 ]
 
 
@@ -318,8 +322,8 @@ def test_forces_are_energy_gradients(name, params, z, a, rc):
     F = conf.a / O.FORCEFACTOR  # unit masses
     # Newton's third law
     assert np.abs(F.sum(axis=0)).max() < 1e-10 * max(1.0, np.abs(F).max())
-    if name == "fastlj":
-        return  # piecewise-constant table: no derivative to compare
+    if name in ("fastlj", "tabulatedpair"):
+        return  # piecewise-constant tables: no derivative to compare
     h = 1e-6
     for i in (0, 17, 50):
         for q in range(3):
