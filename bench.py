@@ -71,6 +71,16 @@ WORKLOADS = {
         rc=2.5 * 3.41, skin=0.3 * 3.41,
         metal=False,
     ),
+    # the same LJ liquid under the leapfrog integrator (one force call per step)
+    "lj_4m_leapfrog": dict(
+        desc="LJ liquid fcc 100^3 = 4,000,000 atoms, rho* 0.8442, T* 0.72, rc 2.5 sigma, skin 0.3 sigma, dt* 0.005, "
+             "leapfrog",
+        nx=100, a=1.679596 * 3.41, z=18, T=0.72 * 0.0103 / 8.6173422e-05, noise=0.0, dt=0.005 * 2161.98, seed=7,
+        pot=("lennardjones", f"sigma 3.41 epsilon 0.0103 cutoff {2.5 * 3.41}"),
+        cm=("verletlist", f"cutoff {2.5 * 3.41} maxneighbors 400 each 5 skin {0.3 * 3.41}"),
+        rc=2.5 * 3.41, skin=0.3 * 3.41,
+        metal=False, integrator="leapfrog",
+    ),
     # the widened potentials, one 131072-atom box each
     "harmonic_131k": dict(
         desc="harmonic springs k 3.4 a 3.0 on fcc 32^3 = 131072 atoms, a 4.243 (nearest neighbours at the spring "
@@ -158,7 +168,10 @@ def cpu_baseline(w, budget_s: float):
             break
     t_force = (time.perf_counter() - t0) / ncalls
     scale = n / ns
-    t_step = 2 * t_force * scale + 0.4 * t_build * scale * scale
+    # velocityverlet (TwoStepIntegrator) calls UpdateForces twice a step,
+    # leapfrog (OneStepIntegrator) once; the list is rebuilt every 5 calls
+    calls = 1 if w.get("integrator") == "leapfrog" else 2
+    t_step = calls * t_force * scale + calls / 5 * t_build * scale * scale
     return {
         "value": n / t_step,
         "unit": "atom-steps/s",
@@ -166,7 +179,7 @@ def cpu_baseline(w, budget_s: float):
         "kind": "port",
         "sample": f"oracle, verletlist each 5, on the {ns}-atom box of the same workload: 1 O(N^2) rebuild "
                   f"({t_build:.2f} s, scaled x{scale * scale:.0f}) + {ncalls} force calls ({t_force:.3f} s each, "
-                  f"scaled x{scale:.0f}); step = 2 force calls + 0.4 rebuild",
+                  f"scaled x{scale:.0f}); step = {calls} force calls + {calls / 5:.1f} rebuild",
         "cpu": cpu_model(),
     }
 
@@ -396,15 +409,20 @@ def main():
     if args.lanes:
         s.set_lanes(args.lanes)
     n = s.n
+    if w.get("integrator") == "leapfrog":
+        s.leapfrog_init(w["dt"])
+        advance = s.leapfrog
+    else:
+        advance = s.velocityverlet
     t0 = time.perf_counter()
-    s.velocityverlet(w["dt"], args.warmup)
+    advance(w["dt"], args.warmup)
     log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.3f} s; stats {s.stats()}")
 
     s.profile(True)
     st0 = s.stats()
     t0 = time.perf_counter()
-    s.velocityverlet(w["dt"], args.steps)
-    dt_wall = time.perf_counter() - t0  # velocityverlet returns after the stream is synchronised
+    advance(w["dt"], args.steps)
+    dt_wall = time.perf_counter() - t0  # the integrator returns after the stream is synchronised
     st1 = s.stats()
     prof = s.profile_read()
     s.profile(False)
@@ -468,7 +486,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64+f32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "atoms": n, "path": "plain single-system", "desc": w["desc"]},
+        "config": {"workload": args.workload, "atoms": n, "path": "plain single-system", "desc": w["desc"],
+                   "integrator": w.get("integrator", "velocityverlet")},
         "roofline": roofline,
         "step_hbm_fraction": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
         "valu": valu,

@@ -148,6 +148,19 @@ LPMDX_API int lpmdx_potential_forces(lpmdx_system* sys, int k, double* acc, doub
 LPMDX_API int lpmdx_velocityverlet(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
                          double* ekin_t);
 
+/* The leapfrog integrator (plugins/leapfrog.cc:58-77).  lpmdx_leapfrog_init is
+ * Leapfrog::Initialize: the old configuration x - v dt, v - a dt
+ * (Integrator::GoBack, integrator.cc:31-42), forces there, and the half-step
+ * velocities v - a dt + (a_old / 2) dt; it leaves x, v and a as they were.
+ * lpmdx_leapfrog then runs `steps` OneStepIntegrator::Advance calls
+ * (onestepintegrator.cc:13-46) with that dt; after each, x is the next
+ * positions and v the velocities of the step's force evaluation, as in the
+ * reference.  Energies every obs_every steps as in lpmdx_velocityverlet.
+ * Without a prior lpmdx_leapfrog_init: LPMDX_ESTATE; another dt: LPMDX_EINVAL. */
+LPMDX_API int lpmdx_leapfrog_init(lpmdx_system* sys, double dt);
+LPMDX_API int lpmdx_leapfrog(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
+                             double* ekin_t);
+
 /* Device-side SetSimulationTags (simulationbuilder.cc:84-124, properties.h:22-81):
  * out[LPMDX_NOBS], see LPMDX_OBS_*.  The potential terms are those of the
  * last force evaluation. */

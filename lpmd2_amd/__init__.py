@@ -60,6 +60,8 @@ def lib():
         L.lpmdx_potential_clear.argtypes = [vp]
         L.lpmdx_update_forces.argtypes = [vp, vp, vp, vp]
         L.lpmdx_velocityverlet.argtypes = [vp, d, l, l, vp, vp]
+        L.lpmdx_leapfrog_init.argtypes = [vp, d]
+        L.lpmdx_leapfrog.argtypes = [vp, d, l, l, vp, vp]
         L.lpmdx_observables.argtypes = [vp, vp]
         L.lpmdx_system_stats.argtypes = [vp, vp]
         L.lpmdx_system_set_lanes.argtypes = [vp, i]
@@ -154,6 +156,17 @@ class System:
         ep = np.zeros(max(k, 1))
         ek = np.zeros(max(k, 1))
         _check(lib().lpmdx_velocityverlet(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
+        return ep[:k], ek[:k]
+
+    def leapfrog_init(self, dt: float):
+        """Leapfrog::Initialize: half-step velocities from the old configuration."""
+        _check(lib().lpmdx_leapfrog_init(self._h, dt))
+
+    def leapfrog(self, dt: float, steps: int, obs_every: int = 0):
+        k = steps // obs_every if obs_every else 0
+        ep = np.zeros(max(k, 1))
+        ek = np.zeros(max(k, 1))
+        _check(lib().lpmdx_leapfrog(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
         return ep[:k], ek[:k]
 
     def neighbor_lists(self, full: bool, rcut: float):

@@ -235,6 +235,10 @@ struct PermuteArgs {
   double* ffm2;
   double* mass2;
   int* perm_new;
+  // up to three more per-atom 3-vectors moved along (null: unused), e.g. the
+  // leapfrog half-step velocities
+  const double* ex[3];
+  double* ex2[3];
   long n;
 };
 
@@ -254,6 +258,12 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   }
   const int z = A.z[s], o = A.perm_old[s];
   const double ffm = A.ffm[s], mass = A.mass[s];
+  double h[3][3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      h[e][q] = A.ex[e] ? A.ex[e][3 * s + q] : 0.0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     A.x2[3 * k + q] = x[q];
@@ -264,6 +274,13 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   A.ffm2[k] = ffm;
   A.mass2[k] = mass;
   A.perm_new[k] = o;
+#pragma unroll
+  for (int e = 0; e < 3; ++e)
+    if (A.ex[e]) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        A.ex2[e][3 * k + q] = h[e][q];
+    }
 }
 
 constexpr int kChunk = 8;

@@ -69,6 +69,37 @@ __global__ __launch_bounds__(kBlock) void k_chunk_begin(const double* __restrict
   }
 }
 
+// Integrator::GoBack (liblpmd/simulation/integrator.cc:31-42) for the leapfrog
+// set-up: x and a saved, vh = v - a dt, x = FittedInside(x - v dt).
+__global__ __launch_bounds__(kBlock) void k_lf_goback(double* __restrict__ x, const double* __restrict__ v,
+                                                      const double* __restrict__ a, double* __restrict__ keep_x,
+                                                      double* __restrict__ keep_a, double* __restrict__ vh, long n,
+                                                      double dt, double l0, double l1, double l2) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const double ll[3] = {l0, l1, l2};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double xq = x[3 * i + q], vq = v[3 * i + q], aq = a[3 * i + q];
+    keep_x[3 * i + q] = xq;
+    keep_a[3 * i + q] = aq;
+    vh[3 * i + q] = vq - aq * dt;
+    x[3 * i + q] = fitted_inside(xq - vq * dt, ll[q]);
+  }
+}
+
+// leapfrog.cc:63-66: vh += (a_old * 0.5) * dt
+__global__ __launch_bounds__(kBlock) void k_lf_halfkick(double* __restrict__ vh, const double* __restrict__ a,
+                                                        long n, double dt) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    vh[3 * i + q] += (a[3 * i + q] * 0.5) * dt;
+}
+
 __global__ void k_fill_double(double* p, long m, double v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m)
@@ -165,6 +196,14 @@ struct lpmdx_system {
   bool list_built = false;
   bool forces_ok = false; // a = F(x) for the user-visible positions
   bool drifted = false;   // user positions are in xn; x = drift of them
+  // leapfrog (plugins/leapfrog.cc): half-step velocities v(t - dt/2), in
+  // internal order, once lpmdx_leapfrog_init has set them
+  DBuf<double> vh, vh2, vhs;
+  bool lf_ready = false;
+  double lf_dt = 0.0;
+  // more arrays that reorders carry along (with their scratch twins)
+  std::vector<std::pair<DBuf<double>*, DBuf<double>*>> carry;
+  DBuf<double> keep_x, keep_x2, keep_a, keep_a2;
   long pos_version = 0;   // bumped whenever the user positions change
   long cell_version = -1; // pos_version at the last UpdateCell
   double disp_cur = 0.0; // largest |x - xb| of the positions of the next force evaluation
@@ -512,9 +551,25 @@ struct lpmdx_system {
     cb.alloc(grid.ncell + 2);
     k_cell_begin<<<blocks_for(n + 1), kBlock, 0, st>>>(ka, n, grid.ncell, cb.p);
     PermuteArgs P{va, perm.p, x.p, v.p, a.p, z.p, ffm.p, mass.p, x2.p, v2.p, a2.p, z2.p,
-                  ffm2.p, mass2.p, perm2.p, n};
+                  ffm2.p, mass2.p, perm2.p, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}, n};
+    // extra per-atom vectors (the leapfrog half-step velocities, and during
+    // its set-up the saved positions and accelerations) follow the atoms
+    std::vector<std::pair<DBuf<double>*, DBuf<double>*>> ex;
+    if (lf_ready)
+      ex.push_back({&vh, &vh2});
+    for (auto& e : carry)
+      ex.push_back(e);
+    if (ex.size() > 3)
+      fail(LPMDX_EDEVICE, "too many per-atom arrays to reorder (internal error)");
+    for (size_t e = 0; e < ex.size(); ++e) {
+      ex[e].second->alloc(3 * n);
+      P.ex[e] = ex[e].first->p;
+      P.ex2[e] = ex[e].second->p;
+    }
     k_permute<<<nb, kBlock, 0, st>>>(P);
     launch_check();
+    for (auto& e : ex)
+      std::swap(e.first->p, e.second->p);
     std::swap(x.p, x2.p);
     std::swap(v.p, v2.p);
     std::swap(a.p, a2.p);
@@ -932,9 +987,8 @@ struct lpmdx_system {
       A.partials = partials.p;
       A.maxd2 = maxd2;
       A.xb = xb.p;
+      A.vh = vh.p;
       A.margin = wrap_margin();
-    if (staged && !(A.margin > 0.0))
-      fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
       if (staged && !(A.margin > 0.0))
         fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
       A.P = device_view(P, rcf, rcr);
@@ -975,6 +1029,7 @@ struct lpmdx_system {
       A.a = a.p;
       A.maxd2 = maxd2;
       A.xb = xb.p;
+      A.vh = vh.p;
       A.b = box;
       for (int k = 0; k < 3; ++k)
         A.ll[k] = ll[k];
@@ -1229,6 +1284,175 @@ struct lpmdx_system {
         // the step's state: v in v, positions in xn
         double ek = kinetic(nullptr);
         long idx = done / obs_every - 1;
+        if (ep_t)
+          ep_t[idx] = e;
+        if (ek_t)
+          ek_t[idx] = ek;
+      }
+    }
+  }
+
+  // ---------------------------------------------------------------- leapfrog
+  // Leapfrog::Initialize (plugins/leapfrog.cc:58-67) through
+  // Integrator::UseOldConfig / GoBack (liblpmd/simulation/integrator.cc:16-42):
+  // the old configuration is FittedInside(x - v dt) with velocities v - a dt;
+  // forces at the old positions give a_old, and the half-step velocities are
+  // vh = (v - a dt) + (a_old * 0.5) dt.  x and a are left as they were.
+  void lf_init(double dt) {
+    if (pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    if (cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (!std::isfinite(dt))
+      fail(LPMDX_ENONFINITE, "dt is not finite");
+    undrift();
+    lf_dt = dt;
+    lf_ready = false;
+    if (n == 0) {
+      lf_ready = true;
+      return;
+    }
+    vh.alloc(3 * n);
+    keep_x.alloc(3 * n);
+    keep_a.alloc(3 * n);
+    k_lf_goback<<<blocks_for(n), kBlock, 0, st>>>(x.p, v.p, a.p, keep_x.p, keep_a.p, vh.p, n, dt, ll[0], ll[1],
+                                                  ll[2]);
+    launch_check();
+    // lists and forces at the old positions; vh and the saved x, a follow
+    // the atoms through any reorder
+    carry = {{&vh, &vh2}, {&keep_x, &keep_x2}, {&keep_a, &keep_a2}};
+    list_built = false;
+    forces_ok = false;
+    update_cell();
+    forces(kForcesOnly, false, 0.0, nullptr); // a <- a_old
+    carry.clear();
+    k_lf_halfkick<<<blocks_for(n), kBlock, 0, st>>>(vh.p, a.p, n, dt);
+    launch_check();
+    hipchk(hipMemcpyAsync(x.p, keep_x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    hipchk(hipMemcpyAsync(a.p, keep_a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    list_built = false; // the lists were built for the old positions
+    forces_ok = false;
+    lf_ready = true;
+    sync();
+  }
+
+  // `steps` calls of OneStepIntegrator::Advance with Leapfrog::Advance
+  // (onestepintegrator.cc:13-46, leapfrog.cc:69-77): forces at x, then per
+  // atom vhalf = vh + a dt, x = FittedInside(x + vhalf dt),
+  // v = 0.5 (vhalf + vh), vh = vhalf (fused into the force kernels).  Chunks
+  // and roll-backs as in vv(); the skin test reads the positions each step's
+  // force sees.
+  void lf(double dt, long steps, long obs_every, double* ep_t, double* ek_t) {
+    if (pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    if (cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (!lf_ready)
+      fail(LPMDX_ESTATE, "leapfrog: lpmdx_leapfrog_init was not called");
+    if (!(dt == lf_dt))
+      fail(LPMDX_EINVAL, "leapfrog: dt differs from the one lpmdx_leapfrog_init used");
+    if (n == 0 || steps <= 0)
+      return;
+    undrift();
+    unsigned int* md = ctrl.p->maxd2;
+    auto read_disp = [&](long s) {
+      float f;
+      memcpy(&f, &hctrl->maxd2[s], sizeof f);
+      return std::sqrt((double)f);
+    };
+    if (list_built && !rebuilds_every_update()) {
+      // displacement of the current positions from the build positions
+      ForceArgs A{};
+      A.x = x.p;
+      A.xn = xb.p;
+      A.maxd2 = md;
+      A.b = box;
+      A.n = n;
+      hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
+      k_disp<<<blocks_for(n), kBlock, 0, st>>>(A);
+      launch_check();
+      hipchk(hipMemcpyAsync(&hctrl->maxd2[0], md, sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
+      sync();
+      disp_cur = read_disp(0);
+    }
+    long done = 0, forced = 0;
+    while (done < steps) {
+      long m = steps - done;
+      if (obs_every > 0)
+        m = std::min(m, obs_every - (done % obs_every));
+      m = std::min<long>(m, kMaxChunk);
+      bool need;
+      const bool replay = forced > 0;
+      if (rebuilds_every_update()) {
+        m = 1;
+        need = true;
+      } else {
+        need = !list_built || list_radius() != rl || disp_cur > budget();
+        m = replay ? std::min(m, forced) : std::min(m, chunk);
+      }
+      forced = 0;
+      if (need) {
+        rebuild();
+        disp_cur = 0.0;
+      }
+      const bool snapshot = !rebuilds_every_update() && m > 1 && !replay;
+      if (snapshot) {
+        xs.alloc(3 * n);
+        vs.alloc(3 * n);
+        as.alloc(3 * n);
+        vhs.alloc(3 * n);
+        hipchk(hipMemcpyAsync(vhs.p, vh.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+      }
+      k_chunk_begin<<<blocks_for(std::max<long>(snapshot ? 3 * n : 0, m)), kBlock, 0, st>>>(
+          x.p, v.p, a.p, xs.p, vs.p, as.p, 3 * n, md, m, snapshot ? 1 : 0);
+      launch_check();
+      for (long s = 0; s < m; ++s) {
+        const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
+        forces(kLeapfrog, obs, dt, md + s);
+        std::swap(x.p, xn.p);
+      }
+      hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
+      sync();
+      // step s's force saw the positions step s-1 made (step 0: the chunk's)
+      const double b = budget();
+      double d = disp_cur, growth = 0.0;
+      long bad = -1;
+      for (long s = 0; s < m; ++s) {
+        if (!rebuilds_every_update() && d > b) {
+          bad = s;
+          break;
+        }
+        const double dn = read_disp(s);
+        growth = std::max(growth, dn - d);
+        d = dn;
+      }
+      if (bad >= 0) {
+        if (!snapshot)
+          fail(LPMDX_EDEVICE, "leapfrog: a chunk without a snapshot failed the skin test (internal error)");
+        hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(a.p, as.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        hipchk(hipMemcpyAsync(vh.p, vhs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        forced = bad;
+        chunk = std::max<long>(1, bad);
+        n_rollbacks++;
+        continue;
+      }
+      disp_cur = d;
+      pos_version++;
+      long next = std::min<long>(2 * chunk, kMaxChunk);
+      if (growth > 0.0)
+        next = std::min<long>(next, std::max<long>(1, (long)(0.9 * (b - disp_cur) / growth)));
+      chunk = std::max<long>(1, next);
+      cm.ucalls += m;
+      done += m;
+      n_steps += m;
+      forces_ok = false; // a is the force of the positions before x
+      if (obs_every > 0 && (done % obs_every) == 0) {
+        double e;
+        read_energies(&e, nullptr, nullptr);
+        const double ek = kinetic(nullptr);
+        const long idx = done / obs_every - 1;
         if (ep_t)
           ep_t[idx] = e;
         if (ek_t)
@@ -1575,6 +1799,22 @@ int lpmdx_velocityverlet(lpmdx_system* S, double dt, long steps, long obs_every,
     if (steps < 0 || obs_every < 0)
       fail(LPMDX_EINVAL, "negative step count");
     S->vv(dt, steps, obs_every, epot_t, ekin_t);
+  });
+}
+
+int lpmdx_leapfrog_init(lpmdx_system* S, double dt) {
+  return guard([&] {
+    need(S, "system");
+    S->lf_init(dt);
+  });
+}
+
+int lpmdx_leapfrog(lpmdx_system* S, double dt, long steps, long obs_every, double* epot_t, double* ekin_t) {
+  return guard([&] {
+    need(S, "system");
+    if (steps < 0 || obs_every < 0)
+      fail(LPMDX_EINVAL, "negative step count");
+    S->lf(dt, steps, obs_every, epot_t, ekin_t);
   });
 }
 

@@ -258,7 +258,8 @@ __device__ __forceinline__ void eam_pair(const DevPot& P, float r2, float wsum, 
 enum StepMode {
   kForcesOnly = 0, // a = F/m (or a += with accumulate)
   kInitDrift = 1,  // a = F/m, then next positions xn = drift(x, v, a)
-  kStep = 2        // a = F/m, v += dt/2 (a_old + a), xn = drift(x, v, a)
+  kStep = 2,       // a = F/m, v += dt/2 (a_old + a), xn = drift(x, v, a)
+  kLeapfrog = 3    // a = F/m, vh' = vh + a dt, xn = FittedInside(x + vh' dt), v = (vh' + vh) / 2, vh = vh'
 };
 
 struct ForceArgs {
@@ -277,6 +278,7 @@ struct ForceArgs {
   double* partials;     // [nblocks * 12] when evaluating energies
   unsigned int* maxd2;  // max over atoms of |x_next - xb|^2 (float bits), or null
   const double* xb;     // [3n] positions at the last list build
+  double* vh;           // [3n] leapfrog half-step velocities (kLeapfrog)
   // Largest separation a listed pair can have now (list radius + padding +
   // twice the displacement allowed since the build), or 0.  An atom at least
   // this far from every face has no listed neighbour across the boundary.
@@ -328,6 +330,35 @@ __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, d
   }
   double* v = A.v + 3 * i;
   const double* x = A.x + 3 * i;
+  if (A.mode == kLeapfrog) {
+    // Leapfrog::Advance (plugins/leapfrog.cc:69-77): vhalf = old v + a dt,
+    // x = FittedInside(x + vhalf dt), v = 0.5 (vhalf + old v), old v = vhalf
+    double* vh = A.vh + 3 * i;
+    const double dt = A.dt;
+    const double am[3] = {ax, ay, az};
+    // every load before the first store (the stores could alias them, which
+    // would serialise each load behind the previous component's stores)
+    const double vo[3] = {vh[0], vh[1], vh[2]};
+    const double xo[3] = {x[0], x[1], x[2]};
+    double p[3], vn[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      vn[q] = vo[q] + am[q] * dt;
+      p[q] = xo[q] + vn[q] * dt;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      v[q] = 0.5 * (vn[q] + vo[q]);
+      vh[q] = vn[q];
+      a[q] = am[q];
+    }
+    d2 = build_displacement2(A, i, p[0], p[1], p[2]);
+    double* xn = A.xn + 3 * i;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      xn[q] = fitted_inside(p[q], A.ll[q]);
+    return;
+  }
   double vx = v[0], vy = v[1], vz = v[2];
   if (A.mode == kStep) {
     const double h = 0.5 * A.dt;

@@ -54,6 +54,7 @@ const std::map<std::string, std::vector<Keyword>>& Keywords() {
       {"minimumimage", {{"cutoff", "0.0"}, {"skin", "0"}}},
       {"lcbinary", {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}}},
       {"velocityverlet", {{"dt", "1.0"}, {"start", "1"}}},
+      {"leapfrog", {{"dt", "1.0"}, {"start", "1"}}},
   };
   return m;
 }
@@ -364,19 +365,61 @@ void DeviceVelocityVerlet::HostStep(lpmd::Simulation& sim, lpmd::Potential& p) {
   last_ = nullptr;
 }
 
-void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
+namespace {
+
+// The device runs the step when every potential is a DevicePotential and the
+// cell manager a DeviceCellManager; returns that cell manager, else null.
+DeviceCellManager* DevicePath(lpmd::Simulation& sim, lpmd::Potential& p, std::vector<const DevicePotential*>& dev) {
   auto* cp = dynamic_cast<lpmd::CombinedPotential*>(&p);
   auto* cm = dynamic_cast<DeviceCellManager*>(&sim.GetCellManager());
-  std::vector<const DevicePotential*> dev;
-  bool device = cm && cp && cp->Size() > 0;
-  for (long k = 0; device && k < cp->Size(); ++k) {
+  dev.clear();
+  if (!cm || !cp || cp->Size() == 0)
+    return nullptr;
+  for (long k = 0; k < cp->Size(); ++k) {
     auto* d = dynamic_cast<DevicePotential*>(&(*cp)[k]);
     if (!d)
-      device = false;
-    else
-      dev.push_back(d);
+      return nullptr;
+    dev.push_back(d);
   }
-  if (!device) {
+  return cm;
+}
+
+// After a device step: x, v, a into the host atoms (and hx, hv, ha), the
+// observables CombinedPotential::UpdateForces leaves behind
+// (combinedpotential.h:39-52) and the potential-energy tag.
+void ReadBack(lpmd::Simulation& sim, lpmdx_system* sys, DeviceCellManager* cm, double ep, std::vector<double>& hx,
+              std::vector<double>& hv, std::vector<double>& ha, const char* who) {
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  hx.resize(3 * n);
+  hv.resize(3 * n);
+  ha.resize(3 * n);
+  check(lpmdx_system_get_state(sys, hx.data(), hv.data(), ha.data()), who);
+  for (long i = 0; i < n; ++i) {
+    atoms[i].Position() = lpmd::Vector(hx[3 * i], hx[3 * i + 1], hx[3 * i + 2]);
+    atoms[i].Velocity() = lpmd::Vector(hv[3 * i], hv[3 * i + 1], hv[3 * i + 2]);
+    atoms[i].Acceleration() = lpmd::Vector(ha[3 * i], ha[3 * i + 1], ha[3 * i + 2]);
+  }
+  cm->Mirror(hx); // host and device hold the same positions
+  double obs[LPMDX_NOBS];
+  check(lpmdx_observables(sys, obs), who);
+  sim.Virial() = obs[LPMDX_OBS_VIRIAL];
+  const lpmd::BasicCell& cell = sim.Cell();
+  const double vol = cell[0][0] * cell[1][1] * cell[2][2];
+  lpmd::Matrix& st = sim.StressTensor();
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      st.Set(a, b, obs[LPMDX_OBS_STRESS + 3 * a + b] * vol / 160217.65); // units.cc:9 pressfactor
+  lpmd::Tag tag("potential-energy");
+  sim.SetTag(sim, tag, ep);
+}
+
+} // namespace
+
+void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
+  if (!cm) {
     HostStep(sim, p);
     return;
   }
@@ -395,30 +438,107 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   }
   double ep = 0.0, ek = 0.0;
   check(lpmdx_velocityverlet(sys, dt, 1, 1, &ep, &ek), "velocityverlet");
+  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "velocityverlet");
+}
+
+DeviceLeapfrog::DeviceLeapfrog(std::string args) : lpmd::Plugin("leapfrog", kVersion) {
+  Define(*this, "leapfrog");
+  ProcessArguments(args);
+  dt = double((*this)["dt"]);
+  start_ = int((*this)["start"]);
+}
+
+void DeviceLeapfrog::ShowHelp() const {
+  std::cout << " Module Name        = leapfrog (MI355X engine)\n"
+            << "      Leapfrog with dt (fs); on the GPU when every potential and the cell\n"
+            << "      manager are MI355X plugins.\n";
+}
+
+void DeviceLeapfrog::Initialize(lpmd::Simulation& sim, lpmd::Potential& p) {
+  // Integrator::UseOldConfig sets the old configuration up once per
+  // integrator object (integrator.cc:16-22)
+  if (mode_ != kUnset)
+    return;
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
   lpmd::BasicParticleSet& atoms = sim.Atoms();
   const long n = atoms.Size();
-  hx_.resize(3 * n);
-  hv_.resize(3 * n);
-  ha_.resize(3 * n);
-  check(lpmdx_system_get_state(sys, hx_.data(), hv_.data(), ha_.data()), "velocityverlet");
-  for (long i = 0; i < n; ++i) {
-    atoms[i].Position() = lpmd::Vector(hx_[3 * i], hx_[3 * i + 1], hx_[3 * i + 2]);
-    atoms[i].Velocity() = lpmd::Vector(hv_[3 * i], hv_[3 * i + 1], hv_[3 * i + 2]);
-    atoms[i].Acceleration() = lpmd::Vector(ha_[3 * i], ha_[3 * i + 1], ha_[3 * i + 2]);
+  if (cm) {
+    lpmdx_system* sys = cm->Bind(sim, false);
+    cm->UsePotentials(dev);
+    std::vector<double> x, v, a(3 * n);
+    Positions(sim, x);
+    Velocities(sim, v);
+    for (long i = 0; i < n; ++i)
+      for (int q = 0; q < 3; ++q)
+        a[3 * i + q] = atoms[i].Acceleration()[q];
+    check(lpmdx_system_set_state(sys, x.data(), v.data(), a.data()), "leapfrog");
+    check(lpmdx_leapfrog_init(sys, dt), "leapfrog");
+    hx_.swap(x);
+    sys_ = sys;
+    mode_ = kDevice;
+    return;
   }
-  cm->Mirror(hx_); // host and device hold the same positions
-  // what CombinedPotential::UpdateForces leaves behind (combinedpotential.h:39-52)
-  double obs[LPMDX_NOBS];
-  check(lpmdx_observables(sys, obs), "velocityverlet");
-  sim.Virial() = obs[LPMDX_OBS_VIRIAL];
-  const lpmd::BasicCell& cell = sim.Cell();
-  const double vol = cell[0][0] * cell[1][1] * cell[2][2];
-  lpmd::Matrix& st = sim.StressTensor();
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      st.Set(a, b, obs[LPMDX_OBS_STRESS + 3 * a + b] * vol / 160217.65); // units.cc:9 pressfactor
-  lpmd::Tag tag("potential-energy");
-  sim.SetTag(sim, tag, ep);
+  // Leapfrog::Initialize (leapfrog.cc:58-67) on the host: the old
+  // configuration of Integrator::GoBack (integrator.cc:31-42) is set in
+  // place, its forces taken, and x, a restored
+  lpmd::BasicCell& cell = sim.Cell();
+  std::vector<lpmd::Vector> keep_x(n), keep_a(n);
+  vh_.resize(n);
+  for (long i = 0; i < n; ++i) {
+    keep_x[i] = atoms[i].Position();
+    keep_a[i] = atoms[i].Acceleration();
+    const lpmd::Vector np = atoms[i].Position() - atoms[i].Velocity() * dt;
+    vh_[i] = atoms[i].Velocity() - atoms[i].Acceleration() * dt;
+    atoms[i].Position() = cell.FittedInside(np);
+  }
+  p.UpdateForces(sim);
+  for (long i = 0; i < n; ++i) {
+    vh_[i] += atoms[i].Acceleration() * 0.5 * dt;
+    atoms[i].Position() = keep_x[i];
+    atoms[i].Acceleration() = keep_a[i];
+  }
+  mode_ = kHost;
+}
+
+void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
+  if (mode_ == kUnset)
+    throw lpmd::RuntimeError("leapfrog (MI355X): Advance before Initialize");
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
+  if ((cm != nullptr) != (mode_ == kDevice))
+    throw lpmd::RuntimeError("leapfrog (MI355X): the potentials or the cell manager changed between Initialize and "
+                             "Advance");
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  if (!cm) {
+    // OneStepIntegrator::Advance with Leapfrog::Advance (onestepintegrator.cc:13-46,
+    // leapfrog.cc:69-77); vh_ plays the old configuration's velocities
+    p.UpdateForces(sim);
+    lpmd::BasicCell& cell = sim.Cell();
+    for (long i = 0; i < n; ++i) {
+      const lpmd::Vector vhalf = vh_[i] + atoms[i].Acceleration() * dt;
+      atoms[i].Position() = cell.FittedInside(atoms[i].Position() + vhalf * dt);
+      atoms[i].Velocity() = 0.5 * (vhalf + vh_[i]);
+      vh_[i] = vhalf;
+    }
+    return;
+  }
+  lpmdx_system* sys = cm->Bind(sim, false);
+  if (sys != sys_)
+    throw lpmd::RuntimeError("leapfrog (MI355X): the cell manager's system changed since Initialize");
+  cm->UsePotentials(dev);
+  std::vector<double> x;
+  Positions(sim, x);
+  if (x != hx_) {
+    // the host moved atoms: Leapfrog::Advance reads the positions and the
+    // half-step velocities only, so the positions alone are sent
+    check(lpmdx_system_set_state(sys, x.data(), nullptr, nullptr), "leapfrog");
+    hx_.swap(x);
+  }
+  double ep = 0.0, ek = 0.0;
+  check(lpmdx_leapfrog(sys, dt, 1, 1, &ep, &ek), "leapfrog");
+  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "leapfrog");
 }
 
 // ------------------------------------------------------------------ factory
@@ -434,6 +554,8 @@ lpmd::Plugin* Create(const std::string& name, const std::string& args) {
     return new DeviceCellManager(name, args);
   if (name == "velocityverlet")
     return new DeviceVelocityVerlet(args);
+  if (name == "leapfrog")
+    return new DeviceLeapfrog(args);
   throw lpmd::RuntimeError("no MI355X plugin named " + name);
 }
 

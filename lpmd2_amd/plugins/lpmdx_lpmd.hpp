@@ -13,6 +13,7 @@
 //                         finnissinclair-ext tabulatedpair (plugins/<name>.cc)
 //   DeviceVelocityVerlet  lpmd::Integrator (liblpmd/lpmd/integrator.h:15-40) as
 //                         "velocityverlet" (plugins/velocityverlet.cc:13-20)
+//   DeviceLeapfrog        lpmd::Integrator as "leapfrog" (plugins/leapfrog.cc:14-23)
 // They take the same plugin names and keyword strings, so a directory holding
 // their <name>.so files placed first on LPMD_PATH replaces the CPU plugins.
 // The classes live in one shared library (liblpmdx_lpmd.so) so that every
@@ -124,6 +125,28 @@ private:
   long start_ = 1;
   std::vector<double> hx_, hv_, ha_;
   lpmdx_system* last_ = nullptr;
+};
+
+class DeviceLeapfrog : public lpmd::Integrator, public lpmd::Plugin {
+public:
+  explicit DeviceLeapfrog(std::string args);
+  void ShowHelp() const override;
+  // Leapfrog::Initialize (leapfrog.cc:58-67): half-step velocities from the
+  // forces at the old configuration, once per integrator object.  On the
+  // device (lpmdx_leapfrog_init) under the same condition as
+  // DeviceVelocityVerlet; the half-step velocities then stay in HBM.
+  void Initialize(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  // One leapfrog step (onestepintegrator.cc:13-46, leapfrog.cc:69-77), on the
+  // device or, when Initialize ran on the host, on the host.
+  void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+
+private:
+  enum Mode { kUnset, kDevice, kHost };
+  Mode mode_ = kUnset;
+  long start_ = 1;
+  std::vector<lpmd::Vector> vh_; // host path: the old configuration's velocities
+  std::vector<double> hx_, hv_, ha_;
+  lpmdx_system* sys_ = nullptr;
 };
 
 // Plugin factory behind every <name>.so (create/destroy, liblpmd/lpmd/plugin.h:42-46).

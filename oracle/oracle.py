@@ -352,6 +352,44 @@ def md_run(conf: Conf, cm: CellManager, pots, dt: float, steps: int):
     return ep, ek, vr
 
 
+def fitted_inside_all(x, L):
+    """OrthogonalCell::FittedInside (orthogonalcell.h:102-114) on every row of
+    x (n, 3), the arithmetic of orc_fitted_inside: outside [0, L] the position
+    moves by L floor(x / L); x == L stays."""
+    x = np.array(x, dtype=np.float64)
+    L = np.asarray(L, dtype=np.float64)
+    out = (x < 0.0) | (x > L)
+    x[out] = (x - L * np.floor(x / L))[out]
+    return x
+
+
+def leapfrog_run(conf: Conf, forces, dt: float, steps: int):
+    """Leapfrog::Initialize then `steps` OneStepIntegrator::Advance calls
+    (plugins/leapfrog.cc:58-77, liblpmd/simulation/onestepintegrator.cc:13-46).
+    forces(c) fills c.a for configuration c and returns its potential energy
+    (update_forces with a cell manager and potentials, or replayed tables).
+    Set-up: the old configuration FittedInside(x - v dt) with velocities v - a dt
+    (Integrator::GoBack, integrator.cc:31-42), forces there, and half-step
+    velocities vh = (v - a dt) + (a_old * 0.5) dt.  Each step: forces at x,
+    vhalf = vh + a dt, x = FittedInside(x + vhalf dt), v = 0.5 (vhalf + vh),
+    vh = vhalf.  Returns the per-step potential energy (at the positions the
+    step's forces saw) and kinetic energy (of the velocities after the step)."""
+    L = np.asarray(conf.L, dtype=np.float64)
+    old = Conf(fitted_inside_all(conf.x - conf.v * dt, L), L, z=conf.z, mass=conf.m, v=conf.v - conf.a * dt)
+    forces(old)
+    vh = old.v + old.a * 0.5 * dt
+    ep = np.zeros(steps)
+    ek = np.zeros(steps)
+    for s in range(steps):
+        ep[s] = forces(conf)
+        vn = vh + conf.a * dt
+        conf.x[:] = fitted_inside_all(conf.x + vn * dt, L)
+        conf.v[:] = 0.5 * (vn + vh)
+        vh = vn
+        ek[s] = kinetic_energy(conf)
+    return ep, ek
+
+
 def vv_position(conf: Conf, dt: float):
     aold = np.zeros((conf.n, 3))
     lib().orc_vv_position(conf.struct(), dt, _p(aold))

@@ -22,11 +22,12 @@ CORE_SRCS := \
   liblpmd/simulation/potential.cc liblpmd/simulation/pairpotential.cc \
   liblpmd/simulation/metalpotential.cc liblpmd/simulation/cellgenerator.cc \
   liblpmd/simulation/integrator.cc liblpmd/simulation/twostepintegrator.cc \
+  liblpmd/simulation/onestepintegrator.cc \
   liblpmd/simulation/simulation.cc liblpmd/simulation/systemmodifier.cc
 
 PLUGINS := linkedcell verletlist minimumimage lcbinary lennardjones fastlj morse buckingham tabulatedpair \
            suttonchen finnissinclair gupta harmonic lennardjonesMod finnissinclair-ext \
-           crystal3d velocityverlet
+           crystal3d velocityverlet leapfrog
 
 CORE_OBJS := $(patsubst %.cc,$(OUT)/obj/%.o,$(CORE_SRCS))
 PLUGIN_SOS := $(patsubst %,$(OUT)/%.so,$(PLUGINS))

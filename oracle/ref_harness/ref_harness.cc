@@ -350,19 +350,19 @@ int ref_crystal(const char* so_path, const char* args, long n, const double* L, 
   });
 }
 
-// ---- velocityverlet + TwoStepIntegrator (plugins/velocityverlet.cc, -------
-// ---- liblpmd/simulation/twostepintegrator.cc, integrator.cc) -------------
-// Runs `steps` calls of TwoStepIntegrator::Advance on conf.  The potential
-// replays acc_tables[call] (3n doubles each, ncalls tables) on its UpdateForces
-// calls, in order; the cell manager `cm` is attached first because
-// Integrator::UseOldConfig needs one.
+// ---- an integrator plugin (plugins/velocityverlet.cc, plugins/leapfrog.cc)
+// ---- over liblpmd/simulation/{two,one}stepintegrator.cc, integrator.cc ----
+// Runs Integrator::Initialize, then `steps` calls of Integrator::Advance on
+// conf.  The potential replays acc_tables[call] (3n doubles each, ncalls
+// tables) on its UpdateForces calls, in order; the cell manager `cm` is
+// attached first because Integrator::UseOldConfig needs one.
 int ref_vv_run(const char* so_path, const char* args, void* conf, void* cm, long steps,
                long ncalls, const double* acc_tables) {
   return Guard([&] {
     Loaded g = Load(so_path, args);
-    TwoStepIntegrator* vv = dynamic_cast<TwoStepIntegrator*>(g.plugin);
+    lpmd::Integrator* vv = dynamic_cast<lpmd::Integrator*>(g.plugin);
     if (!vv)
-      throw std::runtime_error("not a TwoStepIntegrator");
+      throw std::runtime_error("not an Integrator");
     HarnessConf* c = static_cast<HarnessConf*>(conf);
     CellManager* m = dynamic_cast<CellManager*>(static_cast<Loaded*>(cm)->plugin);
     c->SetCellManager(*m);

@@ -173,10 +173,15 @@ def crystal(args: str, n: int, L):
     return pos
 
 
-def vv_run(args: str, conf: RefConf, cm: RefCellManager, steps: int, acc_tables):
+def vv_run(args: str, conf: RefConf, cm: RefCellManager, steps: int, acc_tables, integrator="velocityverlet",
+           path=None):
+    """Initialize + `steps` Advance calls of an integrator plugin (the
+    reference's of that name, or the .so at `path`), its force calls
+    replaying acc_tables in order."""
     acc = np.ascontiguousarray(acc_tables, dtype=np.float64)
     ncalls = acc.shape[0]
-    _check(lib().ref_vv_run(so("velocityverlet"), (args + " debug none").encode(), conf.h, cm.h, steps, ncalls, _p(acc)))
+    lp = path.encode() if path else so(integrator)
+    _check(lib().ref_vv_run(lp, (args + " debug none").encode(), conf.h, cm.h, steps, ncalls, _p(acc)))
 
 
 # ---- plugins loaded through the reference's create(args), and the reference's

@@ -118,11 +118,15 @@ def test_dropin_species_pairs_rocksalt():
     assert abs(e - e0) <= 1e-6 * abs(e0)
 
 
-def native_run(x, L, z, v, cm, cm_args, pot, dt, steps):
+def native_run(x, L, z, v, cm, cm_args, pot, dt, steps, integrator="velocityverlet"):
     s = E.System(x, L, z, v=v)
     s.set_cellmanager(cm, cm_args)
     s.append_potential(*pot)
-    s.velocityverlet(dt, steps)
+    if integrator == "leapfrog":
+        s.leapfrog_init(dt)
+        s.leapfrog(dt, steps)
+    else:
+        s.velocityverlet(dt, steps)
     return s.get_state()
 
 
@@ -154,4 +158,35 @@ def test_dropin_integration_follows_native_path(integrator):
     assert dx <= 1e-10 and dv <= 1e-12
     if integrator == "dropin":
         # the drop-in integrator sets the potential-energy tag every step
+        assert np.all(np.isfinite(ep)) and ep[-1] != 0.0
+
+
+@pytest.mark.parametrize("integrator", ["reference", "dropin"])
+def test_dropin_leapfrog_follows_native_path(integrator):
+    """The reference leapfrog plugin over drop-in forces (its set-up force
+    call on the old configuration included) and the drop-in leapfrog both
+    follow the engine's own leapfrog."""
+    say("drop-in leapfrog", integrator)
+    x, L = E.fcc(5, 3.61)
+    x = np.mod(x + np.random.default_rng(9).normal(0.0, 0.02, x.shape), L)
+    m = np.full(x.shape[0], E.ELEMENT_MASS[29])
+    v = E.maxwell_boltzmann(m, 600.0, seed=5)
+    pot = ("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22")
+    cm_args = "cutoff 7.22 maxneighbors 400 skin 0.3"
+    steps, dt = 30, 2.0
+    conf = R.RefConf(x, L, z=29)
+    conf.set_velocities(v)
+    cm = R.RefPlugin.dropin("verletlist", cm_args)
+    cm.attach(conf)
+    pots = [R.RefPlugin.dropin(*pot)]
+    make = R.RefPlugin.reference if integrator == "reference" else R.RefPlugin.dropin
+    itg = make("leapfrog", f"dt {dt}")
+    ep = R.integrate(itg, conf, pots, steps)
+    xr, vr, ar = conf.get()
+    xn, vn, an = native_run(x, L, 29, v, "verletlist", cm_args, pot, dt, steps, integrator="leapfrog")
+    dx = np.abs(xr - xn).max()
+    dv = np.abs(vr - vn).max()
+    say(f"  max |dx| {dx:.3e}  |dv| {dv:.3e}")
+    assert dx <= 1e-10 and dv <= 1e-12
+    if integrator == "dropin":
         assert np.all(np.isfinite(ep)) and ep[-1] != 0.0

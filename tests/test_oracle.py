@@ -164,6 +164,44 @@ def test_velocityverlet_matches_reference():
     assert np.array_equal(conf.v, d["vf"])
 
 
+def _replay(tables):
+    calls = iter(tables)
+
+    def forces(c):
+        c.a[:] = next(calls)
+        return 0.0
+
+    return forces
+
+
+def test_leapfrog_matches_reference():
+    """Leapfrog::Initialize + 12 Advance calls of the reference plugin with
+    replayed accelerations; one atom crosses the box several times per step."""
+    d = np.load(os.path.join(GOLDEN, "lf_ar108.npz"))
+    conf = O.Conf(d["x0"], d["L"], v=d["v0"])
+    acc = d["acc"]
+    O.leapfrog_run(conf, _replay(acc), 2.0, acc.shape[0] - 1)
+    assert np.array_equal(conf.x, d["xf"])
+    assert np.array_equal(conf.v, d["vf"])
+    assert np.array_equal(conf.a, d["af"])
+
+
+def test_leapfrog_constant_acceleration_closed_form():
+    """With constant a the positions follow x0 + v0 t + a t^2 / 2 exactly and
+    the reported velocity is the one at the last force evaluation, v0 + a (t - dt)."""
+    L = [1e6] * 3
+    x0 = np.array([[10.0, 20.0, 30.0]])
+    v0 = np.array([[0.01, -0.02, 0.005]])
+    a0 = np.array([[1e-4, 2e-4, -3e-4]])
+    conf = O.Conf(x0, L, v=v0)
+    conf.a[:] = a0
+    dt, steps = 0.5, 40
+    O.leapfrog_run(conf, _replay([a0] * (steps + 1)), dt, steps)
+    t = dt * steps
+    np.testing.assert_allclose(conf.x, x0 + v0 * t + 0.5 * a0 * t * t, rtol=1e-12)
+    np.testing.assert_allclose(conf.v, v0 + a0 * (t - dt), rtol=1e-12)
+
+
 def test_velocityverlet_exact_for_constant_acceleration():
     L = [1e6] * 3
     x0 = np.array([[10.0, 20.0, 30.0]])
