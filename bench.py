@@ -81,6 +81,15 @@ WORKLOADS = {
         rc=2.5 * 3.41, skin=0.3 * 3.41,
         metal=False, integrator="leapfrog",
     ),
+    "lj_4m_beeman": dict(
+        desc="LJ liquid fcc 100^3 = 4,000,000 atoms, rho* 0.8442, T* 0.72, rc 2.5 sigma, skin 0.3 sigma, dt* 0.005, "
+             "beeman",
+        nx=100, a=1.679596 * 3.41, z=18, T=0.72 * 0.0103 / 8.6173422e-05, noise=0.0, dt=0.005 * 2161.98, seed=7,
+        pot=("lennardjones", f"sigma 3.41 epsilon 0.0103 cutoff {2.5 * 3.41}"),
+        cm=("verletlist", f"cutoff {2.5 * 3.41} maxneighbors 400 each 5 skin {0.3 * 3.41}"),
+        rc=2.5 * 3.41, skin=0.3 * 3.41,
+        metal=False, integrator="beeman",
+    ),
     # the widened potentials, one 131072-atom box each
     "harmonic_131k": dict(
         desc="harmonic springs k 3.4 a 3.0 on fcc 32^3 = 131072 atoms, a 4.243 (nearest neighbours at the spring "
@@ -412,6 +421,9 @@ def main():
     if w.get("integrator") == "leapfrog":
         s.leapfrog_init(w["dt"])
         advance = s.leapfrog
+    elif w.get("integrator") == "beeman":
+        s.beeman_init(w["dt"])
+        advance = s.beeman
     else:
         advance = s.velocityverlet
     t0 = time.perf_counter()
@@ -440,6 +452,8 @@ def main():
     else:
         bytes_force = n * (148 + 4 * nbar)
         kname = "k_pair_force"
+    if w.get("integrator") == "beeman":
+        bytes_force += n * 48  # the previous step's accelerations, read and written
     if st1["brick"]:
         kname += "_bk"  # the LDS-staged variant, one workgroup per brick of cells
     avg_force_s = f_ms / max(f_n, 1) / 1e3

@@ -161,6 +161,19 @@ LPMDX_API int lpmdx_leapfrog_init(lpmdx_system* sys, double dt);
 LPMDX_API int lpmdx_leapfrog(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
                              double* ekin_t);
 
+/* The Beeman integrator (plugins/beeman.cc:55-84).  lpmdx_beeman_init is
+ * Beeman::Initialize: forces at the old configuration FittedInside(x - v dt)
+ * (Integrator::GoBack, integrator.cc:31-42) become the first step's previous
+ * acceleration; x, v and a are left as they were.  lpmdx_beeman then runs
+ * `steps` TwoStepIntegrator::Advance calls (twostepintegrator.cc:12-68) with
+ * Beeman's AdvancePosition / AdvanceVelocity and that dt, one force
+ * evaluation per step as in lpmdx_velocityverlet.  Energies every obs_every
+ * steps as there.  Without a prior lpmdx_beeman_init: LPMDX_ESTATE; another
+ * dt: LPMDX_EINVAL. */
+LPMDX_API int lpmdx_beeman_init(lpmdx_system* sys, double dt);
+LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
+                           double* ekin_t);
+
 /* Device-side SetSimulationTags (simulationbuilder.cc:84-124, properties.h:22-81):
  * out[LPMDX_NOBS], see LPMDX_OBS_*.  The potential terms are those of the
  * last force evaluation. */

@@ -62,6 +62,8 @@ def lib():
         L.lpmdx_velocityverlet.argtypes = [vp, d, l, l, vp, vp]
         L.lpmdx_leapfrog_init.argtypes = [vp, d]
         L.lpmdx_leapfrog.argtypes = [vp, d, l, l, vp, vp]
+        L.lpmdx_beeman_init.argtypes = [vp, d]
+        L.lpmdx_beeman.argtypes = [vp, d, l, l, vp, vp]
         L.lpmdx_observables.argtypes = [vp, vp]
         L.lpmdx_system_stats.argtypes = [vp, vp]
         L.lpmdx_system_set_lanes.argtypes = [vp, i]
@@ -167,6 +169,17 @@ class System:
         ep = np.zeros(max(k, 1))
         ek = np.zeros(max(k, 1))
         _check(lib().lpmdx_leapfrog(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
+        return ep[:k], ek[:k]
+
+    def beeman_init(self, dt: float):
+        """Beeman::Initialize: the previous acceleration from the old configuration."""
+        _check(lib().lpmdx_beeman_init(self._h, dt))
+
+    def beeman(self, dt: float, steps: int, obs_every: int = 0):
+        k = steps // obs_every if obs_every else 0
+        ep = np.zeros(max(k, 1))
+        ek = np.zeros(max(k, 1))
+        _check(lib().lpmdx_beeman(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
         return ep[:k], ek[:k]
 
     def neighbor_lists(self, full: bool, rcut: float):

@@ -219,6 +219,8 @@ static __global__ __launch_bounds__(kHaloThreads) void k_brick_halo(const int* _
 
 // Reorders the per-atom state into the new (cell-sorted) order.  order[k] is
 // the current slot of the atom placed at k; perm maps slots to caller indices.
+constexpr int kPermuteExtra = 4;
+
 struct PermuteArgs {
   const uint32_t* order;
   const int* perm_old;
@@ -235,10 +237,10 @@ struct PermuteArgs {
   double* ffm2;
   double* mass2;
   int* perm_new;
-  // up to three more per-atom 3-vectors moved along (null: unused), e.g. the
-  // leapfrog half-step velocities
-  const double* ex[3];
-  double* ex2[3];
+  // up to kPermuteExtra more per-atom 3-vectors moved along (null: unused),
+  // e.g. the leapfrog half-step velocities and the Beeman history
+  const double* ex[kPermuteExtra];
+  double* ex2[kPermuteExtra];
   long n;
 };
 
@@ -258,9 +260,9 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   }
   const int z = A.z[s], o = A.perm_old[s];
   const double ffm = A.ffm[s], mass = A.mass[s];
-  double h[3][3];
+  double h[kPermuteExtra][3];
 #pragma unroll
-  for (int e = 0; e < 3; ++e)
+  for (int e = 0; e < kPermuteExtra; ++e)
 #pragma unroll
     for (int q = 0; q < 3; ++q)
       h[e][q] = A.ex[e] ? A.ex[e][3 * s + q] : 0.0;
@@ -275,7 +277,7 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   A.mass2[k] = mass;
   A.perm_new[k] = o;
 #pragma unroll
-  for (int e = 0; e < 3; ++e)
+  for (int e = 0; e < kPermuteExtra; ++e)
     if (A.ex[e]) {
 #pragma unroll
       for (int q = 0; q < 3; ++q)

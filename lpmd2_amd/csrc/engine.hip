@@ -70,7 +70,8 @@ __global__ __launch_bounds__(kBlock) void k_chunk_begin(const double* __restrict
 }
 
 // Integrator::GoBack (liblpmd/simulation/integrator.cc:31-42) for the leapfrog
-// set-up: x and a saved, vh = v - a dt, x = FittedInside(x - v dt).
+// and Beeman set-ups: x and a saved, vh = v - a dt (when vh is given),
+// x = FittedInside(x - v dt).
 __global__ __launch_bounds__(kBlock) void k_lf_goback(double* __restrict__ x, const double* __restrict__ v,
                                                       const double* __restrict__ a, double* __restrict__ keep_x,
                                                       double* __restrict__ keep_a, double* __restrict__ vh, long n,
@@ -84,7 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_lf_goback(double* __restrict__ x, co
     const double xq = x[3 * i + q], vq = v[3 * i + q], aq = a[3 * i + q];
     keep_x[3 * i + q] = xq;
     keep_a[3 * i + q] = aq;
-    vh[3 * i + q] = vq - aq * dt;
+    if (vh)
+      vh[3 * i + q] = vq - aq * dt;
     x[3 * i + q] = fitted_inside(xq - vq * dt, ll[q]);
   }
 }
@@ -200,6 +202,11 @@ struct lpmdx_system {
   // internal order, once lpmdx_leapfrog_init has set them
   DBuf<double> vh, vh2, vhs;
   bool lf_ready = false;
+  // Beeman: the acceleration of the step before a's (ab), its reorder and
+  // snapshot copies; drift_beeman says which scheme made the look-ahead x
+  DBuf<double> ab, ab2, abs_snap;
+  bool bm_ready = false, drift_beeman = false;
+  double bm_dt = 0.0;
   double lf_dt = 0.0;
   // more arrays that reorders carry along (with their scratch twins)
   std::vector<std::pair<DBuf<double>*, DBuf<double>*>> carry;
@@ -557,9 +564,11 @@ struct lpmdx_system {
     std::vector<std::pair<DBuf<double>*, DBuf<double>*>> ex;
     if (lf_ready)
       ex.push_back({&vh, &vh2});
+    if (bm_ready)
+      ex.push_back({&ab, &ab2});
     for (auto& e : carry)
       ex.push_back(e);
-    if (ex.size() > 3)
+    if (ex.size() > (size_t)kPermuteExtra)
       fail(LPMDX_EDEVICE, "too many per-atom arrays to reorder (internal error)");
     for (size_t e = 0; e < ex.size(); ++e) {
       ex[e].second->alloc(3 * n);
@@ -988,6 +997,7 @@ struct lpmdx_system {
       A.maxd2 = maxd2;
       A.xb = xb.p;
       A.vh = vh.p;
+      A.ab = ab.p;
       A.margin = wrap_margin();
       if (staged && !(A.margin > 0.0))
         fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
@@ -1030,6 +1040,7 @@ struct lpmdx_system {
       A.maxd2 = maxd2;
       A.xb = xb.p;
       A.vh = vh.p;
+      A.ab = ab.p;
       A.b = box;
       for (int k = 0; k < 3; ++k)
         A.ll[k] = ll[k];
@@ -1142,15 +1153,27 @@ struct lpmdx_system {
   }
 
   // ------------------------------------------------------------------ VV
-  void vv(double dt, long steps, long obs_every, double* ep_t, double* ek_t) {
+  // beeman: the same two-step scheme with Beeman's updates (beeman.cc:63-84)
+  // in place of velocity Verlet's; ab carries its extra history.
+  void vv(double dt, long steps, long obs_every, double* ep_t, double* ek_t, bool beeman = false) {
+    const char* who = beeman ? "beeman" : "velocityverlet";
     if (pots.empty())
       fail(LPMDX_ESTATE, "no potential set");
     if (cm.kind == kNone)
       fail(LPMDX_ESTATE, "cellmanager not set");
     if (!(dt == dt) || !std::isfinite(dt))
       fail(LPMDX_ENONFINITE, "dt is not finite");
+    if (beeman && !bm_ready)
+      fail(LPMDX_ESTATE, "beeman: lpmdx_beeman_init was not called");
+    if (beeman && !(dt == bm_dt))
+      fail(LPMDX_EINVAL, "beeman: dt differs from the one lpmdx_beeman_init used");
     if (n == 0 || steps <= 0)
       return;
+    // a look-ahead made by the other scheme is taken back (a stays valid)
+    if (drifted && drift_beeman != beeman)
+      undrift();
+    const int init_mode = beeman ? kBeemanInit : kInitDrift;
+    const int step_mode = beeman ? kBeemanStep : kStep;
     unsigned int* md = ctrl.p->maxd2;
     auto read_disp = [&](long s) {
       float f;
@@ -1163,19 +1186,20 @@ struct lpmdx_system {
         update_cell();
       hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
       if (!forces_ok) {
-        forces(kInitDrift, false, dt, md);
+        forces(init_mode, false, dt, md);
       } else {
         ForceArgs A{};
         A.x = x.p;
         A.xn = xn.p;
         A.v = v.p;
         A.a = a.p;
+        A.ab = ab.p;
         A.maxd2 = md;
         A.xb = xb.p;
         A.b = box;
         A.n = n;
         A.dt = dt;
-        A.mode = kStep;
+        A.mode = beeman ? kBeemanInit : kStep;
         for (int k = 0; k < 3; ++k)
           A.ll[k] = ll[k];
         k_drift<<<blocks_for(n), kBlock, 0, st>>>(A);
@@ -1183,6 +1207,7 @@ struct lpmdx_system {
       }
       std::swap(x.p, xn.p);
       drifted = true;
+      drift_beeman = beeman;
       forces_ok = true;
       hipchk(hipMemcpyAsync(&hctrl->maxd2[0], md, sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       sync();
@@ -1228,6 +1253,10 @@ struct lpmdx_system {
         xs.alloc(3 * n);
         vs.alloc(3 * n);
         as.alloc(3 * n);
+        if (beeman) {
+          abs_snap.alloc(3 * n);
+          hipchk(hipMemcpyAsync(abs_snap.p, ab.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        }
       }
       // snapshot and cleared per-step maxima in one launch (m <= kMaxChunk)
       k_chunk_begin<<<blocks_for(std::max<long>(snapshot ? 3 * n : 0, m)), kBlock, 0, st>>>(
@@ -1235,7 +1264,7 @@ struct lpmdx_system {
       launch_check();
       for (long s = 0; s < m; ++s) {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
-        forces(kStep, obs, dt, md + s);
+        forces(step_mode, obs, dt, md + s);
         std::swap(x.p, xn.p);
       }
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
@@ -1255,11 +1284,13 @@ struct lpmdx_system {
       }
       if (bad >= 0) {
         if (!snapshot)
-          fail(LPMDX_EDEVICE, "velocityverlet: a chunk without a snapshot failed the skin test (internal error)");
+          fail(LPMDX_EDEVICE, std::string(who) + ": a chunk without a snapshot failed the skin test (internal error)");
         // back to the chunk start; re-run the valid steps, then rebuild
         hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
         hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
         hipchk(hipMemcpyAsync(a.p, as.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+        if (beeman)
+          hipchk(hipMemcpyAsync(ab.p, abs_snap.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
         drifted = true;
         forced = bad;
         chunk = std::max<long>(1, bad);
@@ -1275,6 +1306,7 @@ struct lpmdx_system {
         next = std::min<long>(next, std::max<long>(1, (long)(0.9 * (b - disp_cur) / growth)));
       chunk = std::max<long>(1, next);
       drifted = true;
+      drift_beeman = beeman;
       cm.ucalls += 2 * m;
       done += m;
       n_steps += m;
@@ -1312,27 +1344,76 @@ struct lpmdx_system {
       lf_ready = true;
       return;
     }
-    vh.alloc(3 * n);
-    keep_x.alloc(3 * n);
-    keep_a.alloc(3 * n);
-    k_lf_goback<<<blocks_for(n), kBlock, 0, st>>>(x.p, v.p, a.p, keep_x.p, keep_a.p, vh.p, n, dt, ll[0], ll[1],
-                                                  ll[2]);
-    launch_check();
-    // lists and forces at the old positions; vh and the saved x, a follow
-    // the atoms through any reorder
-    carry = {{&vh, &vh2}, {&keep_x, &keep_x2}, {&keep_a, &keep_a2}};
-    list_built = false;
-    forces_ok = false;
-    update_cell();
-    forces(kForcesOnly, false, 0.0, nullptr); // a <- a_old
-    carry.clear();
+    old_config_forces(dt, true);
     k_lf_halfkick<<<blocks_for(n), kBlock, 0, st>>>(vh.p, a.p, n, dt);
     launch_check();
+    restore_after_old_config();
+    lf_ready = true;
+    sync();
+  }
+
+  // Integrator::UseOldConfig / GoBack (integrator.cc:16-42) and one force
+  // call on the old configuration FittedInside(x - v dt): leaves a = a_old at
+  // the old positions in x, keep_x / keep_a holding x and a (and, with_vh,
+  // vh = v - a dt), all in the atom order after any reorder of that call.
+  void old_config_forces(double dt, bool with_vh) {
+    carry.clear();
+    keep_x.alloc(3 * n);
+    keep_a.alloc(3 * n);
+    if (with_vh)
+      vh.alloc(3 * n);
+    k_lf_goback<<<blocks_for(n), kBlock, 0, st>>>(x.p, v.p, a.p, keep_x.p, keep_a.p, with_vh ? vh.p : nullptr, n,
+                                                  dt, ll[0], ll[1], ll[2]);
+    launch_check();
+    // lists and forces at the old positions; the saved arrays follow the
+    // atoms through any reorder
+    if (with_vh)
+      carry.push_back({&vh, &vh2});
+    carry.push_back({&keep_x, &keep_x2});
+    carry.push_back({&keep_a, &keep_a2});
+    list_built = false;
+    forces_ok = false;
+    try {
+      update_cell();
+      forces(kForcesOnly, false, 0.0, nullptr); // a <- a_old
+    } catch (...) {
+      carry.clear();
+      throw;
+    }
+    carry.clear();
+  }
+
+  // x and a back from keep_x / keep_a after old_config_forces
+  void restore_after_old_config() {
     hipchk(hipMemcpyAsync(x.p, keep_x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
     hipchk(hipMemcpyAsync(a.p, keep_a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
     list_built = false; // the lists were built for the old positions
     forces_ok = false;
-    lf_ready = true;
+  }
+
+  // ---------------------------------------------------------------- beeman
+  // Beeman::Initialize (plugins/beeman.cc:55-61): the forces at the old
+  // configuration of Integrator::GoBack give the first step's a_old (ab).
+  // x, v and a are left as they were.  The steps are vv(..., beeman = true).
+  void bm_init(double dt) {
+    if (pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    if (cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (!std::isfinite(dt))
+      fail(LPMDX_ENONFINITE, "dt is not finite");
+    undrift();
+    bm_dt = dt;
+    bm_ready = false;
+    if (n == 0) {
+      bm_ready = true;
+      return;
+    }
+    old_config_forces(dt, false);
+    ab.alloc(3 * n);
+    hipchk(hipMemcpyAsync(ab.p, a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    restore_after_old_config();
+    bm_ready = true;
     sync();
   }
 
@@ -1815,6 +1896,22 @@ int lpmdx_leapfrog(lpmdx_system* S, double dt, long steps, long obs_every, doubl
     if (steps < 0 || obs_every < 0)
       fail(LPMDX_EINVAL, "negative step count");
     S->lf(dt, steps, obs_every, epot_t, ekin_t);
+  });
+}
+
+int lpmdx_beeman_init(lpmdx_system* S, double dt) {
+  return guard([&] {
+    need(S, "system");
+    S->bm_init(dt);
+  });
+}
+
+int lpmdx_beeman(lpmdx_system* S, double dt, long steps, long obs_every, double* epot_t, double* ekin_t) {
+  return guard([&] {
+    need(S, "system");
+    if (steps < 0 || obs_every < 0)
+      fail(LPMDX_EINVAL, "negative step count");
+    S->vv(dt, steps, obs_every, epot_t, ekin_t, true);
   });
 }
 

@@ -259,8 +259,16 @@ enum StepMode {
   kForcesOnly = 0, // a = F/m (or a += with accumulate)
   kInitDrift = 1,  // a = F/m, then next positions xn = drift(x, v, a)
   kStep = 2,       // a = F/m, v += dt/2 (a_old + a), xn = drift(x, v, a)
-  kLeapfrog = 3    // a = F/m, vh' = vh + a dt, xn = FittedInside(x + vh' dt), v = (vh' + vh) / 2, vh = vh'
+  kLeapfrog = 3,   // a = F/m, vh' = vh + a dt, xn = FittedInside(x + vh' dt), v = (vh' + vh) / 2, vh = vh'
+  kBeemanInit = 4, // a = F/m, then xn = Beeman position update with a_old = ab
+  kBeemanStep = 5  // a = F/m, Beeman velocity update from (a, old a, ab), ab = old a, xn = position update
 };
+
+// Beeman's position update before FittedInside (beeman.cc:63-76), in its order
+// of operations: x + v dt + (2/3) a dt^2 - (1/6) a_old dt^2.
+__device__ __forceinline__ double beeman_position(double x, double v, double a, double a_old, double dt) {
+  return ((x + v * dt) + ((2.0 / 3.0) * a) * dt * dt) - ((1.0 / 6.0) * a_old) * dt * dt;
+}
 
 struct ForceArgs {
   const double* x;   // [3n] current positions
@@ -279,6 +287,7 @@ struct ForceArgs {
   unsigned int* maxd2;  // max over atoms of |x_next - xb|^2 (float bits), or null
   const double* xb;     // [3n] positions at the last list build
   double* vh;           // [3n] leapfrog half-step velocities (kLeapfrog)
+  double* ab;           // [3n] Beeman: the acceleration of the step before a's (kBeeman*)
   // Largest separation a listed pair can have now (list radius + padding +
   // twice the displacement allowed since the build), or 0.  An atom at least
   // this far from every face has no listed neighbour across the boundary.
@@ -351,6 +360,41 @@ __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, d
       v[q] = 0.5 * (vn[q] + vo[q]);
       vh[q] = vn[q];
       a[q] = am[q];
+    }
+    d2 = build_displacement2(A, i, p[0], p[1], p[2]);
+    double* xn = A.xn + 3 * i;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      xn[q] = fitted_inside(p[q], A.ll[q]);
+    return;
+  }
+  if (A.mode == kBeemanInit || A.mode == kBeemanStep) {
+    // TwoStepIntegrator with Beeman (beeman.cc:63-84).  kBeemanStep ends the
+    // step whose positions are x (AdvanceVelocity with the new a, the old a
+    // and ab), shifts the history (ab = old a) and begins the next step
+    // (AdvancePosition); kBeemanInit only begins one, with a_old = ab.
+    double* ab = A.ab + 3 * i;
+    const double dt = A.dt;
+    const bool step = A.mode == kBeemanStep;
+    const double an[3] = {ax, ay, az};
+    const double ao[3] = {a[0], a[1], a[2]};
+    const double ap[3] = {ab[0], ab[1], ab[2]};
+    const double vo[3] = {v[0], v[1], v[2]};
+    const double xo[3] = {x[0], x[1], x[2]};
+    double vn[3], p[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      vn[q] = step ? ((vo[q] + ((1.0 / 3.0) * an[q]) * dt) + ((5.0 / 6.0) * ao[q]) * dt) - ((1.0 / 6.0) * ap[q]) * dt
+                   : vo[q];
+      p[q] = beeman_position(xo[q], vn[q], an[q], step ? ao[q] : ap[q], dt);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (step) {
+        v[q] = vn[q];
+        ab[q] = ao[q];
+      }
+      a[q] = an[q];
     }
     d2 = build_displacement2(A, i, p[0], p[1], p[2]);
     double* xn = A.xn + 3 * i;
@@ -988,7 +1032,8 @@ static __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const 
   flush_maxd2(A, d2);
 }
 
-// Drift only: xn = FittedInside((x + v dt) + a/2 dt^2) (velocityverlet.cc:60-67).
+// Drift only: xn = FittedInside((x + v dt) + a/2 dt^2) (velocityverlet.cc:60-67),
+// or with mode kBeemanInit Beeman's position update with a_old = ab.
 static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   float d2 = 0.0f;
@@ -996,7 +1041,10 @@ static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
     const double dt = A.dt;
     double p[3];
     for (int q = 0; q < 3; ++q) {
-      p[q] = (A.x[3 * i + q] + A.v[3 * i + q] * dt) + A.a[3 * i + q] * 0.5 * dt * dt;
+      if (A.mode == kBeemanInit)
+        p[q] = beeman_position(A.x[3 * i + q], A.v[3 * i + q], A.a[3 * i + q], A.ab[3 * i + q], dt);
+      else
+        p[q] = (A.x[3 * i + q] + A.v[3 * i + q] * dt) + A.a[3 * i + q] * 0.5 * dt * dt;
       A.xn[3 * i + q] = fitted_inside(p[q], A.ll[q]);
     }
     d2 = build_displacement2(A, i, p[0], p[1], p[2]);

@@ -55,6 +55,7 @@ const std::map<std::string, std::vector<Keyword>>& Keywords() {
       {"lcbinary", {{"cutoff", "7.0"}, {"nx", "0"}, {"ny", "0"}, {"nz", "0"}, {"mode", "noauto"}}},
       {"velocityverlet", {{"dt", "1.0"}, {"start", "1"}}},
       {"leapfrog", {{"dt", "1.0"}, {"start", "1"}}},
+      {"beeman", {{"dt", "1.0"}, {"start", "1"}}},
   };
   return m;
 }
@@ -541,6 +542,111 @@ void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "leapfrog");
 }
 
+DeviceBeeman::DeviceBeeman(std::string args) : lpmd::Plugin("beeman", kVersion) {
+  Define(*this, "beeman");
+  ProcessArguments(args);
+  dt = double((*this)["dt"]);
+  start_ = int((*this)["start"]);
+}
+
+void DeviceBeeman::ShowHelp() const {
+  std::cout << " Module Name        = beeman (MI355X engine)\n"
+            << "      Beeman with dt (fs); on the GPU when every potential and the cell\n"
+            << "      manager are MI355X plugins.\n";
+}
+
+void DeviceBeeman::Initialize(lpmd::Simulation& sim, lpmd::Potential& p) {
+  if (mode_ != kUnset) // UseOldConfig sets the old configuration up once
+    return;
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  if (cm) {
+    lpmdx_system* sys = cm->Bind(sim, false);
+    cm->UsePotentials(dev);
+    std::vector<double> x, v, a(3 * n);
+    Positions(sim, x);
+    Velocities(sim, v);
+    for (long i = 0; i < n; ++i)
+      for (int q = 0; q < 3; ++q)
+        a[3 * i + q] = atoms[i].Acceleration()[q];
+    check(lpmdx_system_set_state(sys, x.data(), v.data(), a.data()), "beeman");
+    check(lpmdx_beeman_init(sys, dt), "beeman");
+    hx_.swap(x);
+    hv_.swap(v);
+    sys_ = sys;
+    mode_ = kDevice;
+    return;
+  }
+  // Beeman::Initialize (beeman.cc:55-61) on the host: forces at the old
+  // configuration of Integrator::GoBack (integrator.cc:31-42), set in place
+  // and then undone, give the previous accelerations
+  lpmd::BasicCell& cell = sim.Cell();
+  std::vector<lpmd::Vector> keep_x(n), keep_a(n);
+  for (long i = 0; i < n; ++i) {
+    keep_x[i] = atoms[i].Position();
+    keep_a[i] = atoms[i].Acceleration();
+    atoms[i].Position() = cell.FittedInside(atoms[i].Position() - atoms[i].Velocity() * dt);
+  }
+  p.UpdateForces(sim);
+  a_old_.resize(n);
+  for (long i = 0; i < n; ++i) {
+    a_old_[i] = atoms[i].Acceleration();
+    atoms[i].Position() = keep_x[i];
+    atoms[i].Acceleration() = keep_a[i];
+  }
+  mode_ = kHost;
+}
+
+void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
+  if (mode_ == kUnset)
+    throw lpmd::RuntimeError("beeman (MI355X): Advance before Initialize");
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
+  if ((cm != nullptr) != (mode_ == kDevice))
+    throw lpmd::RuntimeError("beeman (MI355X): the potentials or the cell manager changed between Initialize and "
+                             "Advance");
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  if (!cm) {
+    // TwoStepIntegrator::Advance with Beeman's AdvancePosition and
+    // AdvanceVelocity (twostepintegrator.cc:12-68, beeman.cc:63-84)
+    lpmd::BasicCell& cell = sim.Cell();
+    p.UpdateForces(sim);
+    std::vector<lpmd::Vector> aux(n);
+    for (long i = 0; i < n; ++i) {
+      aux[i] = a_old_[i];
+      const lpmd::Vector np = atoms[i].Position() + atoms[i].Velocity() * dt +
+                              (2.0 / 3.0) * atoms[i].Acceleration() * dt * dt -
+                              (1.0 / 6.0) * a_old_[i] * dt * dt;
+      a_old_[i] = atoms[i].Acceleration();
+      atoms[i].Position() = cell.FittedInside(np);
+    }
+    p.UpdateForces(sim);
+    for (long i = 0; i < n; ++i)
+      atoms[i].Velocity() = atoms[i].Velocity() + (1.0 / 3.0) * atoms[i].Acceleration() * dt +
+                            (5.0 / 6.0) * a_old_[i] * dt - (1.0 / 6.0) * aux[i] * dt;
+    return;
+  }
+  lpmdx_system* sys = cm->Bind(sim, false);
+  if (sys != sys_)
+    throw lpmd::RuntimeError("beeman (MI355X): the cell manager's system changed since Initialize");
+  cm->UsePotentials(dev);
+  std::vector<double> x, v;
+  Positions(sim, x);
+  Velocities(sim, v);
+  if (x != hx_ || v != hv_) {
+    // the host changed the state: send it (the previous accelerations stay)
+    check(lpmdx_system_set_state(sys, x.data(), v.data(), nullptr), "beeman");
+    hx_.swap(x);
+    hv_.swap(v);
+  }
+  double ep = 0.0, ek = 0.0;
+  check(lpmdx_beeman(sys, dt, 1, 1, &ep, &ek), "beeman");
+  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "beeman");
+}
+
 // ------------------------------------------------------------------ factory
 
 lpmd::Plugin* Create(const std::string& name, const std::string& args) {
@@ -556,6 +662,8 @@ lpmd::Plugin* Create(const std::string& name, const std::string& args) {
     return new DeviceVelocityVerlet(args);
   if (name == "leapfrog")
     return new DeviceLeapfrog(args);
+  if (name == "beeman")
+    return new DeviceBeeman(args);
   throw lpmd::RuntimeError("no MI355X plugin named " + name);
 }
 

@@ -14,6 +14,7 @@
 //   DeviceVelocityVerlet  lpmd::Integrator (liblpmd/lpmd/integrator.h:15-40) as
 //                         "velocityverlet" (plugins/velocityverlet.cc:13-20)
 //   DeviceLeapfrog        lpmd::Integrator as "leapfrog" (plugins/leapfrog.cc:14-23)
+//   DeviceBeeman          lpmd::Integrator as "beeman" (plugins/beeman.cc)
 // They take the same plugin names and keyword strings, so a directory holding
 // their <name>.so files placed first on LPMD_PATH replaces the CPU plugins.
 // The classes live in one shared library (liblpmdx_lpmd.so) so that every
@@ -145,6 +146,28 @@ private:
   Mode mode_ = kUnset;
   long start_ = 1;
   std::vector<lpmd::Vector> vh_; // host path: the old configuration's velocities
+  std::vector<double> hx_, hv_, ha_;
+  lpmdx_system* sys_ = nullptr;
+};
+
+class DeviceBeeman : public lpmd::Integrator, public lpmd::Plugin {
+public:
+  explicit DeviceBeeman(std::string args);
+  void ShowHelp() const override;
+  // Beeman::Initialize (beeman.cc:55-61): the previous accelerations from the
+  // forces at the old configuration, once per integrator object; on the
+  // device (lpmdx_beeman_init) under the same condition as
+  // DeviceVelocityVerlet, the history then staying in HBM.
+  void Initialize(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  // One Beeman step (twostepintegrator.cc:12-68, beeman.cc:63-84), on the
+  // device or, when Initialize ran on the host, on the host.
+  void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+
+private:
+  enum Mode { kUnset, kDevice, kHost };
+  Mode mode_ = kUnset;
+  long start_ = 1;
+  std::vector<lpmd::Vector> a_old_; // host path: the old configuration's accelerations
   std::vector<double> hx_, hv_, ha_;
   lpmdx_system* sys_ = nullptr;
 };

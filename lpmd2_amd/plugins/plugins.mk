@@ -15,7 +15,7 @@ CXX      ?= g++
 CXXFLAGS := -O2 -std=c++17 -fPIC -w -I$(REF)/liblpmd -Iinclude
 NAMES    := lennardjones fastlj morse buckingham suttonchen finnissinclair gupta \
             harmonic lennardjonesMod finnissinclair-ext tabulatedpair \
-            linkedcell verletlist minimumimage lcbinary velocityverlet leapfrog
+            linkedcell verletlist minimumimage lcbinary velocityverlet leapfrog beeman
 RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..' -Wl,-rpath,'$$ORIGIN/../../../oracle/_ref'
 
 all: $(patsubst %,$(OUT)/%.so,$(NAMES))

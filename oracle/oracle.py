@@ -390,6 +390,34 @@ def leapfrog_run(conf: Conf, forces, dt: float, steps: int):
     return ep, ek
 
 
+def beeman_run(conf: Conf, forces, dt: float, steps: int):
+    """Beeman::Initialize then `steps` TwoStepIntegrator::Advance calls
+    (plugins/beeman.cc:55-84, liblpmd/simulation/twostepintegrator.cc:12-68).
+    forces(c) fills c.a for configuration c and returns its potential energy.
+    Set-up: forces at the old configuration FittedInside(x - v dt)
+    (Integrator::GoBack, integrator.cc:31-42) give a_old.  Each step: forces
+    at x (a_n); aux = a_old; x = FittedInside(x + v dt + (2/3) a_n dt^2
+    - (1/6) a_old dt^2); a_old = a_n; forces at the new x (a_n+1);
+    v += (1/3) a_n+1 dt + (5/6) a_old dt - (1/6) aux dt.  Returns the per-step
+    potential energy and kinetic energy at the end of each step."""
+    L = np.asarray(conf.L, dtype=np.float64)
+    old = Conf(fitted_inside_all(conf.x - conf.v * dt, L), L, z=conf.z, mass=conf.m, v=conf.v - conf.a * dt)
+    forces(old)
+    a_old = old.a.copy()
+    ep = np.zeros(steps)
+    ek = np.zeros(steps)
+    for s in range(steps):
+        forces(conf)
+        aux = a_old
+        xn = conf.x + conf.v * dt + (2.0 / 3.0) * conf.a * dt * dt - (1.0 / 6.0) * a_old * dt * dt
+        a_old = conf.a.copy()
+        conf.x[:] = fitted_inside_all(xn, L)
+        ep[s] = forces(conf)
+        conf.v[:] = conf.v + (1.0 / 3.0) * conf.a * dt + (5.0 / 6.0) * a_old * dt - (1.0 / 6.0) * aux * dt
+        ek[s] = kinetic_energy(conf)
+    return ep, ek
+
+
 def vv_position(conf: Conf, dt: float):
     aold = np.zeros((conf.n, 3))
     lib().orc_vv_position(conf.struct(), dt, _p(aold))

@@ -27,7 +27,7 @@ CORE_SRCS := \
 
 PLUGINS := linkedcell verletlist minimumimage lcbinary lennardjones fastlj morse buckingham tabulatedpair \
            suttonchen finnissinclair gupta harmonic lennardjonesMod finnissinclair-ext \
-           crystal3d velocityverlet leapfrog
+           crystal3d velocityverlet leapfrog beeman
 
 CORE_OBJS := $(patsubst %.cc,$(OUT)/obj/%.o,$(CORE_SRCS))
 PLUGIN_SOS := $(patsubst %,$(OUT)/%.so,$(PLUGINS))

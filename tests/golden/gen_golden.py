@@ -210,6 +210,23 @@ def main():
     np.savez_compressed(os.path.join(HERE, "lf_ar108.npz"), x0=x, v0=v, L=L, acc=acc, xf=xf, vf=vf, af=af)
     meta["lf"] = {"ar108": {"dt": 2.0, "steps": steps}}
 
+    # ---- beeman with replayed accelerations (plugins/beeman.cc) ----------------
+    # Initialize's force call at the old configuration, then two per Advance
+    x, L = fcc(3, 5.26)
+    rng = np.random.default_rng(34)
+    n = x.shape[0]
+    v = rng.normal(0, 2e-3, (n, 3))
+    steps = 12
+    acc = rng.normal(0, 1e-3, (2 * steps + 1, n, 3))
+    v[0] = [0.5, -0.7, 40.0]  # across the box edge: FittedInside, also in GoBack
+    rc = R.RefConf(x, L)
+    rc.set_velocities(v)
+    cm = R.RefCellManager("linkedcell", "cutoff 8.5 nx 4 ny 4 nz 4")
+    R.vv_run("dt 2.0", rc, cm, steps, acc.reshape(2 * steps + 1, -1), integrator="beeman")
+    xf, vf, af = rc.get()
+    np.savez_compressed(os.path.join(HERE, "bm_ar108.npz"), x0=x, v0=v, L=L, acc=acc, xf=xf, vf=vf, af=af)
+    meta["beeman"] = {"ar108": {"dt": 2.0, "steps": steps}}
+
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", sorted(os.listdir(HERE)))

@@ -53,23 +53,26 @@ def test_potential_plugins_match_reference_keywords(name, args):
                                        ("minimumimage", "cutoff 8.5"),
                                        ("lcbinary", "cutoff 8.5 mode auto"),
                                        ("velocityverlet", "dt 2.0"),
-                                       ("leapfrog", "dt 2.0")])
+                                       ("leapfrog", "dt 2.0"),
+                                       ("beeman", "dt 2.0")])
 def test_other_plugins_load(name, args):
     p = R.RefPlugin.dropin(name, args)
     p.close()
 
 
-def test_leapfrog_host_path_matches_reference_golden():
-    """With a CPU cell manager the drop-in leapfrog runs the reference's
+@pytest.mark.parametrize("name,fixture,calls_per_step", [("leapfrog", "lf_ar108.npz", 1),
+                                                         ("beeman", "bm_ar108.npz", 2)])
+def test_integrator_host_path_matches_reference_golden(name, fixture, calls_per_step):
+    """With a CPU cell manager the drop-in integrator runs the reference's
     scheme on the host: replayed forces give the reference plugin's
     trajectory bit for bit (no GPU is touched)."""
-    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lf_ar108.npz"))
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", fixture))
     acc = d["acc"]
     conf = R.RefConf(d["x0"], d["L"])
     conf.set_velocities(d["v0"])
     cm = R.RefCellManager("linkedcell", "cutoff 8.5 nx 4 ny 4 nz 4")
-    R.vv_run("dt 2.0", conf, cm, acc.shape[0] - 1, acc.reshape(acc.shape[0], -1),
-             path=os.path.join(R.DROPIN_DIR, "leapfrog.so"))
+    R.vv_run("dt 2.0", conf, cm, (acc.shape[0] - 1) // calls_per_step, acc.reshape(acc.shape[0], -1),
+             path=os.path.join(R.DROPIN_DIR, name + ".so"))
     xf, vf, af = conf.get()
     assert np.array_equal(xf, d["xf"])
     assert np.array_equal(vf, d["vf"])

@@ -125,6 +125,9 @@ def native_run(x, L, z, v, cm, cm_args, pot, dt, steps, integrator="velocityverl
     if integrator == "leapfrog":
         s.leapfrog_init(dt)
         s.leapfrog(dt, steps)
+    elif integrator == "beeman":
+        s.beeman_init(dt)
+        s.beeman(dt, steps)
     else:
         s.velocityverlet(dt, steps)
     return s.get_state()
@@ -161,12 +164,13 @@ def test_dropin_integration_follows_native_path(integrator):
         assert np.all(np.isfinite(ep)) and ep[-1] != 0.0
 
 
+@pytest.mark.parametrize("scheme", ["leapfrog", "beeman"])
 @pytest.mark.parametrize("integrator", ["reference", "dropin"])
-def test_dropin_leapfrog_follows_native_path(integrator):
-    """The reference leapfrog plugin over drop-in forces (its set-up force
-    call on the old configuration included) and the drop-in leapfrog both
-    follow the engine's own leapfrog."""
-    say("drop-in leapfrog", integrator)
+def test_dropin_oldconfig_integrators_follow_native_path(integrator, scheme):
+    """The reference leapfrog / beeman plugin over drop-in forces (its set-up
+    force call on the old configuration included) and the drop-in plugin of
+    the same name both follow the engine's own scheme."""
+    say("drop-in", scheme, integrator)
     x, L = E.fcc(5, 3.61)
     x = np.mod(x + np.random.default_rng(9).normal(0.0, 0.02, x.shape), L)
     m = np.full(x.shape[0], E.ELEMENT_MASS[29])
@@ -180,10 +184,10 @@ def test_dropin_leapfrog_follows_native_path(integrator):
     cm.attach(conf)
     pots = [R.RefPlugin.dropin(*pot)]
     make = R.RefPlugin.reference if integrator == "reference" else R.RefPlugin.dropin
-    itg = make("leapfrog", f"dt {dt}")
+    itg = make(scheme, f"dt {dt}")
     ep = R.integrate(itg, conf, pots, steps)
     xr, vr, ar = conf.get()
-    xn, vn, an = native_run(x, L, 29, v, "verletlist", cm_args, pot, dt, steps, integrator="leapfrog")
+    xn, vn, an = native_run(x, L, 29, v, "verletlist", cm_args, pot, dt, steps, integrator=scheme)
     dx = np.abs(xr - xn).max()
     dv = np.abs(vr - vn).max()
     say(f"  max |dx| {dx:.3e}  |dv| {dv:.3e}")

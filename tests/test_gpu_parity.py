@@ -283,6 +283,120 @@ def test_leapfrog_errors():
     s.leapfrog(2.0, 3)
 
 
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
+def test_beeman_trajectory_matches_oracle(cm):
+    """Beeman set-up (forces at the old configuration, one atom crossing the
+    box edge there) and 20 + 10 steps of LJ argon at 100 K in two calls:
+    positions within the FP32 force noise the trajectory accumulates (1e-7),
+    sampled energies to 1e-6 relative."""
+    say("beeman", cm)
+    x, L = E.fcc(4, 5.26)
+    m = np.full(x.shape[0], 39.948)
+    v = E.maxwell_boltzmann(m, 100.0, seed=10)
+    x[0] = [0.002, 0.003, 0.004]
+    v[0] = [0.003, 0.004, 0.005]
+
+    def cms():
+        if cm == "linkedcell":
+            return "cutoff 8.5 mode auto", O.CellManager("linkedcell", 8.5, mode="auto")
+        return "cutoff 8.5 maxneighbors 200 each 5 skin 1.0", O.CellManager("verletlist", 8.5, maxneighbors=200, each=5)
+
+    args, om = cms()
+    s = E.System(x, L, 18, v=v)
+    s.set_cellmanager(cm, args)
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    s.beeman_init(1.0)
+    ep, ek = s.beeman(1.0, 20, obs_every=5)
+    xg, vg, ag = s.get_state()
+    pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
+    conf = O.Conf(x, L, v=v)
+    e0, k0 = O.beeman_run(conf, lambda c: O.update_forces(c, om, pots)[0], 1.0, 20)
+    dx = xg - conf.x
+    dx -= L * np.round(dx / L)
+    say(f"  max |dx| {np.abs(dx).max():.2e} |dv| {np.abs(vg - conf.v).max():.2e}  E {ep} vs {e0[4::5]}")
+    assert np.abs(dx).max() < 1e-7
+    assert np.abs(vg - conf.v).max() < 2e-8
+    assert rel_rms(ag, conf.a) <= FORCE_TOL
+    np.testing.assert_allclose(ep, e0[4::5], rtol=1e-6)
+    np.testing.assert_allclose(ek, k0[4::5], rtol=1e-6)
+    ep2, ek2 = s.beeman(1.0, 10, obs_every=10)
+    _, om30 = cms()
+    e30, k30 = O.beeman_run(O.Conf(x, L, v=v), lambda c: O.update_forces(c, om30, pots)[0], 1.0, 30)
+    np.testing.assert_allclose(ep2, e30[29:], rtol=1e-6)
+    np.testing.assert_allclose(ek2, k30[29:], rtol=1e-6)
+
+
+def test_beeman_free_flight_is_bit_exact():
+    """Zero forces: the set-up and every step's Beeman position and velocity
+    arithmetic, bit for bit."""
+    say("beeman free flight")
+    x, L = jittered(3, 5.26, 0.3, 7)
+    v = np.random.default_rng(4).normal(0, 0.05, x.shape)
+    s = E.System(x, L, 18, v=v)
+    s.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 300 each 5 skin 1.0")
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0 cutoff 8.5")
+    s.beeman_init(2.0)
+    s.beeman(2.0, 37)
+    xg, vg, _ = s.get_state()
+    conf = O.Conf(x, L, v=v)
+    zero = np.zeros_like(x)
+
+    def forces(c):
+        c.a[:] = zero
+        return 0.0
+
+    O.beeman_run(conf, forces, 2.0, 37)
+    assert np.array_equal(xg, conf.x)
+    assert np.array_equal(vg, conf.v)
+
+
+def test_beeman_errors_and_scheme_switch():
+    """Beeman without its set-up or with another dt is an error; switching
+    between velocityverlet and beeman on one system continues each scheme
+    from the state the other left (the look-ahead positions are taken back)."""
+    say("beeman errors / switch")
+    x, L = jittered(3, 5.26, 0.05, 3)
+    s = E.System(x, L, 18)
+    s.set_cellmanager("linkedcell", "cutoff 8.5 mode auto")
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    with pytest.raises(E.LpmdxError) as e:
+        s.beeman(2.0, 3)
+    assert e.value.code == E.ESTATE
+    s.beeman_init(2.0)
+    with pytest.raises(E.LpmdxError) as e:
+        s.beeman(1.0, 3)
+    assert e.value.code == E.EINVAL
+    # 5 velocity-Verlet steps, then Beeman from there, against the oracle
+    s.velocityverlet(2.0, 5)
+    x5, v5, a5 = s.get_state()
+    s.beeman(2.0, 5)
+    xg, vg, _ = s.get_state()
+    om = O.CellManager("linkedcell", 8.5, mode="auto")
+    pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
+    conf = O.Conf(x, L)
+    O.md_run(conf, om, pots, 2.0, 5)
+    # the Beeman history set at beeman_init (forces at x - v dt with v = 0: x itself)
+    hist = O.Conf(x, L)
+    O.update_forces(hist, om, pots)
+    calls = iter([hist.a] + [None] * 10)
+
+    def forces(c):
+        h = next(calls)
+        if h is not None:
+            c.a[:] = h
+            return 0.0
+        return O.update_forces(c, om, pots)[0]
+
+    b = O.Conf(conf.x, L, v=conf.v)
+    O.beeman_run(b, forces, 2.0, 5)
+    dx = xg - b.x
+    dx -= L * np.round(dx / L)
+    say(f"  after the switch max |dx| {np.abs(dx).max():.2e}")
+    # FP32 force noise over 10 steps; taking velocity Verlet's look-ahead
+    # instead of Beeman's would be off by (a_n - a_n-1) dt^2 / 6 ~ 1e-5
+    assert np.abs(dx).max() < 5e-8
+
+
 def test_nve_energy_matches_oracle_suttonchen():
     """400 steps of Sutton-Chen Cu at 300 K, dt 2 fs, skin-triggered list:
     the sampled total energy follows the FP64 oracle (which runs the

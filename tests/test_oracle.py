@@ -202,6 +202,34 @@ def test_leapfrog_constant_acceleration_closed_form():
     np.testing.assert_allclose(conf.v, v0 + a0 * (t - dt), rtol=1e-12)
 
 
+def test_beeman_matches_reference():
+    """Beeman::Initialize + 12 TwoStepIntegrator::Advance calls of the
+    reference plugin with replayed accelerations (two per step)."""
+    d = np.load(os.path.join(GOLDEN, "bm_ar108.npz"))
+    conf = O.Conf(d["x0"], d["L"], v=d["v0"])
+    acc = d["acc"]
+    O.beeman_run(conf, _replay(acc), 2.0, (acc.shape[0] - 1) // 2)
+    assert np.array_equal(conf.x, d["xf"])
+    assert np.array_equal(conf.v, d["vf"])
+    assert np.array_equal(conf.a, d["af"])
+
+
+def test_beeman_constant_acceleration_closed_form():
+    """Constant a: Beeman's weights sum to the exact kinematics, positions
+    x0 + v0 t + a t^2 / 2 and velocities v0 + a t."""
+    L = [1e6] * 3
+    x0 = np.array([[10.0, 20.0, 30.0]])
+    v0 = np.array([[0.01, -0.02, 0.005]])
+    a0 = np.array([[1e-4, 2e-4, -3e-4]])
+    conf = O.Conf(x0, L, v=v0)
+    conf.a[:] = a0
+    dt, steps = 0.5, 40
+    O.beeman_run(conf, _replay([a0] * (2 * steps + 1)), dt, steps)
+    t = dt * steps
+    np.testing.assert_allclose(conf.x, x0 + v0 * t + 0.5 * a0 * t * t, rtol=1e-12)
+    np.testing.assert_allclose(conf.v, v0 + a0 * t, rtol=1e-12)
+
+
 def test_velocityverlet_exact_for_constant_acceleration():
     L = [1e6] * 3
     x0 = np.array([[10.0, 20.0, 30.0]])
