@@ -174,6 +174,24 @@ LPMDX_API int lpmdx_beeman_init(lpmdx_system* sys, double dt);
 LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
                            double* ekin_t);
 
+/* SimulationEngine::SetTemperature (liblpmd/simulation/simulationbuilder.cc:53-71)
+ * on the device: every velocity times sqrt(temp / T_now), T_now from the
+ * kinetic energy (properties.h:22-53) summed on the device (FP64, in a fixed
+ * order that differs from the reference's sequential sum, so the factor
+ * agrees to FP64 rounding).  A system at rest is left unchanged. */
+LPMDX_API int lpmdx_set_temperature(lpmdx_system* sys, double temp);
+
+/* The tempscaling modifier (plugins/tempscaling.cc:14-63) as the lpmd
+ * application runs it (lpmd/lpmd.cc:89-93, application.cc:404-459): after
+ * every integrator step whose step number k (steps counted since the system
+ * was created, the simulation's CurrentStep) is active for
+ * Stepper(start, end, each) (stepper.h:17-25), the velocities are scaled to
+ * the temperature ValueAtStep(k, from, to) (stepper.h:27-31), on the device
+ * inside lpmdx_velocityverlet / lpmdx_beeman / lpmdx_leapfrog.  end == -1
+ * means no end and is allowed only for from == to (the application replaces
+ * it by the run's length); each == 0 switches the modifier off. */
+LPMDX_API int lpmdx_tempscaling(lpmdx_system* sys, double from, double to, long start, long end, long each);
+
 /* Device-side SetSimulationTags (simulationbuilder.cc:84-124, properties.h:22-81):
  * out[LPMDX_NOBS], see LPMDX_OBS_*.  The potential terms are those of the
  * last force evaluation. */

@@ -64,6 +64,8 @@ def lib():
         L.lpmdx_leapfrog.argtypes = [vp, d, l, l, vp, vp]
         L.lpmdx_beeman_init.argtypes = [vp, d]
         L.lpmdx_beeman.argtypes = [vp, d, l, l, vp, vp]
+        L.lpmdx_set_temperature.argtypes = [vp, d]
+        L.lpmdx_tempscaling.argtypes = [vp, d, d, l, l, l]
         L.lpmdx_observables.argtypes = [vp, vp]
         L.lpmdx_system_stats.argtypes = [vp, vp]
         L.lpmdx_system_set_lanes.argtypes = [vp, i]
@@ -181,6 +183,14 @@ class System:
         ek = np.zeros(max(k, 1))
         _check(lib().lpmdx_beeman(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
         return ep[:k], ek[:k]
+
+    def set_temperature(self, temp: float):
+        """SetTemperature: scale every velocity to the temperature `temp` (K)."""
+        _check(lib().lpmdx_set_temperature(self._h, temp))
+
+    def tempscaling(self, from_: float, to: float, start: int = 0, end: int = -1, each: int = 1):
+        """The tempscaling modifier after every step active for (start, end, each); each=0 switches it off."""
+        _check(lib().lpmdx_tempscaling(self._h, from_, to, start, end, each))
 
     def neighbor_lists(self, full: bool, rcut: float):
         counts = np.zeros(self.n, dtype=np.int64)

@@ -102,6 +102,25 @@ __global__ __launch_bounds__(kBlock) void k_lf_halfkick(double* __restrict__ vh,
     vh[3 * i + q] += (a[3 * i + q] * 0.5) * dt;
 }
 
+// SimulationEngine::SetTemperature (liblpmd/simulation/simulationbuilder.cc:53-71)
+// with the kinetic sum already reduced on the device (red[0] = sum of
+// 0.5 m v^2): T_now = (2/3) K / (kB N), K = KIN2EV * red[0]
+// (properties.h:22-53), then v *= sqrt(T / T_now).  A system at rest is left
+// as it is (the reference would divide by zero).
+__global__ __launch_bounds__(kBlock) void k_set_temperature(double* __restrict__ v, const double* __restrict__ red,
+                                                            long n, double temp) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const double ti = (2.0 / 3.0) * (red[0] * kKin2eV) / (kBoltzmann * double(n));
+  if (!(ti > 0.0))
+    return;
+  const double f = sqrt(temp / ti);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    v[3 * i + q] *= f;
+}
+
 __global__ void k_fill_double(double* p, long m, double v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m)
@@ -219,6 +238,28 @@ struct lpmdx_system {
   double last_e[12] = {0};
   bool last_e_valid = false;
   long n_builds = 0, n_forces = 0, n_steps = 0, max_list = 0, sum_list = 0;
+  // the tempscaling modifier run after every step it is active in
+  // (lpmd/application.cc:404-459, plugins/tempscaling.cc:58-63); steps are
+  // counted by n_steps, the simulation's CurrentStep
+  struct TempSchedule {
+    bool on = false;
+    double from = 0.0, to = 0.0;
+    long start = 0, end = -1, each = 1;
+    // Stepper::IsActiveInStep (liblpmd/lpmd/stepper.h:17-25)
+    bool active(long k) const {
+      if (!on || k < start)
+        return false;
+      if (end != -1 && k > end)
+        return false;
+      return ((k - start) % each) == 0;
+    }
+    // Stepper::ValueAtStep (stepper.h:27-31)
+    double value(long k) const {
+      if (start == end)
+        return from;
+      return from + double(k - start) / double(end - start) * (to - from);
+    }
+  } tsched;
   // optional per-kernel timing with HIP events on this system's stream
   bool profiling = false;
   struct Pending {
@@ -1152,6 +1193,41 @@ struct lpmdx_system {
     return hred[0] * kKin2eV;
   }
 
+  // SetTemperature on the device, ordered on the stream (no host round trip);
+  // partials must hold 4 * blocks_for(n) doubles
+  void scale_to_temperature(double temp) {
+    const long nb = blocks_for(n);
+    k_kinetic<<<nb, kBlock, 0, st>>>(v.p, mass.p, n, partials.p);
+    k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
+    k_set_temperature<<<nb, kBlock, 0, st>>>(v.p, red.p, n, temp);
+    launch_check();
+  }
+
+  // After a step's kernel in the two-step loops: when the schedule is active
+  // in step k, scale v and make the look-ahead positions (in x; the step's
+  // positions are in xn) again from the scaled velocities.
+  void modifiers_after_step(long k, bool beeman, double dt, unsigned int* maxd2) {
+    if (!tsched.active(k))
+      return;
+    scale_to_temperature(tsched.value(k));
+    ForceArgs A{};
+    A.x = xn.p;
+    A.xn = x.p;
+    A.v = v.p;
+    A.a = a.p;
+    A.ab = ab.p;
+    A.maxd2 = maxd2; // the larger of the two look-aheads' displacements: conservative
+    A.xb = xb.p;
+    A.b = box;
+    A.n = n;
+    A.dt = dt;
+    A.mode = beeman ? kBeemanInit : kStep;
+    for (int q = 0; q < 3; ++q)
+      A.ll[q] = ll[q];
+    k_drift<<<blocks_for(n), kBlock, 0, st>>>(A);
+    launch_check();
+  }
+
   // ------------------------------------------------------------------ VV
   // beeman: the same two-step scheme with Beeman's updates (beeman.cc:63-84)
   // in place of velocity Verlet's; ab carries its extra history.
@@ -1174,6 +1250,8 @@ struct lpmdx_system {
       undrift();
     const int init_mode = beeman ? kBeemanInit : kInitDrift;
     const int step_mode = beeman ? kBeemanStep : kStep;
+    if (tsched.on)
+      partials.alloc(std::max<size_t>((size_t)blocks_for(n) * 4, partials.n));
     unsigned int* md = ctrl.p->maxd2;
     auto read_disp = [&](long s) {
       float f;
@@ -1266,6 +1344,7 @@ struct lpmdx_system {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
         forces(step_mode, obs, dt, md + s);
         std::swap(x.p, xn.p);
+        modifiers_after_step(n_steps + s + 1, beeman, dt, md + s);
       }
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       sync();
@@ -1435,6 +1514,8 @@ struct lpmdx_system {
     if (n == 0 || steps <= 0)
       return;
     undrift();
+    if (tsched.on)
+      partials.alloc(std::max<size_t>((size_t)blocks_for(n) * 4, partials.n));
     unsigned int* md = ctrl.p->maxd2;
     auto read_disp = [&](long s) {
       float f;
@@ -1491,6 +1572,10 @@ struct lpmdx_system {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
         forces(kLeapfrog, obs, dt, md + s);
         std::swap(x.p, xn.p);
+        // the next step reads the half-step velocities only, as in the
+        // reference, so the scaled v changes no trajectory
+        if (tsched.active(n_steps + s + 1))
+          scale_to_temperature(tsched.value(n_steps + s + 1));
       }
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       sync();
@@ -1912,6 +1997,42 @@ int lpmdx_beeman(lpmdx_system* S, double dt, long steps, long obs_every, double*
     if (steps < 0 || obs_every < 0)
       fail(LPMDX_EINVAL, "negative step count");
     S->vv(dt, steps, obs_every, epot_t, ekin_t, true);
+  });
+}
+
+int lpmdx_set_temperature(lpmdx_system* S, double temp) {
+  return guard([&] {
+    need(S, "system");
+    if (!std::isfinite(temp) || temp < 0.0)
+      fail(LPMDX_EINVAL, "temperature must be finite and non-negative");
+    if (S->n == 0)
+      return;
+    S->undrift(); // the look-ahead positions were made from the old velocities
+    S->partials.alloc(std::max<size_t>((size_t)blocks_for(S->n) * 4, S->partials.n));
+    S->scale_to_temperature(temp);
+    S->sync();
+  });
+}
+
+int lpmdx_tempscaling(lpmdx_system* S, double from, double to, long start, long end, long each) {
+  return guard([&] {
+    need(S, "system");
+    if (each == 0) {
+      S->tsched.on = false;
+      return;
+    }
+    if (!std::isfinite(from) || !std::isfinite(to) || from < 0.0 || to < 0.0)
+      fail(LPMDX_EINVAL, "tempscaling: temperatures must be finite and non-negative");
+    if (each < 0 || start < 0 || end < -1 || (end != -1 && end < start))
+      fail(LPMDX_EINVAL, "tempscaling: need each >= 1, start >= 0 and end == -1 or end >= start");
+    if (end == -1 && from != to)
+      fail(LPMDX_EINVAL, "tempscaling: a ramp (from != to) needs its last step as end");
+    S->tsched.on = true;
+    S->tsched.from = from;
+    S->tsched.to = to;
+    S->tsched.start = start;
+    S->tsched.end = end;
+    S->tsched.each = each;
   });
 }
 

@@ -363,7 +363,7 @@ def fitted_inside_all(x, L):
     return x
 
 
-def leapfrog_run(conf: Conf, forces, dt: float, steps: int):
+def leapfrog_run(conf: Conf, forces, dt: float, steps: int, after_step=None):
     """Leapfrog::Initialize then `steps` OneStepIntegrator::Advance calls
     (plugins/leapfrog.cc:58-77, liblpmd/simulation/onestepintegrator.cc:13-46).
     forces(c) fills c.a for configuration c and returns its potential energy
@@ -386,11 +386,13 @@ def leapfrog_run(conf: Conf, forces, dt: float, steps: int):
         conf.x[:] = fitted_inside_all(conf.x + vn * dt, L)
         conf.v[:] = 0.5 * (vn + vh)
         vh = vn
+        if after_step:
+            after_step(s + 1, conf)  # the modifiers of step s + 1 (lpmd/lpmd.cc:92-93)
         ek[s] = kinetic_energy(conf)
     return ep, ek
 
 
-def beeman_run(conf: Conf, forces, dt: float, steps: int):
+def beeman_run(conf: Conf, forces, dt: float, steps: int, after_step=None):
     """Beeman::Initialize then `steps` TwoStepIntegrator::Advance calls
     (plugins/beeman.cc:55-84, liblpmd/simulation/twostepintegrator.cc:12-68).
     forces(c) fills c.a for configuration c and returns its potential energy.
@@ -414,6 +416,8 @@ def beeman_run(conf: Conf, forces, dt: float, steps: int):
         conf.x[:] = fitted_inside_all(xn, L)
         ep[s] = forces(conf)
         conf.v[:] = conf.v + (1.0 / 3.0) * conf.a * dt + (5.0 / 6.0) * a_old * dt - (1.0 / 6.0) * aux * dt
+        if after_step:
+            after_step(s + 1, conf)
         ek[s] = kinetic_energy(conf)
     return ep, ek
 
@@ -453,6 +457,34 @@ def kinetic_energy(conf: Conf):
 
 def temperature(conf: Conf):
     return lib().orc_temperature(conf.struct())
+
+
+def set_temperature(conf: Conf, temp: float):
+    """SimulationEngine::SetTemperature (liblpmd/simulation/simulationbuilder.cc:53-71)
+    without fixed atoms: every velocity times sqrt(temp / T_now), T_now the
+    Temperature of the sequential kinetic sum (properties.h:22-53)."""
+    ti = temperature(conf)
+    conf.v *= np.sqrt(temp / ti)
+
+
+class Stepper:
+    """lpmd::Stepper (liblpmd/lpmd/stepper.h:14-35): when a modifier runs and
+    the value it ramps to."""
+
+    def __init__(self, start=0, end=-1, each=1):
+        self.start, self.end, self.each = start, end, each
+
+    def active(self, k):
+        if k < self.start:
+            return False
+        if self.end != -1 and k > self.end:
+            return False
+        return (k - self.start) % self.each == 0
+
+    def value(self, k, frm, to):
+        if self.start == self.end:
+            return frm
+        return frm + float(k - self.start) / float(self.end - self.start) * (to - frm)
 
 
 def momentum(conf: Conf):

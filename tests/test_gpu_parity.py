@@ -397,6 +397,85 @@ def test_beeman_errors_and_scheme_switch():
     assert np.abs(dx).max() < 5e-8
 
 
+def test_set_temperature_matches_oracle():
+    """SetTemperature on the device against the oracle's sequential sums: the
+    same factor to FP64 rounding, and the temperature read back is the target."""
+    say("set temperature")
+    x, L = jittered(5, 3.61, 0.02, 21)
+    m = np.full(x.shape[0], 63.546)
+    v = E.maxwell_boltzmann(m, 300.0, seed=21)
+    s = E.System(x, L, 29, v=v)
+    s.set_temperature(512.0)
+    _, vg, _ = s.get_state()
+    conf = O.Conf(x, L, z=29, v=v)
+    O.set_temperature(conf, 512.0)
+    np.testing.assert_allclose(vg, conf.v, rtol=1e-13, atol=0)
+    assert abs(O.temperature(O.Conf(x, L, z=29, v=vg)) - 512.0) <= 1e-10 * 512.0
+    # a system at rest stays at rest
+    s0 = E.System(x, L, 29)
+    s0.set_temperature(300.0)
+    assert np.all(s0.get_state()[1] == 0.0)
+
+
+@pytest.mark.parametrize("scheme", ["velocityverlet", "beeman", "leapfrog"])
+def test_tempscaling_schedule_matches_oracle(scheme):
+    """A temperature ramp 60 K -> 140 K applied every 4 steps over 40 steps of
+    LJ argon, inside the device step loops, against the oracle running the
+    same modifier after each active step (lpmd/lpmd.cc:89-93)."""
+    say("tempscaling", scheme)
+    x, L = jittered(4, 5.26, 0.05, 22)
+    m = np.full(x.shape[0], 39.948)
+    v = E.maxwell_boltzmann(m, 100.0, seed=22)
+    dt, steps = 2.0, 40
+    s = E.System(x, L, 18, v=v)
+    s.set_cellmanager("linkedcell", "cutoff 8.5 mode auto")
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    s.tempscaling(60.0, 140.0, start=0, end=steps, each=4)
+    st = O.Stepper(0, steps, 4)
+
+    def modifier(k, c):
+        if st.active(k):
+            O.set_temperature(c, st.value(k, 60.0, 140.0))
+
+    om = O.CellManager("linkedcell", 8.5, mode="auto")
+    pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
+    conf = O.Conf(x, L, v=v)
+    forces = lambda c: O.update_forces(c, om, pots)[0]
+    if scheme == "velocityverlet":
+        ep, ek = s.velocityverlet(dt, steps, obs_every=10)
+        e0 = np.zeros(steps)
+        k0 = np.zeros(steps)
+        for k in range(1, steps + 1):
+            forces(conf)
+            aold = O.vv_position(conf, dt)
+            e0[k - 1] = forces(conf)
+            O.vv_velocity(conf, dt, aold)
+            modifier(k, conf)
+            k0[k - 1] = O.kinetic_energy(conf)
+    elif scheme == "beeman":
+        s.beeman_init(dt)
+        ep, ek = s.beeman(dt, steps, obs_every=10)
+        e0, k0 = O.beeman_run(conf, forces, dt, steps, after_step=modifier)
+    else:
+        s.leapfrog_init(dt)
+        ep, ek = s.leapfrog(dt, steps, obs_every=10)
+        e0, k0 = O.leapfrog_run(conf, forces, dt, steps, after_step=modifier)
+    xg, vg, _ = s.get_state()
+    dx = xg - conf.x
+    dx -= L * np.round(dx / L)
+    say(f"  max |dx| {np.abs(dx).max():.2e} |dv| {np.abs(vg - conf.v).max():.2e}  Ek {ek} vs {k0[9::10]}")
+    assert np.abs(dx).max() < 1e-7
+    assert np.abs(vg - conf.v).max() < 2e-8
+    np.testing.assert_allclose(ep, e0[9::10], rtol=1e-6)
+    np.testing.assert_allclose(ek, k0[9::10], rtol=1e-6)
+    # at the last step the modifier set the ramp's end temperature (index 1:
+    # LPMDX_OBS_TEMP)
+    assert abs(s.observables()[1] - 140.0) <= 1e-9 * 140.0
+    with pytest.raises(E.LpmdxError) as e:
+        s.tempscaling(60.0, 140.0, start=0, end=-1, each=4)
+    assert e.value.code == E.EINVAL
+
+
 def test_nve_energy_matches_oracle_suttonchen():
     """400 steps of Sutton-Chen Cu at 300 K, dt 2 fs, skin-triggered list:
     the sampled total energy follows the FP64 oracle (which runs the

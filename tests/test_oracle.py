@@ -230,6 +230,33 @@ def test_beeman_constant_acceleration_closed_form():
     np.testing.assert_allclose(conf.v, v0 + a0 * t, rtol=1e-12)
 
 
+def test_stepper_known_answers():
+    """Stepper::IsActiveInStep / ValueAtStep (stepper.h:17-31) by hand."""
+    st = O.Stepper(10, 50, 20)
+    assert [k for k in range(0, 80) if st.active(k)] == [10, 30, 50]
+    assert [k for k in range(0, 12) if O.Stepper(0, -1, 4).active(k)] == [0, 4, 8]
+    ramp = O.Stepper(0, 100, 1)
+    assert ramp.value(0, 300.0, 600.0) == 300.0
+    assert ramp.value(50, 300.0, 600.0) == 450.0
+    assert ramp.value(100, 300.0, 600.0) == 600.0
+    assert O.Stepper(7, 7, 1).value(7, 300.0, 600.0) == 300.0
+
+
+def test_set_temperature_closed_form():
+    """Two argon atoms with |v|^2 = 1e-4 and 4e-4 A^2/fs^2: K = 0.5 m 5e-4 KIN2EV,
+    T = (2/3) K / (2 kB); after SetTemperature(T') every velocity is scaled by
+    sqrt(T' / T), so the directions stay and the temperature reads T'."""
+    v = np.array([[0.01, 0.0, 0.0], [0.0, 0.0, -0.02]])
+    conf = O.Conf(np.zeros((2, 3)) + [[1, 1, 1], [4, 4, 4]], [20.0] * 3, v=v)
+    m = 39.948
+    t0 = (2.0 / 3.0) * (0.5 * m * 5e-4 * O.KIN2EV) / (2 * O.KBOLTZMANN)
+    assert abs(O.temperature(conf) - t0) <= 1e-12 * t0
+    O.set_temperature(conf, 300.0)
+    f = np.sqrt(300.0 / t0)
+    np.testing.assert_allclose(conf.v, v * f, rtol=1e-14)
+    assert abs(O.temperature(conf) - 300.0) <= 1e-12 * 300.0
+
+
 def test_velocityverlet_exact_for_constant_acceleration():
     L = [1e6] * 3
     x0 = np.array([[10.0, 20.0, 30.0]])
