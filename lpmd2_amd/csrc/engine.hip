@@ -51,22 +51,11 @@ __global__ __launch_bounds__(kBlock) void k_disp(ForceArgs A) {
     atomic_max_sparse(A.maxd2, __float_as_uint(m));
 }
 
-// Start of a velocity-Verlet chunk in one launch: the per-step maxima md[0, m)
-// cleared and, with a snapshot, x, v, a copied for a roll-back (3 n doubles
-// each; thread t takes double t of each array).
-__global__ __launch_bounds__(kBlock) void k_chunk_begin(const double* __restrict__ x, const double* __restrict__ v,
-                                                        const double* __restrict__ a, double* __restrict__ xs,
-                                                        double* __restrict__ vs, double* __restrict__ as, long n3,
-                                                        unsigned int* __restrict__ md, long m, int snapshot) {
+// Start of a chunk of steps: the per-step displacement slots md[0, m) cleared.
+__global__ __launch_bounds__(kBlock) void k_chunk_begin(unsigned int* __restrict__ md, long m) {
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
   if (t < m)
     md[t] = 0u;
-  if (snapshot && t < n3) {
-    const double px = x[t], pv = v[t], pa = a[t];
-    xs[t] = px;
-    vs[t] = pv;
-    as[t] = pa;
-  }
 }
 
 // Integrator::GoBack (liblpmd/simulation/integrator.cc:31-42) for the leapfrog
@@ -107,10 +96,14 @@ __global__ __launch_bounds__(kBlock) void k_lf_halfkick(double* __restrict__ vh,
 // 0.5 m v^2): T_now = (2/3) K / (kB N), K = KIN2EV * red[0]
 // (properties.h:22-53), then v *= sqrt(T / T_now).  A system at rest is left
 // as it is (the reference would divide by zero).
+// gate / gate_b: as for the force kernels of a speculative step.
 __global__ __launch_bounds__(kBlock) void k_set_temperature(double* __restrict__ v, const double* __restrict__ red,
-                                                            long n, double temp) {
+                                                            long n, double temp, const unsigned int* gate,
+                                                            double gate_b) {
   const long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
+    return;
+  if (gate && (*gate == kGateClosed || sqrt((double)__uint_as_float(*gate)) > gate_b))
     return;
   const double ti = (2.0 / 3.0) * (red[0] * kKin2eV) / (kBoltzmann * double(n));
   if (!(ti > 0.0))
@@ -181,8 +174,6 @@ struct lpmdx_system {
   // scratch for re-ordering
   DBuf<double> x2, v2, a2, ffm2, mass2;
   DBuf<int> z2, perm2;
-  // snapshot for the skin check roll-back
-  DBuf<double> xs, vs, as;
   // sort
   DBuf<uint32_t> keys, vals, keys2, vals2, hist, htot;
   DBuf<unsigned long long> sumslots; // spread list-length sum of a build
@@ -219,11 +210,11 @@ struct lpmdx_system {
   bool drifted = false;   // user positions are in xn; x = drift of them
   // leapfrog (plugins/leapfrog.cc): half-step velocities v(t - dt/2), in
   // internal order, once lpmdx_leapfrog_init has set them
-  DBuf<double> vh, vh2, vhs;
+  DBuf<double> vh, vh2;
   bool lf_ready = false;
-  // Beeman: the acceleration of the step before a's (ab), its reorder and
-  // snapshot copies; drift_beeman says which scheme made the look-ahead x
-  DBuf<double> ab, ab2, abs_snap;
+  // Beeman: the acceleration of the step before a's (ab) and its reorder
+  // copy; drift_beeman says which scheme made the look-ahead x
+  DBuf<double> ab, ab2;
   bool bm_ready = false, drift_beeman = false;
   double bm_dt = 0.0;
   double lf_dt = 0.0;
@@ -234,7 +225,7 @@ struct lpmdx_system {
   long cell_version = -1; // pos_version at the last UpdateCell
   double disp_cur = 0.0; // largest |x - xb| of the positions of the next force evaluation
   long chunk = 8;        // steps of the next velocity-Verlet chunk
-  long n_rollbacks = 0;
+  long n_rollbacks = 0; // chunks cut short by the skin test
   double last_e[12] = {0};
   bool last_e_valid = false;
   long n_builds = 0, n_forces = 0, n_steps = 0, max_list = 0, sum_list = 0;
@@ -265,8 +256,25 @@ struct lpmdx_system {
   struct Pending {
     int slot;
     hipEvent_t a, b;
+    long step; // the integrator step the launch belongs to, or -1
   };
   std::vector<Pending> pending;
+  long prof_step = -1; // tag of the launches being recorded
+
+  // Timings of speculative steps that never ran (their kernels returned at
+  // once) leave the profile: every entry tagged with a step >= first.
+  void prof_discard_from(long first) {
+    size_t k = 0;
+    for (auto& p : pending) {
+      if (p.step >= first) {
+        event_pool.push_back(p.a);
+        event_pool.push_back(p.b);
+      } else {
+        pending[k++] = p;
+      }
+    }
+    pending.resize(k);
+  }
   std::vector<hipEvent_t> event_pool;
   double prof_ms[kProfSlots] = {0};
   long prof_n[kProfSlots] = {0};
@@ -294,7 +302,7 @@ struct lpmdx_system {
       return;
     hipEvent_t e = take_event();
     hipchk(hipEventRecord(e, st), "hipEventRecord");
-    pending.push_back({slot, a, e});
+    pending.push_back({slot, a, e, prof_step});
   }
   void prof_collect() {
     for (auto& p : pending) {
@@ -1016,7 +1024,10 @@ struct lpmdx_system {
 
   // Evaluate all potentials at x; mode kForcesOnly / kInitDrift / kStep.
   // With eval, energies land in partials and are reduced into `red`.
-  void forces(int mode, bool eval, double dt, unsigned int* maxd2) {
+  // gate: the previous step's displacement slot of a speculative step (the
+  // kernels return at once when it failed the skin test), or null
+  void forces(int mode, bool eval, double dt, unsigned int* maxd2, const unsigned int* gate = nullptr) {
+    const double gate_b = gate ? budget() : 0.0;
     if (pots.empty())
       fail(LPMDX_ESTATE, "no potential set");
     const long nb = force_blocks();
@@ -1036,6 +1047,8 @@ struct lpmdx_system {
       A.cnt = cnt.p;
       A.partials = partials.p;
       A.maxd2 = maxd2;
+      A.gate = gate;
+      A.gate_b = gate_b;
       A.xb = xb.p;
       A.vh = vh.p;
       A.ab = ab.p;
@@ -1079,6 +1092,8 @@ struct lpmdx_system {
       A.v = v.p;
       A.a = a.p;
       A.maxd2 = maxd2;
+      A.gate = gate;
+      A.gate_b = gate_b;
       A.xb = xb.p;
       A.vh = vh.p;
       A.ab = ab.p;
@@ -1195,22 +1210,24 @@ struct lpmdx_system {
 
   // SetTemperature on the device, ordered on the stream (no host round trip);
   // partials must hold 4 * blocks_for(n) doubles
-  void scale_to_temperature(double temp) {
+  void scale_to_temperature(double temp, const unsigned int* gate = nullptr) {
     const long nb = blocks_for(n);
     k_kinetic<<<nb, kBlock, 0, st>>>(v.p, mass.p, n, partials.p);
     k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
-    k_set_temperature<<<nb, kBlock, 0, st>>>(v.p, red.p, n, temp);
+    k_set_temperature<<<nb, kBlock, 0, st>>>(v.p, red.p, n, temp, gate, gate ? budget() : 0.0);
     launch_check();
   }
 
   // After a step's kernel in the two-step loops: when the schedule is active
   // in step k, scale v and make the look-ahead positions (in x; the step's
   // positions are in xn) again from the scaled velocities.
-  void modifiers_after_step(long k, bool beeman, double dt, unsigned int* maxd2) {
+  void modifiers_after_step(long k, bool beeman, double dt, unsigned int* maxd2, const unsigned int* gate) {
     if (!tsched.active(k))
       return;
-    scale_to_temperature(tsched.value(k));
+    scale_to_temperature(tsched.value(k), gate);
     ForceArgs A{};
+    A.gate = gate;
+    A.gate_b = gate ? budget() : 0.0;
     A.x = xn.p;
     A.xn = x.p;
     A.v = v.p;
@@ -1296,99 +1313,88 @@ struct lpmdx_system {
     // whether the chunk's lists stayed valid (every atom within half the skin
     // of its build position).  A chunk that went past it is rolled back and
     // re-run up to its last valid step; the list is rebuilt there.
+    // Steps run in chunks without a host round trip.  Every step after a
+    // chunk's first is speculative: its kernels run only while the positions
+    // its forces read passed the skin test (every atom within half the skin
+    // of its build position, read from the previous step's displacement slot
+    // on the device), and return at once otherwise.  After the chunk the host
+    // reads the slots, keeps the steps that ran, and rebuilds the list there;
+    // nothing has to be rolled back.
     long done = 0;
-    long forced = 0; // steps known to be valid after a roll-back
     while (done < steps) {
+      prof_step = -1; // rebuilds are no step's launches
       long m = steps - done;
       if (obs_every > 0)
         m = std::min(m, obs_every - (done % obs_every));
       m = std::min<long>(m, kMaxChunk);
+      const bool every = rebuilds_every_update();
       bool need = false;
-      const bool replay = forced > 0; // steps that passed the test before a roll-back
-      if (rebuilds_every_update()) {
+      if (every) {
         m = 1;
         need = true; // UpdateCell rebuilds every call
       } else {
-        const double b = budget();
-        need = !list_built || list_radius() != rl || disp_cur > b;
-        if (replay)
-          m = std::min(m, forced);
-        else
-          m = std::min(m, chunk);
+        need = !list_built || list_radius() != rl || disp_cur > budget();
+        m = std::min(m, chunk);
       }
-      forced = 0;
       // positions x are where the next force evaluation happens
       drifted = false;
       if (need) {
         rebuild();
         disp_cur = 0.0;
       }
-      // Only a chunk of two or more new steps can fail the test (its first
-      // force uses the chunk's starting positions, which passed it): the others
-      // need no snapshot to roll back to.
-      const bool snapshot = !rebuilds_every_update() && m > 1 && !replay;
-      if (snapshot) {
-        xs.alloc(3 * n);
-        vs.alloc(3 * n);
-        as.alloc(3 * n);
-        if (beeman) {
-          abs_snap.alloc(3 * n);
-          hipchk(hipMemcpyAsync(abs_snap.p, ab.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        }
-      }
-      // snapshot and cleared per-step maxima in one launch (m <= kMaxChunk)
-      k_chunk_begin<<<blocks_for(std::max<long>(snapshot ? 3 * n : 0, m)), kBlock, 0, st>>>(
-          x.p, v.p, a.p, xs.p, vs.p, as.p, 3 * n, md, m, snapshot ? 1 : 0);
+      // cleared per-step maxima (m <= kMaxChunk)
+      k_chunk_begin<<<blocks_for(m), kBlock, 0, st>>>(md, m);
       launch_check();
       for (long s = 0; s < m; ++s) {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
-        forces(step_mode, obs, dt, md + s);
+        const unsigned int* gate = (every || s == 0) ? nullptr : md + s - 1;
+        prof_step = n_steps + s;
+        forces(step_mode, obs, dt, md + s, gate);
         std::swap(x.p, xn.p);
-        modifiers_after_step(n_steps + s + 1, beeman, dt, md + s);
+        modifiers_after_step(n_steps + s + 1, beeman, dt, md + s, gate);
       }
+      prof_step = -1;
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
-      sync();
-      // the force of step s saw the positions drifted by step s-1
+      // no profile collection yet: the timings of steps that did not run go first
+      hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+      // the force of step s saw the positions drifted by step s-1; the first
+      // step that failed the test ran no kernel, nor did any after it
       const double b = budget();
       double d = disp_cur, growth = 0.0;
-      long bad = -1;
+      long ran = m;
       for (long s = 0; s < m; ++s) {
-        if (!rebuilds_every_update() && d > b) {
-          bad = s;
+        if (!every && d > b) {
+          ran = s;
           break;
         }
         const double dn = read_disp(s);
         growth = std::max(growth, dn - d);
         d = dn;
       }
-      if (bad >= 0) {
-        if (!snapshot)
-          fail(LPMDX_EDEVICE, std::string(who) + ": a chunk without a snapshot failed the skin test (internal error)");
-        // back to the chunk start; re-run the valid steps, then rebuild
-        hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(a.p, as.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        if (beeman)
-          hipchk(hipMemcpyAsync(ab.p, abs_snap.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        drifted = true;
-        forced = bad;
-        chunk = std::max<long>(1, bad);
+      // each launch swapped the position buffers; the skipped ones wrote none
+      if ((m - ran) % 2 == 1)
+        std::swap(x.p, xn.p);
+      prof_discard_from(n_steps + ran);
+      if (!pending.empty())
+        prof_collect();
+      if (ran < m)
         n_rollbacks++;
-        continue;
-      }
       disp_cur = d;
       pos_version++;
-      // next chunk: double while the displacement stays well inside the
-      // budget, but no longer than its growth rate allows
+      // next chunk: double while nothing moves, else the steps the growth
+      // rate leaves before the budget (from a fresh list after a cut), plus
+      // two so that the chunk usually ends past the limit in one round trip
       long next = std::min<long>(2 * chunk, kMaxChunk);
-      if (growth > 0.0)
-        next = std::min<long>(next, std::max<long>(1, (long)(0.9 * (b - disp_cur) / growth)));
+      if (growth > 0.0) {
+        const double room = disp_cur > b ? b : b - disp_cur;
+        next = std::min<long>(next, (long)(room / growth) + 2);
+      }
       chunk = std::max<long>(1, next);
       drifted = true;
       drift_beeman = beeman;
-      cm.ucalls += 2 * m;
-      done += m;
-      n_steps += m;
+      cm.ucalls += 2 * ran;
+      done += ran;
+      n_steps += ran;
       if (obs_every > 0 && (done % obs_every) == 0) {
         double e;
         read_energies(&e, nullptr, nullptr);
@@ -1537,82 +1543,75 @@ struct lpmdx_system {
       sync();
       disp_cur = read_disp(0);
     }
-    long done = 0, forced = 0;
+    // speculative chunks as in vv()
+    long done = 0;
     while (done < steps) {
+      prof_step = -1; // rebuilds are no step's launches
       long m = steps - done;
       if (obs_every > 0)
         m = std::min(m, obs_every - (done % obs_every));
       m = std::min<long>(m, kMaxChunk);
+      const bool every = rebuilds_every_update();
       bool need;
-      const bool replay = forced > 0;
-      if (rebuilds_every_update()) {
+      if (every) {
         m = 1;
         need = true;
       } else {
         need = !list_built || list_radius() != rl || disp_cur > budget();
-        m = replay ? std::min(m, forced) : std::min(m, chunk);
+        m = std::min(m, chunk);
       }
-      forced = 0;
       if (need) {
         rebuild();
         disp_cur = 0.0;
       }
-      const bool snapshot = !rebuilds_every_update() && m > 1 && !replay;
-      if (snapshot) {
-        xs.alloc(3 * n);
-        vs.alloc(3 * n);
-        as.alloc(3 * n);
-        vhs.alloc(3 * n);
-        hipchk(hipMemcpyAsync(vhs.p, vh.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-      }
-      k_chunk_begin<<<blocks_for(std::max<long>(snapshot ? 3 * n : 0, m)), kBlock, 0, st>>>(
-          x.p, v.p, a.p, xs.p, vs.p, as.p, 3 * n, md, m, snapshot ? 1 : 0);
+      k_chunk_begin<<<blocks_for(m), kBlock, 0, st>>>(md, m);
       launch_check();
       for (long s = 0; s < m; ++s) {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
-        forces(kLeapfrog, obs, dt, md + s);
+        const unsigned int* gate = (every || s == 0) ? nullptr : md + s - 1;
+        prof_step = n_steps + s;
+        forces(kLeapfrog, obs, dt, md + s, gate);
         std::swap(x.p, xn.p);
         // the next step reads the half-step velocities only, as in the
         // reference, so the scaled v changes no trajectory
         if (tsched.active(n_steps + s + 1))
-          scale_to_temperature(tsched.value(n_steps + s + 1));
+          scale_to_temperature(tsched.value(n_steps + s + 1), gate);
       }
+      prof_step = -1;
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
-      sync();
+      // no profile collection yet: the timings of steps that did not run go first
+      hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
       // step s's force saw the positions step s-1 made (step 0: the chunk's)
       const double b = budget();
       double d = disp_cur, growth = 0.0;
-      long bad = -1;
+      long ran = m;
       for (long s = 0; s < m; ++s) {
-        if (!rebuilds_every_update() && d > b) {
-          bad = s;
+        if (!every && d > b) {
+          ran = s;
           break;
         }
         const double dn = read_disp(s);
         growth = std::max(growth, dn - d);
         d = dn;
       }
-      if (bad >= 0) {
-        if (!snapshot)
-          fail(LPMDX_EDEVICE, "leapfrog: a chunk without a snapshot failed the skin test (internal error)");
-        hipchk(hipMemcpyAsync(x.p, xs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(v.p, vs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(a.p, as.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        hipchk(hipMemcpyAsync(vh.p, vhs.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
-        forced = bad;
-        chunk = std::max<long>(1, bad);
+      if ((m - ran) % 2 == 1)
+        std::swap(x.p, xn.p);
+      prof_discard_from(n_steps + ran);
+      if (!pending.empty())
+        prof_collect();
+      if (ran < m)
         n_rollbacks++;
-        continue;
-      }
       disp_cur = d;
       pos_version++;
       long next = std::min<long>(2 * chunk, kMaxChunk);
-      if (growth > 0.0)
-        next = std::min<long>(next, std::max<long>(1, (long)(0.9 * (b - disp_cur) / growth)));
+      if (growth > 0.0) {
+        const double room = disp_cur > b ? b : b - disp_cur;
+        next = std::min<long>(next, (long)(room / growth) + 2);
+      }
       chunk = std::max<long>(1, next);
-      cm.ucalls += m;
-      done += m;
-      n_steps += m;
+      cm.ucalls += ran;
+      done += ran;
+      n_steps += ran;
       forces_ok = false; // a is the force of the positions before x
       if (obs_every > 0 && (done % obs_every) == 0) {
         double e;

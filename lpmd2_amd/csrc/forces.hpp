@@ -285,6 +285,8 @@ struct ForceArgs {
   double4* xw;
   double* partials;     // [nblocks * 12] when evaluating energies
   unsigned int* maxd2;  // max over atoms of |x_next - xb|^2 (float bits), or null
+  const unsigned int* gate; // the previous step's maxd2 slot (speculative steps), or null
+  double gate_b;            // the skin budget that slot is held against
   const double* xb;     // [3n] positions at the last list build
   double* vh;           // [3n] leapfrog half-step velocities (kLeapfrog)
   double* ab;           // [3n] Beeman: the acceleration of the step before a's (kBeeman*)
@@ -425,6 +427,27 @@ __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, d
   xn[0] = fitted_inside(px, A.ll[0]);
   xn[1] = fitted_inside(py, A.ll[1]);
   xn[2] = fitted_inside(pz, A.ll[2]);
+}
+
+// Speculative steps.  A chunk of steps is launched without waiting for the
+// skin test; a step's kernels run only while the positions its forces read
+// stay within the list's budget, i.e. the previous step's largest
+// displacement d2 (the gate slot) has sqrt(d2) <= gate_b, the test the host
+// makes.  Otherwise they return at once and close their own step's slot, so
+// every later step of the chunk returns too and the state stays that of the
+// last valid step.  Uniform over the grid (every thread reads the same slot),
+// so the return comes before any barrier.
+constexpr unsigned int kGateClosed = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool gate_shut(const ForceArgs& A) {
+  if (!A.gate)
+    return false;
+  const unsigned int g = *A.gate;
+  if (g != kGateClosed && !(sqrt((double)__uint_as_float(g)) > A.gate_b))
+    return false;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && A.maxd2)
+    atomicMax(A.maxd2, kGateClosed);
+  return true;
 }
 
 __device__ __forceinline__ void flush_maxd2(const ForceArgs& A, float d2) {
@@ -818,6 +841,8 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& 
 
 template <int KIND, int LPA, bool EVAL, bool SPC>
 static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
+  if (gate_shut(A))
+    return;
   const long t = xcd_block() * kBlock + threadIdx.x;
   Tally tal;
   pair_atom<KIND, LPA, EVAL, false, SPC>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, tal);
@@ -827,6 +852,8 @@ static __global__ __launch_bounds__(kBlock) void k_pair_force(ForceArgs A) {
 
 template <int KIND, int LPA, bool EVAL, int EXP = 0>
 static __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w_out, double* rho_out) {
+  if (gate_shut(A))
+    return;
   const long t = xcd_block() * kBlock + threadIdx.x;
   double emb = 0.0;
   eam_rho_atom<KIND, LPA, EVAL, false, true, EXP>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n, w_out,
@@ -839,6 +866,8 @@ static __global__ __launch_bounds__(kBlock) void k_eam_rho(ForceArgs A, float* w
 
 template <int KIND, int LPA, bool EVAL, bool SPC, int EXP = 0>
 static __global__ __launch_bounds__(kBlock) void k_eam_force(ForceArgs A) {
+  if (gate_shut(A))
+    return;
   const long t = xcd_block() * kBlock + threadIdx.x;
   Tally tal;
   eam_force_atom<KIND, LPA, EVAL, false, SPC, true, EXP, true>(A, Stage{}, t / LPA, (int)(t % LPA), t / LPA < A.n,
@@ -986,6 +1015,8 @@ __device__ __forceinline__ void for_brick_atoms(const ForceArgs& A, const int2* 
 template <int KIND, int LPA, bool EVAL, bool SPC>
 static __global__ __launch_bounds__(1024) void k_pair_force_bk(ForceArgs A, const int2* bhalo,
                                                                                   int brick, int cap, int parts) {
+  if (gate_shut(A))
+    return;
   Tally tal;
   Stage S;
   for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, nullptr, S, [&](long i, int l, bool active) {
@@ -999,6 +1030,8 @@ template <int KIND, int LPA, bool EVAL, int EXP = 0>
 static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const int2* bhalo,
                                                                                int brick, int cap, float* w_out,
                                                                                double* rho_out) {
+  if (gate_shut(A))
+    return;
   double emb = 0.0;
   Stage S;
   for_brick_atoms<LPA>(A, bhalo, brick, cap, 0, nullptr, S, [&](long i, int l, bool active) {
@@ -1013,6 +1046,8 @@ static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const i
 template <int KIND, int LPA, bool EVAL, bool SPC, int EXP = 0>
 static __global__ __launch_bounds__(1024) void k_eam_force_bk(ForceArgs A, const int2* bhalo,
                                                                                  int brick, int cap, int parts) {
+  if (gate_shut(A))
+    return;
   Tally tal;
   Stage S;
   for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, A.w_rho, S, [&](long i, int l, bool active) {
@@ -1025,6 +1060,8 @@ static __global__ __launch_bounds__(1024) void k_eam_force_bk(ForceArgs A, const
 // Velocity Verlet with precomputed accelerations (several potentials):
 // a_new holds the summed accelerations.
 static __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const double* __restrict__ anew) {
+  if (gate_shut(A))
+    return;
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   float d2 = 0.0f;
   if (i < A.n)
@@ -1035,6 +1072,8 @@ static __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const 
 // Drift only: xn = FittedInside((x + v dt) + a/2 dt^2) (velocityverlet.cc:60-67),
 // or with mode kBeemanInit Beeman's position update with a_old = ab.
 static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
+  if (gate_shut(A))
+    return;
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   float d2 = 0.0f;
   if (i < A.n) {
