@@ -417,18 +417,24 @@ def test_set_temperature_matches_oracle():
     assert np.all(s0.get_state()[1] == 0.0)
 
 
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
 @pytest.mark.parametrize("scheme", ["velocityverlet", "beeman", "leapfrog"])
-def test_tempscaling_schedule_matches_oracle(scheme):
+def test_tempscaling_schedule_matches_oracle(scheme, cm):
     """A temperature ramp 60 K -> 140 K applied every 4 steps over 40 steps of
     LJ argon, inside the device step loops, against the oracle running the
-    same modifier after each active step (lpmd/lpmd.cc:89-93)."""
-    say("tempscaling", scheme)
+    same modifier after each active step (lpmd/lpmd.cc:89-93).  With a
+    0.1 A skin the list is rebuilt inside the run, so speculative steps past
+    the skin limit (thermostat kernels included) are launched and skipped."""
+    say("tempscaling", scheme, cm)
     x, L = jittered(4, 5.26, 0.05, 22)
     m = np.full(x.shape[0], 39.948)
     v = E.maxwell_boltzmann(m, 100.0, seed=22)
     dt, steps = 2.0, 40
     s = E.System(x, L, 18, v=v)
-    s.set_cellmanager("linkedcell", "cutoff 8.5 mode auto")
+    if cm == "linkedcell":
+        s.set_cellmanager("linkedcell", "cutoff 8.5 mode auto")
+    else:
+        s.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 300 each 5 skin 0.1")
     s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
     s.tempscaling(60.0, 140.0, start=0, end=steps, each=4)
     st = O.Stepper(0, steps, 4)
@@ -437,7 +443,11 @@ def test_tempscaling_schedule_matches_oracle(scheme):
         if st.active(k):
             O.set_temperature(c, st.value(k, 60.0, 140.0))
 
-    om = O.CellManager("linkedcell", 8.5, mode="auto")
+    # linkedcell: the reference's stencil (two cells a side here, so cells are
+    # visited twice) on both sides; the skin list holds the exact
+    # minimum-image set, which the oracle's minimumimage rebuilds every call
+    om = (O.CellManager("linkedcell", 8.5, mode="auto") if cm == "linkedcell"
+          else O.CellManager("minimumimage", 8.5))
     pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
     conf = O.Conf(x, L, v=v)
     forces = lambda c: O.update_forces(c, om, pots)[0]
@@ -468,6 +478,9 @@ def test_tempscaling_schedule_matches_oracle(scheme):
     assert np.abs(vg - conf.v).max() < 2e-8
     np.testing.assert_allclose(ep, e0[9::10], rtol=1e-6)
     np.testing.assert_allclose(ek, k0[9::10], rtol=1e-6)
+    if cm == "verletlist":
+        say(f"  list builds {s.stats()['builds']}")
+        assert s.stats()["builds"] > 2
     # at the last step the modifier set the ramp's end temperature (index 1:
     # LPMDX_OBS_TEMP)
     assert abs(s.observables()[1] - 140.0) <= 1e-9 * 140.0
