@@ -225,7 +225,7 @@ struct lpmdx_system {
   long cell_version = -1; // pos_version at the last UpdateCell
   double disp_cur = 0.0; // largest |x - xb| of the positions of the next force evaluation
   long chunk = 8;        // steps of the next velocity-Verlet chunk
-  long n_rollbacks = 0; // chunks cut short by the skin test
+  long n_cut_chunks = 0; // chunks cut short by the skin test
   double last_e[12] = {0};
   bool last_e_valid = false;
   long n_builds = 0, n_forces = 0, n_steps = 0, max_list = 0, sum_list = 0;
@@ -1378,7 +1378,7 @@ struct lpmdx_system {
       if (!pending.empty())
         prof_collect();
       if (ran < m)
-        n_rollbacks++;
+        n_cut_chunks++;
       disp_cur = d;
       pos_version++;
       // next chunk: double while nothing moves, else the steps the growth
@@ -1505,9 +1505,9 @@ struct lpmdx_system {
   // `steps` calls of OneStepIntegrator::Advance with Leapfrog::Advance
   // (onestepintegrator.cc:13-46, leapfrog.cc:69-77): forces at x, then per
   // atom vhalf = vh + a dt, x = FittedInside(x + vhalf dt),
-  // v = 0.5 (vhalf + vh), vh = vhalf (fused into the force kernels).  Chunks
-  // and roll-backs as in vv(); the skin test reads the positions each step's
-  // force sees.
+  // v = 0.5 (vhalf + vh), vh = vhalf (fused into the force kernels).
+  // Speculative chunks as in vv(); the skin test reads the positions each
+  // step's force sees.
   void lf(double dt, long steps, long obs_every, double* ep_t, double* ek_t) {
     if (pots.empty())
       fail(LPMDX_ESTATE, "no potential set");
@@ -1600,7 +1600,7 @@ struct lpmdx_system {
       if (!pending.empty())
         prof_collect();
       if (ran < m)
-        n_rollbacks++;
+        n_cut_chunks++;
       disp_cur = d;
       pos_version++;
       long next = std::min<long>(2 * chunk, kMaxChunk);
