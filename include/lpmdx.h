@@ -192,6 +192,15 @@ LPMDX_API int lpmdx_set_temperature(lpmdx_system* sys, double temp);
  * it by the run's length); each == 0 switches the modifier off. */
 LPMDX_API int lpmdx_tempscaling(lpmdx_system* sys, double from, double to, long start, long end, long each);
 
+/* The berendsen modifier (plugins/berendsen.cc:14-80) on the same device
+ * schedule as lpmdx_tempscaling (applied after it when both are set): its
+ * first active step arms it (stop step = that step + tau / dt of the running
+ * integrator, each = 1), each step up to the stop step scales the velocities
+ * by sqrt(1 + (dt / tau) (T / T_now - 1)) with T = ValueAtStep(step, from,
+ * to), and the step after disarms it.  each == 0 switches it off. */
+LPMDX_API int lpmdx_berendsen(lpmdx_system* sys, double from, double to, double tau, long start, long end,
+                              long each);
+
 /* Device-side SetSimulationTags (simulationbuilder.cc:84-124, properties.h:22-81):
  * out[LPMDX_NOBS], see LPMDX_OBS_*.  The potential terms are those of the
  * last force evaluation. */

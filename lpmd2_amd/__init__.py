@@ -66,6 +66,7 @@ def lib():
         L.lpmdx_beeman.argtypes = [vp, d, l, l, vp, vp]
         L.lpmdx_set_temperature.argtypes = [vp, d]
         L.lpmdx_tempscaling.argtypes = [vp, d, d, l, l, l]
+        L.lpmdx_berendsen.argtypes = [vp, d, d, d, l, l, l]
         L.lpmdx_observables.argtypes = [vp, vp]
         L.lpmdx_system_stats.argtypes = [vp, vp]
         L.lpmdx_system_set_lanes.argtypes = [vp, i]
@@ -191,6 +192,10 @@ class System:
     def tempscaling(self, from_: float, to: float, start: int = 0, end: int = -1, each: int = 1):
         """The tempscaling modifier after every step active for (start, end, each); each=0 switches it off."""
         _check(lib().lpmdx_tempscaling(self._h, from_, to, start, end, each))
+
+    def berendsen(self, from_: float, to: float, tau: float = 400.0, start: int = 0, end: int = -1, each: int = 1):
+        """The berendsen modifier on the step schedule (start, end, each); each=0 switches it off."""
+        _check(lib().lpmdx_berendsen(self._h, from_, to, tau, start, end, each))
 
     def neighbor_lists(self, full: bool, rcut: float):
         counts = np.zeros(self.n, dtype=np.int64)

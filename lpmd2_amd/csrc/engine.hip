@@ -98,8 +98,8 @@ __global__ __launch_bounds__(kBlock) void k_lf_halfkick(double* __restrict__ vh,
 // as it is (the reference would divide by zero).
 // gate / gate_b: as for the force kernels of a speculative step.
 __global__ __launch_bounds__(kBlock) void k_set_temperature(double* __restrict__ v, const double* __restrict__ red,
-                                                            long n, double temp, const unsigned int* gate,
-                                                            double gate_b) {
+                                                            long n, double temp, double q,
+                                                            const unsigned int* gate, double gate_b) {
   const long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
     return;
@@ -108,7 +108,9 @@ __global__ __launch_bounds__(kBlock) void k_set_temperature(double* __restrict__
   const double ti = (2.0 / 3.0) * (red[0] * kKin2eV) / (kBoltzmann * double(n));
   if (!(ti > 0.0))
     return;
-  const double f = sqrt(temp / ti);
+  // q < 0: SetTemperature; q = dt / tau: the berendsen modifier's factor
+  // (plugins/berendsen.cc:62-70)
+  const double f = q < 0.0 ? sqrt(temp / ti) : sqrt(1.0 + q * (temp / ti - 1.0));
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     v[3 * i + q] *= f;
@@ -251,6 +253,42 @@ struct lpmdx_system {
       return from + double(k - start) / double(end - start) * (to - from);
     }
   } tsched;
+
+  // the berendsen modifier (plugins/berendsen.cc:14-80): its first active
+  // step only arms it (the stop step = that step + tau / dt, each = 1), then
+  // every step up to the stop step scales v by
+  // sqrt(1 + (dt / tau) (T_set / T - 1)), and the step after it disarms it
+  // (each back to its value).  The state moves with the step numbers only,
+  // so the host advances it as it launches steps.
+  struct Berendsen {
+    bool on = false;
+    double from = 0.0, to = 0.0, tau = 400.0;
+    long start = 0, end = -1, each = 1, old_each = 1, stop = -1;
+    bool active(long k) const {
+      if (!on || k < start)
+        return false;
+      if (end != -1 && k > end)
+        return false;
+      return ((k - start) % each) == 0;
+    }
+    // Apply at step k: true and the target when it scales
+    bool apply(long k, double dt, double& set_temp) {
+      if (!active(k))
+        return false;
+      set_temp = start == end ? from : from + double(k - start) / double(end - start) * (to - from);
+      if (stop == -1) {
+        stop = long(double(k) + tau / dt);
+        old_each = each;
+        each = 1;
+        return false;
+      }
+      if (k <= stop)
+        return true;
+      stop = -1;
+      each = old_each;
+      return false;
+    }
+  } bsched;
   // optional per-kernel timing with HIP events on this system's stream
   bool profiling = false;
   struct Pending {
@@ -1210,21 +1248,46 @@ struct lpmdx_system {
 
   // SetTemperature on the device, ordered on the stream (no host round trip);
   // partials must hold 4 * blocks_for(n) doubles
-  void scale_to_temperature(double temp, const unsigned int* gate = nullptr) {
+  // q: -1 for SetTemperature, dt / tau for a Berendsen step
+  void scale_to_temperature(double temp, const unsigned int* gate = nullptr, double q = -1.0) {
     const long nb = blocks_for(n);
     k_kinetic<<<nb, kBlock, 0, st>>>(v.p, mass.p, n, partials.p);
     k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
-    k_set_temperature<<<nb, kBlock, 0, st>>>(v.p, red.p, n, temp, gate, gate ? budget() : 0.0);
+    k_set_temperature<<<nb, kBlock, 0, st>>>(v.p, red.p, n, temp, q, gate, gate ? budget() : 0.0);
     launch_check();
   }
 
   // After a step's kernel in the two-step loops: when the schedule is active
   // in step k, scale v and make the look-ahead positions (in x; the step's
   // positions are in xn) again from the scaled velocities.
+  // Velocity modifiers of step k (tempscaling, then berendsen); true when
+  // any scaled v.
+  bool velocity_modifiers(long k, double dt, const unsigned int* gate) {
+    bool any = false;
+    if (tsched.active(k)) {
+      scale_to_temperature(tsched.value(k), gate);
+      any = true;
+    }
+    double t = 0.0;
+    if (bsched.apply(k, dt, t)) {
+      scale_to_temperature(t, gate, dt / bsched.tau);
+      any = true;
+    }
+    return any;
+  }
+
+  // After a chunk that ran `ran` of its steps: the berendsen state as the
+  // steps that ran left it (from the state saved at the chunk start).
+  void replay_berendsen(const Berendsen& saved, long first_step, long ran, double dt) {
+    bsched = saved;
+    double t;
+    for (long k = first_step; k < first_step + ran; ++k)
+      bsched.apply(k, dt, t);
+  }
+
   void modifiers_after_step(long k, bool beeman, double dt, unsigned int* maxd2, const unsigned int* gate) {
-    if (!tsched.active(k))
+    if (!velocity_modifiers(k, dt, gate))
       return;
-    scale_to_temperature(tsched.value(k), gate);
     ForceArgs A{};
     A.gate = gate;
     A.gate_b = gate ? budget() : 0.0;
@@ -1267,7 +1330,7 @@ struct lpmdx_system {
       undrift();
     const int init_mode = beeman ? kBeemanInit : kInitDrift;
     const int step_mode = beeman ? kBeemanStep : kStep;
-    if (tsched.on)
+    if (tsched.on || bsched.on)
       partials.alloc(std::max<size_t>((size_t)blocks_for(n) * 4, partials.n));
     unsigned int* md = ctrl.p->maxd2;
     auto read_disp = [&](long s) {
@@ -1345,6 +1408,7 @@ struct lpmdx_system {
       // cleared per-step maxima (m <= kMaxChunk)
       k_chunk_begin<<<blocks_for(m), kBlock, 0, st>>>(md, m);
       launch_check();
+      const Berendsen bsaved = bsched;
       for (long s = 0; s < m; ++s) {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
         const unsigned int* gate = (every || s == 0) ? nullptr : md + s - 1;
@@ -1375,6 +1439,8 @@ struct lpmdx_system {
       if ((m - ran) % 2 == 1)
         std::swap(x.p, xn.p);
       prof_discard_from(n_steps + ran);
+      if (ran < m)
+        replay_berendsen(bsaved, n_steps + 1, ran, dt);
       if (!pending.empty())
         prof_collect();
       if (ran < m)
@@ -1520,7 +1586,7 @@ struct lpmdx_system {
     if (n == 0 || steps <= 0)
       return;
     undrift();
-    if (tsched.on)
+    if (tsched.on || bsched.on)
       partials.alloc(std::max<size_t>((size_t)blocks_for(n) * 4, partials.n));
     unsigned int* md = ctrl.p->maxd2;
     auto read_disp = [&](long s) {
@@ -1566,6 +1632,7 @@ struct lpmdx_system {
       }
       k_chunk_begin<<<blocks_for(m), kBlock, 0, st>>>(md, m);
       launch_check();
+      const Berendsen bsaved = bsched;
       for (long s = 0; s < m; ++s) {
         const bool obs = obs_every > 0 && ((done + s + 1) % obs_every) == 0;
         const unsigned int* gate = (every || s == 0) ? nullptr : md + s - 1;
@@ -1574,8 +1641,7 @@ struct lpmdx_system {
         std::swap(x.p, xn.p);
         // the next step reads the half-step velocities only, as in the
         // reference, so the scaled v changes no trajectory
-        if (tsched.active(n_steps + s + 1))
-          scale_to_temperature(tsched.value(n_steps + s + 1), gate);
+        velocity_modifiers(n_steps + s + 1, dt, gate);
       }
       prof_step = -1;
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
@@ -1597,6 +1663,8 @@ struct lpmdx_system {
       if ((m - ran) % 2 == 1)
         std::swap(x.p, xn.p);
       prof_discard_from(n_steps + ran);
+      if (ran < m)
+        replay_berendsen(bsaved, n_steps + 1, ran, dt);
       if (!pending.empty())
         prof_collect();
       if (ran < m)
@@ -2032,6 +2100,30 @@ int lpmdx_tempscaling(lpmdx_system* S, double from, double to, long start, long 
     S->tsched.start = start;
     S->tsched.end = end;
     S->tsched.each = each;
+  });
+}
+
+int lpmdx_berendsen(lpmdx_system* S, double from, double to, double tau, long start, long end, long each) {
+  return guard([&] {
+    need(S, "system");
+    if (each == 0) {
+      S->bsched = {};
+      return;
+    }
+    if (!std::isfinite(from) || !std::isfinite(to) || from < 0.0 || to < 0.0 || !std::isfinite(tau) || !(tau > 0.0))
+      fail(LPMDX_EINVAL, "berendsen: temperatures must be finite and non-negative, tau positive");
+    if (each < 0 || start < 0 || end < -1 || (end != -1 && end < start))
+      fail(LPMDX_EINVAL, "berendsen: need each >= 1, start >= 0 and end == -1 or end >= start");
+    if (end == -1 && from != to)
+      fail(LPMDX_EINVAL, "berendsen: a ramp (from != to) needs its last step as end");
+    S->bsched = {};
+    S->bsched.on = true;
+    S->bsched.from = from;
+    S->bsched.to = to;
+    S->bsched.tau = tau;
+    S->bsched.start = start;
+    S->bsched.end = end;
+    S->bsched.each = each;
   });
 }
 

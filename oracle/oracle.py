@@ -467,6 +467,39 @@ def set_temperature(conf: Conf, temp: float):
     conf.v *= np.sqrt(temp / ti)
 
 
+class Berendsen:
+    """The berendsen modifier (plugins/berendsen.cc:14-80) as the lpmd
+    application runs it after each step (lpmd/application.cc:404-459): the
+    first active step arms it (stop = step + tau / dt, each = 1), the steps up
+    to stop scale every velocity by sqrt(1 + (dt / tau) (T_set / T - 1)) with
+    T the sequential Temperature, the step after disarms it (each restored)."""
+
+    def __init__(self, frm, to, tau=400.0, start=0, end=-1, each=1):
+        self.frm, self.to, self.tau = frm, to, tau
+        self.st = Stepper(start, end, each)
+        self.stop = -1
+        self.old_each = each
+
+    def apply(self, k, conf, dt):
+        """Returns True when step k scaled the velocities."""
+        if not self.st.active(k):
+            return False
+        set_temp = self.st.value(k, self.frm, self.to)
+        if self.stop == -1:
+            self.stop = int(float(k) + self.tau / dt)
+            self.old_each = self.st.each
+            self.st.each = 1
+            return False
+        if k <= self.stop:
+            ti = temperature(conf)
+            xi = np.sqrt(1.0 + (float(dt) / self.tau) * (set_temp / ti - 1.0))
+            conf.v *= xi
+            return True
+        self.stop = -1
+        self.st.each = self.old_each
+        return False
+
+
 class Stepper:
     """lpmd::Stepper (liblpmd/lpmd/stepper.h:14-35): when a modifier runs and
     the value it ramps to."""

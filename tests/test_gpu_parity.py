@@ -417,9 +417,10 @@ def test_set_temperature_matches_oracle():
     assert np.all(s0.get_state()[1] == 0.0)
 
 
+@pytest.mark.parametrize("mod", ["tempscaling", "berendsen"])
 @pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
 @pytest.mark.parametrize("scheme", ["velocityverlet", "beeman", "leapfrog"])
-def test_tempscaling_schedule_matches_oracle(scheme, cm):
+def test_tempscaling_schedule_matches_oracle(scheme, cm, mod):
     """A temperature ramp 60 K -> 140 K applied every 4 steps over 40 steps of
     LJ argon, inside the device step loops, against the oracle running the
     same modifier after each active step (lpmd/lpmd.cc:89-93).  With a
@@ -436,12 +437,21 @@ def test_tempscaling_schedule_matches_oracle(scheme, cm):
     else:
         s.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 300 each 5 skin 0.1")
     s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
-    s.tempscaling(60.0, 140.0, start=0, end=steps, each=4)
-    st = O.Stepper(0, steps, 4)
+    if mod == "tempscaling":
+        s.tempscaling(60.0, 140.0, start=0, end=steps, each=4)
+        st = O.Stepper(0, steps, 4)
 
-    def modifier(k, c):
-        if st.active(k):
-            O.set_temperature(c, st.value(k, 60.0, 140.0))
+        def modifier(k, c):
+            if st.active(k):
+                O.set_temperature(c, st.value(k, 60.0, 140.0))
+    else:
+        # berendsen with tau 16 fs from step 3 every 12: armed at 3, 23 (stop
+        # 11, 31), scaling at 4..11 and 24..31
+        s.berendsen(60.0, 140.0, tau=16.0, start=3, end=steps, each=12)
+        ob = O.Berendsen(60.0, 140.0, tau=16.0, start=3, end=steps, each=12)
+
+        def modifier(k, c):
+            ob.apply(k, c, dt)
 
     # linkedcell: the reference's stencil (two cells a side here, so cells are
     # visited twice) on both sides; the skin list holds the exact
@@ -481,9 +491,10 @@ def test_tempscaling_schedule_matches_oracle(scheme, cm):
     if cm == "verletlist":
         say(f"  list builds {s.stats()['builds']}")
         assert s.stats()["builds"] > 2
-    # at the last step the modifier set the ramp's end temperature (index 1:
+    # at the last step tempscaling set the ramp's end temperature (index 1:
     # LPMDX_OBS_TEMP)
-    assert abs(s.observables()[1] - 140.0) <= 1e-9 * 140.0
+    if mod == "tempscaling":
+        assert abs(s.observables()[1] - 140.0) <= 1e-9 * 140.0
     with pytest.raises(E.LpmdxError) as e:
         s.tempscaling(60.0, 140.0, start=0, end=-1, each=4)
     assert e.value.code == E.EINVAL

@@ -257,6 +257,22 @@ def test_set_temperature_closed_form():
     assert abs(O.temperature(conf) - 300.0) <= 1e-12 * 300.0
 
 
+def test_berendsen_schedule_and_closed_form():
+    """berendsen.cc:45-80 by hand: start 0, each 10, tau 20 fs, dt 2 fs arms at
+    step 0 (stop 10), scales at steps 1..10, disarms at 11, arms again at 20;
+    one scaling moves T to T + (dt / tau) (T_set - T)."""
+    v = np.array([[0.01, 0.0, 0.0], [0.0, 0.0, -0.02]])
+    conf = O.Conf(np.array([[1.0, 1, 1], [4, 4, 4]]), [20.0] * 3, v=v)
+    b = O.Berendsen(300.0, 300.0, tau=20.0, start=0, end=-1, each=10)
+    scaled = [k for k in range(0, 35) if b.apply(k, conf, 2.0)]
+    assert scaled == list(range(1, 11)) + list(range(21, 31))
+    conf = O.Conf(np.array([[1.0, 1, 1], [4, 4, 4]]), [20.0] * 3, v=v)
+    t0 = O.temperature(conf)
+    b = O.Berendsen(300.0, 300.0, tau=20.0, each=1)
+    assert not b.apply(0, conf, 2.0) and b.apply(1, conf, 2.0)
+    assert abs(O.temperature(conf) - (t0 + 0.1 * (300.0 - t0))) <= 1e-12 * 300.0
+
+
 def test_velocityverlet_exact_for_constant_acceleration():
     L = [1e6] * 3
     x0 = np.array([[10.0, 20.0, 30.0]])
