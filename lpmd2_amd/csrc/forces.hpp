@@ -706,7 +706,11 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
     fs = in ? f : 0.0f;
     e = in ? en : 0.0f;
   } else if (A.one_term) {
-    // at most one term per pair: its parameters selected per lane, one evaluation
+    // at most one term per pair: its parameters selected per lane, one
+    // evaluation.  Every field pair_eval reads is selected (the FP32
+    // parameters, the integer exponents of lennardjonesMod, the table of
+    // fastlj / tabulatedpair); the selects of fields a kind never reads are
+    // dead code for that kind.
     const int ss = si + sj;
     DevPot Q = A.terms[0];
     bool any = Q.spc_sum < 0 || ss == Q.spc_sum;
@@ -717,6 +721,12 @@ __device__ __forceinline__ void pair_terms(const ForceArgs& A, bool valid, int s
       for (int k = 0; k < 12; ++k)
         Q.f[k] = m ? T.f[k] : Q.f[k];
       Q.rc2 = m ? T.rc2 : Q.rc2;
+      Q.ni = m ? T.ni : Q.ni;
+      Q.mi = m ? T.mi : Q.mi;
+      Q.tab_e = m ? T.tab_e : Q.tab_e;
+      Q.tab_f = m ? T.tab_f : Q.tab_f;
+      Q.tab_cut = m ? T.tab_cut : Q.tab_cut;
+      Q.bins = m ? T.bins : Q.bins;
       any = any || m;
     }
     const bool in = valid && any && r2 < Q.rc2;

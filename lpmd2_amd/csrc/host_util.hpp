@@ -428,22 +428,27 @@ inline int lcb_auto_n(double edge, double cutoff) {
 
 inline double edge_module(double L) { return sqrt(L * L + 0.0 * 0.0 + 0.0 * 0.0); }
 
+// ElemMass[Z] of liblpmd/support/elements.cc:20-35 (the mass BasicAtom takes,
+// basicatom.h:57-67); Z outside 0..118, or an entry the table leaves 0, has no mass.
 inline double ElemMass(int z) {
-  // liblpmd/support/elements.cc:20-35 for the species the engine is used with
-  switch (z) {
-  case 1: return 1.00794;
-  case 11: return 22.989768;
-  case 13: return 26.981539;
-  case 17: return 35.4527;
-  case 18: return 39.948;
-  case 26: return 55.847;
-  case 28: return 58.6934;
-  case 29: return 63.546;
-  case 42: return 95.94;
-  case 47: return 107.8682;
-  case 79: return 196.96654;
-  default: return -1.0;
-  }
+  static const double kMass[119] = {
+      5.4859e-4, 1.00794, 4.002602, 6.941, 9.01218, 10.811, 12.011, 14.00674,
+      15.9994, 18.9984303, 20.1797, 22.989768, 24.305, 26.981539, 28.0855, 30.973762,
+      32.066, 35.4527, 39.948, 39.0983, 40.078, 44.95591, 47.88, 50.9415,
+      51.9961, 54.93805, 55.847, 58.9332, 58.6934, 63.546, 65.39, 69.723,
+      72.61, 74.92159, 78.96, 79.904, 83.8, 85.4678, 87.62, 88.90585,
+      91.224, 92.90638, 95.94, 97.9072, 102.07, 102.9055, 106.42, 107.8682,
+      112.411, 114.818, 118.71, 121.760, 127.6, 126.90447, 131.29, 132.90543,
+      137.327, 138.9055, 140.115, 140.90765, 144.24, 144.9127, 150.36, 151.965,
+      157.25, 158.92534, 162.50, 164.93032, 167.26, 168.93421, 173.04, 174.967,
+      178.49, 180.9479, 183.84, 186.207, 190.23, 192.22, 195.08, 196.96654,
+      200.59, 204.3833, 207.2, 208.98037, 208.9824, 209.9871, 222.0176, 223.0197,
+      226.0254, 227.0278, 232.0381, 231.03588, 238.0289, 237.048, 244.0642, 243.0614,
+      247.0703, 247.0703, 251.0796, 252.083, 257.0951, 258.1, 259.1009, 262.11,
+      261, 262, 266, 264, 269, 268, 269, 272,
+      277, 0, 289, 0, 0, 0, 0,
+  };
+  return (z >= 0 && z < 119) ? kMass[z] : -1.0;
 }
 
 inline long blocks_for(long threads) { return (threads + kBlock - 1) / kBlock; }

@@ -134,3 +134,75 @@ def test_arena_admission_and_reuse():
     s3 = b.add(steps=5, **system("lj", 4, 34, 100.0))
     b.run(10)
     assert b.get(s1)[3]["done"] == 5 and b.get(s3)[3]["done"] == 5
+
+
+def oracle_pot(p):
+    """The oracle's potential for a (name, args[, s1, s2]) tuple."""
+    name, args = p[0], p[1]
+    kw = dict(zip(args.split()[0::2], (float(q) for q in args.split()[1::2])))
+    s1, s2 = (p[2], p[3]) if len(p) > 2 else (-1, -1)
+    return O.Pot(name, s1, s2, **kw)
+
+
+def two_species(nx, a, za, zb, seed, noise=0.05):
+    """fcc with the two basis halves of species za and zb (every pair sum occurs)."""
+    x, L = E.fcc(nx, a)
+    rng = np.random.default_rng(seed)
+    x = np.mod(x + rng.normal(0.0, noise, x.shape), L)
+    z = np.where(rng.random(x.shape[0]) < 0.5, za, zb).astype(np.int32)
+    return x, L, z
+
+
+# Several potentials of one kind on disjoint species pairs: the batch evaluates
+# one selected term per pair, and every parameter the kind reads must come from
+# that term (the fastlj table, the lennardjonesMod exponents).
+MULTI = {
+    "fastlj": (lambda: two_species(4, 5.26, 10, 18, 41), [
+        ("fastlj", "sigma 2.78 epsilon 0.0031 cutoff 8.0 bins 300", 10, 10),
+        ("fastlj", "sigma 3.41 epsilon 0.0103 cutoff 8.5 bins 700", 18, 18),
+        ("fastlj", "sigma 3.10 epsilon 0.0057 cutoff 7.5 bins 500", 10, 18)], "cutoff 8.5 skin 0.5"),
+    "lennardjonesMod": (lambda: two_species(4, 5.26, 10, 18, 42), [
+        ("lennardjonesMod", "sigma 2.78 epsilon 0.0031 m 10 n 5 cohesive 0.8 cutoff 8.0", 10, 10),
+        ("lennardjonesMod", "sigma 3.41 epsilon 0.0103 m 12 n 6 cohesive 1.0 cutoff 8.5", 18, 18),
+        ("lennardjonesMod", "sigma 3.10 epsilon 0.0057 m 9 n 7 cohesive 0.9 cutoff 8.5", 10, 18)],
+        "cutoff 8.5 skin 0.5"),
+    "buckingham": (None, None, None),  # the rocksalt NaCl set of the served jobs
+}
+
+
+@pytest.mark.parametrize("kind", sorted(MULTI))
+def test_batch_selected_term_matches_oracle(kind):
+    from lpmd2_amd import workloads as W
+
+    say("batch selected term", kind)
+    if kind == "buckingham":
+        x, L, z = W.rocksalt(3, 3, 3, 5.64)
+        x = np.mod(x + np.random.default_rng(43).normal(0.0, 0.03, x.shape), L)
+        pots = [("buckingham", args, s1, s2) for args, s1, s2 in W.BUCKINGHAM_NACL]
+        cm = "cutoff 8.0 skin 0.4"
+    else:
+        make, pots, cm = MULTI[kind]
+        x, L, z = make()
+    b = Batch(atom_capacity=2048)
+    sid = b.add(x=x, box=L, z=z, potentials=pots, cm=cm, steps=0)
+    b.run(1)
+    _, _, a, info = b.get(sid)
+    assert info["started"] == 1 and info["overflow"] == 0
+    conf = O.Conf(x, L, z=z)
+    cut = float(dict(zip(cm.split()[0::2], cm.split()[1::2]))["cutoff"])
+    ocm = O.CellManager("verletlist", cut, maxneighbors=600, each=5)
+    e0, _, _ = O.update_forces(conf, ocm, [oracle_pot(p) for p in pots])
+    err = np.linalg.norm(a - conf.a) / np.linalg.norm(conf.a)
+    ep, _, _ = b.energies(sid)
+    say(f"  {kind}: {x.shape[0]} atoms, rel force rms {err:.2e}, E {ep:.9g} vs {e0:.9g}")
+    assert err <= 1e-5
+    assert abs(ep - e0) <= 1e-6 * abs(e0)
+    # the plain engine on the same system agrees to the same bar
+    s = E.System(x, L, z)
+    s.set_cellmanager("verletlist", cm + " maxneighbors 600 each 5")
+    for p in pots:
+        s.append_potential(*p)
+    ep2, _, _ = s.update_forces()
+    _, _, a2 = s.get_state()
+    assert np.linalg.norm(a2 - conf.a) / np.linalg.norm(conf.a) <= 1e-5
+    assert abs(ep2 - e0) <= 1e-6 * abs(e0)
