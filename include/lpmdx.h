@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LPMDX_ABI_VERSION 1
+#define LPMDX_ABI_VERSION 2
 
 #if defined(__GNUC__)
 #define LPMDX_API __attribute__((visibility("default")))
@@ -54,7 +54,9 @@ extern "C" {
 #define LPMDX_GUPTA 7
 
 /* observables written by lpmdx_observables, SimulationEngine::SetSimulationTags
- * (liblpmd/simulation/simulationbuilder.cc:84-124) */
+ * (liblpmd/simulation/simulationbuilder.cc:84-124); the kinetic energy,
+ * temperature and momentum leave out atoms carrying fixedvel or fixedpos
+ * when any atom does (properties.h:22-81 with tag = true) */
 #define LPMDX_OBS_KINETIC 0   /* kinetic-energy (eV) */
 #define LPMDX_OBS_TEMP 1      /* temperature (K) */
 #define LPMDX_OBS_PX 2        /* px, py, pz */
@@ -62,7 +64,21 @@ extern "C" {
 #define LPMDX_OBS_VIRIAL 6    /* virial (eV) */
 #define LPMDX_OBS_PRESSURE 7  /* pressure (MPa) */
 #define LPMDX_OBS_STRESS 8    /* 9 entries, s_ab = pressfactor/V * stress[a][b] */
-#define LPMDX_NOBS 17
+#define LPMDX_OBS_TOTAL 17            /* total-energy (eV) */
+#define LPMDX_OBS_MOMENTUM 18         /* momentum |p| */
+#define LPMDX_OBS_VOLUME 19           /* volume (A^3) */
+#define LPMDX_OBS_VOLUME_PER_ATOM 20  /* volume-per-atom */
+#define LPMDX_OBS_PARTICLE_DENSITY 21 /* particle-density (1/A^3) */
+#define LPMDX_OBS_DENSITY 22          /* density (g/cm^3) */
+#define LPMDX_OBS_VIRIAL_PRESSURE 23  /* virial-pressure (MPa) */
+#define LPMDX_OBS_KINETIC_PRESSURE 24 /* kinetic-pressure (MPa) */
+#define LPMDX_OBS_CELL_A 25           /* cell-a, cell-b, cell-c (A) */
+#define LPMDX_NOBS 28
+
+/* per-atom flags of lpmdx_system_set_fixed */
+#define LPMDX_FIX_VEL 1    /* the fixedvel tag: a = 0 after each force evaluation of a step */
+#define LPMDX_FIX_POS 2    /* fixedpos with the value "true": not advanced by a step */
+#define LPMDX_FIX_TAGGED 4 /* carries fixedvel or fixedpos, whatever the value */
 
 typedef struct lpmdx_system lpmdx_system;
 
@@ -140,6 +156,14 @@ LPMDX_API int lpmdx_update_forces(lpmdx_system* sys, double* epot, double* viria
 LPMDX_API int lpmdx_potential_forces(lpmdx_system* sys, int k, double* acc, double* epot, double* virial,
                                      double stress[9]);
 
+/* Potential::AtomEnergy(conf, i) of potential k for every atom i, e[n] in
+ * caller order: PairPotential::AtomEnergy (pairpotential.cc:21-33) sums the
+ * pair energy over atom i's full neighbour list (species rule and cutoff as
+ * in UpdateForces); MetalPotential::AtomEnergy (metalpotential.cc:47-60) adds
+ * F(rho_i) to the pair energies where the species rule applies.  Lists of the
+ * last UpdateCell (run first if the positions changed since). */
+LPMDX_API int lpmdx_atom_energies(lpmdx_system* sys, int k, double* e);
+
 /* The application loop with the velocityverlet integrator (lpmd/lpmd.cc:70-103,
  * plugins/velocityverlet.cc:55-75, twostepintegrator.cc:12-68): `steps`
  * DoSteps of length dt (fs).  State stays device-resident between calls.
@@ -175,11 +199,41 @@ LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_ev
                            double* ekin_t);
 
 /* SimulationEngine::SetTemperature (liblpmd/simulation/simulationbuilder.cc:53-71)
- * on the device: every velocity times sqrt(temp / T_now), T_now from the
- * kinetic energy (properties.h:22-53) summed on the device (FP64, in a fixed
- * order that differs from the reference's sequential sum, so the factor
- * agrees to FP64 rounding).  A system at rest is left unchanged. */
+ * on the device.  While no velocities were set (a new system: velocitiesSet
+ * is false until InitVelocities or Restore) it first runs InitVelocities
+ * (lpmdx_init_velocities), as the reference does; then every velocity is
+ * multiplied by sqrt(temp / T_now), T_now from the kinetic energy
+ * (properties.h:22-53) summed on the device (FP64, in a fixed order that
+ * differs from the reference's sequential sum, so the factor agrees to FP64
+ * rounding).  With tagged atoms (lpmdx_system_set_fixed) the sum, the atom
+ * count and the scaling leave them out, as SetTemperature(temp, true) does
+ * when tempscaling passes HaveAny(fixedvel) || HaveAny(fixedpos)
+ * (plugins/tempscaling.cc:70).  A system at rest is left unchanged (the
+ * reference would divide by zero). */
 LPMDX_API int lpmdx_set_temperature(lpmdx_system* sys, double temp);
+
+/* SimulationEngine::InitVelocities (simulationbuilder.cc:167-185): random
+ * directions from the system's drand48 stream (RandomVector,
+ * liblpmd/lpmd/vector.h:161-167), speed sqrt(3 k_B (1 K) / m), the mean
+ * momentum taken out; marks the velocities as set.  lpmdx_seed48 seeds the
+ * stream as srand48(seed) seeds drand48 (lpmd/application.cc:38 seeds it with
+ * the time); an unseeded stream starts where an unseeded drand48 does.
+ * lpmdx_velocities_set marks the velocities as set (or not) without drawing,
+ * as SimulationEngine::Restore does (simulationbuilder.cc:290). */
+LPMDX_API int lpmdx_init_velocities(lpmdx_system* sys);
+LPMDX_API int lpmdx_seed48(lpmdx_system* sys, long seed);
+LPMDX_API int lpmdx_velocities_set(lpmdx_system* sys, int set);
+
+/* The fixedvel / fixedpos atom tags (TwoStepIntegrator::Advance,
+ * twostepintegrator.cc:12-68; OneStepIntegrator::Advance,
+ * onestepintegrator.cc:13-46): flags[n] in caller order, LPMDX_FIX_* bits
+ * (NULL clears).  The integrator entry points then zero a fixedvel atom's
+ * acceleration after each force evaluation of a step and leave a fixedpos
+ * ("true") atom's position and velocity as they are; forces on the other
+ * atoms are unchanged.  Tagged atoms drop out of the tag-aware observables and
+ * SetTemperature.  An atom with LPMDX_FIX_VEL or LPMDX_FIX_POS must also
+ * carry LPMDX_FIX_TAGGED. */
+LPMDX_API int lpmdx_system_set_fixed(lpmdx_system* sys, const int* flags);
 
 /* The tempscaling modifier (plugins/tempscaling.cc:14-63) as the lpmd
  * application runs it (lpmd/lpmd.cc:89-93, application.cc:404-459): after

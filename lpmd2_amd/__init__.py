@@ -24,7 +24,17 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblpmdx.so")
 
 OK, EINVAL, ENONFINITE, EDEVICE, ECAPACITY, ESTATE, ENOMEM = 0, -1, -2, -3, -4, -5, -6
-NOBS = 17
+NOBS = 28
+# lpmdx_observables slots (include/lpmdx.h LPMDX_OBS_*), the SetSimulationTags tag names
+OBS = {"kinetic-energy": 0, "temperature": 1, "px": 2, "py": 3, "pz": 4, "potential-energy": 5, "virial": 6,
+       "pressure": 7, "total-energy": 17, "momentum": 18, "volume": 19, "volume-per-atom": 20,
+       "particle-density": 21, "density": 22, "virial-pressure": 23, "kinetic-pressure": 24, "cell-a": 25,
+       "cell-b": 26, "cell-c": 27}
+for _q, _a in enumerate("xyz"):
+    for _p_, _b in enumerate("xyz"):
+        # the tag s<q><p> is pressfactor / V * stress[q][p]
+        OBS[f"s{_a}{_b}"] = 8 + 3 * _q + _p_
+FIX_VEL, FIX_POS, FIX_TAGGED = 1, 2, 4
 
 
 class LpmdxError(RuntimeError):
@@ -72,6 +82,11 @@ def lib():
         L.lpmdx_system_set_lanes.argtypes = [vp, i]
         L.lpmdx_system_profile.argtypes = [vp, i]
         L.lpmdx_system_profile_read.argtypes = [vp, vp, vp]
+        L.lpmdx_atom_energies.argtypes = [vp, i, vp]
+        L.lpmdx_seed48.argtypes = [vp, l]
+        L.lpmdx_init_velocities.argtypes = [vp]
+        L.lpmdx_velocities_set.argtypes = [vp, i]
+        L.lpmdx_system_set_fixed.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -186,7 +201,8 @@ class System:
         return ep[:k], ek[:k]
 
     def set_temperature(self, temp: float):
-        """SetTemperature: scale every velocity to the temperature `temp` (K)."""
+        """SetTemperature: scale every velocity to the temperature `temp` (K),
+        after drawing them with InitVelocities while none were set."""
         _check(lib().lpmdx_set_temperature(self._h, temp))
 
     def tempscaling(self, from_: float, to: float, start: int = 0, end: int = -1, each: int = 1):
@@ -211,6 +227,43 @@ class System:
         out = np.zeros(NOBS)
         _check(lib().lpmdx_observables(self._h, _p(out)))
         return out
+
+    def tags(self):
+        """The SetSimulationTags tags by name (include/lpmdx.h LPMDX_OBS_*)."""
+        out = self.observables()
+        return {k: float(out[q]) for k, q in OBS.items()}
+
+    def atom_energies(self, k: int = 0):
+        """Potential::AtomEnergy of potential k for every atom."""
+        e = np.zeros(self.n)
+        _check(lib().lpmdx_atom_energies(self._h, k, _p(e)))
+        return e
+
+    def seed48(self, seed: int):
+        """Seed the system's drand48 stream as srand48(seed) does."""
+        _check(lib().lpmdx_seed48(self._h, seed))
+
+    def init_velocities(self):
+        """SimulationEngine::InitVelocities from the system's drand48 stream."""
+        _check(lib().lpmdx_init_velocities(self._h))
+
+    def velocities_set(self, flag: bool = True):
+        """Mark the velocities as set (SetTemperature then draws none), as Restore does."""
+        _check(lib().lpmdx_velocities_set(self._h, int(flag)))
+
+    def set_fixed(self, fixedvel=None, fixedpos=None, fixedpos_true=None):
+        """The fixedvel / fixedpos atom tags: boolean masks of the atoms carrying
+        each tag; fixedpos_true marks which fixedpos tags read "true" (default:
+        all of them).  No argument clears the tags."""
+        if fixedvel is None and fixedpos is None:
+            _check(lib().lpmdx_system_set_fixed(self._h, None))
+            return
+        fv = np.zeros(self.n, bool) if fixedvel is None else np.asarray(fixedvel, bool)
+        fp = np.zeros(self.n, bool) if fixedpos is None else np.asarray(fixedpos, bool)
+        ft = fp if fixedpos_true is None else (fp & np.asarray(fixedpos_true, bool))
+        flags = (np.where(fv, FIX_VEL, 0) | np.where(ft, FIX_POS, 0) | np.where(fv | fp, FIX_TAGGED, 0))
+        flags = np.ascontiguousarray(flags, dtype=np.int32)
+        _check(lib().lpmdx_system_set_fixed(self._h, _p(flags)))
 
     def stats(self):
         out = np.zeros(8, dtype=np.int64)

@@ -6,9 +6,15 @@ import subprocess
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
-SOURCES = [os.path.join(_HERE, "csrc", f) for f in ("engine.hip", "batch.hip", "server.cpp")]
+# engine_kinds_*.hip instantiate the force-kernel launchers of a few potential
+# kinds each, so that the kernel variants compile in parallel
+SOURCES = [os.path.join(_HERE, "csrc", f) for f in (
+    "batch.hip", "engine_kinds_eam_a.hip", "engine_kinds_pair_a.hip", "engine_kinds_eam_b.hip",
+    "engine_kinds_pair_b.hip", "engine_kinds_pair_c.hip", "engine_kinds_pair_d.hip", "engine_kinds_eam_c.hip",
+    "engine.hip", "server.cpp")]
 HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp",
-                                                     "batch.hpp", "host_util.hpp", "table.hpp")] + [
+                                                     "batch.hpp", "host_util.hpp", "table.hpp",
+                                                     "engine_system.hpp", "engine_launch.hpp")] + [
     os.path.join(ROOT, "include", "lpmdx.h")
 ]
 OUT = os.path.join(_HERE, "liblpmdx.so")

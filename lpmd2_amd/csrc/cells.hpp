@@ -242,6 +242,9 @@ struct PermuteArgs {
   const double* ex[kPermuteExtra];
   double* ex2[kPermuteExtra];
   long n;
+  // per-atom fixed-atom flags moved along (null: none)
+  const int* fix;
+  int* fix2;
 };
 
 // Every load is issued before the first store (the output arrays are distinct
@@ -259,6 +262,7 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
     a[q] = A.a[3 * s + q];
   }
   const int z = A.z[s], o = A.perm_old[s];
+  const int fx = A.fix ? A.fix[s] : 0;
   const double ffm = A.ffm[s], mass = A.mass[s];
   double h[kPermuteExtra][3];
 #pragma unroll
@@ -276,6 +280,8 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
   A.ffm2[k] = ffm;
   A.mass2[k] = mass;
   A.perm_new[k] = o;
+  if (A.fix)
+    A.fix2[k] = fx;
 #pragma unroll
   for (int e = 0; e < kPermuteExtra; ++e)
     if (A.ex[e]) {

@@ -1,0 +1,6 @@
+// Force-kernel launchers of the potential kinds kBuckingham kHarmonic (engine_launch.hpp),
+// compiled on their own so that the kinds build in parallel.
+#include "engine_launch.hpp"
+
+LPMDX_INSTANTIATE_KIND(kBuckingham)
+LPMDX_INSTANTIATE_KIND(kHarmonic)

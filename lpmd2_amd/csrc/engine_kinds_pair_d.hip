@@ -1,0 +1,5 @@
+// Force-kernel launchers of the potential kinds kLJMod (engine_launch.hpp),
+// compiled on their own so that the kinds build in parallel.
+#include "engine_launch.hpp"
+
+LPMDX_INSTANTIATE_KIND(kLJMod)

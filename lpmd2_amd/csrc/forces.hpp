@@ -294,6 +294,12 @@ struct ForceArgs {
   // twice the displacement allowed since the build), or 0.  An atom at least
   // this far from every face has no listed neighbour across the boundary.
   double margin;
+  // per-atom fixed-atom flags (kFixVel, kFixPos; internal order) of a
+  // system where some atom carries the fixedvel or fixedpos tag, else null
+  const int* fix;
+  // per-atom energies (Potential::AtomEnergy) of an evaluating launch, or
+  // null: the pair energies over the atom's full list, plus F(rho_i) for EAM
+  double* eatom;
   DevPot P;
   const DevPot* terms; // several potentials of one kind on one list (species pairs), or null
   int one_term;        // with terms: each species pair meets at most one of them
@@ -321,12 +327,43 @@ __device__ __forceinline__ float build_displacement2(const ForceArgs& A, long i,
   return (float)sqmod(dx, dy, dz);
 }
 
+// Fixed atoms of an integrator step (TwoStepIntegrator::Advance,
+// twostepintegrator.cc:12-68; OneStepIntegrator::Advance, onestepintegrator.cc:13-46):
+// a fixedvel atom's acceleration is set to zero after every force evaluation
+// of a step, a fixedpos atom whose tag reads "true" is neither moved nor
+// has its velocity (or any history the scheme keeps) advanced.
+constexpr int kFixVel = 1;
+constexpr int kFixPos = 2;
+// the atom carries fixedvel or fixedpos with any value: the tag-aware
+// kinetic energy, temperature and momentum (properties.h:22-81) and
+// SetTemperature (simulationbuilder.cc:53-71) leave it out
+constexpr int kFixTagged = 4;
+
 // Velocity Verlet tail for atom i given its new acceleration (FP64):
 // plugins/velocityverlet.cc:60-75, in the reference's order of operations.
 __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, double ay,
                                         double az, float& d2) {
   double* a = A.a + 3 * i;
   d2 = 0.0f;
+  if (A.mode != kForcesOnly && A.fix) {
+    const int f = A.fix[i];
+    if (f & kFixVel)
+      ax = ay = az = 0.0;
+    if (f & kFixPos) {
+      // UpdateForces still sets a; nothing else of the atom changes
+      a[0] = ax;
+      a[1] = ay;
+      a[2] = az;
+      const double* x = A.x + 3 * i;
+      const double p[3] = {x[0], x[1], x[2]};
+      d2 = build_displacement2(A, i, p[0], p[1], p[2]);
+      double* xn = A.xn + 3 * i;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        xn[q] = p[q];
+      return;
+    }
+  }
   if (A.mode == kForcesOnly) {
     if (A.accumulate) {
       a[0] += ax;
@@ -757,6 +794,7 @@ template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC>
 __device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                           Tally& tal) {
   ForceSum F;
+  double ea = 0.0; // the atom's pair energies (PairPotential::AtomEnergy)
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const int si = SPC ? A.z[i] : 0;
@@ -766,10 +804,18 @@ __device__ __forceinline__ void pair_atom(const ForceArgs& A, const Stage& H, lo
       float fs, e;
       pair_terms<KIND, SPC>(A, valid, si, sj, r2, fs, e);
       F.add(fs, dx, dy, dz);
-      if constexpr (EVAL)
+      if constexpr (EVAL) {
         tal.add(fs, (float)dx, (float)dy, (float)dz, (float)r2, e);
+        ea += (double)e;
+      }
     });
   }
+  if constexpr (EVAL)
+    if (A.eatom) {
+      ea = group_sum<LPA>(ea);
+      if (active && l == 0)
+        A.eatom[i] = ea;
+    }
   const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
   float d2 = 0.0f;
   if (active && l == 0) {
@@ -809,8 +855,12 @@ __device__ __forceinline__ void eam_rho_atom(const ForceArgs& A, const Stage& H,
       A.xw[i] = make_double4(xi, yi, zi, (double)w);
     if (rho_out)
       rho_out[i] = rho;
-    if constexpr (EVAL)
-      emb += -(double)A.P.f[11] * sqrt(rho);
+    if constexpr (EVAL) {
+      const double fe = -(double)A.P.f[11] * sqrt(rho);
+      emb += fe;
+      if (A.eatom)
+        A.eatom[i] = fe; // MetalPotential::AtomEnergy's F(rho_i); the force pass adds the pairs
+    }
   }
 }
 
@@ -822,6 +872,7 @@ template <int KIND, int LPA, bool EVAL, bool STAGED, bool SPC, bool PREFETCH = t
 __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& H, long i, int l, bool active,
                                                Tally& tal) {
   ForceSum F;
+  double ea = 0.0; // the atom's pair energies where the species rule applies
   if (active) {
     const double xi = A.x[3 * i], yi = A.x[3 * i + 1], zi = A.x[3 * i + 2];
     const int si = SPC ? A.z[i] : 0;
@@ -836,10 +887,18 @@ __device__ __forceinline__ void eam_force_atom(const ForceArgs& A, const Stage& 
       eam_pair<KIND, EXP>(A.P, fr2, wi + wj, fs, V);
       fs = in ? fs : 0.0f;
       F.add(fs, dx, dy, dz);
-      if constexpr (EVAL)
+      if constexpr (EVAL) {
         tal.add(fs, (float)dx, (float)dy, (float)dz, fr2, in ? V : 0.0f);
+        ea += in ? (double)V : 0.0;
+      }
     });
   }
+  if constexpr (EVAL)
+    if (A.eatom) {
+      ea = group_sum<LPA>(ea);
+      if (active && l == 0)
+        A.eatom[i] += ea;
+    }
   const double Fx = group_sum<LPA>(F.x), Fy = group_sum<LPA>(F.y), Fz = group_sum<LPA>(F.z);
   float d2 = 0.0f;
   if (active && l == 0) {
@@ -1081,6 +1140,9 @@ static __global__ __launch_bounds__(kBlock) void k_integrate(ForceArgs A, const 
 
 // Drift only: xn = FittedInside((x + v dt) + a/2 dt^2) (velocityverlet.cc:60-67),
 // or with mode kBeemanInit Beeman's position update with a_old = ab.
+// Fixed atoms (A.fix): a fixedvel atom's acceleration is zeroed first (the
+// step's force evaluation may be one made outside a step), a fixedpos atom
+// keeps its position.
 static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
   if (gate_shut(A))
     return;
@@ -1089,7 +1151,16 @@ static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
   if (i < A.n) {
     const double dt = A.dt;
     double p[3];
+    const int f = A.fix ? A.fix[i] : 0;
+    if (f & kFixVel)
+      for (int q = 0; q < 3; ++q)
+        A.a[3 * i + q] = 0.0;
     for (int q = 0; q < 3; ++q) {
+      if (f & kFixPos) {
+        p[q] = A.x[3 * i + q];
+        A.xn[3 * i + q] = p[q];
+        continue;
+      }
       if (A.mode == kBeemanInit)
         p[q] = beeman_position(A.x[3 * i + q], A.v[3 * i + q], A.a[3 * i + q], A.ab[3 * i + q], dt);
       else
@@ -1101,14 +1172,17 @@ static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
   flush_maxd2(A, d2);
 }
 
-// Kinetic energy and momentum partials (properties.h:22-81).
+// Kinetic energy and momentum partials (properties.h:22-81).  skip: the
+// tag-aware sums (tag = true there) leave out the atoms whose flags have a
+// bit of `skip` set; null: every atom.
 static __global__ __launch_bounds__(kBlock) void k_kinetic(const double* __restrict__ v,
                                                     const double* __restrict__ mass, long n,
-                                                    double* __restrict__ partials) {
+                                                    double* __restrict__ partials,
+                                                    const int* __restrict__ fix = nullptr, int skip = 0) {
   __shared__ double sh[4 * (kBlock / kWave)];
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   double t[4] = {0, 0, 0, 0};
-  if (i < n) {
+  if (i < n && !(fix && (fix[i] & skip))) {
     const double m = mass[i];
     const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
     t[0] = 0.5 * m * sqmod(vx, vy, vz);

@@ -453,4 +453,47 @@ inline double ElemMass(int z) {
 
 inline long blocks_for(long threads) { return (threads + kBlock - 1) / kBlock; }
 
+// The 48-bit linear congruential generator behind drand48 as POSIX specifies
+// it: X' = (0x5DEECE66D X + 0xB) mod 2^48, drand48() = X' / 2^48;
+// srand48(s) sets X = (low 32 bits of s) * 2^16 + 0x330E; a program that
+// never seeds starts from X = 0x1234ABCD330E.
+struct Drand48 {
+  uint64_t x = 0x1234ABCD330EULL;
+  void seed(long s) { x = ((uint64_t)(uint32_t)s << 16) | 0x330EULL; }
+  double next() {
+    x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
+    return std::ldexp((double)x, -48);
+  }
+};
+
+// SimulationEngine::InitVelocities (liblpmd/simulation/simulationbuilder.cc:167-185):
+// each atom gets a random direction (RandomVector, liblpmd/lpmd/vector.h:161-167:
+// three 2 drand48() - 1 draws scaled to length sqrt(2 K / m), K = (3/2) k_B
+// 1 K with k_B the unit system's 8.6173422e-5 eV/K), then the mean momentum
+// per atom is taken out, v -= (sum m v / N) / m, in the reference's order of
+// operations.  v[3n] in caller order.
+inline void init_velocities(long n, const double* mass, Drand48& rng, double* v) {
+  const double kboltzmann = 8.6173422e-05; // liblpmd/runtime/units.cc:7
+  const double targettemp = 1.0;
+  const double Kpart = (3.0 / 2.0) * kboltzmann * targettemp;
+  double tp[3] = {0.0, 0.0, 0.0};
+  for (long i = 0; i < n; ++i) {
+    const double v2 = 2.0 * Kpart / mass[i];
+    const double m = std::sqrt(v2);
+    double w[3];
+    for (int q = 0; q < 3; ++q)
+      w[q] = 2.0 * rng.next() - 1.0;
+    const double r = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int q = 0; q < 3; ++q) {
+      v[3 * i + q] = m * w[q] / r;
+      tp[q] += v[3 * i + q] * mass[i];
+    }
+  }
+  for (int q = 0; q < 3; ++q)
+    tp[q] = tp[q] / (double)n;
+  for (long i = 0; i < n; ++i)
+    for (int q = 0; q < 3; ++q)
+      v[3 * i + q] -= tp[q] / mass[i];
+}
+
 } // namespace lpmdx

@@ -1009,6 +1009,38 @@ static int pair_update(orc_conf* c, orc_cm* cm, orc_pot* p, double* ecache, doub
   return 0;
 }
 
+/* Potential::AtomEnergy(conf, i) for every atom (the cell manager updated):
+ * PairPotential::AtomEnergy, liblpmd/simulation/pairpotential.cc:21-33 (pair
+ * energies over i's full list where the species rule applies and r2 < rc^2);
+ * MetalPotential::AtomEnergy, metalpotential.cc:47-60 (rho_i over the full
+ * list, the pair energies where the rule applies, plus F(rho_i)). */
+int orc_atom_energies(orc_conf* c, orc_cm* cm, const orc_pot* p, double* e) {
+  orc_nlist nl = {0};
+  const double cutoff = p->cutoff;
+  for (long i = 0; i < c->n; ++i) {
+    double etmp = 0.0, rhoi = 0.0;
+    if (orc_cm_list(cm, c, i, 1, cutoff, &nl)) {
+      orc_nlist_free(&nl);
+      return -1;
+    }
+    for (long k = 0; k < nl.size; ++k) {
+      const long j = nl.j[k];
+      if (orc_pot_is_metal(p)) {
+        rhoi += orc_rhoij(p, sqrt(nl.r2[k]));
+        if (applies_to(p, c->z[i], c->z[j]))
+          etmp += orc_pair_energy(p, sqrt(nl.r2[k]));
+      } else if (applies_to(p, c->z[i], c->z[j]) && nl.r2[k] < cutoff * cutoff) {
+        etmp += orc_pair_energy(p, sqrt(nl.r2[k]));
+      }
+    }
+    if (orc_pot_is_metal(p))
+      etmp += orc_embed(p, rhoi);
+    e[i] = etmp;
+  }
+  orc_nlist_free(&nl);
+  return 0;
+}
+
 /* MetalPotential::UpdateForces, liblpmd/simulation/metalpotential.cc:62-180.
  * The first-call corrections pass (:86-116) only prints, so it is omitted. */
 static int metal_update(orc_conf* c, orc_cm* cm, orc_pot* p, double* ecache, double* virial,
@@ -1132,9 +1164,21 @@ void orc_vv_velocity(orc_conf* c, double dt, const double* aold) {
       c->v[3 * i + q] = c->v[3 * i + q] + (aold[3 * i + q] + c->a[3 * i + q]) * h;
 }
 
+/* TwoStepIntegrator::Advance's tag handling (twostepintegrator.cc:17-26,45-54):
+ * after a force evaluation every fixedvel atom's acceleration is set to zero */
+static void zero_fixedvel(orc_conf* c) {
+  if (!c->fix)
+    return;
+  for (long i = 0; i < c->n; ++i)
+    if (c->fix[i] & ORC_FIX_VEL)
+      c->a[3 * i] = c->a[3 * i + 1] = c->a[3 * i + 2] = 0.0;
+}
+
 /* lpmd/lpmd.cc:70-103 with SimulationEngine::DoStep (simulationbuilder.cc:126-136),
  * Integrator::UseOldConfig (integrator.cc:16-22) and TwoStepIntegrator::Advance
- * (twostepintegrator.cc:12-68) with no fixedvel/fixedpos tags. */
+ * (twostepintegrator.cc:12-68): fixedvel atoms get a = 0 after each force
+ * evaluation, fixedpos ("true") atoms skip AdvancePosition and
+ * AdvanceVelocity (c->fix). */
 int orc_md_run(orc_conf* c, orc_cm* cm, int npot, orc_pot* pots, double dt, long steps,
                double* epot_t, double* ekin_t, double* vir_t) {
   double ep, vir, S[9];
@@ -1152,14 +1196,43 @@ int orc_md_run(orc_conf* c, orc_cm* cm, int npot, orc_pot* pots, double dt, long
     }
     if (orc_update_forces(c, cm, npot, pots, &ep, &vir, S))
       goto err;
-    orc_vv_position(c, dt, aold);
+    zero_fixedvel(c);
+    if (!c->fix) {
+      orc_vv_position(c, dt, aold);
+    } else {
+      /* AdvancePosition for every atom but the fixedpos ones */
+      orc_conf one = *c;
+      for (long i = 0; i < c->n; ++i) {
+        if (c->fix[i] & ORC_FIX_POS)
+          continue;
+        one.n = 1;
+        one.x = c->x + 3 * i;
+        one.v = c->v + 3 * i;
+        one.a = c->a + 3 * i;
+        orc_vv_position(&one, dt, aold + 3 * i);
+      }
+    }
     if (orc_update_forces(c, cm, npot, pots, &ep, &vir, S))
       goto err;
-    orc_vv_velocity(c, dt, aold);
+    zero_fixedvel(c);
+    if (!c->fix) {
+      orc_vv_velocity(c, dt, aold);
+    } else {
+      orc_conf one = *c;
+      for (long i = 0; i < c->n; ++i) {
+        if (c->fix[i] & ORC_FIX_POS)
+          continue;
+        one.n = 1;
+        one.x = c->x + 3 * i;
+        one.v = c->v + 3 * i;
+        one.a = c->a + 3 * i;
+        orc_vv_velocity(&one, dt, aold + 3 * i);
+      }
+    }
     if (epot_t)
       epot_t[s] = ep;
     if (ekin_t)
-      ekin_t[s] = orc_kinetic_energy(c);
+      ekin_t[s] = orc_kinetic_energy_tag(c, c->fix != NULL);
     if (vir_t)
       vir_t[s] = vir;
   }
@@ -1190,9 +1263,122 @@ double orc_temperature(const orc_conf* c) {
 }
 
 /* Momentum, properties.h:64-81 */
-void orc_momentum(const orc_conf* c, double p[3]) {
+void orc_momentum(const orc_conf* c, double p[3]) { orc_momentum_tag(c, 0, p); }
+
+static inline int tagged(const orc_conf* c, int tag, long i) {
+  return tag && c->fix && (c->fix[i] & ORC_FIX_TAGGED);
+}
+
+/* KineticEnergy(atoms, tag), properties.h:22-37 */
+double orc_kinetic_energy_tag(const orc_conf* c, int tag) {
+  double K = 0.0;
+  for (long i = 0; i < c->n; ++i) {
+    if (tagged(c, tag, i))
+      continue;
+    K += 0.5 * c->m[i] * sqmod(c->v + 3 * i);
+  }
+  return K * ORC_KIN2EV;
+}
+
+/* Temperature(atoms, tag), properties.h:39-53: the tagged atoms leave the count */
+double orc_temperature_tag(const orc_conf* c, int tag) {
+  if (c->n == 0)
+    return 0.0;
+  long tmp = 0;
+  if (tag)
+    for (long i = 0; i < c->n; ++i)
+      tmp += tagged(c, tag, i);
+  return (2.0 / 3.0) * orc_kinetic_energy_tag(c, tag) / (ORC_KBOLTZMANN * (double)(c->n - tmp));
+}
+
+/* Momentum(atoms, tag), properties.h:64-81: p += Mass() * Velocity() */
+void orc_momentum_tag(const orc_conf* c, int tag, double p[3]) {
   p[0] = p[1] = p[2] = 0.0;
-  for (long i = 0; i < c->n; ++i)
+  for (long i = 0; i < c->n; ++i) {
+    if (tagged(c, tag, i))
+      continue;
     for (int q = 0; q < 3; ++q)
       p[q] += c->v[3 * i + q] * c->m[i];
+  }
+}
+
+/* drand48 as POSIX defines it: X' = (0x5DEECE66D X + 0xB) mod 2^48, the draw
+ * X' / 2^48; srand48(s) sets X = (low 32 bits of s) 2^16 + 0x330E */
+unsigned long long orc_srand48(long seed) {
+  return ((unsigned long long)(unsigned int)seed << 16) | 0x330EULL;
+}
+
+double orc_drand48(unsigned long long* x) {
+  *x = (0x5DEECE66DULL * *x + 0xBULL) & ((1ULL << 48) - 1);
+  return ldexp((double)*x, -48);
+}
+
+/* SimulationEngine::InitVelocities, simulationbuilder.cc:167-185, with
+ * RandomVector (liblpmd/lpmd/vector.h:161-167) and the unit system's
+ * kboltzmann (units.cc:7) */
+void orc_init_velocities(orc_conf* c, unsigned long long* x) {
+  const double kboltzmann = 8.6173422E-05;
+  const double targettemp = 1.0;
+  const double Kpart = (3.0 / 2.0) * kboltzmann * targettemp;
+  double totalp[3] = {0.0, 0.0, 0.0};
+  for (long i = 0; i < c->n; ++i) {
+    const double v2 = 2.0 * Kpart / c->m[i];
+    const double m = sqrt(v2);
+    double w[3];
+    for (int q = 0; q < 3; ++q)
+      w[q] = 2.0 * orc_drand48(x) - 1.0;
+    const double r = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double vel[3] = {m * w[0] / r, m * w[1] / r, m * w[2] / r};
+    for (int q = 0; q < 3; ++q) {
+      totalp[q] += vel[q] * c->m[i];
+      c->v[3 * i + q] = vel[q];
+    }
+  }
+  for (int q = 0; q < 3; ++q)
+    totalp[q] = totalp[q] / (double)c->n;
+  for (long i = 0; i < c->n; ++i)
+    for (int q = 0; q < 3; ++q)
+      c->v[3 * i + q] -= totalp[q] / c->m[i];
+}
+
+/* SimulationEngine::SetSimulationTags, simulationbuilder.cc:84-124, with the
+ * session's pressfactor and ua3togrcm3 ("160217.65000000", "1.66050000") and
+ * OrthogonalCell::Volume (orthogonalcell.h:168); tag-aware when any atom
+ * carries fixedvel or fixedpos.  out[] in the LPMDX_OBS_* layout. */
+void orc_tags(const orc_conf* c, double epot, double virial, const double stress[9], double out[28]) {
+  const double pressfactor = 160217.65, densityfact = 1.6605;
+  const double v = edge_module(c->L[0]) * edge_module(c->L[1]) * edge_module(c->L[2]);
+  int tag = 0;
+  if (c->fix)
+    for (long i = 0; i < c->n; ++i)
+      tag = tag || (c->fix[i] & ORC_FIX_TAGGED);
+  double P[3];
+  orc_momentum_tag(c, tag, P);
+  for (int k = 0; k < 28; ++k)
+    out[k] = 0.0;
+  const double kin = orc_kinetic_energy_tag(c, tag);
+  out[0] = kin;
+  out[1] = orc_temperature_tag(c, tag);
+  out[2] = P[0];
+  out[3] = P[1];
+  out[4] = P[2];
+  out[5] = epot;
+  out[6] = virial;
+  out[7] = (pressfactor / v) * ((2.0 / 3.0) * kin + (1.0 / 3.0) * virial);
+  for (int q = 0; q < 3; ++q)
+    for (int p = 0; p < 3; ++p)
+      out[8 + 3 * q + p] = (pressfactor / v) * stress[3 * q + p];
+  out[17] = epot + kin;
+  out[18] = sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+  out[19] = v;
+  out[20] = v / (double)c->n;
+  out[21] = (double)c->n / v;
+  double total_mass = 0.0; /* Density, properties.h:55-62 */
+  for (long i = 0; i < c->n; ++i)
+    total_mass += c->m[i];
+  out[22] = densityfact * (total_mass / v);
+  out[23] = (pressfactor / v) * (1.0 / 3.0) * virial;
+  out[24] = (pressfactor / v) * (2.0 / 3.0) * kin;
+  for (int q = 0; q < 3; ++q)
+    out[25 + q] = edge_module(c->L[q]);
 }

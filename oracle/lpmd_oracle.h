@@ -34,7 +34,12 @@ typedef struct {
   double* a;     /* AoS [3n] accelerations */
   int* z;        /* [n] atomic numbers */
   double* m;     /* [n] masses */
+  const int* fix; /* [n] fixed-atom flags ORC_FIX_* (the fixedvel / fixedpos tags), or NULL */
 } orc_conf;
+
+#define ORC_FIX_VEL 1    /* carries fixedvel */
+#define ORC_FIX_POS 2    /* carries fixedpos with the value "true" */
+#define ORC_FIX_TAGGED 4 /* carries fixedvel or fixedpos, whatever the value */
 
 typedef struct {
   long i;
@@ -134,6 +139,20 @@ void orc_fitted_inside(const double L[3], const int per[3], double x[3]);
 double orc_kinetic_energy(const orc_conf* c);
 double orc_temperature(const orc_conf* c);
 void orc_momentum(const orc_conf* c, double p[3]);
+/* Potential::AtomEnergy of every atom into e[n] (cell manager updated) */
+int orc_atom_energies(orc_conf* c, orc_cm* cm, const orc_pot* p, double* e);
+/* the tag-aware forms (tag = 1 leaves out ORC_FIX_TAGGED atoms) */
+double orc_kinetic_energy_tag(const orc_conf* c, int tag);
+double orc_temperature_tag(const orc_conf* c, int tag);
+void orc_momentum_tag(const orc_conf* c, int tag, double p[3]);
+/* drand48's generator: state X (48 bits); orc_srand48 returns the state
+ * srand48(seed) sets, orc_drand48 advances it and returns the draw */
+unsigned long long orc_srand48(long seed);
+double orc_drand48(unsigned long long* x);
+/* SimulationEngine::InitVelocities from the generator state *x */
+void orc_init_velocities(orc_conf* c, unsigned long long* x);
+/* SimulationEngine::SetSimulationTags: out[28] in the LPMDX_OBS_* layout */
+void orc_tags(const orc_conf* c, double epot, double virial, const double stress[9], double out[28]);
 
 /* linkedcell's auto subdivision for one axis (plugins/linkedcell.cc:146-165). */
 int orc_lc_auto_n(double edge, double cutoff);

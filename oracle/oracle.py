@@ -45,6 +45,7 @@ KEYWORDS = {
 }
 DEFAULTS = {"fastlj": {"bins": 500.0}, "lennardjonesMod": {"m": 12.0, "n": 6.0, "cohesive": 1.0}}
 
+FIX_VEL, FIX_POS, FIX_TAGGED = 1, 2, 4  # the ORC_FIX_* flags
 FORCEFACTOR = 0.00964853
 KIN2EV = 103.64269
 KBOLTZMANN = 8.6173422e-05
@@ -73,6 +74,7 @@ class _Conf(C.Structure):
         ("a", C.c_void_p),
         ("z", C.c_void_p),
         ("m", C.c_void_p),
+        ("fix", C.c_void_p),
     ]
 
 
@@ -138,6 +140,18 @@ def lib():
         L.orc_temperature.restype = d
         L.orc_temperature.argtypes = [vp]
         L.orc_momentum.argtypes = [vp, vp]
+        L.orc_kinetic_energy_tag.restype = d
+        L.orc_kinetic_energy_tag.argtypes = [vp, i]
+        L.orc_temperature_tag.restype = d
+        L.orc_temperature_tag.argtypes = [vp, i]
+        L.orc_momentum_tag.argtypes = [vp, i, vp]
+        L.orc_srand48.restype = C.c_ulonglong
+        L.orc_srand48.argtypes = [l]
+        L.orc_drand48.restype = d
+        L.orc_drand48.argtypes = [vp]
+        L.orc_init_velocities.argtypes = [vp, vp]
+        L.orc_tags.argtypes = [vp, d, d, vp, vp]
+        L.orc_atom_energies.argtypes = [vp, vp, vp, vp]
         L.orc_lc_auto_n.restype = i
         L.orc_lc_auto_n.argtypes = [d, d]
         L.orc_last_error.restype = C.c_char_p
@@ -174,7 +188,30 @@ class Conf:
             mass = elements.mass_of(self.z)
         self.m = np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,)).copy()
         self.per = tuple(per)
+        # fixed-atom flags (FIX_*) of the fixedvel / fixedpos tags, or None
+        self.fix = None
+        # SimulationEngine's velocitiesSet and the drand48 state of its draws
+        # (an unseeded drand48 starts at 0x1234ABCD330E)
+        self.velocities_set = False
+        self.rng48 = C.c_ulonglong(0x1234ABCD330E)
         self._s = _Conf()
+
+    def set_fixed(self, fixedvel=None, fixedpos=None, fixedpos_true=None):
+        """Boolean masks of the atoms tagged fixedvel / fixedpos (fixedpos_true:
+        which fixedpos tags read "true", default all); no argument clears."""
+        if fixedvel is None and fixedpos is None:
+            self.fix = None
+            return
+        fv = np.zeros(self.n, bool) if fixedvel is None else np.asarray(fixedvel, bool)
+        fp = np.zeros(self.n, bool) if fixedpos is None else np.asarray(fixedpos, bool)
+        ft = fp if fixedpos_true is None else (fp & np.asarray(fixedpos_true, bool))
+        self.fix = np.ascontiguousarray(np.where(fv, FIX_VEL, 0) | np.where(ft, FIX_POS, 0) |
+                                        np.where(fv | fp, FIX_TAGGED, 0), dtype=np.int32)
+
+    @property
+    def tagged(self):
+        """Whether any atom carries fixedvel or fixedpos (the tag-aware sums apply)."""
+        return self.fix is not None and bool(np.any(self.fix & FIX_TAGGED))
 
     @property
     def n(self):
@@ -187,6 +224,7 @@ class Conf:
             s.L[q] = self.L[q]
             s.per[q] = int(self.per[q])
         s.x, s.v, s.a, s.z, s.m = _p(self.x), _p(self.v), _p(self.a), _p(self.z), _p(self.m)
+        s.fix = None if self.fix is None else _p(self.fix)
         return C.byref(s)
 
 
@@ -342,6 +380,15 @@ def update_forces(conf: Conf, cm: CellManager, pots):
     return float(ep[0]), float(vir[0]), S.reshape(3, 3)
 
 
+def atom_energies(conf: Conf, cm: CellManager, pot):
+    """Potential::AtomEnergy of every atom (pairpotential.cc:21-33,
+    metalpotential.cc:47-60), the cell manager updated first."""
+    cm.update(conf)
+    e = np.zeros(conf.n)
+    _check(lib().orc_atom_energies(conf.struct(), cm.h, C.byref(pot.s), _p(e)))
+    return e
+
+
 def md_run(conf: Conf, cm: CellManager, pots, dt: float, steps: int):
     """The lpmd application loop with velocityverlet; returns per-step (epot, ekin, virial)."""
     arr = _pot_array(pots)
@@ -380,15 +427,21 @@ def leapfrog_run(conf: Conf, forces, dt: float, steps: int, after_step=None):
     vh = old.v + old.a * 0.5 * dt
     ep = np.zeros(steps)
     ek = np.zeros(steps)
+    # OneStepIntegrator::Advance's tags (onestepintegrator.cc:18-38): fixedvel
+    # atoms get a = 0 after the force call, fixedpos ("true") atoms are not advanced
+    fv = None if conf.fix is None else (conf.fix & FIX_VEL) != 0
+    mv = slice(None) if conf.fix is None else (conf.fix & FIX_POS) == 0
     for s in range(steps):
         ep[s] = forces(conf)
+        if fv is not None:
+            conf.a[fv] = 0.0
         vn = vh + conf.a * dt
-        conf.x[:] = fitted_inside_all(conf.x + vn * dt, L)
-        conf.v[:] = 0.5 * (vn + vh)
-        vh = vn
+        conf.x[mv] = fitted_inside_all(conf.x + vn * dt, L)[mv]
+        conf.v[mv] = (0.5 * (vn + vh))[mv]
+        vh[mv] = vn[mv]
         if after_step:
             after_step(s + 1, conf)  # the modifiers of step s + 1 (lpmd/lpmd.cc:92-93)
-        ek[s] = kinetic_energy(conf)
+        ek[s] = kinetic_energy_tag(conf, conf.tagged)
     return ep, ek
 
 
@@ -406,19 +459,30 @@ def beeman_run(conf: Conf, forces, dt: float, steps: int, after_step=None):
     old = Conf(fitted_inside_all(conf.x - conf.v * dt, L), L, z=conf.z, mass=conf.m, v=conf.v - conf.a * dt)
     forces(old)
     a_old = old.a.copy()
+    aux = np.zeros_like(a_old)
     ep = np.zeros(steps)
     ek = np.zeros(steps)
+    # TwoStepIntegrator::Advance's tags (twostepintegrator.cc:17-65): fixedvel
+    # atoms get a = 0 after each force call, fixedpos ("true") atoms skip
+    # AdvancePosition and AdvanceVelocity (so their history stays as it was)
+    fv = None if conf.fix is None else (conf.fix & FIX_VEL) != 0
+    mv = np.ones(conf.n, bool) if conf.fix is None else (conf.fix & FIX_POS) == 0
     for s in range(steps):
         forces(conf)
-        aux = a_old
+        if fv is not None:
+            conf.a[fv] = 0.0
+        aux[mv] = a_old[mv]
         xn = conf.x + conf.v * dt + (2.0 / 3.0) * conf.a * dt * dt - (1.0 / 6.0) * a_old * dt * dt
-        a_old = conf.a.copy()
-        conf.x[:] = fitted_inside_all(xn, L)
+        a_old[mv] = conf.a[mv]
+        conf.x[mv] = fitted_inside_all(xn, L)[mv]
         ep[s] = forces(conf)
-        conf.v[:] = conf.v + (1.0 / 3.0) * conf.a * dt + (5.0 / 6.0) * a_old * dt - (1.0 / 6.0) * aux * dt
+        if fv is not None:
+            conf.a[fv] = 0.0
+        vn = conf.v + (1.0 / 3.0) * conf.a * dt + (5.0 / 6.0) * a_old * dt - (1.0 / 6.0) * aux * dt
+        conf.v[mv] = vn[mv]
         if after_step:
             after_step(s + 1, conf)
-        ek[s] = kinetic_energy(conf)
+        ek[s] = kinetic_energy_tag(conf, conf.tagged)
     return ep, ek
 
 
@@ -459,12 +523,47 @@ def temperature(conf: Conf):
     return lib().orc_temperature(conf.struct())
 
 
-def set_temperature(conf: Conf, temp: float):
-    """SimulationEngine::SetTemperature (liblpmd/simulation/simulationbuilder.cc:53-71)
-    without fixed atoms: every velocity times sqrt(temp / T_now), T_now the
-    Temperature of the sequential kinetic sum (properties.h:22-53)."""
-    ti = temperature(conf)
-    conf.v *= np.sqrt(temp / ti)
+def set_temperature(conf: Conf, temp: float, tag=None):
+    """SimulationEngine::SetTemperature(temp, tag) (liblpmd/simulation/simulationbuilder.cc:53-71):
+    InitVelocities first while no velocities were set; then every velocity
+    times sqrt(temp / T_now), T_now the Temperature of the sequential kinetic
+    sum (properties.h:22-53).  tag (default: whether any atom is tagged, as
+    tempscaling.cc:70 passes it): tagged atoms are neither counted nor scaled."""
+    if not conf.velocities_set:
+        init_velocities(conf)
+    tag = conf.tagged if tag is None else tag
+    ti = lib().orc_temperature_tag(conf.struct(), int(tag))
+    f = np.sqrt(temp / ti)
+    if tag and conf.fix is not None:
+        keep = (conf.fix & FIX_TAGGED) != 0
+        conf.v[~keep] *= f
+    else:
+        conf.v *= f
+
+
+def seed48(conf: Conf, seed: int):
+    """srand48(seed) for the configuration's drand48 draws."""
+    conf.rng48 = C.c_ulonglong(lib().orc_srand48(seed))
+
+
+def init_velocities(conf: Conf):
+    """SimulationEngine::InitVelocities (simulationbuilder.cc:167-185) from
+    the configuration's drand48 stream; marks the velocities as set."""
+    lib().orc_init_velocities(conf.struct(), C.byref(conf.rng48))
+    conf.velocities_set = True
+
+
+def drand48(state: C.c_ulonglong) -> float:
+    return lib().orc_drand48(C.byref(state))
+
+
+def tags(conf: Conf, epot: float, virial: float, stress) -> np.ndarray:
+    """SetSimulationTags (simulationbuilder.cc:84-124): out[28] in the
+    lpmdx_observables layout."""
+    out = np.zeros(28)
+    S = np.ascontiguousarray(np.asarray(stress, dtype=np.float64).reshape(9))
+    lib().orc_tags(conf.struct(), float(epot), float(virial), _p(S), _p(out))
+    return out
 
 
 class Berendsen:
@@ -520,10 +619,21 @@ class Stepper:
         return frm + float(k - self.start) / float(self.end - self.start) * (to - frm)
 
 
-def momentum(conf: Conf):
+def momentum(conf: Conf, tag: bool = False):
+    """Momentum(atoms, tag), properties.h:64-81."""
     p = np.zeros(3)
-    lib().orc_momentum(conf.struct(), _p(p))
+    lib().orc_momentum_tag(conf.struct(), int(tag), _p(p))
     return p
+
+
+def kinetic_energy_tag(conf: Conf, tag: bool):
+    """KineticEnergy(atoms, tag), properties.h:22-37."""
+    return lib().orc_kinetic_energy_tag(conf.struct(), int(tag))
+
+
+def temperature_tag(conf: Conf, tag: bool):
+    """Temperature(atoms, tag), properties.h:39-53."""
+    return lib().orc_temperature_tag(conf.struct(), int(tag))
 
 
 def lc_auto_n(edge: float, cutoff: float) -> int:

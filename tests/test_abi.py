@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = lpmd2_amd.lib()
     for name in declared():
         assert hasattr(lib, name), name
-    assert lib.lpmdx_abi_version() == 1
+    assert lib.lpmdx_abi_version() == 2
 
 
 def test_no_device_fails_loudly():

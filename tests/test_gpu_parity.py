@@ -411,10 +411,39 @@ def test_set_temperature_matches_oracle():
     O.set_temperature(conf, 512.0)
     np.testing.assert_allclose(vg, conf.v, rtol=1e-13, atol=0)
     assert abs(O.temperature(O.Conf(x, L, z=29, v=vg)) - 512.0) <= 1e-10 * 512.0
-    # a system at rest stays at rest
+    # the first call drew the velocities (InitVelocities): the given ones are gone
+    assert np.abs(vg - v).max() > 1e-4
+    # with the velocities marked as set, SetTemperature only scales them
+    s1 = E.System(x, L, 29, v=v)
+    s1.velocities_set(True)
+    s1.set_temperature(512.0)
+    c1 = O.Conf(x, L, z=29, v=v)
+    c1.velocities_set = True
+    O.set_temperature(c1, 512.0)
+    np.testing.assert_allclose(s1.get_state()[1], c1.v, rtol=1e-13, atol=0)
+    # a system at rest whose velocities count as set stays at rest
     s0 = E.System(x, L, 29)
+    s0.velocities_set(True)
     s0.set_temperature(300.0)
     assert np.all(s0.get_state()[1] == 0.0)
+
+
+def test_init_velocities_match_oracle():
+    """InitVelocities (simulationbuilder.cc:167-185) from a seeded drand48
+    stream: drawn on the host in the reference's order, so bit-identical to
+    the oracle's restatement; a second draw continues the stream."""
+    say("init velocities")
+    x, L = jittered(4, 3.61, 0.02, 23)
+    z = np.where(np.arange(x.shape[0]) % 4 == 0, 28, 29)
+    s = E.System(x, L, z)
+    conf = O.Conf(x, L, z=z)
+    for seed in (1, 987654321):
+        s.seed48(seed)
+        O.seed48(conf, seed)
+        for _ in range(2):
+            s.init_velocities()
+            O.init_velocities(conf)
+            np.testing.assert_array_equal(s.get_state()[1], conf.v)
 
 
 @pytest.mark.parametrize("mod", ["tempscaling", "berendsen"])
@@ -542,3 +571,174 @@ def test_error_paths():
     with pytest.raises(E.LpmdxError) as e:
         s.append_potential("nosuchpotential", "")
     assert e.value.code == E.EINVAL
+
+
+# --------------------------------------------------------------- fixed atoms, tags, per-atom energies
+def fixed_masks(n, seed):
+    """fixedvel on every 7th atom, fixedpos "true" on every 11th, a fixedpos
+    tag reading "false" on every 13th (tagged, still integrated)."""
+    k = np.arange(n)
+    fv = (k % 7) == 3
+    fp = ((k % 11) == 5) | ((k % 13) == 6)
+    ft = (k % 11) == 5
+    return fv, fp, ft
+
+
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
+@pytest.mark.parametrize("scheme", ["velocityverlet", "beeman", "leapfrog"])
+def test_fixed_atoms_match_oracle(scheme, cm):
+    """The fixedvel / fixedpos tags inside the device step loops (a = 0 for
+    fixedvel atoms after each force evaluation of a step; fixedpos "true"
+    atoms neither move nor change velocity) against the oracle's
+    TwoStepIntegrator / OneStepIntegrator restatements; with velocity Verlet
+    a tempscaling ramp runs too, which with tags draws no velocities for and
+    scales none of the tagged atoms and counts only the untagged ones."""
+    say("fixed atoms", scheme, cm)
+    x, L = jittered(4, 5.26, 0.05, 31)
+    n = x.shape[0]
+    m = np.full(n, 39.948)
+    v = E.maxwell_boltzmann(m, 100.0, seed=31)
+    dt, steps = 2.0, 30
+    fv, fp, ft = fixed_masks(n, 31)
+    s = E.System(x, L, 18, v=v)
+    if cm == "linkedcell":
+        s.set_cellmanager("linkedcell", "cutoff 8.5 mode auto")
+        om = O.CellManager("linkedcell", 8.5, mode="auto")
+    else:
+        s.set_cellmanager("verletlist", "cutoff 8.5 maxneighbors 300 each 5 skin 0.1")
+        om = O.CellManager("minimumimage", 8.5)
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    s.set_fixed(fixedvel=fv, fixedpos=fp, fixedpos_true=ft)
+    s.velocities_set(True)
+    pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
+    conf = O.Conf(x, L, v=v)
+    conf.set_fixed(fixedvel=fv, fixedpos=fp, fixedpos_true=ft)
+    conf.velocities_set = True
+    forces = lambda c: O.update_forces(c, om, pots)[0]
+    if scheme == "velocityverlet":
+        s.tempscaling(80.0, 120.0, start=0, end=steps, each=5)
+        st = O.Stepper(0, steps, 5)
+        ep, ek = s.velocityverlet(dt, steps, obs_every=10)
+        e0 = np.zeros(steps)
+        k0 = np.zeros(steps)
+        hold = (conf.fix & O.FIX_POS) != 0
+        for k in range(1, steps + 1):
+            forces(conf)
+            conf.a[fv] = 0.0
+            x_keep, v_keep = conf.x[hold].copy(), conf.v[hold].copy()
+            aold = O.vv_position(conf, dt)
+            conf.x[hold] = x_keep
+            e0[k - 1] = forces(conf)
+            conf.a[fv] = 0.0
+            O.vv_velocity(conf, dt, aold)
+            conf.v[hold] = v_keep
+            if st.active(k):
+                O.set_temperature(conf, st.value(k, 80.0, 120.0))
+            k0[k - 1] = O.kinetic_energy_tag(conf, True)
+    elif scheme == "beeman":
+        s.beeman_init(dt)
+        ep, ek = s.beeman(dt, steps, obs_every=10)
+        e0, k0 = O.beeman_run(conf, forces, dt, steps)
+    else:
+        s.leapfrog_init(dt)
+        ep, ek = s.leapfrog(dt, steps, obs_every=10)
+        e0, k0 = O.leapfrog_run(conf, forces, dt, steps)
+    xg, vg, ag = s.get_state()
+    dx = xg - conf.x
+    dx -= L * np.round(dx / L)
+    say(f"  max |dx| {np.abs(dx).max():.2e} |dv| {np.abs(vg - conf.v).max():.2e}")
+    hold = ft
+    np.testing.assert_array_equal(xg[hold], x[hold])
+    np.testing.assert_array_equal(vg[hold], v[hold])
+    assert np.all(ag[fv] == 0.0)
+    assert np.abs(dx).max() < 1e-7
+    assert np.abs(vg - conf.v).max() < 2e-8
+    np.testing.assert_allclose(ep, e0[9::10], rtol=1e-6)
+    np.testing.assert_allclose(ek, k0[9::10], rtol=1e-6)
+    if scheme == "velocityverlet":
+        # the tag-aware temperature after the last step is the ramp's end
+        assert abs(s.tags()["temperature"] - 120.0) <= 1e-9 * 120.0
+
+
+@pytest.mark.parametrize("tagged", [False, True])
+def test_observables_match_oracle(tagged):
+    """SetSimulationTags on the device (lpmdx_observables) against the
+    oracle's restatement after one force evaluation: kinetic energy,
+    temperature and momentum (tag-aware when atoms carry fixedvel or
+    fixedpos), potential energy, virial, pressure and its parts, the stress
+    tags, total energy, |p|, volume, densities and cell edges."""
+    say("observables", tagged)
+    x, L = jittered(5, 3.61, 0.02, 41)
+    L = L * np.array([1.0, 1.0, 1.0])
+    n = x.shape[0]
+    m = np.full(n, 63.546)
+    v = E.maxwell_boltzmann(m, 450.0, seed=41) + 1e-3  # a net momentum
+    sc = "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22"
+    s = E.System(x, L, 29, v=v)
+    s.set_cellmanager("verletlist", "cutoff 7.22 maxneighbors 300 each 5")
+    s.append_potential("suttonchen", sc)
+    conf = O.Conf(x, L, z=29, v=v)
+    if tagged:
+        fv, fp, ft = fixed_masks(n, 41)
+        s.set_fixed(fixedvel=fv, fixedpos=fp, fixedpos_true=ft)
+        conf.set_fixed(fixedvel=fv, fixedpos=fp, fixedpos_true=ft)
+    eg, wg, Sg = s.update_forces()
+    out = s.observables()
+    e0, w0, S0 = O.update_forces(conf, O.CellManager("verletlist", 7.22, maxneighbors=300, each=5),
+                                 [O.Pot("suttonchen", e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=7.22)])
+    # the energies, virial and stress of the FP32 pair arithmetic against FP64:
+    # the virial and stress are sums of pair terms that largely cancel near
+    # the lattice constant, so their bar is set by the energy scale
+    esc = abs(e0)
+    assert abs(eg - e0) <= 1e-6 * esc
+    assert abs(wg - w0) <= 1e-5 * max(abs(w0), esc)
+    assert np.abs(Sg - S0).max() <= 1e-5 * max(np.abs(S0).max(), esc)
+    # every tag is the SetSimulationTags formula applied to this state and the
+    # engine's own energies (sums in another order: 1e-12)
+    ref = O.tags(conf, eg, wg, Sg)
+    exact = [19, 20, 21, 22, 25, 26, 27]  # from the box and the masses
+    np.testing.assert_array_equal(out[exact], ref[exact])
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-18)
+    if tagged:
+        untagged = conf.fix & O.FIX_TAGGED == 0
+        assert abs(out[1] - O.temperature_tag(conf, True)) <= 1e-12 * out[1]
+        assert untagged.sum() < n
+
+
+@pytest.mark.parametrize("case", ["lennardjones", "suttonchen", "rocksalt"])
+def test_atom_energies_match_oracle(case):
+    """Potential::AtomEnergy on the device (lpmdx_atom_energies) against the
+    oracle's restatement: pair energies over each atom's full list, plus
+    F(rho_i) for EAM; with species rules for the rocksalt Buckingham terms."""
+    from lpmd2_amd import workloads as W
+
+    say("atom energies", case)
+    if case == "lennardjones":
+        x, L = jittered(4, 5.26, 0.05, 51)
+        z = np.full(x.shape[0], 18)
+        pots = [("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5", -1, -1)]
+        cm = ("linkedcell", "cutoff 8.5 mode auto")
+    elif case == "suttonchen":
+        x, L = jittered(5, 3.61, 0.02, 52)
+        z = np.full(x.shape[0], 29)
+        pots = [("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22", -1, -1)]
+        cm = ("verletlist", "cutoff 7.22 maxneighbors 300 each 5")
+    else:
+        x, L, z = W.rocksalt(3, 3, 3, 5.64)
+        x = np.mod(x + np.random.default_rng(53).normal(0, 0.03, x.shape), L)
+        pots = [("buckingham", a, s1, s2) for a, s1, s2 in W.BUCKINGHAM_NACL]
+        cm = ("verletlist", "cutoff 8.0 maxneighbors 300 each 5")
+    s = E.System(x, L, z)
+    s.set_cellmanager(*cm)
+    for p in pots:
+        s.append_potential(*p)
+    conf = O.Conf(x, L, z=z)
+    ocm = oracle_cm(*cm)
+    for k, (name, args, s1, s2) in enumerate(pots):
+        kw = dict(zip(args.split()[0::2], (float(q) for q in args.split()[1::2])))
+        pot = O.Pot(name, s1, s2, **kw)
+        eg = s.atom_energies(k)
+        e0 = O.atom_energies(conf, ocm, pot)
+        err = rel_rms(eg, e0)
+        say(f"  term {k}: rel rms {err:.2e}")
+        assert err <= 1e-6

@@ -251,6 +251,7 @@ def test_set_temperature_closed_form():
     m = 39.948
     t0 = (2.0 / 3.0) * (0.5 * m * 5e-4 * O.KIN2EV) / (2 * O.KBOLTZMANN)
     assert abs(O.temperature(conf) - t0) <= 1e-12 * t0
+    conf.velocities_set = True  # as after Restore: no InitVelocities draw
     O.set_temperature(conf, 300.0)
     f = np.sqrt(300.0 / t0)
     np.testing.assert_allclose(conf.v, v * f, rtol=1e-14)
@@ -319,6 +320,218 @@ def test_properties_known_answers():
     conf = O.Conf(np.zeros((108, 3)), [10.0] * 3, z=18, v=v)
     assert abs(O.kinetic_energy(conf) - 0.69800472) < 1e-5
     assert abs(O.temperature(conf) - 50.0) < 1e-5
+    # momentum: 108 argon atoms moving along x, p = 108 m v0 e1
+    p = O.momentum(conf)
+    assert abs(p[0] - 108 * 39.948 * 0.0017669141) <= 1e-12 * p[0] and p[1] == 0.0 and p[2] == 0.0
+
+
+def test_tagged_kinetic_energy_known_answers():
+    """liblpmd/tests/propertiestest.unit:24-49: four H atoms with v = (1,1,1),
+    (.5,.5,.5), (2,2,2), (4,4,4); tagging atom 0 fixedvel (then fixedpos too)
+    leaves it out of the tag-aware kinetic energy, tagging atom 1 fixedpos as
+    well leaves both out; the plain sum counts everyone."""
+    v = np.array([[1.0] * 3, [0.5] * 3, [2.0] * 3, [4.0] * 3])
+    conf = O.Conf(np.zeros((4, 3)), [10.0] * 3, z=1, v=v)
+    m = 1.00794
+    ke = lambda rows: sum(0.5 * (m * float(v[i] @ v[i])) for i in rows) * O.KIN2EV
+    assert abs(O.kinetic_energy(conf) - ke(range(4))) < 1e-5
+    conf.set_fixed(fixedvel=[1, 0, 0, 0])
+    assert abs(O.kinetic_energy_tag(conf, conf.tagged) - ke(range(1, 4))) < 1e-5
+    conf.set_fixed(fixedvel=[1, 0, 0, 0], fixedpos=[1, 0, 0, 0])
+    assert abs(O.kinetic_energy_tag(conf, conf.tagged) - ke(range(1, 4))) < 1e-5
+    conf.set_fixed(fixedvel=[1, 0, 0, 0], fixedpos=[1, 1, 0, 0])
+    assert abs(O.kinetic_energy_tag(conf, conf.tagged) - ke(range(2, 4))) < 1e-5
+    # the temperature counts the untagged atoms only, the momentum sums them
+    t = (2.0 / 3.0) * ke(range(2, 4)) / (O.KBOLTZMANN * 2)
+    assert abs(O.temperature_tag(conf, True) - t) <= 1e-12 * t
+    np.testing.assert_array_equal(O.momentum(conf, True), m * (v[2] + v[3]))
+    # a fixedpos tag whose value is not "true" still counts as tagged
+    conf.set_fixed(fixedpos=[0, 1, 0, 0], fixedpos_true=[0, 0, 0, 0])
+    assert conf.fix[1] == O.FIX_TAGGED
+    assert abs(O.kinetic_energy_tag(conf, True) - ke((0, 2, 3))) < 1e-12
+
+
+def test_drand48_matches_the_c_library():
+    """The oracle's drand48 restatement against the C library's srand48 /
+    drand48 on this host: the same 48-bit state sequence, draw for draw."""
+    import ctypes
+
+    libc = ctypes.CDLL(None)
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    for seed in (0, 1, 12345, -7, 2**40 + 3):
+        libc.srand48(seed)
+        st = ctypes.c_ulonglong(O.lib().orc_srand48(seed))
+        for _ in range(200):
+            assert O.drand48(st) == libc.drand48()
+
+
+def test_init_velocities_from_the_c_library_draws():
+    """InitVelocities (simulationbuilder.cc:167-185): every atom's velocity
+    before the momentum correction is 2 drand48() - 1 per axis (three draws
+    per atom, in atom order) scaled to the speed sqrt(2 (3/2) kB / m); the
+    correction subtracts the mean momentum over the mass, so the total
+    momentum vanishes.  Checked against the C library's own drand48."""
+    import ctypes
+
+    libc = ctypes.CDLL(None)
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    n = 57
+    z = np.where(np.arange(n) % 3 == 0, 29, 18)
+    conf = O.Conf(np.zeros((n, 3)), [10.0] * 3, z=z)
+    O.seed48(conf, 4242)
+    O.init_velocities(conf)
+    assert conf.velocities_set
+    libc.srand48(4242)
+    w = np.array([[2.0 * libc.drand48() - 1.0 for _ in range(3)] for _ in range(n)])
+    speed = np.sqrt(2.0 * (1.5 * 8.6173422e-05) / conf.m)
+    raw = speed[:, None] * w / np.linalg.norm(w, axis=1)[:, None]
+    tp = (raw * conf.m[:, None]).sum(axis=0) / n
+    np.testing.assert_allclose(conf.v, raw - tp / conf.m[:, None], rtol=0, atol=1e-17)
+    p = O.momentum(conf)
+    assert np.abs(p).max() < 1e-15
+    # SetTemperature on a fresh configuration draws first, then scales to T
+    c2 = O.Conf(np.zeros((n, 3)), [10.0] * 3, z=z, v=np.ones((n, 3)))
+    O.seed48(c2, 4242)
+    O.set_temperature(c2, 300.0)
+    f = np.sqrt(300.0 / O.temperature(conf))
+    np.testing.assert_allclose(c2.v, conf.v * f, rtol=1e-14, atol=0)
+
+
+def test_tags_closed_forms():
+    """SetSimulationTags (simulationbuilder.cc:84-124) on a known state: the
+    volume and cell edges of the box, N / V, the density of the masses in
+    g/cm^3 (ua3togrcm3 1.6605), the pressure split into its kinetic and
+    virial parts, the stress tags pressfactor / V times the stress matrix."""
+    v = np.zeros((108, 3))
+    v[:, 0] = 0.0017669141
+    L = [10.0, 12.0, 15.0]
+    conf = O.Conf(np.zeros((108, 3)), L, z=18, v=v)
+    S = np.arange(9.0).reshape(3, 3) - 4.0
+    out = O.tags(conf, -3.5, 2.25, S)
+    V = 1800.0
+    assert out[19] == V and out[20] == V / 108 and out[21] == 108 / V
+    np.testing.assert_array_equal(out[25:28], L)
+    assert abs(out[22] - 1.6605 * 108 * 39.948 / V) <= 1e-15 * out[22]
+    ke = O.kinetic_energy(conf)
+    assert out[0] == ke and out[17] == -3.5 + ke and out[5] == -3.5 and out[6] == 2.25
+    assert abs(out[7] - (out[23] + out[24])) <= 1e-12 * abs(out[7])
+    assert abs(out[24] - 160217.65 / V * (2.0 / 3.0) * ke) <= 1e-15 * out[24]
+    assert abs(out[23] - 160217.65 / V * 2.25 / 3.0) <= 1e-15 * out[23]
+    np.testing.assert_allclose(out[8:17], 160217.65 / V * S.reshape(9), rtol=1e-15)
+    assert out[18] == np.linalg.norm(out[2:5])
+
+
+def test_md_fixed_atoms_closed_forms():
+    """TwoStepIntegrator::Advance with tags (twostepintegrator.cc:12-68): a
+    fixedpos ("true") atom stays exactly where it was with its velocity; a
+    fixedvel atom has a = 0 and moves in free flight, x' = FittedInside(x + v dt);
+    the untagged atoms, which no potential couples to the tagged species here,
+    follow the same trajectory as a run without the tagged atoms."""
+    x, L = fcc(3, 5.26)
+    rng = np.random.default_rng(5)
+    x = np.mod(x + rng.normal(0, 0.05, x.shape), L)
+    n = x.shape[0]
+    z = np.full(n, 18)
+    tag_idx = np.array([3, 17, 40, 77])
+    z[tag_idx] = 10  # neon, outside every potential's species rule (18 + 18)
+    v = rng.normal(0, 0.003, (n, 3))
+    lj = dict(sigma=3.41, epsilon=0.0103, cutoff=8.5)
+    conf = O.Conf(x, L, z=z, v=v)
+    fv = np.zeros(n, bool)
+    fp = np.zeros(n, bool)
+    fv[[3, 40]] = True
+    fp[[17, 77]] = True
+    conf.set_fixed(fixedvel=fv, fixedpos=fp)
+    cm = O.CellManager("linkedcell", 8.5, 3, 3, 3, mode="noauto")
+    dt, steps = 2.0, 25
+    O.md_run(conf, cm, [O.Pot("lennardjones", 18, 18, **lj)], dt, steps)
+    np.testing.assert_array_equal(conf.x[fp], x[fp])
+    np.testing.assert_array_equal(conf.v[fp], v[fp])
+    xf = x[fv].copy()
+    for _ in range(steps):
+        xf = O.fitted_inside_all(xf + v[fv] * dt + (0.0 * 0.5) * dt * dt, L)
+    np.testing.assert_array_equal(conf.x[fv], xf)
+    np.testing.assert_array_equal(conf.v[fv], v[fv])
+    assert np.all(conf.a[fv] == 0.0)
+    keep = z == 18
+    alone = O.Conf(x[keep], L, z=18, v=v[keep])
+    O.md_run(alone, O.CellManager("linkedcell", 8.5, 3, 3, 3, mode="noauto"),
+             [O.Pot("lennardjones", 18, 18, **lj)], dt, steps)
+    np.testing.assert_allclose(conf.x[keep], alone.x, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(conf.v[keep], alone.v, rtol=0, atol=1e-13)
+
+
+def test_atom_energies_closed_forms():
+    """AtomEnergy (pairpotential.cc:21-33, metalpotential.cc:47-60): a pair
+    potential's per-atom energies count each pair from both ends, so they sum
+    to twice UpdateForces' energy; an EAM potential's add F(rho_i) once per
+    atom, so they sum to the energy plus the pair part again.  Two atoms: each
+    has V(r) (plus F(rho(r)) for EAM)."""
+    x, L = fcc(3, 5.26)
+    x = np.mod(x + np.random.default_rng(8).normal(0, 0.05, x.shape), L)
+    lj = O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)
+    conf = O.Conf(x, L, z=18)
+    cm = O.CellManager("linkedcell", 8.5, 3, 3, 3, mode="noauto")
+    e, _, _ = O.update_forces(conf, cm, [lj])
+    ea = O.atom_energies(conf, cm, lj)
+    assert abs(ea.sum() - 2 * e) <= 1e-12 * abs(e)
+    xc, Lc = fcc(3, 3.61)
+    xc = np.mod(xc + np.random.default_rng(9).normal(0, 0.02, xc.shape), Lc)
+    sc = O.Pot("suttonchen", e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=5.0)
+    c2 = O.Conf(xc, Lc, z=29)
+    cm2 = O.CellManager("verletlist", 5.0, maxneighbors=100, each=5)
+    e2, _, _ = O.update_forces(c2, cm2, [sc])
+    ea2 = O.atom_energies(c2, cm2, sc)
+    # sum of F(rho_i) = sum(ea2) - 2 pair; energy = pair + sum F  =>  sum(ea2) = 2 e2 - sum F
+    cm2.update(c2)
+    counts, js, r2, _ = cm2.lists(c2, True, 5.0)
+    rho = np.add.reduceat(sc.rhoij(np.sqrt(r2)), np.concatenate([[0], np.cumsum(counts)[:-1]]))
+    fsum = sc.embed(rho).sum()
+    assert abs(ea2.sum() - (2 * e2 - fsum)) <= 1e-11 * abs(e2)
+    two = O.Conf(np.array([[1.0, 1.0, 1.0], [3.6, 1.0, 1.0]]), [20.0] * 3, z=18)
+    ea3 = O.atom_energies(two, O.CellManager("verletlist", 8.5, maxneighbors=10, each=5), lj)
+    np.testing.assert_array_equal(ea3, lj.pair_energy([2.6, 2.6]))
+
+
+def test_metal_species_rule_closed_forms():
+    """MetalPotential::UpdateForces with a species rule (metalpotential.cc:121-169):
+    rho sums every neighbour whatever its species, while the pair energy, the
+    pair force and the many-body force enter only for pairs the rule applies
+    to.  Two atoms of species A and B under a rule for (A, A): E = F(rho(r))
+    for each atom, and no force at all.  Three atoms A-A-B in a line: the
+    embedding energies see all neighbours; only the A-A pair adds V, and the
+    forces on the A atoms are PairForce + ManyBodies(rho_1, rho_2) along their
+    own bond only (the B atom feels nothing)."""
+    for name, kw in (("suttonchen", dict(e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=7.22)),
+                     ("gupta", dict(A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=7.22))):
+        pot = O.Pot(name, 29, 29, **kw)
+        r = 2.7
+        L = [30.0] * 3
+        x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]])
+        conf = O.Conf(x, L, z=[29, 47])
+        cm = O.CellManager("verletlist", 7.22, maxneighbors=10, each=5)
+        e, w, _ = O.update_forces(conf, cm, [pot])
+        rho = pot.rhoij(r)[0]
+        assert abs(e - 2 * pot.embed(rho)[0]) <= 1e-14 * abs(e)
+        assert np.all(conf.a == 0.0) and w == 0.0
+        # A-A-B along x
+        r1, r2 = 2.6, 3.1
+        x3 = np.array([[5.0, 5.0, 5.0], [5.0 + r1, 5.0, 5.0], [5.0 + r1 + r2, 5.0, 5.0]])
+        c3 = O.Conf(x3, L, z=[29, 29, 47])
+        e3, _, _ = O.update_forces(c3, O.CellManager("verletlist", 7.22, maxneighbors=10, each=5), [pot])
+        rho0 = pot.rhoij([r1, r1 + r2]).sum()
+        rho1 = pot.rhoij([r1, r2]).sum()
+        rho2 = pot.rhoij([r1 + r2, r2]).sum()
+        e_ref = pot.pair_energy(r1)[0] + sum(pot.embed(q)[0] for q in (rho0, rho1, rho2))
+        assert abs(e3 - e_ref) <= 1e-13 * abs(e_ref)
+        pf, mb = pot.metal_forces(np.array([[r1, 0.0, 0.0]]), [rho0], [rho1])
+        f0 = pf[0] + mb[0]  # force on atom 0 from its bond with atom 1 (rij = x1 - x0)
+        m = 63.546
+        np.testing.assert_allclose(c3.a[0], f0 * O.FORCEFACTOR / m, rtol=1e-13)
+        np.testing.assert_allclose(c3.a[1], -f0 * O.FORCEFACTOR / m, rtol=1e-13)
+        assert np.all(c3.a[2] == 0.0)
 
 
 def test_fitted_inside_known_answers():
