@@ -260,6 +260,17 @@ LPMDX_API int lpmdx_berendsen(lpmdx_system* sys, double from, double to, double 
  * last force evaluation. */
 LPMDX_API int lpmdx_observables(lpmdx_system* sys, double out[LPMDX_NOBS]);
 
+/* lpmdx_observables without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * kinetic partials and their sums live in a static device array of the
+ * library (512 KiB + 64 B per GPU, held until the process ends), so systems of
+ * at most 4194304 atoms are accepted (more: LPMDX_EINVAL); out[] is
+ * bit-identical to lpmdx_observables'.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (several threads, or systems on
+ * different streams) race and must be serialised by the caller. */
+LPMDX_API int lpmdx_observables_noalloc(lpmdx_system* sys, double out[LPMDX_NOBS]);
+
 /* Counters of the system's engine: [0] list builds, [1] force evaluations,
  * [2] steps, [3] list capacity per atom, [4] largest list, [5] lanes per atom,
  * [6] brick edge in cells of the LDS-staged force kernels (0: unstaged),

@@ -4,6 +4,8 @@
 // interface it serves.
 #include "engine_system.hpp"
 
+#include <mutex>
+
 namespace lpmdx {
 thread_local std::string g_err;
 }
@@ -64,6 +66,7 @@ int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const i
         fail(LPMDX_EINVAL, "box edges must be positive");
     ensure_device();
     auto S = std::make_unique<lpmdx_system>();
+    hipchk(hipGetDevice(&S->device), "hipGetDevice");
     S->n = n;
     for (int q = 0; q < 3; ++q) {
       S->box.L[q] = box[q];
@@ -530,18 +533,13 @@ int lpmdx_berendsen(lpmdx_system* S, double from, double to, double tau, long st
   });
 }
 
-int lpmdx_observables(lpmdx_system* S, double out[LPMDX_NOBS]) {
-  return guard([&] {
-    need(S, "system");
-    need(out, "out");
-    for (int k = 0; k < LPMDX_NOBS; ++k)
-      out[k] = 0.0;
-    if (S->n == 0)
-      return;
-    // tag-aware sums when some atom carries fixedvel or fixedpos
-    // (SetSimulationTags, simulationbuilder.cc:84-124)
-    double p[3];
-    const double ke = S->kinetic(p, true);
+namespace {
+
+// SetSimulationTags (simulationbuilder.cc:84-124) from the tag-aware kinetic
+// energy ke and momentum p of the system's current velocities and the
+// energies of its last force evaluation.
+void fill_observables(const lpmdx_system* S, double ke, const double p[3], double out[LPMDX_NOBS]) {
+  {
     const double vol = S->ll[0] * S->ll[1] * S->ll[2]; // OrthogonalCell::Volume (orthogonalcell.h:168)
     out[LPMDX_OBS_KINETIC] = ke;
     out[LPMDX_OBS_TEMP] = (2.0 / 3.0) * ke / (kBoltzmann * double(S->n - S->n_tagged));
@@ -567,6 +565,83 @@ int lpmdx_observables(lpmdx_system* S, double out[LPMDX_NOBS]) {
       for (int k = 0; k < 9; ++k)
         out[LPMDX_OBS_STRESS + k] = (kPressFactor / vol) * S->last_e[2 + k];
     }
+  }
+}
+
+// Scratch of lpmdx_observables_noalloc: the per-block kinetic partials (four
+// per block of kBlock atoms) and their reduced sums, for up to
+// kNoAllocAtoms atoms.  Each GPU holds its own copy; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device.
+constexpr int kMaxDevices = 64;
+constexpr long kNoAllocAtoms = 1L << 22;
+constexpr long kNoAllocRows = kNoAllocAtoms / kBlock;
+static __device__ __align__(256) double scratch[4 * kNoAllocRows + 8];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// The device address of `scratch` on `device` (looked up once per device,
+// with that device current).
+double* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    fail(LPMDX_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    hipchk(hipGetDevice(&cur), "hipGetDevice");
+    hipchk(hipSetDevice(device), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    hipchk(hipSetDevice(cur), "hipSetDevice");
+    hipchk(rc, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+} // namespace
+
+int lpmdx_observables(lpmdx_system* S, double out[LPMDX_NOBS]) {
+  return guard([&] {
+    need(S, "system");
+    need(out, "out");
+    for (int k = 0; k < LPMDX_NOBS; ++k)
+      out[k] = 0.0;
+    if (S->n == 0)
+      return;
+    // tag-aware sums when some atom carries fixedvel or fixedpos
+    double p[3];
+    const double ke = S->kinetic(p, true);
+    fill_observables(S, ke, p, out);
+  });
+}
+
+int lpmdx_observables_noalloc(lpmdx_system* S, double out[LPMDX_NOBS]) {
+  return guard([&] {
+    need(S, "system");
+    need(out, "out");
+    for (int k = 0; k < LPMDX_NOBS; ++k)
+      out[k] = 0.0;
+    if (S->n == 0)
+      return;
+    if (S->n > kNoAllocAtoms)
+      fail(LPMDX_EINVAL, "lpmdx_observables_noalloc: " + std::to_string(S->n) + " atoms, at most " +
+                             std::to_string(kNoAllocAtoms));
+    double* sp = scratch_on(S->device);
+    // the kernels and the reduction of lpmdx_observables (the same sums in the
+    // same order), with the partials in the scratch array; every partial row
+    // and the four sums are written before they are read, so nothing needs
+    // clearing
+    const long nb = blocks_for(S->n);
+    const int* fx = S->n_tagged > 0 ? S->fix.p : nullptr;
+    double* sums = sp + 4 * kNoAllocRows;
+    k_kinetic<<<nb, kBlock, 0, S->st>>>(S->v.p, S->mass.p, S->n, sp, fx, kFixTagged);
+    k_reduce_partials<<<1, kBlock, 0, S->st>>>(sp, nb, 4, sums);
+    S->launch_check();
+    hipchk(hipMemcpyAsync(S->hred, sums, 4 * sizeof(double), hipMemcpyDeviceToHost, S->st), "memcpy");
+    S->sync();
+    const double p[3] = {S->hred[1], S->hred[2], S->hred[3]};
+    fill_observables(S, S->hred[0] * kKin2eV, p, out);
   });
 }
 

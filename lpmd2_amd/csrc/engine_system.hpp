@@ -184,6 +184,7 @@ template <int KIND, int LPA, bool EVAL> struct BrickKernels {
 // the system
 
 struct lpmdx_system {
+  int device = 0; // the HIP device current when the system was created
   long n = 0;
   Box box{};
   double ll[3] = {0, 0, 0};
