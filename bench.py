@@ -12,6 +12,12 @@ roofline (HIP events on the engine's stream) and a CPU baseline (the FP64
 oracle, a restatement of the reference path, on one host core, timed on a
 bounded sample and extrapolated to one reference step).
 
+Every plain-path workload reports, after its timed steps, its relative force
+RMS against the FP64 oracle (all atoms of a small box, a seeded sample of a
+large one) and its total-energy drift over the timed steps ("parity"); "legs"
+holds the same record for the 4,000,000-atom LJ liquid (the HBM-roofline
+case) and the reference's 2048-atom argon case.
+
 The same line carries the two other legs of the metric:
   "batched": 512 periodic systems, sizes log-uniform in [256, 16384] atoms,
       half LJ argon, half Sutton-Chen Cu/Ni/Ag, 1000 steps each in one batch
@@ -55,6 +61,16 @@ FP32_PER_CUT_PAIR = {"lennardjones": 9, "suttonchen": 13,
                      "harmonic": 4, "lennardjonesMod": 14, "finnissinclair-ext": 9}
 
 WORKLOADS = {
+    # the reference's own CPU-runnable case: 2048 argon atoms on the linkedcell
+    # stencil of `mode auto` (12^3 cells, two a side), 1000 steps of 1 fs
+    "lj_ar_2048": dict(
+        desc="LJ argon fcc 8^3 = 2048 atoms, a 5.26, rc 8.5, linkedcell mode auto, dt 1 fs, 100 K",
+        nx=8, a=5.26, z=18, T=100.0, noise=0.0, dt=1.0, seed=2048,
+        pot=("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5"),
+        cm=("linkedcell", "cutoff 8.5 mode auto"),
+        rc=8.5, skin=0.0,
+        metal=False, steps=1000,
+    ),
     "sc_cu_131k": dict(
         desc="Sutton-Chen Cu fcc 32^3 = 131072 atoms, rc 7.22, skin 0.3, dt 2 fs, 300 K",
         nx=32, a=3.61, z=29, T=300.0, noise=0.01, dt=2.0, seed=2024,
@@ -376,47 +392,45 @@ def served_leg(args):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="sc_cu_131k", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sampling")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--lanes", type=int, default=0, help="lanes per atom (0 = automatic)")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
-    ap.add_argument("--batch-systems", type=int, default=512)
-    ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--plain-steps", type=int, default=100, help="steps per system of the plain comparison")
-    ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
-    ap.add_argument("--serve-jobs", type=int, default=64)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 1e6])
-    ap.add_argument("--serve-budget", type=int, default=16, help="server HBM budget in GiB")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="first copy MIB MiB device to device (a copy of known size that calibrates "
-                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--no-plain", action="store_true",
-                    help="skip the one-at-a-time plain comparisons of the batched and served legs")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: each box has one MI355X and multi-GPU runs are not part of this build",
-              file=sys.stderr)
-        sys.exit(2)
+def force_check(w, s, n_sample=20000, seed=99):
+    """The relative force RMS ||F_gpu - F_64|| / ||F_64|| of the plain path at
+    its current positions against the FP64 oracle (the checker): the
+    accelerations of the last step's force evaluation against the oracle's on
+    the same positions, on every atom of a small box (through the oracle's own
+    cell manager of the workload) or on a seeded sample of atoms of a large
+    one (the exact minimum-image set the skin list holds)."""
+    from oracle import oracle as O
 
-    import lpmd2_amd as E
+    x, _, a = s.get_state()
+    n = x.shape[0]
+    kw = dict(zip(w["pot"][1].split()[0::2], (float(q) for q in w["pot"][1].split()[1::2])))
+    pot = O.Pot(w["pot"][0], **kw)
+    L = np.full(3, w["nx"] * w["a"])
+    conf = O.Conf(x, L, z=w["z"])
+    if w["cm"][0] == "linkedcell":
+        args = dict(zip(w["cm"][1].split()[0::2], w["cm"][1].split()[1::2]))
+        O.update_forces(conf, O.CellManager("linkedcell", float(args["cutoff"]), mode=args.get("mode", "noauto")),
+                        [pot])
+        ref, got, sample = conf.a, a, f"all {n} atoms, oracle linkedcell {w['cm'][1]}"
+    else:
+        idx = np.sort(np.random.default_rng(seed).choice(n, size=min(n, n_sample), replace=False))
+        ref = O.sample_forces(conf, [pot], w["rc"], idx)
+        got, sample = a[idx], f"{len(idx)} seeded atoms, exact minimum-image set within rc"
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    return {"rel_force_rms": err, "tolerance": 1e-5, "sample": sample}
 
-    if E.device_count() < 1:
-        print("no HIP device visible", file=sys.stderr)
-        sys.exit(1)
 
-    if args.copy_probe:
-        copy_probe(args.copy_probe << 20)
-    w = WORKLOADS[args.workload]
-    log(f"workload {args.workload}: {w['desc']}")
+def plain_leg(name: str, steps: int, warmup: int, lanes: int):
+    """One workload on the plain single-system path: `warmup` untimed steps,
+    then `steps` timed ones (one integrator call, the state resident in HBM),
+    the force kernel's roofline from HIP events on the engine's stream, the
+    VALU rates, and after the timed region the relative force RMS against
+    the FP64 oracle and the total-energy drift over the timed steps."""
+    w = WORKLOADS[name]
+    log(f"workload {name}: {w['desc']}")
     s, x, L, v = make_system(w)
-    if args.lanes:
-        s.set_lanes(args.lanes)
+    if lanes:
+        s.set_lanes(lanes)
     n = s.n
     if w.get("integrator") == "leapfrog":
         s.leapfrog_init(w["dt"])
@@ -427,20 +441,19 @@ def main():
     else:
         advance = s.velocityverlet
     t0 = time.perf_counter()
-    advance(w["dt"], args.warmup)
-    log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.3f} s; stats {s.stats()}")
+    ep0, ek0 = advance(w["dt"], max(warmup, 1), obs_every=max(warmup, 1))
+    log(f"warmup {warmup} steps: {time.perf_counter() - t0:.3f} s; stats {s.stats()}")
 
     s.profile(True)
     st0 = s.stats()
     t0 = time.perf_counter()
-    advance(w["dt"], args.steps)
+    advance(w["dt"], steps)
     dt_wall = time.perf_counter() - t0  # the integrator returns after the stream is synchronised
     st1 = s.stats()
     prof = s.profile_read()
     s.profile(False)
-    value = n * args.steps / dt_wall
-    ms_per_step = 1e3 * dt_wall / args.steps
-    log(f"timed {args.steps} steps: {dt_wall:.4f} s, {value:.4e} atom-steps/s, profile {prof}, stats {st1}")
+    value = n * steps / dt_wall
+    log(f"timed {steps} steps: {dt_wall:.4f} s, {value:.4e} atom-steps/s, profile {prof}, stats {st1}")
 
     # dominant kernel: the force pass (EAM pass 2 or the pair kernel)
     nbar = st1["list_entries"] / n
@@ -465,7 +478,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(args.workload, kname),
+        "traffic": measured_traffic(name, kname),
         "avg_launch_us": round(avg_force_s * 1e6, 2),
         "launches": f_n,
         "bytes_per_launch": int(bytes_force),
@@ -487,27 +500,95 @@ def main():
         "pairs_in_cutoff_per_atom": round(ncut, 2),
     }
     step_bytes = (160 + 8 * nbar) if w["metal"] else (144 + 4 * nbar)
+    # accuracy after the timed steps (outside the timed region): forces
+    # against FP64, and the total energy one step on against the warmup's last
+    parity = force_check(w, s)
+    ep1, ek1 = advance(w["dt"], 1, obs_every=1)
+    e0, e1 = float(ep0[-1] + ek0[-1]), float(ep1[-1] + ek1[-1])
+    parity["energy_drift_rel"] = (e1 - e0) / abs(e0)
+    parity["energy_drift_steps"] = steps + 1
+    log(f"  parity {parity}")
     out = {
-        "metric": "atom-steps/s plain vs batched/served at <=1e-5 rel force err; p50/p99 latency",
         "value": value,
-        "unit": "atom-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64+f32",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "atoms": n, "path": "plain single-system", "desc": w["desc"],
+        "ms_per_step": 1e3 * dt_wall / steps,
+        "steps": steps,
+        "config": {"workload": name, "atoms": n, "path": "plain single-system", "desc": w["desc"],
                    "integrator": w.get("integrator", "velocityverlet")},
         "roofline": roofline,
         "step_hbm_fraction": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
         "valu": valu,
         "list_builds": st1["builds"] - st0["builds"],
+        "parity": parity,
     }
     s.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="sc_cu_131k", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline sampling")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per atom (0 = automatic)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--plain-steps", type=int, default=100, help="steps per system of the plain comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
+    ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 1e6])
+    ap.add_argument("--serve-budget", type=int, default=16, help="server HBM budget in GiB")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a copy of known size that calibrates "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the one-at-a-time plain comparisons of the batched and served legs")
+    ap.add_argument("--legs", nargs="*", default=["lj_4m", "lj_ar_2048"], choices=sorted(WORKLOADS),
+                    help="further plain-path workloads reported beside the main one")
+    ap.add_argument("--no-legs", action="store_true", help="skip the further plain-path workloads")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: each box has one MI355X and multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import lpmd2_amd as E
+
+    if E.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
+    out = plain_leg(args.workload, args.steps, args.warmup, args.lanes)
+    out = {
+        "metric": "atom-steps/s plain vs batched/served at <=1e-5 rel force err; p50/p99 latency",
+        "value": out["value"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        **{k: v for k, v in out.items() if k not in ("value", "ms_per_step")},
+    }
+    w = WORKLOADS[args.workload]
+    # the other plain legs the targets are stated on: the 4M-atom LJ step
+    # (HBM roofline) and the reference's 2048-atom argon case
+    if not args.no_legs:
+        out["legs"] = {}
+        for name in args.legs:
+            if name == args.workload:
+                continue
+            lw = WORKLOADS[name]
+            out["legs"][name] = plain_leg(name, lw.get("steps", 200), 20, 0)
     if not args.no_batch:
         out["batched"] = batched_leg(args)
     if not args.no_serve:

@@ -1137,6 +1137,171 @@ int orc_update_forces(orc_conf* c, orc_cm* cm, int npot, orc_pot* pots, double* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* forces of selected atoms on the exact neighbour set                        */
+/* ------------------------------------------------------------------------ */
+
+/* Neighbours of atom i within rlist by their single minimum image
+ * (OrthogonalCell::Displacement, orthogonalcell.h:131-149), in any order:
+ * from a cell grid of at least three cells a side with edges >= rlist, or by
+ * brute force when the box is too small for one. */
+typedef struct {
+  int n[3];
+  long* head;
+  long* next;
+} orc_grid;
+
+static long grid_cell(const orc_conf* c, const orc_grid* g, const double* x, int q3[3]) {
+  for (int q = 0; q < 3; ++q) {
+    long k = (long)floor(x[q] / c->L[q] * g->n[q]);
+    k %= g->n[q];
+    if (k < 0)
+      k += g->n[q];
+    q3[q] = (int)k;
+  }
+  return ((long)q3[2] * g->n[1] + q3[1]) * g->n[0] + q3[0];
+}
+
+static long neighbours(const orc_conf* c, const orc_grid* g, long i, double rlist, long* js, double* rij,
+                       double* r2) {
+  long m = 0;
+  const double rl2 = rlist * rlist;
+  if (!g->head) {
+    for (long j = 0; j < c->n; ++j) {
+      if (j == i)
+        continue;
+      double d[3];
+      orc_displacement(c->L, c->per, c->x + 3 * i, c->x + 3 * j, d);
+      const double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (s < rl2) {
+        js[m] = j;
+        memcpy(rij + 3 * m, d, sizeof d);
+        r2[m++] = s;
+      }
+    }
+    return m;
+  }
+  int q3[3];
+  grid_cell(c, g, c->x + 3 * i, q3);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int a = (q3[0] + dx + g->n[0]) % g->n[0];
+        const int b = (q3[1] + dy + g->n[1]) % g->n[1];
+        const int e = (q3[2] + dz + g->n[2]) % g->n[2];
+        for (long j = g->head[((long)e * g->n[1] + b) * g->n[0] + a]; j >= 0; j = g->next[j]) {
+          if (j == i)
+            continue;
+          double d[3];
+          orc_displacement(c->L, c->per, c->x + 3 * i, c->x + 3 * j, d);
+          const double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (s < rl2) {
+            js[m] = j;
+            memcpy(rij + 3 * m, d, sizeof d);
+            r2[m++] = s;
+          }
+        }
+      }
+  return m;
+}
+
+/* CombinedPotential::UpdateForces' acceleration of the atoms idx[0..m) when
+ * the cell manager lists every pair within rlist by its minimum image (the
+ * set verletlist and minimumimage deliver, verletlist.cc:135-167): pair
+ * terms where the species rule applies and r2 < rc^2 (pairpotential.cc:48-72);
+ * EAM rho over every listed neighbour within the cutoff, pair and many-body
+ * terms where the rule applies (metalpotential.cc:121-169).  Summed per atom
+ * over its neighbours rather than in the reference's half-list order, so it
+ * agrees with orc_update_forces to FP64 rounding.  acc[3m]. */
+int orc_sample_forces(const orc_conf* c, int npot, const orc_pot* pots, double rlist, long m, const long* idx,
+                      double* acc) {
+  orc_grid g = {{1, 1, 1}, NULL, NULL};
+  int grid = 1;
+  for (int q = 0; q < 3; ++q) {
+    g.n[q] = (int)floor(c->L[q] / rlist);
+    if (g.n[q] < 3)
+      grid = 0;
+  }
+  if (grid) {
+    const long ncell = (long)g.n[0] * g.n[1] * g.n[2];
+    g.head = malloc(ncell * sizeof(long));
+    g.next = malloc(c->n * sizeof(long));
+    for (long k = 0; k < ncell; ++k)
+      g.head[k] = -1;
+    for (long j = c->n - 1; j >= 0; --j) {
+      int q3[3];
+      const long k = grid_cell(c, &g, c->x + 3 * j, q3);
+      g.next[j] = g.head[k];
+      g.head[k] = j;
+    }
+  }
+  const long cap = c->n;
+  long* js = malloc(cap * sizeof(long));
+  long* js2 = malloc(cap * sizeof(long));
+  double* rij = malloc(3 * cap * sizeof(double));
+  double* rij2 = malloc(3 * cap * sizeof(double));
+  double* r2 = malloc(cap * sizeof(double));
+  double* r22 = malloc(cap * sizeof(double));
+  for (long s = 0; s < m; ++s) {
+    const long i = idx[s];
+    double f[3] = {0.0, 0.0, 0.0};
+    const long nn = neighbours(c, &g, i, rlist, js, rij, r2);
+    for (int p = 0; p < npot; ++p) {
+      const orc_pot* P = &pots[p];
+      const double rc2 = P->cutoff * P->cutoff;
+      if (!orc_pot_is_metal(P)) {
+        for (long k = 0; k < nn; ++k) {
+          const long j = js[k];
+          if (applies_to(P, c->z[i], c->z[j]) && r2[k] < rc2) {
+            double ff[3];
+            orc_pair_force(P, rij + 3 * k, ff);
+            for (int q = 0; q < 3; ++q)
+              f[q] += ff[q];
+          }
+        }
+        continue;
+      }
+      /* rho of i and of each neighbour (rho ignores species, metalpotential.cc:121-132) */
+      double rhoi = 0.0;
+      for (long k = 0; k < nn; ++k)
+        if (r2[k] < rc2)
+          rhoi += orc_rhoij(P, sqrt(r2[k]));
+      for (long k = 0; k < nn; ++k) {
+        const long j = js[k];
+        if (!(r2[k] < rc2) || !applies_to(P, c->z[i], c->z[j]))
+          continue;
+        const long n2 = neighbours(c, &g, j, rlist, js2, rij2, r22);
+        double rhoj = 0.0;
+        for (long q = 0; q < n2; ++q)
+          if (r22[q] < rc2)
+            rhoj += orc_rhoij(P, sqrt(r22[q]));
+        const double r = sqrt(r2[k]);
+        double nrm[3] = {rij[3 * k], rij[3 * k + 1], rij[3 * k + 2]};
+        const double mod = sqrt(sqmod(nrm));
+        for (int q = 0; q < 3; ++q)
+          nrm[q] /= mod;
+        double pf[3], mb[3];
+        orc_metal_pairforce(P, nrm, r, pf);
+        orc_metal_manybodies(P, nrm, rhoi, rhoj, r, mb);
+        for (int q = 0; q < 3; ++q)
+          f[q] += pf[q] + mb[q];
+      }
+    }
+    const double fi = ORC_FORCEFACTOR / c->m[i];
+    for (int q = 0; q < 3; ++q)
+      acc[3 * s + q] = f[q] * fi;
+  }
+  free(js);
+  free(js2);
+  free(rij);
+  free(rij2);
+  free(r2);
+  free(r22);
+  free(g.head);
+  free(g.next);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* integration                                                                */
 /* ------------------------------------------------------------------------ */
 

@@ -139,6 +139,10 @@ void orc_fitted_inside(const double L[3], const int per[3], double x[3]);
 double orc_kinetic_energy(const orc_conf* c);
 double orc_temperature(const orc_conf* c);
 void orc_momentum(const orc_conf* c, double p[3]);
+/* CombinedPotential::UpdateForces' accelerations of the atoms idx[0..m) on
+ * the exact minimum-image neighbour set within rlist, into acc[3m] */
+int orc_sample_forces(const orc_conf* c, int npot, const orc_pot* pots, double rlist, long m, const long* idx,
+                      double* acc);
 /* Potential::AtomEnergy of every atom into e[n] (cell manager updated) */
 int orc_atom_energies(orc_conf* c, orc_cm* cm, const orc_pot* p, double* e);
 /* the tag-aware forms (tag = 1 leaves out ORC_FIX_TAGGED atoms) */

@@ -152,6 +152,7 @@ def lib():
         L.orc_init_velocities.argtypes = [vp, vp]
         L.orc_tags.argtypes = [vp, d, d, vp, vp]
         L.orc_atom_energies.argtypes = [vp, vp, vp, vp]
+        L.orc_sample_forces.argtypes = [vp, i, vp, d, l, vp, vp]
         L.orc_lc_auto_n.restype = i
         L.orc_lc_auto_n.argtypes = [d, d]
         L.orc_last_error.restype = C.c_char_p
@@ -378,6 +379,18 @@ def update_forces(conf: Conf, cm: CellManager, pots):
     S = np.zeros(9)
     _check(lib().orc_update_forces(conf.struct(), cm.h, len(pots), arr, _p(ep), _p(vir), _p(S)))
     return float(ep[0]), float(vir[0]), S.reshape(3, 3)
+
+
+def sample_forces(conf: Conf, pots, rlist: float, idx):
+    """The accelerations CombinedPotential::UpdateForces gives the atoms idx
+    when every pair within rlist is listed by its minimum image (verletlist /
+    minimumimage sets); (len(idx), 3).  For boxes too large for a full
+    evaluation: each sampled atom's neighbours come from a cell grid."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = np.zeros((len(idx), 3))
+    arr = _pot_array(pots)
+    _check(lib().orc_sample_forces(conf.struct(), len(pots), arr, float(rlist), len(idx), _p(idx), _p(out)))
+    return out
 
 
 def atom_energies(conf: Conf, cm: CellManager, pot):

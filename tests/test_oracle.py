@@ -463,6 +463,46 @@ def test_md_fixed_atoms_closed_forms():
     np.testing.assert_allclose(conf.v[keep], alone.v, rtol=0, atol=1e-13)
 
 
+@pytest.mark.parametrize("case", ["lj", "sc_species", "rocksalt", "small_box"])
+def test_sample_forces_match_full_evaluation(case):
+    """orc_sample_forces (the exact-set forces of chosen atoms, for boxes too
+    large for a full evaluation) against orc_update_forces with a verletlist
+    listing the same set: equal to FP64 rounding, for a pair potential, an
+    EAM potential with a species rule on a two-species lattice, three
+    species-pair Buckingham terms, and a box too small for a cell grid."""
+    rng = np.random.default_rng(61)
+    if case == "lj":
+        x, L = fcc(6, 5.26)
+        z = np.full(len(x), 18)
+        pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
+        rl = 8.5
+    elif case == "sc_species":
+        x, L = fcc(6, 3.61)
+        z = np.where(rng.random(len(x)) < 0.3, 28, 29)
+        pots = [O.Pot("suttonchen", 29, 29, e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=5.5),
+                O.Pot("gupta", 28, 29, A=0.0855, r0=2.556, p=10.960, B=1.2240, q=2.278, cutoff=5.0)]
+        rl = 5.5
+    elif case == "rocksalt":
+        import lpmd2_amd.workloads as W
+
+        x, L, z = W.rocksalt(3, 3, 3, 5.64)
+        pots = [O.Pot("buckingham", s1, s2, **dict(zip(a.split()[0::2], map(float, a.split()[1::2]))))
+                for a, s1, s2 in W.BUCKINGHAM_NACL]
+        rl = 8.0
+    else:
+        x, L = fcc(3, 3.61)
+        z = np.full(len(x), 29)
+        pots = [O.Pot("suttonchen", e=1.2382e-2, a=3.61, n=9, m=6, c=39.432, cutoff=5.0)]
+        rl = 5.0
+    x = np.mod(x + rng.normal(0, 0.05, x.shape), L)
+    conf = O.Conf(x, L, z=z)
+    O.update_forces(conf, O.CellManager("verletlist", rl, maxneighbors=1000, each=5), pots)
+    idx = rng.choice(len(x), size=min(60, len(x)), replace=False)
+    got = O.sample_forces(conf, pots, rl, idx)
+    scale = np.abs(conf.a).max()
+    np.testing.assert_allclose(got, conf.a[idx], rtol=0, atol=1e-12 * scale)
+
+
 def test_atom_energies_closed_forms():
     """AtomEnergy (pairpotential.cc:21-33, metalpotential.cc:47-60): a pair
     potential's per-atom energies count each pair from both ends, so they sum
