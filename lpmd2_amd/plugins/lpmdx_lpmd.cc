@@ -95,6 +95,45 @@ void Velocities(lpmd::Configuration& conf, std::vector<double>& v) {
   }
 }
 
+// The fixedvel / fixedpos tags as TwoStepIntegrator::Advance and
+// OneStepIntegrator::Advance read them (twostepintegrator.cc:17-65,
+// onestepintegrator.cc:18-38): zero[i] for a fixedvel atom (its acceleration
+// is set to zero after each force evaluation), hold[i] for a fixedpos atom
+// whose tag reads "true" (not advanced).
+struct HostTags {
+  bool any = false;
+  std::vector<char> zero, hold;
+};
+
+HostTags ReadTags(lpmd::Simulation& sim) {
+  HostTags t;
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const bool hv = atoms.HaveAny(lpmd::Tag("fixedvel")), hp = atoms.HaveAny(lpmd::Tag("fixedpos"));
+  if (!hv && !hp)
+    return t;
+  const long n = atoms.Size();
+  t.any = true;
+  t.zero.assign(n, 0);
+  t.hold.assign(n, 0);
+  for (long i = 0; i < n; ++i) {
+    const lpmd::BasicAtom& at = atoms[i];
+    t.zero[i] = hv && atoms.Have(at, lpmd::Tag("fixedvel"));
+    t.hold[i] = hp && atoms.Have(at, lpmd::Tag("fixedpos")) && atoms.GetTag(at, lpmd::Tag("fixedpos")) == "true";
+  }
+  return t;
+}
+
+void ZeroFixedVel(lpmd::Simulation& sim, const HostTags& t) {
+  if (!t.any)
+    return;
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  for (long i = 0; i < atoms.Size(); ++i)
+    if (t.zero[i])
+      atoms[i].Acceleration() = lpmd::Vector(0.0, 0.0, 0.0);
+}
+
+bool Held(const HostTags& t, long i) { return t.any && t.hold[i]; }
+
 } // namespace
 
 // ------------------------------------------------------------------ cell manager
@@ -151,6 +190,8 @@ void DeviceCellManager::Create(lpmd::Configuration& conf) {
     box[q] = cell[q][q];
   Reset();
   check(lpmdx_system_create(&sys_, n, box, z.data(), m.data()), kind_);
+  ++version_;
+  fixed_.clear();
   n_ = n;
   for (int q = 0; q < 3; ++q)
     box_[q] = box[q];
@@ -175,6 +216,7 @@ lpmdx_system* DeviceCellManager::Bind(lpmd::Configuration& conf, bool upload) {
       xbuf_.swap(x);
       check(lpmdx_system_set_state(sys_, xbuf_.data(), nullptr, nullptr), kind_);
       lists_ok_ = false;
+      ++version_;
     }
   }
   return sys_;
@@ -244,6 +286,7 @@ int DeviceCellManager::IndexOf(const DevicePotential& p) {
   p.Species(s1, s2);
   check(lpmdx_potential_append(sys_, p.Kind().c_str(), p.Args().c_str(), s1, s2), p.Kind());
   order_.push_back(&p);
+  ++version_;
   return (int)order_.size() - 1;
 }
 
@@ -252,58 +295,114 @@ void DeviceCellManager::UsePotentials(const std::vector<const DevicePotential*>&
     return;
   check(lpmdx_potential_clear(sys_), kind_);
   order_.clear();
+  ++version_;
   for (const DevicePotential* p : pots)
     IndexOf(*p);
 }
 
-// ------------------------------------------------------------------ potentials
-
-DevicePotential::DevicePotential(const std::string& name, std::string args)
-    : lpmd::Plugin(name, kVersion), kind_(name), args_(args) {
-  Define(*this, name);
-  ProcessArguments(args);
-  if (name == "tabulatedpair") {
-    // cutoff = the table's largest r (tabulatedpair.cc:34-79); the reference
-    // ends the program when the file does not open, this plugin throws
-    const lpmdx::PairTable T = lpmdx::read_pair_table(std::string((*this)["file"]));
-    if (!T.ok)
-      throw lpmd::RuntimeError("Impossible open the potential data file.");
-    SetCutoff(T.cutoff);
-  } else if (name == "finnissinclair" || name == "finnissinclair-ext") {
-    // cutoff = max(c, d) (finnissinclair.cc:10-31, finnissinclair-ext.cc:11-36)
-    const double c = double((*this)["c"]), d = double((*this)["d"]);
-    SetCutoff(d >= c ? d : c);
-  } else if (name != "morse" && name != "harmonic") {
-    // morse and harmonic keep Potential's HUGE_VAL (morse.cc:11-21, harmonic.cc:11-19)
-    SetCutoff(double((*this)["cutoff"]));
+void DeviceCellManager::SyncFixed(lpmd::Configuration& conf) {
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const bool hv = atoms.HaveAny(lpmd::Tag("fixedvel")), hp = atoms.HaveAny(lpmd::Tag("fixedpos"));
+  std::vector<int> f;
+  if (hv || hp) {
+    // TwoStepIntegrator::Advance (twostepintegrator.cc:17-65): fixedvel by
+    // presence, fixedpos when it reads "true"; either tag, whatever its
+    // value, leaves the atom out of the tag-aware sums (properties.h:22-81)
+    const long n = atoms.Size();
+    f.assign(n, 0);
+    for (long i = 0; i < n; ++i) {
+      const lpmd::BasicAtom& at = atoms[i];
+      if (hv && atoms.Have(at, lpmd::Tag("fixedvel")))
+        f[i] |= LPMDX_FIX_VEL | LPMDX_FIX_TAGGED;
+      if (hp && atoms.Have(at, lpmd::Tag("fixedpos"))) {
+        f[i] |= LPMDX_FIX_TAGGED;
+        if (atoms.GetTag(at, lpmd::Tag("fixedpos")) == "true")
+          f[i] |= LPMDX_FIX_POS;
+      }
+    }
   }
+  if (f == fixed_)
+    return;
+  check(lpmdx_system_set_fixed(sys_, f.empty() ? nullptr : f.data()), kind_);
+  fixed_.swap(f);
 }
 
-void DevicePotential::ShowHelp() const {
+// ------------------------------------------------------------------ potentials
+
+namespace {
+
+// Potential::SetCutoff as each plugin's constructor calls it: the cutoff
+// keyword; max(c, d) for finnissinclair and finnissinclair-ext
+// (finnissinclair.cc:10-31, finnissinclair-ext.cc:11-36); the table's last
+// increasing r for tabulatedpair (tabulatedpair.cc:34-79); morse and harmonic
+// keep Potential's HUGE_VAL (morse.cc:11-21, harmonic.cc:11-19).
+template <typename P> PairForm Configure(P& pot, const std::string& name) {
+  if (name == "tabulatedpair") {
+    // the reference ends the program when the file does not open
+    const lpmdx::PairTable T = lpmdx::read_pair_table(std::string(pot["file"]));
+    if (!T.ok)
+      throw lpmd::RuntimeError("Impossible open the potential data file.");
+    pot.SetCutoff(T.cutoff);
+    return MakeForm(name, [](const char*) { return 0.0; }, &T.e, &T.f, T.cutoff);
+  }
+  PairForm F = MakeForm(name, [&](const char* k) { return double(pot[k]); });
+  if (name != "morse" && name != "harmonic")
+    pot.SetCutoff(F.cutoff);
+  return F;
+}
+
+DeviceCellManager* DeviceOf(lpmd::Configuration& conf, const std::string& who) {
+  auto* cm = dynamic_cast<DeviceCellManager*>(&conf.GetCellManager());
+  if (!cm)
+    throw lpmd::RuntimeError(who + " (MI355X): the cell manager must be an MI355X cell manager plugin "
+                                   "(linkedcell, verletlist, minimumimage, lcbinary)");
+  return cm;
+}
+
+} // namespace
+
+DevicePairPotential::DevicePairPotential(const std::string& name, std::string args)
+    : lpmd::Plugin(name, kVersion), DevicePotential(name, args) {
+  Define(*this, name);
+  ProcessArguments(args);
+  form_ = Configure(*this, name);
+}
+
+void DevicePairPotential::ShowHelp() const {
   std::cout << " Module Name        = " << kind_ << " (MI355X engine)\n"
             << "      Forces on the GPU through the lpmdx engine; keywords as the CPU plugin.\n";
 }
 
+DeviceMetalPotential::DeviceMetalPotential(const std::string& name, std::string args)
+    : lpmd::Plugin(name, kVersion), DevicePotential(name, args) {
+  Define(*this, name);
+  ProcessArguments(args);
+  form_ = Configure(*this, name);
+}
+
+void DeviceMetalPotential::ShowHelp() const {
+  std::cout << " Module Name        = " << kind_ << " (MI355X engine)\n"
+            << "      Two-pass EAM forces on the GPU through the lpmdx engine; keywords as the CPU plugin.\n";
+}
+
 void DevicePotential::Species(int& s1, int& s2) const {
+  const lpmd::Potential& P = Pot();
   const int far = 1 << 20;
-  if (AppliesTo(0, far) && AppliesTo(0, far + 1)) {
+  if (P.AppliesTo(0, far) && P.AppliesTo(0, far + 1)) {
     s1 = s2 = -1; // wildcard
     return;
   }
   s1 = 0;
   s2 = far;
   for (int v = 0; v < 4096; ++v)
-    if (AppliesTo(0, v)) {
+    if (P.AppliesTo(0, v)) {
       s2 = v;
       return;
     }
 }
 
-void DevicePotential::UpdateForces(lpmd::Configuration& conf) {
-  auto* cm = dynamic_cast<DeviceCellManager*>(&conf.GetCellManager());
-  if (!cm)
-    throw lpmd::RuntimeError(kind_ +
-                             " (MI355X): the cell manager must be an MI355X cell manager plugin (linkedcell, verletlist, minimumimage, lcbinary)");
+double DevicePotential::DeviceForces(lpmd::Configuration& conf) {
+  DeviceCellManager* cm = DeviceOf(conf, kind_);
   lpmdx_system* sys = cm->Bind(conf, true);
   const int k = cm->IndexOf(*this);
   lpmd::BasicParticleSet& atoms = conf.Atoms();
@@ -313,18 +412,28 @@ void DevicePotential::UpdateForces(lpmd::Configuration& conf) {
   check(lpmdx_potential_forces(sys, k, acc_.data(), &e, &w, s), kind_);
   for (long i = 0; i < n; ++i)
     atoms[i].Acceleration() += lpmd::Vector(acc_[3 * i], acc_[3 * i + 1], acc_[3 * i + 2]);
-  energycache = e;
   conf.Virial() += w;
   lpmd::Matrix& st = conf.StressTensor();
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b)
       st.Set(a, b, st.Get(a, b) + s[3 * a + b]);
+  return e;
 }
 
-double DevicePotential::energy(lpmd::Configuration&) { return energycache; }
-
-double DevicePotential::AtomEnergy(lpmd::Configuration&, long) {
-  throw lpmd::RuntimeError(kind_ + " (MI355X): per-atom energies are not computed by the GPU engine");
+double DevicePotential::DeviceAtomEnergy(lpmd::Configuration& conf, long i) {
+  DeviceCellManager* cm = DeviceOf(conf, kind_);
+  lpmdx_system* sys = cm->Bind(conf, true);
+  const int k = cm->IndexOf(*this);
+  const long n = conf.Atoms().Size();
+  if (i < 0 || i >= n)
+    throw lpmd::RuntimeError(kind_ + " (MI355X): AtomEnergy of an atom index out of range");
+  if (eatom_sys_ != sys || eatom_version_ != cm->Version() || (long)eatom_.size() != n) {
+    eatom_.assign(n > 0 ? n : 1, 0.0);
+    check(lpmdx_atom_energies(sys, k, eatom_.data()), kind_);
+    eatom_sys_ = sys;
+    eatom_version_ = cm->Version();
+  }
+  return eatom_[i];
 }
 
 // ------------------------------------------------------------------ integrator
@@ -349,20 +458,26 @@ void DeviceVelocityVerlet::Initialize(lpmd::Simulation&, lpmd::Potential&) {
 void DeviceVelocityVerlet::HostStep(lpmd::Simulation& sim, lpmd::Potential& p) {
   // TwoStepIntegrator::Advance with VelocityVerlet's position and velocity
   // updates, in the reference's order of operations
+  const HostTags tags = ReadTags(sim);
   p.UpdateForces(sim);
+  ZeroFixedVel(sim, tags);
   lpmd::BasicParticleSet& atoms = sim.Atoms();
   lpmd::BasicCell& cell = sim.Cell();
   const long n = atoms.Size();
   std::vector<lpmd::Vector> aold(n);
   for (long i = 0; i < n; ++i) {
+    if (Held(tags, i))
+      continue;
     const lpmd::Vector np =
         atoms[i].Position() + atoms[i].Velocity() * dt + 0.5 * atoms[i].Acceleration() * dt * dt;
     aold[i] = atoms[i].Acceleration();
     atoms[i].Position() = cell.FittedInside(np);
   }
   p.UpdateForces(sim);
+  ZeroFixedVel(sim, tags);
   for (long i = 0; i < n; ++i)
-    atoms[i].Velocity() = atoms[i].Velocity() + 0.5 * dt * (aold[i] + atoms[i].Acceleration());
+    if (!Held(tags, i))
+      atoms[i].Velocity() = atoms[i].Velocity() + 0.5 * dt * (aold[i] + atoms[i].Acceleration());
   last_ = nullptr;
 }
 
@@ -426,6 +541,7 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   }
   lpmdx_system* sys = cm->Bind(sim, false);
   cm->UsePotentials(dev);
+  cm->SyncFixed(sim);
   std::vector<double> x, v;
   Positions(sim, x);
   Velocities(sim, v);
@@ -515,9 +631,13 @@ void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   if (!cm) {
     // OneStepIntegrator::Advance with Leapfrog::Advance (onestepintegrator.cc:13-46,
     // leapfrog.cc:69-77); vh_ plays the old configuration's velocities
+    const HostTags tags = ReadTags(sim);
     p.UpdateForces(sim);
+    ZeroFixedVel(sim, tags);
     lpmd::BasicCell& cell = sim.Cell();
     for (long i = 0; i < n; ++i) {
+      if (Held(tags, i))
+        continue;
       const lpmd::Vector vhalf = vh_[i] + atoms[i].Acceleration() * dt;
       atoms[i].Position() = cell.FittedInside(atoms[i].Position() + vhalf * dt);
       atoms[i].Velocity() = 0.5 * (vhalf + vh_[i]);
@@ -529,6 +649,7 @@ void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   if (sys != sys_)
     throw lpmd::RuntimeError("leapfrog (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
+  cm->SyncFixed(sim);
   std::vector<double> x;
   Positions(sim, x);
   if (x != hx_) {
@@ -613,9 +734,13 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     // TwoStepIntegrator::Advance with Beeman's AdvancePosition and
     // AdvanceVelocity (twostepintegrator.cc:12-68, beeman.cc:63-84)
     lpmd::BasicCell& cell = sim.Cell();
+    const HostTags tags = ReadTags(sim);
     p.UpdateForces(sim);
+    ZeroFixedVel(sim, tags);
     std::vector<lpmd::Vector> aux(n);
     for (long i = 0; i < n; ++i) {
+      if (Held(tags, i))
+        continue;
       aux[i] = a_old_[i];
       const lpmd::Vector np = atoms[i].Position() + atoms[i].Velocity() * dt +
                               (2.0 / 3.0) * atoms[i].Acceleration() * dt * dt -
@@ -624,15 +749,18 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
       atoms[i].Position() = cell.FittedInside(np);
     }
     p.UpdateForces(sim);
+    ZeroFixedVel(sim, tags);
     for (long i = 0; i < n; ++i)
-      atoms[i].Velocity() = atoms[i].Velocity() + (1.0 / 3.0) * atoms[i].Acceleration() * dt +
-                            (5.0 / 6.0) * a_old_[i] * dt - (1.0 / 6.0) * aux[i] * dt;
+      if (!Held(tags, i))
+        atoms[i].Velocity() = atoms[i].Velocity() + (1.0 / 3.0) * atoms[i].Acceleration() * dt +
+                              (5.0 / 6.0) * a_old_[i] * dt - (1.0 / 6.0) * aux[i] * dt;
     return;
   }
   lpmdx_system* sys = cm->Bind(sim, false);
   if (sys != sys_)
     throw lpmd::RuntimeError("beeman (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
+  cm->SyncFixed(sim);
   std::vector<double> x, v;
   Positions(sim, x);
   Velocities(sim, v);
@@ -650,12 +778,15 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
 // ------------------------------------------------------------------ factory
 
 lpmd::Plugin* Create(const std::string& name, const std::string& args) {
-  static const char* pots[] = {"lennardjones", "fastlj",   "morse",           "buckingham",
-                               "suttonchen",   "finnissinclair", "gupta", "harmonic",
-                               "lennardjonesMod", "finnissinclair-ext", "tabulatedpair"};
-  for (const char* p : pots)
+  static const char* pair[] = {"lennardjones", "fastlj", "morse", "buckingham", "harmonic", "lennardjonesMod",
+                               "tabulatedpair"};
+  static const char* metal[] = {"suttonchen", "finnissinclair", "gupta", "finnissinclair-ext"};
+  for (const char* p : pair)
     if (name == p)
-      return new DevicePotential(name, args);
+      return new DevicePairPotential(name, args);
+  for (const char* p : metal)
+    if (name == p)
+      return new DeviceMetalPotential(name, args);
   if (name == "linkedcell" || name == "verletlist" || name == "minimumimage" || name == "lcbinary")
     return new DeviceCellManager(name, args);
   if (name == "velocityverlet")
@@ -665,6 +796,19 @@ lpmd::Plugin* Create(const std::string& name, const std::string& args) {
   if (name == "beeman")
     return new DeviceBeeman(args);
   throw lpmd::RuntimeError("no MI355X plugin named " + name);
+}
+
+lpmd::Plugin* CreateOrExit(const std::string& name, const std::string& args) {
+  try {
+    return Create(name, args);
+  } catch (const lpmd::Error& e) {
+    std::cerr << "[Error] " << e.what() << '\n';
+  } catch (const std::exception& e) {
+    std::cerr << "[Error] " << name << ": " << e.what() << '\n';
+  } catch (...) {
+    std::cerr << "[Error] " << name << ": unknown error while creating the plugin\n";
+  }
+  std::exit(1);
 }
 
 } // namespace lpmdx_lpmd

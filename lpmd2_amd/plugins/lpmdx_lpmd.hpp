@@ -7,10 +7,11 @@
 //                         "verletlist" (plugins/verletlist.cc:12-29),
 //                         "minimumimage" (plugins/minimumimage.cc:12-19) and
 //                         "lcbinary" (plugins/lcbinary.cc:9-33)
-//   DevicePotential       lpmd::Potential (liblpmd/lpmd/potential.h:11-36) as
-//                         lennardjones fastlj morse buckingham suttonchen
-//                         finnissinclair gupta harmonic lennardjonesMod
-//                         finnissinclair-ext tabulatedpair (plugins/<name>.cc)
+//   DevicePairPotential   lpmd::PairPotential (liblpmd/lpmd/pairpotential.h:15-26)
+//                         as lennardjones fastlj morse buckingham harmonic
+//                         lennardjonesMod tabulatedpair (plugins/<name>.cc)
+//   DeviceMetalPotential  lpmd::MetalPotential (liblpmd/lpmd/metalpotential.h:13-41)
+//                         as suttonchen finnissinclair gupta finnissinclair-ext
 //   DeviceVelocityVerlet  lpmd::Integrator (liblpmd/lpmd/integrator.h:15-40) as
 //                         "velocityverlet" (plugins/velocityverlet.cc:13-20)
 //   DeviceLeapfrog        lpmd::Integrator as "leapfrog" (plugins/leapfrog.cc:14-23)
@@ -25,6 +26,8 @@
 #include <lpmd/cellmanager.h>
 #include <lpmd/configuration.h>
 #include <lpmd/integrator.h>
+#include <lpmd/metalpotential.h>
+#include <lpmd/pairpotential.h>
 #include <lpmd/plugin.h>
 #include <lpmd/potential.h>
 
@@ -33,6 +36,7 @@
 #include <vector>
 
 #include "lpmdx.h"
+#include "pair_forms.hpp"
 
 namespace lpmdx_lpmd {
 
@@ -66,7 +70,14 @@ public:
   void Mirror(const std::vector<double>& x) {
     xbuf_ = x;
     lists_ok_ = false;
+    ++version_;
   }
+  // Changes whenever the device positions, the system or its potentials
+  // change (what per-atom results computed before no longer describe).
+  long Version() const { return version_; }
+  // The fixed-atom flags of conf's fixedvel / fixedpos tags sent to the
+  // device (only when they differ from the last ones sent).
+  void SyncFixed(lpmd::Configuration& conf);
 
 private:
   void Create(lpmd::Configuration& conf);
@@ -77,6 +88,8 @@ private:
   double box_[3] = {0, 0, 0};
   std::vector<const DevicePotential*> order_;
   std::vector<double> xbuf_;
+  long version_ = 0;
+  std::vector<int> fixed_; // flags last sent (empty: none)
   // neighbour lists exported from the device for BuildNeighborList callers
   bool lists_ok_ = false;
   bool lists_full_ = true;
@@ -85,28 +98,81 @@ private:
   std::vector<double> r2_;
 };
 
-class DevicePotential : public lpmd::Potential, public lpmd::Plugin {
+// What the pair and EAM drop-ins share: the device term of the potential, its
+// forces and per-atom energies through the cell manager's device system.
+class DevicePotential {
 public:
-  DevicePotential(const std::string& name, std::string args);
-  void ShowHelp() const override;
-
-  // Potential::UpdateForces of this one potential: adds its accelerations,
-  // virial and stress to conf, as PairPotential::UpdateForces
-  // (liblpmd/simulation/pairpotential.cc:35-84) and
-  // MetalPotential::UpdateForces (metalpotential.cc:62-180) do.
-  void UpdateForces(lpmd::Configuration& conf) override;
-  double energy(lpmd::Configuration& conf) override;
-  double AtomEnergy(lpmd::Configuration& conf, long i) override;
-
+  virtual ~DevicePotential() = default;
   const std::string& Kind() const { return kind_; }
   const std::string& Args() const { return args_; }
   // Species rule as (s1, s2) for the engine, read back through AppliesTo
   // (potential.cc:23-27: spc_sum < 0 or s1 + s2 == spc_sum).
   void Species(int& s1, int& s2) const;
+  virtual const lpmd::Potential& Pot() const = 0;
+
+protected:
+  DevicePotential(const std::string& kind, const std::string& args) : kind_(kind), args_(args) {}
+  // Potential::UpdateForces of this one potential: adds its accelerations,
+  // virial and stress to conf, as PairPotential::UpdateForces
+  // (liblpmd/simulation/pairpotential.cc:35-84) and
+  // MetalPotential::UpdateForces (metalpotential.cc:62-180) do; returns its energy.
+  double DeviceForces(lpmd::Configuration& conf);
+  // Potential::AtomEnergy (pairpotential.cc:21-33, metalpotential.cc:47-60)
+  // from the device's per-atom energies, kept until the positions change.
+  double DeviceAtomEnergy(lpmd::Configuration& conf, long i);
+  std::string kind_, args_;
+  std::vector<double> acc_, eatom_;
+  const void* eatom_sys_ = nullptr;
+  long eatom_version_ = -1;
+};
+
+// lpmd::PairPotential drop-in: lennardjones fastlj morse buckingham harmonic
+// lennardjonesMod tabulatedpair.  UpdateForces, energy and AtomEnergy run on
+// the device; pairEnergy / pairForce (pairpotential.h:23-25) are the
+// plugin's FP64 forms on the host, for callers that evaluate single pairs.
+class DevicePairPotential : public lpmd::PairPotential, public lpmd::Plugin, public DevicePotential {
+public:
+  DevicePairPotential(const std::string& name, std::string args);
+  void ShowHelp() const override;
+  void UpdateForces(lpmd::Configuration& conf) override { energycache = DeviceForces(conf); }
+  double energy(lpmd::Configuration&) override { return energycache; }
+  double AtomEnergy(lpmd::Configuration& conf, long i) override { return DeviceAtomEnergy(conf, i); }
+  double pairEnergy(const double& r) const override { return form_.pairEnergy(r); }
+  lpmd::Vector pairForce(const lpmd::Vector& r) const override { return form_.pairForce(r); }
+  const lpmd::Potential& Pot() const override { return *this; }
 
 private:
-  std::string kind_, args_;
-  std::vector<double> acc_;
+  PairForm form_;
+};
+
+// lpmd::MetalPotential drop-in: suttonchen finnissinclair gupta
+// finnissinclair-ext.  UpdateForces, energy and AtomEnergy run on the
+// device; pairEnergy, rhoij, F, PairForce, ManyBodies and UpdateCorrections
+// (metalpotential.h:26-37) are the plugin's FP64 forms on the host.
+class DeviceMetalPotential : public lpmd::MetalPotential, public lpmd::Plugin, public DevicePotential {
+public:
+  DeviceMetalPotential(const std::string& name, std::string args);
+  void ShowHelp() const override;
+  void UpdateForces(lpmd::Configuration& conf) override { energycache = DeviceForces(conf); }
+  double energy(lpmd::Configuration&) override { return energycache; }
+  double AtomEnergy(lpmd::Configuration& conf, long i) override { return DeviceAtomEnergy(conf, i); }
+  double pairEnergy(const double& r) const override { return form_.pairEnergy(r); }
+  double rhoij(const double& r) const override { return form_.rhoij(r); }
+  double F(const double& rho) const override { return form_.F(rho); }
+  lpmd::Vector PairForce(const lpmd::Vector& normr, const double& mod) const override {
+    return form_.PairForce(normr, mod);
+  }
+  lpmd::Vector ManyBodies(const lpmd::Vector& normrij, const double& rhoi, const double& rhoj,
+                          const double& mod) const override {
+    return form_.ManyBodies(normrij, rhoi, rhoj, mod);
+  }
+  lpmd::Vector UpdateCorrections(const double& rho, const int& N, const double& sinv) const override {
+    return form_.UpdateCorrections(rho, N, sinv);
+  }
+  const lpmd::Potential& Pot() const override { return *this; }
+
+private:
+  PairForm form_;
 };
 
 class DeviceVelocityVerlet : public lpmd::Integrator, public lpmd::Plugin {
@@ -174,5 +240,12 @@ private:
 
 // Plugin factory behind every <name>.so (create/destroy, liblpmd/lpmd/plugin.h:42-46).
 lpmd::Plugin* Create(const std::string& name, const std::string& args);
+// Create for the create() entry point: an error (a bad keyword string, an
+// unknown name, a table file that does not open) ends the program with the
+// message, as lpmd::EndWithError does (liblpmd/lpmd/util.h:19-22), instead of
+// unwinding out of the dlopen'd plugin: the loader dlcloses the plugin while
+// the exception is in flight, and its type information goes with it
+// (liblpmd/plugins/plugin.cc:32-60).
+lpmd::Plugin* CreateOrExit(const std::string& name, const std::string& args);
 
 } // namespace lpmdx_lpmd

@@ -7,5 +7,7 @@
 #error "build with -DLPMDX_PLUGIN_NAME='\"<reference plugin name>\"'"
 #endif
 
-lpmd::Plugin* create(std::string args) { return lpmdx_lpmd::Create(LPMDX_PLUGIN_NAME, args); }
+// No exception leaves create(): an error ends the program with its message
+// (lpmdx_lpmd::CreateOrExit).
+lpmd::Plugin* create(std::string args) { return lpmdx_lpmd::CreateOrExit(LPMDX_PLUGIN_NAME, args); }
 void destroy(lpmd::Plugin* m) { delete m; }

@@ -3,20 +3,23 @@
 #
 #   make -f lpmd2_amd/plugins/plugins.mk       (from the repository root)
 #
-# Needs the reference headers ($(REF)/liblpmd) and its core library; here the
-# core built from the reference sources by oracle/ref.mk (oracle/_ref), in a
-# real deployment the reference's own liblpmd.  Writes
-# lpmd2_amd/plugins/_build/{liblpmdx_lpmd.so, <name>.so}: putting that
-# directory first on LPMD_PATH replaces the CPU plugins of the same names.
+# Needs the reference headers ($(REF)/liblpmd) and its core library to link
+# against: here the core built from the reference sources by oracle/ref.mk
+# (oracle/_ref), in a real deployment the reference's own liblpmd (CORE=...).
+# The plugins carry no run path to the core: they use the one the host
+# program already loaded.  Writes lpmd2_amd/plugins/_build/{liblpmdx_lpmd.so,
+# <name>.so}: putting that directory first on LPMD_PATH replaces the CPU
+# plugins of the same names.
 REF      ?= /root/reference
 OUT      := lpmd2_amd/plugins/_build
-CORE     := oracle/_ref
+CORE     ?= oracle/_ref
+CORELIB  ?= lpmdref_core
 CXX      ?= g++
 CXXFLAGS := -O2 -std=c++17 -fPIC -w -I$(REF)/liblpmd -Iinclude
 NAMES    := lennardjones fastlj morse buckingham suttonchen finnissinclair gupta \
             harmonic lennardjonesMod finnissinclair-ext tabulatedpair \
             linkedcell verletlist minimumimage lcbinary velocityverlet leapfrog beeman
-RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..' -Wl,-rpath,'$$ORIGIN/../../../oracle/_ref'
+RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..'
 
 all: $(patsubst %,$(OUT)/%.so,$(NAMES))
 
@@ -24,12 +27,12 @@ $(OUT)/liblpmdx_lpmd.so: lpmd2_amd/plugins/lpmdx_lpmd.cc lpmd2_amd/plugins/lpmdx
                          lpmd2_amd/csrc/table.hpp \
                          lpmd2_amd/liblpmdx.so $(CORE)/liblpmdref_core.so
 	@mkdir -p $(OUT)
-	$(CXX) $(CXXFLAGS) -shared -o $@ $< -L$(CORE) -llpmdref_core -Llpmd2_amd -l:liblpmdx.so $(RPATH) \
+	$(CXX) $(CXXFLAGS) -shared -o $@ $< -L$(CORE) -l$(CORELIB) -Llpmd2_amd -l:liblpmdx.so $(RPATH) \
 	  -Wl,--no-undefined
 
 $(OUT)/%.so: lpmd2_amd/plugins/plugin_entry.cc $(OUT)/liblpmdx_lpmd.so
 	$(CXX) $(CXXFLAGS) -DLPMDX_PLUGIN_NAME='"$*"' -shared -o $@ $< -L$(OUT) -l:liblpmdx_lpmd.so \
-	  -L$(CORE) -llpmdref_core $(RPATH) -Wl,--no-undefined
+	  -L$(CORE) -l$(CORELIB) $(RPATH) -Wl,--no-undefined
 
 clean:
 	rm -rf $(OUT)

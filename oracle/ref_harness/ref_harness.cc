@@ -427,6 +427,30 @@ int ref_combined_forces(void* conf, int npot, void** pots, const int* s1, const 
   });
 }
 
+// Potential::AtomEnergy(conf, i) of a loaded potential, its species rule set
+// to (s1, s2) first as the application sets it (SetValidSpecies), for the
+// atoms i < n (its cell manager attached to conf before).
+int ref_atom_energies(void* pot, void* conf, int s1, int s2, long n, double* e) {
+  return Guard([&] {
+    Potential* p = dynamic_cast<Potential*>(static_cast<Loaded*>(pot)->plugin);
+    if (!p)
+      throw std::runtime_error("not a Potential");
+    p->SetValidSpecies(s1, s2);
+    HarnessConf* c = static_cast<HarnessConf*>(conf);
+    for (long i = 0; i < n; ++i)
+      e[i] = p->AtomEnergy(*c, i);
+  });
+}
+
+// BasicParticleSet::SetTag (liblpmd/lpmd/taghandler.h:124-131) on atom i.
+int ref_set_atom_tag(void* conf, long i, const char* key, const char* value) {
+  return Guard([&] {
+    HarnessConf* c = static_cast<HarnessConf*>(conf);
+    BasicParticleSet& atoms = c->Atoms();
+    atoms.SetTag(atoms[i], Tag(key), std::string(value));
+  });
+}
+
 // Integrator::Initialize then `steps` Integrator::Advance calls on conf with
 // the given potentials (appended to the configuration's CombinedPotential);
 // epot[s] = the potential-energy tag after step s, when the integrator sets it.

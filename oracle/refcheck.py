@@ -55,6 +55,8 @@ def lib():
         L.ref_plugin_close.argtypes = [vp]
         L.ref_combined_forces.argtypes = [vp, i, vp, vp, vp, vp, vp]
         L.ref_integrate.argtypes = [vp, vp, i, vp, vp, vp, l, vp]
+        L.ref_atom_energies.argtypes = [vp, vp, i, i, l, vp]
+        L.ref_set_atom_tag.argtypes = [vp, l, C.c_char_p, C.c_char_p]
         _lib = L
     return _lib
 
@@ -132,9 +134,12 @@ class RefCellManager:
 
 
 class RefPot:
-    def __init__(self, name: str, args: str):
+    """A potential plugin loaded through its create(args): the reference's of
+    that name, or the .so at `path` (a drop-in)."""
+
+    def __init__(self, name: str, args: str, path: str | None = None):
         self.name = name
-        self.h = lib().ref_pot_open(so(name), (args + " debug none").encode())
+        self.h = lib().ref_pot_open(path.encode() if path else so(name), (args + " debug none").encode())
         if not self.h:
             raise RuntimeError(lib().ref_last_error().decode())
 
@@ -247,6 +252,19 @@ def combined_forces(conf: RefConf, pots, species=None):
     out = np.zeros(11)
     _check(lib().ref_combined_forces(conf.h, len(pots), hp, _p(s1), _p(s2), _p(acc), _p(out)))
     return acc, float(out[0]), float(out[1]), out[2:].reshape(3, 3)
+
+
+def atom_energies(pot: RefPlugin, conf: RefConf, s1: int = -1, s2: int = -1):
+    """Potential::AtomEnergy(conf, i) of every atom through the plugin, its
+    species rule (s1, s2) (-1, -1: every pair)."""
+    e = np.zeros(conf.n)
+    _check(lib().ref_atom_energies(pot.h, conf.h, s1, s2, conf.n, _p(e)))
+    return e
+
+
+def set_atom_tag(conf: RefConf, i: int, key: str, value: str = "true"):
+    """BasicParticleSet::SetTag(atoms[i], key, value)."""
+    _check(lib().ref_set_atom_tag(conf.h, int(i), key.encode(), value.encode()))
 
 
 def integrate(integrator: RefPlugin, conf: RefConf, pots, steps: int, species=None):

@@ -31,20 +31,68 @@ POTS = [
 ]
 
 
+METALS = ("suttonchen", "finnissinclair", "gupta", "finnissinclair-ext")
+
+
+def create_in_child(name, args):
+    """create(args) of the drop-in `name` in a child process through the
+    reference's loader: (exit status, stderr)."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r); from oracle import refcheck as R; "
+            "R.RefPlugin.dropin(%r, %r); print('created')" % (ROOT, name, args))
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    return p.returncode, p.stderr
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
 @pytest.mark.parametrize("name,args", POTS, ids=[f"{p[0]}-{k}" for k, p in enumerate(POTS)])
 def test_potential_plugins_match_reference_keywords(name, args):
     ours = R.RefPlugin.dropin(name, args)
     ref = R.RefPlugin.reference(name, args)
     # Potential::GetCutoff through the reference's interface
     assert R.lib().ref_pot_cutoff(ours.h) == R.lib().ref_pot_cutoff(ref.h)
-    # a dangling keyword is the same error in both
-    msgs = []
-    for make in (R.RefPlugin.dropin, R.RefPlugin.reference):
-        with pytest.raises(RuntimeError) as e:
-            make(name, args + " cutoff")
-        msgs.append(str(e.value))
-    assert msgs[0] == msgs[1]
     ours.close()
+
+
+@pytest.mark.parametrize("name,args", POTS[:2] + POTS[4:5],
+                         ids=[p[0] for p in POTS[:2] + POTS[4:5]])
+def test_bad_arguments_end_the_program_without_unwinding_out_of_create(name, args):
+    """A dangling keyword: the reference's create() throws (which the
+    reference's own loader cannot survive, the plugin being dlclosed while the
+    exception is in flight); the drop-in's create() prints the same message,
+    as lpmd::EndWithError does, and exits with status 1."""
+    with pytest.raises(RuntimeError) as e:
+        R.RefPlugin.reference(name, args + " cutoff")
+    rc, err = create_in_child(name, args + " cutoff")
+    assert rc == 1, err
+    assert "[Error] " + str(e.value) in err
+    rc, err = create_in_child(name, args)
+    assert rc == 0, err
+
+
+@pytest.mark.parametrize("name,args", POTS, ids=[f"{p[0]}-{k}" for k, p in enumerate(POTS)])
+def test_per_pair_virtuals_match_reference(name, args):
+    """The drop-ins are lpmd::PairPotential / lpmd::MetalPotential objects:
+    through dynamic_cast (as plugins/localpressure.cc:128-129 uses them) their
+    pairEnergy / pairForce, or pairEnergy / rhoij / F / PairForce / ManyBodies,
+    give the reference plugin's values bit for bit on 300 random pairs."""
+    rng = np.random.default_rng(abs(hash(name)) % 2**32)
+    ours = R.RefPot(name, args, path=os.path.join(R.DROPIN_DIR, name + ".so"))
+    ref = R.RefPot(name, args)
+    rc = ref.cutoff if np.isfinite(ref.cutoff) else 7.0
+    d = rng.normal(size=(300, 3))
+    rij = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0.6 * rc, 0.999 * rc, 300)[:, None]
+    if name in METALS:
+        rhoi, rhoj = rng.uniform(0.5, 40.0, 300), rng.uniform(0.5, 40.0, 300)
+        for a, b in zip(ours.metal(rij, rhoi, rhoj), ref.metal(rij, rhoi, rhoj)):
+            np.testing.assert_array_equal(a, b)
+    else:
+        for a, b in zip(ours.pair(rij), ref.pair(rij)):
+            np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("name,args", [("linkedcell", "cutoff 8.5 mode auto"),

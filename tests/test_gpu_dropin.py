@@ -194,3 +194,87 @@ def test_dropin_oldconfig_integrators_follow_native_path(integrator, scheme):
     assert dx <= 1e-10 and dv <= 1e-12
     if integrator == "dropin":
         assert np.all(np.isfinite(ep)) and ep[-1] != 0.0
+
+
+@pytest.mark.parametrize("name,args,kw,z,a", CASES[:2], ids=[c[0] for c in CASES[:2]])
+def test_dropin_atom_energy_matches_oracle(name, args, kw, z, a):
+    """Potential::AtomEnergy through the drop-in (PairPotential::AtomEnergy,
+    pairpotential.cc:21-33; MetalPotential::AtomEnergy, metalpotential.cc:47-60)
+    for every atom, computed on the device, against the oracle's restatement;
+    moving the atoms refreshes them."""
+    say("drop-in atom energy", name)
+    x, L = E.fcc(5, a)
+    x = np.mod(x + np.random.default_rng(12).normal(0.0, 0.03, x.shape), L)
+    rc = kw["cutoff"]
+    conf = R.RefConf(x, L, z=z)
+    cm = R.RefPlugin.dropin("verletlist", f"cutoff {rc} maxneighbors 400 each 5")
+    cm.attach(conf)
+    pot = R.RefPlugin.dropin(name, args)
+    oc = O.Conf(x, L, z=z)
+    ocm = O.CellManager("verletlist", rc, maxneighbors=400, each=5)
+    err = rel_rms(R.atom_energies(pot, conf), O.atom_energies(oc, ocm, O.Pot(name, **kw)))
+    say(f"  rel rms {err:.2e}")
+    assert err <= 1e-6
+    x2 = np.mod(x + 0.05, L)
+    conf.set_positions(x2)
+    oc2 = O.Conf(x2, L, z=z)
+    err2 = rel_rms(R.atom_energies(pot, conf), O.atom_energies(oc2, ocm, O.Pot(name, **kw)))
+    assert err2 <= 1e-6
+
+
+@pytest.mark.parametrize("scheme", ["velocityverlet", "leapfrog", "beeman"])
+def test_dropin_integrators_honour_fixed_atoms(scheme):
+    """Atoms tagged fixedpos ("true") and fixedvel through the reference's
+    BasicParticleSet::SetTag: the drop-in integrator's device steps keep the
+    fixedpos atoms where they are with their velocities and zero the fixedvel
+    atoms' accelerations, following the oracle's TwoStepIntegrator /
+    OneStepIntegrator restatement (velocity Verlet) and the engine's own
+    tagged run (all three schemes)."""
+    say("drop-in fixed atoms", scheme)
+    x, L = E.fcc(4, 5.26)
+    x = np.mod(x + np.random.default_rng(13).normal(0.0, 0.05, x.shape), L)
+    n = x.shape[0]
+    v = E.maxwell_boltzmann(np.full(n, E.ELEMENT_MASS[18]), 100.0, seed=13)
+    fv = (np.arange(n) % 9) == 2
+    fp = (np.arange(n) % 10) == 5
+    pot = ("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    cm_args = "cutoff 8.5 maxneighbors 400 each 5 skin 0.5"
+    steps, dt = 25, 2.0
+    conf = R.RefConf(x, L, z=18)
+    conf.set_velocities(v)
+    for i in np.nonzero(fv)[0]:
+        R.set_atom_tag(conf, i, "fixedvel", "true")
+    for i in np.nonzero(fp)[0]:
+        R.set_atom_tag(conf, i, "fixedpos", "true")
+    cm = R.RefPlugin.dropin("verletlist", cm_args)
+    cm.attach(conf)
+    pots = [R.RefPlugin.dropin(*pot)]
+    itg = R.RefPlugin.dropin(scheme, f"dt {dt}")
+    R.integrate(itg, conf, pots, steps)
+    xr, vr, ar = conf.get()
+    np.testing.assert_array_equal(xr[fp], x[fp])
+    np.testing.assert_array_equal(vr[fp], v[fp])
+    assert np.all(ar[fv] == 0.0)
+    s = E.System(x, L, 18, v=v)
+    s.set_cellmanager("verletlist", cm_args)
+    s.append_potential(*pot)
+    s.set_fixed(fixedvel=fv, fixedpos=fp)
+    if scheme == "leapfrog":
+        s.leapfrog_init(dt)
+        s.leapfrog(dt, steps)
+    elif scheme == "beeman":
+        s.beeman_init(dt)
+        s.beeman(dt, steps)
+    else:
+        s.velocityverlet(dt, steps)
+    xn, vn, an = s.get_state()
+    assert np.abs(xr - xn).max() <= 1e-10 and np.abs(vr - vn).max() <= 1e-12
+    if scheme == "velocityverlet":
+        oc = O.Conf(x, L, z=18, v=v)
+        oc.set_fixed(fixedvel=fv, fixedpos=fp)
+        O.md_run(oc, O.CellManager("minimumimage", 8.5), [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103,
+                                                                   cutoff=8.5)], dt, steps)
+        dx = xr - oc.x
+        dx -= L * np.round(dx / L)
+        say(f"  vs oracle max |dx| {np.abs(dx).max():.2e} |dv| {np.abs(vr - oc.v).max():.2e}")
+        assert np.abs(dx).max() < 1e-7 and np.abs(vr - oc.v).max() < 2e-8
