@@ -463,6 +463,12 @@ struct lpmdx_batch {
 
   // ---------------------------------------------------------------- launch plan
   void plan() {
+    // The graph of the last plan goes first: its kernels read the tables and
+    // buffers reallocated below.  The stream is idle (every run ends with a
+    // synchronisation); it is waited on again so that no launch of the old
+    // graph can still be in flight when it is destroyed.
+    sync();
+    release_graph();
     // system table and sort chunks
     std::vector<int2> ch;
     passes = 1;
@@ -515,6 +521,10 @@ struct lpmdx_batch {
              "memcpy");
       groups.push_back(std::move(G));
     }
+    // the branch streams and events of the step exist before the capture
+    // starts: nothing is created while the stream is being captured
+    if (groups.size() > 1)
+      ensure_branches(groups.size());
     sync();
     capture();
     dirty = false;
@@ -594,7 +604,8 @@ struct lpmdx_batch {
       launch_kind(*groups[0], st);
     } else if (groups.size() > 1) {
       // kinds touch disjoint systems: one branch each, joined before the step ends
-      ensure_branches(groups.size());
+      if (bst.size() < groups.size() || !ev_fork)
+        fail(LPMDX_EDEVICE, "batch: branch streams missing (internal error)");
       hipchk(hipEventRecord(ev_fork, st), "hipEventRecord");
       for (size_t g = 0; g < groups.size(); ++g) {
         hipchk(hipStreamWaitEvent(bst[g], ev_fork, 0), "hipStreamWaitEvent");
@@ -606,13 +617,17 @@ struct lpmdx_batch {
     hipchk(hipGetLastError(), "kernel launch");
   }
 
-  void capture() {
+  void release_graph() {
     if (gexec)
       (void)hipGraphExecDestroy(gexec);
     if (graph)
       (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
+  }
+
+  void capture() {
+    release_graph();
     hipchk(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     enqueue_step();
     hipchk(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
