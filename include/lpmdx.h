@@ -198,6 +198,26 @@ LPMDX_API int lpmdx_beeman_init(lpmdx_system* sys, double dt);
 LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
                            double* ekin_t);
 
+/* The nosehoover integrator (plugins/nosehoover.cc:62-96), a two-step scheme
+ * over Beeman's updates with a friction term.  lpmdx_nosehoover_init is
+ * NoseHoover::Initialize with the plugin's keywords dt, fmass (Q) and t: the
+ * forces at the old configuration FittedInside(x - v dt)
+ * (Integrator::GoBack, integrator.cc:31-42) become the first old
+ * acceleration, the friction starts at zero; x, v and a are left as they
+ * were.  lpmdx_nosehoover then runs `steps` TwoStepIntegrator::Advance calls
+ * (twostepintegrator.cc:12-68): each position update adds to the friction
+ * D = (3 N / Q) k_B (T - t) dt once per atom in atom order (T the temperature
+ * of the velocities, k_B the session's kboltzmann 0.00008617), so atom i
+ * integrates with newacc = a - v f_i, f_i the friction after its i + 1
+ * increments (formed as F + (i + 1) D rather than as the reference's
+ * sequential sum: the same value to FP64 rounding).  One force evaluation per
+ * step as in lpmdx_velocityverlet; energies every obs_every steps as there.
+ * Without a prior init: LPMDX_ESTATE; another dt: LPMDX_EINVAL; with the
+ * tempscaling / berendsen modifiers or fixedpos atoms: LPMDX_EINVAL. */
+LPMDX_API int lpmdx_nosehoover_init(lpmdx_system* sys, double dt, double fmass, double t);
+LPMDX_API int lpmdx_nosehoover(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
+                               double* ekin_t);
+
 /* SimulationEngine::SetTemperature (liblpmd/simulation/simulationbuilder.cc:53-71)
  * on the device.  While no velocities were set (a new system: velocitiesSet
  * is false until InitVelocities or Restore) it first runs InitVelocities

@@ -87,6 +87,8 @@ def lib():
         L.lpmdx_init_velocities.argtypes = [vp]
         L.lpmdx_velocities_set.argtypes = [vp, i]
         L.lpmdx_system_set_fixed.argtypes = [vp, vp]
+        L.lpmdx_nosehoover_init.argtypes = [vp, d, d, d]
+        L.lpmdx_nosehoover.argtypes = [vp, d, l, l, vp, vp]
         _lib = L
     return _lib
 
@@ -198,6 +200,17 @@ class System:
         ep = np.zeros(max(k, 1))
         ek = np.zeros(max(k, 1))
         _check(lib().lpmdx_beeman(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
+        return ep[:k], ek[:k]
+
+    def nosehoover_init(self, dt: float, fmass: float, t: float):
+        """NoseHoover::Initialize (keywords dt, fmass, t of plugins/nosehoover.cc)."""
+        _check(lib().lpmdx_nosehoover_init(self._h, dt, fmass, t))
+
+    def nosehoover(self, dt: float, steps: int, obs_every: int = 0):
+        k = steps // obs_every if obs_every else 0
+        ep = np.zeros(max(k, 1))
+        ek = np.zeros(max(k, 1))
+        _check(lib().lpmdx_nosehoover(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
         return ep[:k], ek[:k]
 
     def set_temperature(self, temp: float):

@@ -219,7 +219,7 @@ static __global__ __launch_bounds__(kHaloThreads) void k_brick_halo(const int* _
 
 // Reorders the per-atom state into the new (cell-sorted) order.  order[k] is
 // the current slot of the atom placed at k; perm maps slots to caller indices.
-constexpr int kPermuteExtra = 4;
+constexpr int kPermuteExtra = 6;
 
 struct PermuteArgs {
   const uint32_t* order;

@@ -384,6 +384,22 @@ int lpmdx_beeman(lpmdx_system* S, double dt, long steps, long obs_every, double*
   });
 }
 
+int lpmdx_nosehoover_init(lpmdx_system* S, double dt, double fmass, double t) {
+  return guard([&] {
+    need(S, "system");
+    S->nh_init(dt, fmass, t);
+  });
+}
+
+int lpmdx_nosehoover(lpmdx_system* S, double dt, long steps, long obs_every, double* epot_t, double* ekin_t) {
+  return guard([&] {
+    need(S, "system");
+    if (steps < 0 || obs_every < 0)
+      fail(LPMDX_EINVAL, "negative step count");
+    S->vv(dt, steps, obs_every, epot_t, ekin_t, false, true);
+  });
+}
+
 int lpmdx_set_temperature(lpmdx_system* S, double temp) {
   return guard([&] {
     need(S, "system");
@@ -469,6 +485,11 @@ int lpmdx_system_set_fixed(lpmdx_system* S, const int* flags) {
         tagged += (flags[i] & LPMDX_FIX_TAGGED) ? 1 : 0;
       }
     S->undrift(); // a look-ahead was made without these flags
+    bool anypos = false;
+    if (flags)
+      for (long i = 0; i < n; ++i)
+        anypos = anypos || (flags[i] & LPMDX_FIX_POS) != 0;
+    S->fixpos_any = anypos;
     if (tagged == 0) {
       S->any_fix = false;
       S->n_tagged = 0;

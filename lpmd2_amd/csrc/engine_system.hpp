@@ -196,6 +196,7 @@ struct lpmdx_system {
   // order) while some atom carries the fixedvel or fixedpos tag
   DBuf<int> fix, fix2;
   bool any_fix = false; // some atom has kFixVel or kFixPos
+  bool fixpos_any = false; // some atom has kFixPos
   long n_tagged = 0;    // atoms with kFixTagged
   // SimulationEngine's velocitiesSet (simulationbuilder.cc:54-55,184,290) and
   // the system's drand48 stream for InitVelocities; masses in caller order
@@ -251,6 +252,12 @@ struct lpmdx_system {
   // copy; drift_beeman says which scheme made the look-ahead x
   DBuf<double> ab, ab2;
   bool bm_ready = false, drift_beeman = false;
+  // Nose-Hoover (plugins/nosehoover.cc): ab holds old.a, nn the
+  // friction-modified acceleration of the position update still to be ended,
+  // nh = {friction before the step's increments, increment per atom}
+  DBuf<double> nn, nn2, nh;
+  bool nh_ready = false, drift_nose = false;
+  double nh_dt = 0.0, nh_q = 0.0, nh_t = 0.0;
   double bm_dt = 0.0;
   double lf_dt = 0.0;
   // more arrays that reorders carry along (with their scratch twins)
@@ -691,8 +698,10 @@ struct lpmdx_system {
     std::vector<std::pair<DBuf<double>*, DBuf<double>*>> ex;
     if (lf_ready)
       ex.push_back({&vh, &vh2});
-    if (bm_ready)
+    if (bm_ready || nh_ready)
       ex.push_back({&ab, &ab2});
+    if (nh_ready)
+      ex.push_back({&nn, &nn2});
     for (auto& e : carry)
       ex.push_back(e);
     if (ex.size() > (size_t)kPermuteExtra)
@@ -1004,6 +1013,7 @@ struct lpmdx_system {
       A.xb = xb.p;
       A.vh = vh.p;
       A.ab = ab.p;
+      A.nn = nn.p;
       A.fix = fix_arg();
       A.margin = wrap_margin();
       if (staged && !(A.margin > 0.0))
@@ -1050,6 +1060,7 @@ struct lpmdx_system {
       A.xb = xb.p;
       A.vh = vh.p;
       A.ab = ab.p;
+      A.nn = nn.p;
       A.fix = fix_arg();
       A.b = box;
       for (int k = 0; k < 3; ++k)
@@ -1274,8 +1285,9 @@ struct lpmdx_system {
   // ------------------------------------------------------------------ VV
   // beeman: the same two-step scheme with Beeman's updates (beeman.cc:63-84)
   // in place of velocity Verlet's; ab carries its extra history.
-  void vv(double dt, long steps, long obs_every, double* ep_t, double* ek_t, bool beeman = false) {
-    const char* who = beeman ? "beeman" : "velocityverlet";
+  void vv(double dt, long steps, long obs_every, double* ep_t, double* ek_t, bool beeman = false,
+          bool nose = false) {
+    const char* who = nose ? "nosehoover" : beeman ? "beeman" : "velocityverlet";
     if (pots.empty())
       fail(LPMDX_ESTATE, "no potential set");
     if (cm.kind == kNone)
@@ -1286,13 +1298,23 @@ struct lpmdx_system {
       fail(LPMDX_ESTATE, "beeman: lpmdx_beeman_init was not called");
     if (beeman && !(dt == bm_dt))
       fail(LPMDX_EINVAL, "beeman: dt differs from the one lpmdx_beeman_init used");
+    if (nose && !nh_ready)
+      fail(LPMDX_ESTATE, "nosehoover: lpmdx_nosehoover_init was not called");
+    if (nose && !(dt == nh_dt))
+      fail(LPMDX_EINVAL, "nosehoover: dt differs from the one lpmdx_nosehoover_init used");
+    if (nose && (tsched.on || bsched.on))
+      fail(LPMDX_EINVAL, "nosehoover: the tempscaling and berendsen modifiers are not run with it");
+    if (nose && fixpos_any)
+      fail(LPMDX_EINVAL, "nosehoover: fixedpos atoms are not supported (the friction count skips them)");
     if (n == 0 || steps <= 0)
       return;
     // a look-ahead made by the other scheme is taken back (a stays valid)
-    if (drifted && drift_beeman != beeman)
+    if (drifted && (drift_beeman != beeman || drift_nose != nose))
       undrift();
-    const int init_mode = beeman ? kBeemanInit : kInitDrift;
-    const int step_mode = beeman ? kBeemanStep : kStep;
+    if (nose && (!nn.p || !ab.p || !nh.p))
+      fail(LPMDX_EDEVICE, "nosehoover: history arrays missing (internal error)");
+    const int init_mode = nose ? kNHStart : beeman ? kBeemanInit : kInitDrift;
+    const int step_mode = nose ? kNHStep : beeman ? kBeemanStep : kStep;
     if (tsched.on || bsched.on)
       partials.alloc(std::max<size_t>((size_t)blocks_for(n) * 4, partials.n));
     unsigned int* md = ctrl.p->maxd2;
@@ -1306,7 +1328,11 @@ struct lpmdx_system {
       if (!forces_ok)
         update_cell();
       hipchk(hipMemsetAsync(md, 0, sizeof(unsigned), st), "memset");
-      if (!forces_ok) {
+      if (nose) {
+        if (!forces_ok)
+          forces(init_mode, false, dt, md);
+        nh_position(dt, md, nullptr);
+      } else if (!forces_ok) {
         forces(init_mode, false, dt, md);
       } else {
         ForceArgs A{};
@@ -1330,6 +1356,7 @@ struct lpmdx_system {
       std::swap(x.p, xn.p);
       drifted = true;
       drift_beeman = beeman;
+      drift_nose = nose;
       forces_ok = true;
       hipchk(hipMemcpyAsync(&hctrl->maxd2[0], md, sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       sync();
@@ -1378,6 +1405,8 @@ struct lpmdx_system {
         const unsigned int* gate = (every || s == 0) ? nullptr : md + s - 1;
         prof_step = n_steps + s;
         forces(step_mode, obs, dt, md + s, gate);
+        if (nose)
+          nh_position(dt, md + s, gate);
         std::swap(x.p, xn.p);
         modifiers_after_step(n_steps + s + 1, beeman, dt, md + s, gate);
       }
@@ -1422,6 +1451,7 @@ struct lpmdx_system {
       chunk = std::max<long>(1, next);
       drifted = true;
       drift_beeman = beeman;
+      drift_nose = nose;
       cm.ucalls += 2 * ran;
       done += ran;
       n_steps += ran;
@@ -1529,6 +1559,75 @@ struct lpmdx_system {
     hipchk(hipMemcpyAsync(ab.p, a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
     restore_after_old_config();
     bm_ready = true;
+    sync();
+  }
+
+  // Nose-Hoover: the position update of the step whose forces a holds (x the
+  // positions, v the velocities after the last step): the kinetic sum of v,
+  // the step's friction (k_nh_friction), then k_nh_drift writes xn and nn.
+  void nh_position(double dt, unsigned int* maxd2, const unsigned int* gate) {
+    const long nb = blocks_for(n);
+    partials.alloc(std::max<size_t>((size_t)nb * 4, partials.n));
+    k_kinetic<<<nb, kBlock, 0, st>>>(v.p, mass.p, n, partials.p);
+    k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
+    // the session hands kboltzmann over as the string "0.00008617"
+    // (liblpmd/runtime/session.cc:17-23,32)
+    k_nh_friction<<<1, 1, 0, st>>>(red.p, nh.p, n, nh_q, nh_t, dt, 0.00008617, gate, gate ? budget() : 0.0);
+    ForceArgs A{};
+    A.x = x.p;
+    A.xn = xn.p;
+    A.v = v.p;
+    A.a = a.p;
+    A.ab = ab.p;
+    A.nn = nn.p;
+    A.nh = nh.p;
+    A.perm = perm.p;
+    A.fix = fix_arg();
+    A.maxd2 = maxd2;
+    A.gate = gate;
+    A.gate_b = gate ? budget() : 0.0;
+    A.xb = xb.p;
+    A.b = box;
+    A.n = n;
+    A.dt = dt;
+    // a stepping mode: flush_maxd2 records the displacement only outside kForcesOnly
+    A.mode = kNHStep;
+    for (int q = 0; q < 3; ++q)
+      A.ll[q] = ll[q];
+    k_nh_drift<<<nb, kBlock, 0, st>>>(A);
+    launch_check();
+  }
+
+  // NoseHoover::Initialize (plugins/nosehoover.cc:62-68): the forces at the
+  // old configuration of Integrator::GoBack become old.a (in ab), the
+  // friction starts at zero; x, v and a are left as they were.
+  void nh_init(double dt, double fmass, double t) {
+    if (pots.empty())
+      fail(LPMDX_ESTATE, "no potential set");
+    if (cm.kind == kNone)
+      fail(LPMDX_ESTATE, "cellmanager not set");
+    if (!std::isfinite(dt) || !std::isfinite(fmass) || !std::isfinite(t))
+      fail(LPMDX_ENONFINITE, "nosehoover: dt, fmass and t must be finite");
+    if (fmass == 0.0)
+      fail(LPMDX_EINVAL, "nosehoover: fmass must not be zero");
+    undrift();
+    nh_ready = false;
+    nh_dt = dt;
+    nh_q = fmass;
+    nh_t = t;
+    nh.alloc(2);
+    k_fill_double<<<1, 32, 0, st>>>(nh.p, 2, 0.0);
+    launch_check();
+    if (n == 0) {
+      nh_ready = true;
+      return;
+    }
+    old_config_forces(dt, false);
+    ab.alloc(3 * n);
+    nn.alloc(3 * n);
+    hipchk(hipMemcpyAsync(ab.p, a.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    restore_after_old_config();
+    nh_ready = true;
     sync();
   }
 

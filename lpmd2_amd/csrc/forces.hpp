@@ -261,7 +261,12 @@ enum StepMode {
   kStep = 2,       // a = F/m, v += dt/2 (a_old + a), xn = drift(x, v, a)
   kLeapfrog = 3,   // a = F/m, vh' = vh + a dt, xn = FittedInside(x + vh' dt), v = (vh' + vh) / 2, vh = vh'
   kBeemanInit = 4, // a = F/m, then xn = Beeman position update with a_old = ab
-  kBeemanStep = 5  // a = F/m, Beeman velocity update from (a, old a, ab), ab = old a, xn = position update
+  kBeemanStep = 5, // a = F/m, Beeman velocity update from (a, old a, ab), ab = old a, xn = position update
+  // Nose-Hoover (plugins/nosehoover.cc:66-96): the force kernel ends the step
+  // whose positions it sees; the next step's position update follows in
+  // k_nh_drift, once the step's friction is known
+  kNHStart = 6, // a = F/m
+  kNHStep = 7   // a = F/m, v += (a/3 + 5 nn/6 - ab/6) dt, ab = nn
 };
 
 // Beeman's position update before FittedInside (beeman.cc:63-76), in its order
@@ -289,7 +294,11 @@ struct ForceArgs {
   double gate_b;            // the skin budget that slot is held against
   const double* xb;     // [3n] positions at the last list build
   double* vh;           // [3n] leapfrog half-step velocities (kLeapfrog)
-  double* ab;           // [3n] Beeman: the acceleration of the step before a's (kBeeman*)
+  double* ab;           // [3n] Beeman: the acceleration of the step before a's (kBeeman*);
+                        // Nose-Hoover: the friction-modified acceleration of the last position update
+  double* nn;           // [3n] Nose-Hoover: that of the position update still to be ended
+  const double* nh;     // Nose-Hoover: {friction before this step's increments, increment per atom}
+  const int* perm;      // caller index of each slot (Nose-Hoover: the order of the increments)
   // Largest separation a listed pair can have now (list radius + padding +
   // twice the displacement allowed since the build), or 0.  An atom at least
   // this far from every face has no listed neighbour across the boundary.
@@ -405,6 +414,30 @@ __device__ __forceinline__ void vv_tail(const ForceArgs& A, long i, double ax, d
 #pragma unroll
     for (int q = 0; q < 3; ++q)
       xn[q] = fitted_inside(p[q], A.ll[q]);
+    return;
+  }
+  if (A.mode == kNHStart || A.mode == kNHStep) {
+    // Nose-Hoover: NoseHoover::AdvanceVelocity (nosehoover.cc:90-96) of the
+    // step whose positions these forces see, with old.a = nn (the
+    // friction-modified acceleration of its position update) and aux = ab;
+    // the history then moves on (ab = nn).  The start only sets a.
+    const double an[3] = {ax, ay, az};
+    if (A.mode == kNHStep && !(A.fix && (A.fix[i] & kFixPos))) {
+      double* ab = A.ab + 3 * i;
+      const double* nn = A.nn + 3 * i;
+      const double dt = A.dt;
+      const double vo[3] = {v[0], v[1], v[2]};
+      const double no[3] = {nn[0], nn[1], nn[2]};
+      const double ap[3] = {ab[0], ab[1], ab[2]};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        v[q] = ((vo[q] + ((1.0 / 3.0) * an[q]) * dt) + ((5.0 / 6.0) * no[q]) * dt) - ((1.0 / 6.0) * ap[q]) * dt;
+        ab[q] = no[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      a[q] = an[q];
     return;
   }
   if (A.mode == kBeemanInit || A.mode == kBeemanStep) {
@@ -1172,6 +1205,54 @@ static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
     d2 = build_displacement2(A, i, p[0], p[1], p[2]);
   }
   flush_maxd2(A, d2);
+}
+
+// Nose-Hoover's position update (NoseHoover::AdvancePosition,
+// plugins/nosehoover.cc:73-88) once the step's friction increment is known:
+// atom i (caller index c) sees the friction after c + 1 increments,
+// f = F + (c + 1) D, with A.nh = {F, D}; newacc = a - v f, the position
+// ((x + v dt) + (2/3 newacc) dt^2) - (1/6 ab) dt^2 folded into the cell,
+// and nn = newacc (the next velocity update's old.a).  a keeps the force.
+static __global__ __launch_bounds__(kBlock) void k_nh_drift(ForceArgs A) {
+  if (gate_shut(A))
+    return;
+  long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  float d2 = 0.0f;
+  if (i < A.n) {
+    const double f = A.nh[0] + (double)(A.perm[i] + 1) * A.nh[1];
+    const double dt = A.dt;
+    // a fixedvel atom's acceleration is zero when the position update runs
+    // (the step's force call may have been made outside a step)
+    const bool zero = A.fix && (A.fix[i] & kFixVel);
+    double p[3];
+    for (int q = 0; q < 3; ++q) {
+      if (zero)
+        A.a[3 * i + q] = 0.0;
+      const double na = A.a[3 * i + q] - A.v[3 * i + q] * f;
+      p[q] = beeman_position(A.x[3 * i + q], A.v[3 * i + q], na, A.ab[3 * i + q], dt);
+      A.nn[3 * i + q] = na;
+      A.xn[3 * i + q] = fitted_inside(p[q], A.ll[q]);
+    }
+    d2 = build_displacement2(A, i, p[0], p[1], p[2]);
+  }
+  flush_maxd2(A, d2);
+}
+
+// The friction of a Nose-Hoover step from the kinetic sum red[0]
+// (sum of m v^2 / 2): nh[0] moves past the previous step's N increments,
+// nh[1] = D = ((3 N / Q) k_B (T - T_set)) dt with T = (2/3) K / (k_B' N)
+// (properties.h:39-53, k_B' its KBOLTZMANN) and k_B the session's
+// kboltzmann (nosehoover.cc:69,78-79).  gate as the force kernels'.
+static __global__ void k_nh_friction(const double* __restrict__ red, double* __restrict__ nh, long n, double q,
+                                     double temp, double dt, double kboltzmann, const unsigned int* gate,
+                                     double gate_b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  if (gate && (*gate == kGateClosed || sqrt((double)__uint_as_float(*gate)) > gate_b))
+    return;
+  const double T = (2.0 / 3.0) * (red[0] * kKin2eV) / (kBoltzmann * double(n));
+  nh[0] = nh[0] + double(n) * nh[1];
+  nh[1] = ((3.0 * double(n) / q) * kboltzmann * (T - temp)) * dt;
 }
 
 // Kinetic energy and momentum partials (properties.h:22-81).  skip: the

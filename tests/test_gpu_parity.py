@@ -326,6 +326,67 @@ def test_beeman_trajectory_matches_oracle(cm):
     np.testing.assert_allclose(ek2, k30[29:], rtol=1e-6)
 
 
+@pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
+def test_nosehoover_trajectory_matches_oracle(cm):
+    """The nosehoover integrator (plugins/nosehoover.cc:62-96): set-up on the
+    old configuration, then 20 + 10 steps of LJ argon started at 100 K and
+    driven toward 300 K (Q 1e7), in two calls, one
+    fixedvel atom among them: positions within the FP32 force noise the
+    trajectory accumulates, velocities and the sampled energies against the
+    oracle's sequential restatement; the friction has visibly heated the box.
+    (The reference's friction increment per atom is (3 N / Q) k_B (T - t) dt,
+    so a gentle thermostat for 256 atoms needs Q near 1e7.)"""
+    say("nosehoover", cm)
+    x, L = E.fcc(4, 5.26)
+    m = np.full(x.shape[0], 39.948)
+    v = E.maxwell_boltzmann(m, 100.0, seed=14)
+    x[0] = [0.002, 0.003, 0.004]
+    v[0] = [0.003, 0.004, 0.005]
+    fv = np.zeros(x.shape[0], bool)
+    fv[7] = True
+    Q, T0, dt = 1e7, 300.0, 2.0
+
+    def cms():
+        if cm == "linkedcell":
+            return "cutoff 8.5 mode auto", O.CellManager("linkedcell", 8.5, mode="auto")
+        return "cutoff 8.5 maxneighbors 200 each 5 skin 0.5", O.CellManager("minimumimage", 8.5)
+
+    args, om = cms()
+    s = E.System(x, L, 18, v=v)
+    s.set_cellmanager(cm, args)
+    s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
+    s.set_fixed(fixedvel=fv)
+    s.nosehoover_init(dt, Q, T0)
+    ep, ek = s.nosehoover(dt, 20, obs_every=5)
+    xg, vg, ag = s.get_state()
+    pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
+    conf = O.Conf(x, L, v=v)
+    conf.set_fixed(fixedvel=fv)
+    e0, k0 = O.nosehoover_run(conf, lambda c: O.update_forces(c, om, pots)[0], dt, 20, Q, T0)
+    dx = xg - conf.x
+    dx -= L * np.round(dx / L)
+    say(f"  max |dx| {np.abs(dx).max():.2e} |dv| {np.abs(vg - conf.v).max():.2e}  Ek {ek} vs {k0[4::5]}")
+    assert np.abs(dx).max() < 1e-7
+    assert np.abs(vg - conf.v).max() < 2e-8
+    assert rel_rms(ag, conf.a) <= FORCE_TOL
+    np.testing.assert_allclose(ep, e0[4::5], rtol=1e-6)
+    np.testing.assert_allclose(ek, k0[4::5], rtol=1e-6)
+    assert ek[-1] > 1.5 * ek[0]  # the thermostat pulls toward 300 K
+    ep2, ek2 = s.nosehoover(dt, 10, obs_every=10)
+    _, om30 = cms()
+    c30 = O.Conf(x, L, v=v)
+    c30.set_fixed(fixedvel=fv)
+    e30, k30 = O.nosehoover_run(c30, lambda c: O.update_forces(c, om30, pots)[0], dt, 30, Q, T0)
+    np.testing.assert_allclose(ep2, e30[29:], rtol=1e-6)
+    np.testing.assert_allclose(ek2, k30[29:], rtol=1e-6)
+    if cm == "verletlist":
+        # the skin test saw the Nose-Hoover position updates: the list was rebuilt
+        assert s.stats()["builds"] >= 2
+    with pytest.raises(E.LpmdxError) as e:
+        s.nosehoover(1.0, 1)
+    assert e.value.code == E.EINVAL
+
+
 def test_beeman_free_flight_is_bit_exact():
     """Zero forces: the set-up and every step's Beeman position and velocity
     arithmetic, bit for bit."""

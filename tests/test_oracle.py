@@ -230,6 +230,48 @@ def test_beeman_constant_acceleration_closed_form():
     np.testing.assert_allclose(conf.v, v0 + a0 * t, rtol=1e-12)
 
 
+def test_nosehoover_without_friction_is_the_reference_beeman():
+    """NoseHoover with a fictitious mass so large that the friction underflows
+    the accelerations is Beeman's scheme (the same set-up and weights): with
+    the forces of the golden Beeman run replayed, the trajectory is that
+    reference plugin's, bit for bit (bm_ar108, written by plugins/beeman.cc)."""
+    d = np.load(os.path.join(GOLDEN, "bm_ar108.npz"))
+    conf = O.Conf(d["x0"], d["L"], v=d["v0"])
+    acc = d["acc"]
+    O.nosehoover_run(conf, _replay(acc), 2.0, (acc.shape[0] - 1) // 2, fmass=1e300, temp=300.0)
+    assert np.array_equal(conf.x, d["xf"])
+    assert np.array_equal(conf.v, d["vf"])
+
+
+def test_nosehoover_friction_closed_form():
+    """Two free atoms (no forces), one step by hand (nosehoover.cc:73-96):
+    T from the initial velocities, increment D = (3 N / Q) k_B (T - T0) dt;
+    atom 0 integrates with friction D, atom 1 with 2 D (the friction grows
+    once per atom), newacc_i = -v_i f_i, x' = x + v dt + (2/3) newacc dt^2,
+    v' = v + (5/6) newacc dt; the second step starts from friction 2 D."""
+    v = np.array([[0.01, 0.0, -0.004], [0.0, -0.006, 0.002]])
+    x = np.array([[5.0, 5.0, 5.0], [12.0, 12.0, 12.0]])
+    conf = O.Conf(x, [30.0] * 3, z=18, v=v)
+    zero = lambda c: (c.a.__setitem__(slice(None), 0.0), 0.0)[1]
+    Q, T0, dt, kb = 2.5, 40.0, 2.0, 0.00008617
+    T = O.temperature(conf)
+    D = ((3.0 * 2 / Q) * kb * (T - T0) * dt)
+    O.nosehoover_run(conf, zero, dt, 1, fmass=Q, temp=T0)
+    f = np.array([D, D + D])[:, None]
+    na = -v * f
+    np.testing.assert_allclose(conf.x, x + v * dt + (2.0 / 3.0) * na * dt * dt, rtol=1e-15)
+    np.testing.assert_allclose(conf.v, v + (5.0 / 6.0) * na * dt, rtol=1e-15)
+    # step two: friction 2D + D for atom 0, 2D + 2D for atom 1, T of the new velocities
+    conf2 = O.Conf(x, [30.0] * 3, z=18, v=v)
+    O.nosehoover_run(conf2, zero, dt, 2, fmass=Q, temp=T0)
+    v1 = v + (5.0 / 6.0) * na * dt
+    T1 = O.temperature(O.Conf(x, [30.0] * 3, z=18, v=v1))
+    D1 = ((3.0 * 2 / Q) * kb * (T1 - T0) * dt)
+    f2 = np.array([D + D + D1, D + D + D1 + D1])[:, None]
+    na2 = -v1 * f2
+    np.testing.assert_allclose(conf2.v, v1 + (5.0 / 6.0) * na2 * dt - (1.0 / 6.0) * na * dt, rtol=1e-14)
+
+
 def test_stepper_known_answers():
     """Stepper::IsActiveInStep / ValueAtStep (stepper.h:17-31) by hand."""
     st = O.Stepper(10, 50, 20)
