@@ -208,12 +208,15 @@ LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_ev
  * (twostepintegrator.cc:12-68): each position update adds to the friction
  * D = (3 N / Q) k_B (T - t) dt once per atom in atom order (T the temperature
  * of the velocities, k_B the session's kboltzmann 0.00008617), so atom i
- * integrates with newacc = a - v f_i, f_i the friction after its i + 1
- * increments (formed as F + (i + 1) D rather than as the reference's
- * sequential sum: the same value to FP64 rounding).  One force evaluation per
+ * integrates with newacc = a - v f_i, f_i the friction after its r_i + 1
+ * increments, r_i the number of moving atoms before it (formed as
+ * F + (r_i + 1) D rather than as the reference's sequential sum: the same
+ * value to FP64 rounding).  One force evaluation per
  * step as in lpmdx_velocityverlet; energies every obs_every steps as there.
- * Without a prior init: LPMDX_ESTATE; another dt: LPMDX_EINVAL; with the
- * tempscaling / berendsen modifiers or fixedpos atoms: LPMDX_EINVAL. */
+ * A fixedpos "true" atom is not advanced and adds no increment
+ * (twostepintegrator.cc:35-45); the tempscaling / berendsen modifiers act
+ * after each step, before the next position update reads the temperature.
+ * Without a prior init: LPMDX_ESTATE; another dt: LPMDX_EINVAL. */
 LPMDX_API int lpmdx_nosehoover_init(lpmdx_system* sys, double dt, double fmass, double t);
 LPMDX_API int lpmdx_nosehoover(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
                                double* ekin_t);

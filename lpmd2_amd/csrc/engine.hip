@@ -490,6 +490,19 @@ int lpmdx_system_set_fixed(lpmdx_system* S, const int* flags) {
       for (long i = 0; i < n; ++i)
         anypos = anypos || (flags[i] & LPMDX_FIX_POS) != 0;
     S->fixpos_any = anypos;
+    if (anypos) {
+      // the Nose-Hoover friction grows once per moving atom, in caller order
+      std::vector<int> rank(n);
+      long moving = 0;
+      for (long i = 0; i < n; ++i) {
+        rank[i] = (int)moving;
+        moving += (flags[i] & LPMDX_FIX_POS) ? 0 : 1;
+      }
+      S->mv_rank.alloc(n);
+      hipchk(hipMemcpyAsync(S->mv_rank.p, rank.data(), n * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
+      S->sync();
+      S->n_moving = moving;
+    }
     if (tagged == 0) {
       S->any_fix = false;
       S->n_tagged = 0;

@@ -258,6 +258,10 @@ struct lpmdx_system {
   DBuf<double> nn, nn2, nh;
   bool nh_ready = false, drift_nose = false;
   double nh_dt = 0.0, nh_q = 0.0, nh_t = 0.0;
+  // moving atoms (no fixedpos "true"): their count, and per caller index the
+  // number of moving atoms before it (filled while any atom is held)
+  long n_moving = 0;
+  DBuf<int> mv_rank;
   double bm_dt = 0.0;
   double lf_dt = 0.0;
   // more arrays that reorders carry along (with their scratch twins)
@@ -1302,10 +1306,6 @@ struct lpmdx_system {
       fail(LPMDX_ESTATE, "nosehoover: lpmdx_nosehoover_init was not called");
     if (nose && !(dt == nh_dt))
       fail(LPMDX_EINVAL, "nosehoover: dt differs from the one lpmdx_nosehoover_init used");
-    if (nose && (tsched.on || bsched.on))
-      fail(LPMDX_EINVAL, "nosehoover: the tempscaling and berendsen modifiers are not run with it");
-    if (nose && fixpos_any)
-      fail(LPMDX_EINVAL, "nosehoover: fixedpos atoms are not supported (the friction count skips them)");
     if (n == 0 || steps <= 0)
       return;
     // a look-ahead made by the other scheme is taken back (a stays valid)
@@ -1405,10 +1405,16 @@ struct lpmdx_system {
         const unsigned int* gate = (every || s == 0) ? nullptr : md + s - 1;
         prof_step = n_steps + s;
         forces(step_mode, obs, dt, md + s, gate);
-        if (nose)
+        if (nose) {
+          // the step's modifiers act before the next position update, whose
+          // friction reads the temperature of the velocities they leave
+          velocity_modifiers(n_steps + s + 1, dt, gate);
           nh_position(dt, md + s, gate);
-        std::swap(x.p, xn.p);
-        modifiers_after_step(n_steps + s + 1, beeman, dt, md + s, gate);
+          std::swap(x.p, xn.p);
+        } else {
+          std::swap(x.p, xn.p);
+          modifiers_after_step(n_steps + s + 1, beeman, dt, md + s, gate);
+        }
       }
       prof_step = -1;
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
@@ -1572,7 +1578,8 @@ struct lpmdx_system {
     k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
     // the session hands kboltzmann over as the string "0.00008617"
     // (liblpmd/runtime/session.cc:17-23,32)
-    k_nh_friction<<<1, 1, 0, st>>>(red.p, nh.p, n, nh_q, nh_t, dt, 0.00008617, gate, gate ? budget() : 0.0);
+    k_nh_friction<<<1, 1, 0, st>>>(red.p, nh.p, n, fixpos_any ? n_moving : n, nh_q, nh_t, dt, 0.00008617, gate,
+                                   gate ? budget() : 0.0);
     ForceArgs A{};
     A.x = x.p;
     A.xn = xn.p;
@@ -1594,7 +1601,9 @@ struct lpmdx_system {
     A.mode = kNHStep;
     for (int q = 0; q < 3; ++q)
       A.ll[q] = ll[q];
-    k_nh_drift<<<nb, kBlock, 0, st>>>(A);
+    if (fixpos_any && !mv_rank.p)
+      fail(LPMDX_EDEVICE, "nosehoover: moving-atom ranks missing (internal error)");
+    k_nh_drift<<<nb, kBlock, 0, st>>>(A, fixpos_any ? mv_rank.p : nullptr);
     launch_check();
   }
 

@@ -1209,29 +1209,41 @@ static __global__ __launch_bounds__(kBlock) void k_drift(ForceArgs A) {
 
 // Nose-Hoover's position update (NoseHoover::AdvancePosition,
 // plugins/nosehoover.cc:73-88) once the step's friction increment is known:
-// atom i (caller index c) sees the friction after c + 1 increments,
-// f = F + (c + 1) D, with A.nh = {F, D}; newacc = a - v f, the position
-// ((x + v dt) + (2/3 newacc) dt^2) - (1/6 ab) dt^2 folded into the cell,
-// and nn = newacc (the next velocity update's old.a).  a keeps the force.
-static __global__ __launch_bounds__(kBlock) void k_nh_drift(ForceArgs A) {
+// atom i (caller index c) sees the friction after r + 1 increments,
+// f = F + (r + 1) D, with A.nh = {F, D} and r = rank[c], the number of moving
+// atoms before c (r = c with rank null: no atom is held); newacc = a - v f,
+// the position ((x + v dt) + (2/3 newacc) dt^2) - (1/6 ab) dt^2 folded into
+// the cell, and nn = newacc (the next velocity update's old.a).  a keeps the
+// force.  A fixedpos "true" atom is skipped (twostepintegrator.cc:35-45): it
+// stays where it is and adds no increment.
+static __global__ __launch_bounds__(kBlock) void k_nh_drift(ForceArgs A, const int* __restrict__ rank) {
   if (gate_shut(A))
     return;
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   float d2 = 0.0f;
   if (i < A.n) {
-    const double f = A.nh[0] + (double)(A.perm[i] + 1) * A.nh[1];
-    const double dt = A.dt;
+    const int fl = A.fix ? A.fix[i] : 0;
     // a fixedvel atom's acceleration is zero when the position update runs
     // (the step's force call may have been made outside a step)
-    const bool zero = A.fix && (A.fix[i] & kFixVel);
-    double p[3];
-    for (int q = 0; q < 3; ++q) {
-      if (zero)
+    if (fl & kFixVel)
+      for (int q = 0; q < 3; ++q)
         A.a[3 * i + q] = 0.0;
-      const double na = A.a[3 * i + q] - A.v[3 * i + q] * f;
-      p[q] = beeman_position(A.x[3 * i + q], A.v[3 * i + q], na, A.ab[3 * i + q], dt);
-      A.nn[3 * i + q] = na;
-      A.xn[3 * i + q] = fitted_inside(p[q], A.ll[q]);
+    double p[3];
+    if (fl & kFixPos) {
+      for (int q = 0; q < 3; ++q) {
+        p[q] = A.x[3 * i + q];
+        A.xn[3 * i + q] = p[q];
+      }
+    } else {
+      const int c = A.perm[i];
+      const double f = A.nh[0] + (double)((rank ? rank[c] : c) + 1) * A.nh[1];
+      const double dt = A.dt;
+      for (int q = 0; q < 3; ++q) {
+        const double na = A.a[3 * i + q] - A.v[3 * i + q] * f;
+        p[q] = beeman_position(A.x[3 * i + q], A.v[3 * i + q], na, A.ab[3 * i + q], dt);
+        A.nn[3 * i + q] = na;
+        A.xn[3 * i + q] = fitted_inside(p[q], A.ll[q]);
+      }
     }
     d2 = build_displacement2(A, i, p[0], p[1], p[2]);
   }
@@ -1239,19 +1251,20 @@ static __global__ __launch_bounds__(kBlock) void k_nh_drift(ForceArgs A) {
 }
 
 // The friction of a Nose-Hoover step from the kinetic sum red[0]
-// (sum of m v^2 / 2): nh[0] moves past the previous step's N increments,
-// nh[1] = D = ((3 N / Q) k_B (T - T_set)) dt with T = (2/3) K / (k_B' N)
-// (properties.h:39-53, k_B' its KBOLTZMANN) and k_B the session's
-// kboltzmann (nosehoover.cc:69,78-79).  gate as the force kernels'.
-static __global__ void k_nh_friction(const double* __restrict__ red, double* __restrict__ nh, long n, double q,
-                                     double temp, double dt, double kboltzmann, const unsigned int* gate,
-                                     double gate_b) {
+// (sum of m v^2 / 2): nh[0] moves past the previous step's increments, one
+// per moving atom (n_moving), nh[1] = D = ((3 N / Q) k_B (T - T_set)) dt
+// with T = (2/3) K / (k_B' N) over all N atoms (properties.h:39-53, k_B' its
+// KBOLTZMANN) and k_B the session's kboltzmann (nosehoover.cc:69,78-79).
+// gate as the force kernels'.
+static __global__ void k_nh_friction(const double* __restrict__ red, double* __restrict__ nh, long n,
+                                     long n_moving, double q, double temp, double dt, double kboltzmann,
+                                     const unsigned int* gate, double gate_b) {
   if (threadIdx.x != 0 || blockIdx.x != 0)
     return;
   if (gate && (*gate == kGateClosed || sqrt((double)__uint_as_float(*gate)) > gate_b))
     return;
   const double T = (2.0 / 3.0) * (red[0] * kKin2eV) / (kBoltzmann * double(n));
-  nh[0] = nh[0] + double(n) * nh[1];
+  nh[0] = nh[0] + double(n_moving) * nh[1];
   nh[1] = ((3.0 * double(n) / q) * kboltzmann * (T - temp)) * dt;
 }
 

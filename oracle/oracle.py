@@ -500,7 +500,7 @@ def beeman_run(conf: Conf, forces, dt: float, steps: int, after_step=None):
 
 
 def nosehoover_run(conf: Conf, forces, dt: float, steps: int, fmass: float, temp: float,
-                   kboltzmann: float = 0.00008617):
+                   kboltzmann: float = 0.00008617, after_step=None):
     """NoseHoover::Initialize then `steps` TwoStepIntegrator::Advance calls
     (plugins/nosehoover.cc:62-96, liblpmd/simulation/twostepintegrator.cc:12-68).
     forces(c) fills c.a and returns the potential energy.  Set-up: forces at
@@ -511,7 +511,10 @@ def nosehoover_run(conf: Conf, forces, dt: float, steps: int, fmass: float, temp
     a = newacc; x = FittedInside(x + v dt + (2/3) newacc dt^2 - (1/6) old.a dt^2);
     old.a = newacc.  After the second force call AdvanceVelocity:
     v += (1/3) a dt + (5/6) old.a dt - (1/6) aux dt.  fixedvel atoms get a = 0
-    after each force call.  Returns the per-step potential and kinetic energy."""
+    after each force call; fixedpos ("true") atoms skip both updates, so the
+    friction grows once per atom that moves.  after_step(k, conf) runs the
+    modifiers of step k after it.  Returns the per-step potential and kinetic
+    energy."""
     L = np.asarray(conf.L, dtype=np.float64)
     old = Conf(fitted_inside_all(conf.x - conf.v * dt, L), L, z=conf.z, mass=conf.m, v=conf.v - conf.a * dt)
     forces(old)
@@ -520,8 +523,7 @@ def nosehoover_run(conf: Conf, forces, dt: float, steps: int, fmass: float, temp
     friction = 0.0
     n = conf.n
     fv = None if conf.fix is None else (conf.fix & FIX_VEL) != 0
-    if conf.fix is not None and np.any(conf.fix & FIX_POS):
-        raise ValueError("nosehoover_run: fixedpos atoms are not restated")
+    mv = np.ones(n, bool) if conf.fix is None else (conf.fix & FIX_POS) == 0
     ep = np.zeros(steps)
     ek = np.zeros(steps)
     for s in range(steps):
@@ -530,20 +532,24 @@ def nosehoover_run(conf: Conf, forces, dt: float, steps: int, fmass: float, temp
             conf.a[fv] = 0.0
         T = temperature(conf)  # the velocities do not change during the position loop
         incr = ((3.0 * n / fmass) * kboltzmann * (T - temp) * dt)
-        fr = np.empty(n)
-        for i in range(n):  # the friction grows once per atom, in atom order
-            friction += incr
-            fr[i] = friction
-        aux[:] = old_a
-        newacc = conf.a - conf.v * fr[:, None]
-        conf.a[:] = newacc
-        xn = conf.x + conf.v * dt + (2.0 / 3.0) * newacc * dt * dt - (1.0 / 6.0) * old_a * dt * dt
-        old_a = newacc.copy()
-        conf.x[:] = fitted_inside_all(xn, L)
+        fr = np.zeros(n)
+        for i in range(n):  # the friction grows once per moving atom, in atom order
+            if mv[i]:
+                friction += incr
+                fr[i] = friction
+        aux[mv] = old_a[mv]
+        newacc = conf.a[mv] - conf.v[mv] * fr[mv][:, None]
+        conf.a[mv] = newacc
+        xn = conf.x[mv] + conf.v[mv] * dt + (2.0 / 3.0) * newacc * dt * dt - (1.0 / 6.0) * old_a[mv] * dt * dt
+        old_a[mv] = newacc
+        conf.x[mv] = fitted_inside_all(xn, L)
         ep[s] = forces(conf)
         if fv is not None:
             conf.a[fv] = 0.0
-        conf.v[:] = conf.v + (1.0 / 3.0) * conf.a * dt + (5.0 / 6.0) * old_a * dt - (1.0 / 6.0) * aux * dt
+        conf.v[mv] = (conf.v[mv] + (1.0 / 3.0) * conf.a[mv] * dt + (5.0 / 6.0) * old_a[mv] * dt
+                      - (1.0 / 6.0) * aux[mv] * dt)
+        if after_step:
+            after_step(s + 1, conf)  # the modifiers of step s + 1 (lpmd/lpmd.cc:92-93)
         ek[s] = kinetic_energy_tag(conf, conf.tagged)
     return ep, ek
 

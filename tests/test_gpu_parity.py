@@ -330,10 +330,11 @@ def test_beeman_trajectory_matches_oracle(cm):
 def test_nosehoover_trajectory_matches_oracle(cm):
     """The nosehoover integrator (plugins/nosehoover.cc:62-96): set-up on the
     old configuration, then 20 + 10 steps of LJ argon started at 100 K and
-    driven toward 300 K (Q 1e7), in two calls, one
-    fixedvel atom among them: positions within the FP32 force noise the
-    trajectory accumulates, velocities and the sampled energies against the
-    oracle's sequential restatement; the friction has visibly heated the box.
+    driven toward 300 K (Q 1e7), in two calls, one fixedvel atom and two
+    fixedpos atoms (which stay put and add no friction increment) among them:
+    positions within the FP32 force noise the trajectory accumulates,
+    velocities and the sampled energies against the oracle's sequential
+    restatement; the friction has visibly heated the box.
     (The reference's friction increment per atom is (3 N / Q) k_B (T - t) dt,
     so a gentle thermostat for 256 atoms needs Q near 1e7.)"""
     say("nosehoover", cm)
@@ -344,6 +345,8 @@ def test_nosehoover_trajectory_matches_oracle(cm):
     v[0] = [0.003, 0.004, 0.005]
     fv = np.zeros(x.shape[0], bool)
     fv[7] = True
+    fp = np.zeros(x.shape[0], bool)
+    fp[[20, 131]] = True
     Q, T0, dt = 1e7, 300.0, 2.0
 
     def cms():
@@ -355,13 +358,13 @@ def test_nosehoover_trajectory_matches_oracle(cm):
     s = E.System(x, L, 18, v=v)
     s.set_cellmanager(cm, args)
     s.append_potential("lennardjones", "sigma 3.41 epsilon 0.0103 cutoff 8.5")
-    s.set_fixed(fixedvel=fv)
+    s.set_fixed(fixedvel=fv, fixedpos=fp)
     s.nosehoover_init(dt, Q, T0)
     ep, ek = s.nosehoover(dt, 20, obs_every=5)
     xg, vg, ag = s.get_state()
     pots = [O.Pot("lennardjones", sigma=3.41, epsilon=0.0103, cutoff=8.5)]
     conf = O.Conf(x, L, v=v)
-    conf.set_fixed(fixedvel=fv)
+    conf.set_fixed(fixedvel=fv, fixedpos=fp)
     e0, k0 = O.nosehoover_run(conf, lambda c: O.update_forces(c, om, pots)[0], dt, 20, Q, T0)
     dx = xg - conf.x
     dx -= L * np.round(dx / L)
@@ -372,10 +375,11 @@ def test_nosehoover_trajectory_matches_oracle(cm):
     np.testing.assert_allclose(ep, e0[4::5], rtol=1e-6)
     np.testing.assert_allclose(ek, k0[4::5], rtol=1e-6)
     assert ek[-1] > 1.5 * ek[0]  # the thermostat pulls toward 300 K
+    assert np.array_equal(xg[fp], x[fp]) and np.array_equal(vg[fp], v[fp])
     ep2, ek2 = s.nosehoover(dt, 10, obs_every=10)
     _, om30 = cms()
     c30 = O.Conf(x, L, v=v)
-    c30.set_fixed(fixedvel=fv)
+    c30.set_fixed(fixedvel=fv, fixedpos=fp)
     e30, k30 = O.nosehoover_run(c30, lambda c: O.update_forces(c, om30, pots)[0], dt, 30, Q, T0)
     np.testing.assert_allclose(ep2, e30[29:], rtol=1e-6)
     np.testing.assert_allclose(ek2, k30[29:], rtol=1e-6)
@@ -509,7 +513,7 @@ def test_init_velocities_match_oracle():
 
 @pytest.mark.parametrize("mod", ["tempscaling", "berendsen"])
 @pytest.mark.parametrize("cm", ["linkedcell", "verletlist"])
-@pytest.mark.parametrize("scheme", ["velocityverlet", "beeman", "leapfrog"])
+@pytest.mark.parametrize("scheme", ["velocityverlet", "beeman", "leapfrog", "nosehoover"])
 def test_tempscaling_schedule_matches_oracle(scheme, cm, mod):
     """A temperature ramp 60 K -> 140 K applied every 4 steps over 40 steps of
     LJ argon, inside the device step loops, against the oracle running the
@@ -566,10 +570,16 @@ def test_tempscaling_schedule_matches_oracle(scheme, cm, mod):
         s.beeman_init(dt)
         ep, ek = s.beeman(dt, steps, obs_every=10)
         e0, k0 = O.beeman_run(conf, forces, dt, steps, after_step=modifier)
-    else:
+    elif scheme == "leapfrog":
         s.leapfrog_init(dt)
         ep, ek = s.leapfrog(dt, steps, obs_every=10)
         e0, k0 = O.leapfrog_run(conf, forces, dt, steps, after_step=modifier)
+    else:
+        # a Nose-Hoover bath at 100 K under the ramp: the friction of each
+        # step reads the temperature the modifier left
+        s.nosehoover_init(dt, 1e7, 100.0)
+        ep, ek = s.nosehoover(dt, steps, obs_every=10)
+        e0, k0 = O.nosehoover_run(conf, forces, dt, steps, 1e7, 100.0, after_step=modifier)
     xg, vg, _ = s.get_state()
     dx = xg - conf.x
     dx -= L * np.round(dx / L)

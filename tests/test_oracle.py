@@ -272,6 +272,43 @@ def test_nosehoover_friction_closed_form():
     np.testing.assert_allclose(conf2.v, v1 + (5.0 / 6.0) * na2 * dt - (1.0 / 6.0) * na * dt, rtol=1e-14)
 
 
+def test_nosehoover_fixedpos_skips_the_friction_increment():
+    """Three free atoms, the middle one fixedpos "true": TwoStepIntegrator
+    (twostepintegrator.cc:35-45) skips its AdvancePosition, so the friction
+    grows twice in the step (atom 0 sees D, atom 2 sees 2 D), while N = 3 and
+    the temperature still count every atom (nosehoover.cc:78-79); the held atom
+    keeps its position and velocity bit for bit, and the next step starts
+    from friction 2 D."""
+    v = np.array([[0.01, 0.0, -0.004], [0.02, 0.01, 0.0], [0.0, -0.006, 0.002]])
+    x = np.array([[5.0, 5.0, 5.0], [9.0, 9.0, 9.0], [12.0, 12.0, 12.0]])
+    held = np.array([False, True, False])
+    zero = lambda c: (c.a.__setitem__(slice(None), 0.0), 0.0)[1]
+    Q, T0, dt, kb = 2.5, 40.0, 2.0, 0.00008617
+
+    def run(steps):
+        c = O.Conf(x, [30.0] * 3, z=18, v=v)
+        c.set_fixed(fixedpos=held)
+        O.nosehoover_run(c, zero, dt, steps, fmass=Q, temp=T0)
+        return c
+
+    c = run(1)
+    T = O.temperature(O.Conf(x, [30.0] * 3, z=18, v=v))
+    D = ((3.0 * 3 / Q) * kb * (T - T0) * dt)
+    f = np.array([D, 0.0, D + D])[:, None]
+    na = -v * f
+    mv = ~held
+    np.testing.assert_allclose(c.x[mv], (x + v * dt + (2.0 / 3.0) * na * dt * dt)[mv], rtol=1e-15)
+    np.testing.assert_allclose(c.v[mv], (v + (5.0 / 6.0) * na * dt)[mv], rtol=1e-15)
+    assert np.array_equal(c.x[1], x[1]) and np.array_equal(c.v[1], v[1])
+    c2 = run(2)
+    v1 = c.v
+    D1 = ((3.0 * 3 / Q) * kb * (O.temperature(c) - T0) * dt)
+    f2 = np.array([D + D + D1, 0.0, D + D + D1 + D1])[:, None]
+    na2 = -v1 * f2
+    np.testing.assert_allclose(c2.v[mv], (v1 + (5.0 / 6.0) * na2 * dt - (1.0 / 6.0) * na * dt)[mv], rtol=1e-14)
+    assert np.array_equal(c2.x[1], x[1]) and np.array_equal(c2.v[1], v[1])
+
+
 def test_stepper_known_answers():
     """Stepper::IsActiveInStep / ValueAtStep (stepper.h:17-31) by hand."""
     st = O.Stepper(10, 50, 20)
