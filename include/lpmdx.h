@@ -200,14 +200,16 @@ LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_ev
 
 /* The nosehoover integrator (plugins/nosehoover.cc:62-96), a two-step scheme
  * over Beeman's updates with a friction term.  lpmdx_nosehoover_init is
- * NoseHoover::Initialize with the plugin's keywords dt, fmass (Q) and t: the
+ * NoseHoover::Initialize with the plugin's keywords dt, fmass (Q) and t and
+ * the session's kboltzmann (the string "0.00008617" by default,
+ * liblpmd/runtime/session.cc:17-23,32, read with double()): the
  * forces at the old configuration FittedInside(x - v dt)
  * (Integrator::GoBack, integrator.cc:31-42) become the first old
  * acceleration, the friction starts at zero; x, v and a are left as they
  * were.  lpmdx_nosehoover then runs `steps` TwoStepIntegrator::Advance calls
  * (twostepintegrator.cc:12-68): each position update adds to the friction
  * D = (3 N / Q) k_B (T - t) dt once per atom in atom order (T the temperature
- * of the velocities, k_B the session's kboltzmann 0.00008617), so atom i
+ * of the velocities, k_B = kboltzmann), so atom i
  * integrates with newacc = a - v f_i, f_i the friction after its r_i + 1
  * increments, r_i the number of moving atoms before it (formed as
  * F + (r_i + 1) D rather than as the reference's sequential sum: the same
@@ -217,7 +219,7 @@ LPMDX_API int lpmdx_beeman(lpmdx_system* sys, double dt, long steps, long obs_ev
  * (twostepintegrator.cc:35-45); the tempscaling / berendsen modifiers act
  * after each step, before the next position update reads the temperature.
  * Without a prior init: LPMDX_ESTATE; another dt: LPMDX_EINVAL. */
-LPMDX_API int lpmdx_nosehoover_init(lpmdx_system* sys, double dt, double fmass, double t);
+LPMDX_API int lpmdx_nosehoover_init(lpmdx_system* sys, double dt, double fmass, double t, double kboltzmann);
 LPMDX_API int lpmdx_nosehoover(lpmdx_system* sys, double dt, long steps, long obs_every, double* epot_t,
                                double* ekin_t);
 

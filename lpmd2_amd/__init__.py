@@ -87,7 +87,7 @@ def lib():
         L.lpmdx_init_velocities.argtypes = [vp]
         L.lpmdx_velocities_set.argtypes = [vp, i]
         L.lpmdx_system_set_fixed.argtypes = [vp, vp]
-        L.lpmdx_nosehoover_init.argtypes = [vp, d, d, d]
+        L.lpmdx_nosehoover_init.argtypes = [vp, d, d, d, d]
         L.lpmdx_nosehoover.argtypes = [vp, d, l, l, vp, vp]
         _lib = L
     return _lib
@@ -202,9 +202,10 @@ class System:
         _check(lib().lpmdx_beeman(self._h, dt, steps, obs_every, _p(ep), _p(ek)))
         return ep[:k], ek[:k]
 
-    def nosehoover_init(self, dt: float, fmass: float, t: float):
-        """NoseHoover::Initialize (keywords dt, fmass, t of plugins/nosehoover.cc)."""
-        _check(lib().lpmdx_nosehoover_init(self._h, dt, fmass, t))
+    def nosehoover_init(self, dt: float, fmass: float, t: float, kboltzmann: float = 0.00008617):
+        """NoseHoover::Initialize (keywords dt, fmass, t of plugins/nosehoover.cc;
+        kboltzmann the session's value, "0.00008617" unless changed)."""
+        _check(lib().lpmdx_nosehoover_init(self._h, dt, fmass, t, kboltzmann))
 
     def nosehoover(self, dt: float, steps: int, obs_every: int = 0):
         k = steps // obs_every if obs_every else 0

@@ -384,10 +384,10 @@ int lpmdx_beeman(lpmdx_system* S, double dt, long steps, long obs_every, double*
   });
 }
 
-int lpmdx_nosehoover_init(lpmdx_system* S, double dt, double fmass, double t) {
+int lpmdx_nosehoover_init(lpmdx_system* S, double dt, double fmass, double t, double kboltzmann) {
   return guard([&] {
     need(S, "system");
-    S->nh_init(dt, fmass, t);
+    S->nh_init(dt, fmass, t, kboltzmann);
   });
 }
 

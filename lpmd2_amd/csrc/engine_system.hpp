@@ -257,7 +257,7 @@ struct lpmdx_system {
   // nh = {friction before the step's increments, increment per atom}
   DBuf<double> nn, nn2, nh;
   bool nh_ready = false, drift_nose = false;
-  double nh_dt = 0.0, nh_q = 0.0, nh_t = 0.0;
+  double nh_dt = 0.0, nh_q = 0.0, nh_t = 0.0, nh_kb = 0.0;
   // moving atoms (no fixedpos "true"): their count, and per caller index the
   // number of moving atoms before it (filled while any atom is held)
   long n_moving = 0;
@@ -1576,9 +1576,7 @@ struct lpmdx_system {
     partials.alloc(std::max<size_t>((size_t)nb * 4, partials.n));
     k_kinetic<<<nb, kBlock, 0, st>>>(v.p, mass.p, n, partials.p);
     k_reduce_partials<<<1, kBlock, 0, st>>>(partials.p, nb, 4, red.p);
-    // the session hands kboltzmann over as the string "0.00008617"
-    // (liblpmd/runtime/session.cc:17-23,32)
-    k_nh_friction<<<1, 1, 0, st>>>(red.p, nh.p, n, fixpos_any ? n_moving : n, nh_q, nh_t, dt, 0.00008617, gate,
+    k_nh_friction<<<1, 1, 0, st>>>(red.p, nh.p, n, fixpos_any ? n_moving : n, nh_q, nh_t, dt, nh_kb, gate,
                                    gate ? budget() : 0.0);
     ForceArgs A{};
     A.x = x.p;
@@ -1610,13 +1608,13 @@ struct lpmdx_system {
   // NoseHoover::Initialize (plugins/nosehoover.cc:62-68): the forces at the
   // old configuration of Integrator::GoBack become old.a (in ab), the
   // friction starts at zero; x, v and a are left as they were.
-  void nh_init(double dt, double fmass, double t) {
+  void nh_init(double dt, double fmass, double t, double kboltzmann) {
     if (pots.empty())
       fail(LPMDX_ESTATE, "no potential set");
     if (cm.kind == kNone)
       fail(LPMDX_ESTATE, "cellmanager not set");
-    if (!std::isfinite(dt) || !std::isfinite(fmass) || !std::isfinite(t))
-      fail(LPMDX_ENONFINITE, "nosehoover: dt, fmass and t must be finite");
+    if (!std::isfinite(dt) || !std::isfinite(fmass) || !std::isfinite(t) || !std::isfinite(kboltzmann))
+      fail(LPMDX_ENONFINITE, "nosehoover: dt, fmass, t and kboltzmann must be finite");
     if (fmass == 0.0)
       fail(LPMDX_EINVAL, "nosehoover: fmass must not be zero");
     undrift();
@@ -1624,6 +1622,7 @@ struct lpmdx_system {
     nh_dt = dt;
     nh_q = fmass;
     nh_t = t;
+    nh_kb = kboltzmann;
     nh.alloc(2);
     k_fill_double<<<1, 32, 0, st>>>(nh.p, 2, 0.0);
     launch_check();

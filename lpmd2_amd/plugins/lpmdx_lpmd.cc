@@ -6,7 +6,9 @@
 #include <lpmd/basicparticleset.h>
 #include <lpmd/combinedpotential.h>
 #include <lpmd/error.h>
+#include <lpmd/properties.h>
 #include <lpmd/simulation.h>
+#include <runtime/runtime_context.h>
 
 #include <cmath>
 #include <iostream>
@@ -56,6 +58,7 @@ const std::map<std::string, std::vector<Keyword>>& Keywords() {
       {"velocityverlet", {{"dt", "1.0"}, {"start", "1"}}},
       {"leapfrog", {{"dt", "1.0"}, {"start", "1"}}},
       {"beeman", {{"dt", "1.0"}, {"start", "1"}}},
+      {"nosehoover", {{"fmass", nullptr}, {"dt", "1.0"}, {"start", "1"}, {"t", "300.0"}}},
   };
   return m;
 }
@@ -133,6 +136,17 @@ void ZeroFixedVel(lpmd::Simulation& sim, const HostTags& t) {
 }
 
 bool Held(const HostTags& t, long i) { return t.any && t.hold[i]; }
+
+// The session's kboltzmann as NoseHoover reads it (nosehoover.cc:69): a
+// string parameter, "0.00008617" for the default unit system
+// (runtime/units.cc:7 formatted by runtime/session.cc:17-23,32).  A
+// configuration without a runtime context (a harness driving the plugin
+// directly, where the reference plugin would throw) gets that string's value.
+double SessionKBoltzmann(lpmd::Simulation& sim) {
+  if (!sim.HasRuntimeContext())
+    return 0.00008617;
+  return double(sim.Context().session()["kboltzmann"]);
+}
 
 } // namespace
 
@@ -775,6 +789,131 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "beeman");
 }
 
+// ------------------------------------------------------------------ nosehoover
+
+DeviceNoseHoover::DeviceNoseHoover(std::string args) : lpmd::Plugin("nosehoover", kVersion) {
+  Define(*this, "nosehoover");
+  ProcessArguments(args);
+  dt = double((*this)["dt"]);
+  start_ = int((*this)["start"]);
+  q_ = double((*this)["fmass"]);
+  temp_ = double((*this)["t"]);
+}
+
+void DeviceNoseHoover::ShowHelp() const {
+  std::cout << " Module Name        = nosehoover (MI355X engine)\n"
+            << "      Nose-Hoover thermostat (NVT) with dt (fs), fmass (the fictitious mass Q)\n"
+            << "      and t (the reservoir temperature, K); on the GPU when every potential\n"
+            << "      and the cell manager are MI355X plugins.\n";
+}
+
+void DeviceNoseHoover::Initialize(lpmd::Simulation& sim, lpmd::Potential& p) {
+  if (mode_ != kUnset) // UseOldConfig sets the old configuration up once
+    return;
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  const double kb = SessionKBoltzmann(sim);
+  if (cm) {
+    lpmdx_system* sys = cm->Bind(sim, false);
+    cm->UsePotentials(dev);
+    std::vector<double> x, v, a(3 * n);
+    Positions(sim, x);
+    Velocities(sim, v);
+    for (long i = 0; i < n; ++i)
+      for (int q = 0; q < 3; ++q)
+        a[3 * i + q] = atoms[i].Acceleration()[q];
+    check(lpmdx_system_set_state(sys, x.data(), v.data(), a.data()), "nosehoover");
+    check(lpmdx_nosehoover_init(sys, dt, q_, temp_, kb), "nosehoover");
+    hx_.swap(x);
+    hv_.swap(v);
+    sys_ = sys;
+    mode_ = kDevice;
+    return;
+  }
+  // on the host: forces at the old configuration of Integrator::GoBack
+  // (integrator.cc:31-42), set in place and then undone, give old.a
+  lpmd::BasicCell& cell = sim.Cell();
+  std::vector<lpmd::Vector> keep_x(n), keep_a(n);
+  for (long i = 0; i < n; ++i) {
+    keep_x[i] = atoms[i].Position();
+    keep_a[i] = atoms[i].Acceleration();
+    atoms[i].Position() = cell.FittedInside(atoms[i].Position() - atoms[i].Velocity() * dt);
+  }
+  p.UpdateForces(sim);
+  a_old_.resize(n);
+  aux_.assign(n, lpmd::Vector());
+  for (long i = 0; i < n; ++i) {
+    a_old_[i] = atoms[i].Acceleration();
+    atoms[i].Position() = keep_x[i];
+    atoms[i].Acceleration() = keep_a[i];
+  }
+  friction_ = 0.0;
+  mode_ = kHost;
+}
+
+void DeviceNoseHoover::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
+  if (mode_ == kUnset)
+    throw lpmd::RuntimeError("nosehoover (MI355X): Advance before Initialize");
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = DevicePath(sim, p, dev);
+  if ((cm != nullptr) != (mode_ == kDevice))
+    throw lpmd::RuntimeError("nosehoover (MI355X): the potentials or the cell manager changed between Initialize "
+                             "and Advance");
+  lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const long n = atoms.Size();
+  if (!cm) {
+    // TwoStepIntegrator::Advance with NoseHoover's AdvancePosition and
+    // AdvanceVelocity (twostepintegrator.cc:12-68, nosehoover.cc:70-96).  The
+    // temperature the friction reads is that of the velocities, which the
+    // position loop does not change: one evaluation serves every atom.
+    const double kb = SessionKBoltzmann(sim);
+    lpmd::BasicCell& cell = sim.Cell();
+    const HostTags tags = ReadTags(sim);
+    p.UpdateForces(sim);
+    ZeroFixedVel(sim, tags);
+    const double T = lpmd::Temperature(atoms);
+    for (long i = 0; i < n; ++i) {
+      if (Held(tags, i))
+        continue;
+      aux_[i] = a_old_[i];
+      friction_ += ((3.0 * n / q_) * kb * (T - temp_) * dt);
+      const lpmd::Vector newacc = atoms[i].Acceleration() - atoms[i].Velocity() * friction_;
+      atoms[i].Acceleration() = newacc;
+      const lpmd::Vector np = atoms[i].Position() + atoms[i].Velocity() * dt + (2.0 / 3.0) * newacc * dt * dt -
+                              (1.0 / 6.0) * a_old_[i] * dt * dt;
+      a_old_[i] = newacc;
+      atoms[i].Position() = cell.FittedInside(np);
+    }
+    p.UpdateForces(sim);
+    ZeroFixedVel(sim, tags);
+    for (long i = 0; i < n; ++i)
+      if (!Held(tags, i))
+        atoms[i].Velocity() = atoms[i].Velocity() + (1.0 / 3.0) * atoms[i].Acceleration() * dt +
+                              (5.0 / 6.0) * a_old_[i] * dt - (1.0 / 6.0) * aux_[i] * dt;
+    return;
+  }
+  lpmdx_system* sys = cm->Bind(sim, false);
+  if (sys != sys_)
+    throw lpmd::RuntimeError("nosehoover (MI355X): the cell manager's system changed since Initialize");
+  cm->UsePotentials(dev);
+  cm->SyncFixed(sim);
+  std::vector<double> x, v;
+  Positions(sim, x);
+  Velocities(sim, v);
+  if (x != hx_ || v != hv_) {
+    // the host changed the state (a modifier, a tag): send it; the history
+    // and the friction stay
+    check(lpmdx_system_set_state(sys, x.data(), v.data(), nullptr), "nosehoover");
+    hx_.swap(x);
+    hv_.swap(v);
+  }
+  double ep = 0.0, ek = 0.0;
+  check(lpmdx_nosehoover(sys, dt, 1, 1, &ep, &ek), "nosehoover");
+  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "nosehoover");
+}
+
 // ------------------------------------------------------------------ factory
 
 lpmd::Plugin* Create(const std::string& name, const std::string& args) {
@@ -795,6 +934,8 @@ lpmd::Plugin* Create(const std::string& name, const std::string& args) {
     return new DeviceLeapfrog(args);
   if (name == "beeman")
     return new DeviceBeeman(args);
+  if (name == "nosehoover")
+    return new DeviceNoseHoover(args);
   throw lpmd::RuntimeError("no MI355X plugin named " + name);
 }
 

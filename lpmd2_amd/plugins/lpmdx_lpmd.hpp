@@ -16,6 +16,7 @@
 //                         "velocityverlet" (plugins/velocityverlet.cc:13-20)
 //   DeviceLeapfrog        lpmd::Integrator as "leapfrog" (plugins/leapfrog.cc:14-23)
 //   DeviceBeeman          lpmd::Integrator as "beeman" (plugins/beeman.cc)
+//   DeviceNoseHoover      lpmd::Integrator as "nosehoover" (plugins/nosehoover.cc)
 // They take the same plugin names and keyword strings, so a directory holding
 // their <name>.so files placed first on LPMD_PATH replaces the CPU plugins.
 // The classes live in one shared library (liblpmdx_lpmd.so) so that every
@@ -234,6 +235,32 @@ private:
   Mode mode_ = kUnset;
   long start_ = 1;
   std::vector<lpmd::Vector> a_old_; // host path: the old configuration's accelerations
+  std::vector<double> hx_, hv_, ha_;
+  lpmdx_system* sys_ = nullptr;
+};
+
+class DeviceNoseHoover : public lpmd::Integrator, public lpmd::Plugin {
+public:
+  explicit DeviceNoseHoover(std::string args);
+  void ShowHelp() const override;
+  // NoseHoover::Initialize (nosehoover.cc:62-68): the forces at the old
+  // configuration give the first old accelerations, the friction starts at
+  // zero; once per integrator object, on the device (lpmdx_nosehoover_init,
+  // with the session's kboltzmann) under the same condition as
+  // DeviceVelocityVerlet, the history and the friction then staying in HBM.
+  void Initialize(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  // One Nose-Hoover step (twostepintegrator.cc:12-68, nosehoover.cc:70-96),
+  // on the device or, when Initialize ran on the host, on the host.
+  void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+
+private:
+  enum Mode { kUnset, kDevice, kHost };
+  Mode mode_ = kUnset;
+  long start_ = 1;
+  double q_ = 0.0, temp_ = 300.0;
+  // host path: the friction, old.a and NoseHoover's auxlist
+  double friction_ = 0.0;
+  std::vector<lpmd::Vector> a_old_, aux_;
   std::vector<double> hx_, hv_, ha_;
   lpmdx_system* sys_ = nullptr;
 };

@@ -18,7 +18,7 @@ CXX      ?= g++
 CXXFLAGS := -O2 -std=c++17 -fPIC -w -I$(REF)/liblpmd -Iinclude
 NAMES    := lennardjones fastlj morse buckingham suttonchen finnissinclair gupta \
             harmonic lennardjonesMod finnissinclair-ext tabulatedpair \
-            linkedcell verletlist minimumimage lcbinary velocityverlet leapfrog beeman
+            linkedcell verletlist minimumimage lcbinary velocityverlet leapfrog beeman nosehoover
 RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..'
 
 all: $(patsubst %,$(OUT)/%.so,$(NAMES))

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import lpmd2_amd as E
+from oracle import oracle as O
 from oracle import refcheck as R
 
 pytestmark = pytest.mark.skipif(not R.dropin_available(), reason="drop-in plugins not built")
@@ -102,7 +103,8 @@ def test_per_pair_virtuals_match_reference(name, args):
                                        ("lcbinary", "cutoff 8.5 mode auto"),
                                        ("velocityverlet", "dt 2.0"),
                                        ("leapfrog", "dt 2.0"),
-                                       ("beeman", "dt 2.0")])
+                                       ("beeman", "dt 2.0"),
+                                       ("nosehoover", "dt 2.0 fmass 1e7 t 300.0")])
 def test_other_plugins_load(name, args):
     p = R.RefPlugin.dropin(name, args)
     p.close()
@@ -125,6 +127,43 @@ def test_integrator_host_path_matches_reference_golden(name, fixture, calls_per_
     assert np.array_equal(xf, d["xf"])
     assert np.array_equal(vf, d["vf"])
     assert np.array_equal(af, d["af"])
+
+
+@pytest.mark.parametrize("held", [False, True])
+def test_nosehoover_host_path_matches_oracle(held):
+    """The drop-in nosehoover with a CPU cell manager runs NoseHoover over
+    TwoStepIntegrator on the host (nosehoover.cc:62-96,
+    twostepintegrator.cc:12-68): with the forces of the golden Beeman run
+    replayed (a box at 2.4e8 K, where atoms cross the cell each step) and a
+    thermostat strong enough to cool it by a fifth (Q 1e12, bath 30 K), the
+    trajectory is the oracle's sequential restatement bit for bit, with or
+    without fixedpos atoms (which add no friction increment)."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bm_ar108.npz"))
+    acc = d["acc"]
+    steps = (acc.shape[0] - 1) // 2
+    conf = R.RefConf(d["x0"], d["L"])
+    conf.set_velocities(d["v0"])
+    oc = O.Conf(d["x0"], d["L"], v=d["v0"])
+    if held:
+        fp = (np.arange(conf.n) % 7) == 3
+        for i in np.nonzero(fp)[0]:
+            R.set_atom_tag(conf, i, "fixedpos", "true")
+        oc.set_fixed(fixedpos=fp)
+    cm = R.RefCellManager("linkedcell", "cutoff 8.5 nx 4 ny 4 nz 4")
+    R.vv_run("dt 2.0 fmass 1e12 t 30.0", conf, cm, steps, acc.reshape(acc.shape[0], -1),
+             path=os.path.join(R.DROPIN_DIR, "nosehoover.so"))
+    calls = iter(acc)
+
+    def replay(c):
+        c.a[:] = next(calls)
+        return 0.0
+
+    O.nosehoover_run(oc, replay, 2.0, steps, fmass=1e12, temp=30.0)
+    xf, vf, af = conf.get()
+    assert not np.array_equal(vf, d["vf"])  # the friction acted
+    assert np.array_equal(xf, oc.x)
+    assert np.array_equal(vf, oc.v)
+    assert np.array_equal(af, oc.a)
 
 
 def test_without_gpu_the_plugins_fail_loudly():

@@ -128,9 +128,22 @@ def native_run(x, L, z, v, cm, cm_args, pot, dt, steps, integrator="velocityverl
     elif integrator == "beeman":
         s.beeman_init(dt)
         s.beeman(dt, steps)
+    elif integrator == "nosehoover":
+        s.nosehoover_init(dt, NH_Q, NH_T)
+        s.nosehoover(dt, steps)
     else:
         s.velocityverlet(dt, steps)
     return s.get_state()
+
+
+# the nosehoover keywords of the drop-in runs below (fmass, t)
+NH_Q, NH_T = 1e7, 300.0
+
+
+def integrator_args(scheme, dt):
+    if scheme == "nosehoover":
+        return f"dt {dt} fmass {NH_Q} t {NH_T}"
+    return f"dt {dt}"
 
 
 @pytest.mark.parametrize("integrator", ["reference", "dropin"])
@@ -164,12 +177,15 @@ def test_dropin_integration_follows_native_path(integrator):
         assert np.all(np.isfinite(ep)) and ep[-1] != 0.0
 
 
-@pytest.mark.parametrize("scheme", ["leapfrog", "beeman"])
-@pytest.mark.parametrize("integrator", ["reference", "dropin"])
+@pytest.mark.parametrize("scheme,integrator", [("leapfrog", "reference"), ("leapfrog", "dropin"),
+                                                ("beeman", "reference"), ("beeman", "dropin"),
+                                                ("nosehoover", "dropin")])
 def test_dropin_oldconfig_integrators_follow_native_path(integrator, scheme):
     """The reference leapfrog / beeman plugin over drop-in forces (its set-up
     force call on the old configuration included) and the drop-in plugin of
-    the same name both follow the engine's own scheme."""
+    the same name both follow the engine's own scheme.  (The reference
+    nosehoover reads the session's kboltzmann, which needs a runtime context
+    this harness cannot build: only the drop-in runs it here.)"""
     say("drop-in", scheme, integrator)
     x, L = E.fcc(5, 3.61)
     x = np.mod(x + np.random.default_rng(9).normal(0.0, 0.02, x.shape), L)
@@ -184,7 +200,7 @@ def test_dropin_oldconfig_integrators_follow_native_path(integrator, scheme):
     cm.attach(conf)
     pots = [R.RefPlugin.dropin(*pot)]
     make = R.RefPlugin.reference if integrator == "reference" else R.RefPlugin.dropin
-    itg = make(scheme, f"dt {dt}")
+    itg = make(scheme, integrator_args(scheme, dt))
     ep = R.integrate(itg, conf, pots, steps)
     xr, vr, ar = conf.get()
     xn, vn, an = native_run(x, L, 29, v, "verletlist", cm_args, pot, dt, steps, integrator=scheme)
@@ -222,14 +238,14 @@ def test_dropin_atom_energy_matches_oracle(name, args, kw, z, a):
     assert err2 <= 1e-6
 
 
-@pytest.mark.parametrize("scheme", ["velocityverlet", "leapfrog", "beeman"])
+@pytest.mark.parametrize("scheme", ["velocityverlet", "leapfrog", "beeman", "nosehoover"])
 def test_dropin_integrators_honour_fixed_atoms(scheme):
     """Atoms tagged fixedpos ("true") and fixedvel through the reference's
     BasicParticleSet::SetTag: the drop-in integrator's device steps keep the
     fixedpos atoms where they are with their velocities and zero the fixedvel
     atoms' accelerations, following the oracle's TwoStepIntegrator /
     OneStepIntegrator restatement (velocity Verlet) and the engine's own
-    tagged run (all three schemes)."""
+    tagged run (all four schemes)."""
     say("drop-in fixed atoms", scheme)
     x, L = E.fcc(4, 5.26)
     x = np.mod(x + np.random.default_rng(13).normal(0.0, 0.05, x.shape), L)
@@ -249,7 +265,7 @@ def test_dropin_integrators_honour_fixed_atoms(scheme):
     cm = R.RefPlugin.dropin("verletlist", cm_args)
     cm.attach(conf)
     pots = [R.RefPlugin.dropin(*pot)]
-    itg = R.RefPlugin.dropin(scheme, f"dt {dt}")
+    itg = R.RefPlugin.dropin(scheme, integrator_args(scheme, dt))
     R.integrate(itg, conf, pots, steps)
     xr, vr, ar = conf.get()
     np.testing.assert_array_equal(xr[fp], x[fp])
@@ -265,6 +281,9 @@ def test_dropin_integrators_honour_fixed_atoms(scheme):
     elif scheme == "beeman":
         s.beeman_init(dt)
         s.beeman(dt, steps)
+    elif scheme == "nosehoover":
+        s.nosehoover_init(dt, NH_Q, NH_T)
+        s.nosehoover(dt, steps)
     else:
         s.velocityverlet(dt, steps)
     xn, vn, an = s.get_state()
