@@ -299,8 +299,9 @@ LPMDX_API int lpmdx_observables_noalloc(lpmdx_system* sys, double out[LPMDX_NOBS
 /* Counters of the system's engine: [0] list builds, [1] force evaluations,
  * [2] steps, [3] list capacity per atom, [4] largest list, [5] lanes per atom,
  * [6] brick edge in cells of the LDS-staged force kernels (0: unstaged),
- * [7] total list entries of the last build. */
-LPMDX_API int lpmdx_system_stats(const lpmdx_system* sys, long out[8]);
+ * [7] total list entries of the last build, [8] step chunks cut short by the
+ * skin test (their speculative steps past the cut ran no kernel). */
+LPMDX_API int lpmdx_system_stats(const lpmdx_system* sys, long out[9]);
 
 /* Per-kernel timing with HIP events recorded on the system's own stream:
  * enable (1) or disable (0); enabling clears the totals.  read returns the

@@ -280,9 +280,10 @@ class System:
         _check(lib().lpmdx_system_set_fixed(self._h, _p(flags)))
 
     def stats(self):
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(9, dtype=np.int64)
         _check(lib().lpmdx_system_stats(self._h, _p(out)))
-        keys = ("builds", "force_evals", "steps", "list_cap", "max_list", "lanes", "brick", "list_entries")
+        keys = ("builds", "force_evals", "steps", "list_cap", "max_list", "lanes", "brick", "list_entries",
+                "cut_chunks")
         return dict(zip(keys, (int(v) for v in out)))
 
     def profile(self, enable: bool = True):

@@ -679,7 +679,7 @@ int lpmdx_observables_noalloc(lpmdx_system* S, double out[LPMDX_NOBS]) {
   });
 }
 
-int lpmdx_system_stats(const lpmdx_system* S, long out[8]) {
+int lpmdx_system_stats(const lpmdx_system* S, long out[9]) {
   return guard([&] {
     need(S, "system");
     need(out, "out");
@@ -691,6 +691,7 @@ int lpmdx_system_stats(const lpmdx_system* S, long out[8]) {
     out[5] = S->lpa;
     out[6] = S->staged ? S->grid.brick : 0;
     out[7] = S->sum_list;
+    out[8] = S->n_cut_chunks;
   });
 }
 
