@@ -120,36 +120,6 @@ long lpmdx_system_size(const lpmdx_system* sys) { return sys ? sys->n : -1; }
 
 }
 
-namespace {
-
-// caller order <-> internal order through perm (internal -> caller)
-void to_internal(lpmdx_system* S, const double* h, double* d) {
-  const long n = S->n;
-  std::vector<int> perm(n);
-  hipchk(hipMemcpyAsync(perm.data(), S->perm.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
-  S->sync();
-  std::vector<double> tmp(3 * n);
-  for (long i = 0; i < n; ++i)
-    for (int q = 0; q < 3; ++q)
-      tmp[3 * i + q] = h[3 * perm[i] + q];
-  hipchk(hipMemcpyAsync(d, tmp.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
-  S->sync();
-}
-
-void to_caller(lpmdx_system* S, const double* d, double* h) {
-  const long n = S->n;
-  std::vector<int> perm(n);
-  std::vector<double> tmp(3 * n);
-  hipchk(hipMemcpyAsync(perm.data(), S->perm.p, n * sizeof(int), hipMemcpyDeviceToHost, S->st), "memcpy");
-  hipchk(hipMemcpyAsync(tmp.data(), d, 3 * n * sizeof(double), hipMemcpyDeviceToHost, S->st), "memcpy");
-  S->sync();
-  for (long i = 0; i < n; ++i)
-    for (int q = 0; q < 3; ++q)
-      h[3 * perm[i] + q] = tmp[3 * i + q];
-}
-
-} // namespace
-
 extern "C" {
 
 int lpmdx_system_set_state(lpmdx_system* S, const double* x, const double* v, const double* a) {
@@ -164,17 +134,15 @@ int lpmdx_system_set_state(lpmdx_system* S, const double* x, const double* v, co
     if (a)
       finite(a, 3 * S->n, "accelerations");
     S->undrift();
+    const double* const src[3] = {x, v, a};
+    double* const dst[3] = {S->x.p, S->v.p, S->a.p};
+    S->upload(src, dst);
     if (x) {
-      to_internal(S, x, S->x.p);
       S->forces_ok = false;
       S->pos_version++;
     }
-    if (v)
-      to_internal(S, v, S->v.p);
-    if (a) {
-      to_internal(S, a, S->a.p);
+    if (a)
       S->forces_ok = false;
-    }
   });
 }
 
@@ -184,12 +152,9 @@ int lpmdx_system_get_state(lpmdx_system* S, double* x, double* v, double* a) {
     if (S->n == 0)
       return;
     const double* xu = S->drifted ? S->xn.p : S->x.p;
-    if (x)
-      to_caller(S, xu, x);
-    if (v)
-      to_caller(S, S->v.p, v);
-    if (a)
-      to_caller(S, S->a.p, a);
+    const double* src[3] = {xu, S->v.p, S->a.p};
+    double* const dst[3] = {x, v, a};
+    S->download(src, dst);
   });
 }
 
@@ -329,8 +294,11 @@ int lpmdx_potential_forces(lpmdx_system* S, int k, double* acc, double* epot, do
       w = pr[2];
       for (int q = 0; q < 9; ++q)
         t[q] = pr[3 + q];
-      if (acc)
-        to_caller(S, S->asum.p, acc);
+      if (acc) {
+        const double* src[3] = {S->asum.p, nullptr, nullptr};
+        double* const dst[3] = {acc, nullptr, nullptr};
+        S->download(src, dst);
+      }
     }
     if (epot)
       *epot = e;

@@ -134,6 +134,32 @@ __global__ __launch_bounds__(kBlock) void k_gather_velocities(double* __restrict
     v[3 * i + q] = vc[3 * c + q];
 }
 
+// Slot-order [3n] rows into caller order: out[perm[i]] = d[i].
+__global__ __launch_bounds__(kBlock) void k_to_caller(const double* __restrict__ d, const int* __restrict__ perm,
+                                                      long n, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const long c = perm[i];
+  const double r[3] = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    out[3 * c + q] = r[q];
+}
+
+// Caller-order [3n] rows into slot order: d[i] = h[perm[i]].
+__global__ __launch_bounds__(kBlock) void k_from_caller(const double* __restrict__ h, const int* __restrict__ perm,
+                                                        long n, double* __restrict__ d) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  const long c = perm[i];
+  const double r[3] = {h[3 * c], h[3 * c + 1], h[3 * c + 2]};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    d[3 * i + q] = r[q];
+}
+
 __global__ void k_fill_double(double* p, long m, double v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m)
@@ -238,6 +264,10 @@ struct lpmdx_system {
   DBuf<Ctrl> ctrl;
   Ctrl* hctrl = nullptr; // pinned
   double* hred = nullptr;
+  // caller-order copies of x, v, a on the device and their pinned host stage
+  DBuf<double> cstage;
+  double* hstage = nullptr;
+  size_t hstage_n = 0;
   // host-side model
   CellMgr cm;
   std::vector<std::unique_ptr<PotParams>> pots;
@@ -411,9 +441,80 @@ struct lpmdx_system {
       (void)hipHostFree(hctrl);
     if (hred)
       (void)hipHostFree(hred);
+    if (hstage)
+      (void)hipHostFree(hstage);
     pots.clear();
     if (st)
       (void)hipStreamDestroy(st);
+  }
+
+  // Room for k caller-order [3n] arrays in the device and pinned stages.
+  void stage(int k) {
+    const size_t m = (size_t)3 * n * k;
+    cstage.alloc(m);
+    if (hstage_n < m) {
+      if (hstage)
+        (void)hipHostFree(hstage);
+      hstage = nullptr;
+      hstage_n = 0;
+      hipchk(hipHostMalloc(&hstage, m * sizeof(double)), "hipHostMalloc");
+      hstage_n = m;
+    }
+  }
+
+  // Copies up to three slot-order [3n] device arrays into caller-order host
+  // arrays (null entries skipped): one gather kernel each into a device
+  // stage, one copy into pinned memory, then into the caller's arrays.
+  void download(const double* const src[3], double* const dst[3]) {
+    int k = 0;
+    for (int j = 0; j < 3; ++j)
+      k += (src[j] && dst[j]) ? 1 : 0;
+    if (n == 0 || k == 0)
+      return;
+    const size_t m = (size_t)3 * n;
+    stage(k);
+    int s = 0;
+    for (int j = 0; j < 3; ++j)
+      if (src[j] && dst[j]) {
+        k_to_caller<<<blocks_for(n), kBlock, 0, st>>>(src[j], perm.p, n, cstage.p + m * s);
+        ++s;
+      }
+    launch_check();
+    hipchk(hipMemcpyAsync(hstage, cstage.p, m * k * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy");
+    sync();
+    s = 0;
+    for (int j = 0; j < 3; ++j)
+      if (src[j] && dst[j]) {
+        memcpy(dst[j], hstage + m * s, m * sizeof(double));
+        ++s;
+      }
+  }
+
+  // The reverse: caller-order host arrays into slot-order device arrays.
+  void upload(const double* const src[3], double* const dst[3]) {
+    int k = 0;
+    for (int j = 0; j < 3; ++j)
+      k += (src[j] && dst[j]) ? 1 : 0;
+    if (n == 0 || k == 0)
+      return;
+    const size_t m = (size_t)3 * n;
+    stage(k);
+    int s = 0;
+    for (int j = 0; j < 3; ++j)
+      if (src[j] && dst[j]) {
+        memcpy(hstage + m * s, src[j], m * sizeof(double));
+        ++s;
+      }
+    hipchk(hipMemcpyAsync(cstage.p, hstage, m * k * sizeof(double), hipMemcpyHostToDevice, st), "memcpy");
+    s = 0;
+    for (int j = 0; j < 3; ++j)
+      if (src[j] && dst[j]) {
+        k_from_caller<<<blocks_for(n), kBlock, 0, st>>>(cstage.p + m * s, perm.p, n, dst[j]);
+        ++s;
+      }
+    launch_check();
+    // the pinned stage is reused by the next call
+    sync();
   }
 
   void sync() {

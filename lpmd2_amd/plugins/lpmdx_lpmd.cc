@@ -98,6 +98,28 @@ void Velocities(lpmd::Configuration& conf, std::vector<double>& v) {
   }
 }
 
+// Whether the host still holds the positions (and, with hv, the velocities)
+// the last device step handed back: one pass over the atoms, nothing copied.
+bool HostUnchanged(lpmd::Configuration& conf, const std::vector<double>& hx, const std::vector<double>* hv) {
+  lpmd::BasicParticleSet& atoms = conf.Atoms();
+  const long n = atoms.Size();
+  if ((long)hx.size() != 3 * n || (hv && (long)hv->size() != 3 * n))
+    return false;
+  for (long i = 0; i < n; ++i) {
+    const lpmd::BasicAtom& at = atoms[i];
+    const lpmd::Vector& p = at.Position();
+    if (p[0] != hx[3 * i] || p[1] != hx[3 * i + 1] || p[2] != hx[3 * i + 2])
+      return false;
+    if (hv) {
+      const lpmd::Vector& u = at.Velocity();
+      const double* w = hv->data() + 3 * i;
+      if (u[0] != w[0] || u[1] != w[1] || u[2] != w[2])
+        return false;
+    }
+  }
+  return true;
+}
+
 // The fixedvel / fixedpos tags as TwoStepIntegrator::Advance and
 // OneStepIntegrator::Advance read them (twostepintegrator.cc:17-65,
 // onestepintegrator.cc:18-38): zero[i] for a fixedvel atom (its acceleration
@@ -223,15 +245,13 @@ lpmdx_system* DeviceCellManager::Bind(lpmd::Configuration& conf, bool upload) {
     Create(conf);
     return sys_;
   }
-  if (upload) {
+  if (upload && !HostUnchanged(conf, xbuf_, nullptr)) {
     std::vector<double> x;
     Positions(conf, x);
-    if (x != xbuf_) {
-      xbuf_.swap(x);
-      check(lpmdx_system_set_state(sys_, xbuf_.data(), nullptr, nullptr), kind_);
-      lists_ok_ = false;
-      ++version_;
-    }
+    xbuf_.swap(x);
+    check(lpmdx_system_set_state(sys_, xbuf_.data(), nullptr, nullptr), kind_);
+    lists_ok_ = false;
+    ++version_;
   }
   return sys_;
 }
@@ -556,12 +576,12 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
   lpmdx_system* sys = cm->Bind(sim, false);
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  std::vector<double> x, v;
-  Positions(sim, x);
-  Velocities(sim, v);
-  if (sys != last_ || x != hx_ || v != hv_) {
+  if (sys != last_ || !HostUnchanged(sim, hx_, &hv_)) {
     // the host changed the state (or this is the first step): send it; the
     // engine then starts from a fresh force evaluation
+    std::vector<double> x, v;
+    Positions(sim, x);
+    Velocities(sim, v);
     check(lpmdx_system_set_state(sys, x.data(), v.data(), nullptr), "velocityverlet");
     hx_.swap(x);
     hv_.swap(v);
@@ -664,11 +684,11 @@ void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     throw lpmd::RuntimeError("leapfrog (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  std::vector<double> x;
-  Positions(sim, x);
-  if (x != hx_) {
+  if (!HostUnchanged(sim, hx_, nullptr)) {
     // the host moved atoms: Leapfrog::Advance reads the positions and the
     // half-step velocities only, so the positions alone are sent
+    std::vector<double> x;
+    Positions(sim, x);
     check(lpmdx_system_set_state(sys, x.data(), nullptr, nullptr), "leapfrog");
     hx_.swap(x);
   }
@@ -775,10 +795,10 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     throw lpmd::RuntimeError("beeman (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  std::vector<double> x, v;
-  Positions(sim, x);
-  Velocities(sim, v);
-  if (x != hx_ || v != hv_) {
+  if (!HostUnchanged(sim, hx_, &hv_)) {
+    std::vector<double> x, v;
+    Positions(sim, x);
+    Velocities(sim, v);
     // the host changed the state: send it (the previous accelerations stay)
     check(lpmdx_system_set_state(sys, x.data(), v.data(), nullptr), "beeman");
     hx_.swap(x);
@@ -899,10 +919,10 @@ void DeviceNoseHoover::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     throw lpmd::RuntimeError("nosehoover (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  std::vector<double> x, v;
-  Positions(sim, x);
-  Velocities(sim, v);
-  if (x != hx_ || v != hv_) {
+  if (!HostUnchanged(sim, hx_, &hv_)) {
+    std::vector<double> x, v;
+    Positions(sim, x);
+    Velocities(sim, v);
     // the host changed the state (a modifier, a tag): send it; the history
     // and the friction stay
     check(lpmdx_system_set_state(sys, x.data(), v.data(), nullptr), "nosehoover");
