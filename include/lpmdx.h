@@ -406,6 +406,83 @@ LPMDX_API int lpmdx_server_stats(lpmdx_server* s, double out[LPMDX_NSERVER_STATS
 /* Lanes per atom of one potential kind's force launch (1, 2 or 4). */
 LPMDX_API int lpmdx_server_set_lanes(lpmdx_server* s, int kind, int lanes);
 
+/* ---- snapshots and configuration files -------------------------------------
+ * Cells are nine numbers, the three cell vectors row by row (the reference's
+ * cell[i][q]); the writers take orthogonal cells.  Positions at this boundary
+ * are Cartesian and in the caller's atom order, as everywhere above. */
+
+/* A snapshot of a system's state for checkpoints and trajectory files.
+ * lpmdx_snapshot_take enqueues the copy and returns at once: x, v and a are
+ * gathered on the system's stream into the snapshot's device buffer, then
+ * copied to its pinned host buffer on a stream of its own, so the system's
+ * next steps overlap the copy.  lpmdx_snapshot_wait (and info / state, which
+ * wait first) block until it has landed.  Taking a new snapshot into the same
+ * object waits for the previous one.  No reference counterpart: it feeds the
+ * writers below without stopping the step loop. */
+typedef struct lpmdx_snapshot lpmdx_snapshot;
+LPMDX_API int lpmdx_snapshot_create(lpmdx_snapshot** out);
+LPMDX_API int lpmdx_snapshot_destroy(lpmdx_snapshot* snap);
+LPMDX_API int lpmdx_snapshot_take(lpmdx_system* sys, lpmdx_snapshot* snap, long step);
+LPMDX_API int lpmdx_snapshot_wait(lpmdx_snapshot* snap);
+LPMDX_API int lpmdx_snapshot_info(lpmdx_snapshot* snap, long* n, long* step, double cell[9]);
+/* z[n], x / v / a [3n] and the fixed-atom flags [n] (0 where none were set);
+ * any pointer may be NULL. */
+LPMDX_API int lpmdx_snapshot_state(lpmdx_snapshot* snap, int* z, double* x, double* v, double* a, int* flags);
+
+/* SimulationEngine::Dump (liblpmd/simulation/simulationbuilder.cc:195-239):
+ * the step, the cell rows, then per atom "Z x y z vx vy vz ax ay az", a colour
+ * line ("false": colours are not kept) and a tag line, numbers in the
+ * stream's default format (%g, six significant digits, as the reference
+ * writes them).  The tags written are the fixed-atom flags (may be NULL):
+ * fixedvel true, fixedpos true, or fixedpos false for an atom that is only
+ * tagged. */
+LPMDX_API int lpmdx_dump_write(const char* path, long step, const double cell[9], long n, const int* z,
+                               const double* x, const double* v, const double* a, const int* flags);
+/* SimulationEngine::Restore (simulationbuilder.cc:241-290): *n receives the
+ * atom count, *step and cell (may be NULL) the step and cell; the arrays
+ * (each may be NULL) are filled only when cap >= *n, so a first call with
+ * cap 0 sizes them.  Colours are skipped; fixedvel / fixedpos tags become
+ * LPMDX_FIX_* flags. */
+LPMDX_API int lpmdx_dump_read(const char* path, long* step, double cell[9], long* n, long cap, int* z, double* x,
+                              double* v, double* a, int* flags);
+
+/* The lpmd / zlp configuration format (plugins/lpmd.cc).  Writing
+ * (LPMDFormat::WriteHeader / WriteCell, lpmd.cc:525-642): args are the
+ * plugin's keywords level (0 positions, 1 + velocities, 2 + accelerations),
+ * type (lpmd or zlp; a file name ending in "lpmd" or ".zlp" decides, as
+ * there), blocksize, compression and extra (TYPE only: colours are not kept);
+ * header "LPMD 2.0 L|Z" and "HDR SYM X Y Z ...", then per frame the atom
+ * count, the cell rows and per atom the symbol, the scaled position
+ * x / |cell row| and the velocity / acceleration, all in %f.  zlp frames are
+ * deflated on one stream, in blocksize pieces each ending with a sync flush,
+ * and prefixed by a byte giving sizeof(unsigned long) and the size in
+ * network order in that many bytes. */
+typedef struct lpmdx_cfgfile lpmdx_cfgfile;
+LPMDX_API int lpmdx_lpmd_open_write(lpmdx_cfgfile** out, const char* path, const char* args);
+LPMDX_API int lpmdx_lpmd_write(lpmdx_cfgfile* f, long n, const double cell[9], const int* z, const double* x,
+                               const double* v, const double* a);
+/* Reading (LPMDFormat::ReadHeader / ReadCell, lpmd.cc:125-430): LPMD 1.0
+ * and 2.0 text, LPMD 2.0 compressed, and the legacy ZLP 1 stream (a "Z4L2P"
+ * header).  *level receives the header's level (-1 for ZLP 1, whose frames
+ * carry their own).  lpmdx_lpmd_next reads the next frame and sets *n to its
+ * atom count, or -1 at the end of the file; lpmdx_lpmd_frame then copies its
+ * cell, atomic numbers (ElemNum: 0 for an unknown symbol), scaled positions s
+ * as written in the file (the reader's Cartesian(s) is left to the caller,
+ * who knows which cell applies), velocities, accelerations and level. */
+LPMDX_API int lpmdx_lpmd_open_read(lpmdx_cfgfile** out, const char* path, int* level);
+LPMDX_API int lpmdx_lpmd_next(lpmdx_cfgfile* f, long* n);
+LPMDX_API int lpmdx_lpmd_frame(lpmdx_cfgfile* f, double cell[9], int* z, double* s, double* v, double* a,
+                               int* level);
+LPMDX_API int lpmdx_lpmd_close(lpmdx_cfgfile* f);
+
+/* XYZFormat::WriteCell (plugins/xyz.cc:218-249): the atom count, an empty
+ * line, then per atom the symbol and the position (velocity, acceleration at
+ * level 1, 2) in %.<precout>f; args are the plugin's keywords (level,
+ * precout, inside: positions folded into the cell as FittedInside does).
+ * append = 1 adds a frame to an existing file. */
+LPMDX_API int lpmdx_xyz_write(const char* path, int append, long n, const double cell[9], const int* z,
+                              const double* x, const double* v, const double* a, const char* args);
+
 #ifdef __cplusplus
 }
 #endif

@@ -279,6 +279,14 @@ class System:
         flags = np.ascontiguousarray(flags, dtype=np.int32)
         _check(lib().lpmdx_system_set_fixed(self._h, _p(flags)))
 
+    def snapshot(self, step: int = 0, into=None):
+        """Copy x, v, a to the host without waiting for it (lpmdx_snapshot_take):
+        a snapshots.Snapshot whose state() waits for the copy; `into` reuses one."""
+        from .snapshots import Snapshot
+
+        snap = into if into is not None else Snapshot()
+        return snap.take(self, step)
+
     def stats(self):
         out = np.zeros(9, dtype=np.int64)
         _check(lib().lpmdx_system_stats(self._h, _p(out)))

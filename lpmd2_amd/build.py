@@ -11,7 +11,7 @@ ROOT = os.path.dirname(_HERE)
 SOURCES = [os.path.join(_HERE, "csrc", f) for f in (
     "batch.hip", "engine_kinds_eam_a.hip", "engine_kinds_pair_a.hip", "engine_kinds_eam_b.hip",
     "engine_kinds_pair_b.hip", "engine_kinds_pair_c.hip", "engine_kinds_pair_d.hip", "engine_kinds_eam_c.hip",
-    "engine.hip", "server.cpp")]
+    "engine.hip", "server.cpp", "formats.cpp")]
 HEADERS = [os.path.join(_HERE, "csrc", f) for f in ("common.hpp", "sort.hpp", "cells.hpp", "forces.hpp",
                                                      "batch.hpp", "host_util.hpp", "table.hpp",
                                                      "engine_system.hpp", "engine_launch.hpp")] + [
@@ -58,6 +58,6 @@ def build_library(force: bool = True) -> str:
     if failed:
         raise subprocess.CalledProcessError(1, f"hipcc {' '.join(failed)}")
     tmp = OUT + ".tmp"
-    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lz"])
     os.replace(tmp, OUT)
     return OUT

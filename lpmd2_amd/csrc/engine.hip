@@ -101,6 +101,7 @@ int lpmdx_system_create(lpmdx_system** out, long n, const double box[3], const i
       pm[i] = (int)i;
     }
     S->hmass.assign(ms.begin(), ms.begin() + n);
+    S->hz.assign(zz.begin(), zz.begin() + n);
     hipchk(hipMemcpyAsync(S->mass.p, ms.data(), m * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
     hipchk(hipMemcpyAsync(S->ffm.p, ff.data(), m * sizeof(double), hipMemcpyHostToDevice, S->st), "memcpy");
     hipchk(hipMemcpyAsync(S->z.p, zz.data(), m * sizeof(int), hipMemcpyHostToDevice, S->st), "memcpy");
@@ -453,6 +454,10 @@ int lpmdx_system_set_fixed(lpmdx_system* S, const int* flags) {
         tagged += (flags[i] & LPMDX_FIX_TAGGED) ? 1 : 0;
       }
     S->undrift(); // a look-ahead was made without these flags
+    if (flags)
+      S->hfix.assign(flags, flags + n);
+    else
+      S->hfix.clear();
     bool anypos = false;
     if (flags)
       for (long i = 0; i < n; ++i)
@@ -800,6 +805,149 @@ int lpmdx_neighbor_lists(lpmdx_system* S, int full, double rcut, long* counts, l
       }
     }
     *total = k;
+  });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- snapshots
+// An asynchronous copy of a system's state for checkpoints and trajectory
+// files (Dump, lpmd / zlp, xyz; lpmd2_amd/csrc/formats.cpp): x, v and a are
+// gathered into caller order on the system's stream, into the snapshot's own
+// device buffer, and copied to pinned host memory on the snapshot's stream,
+// so the system's next steps run while the copy is in flight.
+struct lpmdx_snapshot {
+  int device = -1;
+  long n = 0, step = 0;
+  double cell[9] = {0};
+  std::vector<int> z, fix;
+  DBuf<double> d;
+  double* h = nullptr;
+  size_t hn = 0;
+  hipStream_t cs = nullptr;
+  hipEvent_t ready = nullptr, done = nullptr;
+  bool pending = false;
+
+  ~lpmdx_snapshot() {
+    if (pending && done)
+      (void)hipEventSynchronize(done);
+    if (cs)
+      (void)hipStreamSynchronize(cs);
+    if (h)
+      (void)hipHostFree(h);
+    if (ready)
+      (void)hipEventDestroy(ready);
+    if (done)
+      (void)hipEventDestroy(done);
+    if (cs)
+      (void)hipStreamDestroy(cs);
+  }
+
+  void wait() {
+    if (pending) {
+      hipchk(hipEventSynchronize(done), "hipEventSynchronize");
+      pending = false;
+    }
+  }
+};
+
+extern "C" {
+
+int lpmdx_snapshot_create(lpmdx_snapshot** out) {
+  return guard([&] {
+    need(out, "out");
+    *out = new lpmdx_snapshot();
+  });
+}
+
+int lpmdx_snapshot_destroy(lpmdx_snapshot* P) {
+  return guard([&] { delete P; });
+}
+
+int lpmdx_snapshot_take(lpmdx_system* S, lpmdx_snapshot* P, long step) {
+  return guard([&] {
+    need(S, "system");
+    need(P, "snapshot");
+    // the previous copy lands before its buffers are reused
+    P->wait();
+    if (P->device >= 0 && P->device != S->device)
+      fail(LPMDX_EINVAL, "snapshot: taken on another device before");
+    P->device = S->device;
+    if (!P->cs) {
+      hipchk(hipStreamCreateWithFlags(&P->cs, hipStreamNonBlocking), "hipStreamCreate");
+      hipchk(hipEventCreateWithFlags(&P->ready, hipEventDisableTiming), "hipEventCreate");
+      hipchk(hipEventCreateWithFlags(&P->done, hipEventDisableTiming), "hipEventCreate");
+    }
+    const long n = S->n;
+    P->n = n;
+    P->step = step;
+    for (int k = 0; k < 9; ++k)
+      P->cell[k] = (k % 4 == 0) ? S->box.L[k / 4] : 0.0;
+    P->z = S->hz;
+    P->fix = S->hfix;
+    if (n == 0)
+      return;
+    const size_t m = (size_t)9 * n;
+    P->d.alloc(m);
+    if (P->hn < m) {
+      if (P->h)
+        (void)hipHostFree(P->h);
+      P->h = nullptr;
+      P->hn = 0;
+      hipchk(hipHostMalloc(&P->h, m * sizeof(double)), "hipHostMalloc");
+      P->hn = m;
+    }
+    const double* src[3] = {S->drifted ? S->xn.p : S->x.p, S->v.p, S->a.p};
+    for (int j = 0; j < 3; ++j)
+      k_to_caller<<<blocks_for(n), kBlock, 0, S->st>>>(src[j], S->perm.p, n, P->d.p + (size_t)3 * n * j);
+    S->launch_check();
+    hipchk(hipEventRecord(P->ready, S->st), "hipEventRecord");
+    hipchk(hipStreamWaitEvent(P->cs, P->ready, 0), "hipStreamWaitEvent");
+    hipchk(hipMemcpyAsync(P->h, P->d.p, m * sizeof(double), hipMemcpyDeviceToHost, P->cs), "memcpy");
+    hipchk(hipEventRecord(P->done, P->cs), "hipEventRecord");
+    P->pending = true;
+  });
+}
+
+int lpmdx_snapshot_wait(lpmdx_snapshot* P) {
+  return guard([&] {
+    need(P, "snapshot");
+    P->wait();
+  });
+}
+
+int lpmdx_snapshot_info(lpmdx_snapshot* P, long* n, long* step, double cell[9]) {
+  return guard([&] {
+    need(P, "snapshot");
+    P->wait();
+    if (n)
+      *n = P->n;
+    if (step)
+      *step = P->step;
+    if (cell)
+      memcpy(cell, P->cell, sizeof P->cell);
+  });
+}
+
+int lpmdx_snapshot_state(lpmdx_snapshot* P, int* z, double* x, double* v, double* a, int* flags) {
+  return guard([&] {
+    need(P, "snapshot");
+    P->wait();
+    const size_t n = (size_t)P->n;
+    if (!n)
+      return;
+    if (z)
+      memcpy(z, P->z.data(), n * sizeof(int));
+    double* dst[3] = {x, v, a};
+    for (int j = 0; j < 3; ++j)
+      if (dst[j])
+        memcpy(dst[j], P->h + 3 * n * j, 3 * n * sizeof(double));
+    if (flags) {
+      if (P->fix.size() == n)
+        memcpy(flags, P->fix.data(), n * sizeof(int));
+      else
+        memset(flags, 0, n * sizeof(int));
+    }
   });
 }
 

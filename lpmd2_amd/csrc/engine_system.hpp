@@ -230,6 +230,8 @@ struct lpmdx_system {
   Drand48 rng48;
   long vinit_step = -1; // the step whose modifier drew the velocities
   std::vector<double> hmass, hvinit;
+  // atomic numbers and the fixed-atom flags last set, in caller order
+  std::vector<int> hz, hfix;
   DBuf<double> vinit;
   DBuf<double> eatom; // per-atom energies of lpmdx_atom_energies
   const int* fix_arg() const { return any_fix ? fix.p : nullptr; }
