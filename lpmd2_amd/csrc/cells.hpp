@@ -586,4 +586,89 @@ static __global__ __launch_bounds__(kBlock) void k_build_list(BuildArgs A) {
   build_atom(A, i, i < A.n, ring);
 }
 
+// Stencil cells [t0, t1) of a linkedcell stencil (t = ((dz + s) S + dy + s) S
+// + dx + s, the order build_atom visits them) against atom i: the number of
+// candidates accepted, written from list position off on when write is set.
+// The test is scan_cell's, so the entries are the same.
+template <bool EXACT>
+__device__ __forceinline__ int lc_scan_cells(const BuildArgs& A, long i, int t0, int t1, int ci, int cj, int ck,
+                                             const double* xi, float4 fi, int off, bool write) {
+  const Grid& g = A.g;
+  const int s = g.side, S = 2 * s + 1;
+  int c = 0;
+  for (int t = t0; t < t1; ++t) {
+    const int dx = t % S - s, dy = (t / S) % S - s, dz = t / (S * S) - s;
+    const int z2 = wrap_around(ck + dz, g.nz), z1 = wrap_around(cj + dy, g.ny), z0 = wrap_around(ci + dx, g.nx);
+    const long p = (long)z2 * (g.nx * g.ny) + (long)z1 * g.nx + z0;
+    if (p >= g.ncell)
+      continue; // WrapAround's -n case: an empty cell, as in build_atom
+    const int lo = A.cb[p], hi = A.cb[p + 1];
+    for (int k = lo; k < hi; ++k) {
+      bool take;
+      if (EXACT) {
+        double ex = A.x[3 * (long)k] - xi[0], ey = A.x[3 * (long)k + 1] - xi[1], ez = A.x[3 * (long)k + 2] - xi[2];
+        min_image(A.b, ex, ey, ez);
+        take = sqmod(ex, ey, ez) < A.rl2;
+      } else {
+        const float4 q = A.xf[k];
+        const float fx = wrap1f(q.x - fi.x, A.Lf[0], A.halff[0]);
+        const float fy = wrap1f(q.y - fi.y, A.Lf[1], A.halff[1]);
+        const float fz = wrap1f(q.z - fi.z, A.Lf[2], A.halff[2]);
+        take = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz)) < A.rl2f;
+      }
+      if (take && k != (int)i) {
+        if (write && off + c < A.cap)
+          A.nbr[list_index(i, off + c, A.lpa, A.cap)] = k;
+        ++c;
+      }
+    }
+  }
+  return c;
+}
+
+// Small systems on a linkedcell stencil: one wavefront per atom.  With few
+// atoms the thread-per-atom build leaves most of the GPU idle while each
+// thread walks its stencil (125 cells at two cells a side) one dependent
+// cell-range load after another.  Here lane l takes stencil cells
+// [l c, (l + 1) c) (c = cells / 64, rounded up) in the stencil's order and
+// counts what it accepts; a wavefront prefix sum gives each lane its first
+// list position, and the lanes scan again and write there.  The list is the
+// thread-per-atom build's, entry for entry (stencil order, sorted order
+// within a cell).  Grid: one wavefront per atom.
+template <bool EXACT>
+static __global__ __launch_bounds__(kBlock) void k_build_list_lc_wave(BuildArgs A) {
+  const long i = (xcd_block() * kBlock + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  int total = 0;
+  if (i < A.n) { // the whole wavefront, or none of it
+    const Grid& g = A.g;
+    const uint32_t key = A.skeys[i];
+    if (key < (uint32_t)g.ncell) {
+      const double xi[3] = {A.x[3 * i], A.x[3 * i + 1], A.x[3 * i + 2]};
+      const float4 fi = A.xf[i];
+      int ci, cj, ck;
+      cell_coords(g, (int)key, ci, cj, ck);
+      const int S = 2 * g.side + 1, nst = S * S * S;
+      const int per = (nst + kWave - 1) / kWave;
+      const int t0 = min(lane * per, nst), t1 = min(t0 + per, nst);
+      const int c = lc_scan_cells<EXACT>(A, i, t0, t1, ci, cj, ck, xi, fi, 0, false);
+      int inc = c; // inclusive prefix over the lanes
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int u = __shfl_up(inc, o, kWave);
+        if (lane >= o)
+          inc += u;
+      }
+      total = __shfl(inc, kWave - 1, kWave);
+      lc_scan_cells<EXACT>(A, i, t0, t1, ci, cj, ck, xi, fi, inc - c, true);
+    }
+    if (lane == 0) {
+      A.cnt[i] = total;
+      if (total > 0) {
+        atomic_max_sparse(A.max_count, (unsigned int)total);
+        atomic_add_spread(A.sum_count, (unsigned long long)total);
+      }
+    }
+  }
+}
+
 } // namespace lpmdx

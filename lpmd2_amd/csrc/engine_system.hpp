@@ -30,6 +30,10 @@ using namespace lpmdx;
 // timing slots of lpmdx_system_profile_read
 enum ProfSlot { kProfForce = 0, kProfRho = 1, kProfBuild = 2, kProfOther = 3, kProfSlots = 4 };
 
+// Linkedcell / lcbinary stencils of systems up to this many atoms are listed
+// one wavefront per atom (k_build_list_lc_wave) rather than one thread per atom.
+constexpr long kWaveBuildMaxAtoms = 32768;
+
 namespace {
 
 // max |x - xb|^2 over atoms through the minimum image (xb: positions at the build)
@@ -914,7 +918,15 @@ struct lpmdx_system {
       B.lpa = lpa;
       B.path = path;
       hipEvent_t e0 = prof_begin();
-      k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
+      if (grid.mode == kLinkedCell && !staged && n <= kWaveBuildMaxAtoms) {
+        // few atoms on a linkedcell stencil: a wavefront per atom
+        if (path == kPathExact64)
+          k_build_list_lc_wave<true><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
+        else
+          k_build_list_lc_wave<false><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
+      } else {
+        k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
+      }
       prof_end(kProfBuild, e0);
       k_sum_slots<<<1, kWave, 0, st>>>(sumslots.p, &ctrl.p->sum_count);
       launch_check();
