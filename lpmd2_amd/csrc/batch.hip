@@ -73,6 +73,7 @@ struct HostSys {
 };
 
 constexpr long kBytesPerAtom = 264; // per atom slot over all per-atom arenas
+constexpr int kMaxCycle = 16;       // longest cycle of one full and cyc - 1 lite steps
 
 } // namespace
 
@@ -113,9 +114,10 @@ struct lpmdx_batch {
   // (served Gupta / Buckingham / Finnis-Sinclair jobs measured 18 / 31 / 3 %
   // faster than at one lane; the tile kernels keep few atoms per workgroup)
   int lpa_of_kind[kNumKinds + 1] = {0, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
-  // graph of one step
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // graphs of one full step and of one cycle (a full step and cyc - 1 lite steps)
+  hipGraph_t graph = nullptr, cgraph = nullptr;
+  hipGraphExec_t gexec = nullptr, cexec = nullptr;
+  int cyc = 4;
   bool dirty = true;
   // evaluation scratch
   DBuf<double> evala, partials, red;
@@ -125,10 +127,7 @@ struct lpmdx_batch {
   ~lpmdx_batch() {
     if (st)
       (void)hipStreamSynchronize(st);
-    if (gexec)
-      (void)hipGraphExecDestroy(gexec);
-    if (graph)
-      (void)hipGraphDestroy(graph);
+    release_graph();
     if (hst)
       (void)hipHostFree(hst);
     if (hred)
@@ -584,7 +583,7 @@ struct lpmdx_batch {
   void enqueue_step() {
     const int ns = (int)hsys.size();
     BArgs B = args(tiles_all.p, n_all);
-    k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B);
+    k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B, 0, -1);
     if (nchunks) {
       for (int p = 0; p < passes; ++p) {
         k_b_count<<<nchunks, kBlock, 0, st>>>(B, p);
@@ -617,21 +616,61 @@ struct lpmdx_batch {
     hipchk(hipGetLastError(), "kernel launch");
   }
 
-  void release_graph() {
-    if (gexec)
-      (void)hipGraphExecDestroy(gexec);
-    if (graph)
-      (void)hipGraphDestroy(graph);
-    gexec = nullptr;
-    graph = nullptr;
+  // `nlite` steps without the rebuild chain: per potential kind a chain of
+  // (control, rho, force) launches over that kind's systems, the kinds'
+  // chains on branches of their own, joined once at the end.
+  void enqueue_lite(int nlite) {
+    const int ns = (int)hsys.size();
+    BArgs B = args(tiles_all.p, n_all);
+    if (groups.size() == 1) {
+      for (int k = 0; k < nlite; ++k) {
+        k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B, 1, groups[0]->kind);
+        launch_kind(*groups[0], st);
+      }
+    } else if (groups.size() > 1) {
+      if (bst.size() < groups.size() || !ev_fork)
+        fail(LPMDX_EDEVICE, "batch: branch streams missing (internal error)");
+      hipchk(hipEventRecord(ev_fork, st), "hipEventRecord");
+      for (size_t g = 0; g < groups.size(); ++g) {
+        hipchk(hipStreamWaitEvent(bst[g], ev_fork, 0), "hipStreamWaitEvent");
+        for (int k = 0; k < nlite; ++k) {
+          k_b_control<<<blocks_for(ns), kBlock, 0, bst[g]>>>(B, 1, groups[g]->kind);
+          launch_kind(*groups[g], bst[g]);
+        }
+        hipchk(hipEventRecord(ev_join[g], bst[g]), "hipEventRecord");
+        hipchk(hipStreamWaitEvent(st, ev_join[g], 0), "hipStreamWaitEvent");
+      }
+    }
+    hipchk(hipGetLastError(), "kernel launch");
   }
 
+  void release_graph() {
+    for (hipGraphExec_t* e : {&gexec, &cexec})
+      if (*e) {
+        (void)hipGraphExecDestroy(*e);
+        *e = nullptr;
+      }
+    for (hipGraph_t* g : {&graph, &cgraph})
+      if (*g) {
+        (void)hipGraphDestroy(*g);
+        *g = nullptr;
+      }
+  }
+
+  // Two graphs: one full step (control, the rebuild chain, forces), and a
+  // cycle of one full step followed by cyc - 1 lite steps (no rebuild chain:
+  // a system that needs a new list holds until the next full step).
   void capture() {
     release_graph();
     hipchk(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     enqueue_step();
     hipchk(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
     hipchk(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hipchk(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    enqueue_step();
+    enqueue_lite(cyc - 1);
+    hipchk(hipStreamEndCapture(st, &cgraph), "hipStreamEndCapture");
+    hipchk(hipGraphInstantiate(&cexec, cgraph, nullptr, nullptr, 0), "hipGraphInstantiate");
   }
 
   // ---------------------------------------------------------------- running
@@ -649,19 +688,63 @@ struct lpmdx_batch {
     return m;
   }
 
+  // Steps between list builds of the batch's fastest-rebuilding system (its
+  // mean so far), the cycle length that wastes the fewest lite steps on
+  // holds; 4 before any system has rebuilt twice.
+  int cycle_length() const {
+    long best = 0;
+    for (size_t s = 0; s < hsys.size(); ++s)
+      if (meta[s].live && hst[s].builds >= 2) {
+        const long iv = hst[s].done / hst[s].builds;
+        best = best ? std::min(best, iv) : iv;
+      }
+    if (!best)
+      return 4;
+    return (int)std::max<long>(2, std::min<long>(best, kMaxCycle));
+  }
+
+  // graph steps still owed to the live systems by the current run
+  long owed() const {
+    long m = 0;
+    for (size_t s = 0; s < hsys.size(); ++s)
+      if (meta[s].live && !hst[s].overflow)
+        m = std::max(m, std::min(hst[s].quota, hst[s].left + (hst[s].init == 1 ? 1 : 0)));
+    return m;
+  }
+
   // advances every live system by up to `steps` steps (the first step of a new
-  // system is its initial force evaluation)
+  // system is its initial force evaluation): each system gets a quota of
+  // min(steps, its pending steps), spent in cycles of one full step and
+  // cyc - 1 lite steps; systems that held in a lite step are owed steps, and
+  // more cycles run until every quota is spent.
   void run(long steps) {
     if (dirty)
       plan();
     if (hsys.empty())
       return;
     pull_state();
-    long todo = std::min(steps, pending_steps());
-    for (long k = 0; k < todo; ++k)
-      hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
-    steps_run += todo;
-    pull_state();
+    const long todo = std::min(steps, pending_steps());
+    if (todo <= 0)
+      return;
+    const int want = cycle_length();
+    if (want < cyc || want >= cyc + 2) {
+      cyc = want;
+      sync();
+      capture();
+    }
+    for (size_t s = 0; s < hsys.size(); ++s)
+      hst[s].quota = meta[s].live ? todo : 0;
+    hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+    long rem = todo;
+    while (rem > 0) {
+      for (long k = 0; k < rem / cyc; ++k)
+        hipchk(hipGraphLaunch(cexec, st), "hipGraphLaunch");
+      for (long k = 0; k < rem % cyc; ++k)
+        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+      steps_run += (rem / cyc) * cyc + rem % cyc;
+      pull_state();
+      rem = owed();
+    }
     std::string bad;
     for (size_t s = 0; s < hsys.size(); ++s) {
       if (meta[s].live && hst[s].overflow)
@@ -810,6 +893,7 @@ struct lpmdx_batch {
         hst[s] = saved[s];
         hst[s].built = 0;
         hst[s].left = std::max<long>(hst[s].left, trial_steps + 1);
+        hst[s].quota = trial_steps + 8;
       }
       hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
       sync();

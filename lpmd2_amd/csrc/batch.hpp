@@ -76,6 +76,7 @@ struct BState {
   unsigned int pad;
   unsigned long long sumcnt;  // list entries of the last build
   long builds;                // list builds so far
+  long quota;                 // steps the current run may still advance it
 };
 
 // Arena pointers (one entry per atom slot unless noted).
@@ -118,11 +119,19 @@ struct BArgs {
 };
 
 // ---------------------------------------------------------------- control
-static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B) {
+// lite: a step launched without the rebuild chain, in which a system whose
+// list has run out (or that still waits for its first list) holds: it does
+// not advance and keeps its displacement maximum, so the next full step
+// reaches the same rebuild decision.  kind >= 0: only the systems of that
+// potential kind (the lite steps run on one branch per kind).  A system
+// advances only while its quota of steps for the current run lasts.
+static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B, int lite, int kind) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= B.nsys)
     return;
   const BSys& S = B.sys[s];
+  if (kind >= 0 && S.kind != kind)
+    return;
   BState& st = B.st[s];
   // the last step's drift moved the positions into the other buffer
   if (st.act)
@@ -133,14 +142,18 @@ static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B) {
   }
   if (st.init == 2)
     st.init = 0;
-  float m;
-  {
-    unsigned int bits = st.maxd2;
-    m = __uint_as_float(bits);
+  if ((st.init != 1 && st.left <= 0) || st.quota <= 0) {
+    st.act = 0;
+    return;
   }
-  st.maxd2 = 0u;
   if (st.init == 1) {
+    if (lite) {
+      st.act = 0;
+      return;
+    }
     // first force evaluation of the system: a = F(x0), then the drift to x1
+    st.maxd2 = 0u;
+    st.quota -= 1;
     st.init = 2;
     st.act = 1;
     st.flag = 1;
@@ -149,15 +162,24 @@ static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B) {
     st.upar = st.par;
     return;
   }
-  if (st.left <= 0) {
+  float m;
+  {
+    unsigned int bits = st.maxd2;
+    m = __uint_as_float(bits);
+  }
+  const double disp = sqrt((double)m); // largest |x - xb| of the positions about to be used
+  const int need = (!st.built) || (disp > S.budget);
+  if (lite && need) {
     st.act = 0;
     return;
   }
+  st.maxd2 = 0u;
+  st.quota -= 1;
   st.act = 1;
   st.left -= 1;
   st.done += 1;
-  st.disp = sqrt((double)m); // largest |x - xb| of the positions about to be used
-  st.flag = (!st.built) || (st.disp > S.budget);
+  st.disp = disp;
+  st.flag = need;
   if (st.flag) {
     st.disp = 0.0;
     st.built = 1;
