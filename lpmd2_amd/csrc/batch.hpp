@@ -543,7 +543,7 @@ static __global__ __launch_bounds__(kWave * LPA) void k_b_force(BArgs B) {
   Tally tal;
   // species are read only by the systems whose potentials need them (the
   // branch is uniform: one workgroup, one system)
-  if (B.sys[s].metal) {
+  if constexpr (metal_kind(KIND)) {
     if (A.P.spc_sum >= 0)
       eam_force_atom<KIND, LPA, false, false, true, false>(A, Stage{}, i, l, i < A.n, tal);
     else
@@ -572,7 +572,7 @@ static __global__ __launch_bounds__(kWave * LPA) void k_b_eval(BArgs B, int s, d
   A.maxd2 = nullptr;
   Tally tal;
   double emb = 0.0;
-  if (S.metal) {
+  if constexpr (metal_kind(KIND)) {
     if (rho_pass)
       eam_rho_atom<KIND, LPA, true, false>(A, Stage{}, i, l, i < A.n, B.R.w + S.a0, nullptr, emb);
     else

@@ -21,6 +21,8 @@ constexpr double kKin2eV = 103.64269;
 constexpr double kBoltzmann = 8.6173422E-05;
 constexpr double kPressFactor = 160217.65;
 constexpr double kUa3ToGrCm3 = 1.6605; // amu/A^3 -> g/cm^3 (units.cc:10), the session's "1.66050000"
+// log2 e, for the kernels' base-2 exponentials
+constexpr double kLog2e = 1.4426950408889634;
 
 struct Box {
   double L[3];

@@ -56,6 +56,13 @@ struct DevPot {
 
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// 2^x on the hardware exponential (v_exp_f32, 1 ulp).  The kernels take e^y
+// as 2^(y log2 e) with log2 e folded into the parameters on the host, so an
+// exponential is one instruction: the FP32 argument's own rounding (a few
+// ulp of its largest term) already bounds the error, which the library
+// expf's range reduction and overflow selects (about eleven more
+// instructions) would not lower.
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
 
 // t^k for 0 <= k <= 31 without branches (square-and-multiply over the bits of k)
 __device__ __forceinline__ float powu(float t, int k) {
@@ -135,7 +142,7 @@ __device__ __forceinline__ void pair_eval(const DevPot& P, double r2d, float r2,
     // morse.cc:53-62: E = D (1 - e^{-a(r-re)})^2, ff = 2 a D / r (1 - ex) ex
     const float ir = rsq(r2);
     const float r = r2 * ir;
-    const float ex = expf(-P.f[0] * (r - P.f[1]));
+    const float ex = exp2_hw(-P.f[4] * (r - P.f[1]));
     const float om = 1.0f - ex;
     fscal = P.f[3] * ir * om * ex;
     e = P.f[2] * om * om;
@@ -145,7 +152,7 @@ __device__ __forceinline__ void pair_eval(const DevPot& P, double r2d, float r2,
     const float r = r2 * ir;
     const float ir2 = ir * ir;
     const float ir6 = ir2 * ir2 * ir2;
-    const float ex = P.f[0] * expf(-r * P.f[3]);
+    const float ex = P.f[0] * exp2_hw(-r * P.f[5]);
     fscal = P.f[4] * ir6 * ir2 - ex * P.f[3] * ir;
     e = ex - P.f[1] * ir6;
   } else if constexpr (KIND == kHarmonic) {
@@ -176,7 +183,7 @@ template <int KIND, int EXP = 0> __device__ __forceinline__ float eam_rho(const 
     return sc_pow_m<EXP>(P, P.f[1] * rsq(r2));
   } else if constexpr (KIND == kGupta) { // gupta.cc:69
     const float r = r2 * rsq(r2);
-    return expf(P.f[6] - P.f[7] * r);
+    return exp2_hw(P.f[6] - P.f[7] * r);
   } else if constexpr (KIND == kFinnisSinclairExt) { // finnissinclair-ext.cc:93-99
     const float r = r2 * rsq(r2);
     const float d = P.f[6];
@@ -208,10 +215,11 @@ __device__ __forceinline__ void eam_pair(const DevPot& P, float r2, float wsum, 
     V = P.f[0] * tn;
     fscal = (ir * ir) * (P.f[8] * tn + P.f[9] * wsum * tm);
   } else if constexpr (KIND == kGupta) {
-    // gupta.cc:67-82, f = {A, r0, p, B, q, p, 2q, 2q/r0, p/r0, -A p/r0, B q/r0}
+    // gupta.cc:67-82, f = {A, r0, p, B, q, p L, 2q L, 2q/r0 L, p/r0 L, -A p/r0, B q/r0},
+    // L = log2 e
     const float r = r2 * ir;
-    const float ep = expf(P.f[5] - P.f[8] * r);
-    const float eq = expf(P.f[6] - P.f[7] * r);
+    const float ep = exp2_hw(P.f[5] - P.f[8] * r);
+    const float eq = exp2_hw(P.f[6] - P.f[7] * r);
     V = P.f[0] * ep;
     fscal = ir * (P.f[9] * ep + P.f[10] * wsum * eq);
   } else if constexpr (KIND == kFinnisSinclairExt) {

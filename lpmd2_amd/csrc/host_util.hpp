@@ -282,17 +282,21 @@ inline DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) 
     d.bins = (int)P.bins;
     break;
   case kMorse:
+    // {a, re, D, 2 a D, a log2 e}: the kernels' exponentials are base 2
     d.f[0] = (float)p[1];
     d.f[1] = (float)p[2];
     d.f[2] = (float)p[0];
     d.f[3] = (float)(2.0 * p[1] * p[0]);
+    d.f[4] = (float)(p[1] * kLog2e);
     break;
   case kBuckingham:
+    // {B1, B2, Ro, 1/Ro, 6 B2, log2 e / Ro}
     d.f[0] = (float)p[0];
     d.f[1] = (float)p[1];
     d.f[2] = (float)p[2];
     d.f[3] = (float)(1.0 / p[2]);
     d.f[4] = (float)(6.0 * p[1]);
+    d.f[5] = (float)(kLog2e / p[2]);
     break;
   case kSuttonChen:
     // {e, a, n, m, c, -, -, -, -n e, m c e / 2, -, K = c e}
@@ -305,14 +309,14 @@ inline DevPot device_view(const PotParams& P, double rc2_force, double rc2_rho) 
     d.f[11] = (float)(p[4] * p[0]);
     break;
   case kGupta:
-    // {A, r0, p, B, q, p, 2q, 2q/r0, p/r0, -A p/r0, B q/r0, K = B}:
-    // e^{-p(r-r0)/r0} = e^{p - (p/r0) r}, e^{-2q(r-r0)/r0} = e^{2q - (2q/r0) r}
+    // {A, r0, p, B, q, p L, 2q L, 2q/r0 L, p/r0 L, -A p/r0, B q/r0, K = B}, L = log2 e:
+    // e^{-p(r-r0)/r0} = 2^{(p - (p/r0) r) L}, e^{-2q(r-r0)/r0} = 2^{(2q - (2q/r0) r) L}
     for (int q = 0; q < 5; ++q)
       d.f[q] = (float)p[q];
-    d.f[5] = (float)p[2];
-    d.f[6] = (float)(2.0 * p[4]);
-    d.f[7] = (float)(2.0 * p[4] / p[1]);
-    d.f[8] = (float)(p[2] / p[1]);
+    d.f[5] = (float)(p[2] * kLog2e);
+    d.f[6] = (float)(2.0 * p[4] * kLog2e);
+    d.f[7] = (float)(2.0 * p[4] / p[1] * kLog2e);
+    d.f[8] = (float)(p[2] / p[1] * kLog2e);
     d.f[9] = (float)(-p[0] * p[2] / p[1]);
     d.f[10] = (float)(p[3] * p[4] / p[1]);
     d.f[11] = (float)p[3];
