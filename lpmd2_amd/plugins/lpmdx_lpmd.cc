@@ -10,7 +10,10 @@
 #include <lpmd/simulation.h>
 #include <runtime/runtime_context.h>
 
+#include <lpmd/colorhandler.h>
+
 #include <cmath>
+#include <fstream>
 #include <iostream>
 
 namespace lpmdx_lpmd {
@@ -515,10 +518,6 @@ void DeviceVelocityVerlet::HostStep(lpmd::Simulation& sim, lpmd::Potential& p) {
   last_ = nullptr;
 }
 
-namespace {
-
-// The device runs the step when every potential is a DevicePotential and the
-// cell manager a DeviceCellManager; returns that cell manager, else null.
 DeviceCellManager* DevicePath(lpmd::Simulation& sim, lpmd::Potential& p, std::vector<const DevicePotential*>& dev) {
   auto* cp = dynamic_cast<lpmd::CombinedPotential*>(&p);
   auto* cm = dynamic_cast<DeviceCellManager*>(&sim.GetCellManager());
@@ -533,6 +532,8 @@ DeviceCellManager* DevicePath(lpmd::Simulation& sim, lpmd::Potential& p, std::ve
   }
   return cm;
 }
+
+namespace {
 
 // After a device step: x, v, a into the host atoms (and hx, hv, ha), the
 // observables CombinedPotential::UpdateForces leaves behind
@@ -573,10 +574,17 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     HostStep(sim, p);
     return;
   }
+  const bool upload = cm->System() != last_ || !HostUnchanged(sim, hx_, &hv_);
+  const double ep = DeviceRun(sim, cm, dev, 1, upload);
+  ReadBack(sim, cm->System(), cm, ep, hx_, hv_, ha_, "velocityverlet");
+}
+
+double DeviceVelocityVerlet::DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm,
+                                       const std::vector<const DevicePotential*>& dev, long steps, bool upload) {
   lpmdx_system* sys = cm->Bind(sim, false);
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  if (sys != last_ || !HostUnchanged(sim, hx_, &hv_)) {
+  if (upload || sys != last_) {
     // the host changed the state (or this is the first step): send it; the
     // engine then starts from a fresh force evaluation
     std::vector<double> x, v;
@@ -588,8 +596,8 @@ void DeviceVelocityVerlet::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     last_ = sys;
   }
   double ep = 0.0, ek = 0.0;
-  check(lpmdx_velocityverlet(sys, dt, 1, 1, &ep, &ek), "velocityverlet");
-  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "velocityverlet");
+  check(lpmdx_velocityverlet(sys, dt, steps, steps, &ep, &ek), "velocityverlet");
+  return ep;
 }
 
 DeviceLeapfrog::DeviceLeapfrog(std::string args) : lpmd::Plugin("leapfrog", kVersion) {
@@ -679,12 +687,18 @@ void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     }
     return;
   }
+  const double ep = DeviceRun(sim, cm, dev, 1, !HostUnchanged(sim, hx_, nullptr));
+  ReadBack(sim, sys_, cm, ep, hx_, hv_, ha_, "leapfrog");
+}
+
+double DeviceLeapfrog::DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm,
+                                 const std::vector<const DevicePotential*>& dev, long steps, bool upload) {
   lpmdx_system* sys = cm->Bind(sim, false);
-  if (sys != sys_)
+  if (mode_ != kDevice || sys != sys_)
     throw lpmd::RuntimeError("leapfrog (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  if (!HostUnchanged(sim, hx_, nullptr)) {
+  if (upload) {
     // the host moved atoms: Leapfrog::Advance reads the positions and the
     // half-step velocities only, so the positions alone are sent
     std::vector<double> x;
@@ -693,8 +707,8 @@ void DeviceLeapfrog::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     hx_.swap(x);
   }
   double ep = 0.0, ek = 0.0;
-  check(lpmdx_leapfrog(sys, dt, 1, 1, &ep, &ek), "leapfrog");
-  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "leapfrog");
+  check(lpmdx_leapfrog(sys, dt, steps, steps, &ep, &ek), "leapfrog");
+  return ep;
 }
 
 DeviceBeeman::DeviceBeeman(std::string args) : lpmd::Plugin("beeman", kVersion) {
@@ -790,12 +804,18 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
                               (5.0 / 6.0) * a_old_[i] * dt - (1.0 / 6.0) * aux[i] * dt;
     return;
   }
+  const double ep = DeviceRun(sim, cm, dev, 1, !HostUnchanged(sim, hx_, &hv_));
+  ReadBack(sim, sys_, cm, ep, hx_, hv_, ha_, "beeman");
+}
+
+double DeviceBeeman::DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm,
+                               const std::vector<const DevicePotential*>& dev, long steps, bool upload) {
   lpmdx_system* sys = cm->Bind(sim, false);
-  if (sys != sys_)
+  if (mode_ != kDevice || sys != sys_)
     throw lpmd::RuntimeError("beeman (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  if (!HostUnchanged(sim, hx_, &hv_)) {
+  if (upload) {
     std::vector<double> x, v;
     Positions(sim, x);
     Velocities(sim, v);
@@ -805,8 +825,8 @@ void DeviceBeeman::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     hv_.swap(v);
   }
   double ep = 0.0, ek = 0.0;
-  check(lpmdx_beeman(sys, dt, 1, 1, &ep, &ek), "beeman");
-  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "beeman");
+  check(lpmdx_beeman(sys, dt, steps, steps, &ep, &ek), "beeman");
+  return ep;
 }
 
 // ------------------------------------------------------------------ nosehoover
@@ -914,12 +934,18 @@ void DeviceNoseHoover::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
                               (5.0 / 6.0) * a_old_[i] * dt - (1.0 / 6.0) * aux_[i] * dt;
     return;
   }
+  const double ep = DeviceRun(sim, cm, dev, 1, !HostUnchanged(sim, hx_, &hv_));
+  ReadBack(sim, sys_, cm, ep, hx_, hv_, ha_, "nosehoover");
+}
+
+double DeviceNoseHoover::DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm,
+                                   const std::vector<const DevicePotential*>& dev, long steps, bool upload) {
   lpmdx_system* sys = cm->Bind(sim, false);
-  if (sys != sys_)
+  if (mode_ != kDevice || sys != sys_)
     throw lpmd::RuntimeError("nosehoover (MI355X): the cell manager's system changed since Initialize");
   cm->UsePotentials(dev);
   cm->SyncFixed(sim);
-  if (!HostUnchanged(sim, hx_, &hv_)) {
+  if (upload) {
     std::vector<double> x, v;
     Positions(sim, x);
     Velocities(sim, v);
@@ -930,8 +956,449 @@ void DeviceNoseHoover::Advance(lpmd::Simulation& sim, lpmd::Potential& p) {
     hv_.swap(v);
   }
   double ep = 0.0, ek = 0.0;
-  check(lpmdx_nosehoover(sys, dt, 1, 1, &ep, &ek), "nosehoover");
-  ReadBack(sim, sys, cm, ep, hx_, hv_, ha_, "nosehoover");
+  check(lpmdx_nosehoover(sys, dt, steps, steps, &ep, &ek), "nosehoover");
+  return ep;
+}
+
+// ------------------------------------------------------------------ simulation
+
+namespace {
+
+// Default unit system's session values (runtime/units.cc:5-13 through the
+// session's 8-decimal strings, runtime/session.cc:17-38), for a simulation
+// without a runtime context.
+constexpr double kPressFactorDefault = 160217.65;
+constexpr double kUa3ToGrCm3Default = 1.6605;
+constexpr double kBoltzmannDefault = 8.6173422E-05;
+
+std::vector<std::unique_ptr<DeviceSimulation>>& Owned() {
+  static std::vector<std::unique_ptr<DeviceSimulation>> owned;
+  return owned;
+}
+
+} // namespace
+
+DeviceSimulation::DeviceSimulation(long natoms, const lpmd::BasicAtom& at) : atoms_(natoms, at) {
+  SetTag(*this, lpmd::Tag("step"), step_);
+}
+
+void DeviceSimulation::Refresh() const {
+  // the device holds the newer state: x, v and a into the host atoms
+  if (!device_ahead_ || raw_view_)
+    return;
+  lpmdx_system* sys = dev_ ? dev_->System() : nullptr;
+  device_ahead_ = false;
+  if (!sys)
+    return;
+  auto& atoms = const_cast<lpmd::FixedSizeParticleSet&>(atoms_);
+  const long n = atoms.Size();
+  std::vector<double> a(3 * n);
+  agreed_x_.resize(3 * n);
+  agreed_v_.resize(3 * n);
+  check(lpmdx_system_get_state(sys, agreed_x_.data(), agreed_v_.data(), a.data()), "simulation");
+  for (long i = 0; i < n; ++i) {
+    const double* x = agreed_x_.data() + 3 * i;
+    const double* v = agreed_v_.data() + 3 * i;
+    atoms[i].Position() = lpmd::Vector(x[0], x[1], x[2]);
+    atoms[i].Velocity() = lpmd::Vector(v[0], v[1], v[2]);
+    atoms[i].Acceleration() = lpmd::Vector(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+  }
+  dev_->Mirror(agreed_x_); // the device holds these positions
+  ++refreshes_;
+}
+
+lpmd::BasicParticleSet& DeviceSimulation::OriginalAtoms() {
+  if (raw_view_)
+    return atoms_;
+  Refresh();
+  handed_out_ = true;
+  return atoms_;
+}
+
+const lpmd::BasicParticleSet& DeviceSimulation::OriginalAtoms() const {
+  Refresh();
+  return atoms_;
+}
+
+bool DeviceSimulation::HostChanged() const {
+  // positions or velocities written since host and device last agreed
+  const long n = atoms_.Size();
+  if ((long)agreed_x_.size() != 3 * n || (long)agreed_v_.size() != 3 * n)
+    return true;
+  for (long i = 0; i < n; ++i) {
+    const lpmd::BasicAtom& at = atoms_[i];
+    const lpmd::Vector& p = at.Position();
+    const lpmd::Vector& u = at.Velocity();
+    const double* x = agreed_x_.data() + 3 * i;
+    const double* v = agreed_v_.data() + 3 * i;
+    if (p[0] != x[0] || p[1] != x[1] || p[2] != x[2] || u[0] != v[0] || u[1] != v[1] || u[2] != v[2])
+      return true;
+  }
+  return false;
+}
+
+double DeviceSimulation::Session(const char* key, double fallback) const {
+  if (!HasRuntimeContext())
+    return fallback;
+  return double(const_cast<DeviceSimulation*>(this)->Context().session()[key]);
+}
+
+lpmd::Integrator& DeviceSimulation::Integrator() {
+  if (integ_ == nullptr)
+    throw lpmd::MissingComponent("integrator");
+  return *integ_;
+}
+
+void DeviceSimulation::AdjustCurrentStep(long s) {
+  step_ = s;
+  SetTag(*this, lpmd::Tag("step"), step_);
+}
+
+// SimulationEngine::Initialize (simulationbuilder.cc:187-193)
+void DeviceSimulation::Initialize() {
+  initial_step_ = int(lpmd::Parameter(GetTag(*this, lpmd::Tag("step"))));
+  Integrator().Initialize(*this, potarray_);
+  step_ = initial_step_;
+  SetTag(*this, lpmd::Tag("step"), step_);
+  initialized_ = true;
+}
+
+// SimulationEngine::InitVelocities (simulationbuilder.cc:167-185): a random
+// direction of fixed speed per atom (drand48 through RandomVector), the mean
+// momentum over the mass taken out.
+void DeviceSimulation::InitVelocities() {
+  lpmd::BasicParticleSet& atoms = OriginalAtoms();
+  const long nparts = atoms.Size();
+  const double kb = HasRuntimeContext() ? Context().units().kboltzmann : kBoltzmannDefault;
+  const double Kpart = (3.0 / 2.0) * kb * 1.0;
+  lpmd::Vector vel, totalp;
+  for (long i = 0; i < nparts; ++i) {
+    const double v2 = 2.0 * Kpart / atoms[i].Mass();
+    vel = lpmd::RandomVector(sqrt(v2));
+    totalp += vel * atoms[i].Mass();
+    atoms[i].Velocity() = vel;
+  }
+  totalp = totalp / nparts;
+  for (long i = 0; i < nparts; ++i)
+    atoms[i].Velocity() -= (totalp / atoms[i].Mass());
+  velocities_set_ = true;
+}
+
+// SimulationEngine::SetTemperature (simulationbuilder.cc:53-71), on the host
+// atoms; a changed state is sent before the next device step.
+void DeviceSimulation::SetTemperature(double temp, bool tag) {
+  if (!velocities_set_)
+    InitVelocities();
+  lpmd::BasicParticleSet& atoms = Atoms();
+  const double ti = lpmd::Temperature(atoms, tag);
+  for (long i = 0; i < atoms.Size(); ++i) {
+    if (tag && (atoms.Have(atoms[i], lpmd::Tag("fixedvel")) || atoms.Have(atoms[i], lpmd::Tag("fixedpos"))))
+      continue;
+    atoms[i].Velocity() *= sqrt(temp / ti);
+  }
+}
+
+// SimulationEngine::SetSimulationTags (simulationbuilder.cc:84-124) from the host atoms
+void DeviceSimulation::SetSimulationTags() {
+  const double pressfactor = Session("pressfactor", kPressFactorDefault);
+  const double densityfact = Session("ua3togrcm3", kUa3ToGrCm3Default);
+  lpmd::BasicParticleSet& atoms = Atoms();
+  const lpmd::BasicCell& cell = Cell();
+  const double v = cell.Volume();
+  const bool tagged = atoms.HaveAny(lpmd::Tag("fixedvel")) || atoms.HaveAny(lpmd::Tag("fixedpos"));
+  const lpmd::Vector P = lpmd::Momentum(atoms, tagged);
+  SetTag(*this, lpmd::Tag("step"), step_);
+  SetTag(*this, lpmd::Tag("temperature"), lpmd::Temperature(atoms, tagged));
+  SetTag(*this, lpmd::Tag("volume"), v);
+  SetTag(*this, lpmd::Tag("volume-per-atom"), v / double(atoms.Size()));
+  SetTag(*this, lpmd::Tag("cell-a"), cell[0].Module());
+  SetTag(*this, lpmd::Tag("cell-b"), cell[1].Module());
+  SetTag(*this, lpmd::Tag("cell-c"), cell[2].Module());
+  SetTag(*this, lpmd::Tag("particle-density"), double(atoms.Size()) / v);
+  SetTag(*this, lpmd::Tag("density"), densityfact * lpmd::Density(atoms, cell));
+  SetTag(*this, lpmd::Tag("momentum"), P.Module());
+  SetTag(*this, lpmd::Tag("px"), P[0]);
+  SetTag(*this, lpmd::Tag("py"), P[1]);
+  SetTag(*this, lpmd::Tag("pz"), P[2]);
+  const double potenerg = double(lpmd::Parameter(GetTag(*this, lpmd::Tag("potential-energy"))));
+  const double kinenerg = lpmd::KineticEnergy(atoms, tagged);
+  SetTag(*this, lpmd::Tag("kinetic-energy"), kinenerg);
+  SetTag(*this, lpmd::Tag("total-energy"), potenerg + kinenerg);
+  const lpmd::Matrix& stress = StressTensor();
+  SetTag(*this, lpmd::Tag("virial-pressure"), (pressfactor / v) * (1.0 / 3.0) * Virial());
+  SetTag(*this, lpmd::Tag("kinetic-pressure"), (pressfactor / v) * (2.0 / 3.0) * kinenerg);
+  SetTag(*this, lpmd::Tag("pressure"), (pressfactor / v) * ((2.0 / 3.0) * kinenerg + (1.0 / 3.0) * Virial()));
+  const char letter[3] = {'x', 'y', 'z'};
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) {
+      const char label[4] = {'s', letter[q], letter[p], '\0'};
+      SetTag(*this, lpmd::Tag(std::string(label)), (pressfactor / v) * stress.Get(q, p));
+    }
+}
+
+// The same tags from the engine's device reductions after a device run
+// (lpmdx_observables: the kinetic sums leave tagged atoms out when any atom
+// carries fixedvel or fixedpos, as SetSimulationTags' tag argument does), the
+// potential-energy tag CombinedPotential::UpdateForces sets
+// (combinedpotential.h:39-52), and the virial and stress of the last force
+// evaluation.
+void DeviceSimulation::DeviceTags(lpmdx_system* sys, double epot) {
+  double obs[LPMDX_NOBS];
+  check(lpmdx_observables(sys, obs), "simulation");
+  const double pressfactor = Session("pressfactor", kPressFactorDefault);
+  const double densityfact = Session("ua3togrcm3", kUa3ToGrCm3Default);
+  const double v = cell_.Volume();
+  Virial() = obs[LPMDX_OBS_VIRIAL];
+  lpmd::Matrix& st = StressTensor();
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      st.Set(a, b, obs[LPMDX_OBS_STRESS + 3 * a + b] * v / kPressFactorDefault);
+  SetTag(*this, lpmd::Tag("potential-energy"), epot);
+  const double px = obs[LPMDX_OBS_PX], py = obs[LPMDX_OBS_PX + 1], pz = obs[LPMDX_OBS_PX + 2];
+  const double kinenerg = obs[LPMDX_OBS_KINETIC];
+  SetTag(*this, lpmd::Tag("step"), step_);
+  SetTag(*this, lpmd::Tag("temperature"), obs[LPMDX_OBS_TEMP]);
+  SetTag(*this, lpmd::Tag("volume"), v);
+  SetTag(*this, lpmd::Tag("volume-per-atom"), v / double(atoms_.Size()));
+  SetTag(*this, lpmd::Tag("cell-a"), cell_[0].Module());
+  SetTag(*this, lpmd::Tag("cell-b"), cell_[1].Module());
+  SetTag(*this, lpmd::Tag("cell-c"), cell_[2].Module());
+  SetTag(*this, lpmd::Tag("particle-density"), double(atoms_.Size()) / v);
+  SetTag(*this, lpmd::Tag("density"), densityfact * obs[LPMDX_OBS_DENSITY] / kUa3ToGrCm3Default);
+  SetTag(*this, lpmd::Tag("momentum"), obs[LPMDX_OBS_MOMENTUM]);
+  SetTag(*this, lpmd::Tag("px"), px);
+  SetTag(*this, lpmd::Tag("py"), py);
+  SetTag(*this, lpmd::Tag("pz"), pz);
+  SetTag(*this, lpmd::Tag("kinetic-energy"), kinenerg);
+  SetTag(*this, lpmd::Tag("total-energy"), epot + kinenerg);
+  SetTag(*this, lpmd::Tag("virial-pressure"), (pressfactor / v) * (1.0 / 3.0) * Virial());
+  SetTag(*this, lpmd::Tag("kinetic-pressure"), (pressfactor / v) * (2.0 / 3.0) * kinenerg);
+  SetTag(*this, lpmd::Tag("pressure"), (pressfactor / v) * ((2.0 / 3.0) * kinenerg + (1.0 / 3.0) * Virial()));
+  const char letter[3] = {'x', 'y', 'z'};
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) {
+      const char label[4] = {'s', letter[q], letter[p], '\0'};
+      SetTag(*this, lpmd::Tag(std::string(label)), (pressfactor / v) * st.Get(q, p));
+    }
+}
+
+// SimulationEngine::DoStep (simulationbuilder.cc:126-136) on the host
+void DeviceSimulation::HostStep() {
+  Virial() = 0.0;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q)
+      StressTensor().Set(q, p, 0.0);
+  if (!initialized_)
+    Initialize();
+  Integrator().Advance(*this, potarray_);
+  step_++;
+  SetSimulationTags();
+  ++host_steps_;
+}
+
+void DeviceSimulation::DoSteps(long n) {
+  if (n <= 0)
+    return;
+  const bool first = !initialized_;
+  if (first)
+    Initialize(); // as the first DoStep does, before anything else
+  auto* di = dynamic_cast<DeviceIntegrator*>(&Integrator());
+  std::vector<const DevicePotential*> dev;
+  DeviceCellManager* cm = di ? DevicePath(*this, potarray_, dev) : nullptr;
+  if (!cm || !di->DeviceReady()) {
+    for (long k = 0; k < n; ++k)
+      HostStep();
+    return;
+  }
+  if (first) {
+    // Initialize has just handed the host state to the integrator (the
+    // schemes with an old configuration sent it to the device there;
+    // velocity Verlet sends it with its first run): host and device agree
+    const long na = atoms_.Size();
+    agreed_x_.resize(3 * na);
+    agreed_v_.resize(3 * na);
+    for (long i = 0; i < na; ++i)
+      for (int q = 0; q < 3; ++q) {
+        agreed_x_[3 * i + q] = atoms_[i].Position()[q];
+        agreed_v_[3 * i + q] = atoms_[i].Velocity()[q];
+      }
+    dev_ = cm;
+    device_ahead_ = false;
+    handed_out_ = false;
+  }
+  // the host state goes to the device when the host holds the current state
+  // and it differs from what host and device last agreed on (or the device
+  // never held this simulation's state)
+  const bool upload = !device_ahead_ && (dev_ != cm || (handed_out_ && HostChanged()));
+  Virial() = 0.0;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q)
+      StressTensor().Set(q, p, 0.0);
+  double ep = 0.0;
+  raw_view_ = !upload; // the integrator reads tags and sizes only, unless it sends the host state
+  try {
+    ep = di->DeviceRun(*this, cm, dev, n, upload);
+  } catch (...) {
+    raw_view_ = false;
+    throw;
+  }
+  raw_view_ = false;
+  if (upload) {
+    // what was sent is what host and device agree on until the run moves it
+    ++uploads_;
+  }
+  dev_ = cm;
+  device_ahead_ = true;
+  handed_out_ = false;
+  ++runs_;
+  step_ += n;
+  DeviceTags(cm->System(), ep);
+}
+
+// SimulationEngine::RescalePositions (simulationbuilder.cc:155-162)
+void DeviceSimulation::RescalePositions(const lpmd::BasicCell& old_cell) {
+  lpmd::BasicParticleSet& t_atoms = Atoms();
+  lpmd::BasicCell& t_cell = Cell();
+  for (long i = 0; i < t_atoms.Size(); ++i)
+    t_atoms[i].Position() = t_cell.FittedInside(t_cell.Cartesian(old_cell.Fractional(t_atoms[i].Position())));
+}
+
+// SimulationEngine::Dump (simulationbuilder.cc:195-236): step, cell vectors,
+// atoms (Z, x, v, a), colour flags, atom tags
+void DeviceSimulation::Dump(const std::string& path) const {
+  const lpmd::BasicCell& cell = Cell();
+  const lpmd::BasicParticleSet& atoms = Atoms();
+  std::ofstream f(path.c_str());
+  f << step_ << '\n';
+  for (int i = 0; i < 3; ++i) {
+    for (int q = 0; q < 3; ++q)
+      f << cell[i][q] << " ";
+    f << '\n';
+  }
+  f << atoms.Size() << '\n';
+  for (long i = 0; i < atoms.Size(); ++i) {
+    const lpmd::Vector& pos = atoms[i].Position();
+    const lpmd::Vector& vel = atoms[i].Velocity();
+    const lpmd::Vector& acc = atoms[i].Acceleration();
+    f << atoms[i].Z() << " ";
+    for (int q = 0; q < 3; ++q)
+      f << pos[q] << " ";
+    for (int q = 0; q < 3; ++q)
+      f << vel[q] << " ";
+    for (int q = 0; q < 3; ++q)
+      f << acc[q] << " ";
+    f << '\n';
+  }
+  for (long i = 0; i < atoms.Size(); ++i) {
+    const bool has = lpmd::ColorHandler::HaveColor(atoms[i]);
+    f << std::boolalpha << has << " ";
+    if (has) {
+      const lpmd::Color& color = lpmd::ColorHandler::ColorOfAtom(atoms[i]);
+      for (int q = 0; q < 3; ++q)
+        f << color[q] << " ";
+    }
+    f << '\n';
+  }
+  for (long i = 0; i < atoms.Size(); ++i) {
+    const lpmd::Array<lpmd::Parameter> ap = atoms.GetTags(atoms[i]);
+    f << ap.Size() << " ";
+    for (int q = 0; q < ap.Size(); ++q)
+      f << ap[q] << " " << atoms.GetTag(atoms[i], lpmd::Tag(ap[q])) << " ";
+    f << '\n';
+  }
+}
+
+// SimulationEngine::Restore (simulationbuilder.cc:238-290): the inverse of
+// Dump; the velocities count as set and the integrator is initialised again
+// at the next step
+void DeviceSimulation::Restore(const std::string& path) {
+  lpmd::BasicCell& cell = Cell();
+  lpmd::BasicParticleSet& atoms = Atoms();
+  std::ifstream f(path.c_str());
+  f >> initial_step_;
+  SetTag(*this, lpmd::Tag("step"), initial_step_);
+  for (int i = 0; i < 3; ++i)
+    for (int q = 0; q < 3; ++q)
+      f >> cell[i][q];
+  long natoms;
+  f >> natoms;
+  atoms.Clear();
+  for (long i = 0; i < natoms; ++i) {
+    int z;
+    f >> z;
+    atoms.Append(lpmd::Atom(z));
+    lpmd::Vector& pos = atoms[i].Position();
+    lpmd::Vector& vel = atoms[i].Velocity();
+    lpmd::Vector& acc = atoms[i].Acceleration();
+    for (int q = 0; q < 3; ++q)
+      f >> pos[q];
+    for (int q = 0; q < 3; ++q)
+      f >> vel[q];
+    for (int q = 0; q < 3; ++q)
+      f >> acc[q];
+  }
+  std::string key, value;
+  for (long i = 0; i < natoms; ++i) {
+    f >> key;
+    if (key == "true") {
+      lpmd::Color color;
+      for (int q = 0; q < 3; ++q)
+        f >> color[q];
+      lpmd::ColorHandler::ColorOfAtom(atoms[i]) = color;
+    }
+  }
+  for (long i = 0; i < natoms; ++i) {
+    long tag_number;
+    f >> tag_number;
+    for (long q = 0; q < tag_number; ++q) {
+      f >> key;
+      f >> value;
+      atoms.SetTag(atoms[i], lpmd::Tag(key), value);
+    }
+  }
+  velocities_set_ = true;
+  initialized_ = false;
+  step_ = initial_step_;
+}
+
+lpmd::Simulation& DeviceSimulationBuilder::CreateFixedOrthogonal(lpmd::RuntimeContext& context, long atoms,
+                                                                 const lpmd::BasicAtom& at) {
+  lpmd::Simulation& s = CreateFixedOrthogonal(atoms, at);
+  s.SetRuntimeContext(context);
+  return s;
+}
+
+lpmd::Simulation& DeviceSimulationBuilder::CreateFixedOrthogonal(long atoms, const lpmd::BasicAtom& at) {
+  Owned().push_back(std::make_unique<DeviceSimulation>(atoms, at));
+  return *Owned().back();
+}
+
+lpmd::Simulation& DeviceSimulationBuilder::CloneOptimized(lpmd::RuntimeContext& context, const lpmd::Simulation& sim) {
+  const lpmd::BasicParticleSet& atoms = sim.Atoms();
+  const lpmd::BasicCell& cell = sim.Cell();
+  if (!cell.IsOrthogonal())
+    throw lpmd::RuntimeError("DeviceSimulationBuilder (MI355X): the cell must be orthogonal");
+  lpmd::Simulation& s = CreateFixedOrthogonal(context, atoms.Size(), atoms[0]);
+  s.AdjustCurrentStep(int(lpmd::Parameter(sim.GetTag(sim, lpmd::Tag("step")))));
+  lpmd::BasicParticleSet& newatoms = s.Atoms();
+  for (long i = 0; i < atoms.Size(); ++i) {
+    newatoms[i] = atoms[i];
+    const lpmd::Array<lpmd::Parameter> ap = atoms.GetTags(atoms[i]);
+    for (int q = 0; q < ap.Size(); ++q)
+      newatoms.SetTag(newatoms[i], lpmd::Tag(ap[q]), atoms.GetTag(atoms[i], lpmd::Tag(ap[q])));
+  }
+  for (int q = 0; q < 3; ++q)
+    s.Cell()[q] = cell[q];
+  return s;
+}
+
+void DeviceSimulationBuilder::Destroy(lpmd::Simulation& sim) {
+  auto& v = Owned();
+  for (auto it = v.begin(); it != v.end(); ++it)
+    if (it->get() == &sim) {
+      v.erase(it);
+      return;
+    }
 }
 
 // ------------------------------------------------------------------ factory

@@ -17,6 +17,9 @@
 //   DeviceLeapfrog        lpmd::Integrator as "leapfrog" (plugins/leapfrog.cc:14-23)
 //   DeviceBeeman          lpmd::Integrator as "beeman" (plugins/beeman.cc)
 //   DeviceNoseHoover      lpmd::Integrator as "nosehoover" (plugins/nosehoover.cc)
+//   DeviceSimulation      lpmd::Simulation (liblpmd/lpmd/simulation.h:19-46) whose
+//                         steps stay on the device, built by DeviceSimulationBuilder
+//                         with SimulationBuilder's signatures (simulationbuilder.h:16-29)
 // They take the same plugin names and keyword strings, so a directory holding
 // their <name>.so files placed first on LPMD_PATH replaces the CPU plugins.
 // The classes live in one shared library (liblpmdx_lpmd.so) so that every
@@ -25,14 +28,19 @@
 #pragma once
 
 #include <lpmd/cellmanager.h>
+#include <lpmd/combinedpotential.h>
 #include <lpmd/configuration.h>
+#include <lpmd/fixedsizeparticleset.h>
 #include <lpmd/integrator.h>
 #include <lpmd/metalpotential.h>
+#include <lpmd/orthogonalcell.h>
 #include <lpmd/pairpotential.h>
 #include <lpmd/plugin.h>
 #include <lpmd/potential.h>
+#include <lpmd/simulation.h>
 
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -176,7 +184,29 @@ private:
   PairForm form_;
 };
 
-class DeviceVelocityVerlet : public lpmd::Integrator, public lpmd::Plugin {
+// The device runs the steps when every potential is a DevicePotential and the
+// cell manager a DeviceCellManager; returns that cell manager (and the
+// potentials in dev), else null.
+DeviceCellManager* DevicePath(lpmd::Simulation& sim, lpmd::Potential& p, std::vector<const DevicePotential*>& dev);
+
+// What the drop-in integrators share with DeviceSimulation: several steps on
+// the device in one call, with nothing copied back.
+class DeviceIntegrator {
+public:
+  virtual ~DeviceIntegrator() = default;
+  // Runs `steps` steps of the scheme on cm's device system (DevicePath holds;
+  // the schemes with an old configuration have run Initialize on the
+  // device).  upload: the host holds the current state, which is sent first
+  // as Advance sends a state the host changed; otherwise the device holds it.
+  // Returns the potential energy of the last step's force evaluation.
+  virtual double DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm, const std::vector<const DevicePotential*>& dev,
+                           long steps, bool upload) = 0;
+  // Whether the scheme can run on the device for this simulation now (its
+  // Initialize ran on the device, or it needs none).
+  virtual bool DeviceReady() const = 0;
+};
+
+class DeviceVelocityVerlet : public lpmd::Integrator, public lpmd::Plugin, public DeviceIntegrator {
 public:
   explicit DeviceVelocityVerlet(std::string args);
   void ShowHelp() const override;
@@ -187,6 +217,9 @@ public:
   // tags and monitors); otherwise the reference's two-step scheme runs on the
   // host (twostepintegrator.cc:12-68, velocityverlet.cc:55-75).
   void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  double DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm, const std::vector<const DevicePotential*>& dev,
+                   long steps, bool upload) override;
+  bool DeviceReady() const override { return true; }
 
 private:
   void HostStep(lpmd::Simulation& sim, lpmd::Potential& p);
@@ -195,7 +228,7 @@ private:
   lpmdx_system* last_ = nullptr;
 };
 
-class DeviceLeapfrog : public lpmd::Integrator, public lpmd::Plugin {
+class DeviceLeapfrog : public lpmd::Integrator, public lpmd::Plugin, public DeviceIntegrator {
 public:
   explicit DeviceLeapfrog(std::string args);
   void ShowHelp() const override;
@@ -207,6 +240,9 @@ public:
   // One leapfrog step (onestepintegrator.cc:13-46, leapfrog.cc:69-77), on the
   // device or, when Initialize ran on the host, on the host.
   void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  double DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm, const std::vector<const DevicePotential*>& dev,
+                   long steps, bool upload) override;
+  bool DeviceReady() const override { return mode_ == kDevice; }
 
 private:
   enum Mode { kUnset, kDevice, kHost };
@@ -217,7 +253,7 @@ private:
   lpmdx_system* sys_ = nullptr;
 };
 
-class DeviceBeeman : public lpmd::Integrator, public lpmd::Plugin {
+class DeviceBeeman : public lpmd::Integrator, public lpmd::Plugin, public DeviceIntegrator {
 public:
   explicit DeviceBeeman(std::string args);
   void ShowHelp() const override;
@@ -229,6 +265,9 @@ public:
   // One Beeman step (twostepintegrator.cc:12-68, beeman.cc:63-84), on the
   // device or, when Initialize ran on the host, on the host.
   void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  double DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm, const std::vector<const DevicePotential*>& dev,
+                   long steps, bool upload) override;
+  bool DeviceReady() const override { return mode_ == kDevice; }
 
 private:
   enum Mode { kUnset, kDevice, kHost };
@@ -239,7 +278,7 @@ private:
   lpmdx_system* sys_ = nullptr;
 };
 
-class DeviceNoseHoover : public lpmd::Integrator, public lpmd::Plugin {
+class DeviceNoseHoover : public lpmd::Integrator, public lpmd::Plugin, public DeviceIntegrator {
 public:
   explicit DeviceNoseHoover(std::string args);
   void ShowHelp() const override;
@@ -252,6 +291,9 @@ public:
   // One Nose-Hoover step (twostepintegrator.cc:12-68, nosehoover.cc:70-96),
   // on the device or, when Initialize ran on the host, on the host.
   void Advance(lpmd::Simulation& sim, lpmd::Potential& p) override;
+  double DeviceRun(lpmd::Simulation& sim, DeviceCellManager* cm, const std::vector<const DevicePotential*>& dev,
+                   long steps, bool upload) override;
+  bool DeviceReady() const override { return mode_ == kDevice; }
 
 private:
   enum Mode { kUnset, kDevice, kHost };
@@ -263,6 +305,103 @@ private:
   std::vector<lpmd::Vector> a_old_, aux_;
   std::vector<double> hx_, hv_, ha_;
   lpmdx_system* sys_ = nullptr;
+};
+
+// lpmd::Simulation whose steps stay on the device: SimulationEngine
+// (liblpmd/simulation/simulationbuilder.cc:23-293) over the reference's own
+// FixedSizeParticleSet and OrthogonalCell, with DoStep / DoSteps split in
+// two paths.
+//  * Device path (the integrator is a DeviceIntegrator ready on the device,
+//    every potential a DevicePotential, the cell manager a
+//    DeviceCellManager): DoSteps(n) runs the n steps in one call of the
+//    engine and sets the step's tags (SetSimulationTags,
+//    simulationbuilder.cc:84-124) and the potential-energy tag, virial and
+//    stress from the engine's device reductions.  Nothing is copied back:
+//    the host atoms are refreshed from the device when they are next read
+//    (OriginalAtoms, and so Atoms()), and a host state that changed since
+//    the device last saw it (positions or velocities written through
+//    Atoms()) is sent before the next device step.  References to atoms
+//    taken before a DoSteps see the host copy of that time; read Atoms()
+//    again after a DoSteps.
+//  * Otherwise each step is SimulationEngine::DoStep on the host (Virial and
+//    stress zeroed, Initialize once, Integrator::Advance, step + 1,
+//    SetSimulationTags).
+// Without a runtime context the session values the tags read (pressfactor,
+// ua3togrcm3, kboltzmann) are those of the default unit system
+// (runtime/units.cc:5-13 as runtime/session.cc:17-38 formats them).
+class DeviceSimulation : public lpmd::Simulation {
+public:
+  DeviceSimulation(long natoms, const lpmd::BasicAtom& at);
+  ~DeviceSimulation() override = default;
+
+  lpmd::BasicParticleSet& OriginalAtoms() override;
+  const lpmd::BasicParticleSet& OriginalAtoms() const override;
+  lpmd::BasicCell& OriginalCell() override { return cell_; }
+  const lpmd::BasicCell& OriginalCell() const override { return cell_; }
+
+  void SetTemperature(double temp, bool tag) override;
+  lpmd::CombinedPotential& Potentials() override { return potarray_; }
+  const lpmd::CombinedPotential& Potentials() const override { return potarray_; }
+  void DoStep() override { DoSteps(1); }
+  void DoSteps(long n) override;
+  long CurrentStep() const override { return step_; }
+  void AdjustCurrentStep(long s) override;
+  void SetIntegrator(lpmd::Integrator& itg) override { integ_ = &itg; }
+  lpmd::Integrator& Integrator() override;
+  void RescalePositions(const lpmd::BasicCell& old_cell) override;
+  void Dump(const std::string& path) const override;
+  void Restore(const std::string& path) override;
+
+  // Counters: host refreshes from the device, device runs (one per DoSteps
+  // on the device path), states sent to the device by a DoSteps, host steps.
+  long HostRefreshes() const { return refreshes_; }
+  long DeviceRuns() const { return runs_; }
+  long Uploads() const { return uploads_; }
+  long HostSteps() const { return host_steps_; }
+
+private:
+  void Initialize();
+  void InitVelocities();
+  void HostStep();
+  void SetSimulationTags();
+  void DeviceTags(lpmdx_system* sys, double epot);
+  void Refresh() const;
+  double Session(const char* key, double fallback) const;
+  bool HostChanged() const;
+
+  lpmd::FixedSizeParticleSet atoms_;
+  lpmd::OrthogonalCell cell_;
+  lpmd::CombinedPotential potarray_;
+  lpmd::Integrator* integ_ = nullptr;
+  bool velocities_set_ = false, initialized_ = false;
+  long step_ = 0, initial_step_ = 0;
+  // device view: the device holds a state newer than atoms_ (of dev_'s system)
+  mutable bool device_ahead_ = false;
+  // atoms_ were handed out writable since host and device last agreed
+  mutable bool handed_out_ = true;
+  // during a device run: OriginalAtoms hands out atoms_ as they are (their
+  // tags, sizes and masses are current; positions are not read)
+  bool raw_view_ = false;
+  mutable DeviceCellManager* dev_ = nullptr;
+  mutable std::vector<double> agreed_x_, agreed_v_; // the state host and device last agreed on
+  mutable long refreshes_ = 0;
+  long runs_ = 0, uploads_ = 0, host_steps_ = 0;
+};
+
+// SimulationBuilder (liblpmd/lpmd/simulationbuilder.h:16-29) for DeviceSimulation:
+// the same creation calls; the builder owns the simulations it makes.
+class DeviceSimulationBuilder {
+public:
+  // SimulationBuilder::CreateFixedOrthogonal (simulationbuilder.cc:321-326)
+  static lpmd::Simulation& CreateFixedOrthogonal(lpmd::RuntimeContext& context, long atoms, const lpmd::BasicAtom& at);
+  // the same without a runtime context (the default unit system's session values)
+  static lpmd::Simulation& CreateFixedOrthogonal(long atoms, const lpmd::BasicAtom& at);
+  // SimulationBuilder::CloneOptimized (simulationbuilder.cc:342-375) for an
+  // orthogonal cell: atoms, their tags, the cell and the step tag copied
+  static lpmd::Simulation& CloneOptimized(lpmd::RuntimeContext& context, const lpmd::Simulation& sim);
+  // Frees a simulation this builder made (and its device memory, through
+  // its cell manager's system when that is released).
+  static void Destroy(lpmd::Simulation& sim);
 };
 
 // Plugin factory behind every <name>.so (create/destroy, liblpmd/lpmd/plugin.h:42-46).

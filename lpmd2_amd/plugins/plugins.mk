@@ -21,7 +21,7 @@ NAMES    := lennardjones fastlj morse buckingham suttonchen finnissinclair gupta
             linkedcell verletlist minimumimage lcbinary velocityverlet leapfrog beeman nosehoover
 RPATH    := -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../..'
 
-all: $(patsubst %,$(OUT)/%.so,$(NAMES))
+all: $(patsubst %,$(OUT)/%.so,$(NAMES)) $(OUT)/libsim_driver.so
 
 $(OUT)/liblpmdx_lpmd.so: lpmd2_amd/plugins/lpmdx_lpmd.cc lpmd2_amd/plugins/lpmdx_lpmd.hpp include/lpmdx.h \
                          lpmd2_amd/csrc/table.hpp \
@@ -33,6 +33,11 @@ $(OUT)/liblpmdx_lpmd.so: lpmd2_amd/plugins/lpmdx_lpmd.cc lpmd2_amd/plugins/lpmdx
 $(OUT)/%.so: lpmd2_amd/plugins/plugin_entry.cc $(OUT)/liblpmdx_lpmd.so
 	$(CXX) $(CXXFLAGS) -DLPMDX_PLUGIN_NAME='"$*"' -shared -o $@ $< -L$(OUT) -l:liblpmdx_lpmd.so \
 	  -L$(CORE) -l$(CORELIB) $(RPATH) -Wl,--no-undefined
+
+# test driver of DeviceSimulation (tests only: C entry points for ctypes)
+$(OUT)/libsim_driver.so: tests/dropin_driver/sim_driver.cc $(OUT)/liblpmdx_lpmd.so
+	$(CXX) $(CXXFLAGS) -shared -o $@ $< -L$(OUT) -l:liblpmdx_lpmd.so -L$(CORE) -l$(CORELIB) $(RPATH) -ldl \
+	  -Wl,--no-undefined
 
 clean:
 	rm -rf $(OUT)

@@ -18,7 +18,7 @@ CORE_SRCS := \
   liblpmd/core/configuration.cc liblpmd/core/matrix.cc liblpmd/core/paramlist.cc \
   liblpmd/core/particleset.cc liblpmd/core/value.cc \
   liblpmd/plugins/module.cc liblpmd/plugins/moduleinfo.cc liblpmd/plugins/plugin.cc \
-  liblpmd/support/error.cc liblpmd/support/util.cc liblpmd/support/elements.cc \
+  liblpmd/support/error.cc liblpmd/support/util.cc liblpmd/support/elements.cc liblpmd/support/colorhandler.cc \
   liblpmd/simulation/potential.cc liblpmd/simulation/pairpotential.cc \
   liblpmd/simulation/metalpotential.cc liblpmd/simulation/cellgenerator.cc \
   liblpmd/simulation/integrator.cc liblpmd/simulation/twostepintegrator.cc \
