@@ -278,14 +278,16 @@ def batched_leg(args):
     atoms = sum(j["x"].shape[0] for j in ens)
     padded = sum(-(-j["x"].shape[0] // 64) * 64 for j in ens)
     log(f"batched: {len(ens)} systems, {atoms} atoms")
-    b = Batch(atom_capacity=padded + 64, list_words=padded * 320, cell_slots=max(4096, padded // 4))
+    b = Batch(atom_capacity=padded + 64, list_words=padded * 320, cell_slots=max(4096, padded))
     t0 = time.perf_counter()
     sids = [b.add(x=j["x"], box=j["box"], z=j["z"], potentials=j["potentials"], cm=j["cm"], v=j["v"], dt=j["dt"],
                   steps=j["steps"]) for j in ens]
     log(f"  added in {time.perf_counter() - t0:.2f} s")
     t0 = time.perf_counter()
-    lanes = b.autotune(trial_steps=5)
-    log(f"  autotune {time.perf_counter() - t0:.2f} s: {lanes}")
+    shape = b.autotune(trial_steps=5, cells=True)
+    lanes = {k: v[0] for k, v in shape.items()}
+    cells = {k: v[1] for k, v in shape.items()}
+    log(f"  autotune {time.perf_counter() - t0:.2f} s: lanes {lanes}, cells per list radius {cells}")
     b.run(warm + 1)  # initial force evaluation + warmup steps
     t0 = time.perf_counter()
     b.run(steps)
@@ -304,6 +306,7 @@ def batched_leg(args):
         "steps": steps,
         "ms_per_step": 1e3 * t_b / steps,
         "lanes": {k: v for k, v in lanes.items() if k in ("lennardjones", "suttonchen")},
+        "cells_per_list_radius": {k: v for k, v in cells.items() if k in ("lennardjones", "suttonchen")},
         "workload": "512 systems, N log-uniform [256, 16384], half LJ Ar rho* 0.8-1.0 (skin 1.0), "
                     "half Sutton-Chen Cu/Ni/Ag rc 2a (skin 0.3), 100-900 K, seed 3",
     }

@@ -345,11 +345,21 @@ LPMDX_API int lpmdx_batch_run(lpmdx_batch* b, long steps);
 LPMDX_API int lpmdx_batch_get(lpmdx_batch* b, long sid, double* x, double* v, double* a, long info[8]);
 /* Potential energy, virial and kinetic energy of one system at its last step. */
 LPMDX_API int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]);
-/* Times trial_steps steps per candidate lane count (1, 2, 4) for each
- * potential kind on a snapshot of the batch and keeps the fastest;
- * lanes_out[kind] for kind 0 .. LPMDX_NKINDS (LPMDX_NKINDS + 1 entries). */
-LPMDX_API int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1]);
+/* No counterpart in the reference (its runs have no launch shape to tune).
+ * Times trial_steps steps for each potential kind on a snapshot of the batch
+ * per candidate pair of lanes per atom (1, 2, 4) of its force launches and
+ * cells per list radius (1, 2) of its systems' cell grids, and keeps the
+ * fastest pair; lanes_out[kind] and cells_out[kind] for kind
+ * 0 .. LPMDX_NKINDS (LPMDX_NKINDS + 1 entries each; either may be NULL).
+ * The snapshot is restored: trajectories are those of the chosen shape. */
+LPMDX_API int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1],
+                                   int cells_out[LPMDX_NKINDS + 1]);
 LPMDX_API int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes);
+/* Cells per list radius (1 or 2, default 1) of the grids of one kind's
+ * systems: 2 applies where five half-radius cells fit along every axis (the
+ * list build then scans the rows of a 5 x 5 x 5 stencil trimmed per atom);
+ * lists are built again at the next step.  No counterpart in the reference. */
+LPMDX_API int lpmdx_batch_set_cells(lpmdx_batch* b, int kind, int per_radius);
 /* [0] live systems, [1] atoms, [2] atom capacity, [3] largest free atom run,
  * [4] largest free list run (words), [5] largest free cell run, [6] steps run,
  * [7] system slots. */

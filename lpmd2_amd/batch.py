@@ -29,8 +29,9 @@ def _bind():
     L.lpmdx_batch_run.argtypes = [vp, l]
     L.lpmdx_batch_get.argtypes = [vp, l, vp, vp, vp, vp]
     L.lpmdx_batch_energies.argtypes = [vp, l, vp]
-    L.lpmdx_batch_autotune.argtypes = [vp, l, vp]
+    L.lpmdx_batch_autotune.argtypes = [vp, l, vp, vp]
     L.lpmdx_batch_set_lanes.argtypes = [vp, i, i]
+    L.lpmdx_batch_set_cells.argtypes = [vp, i, i]
     L.lpmdx_batch_info.argtypes = [vp, vp]
     L.lpmdx_batch_bytes_per_atom.restype = l
     L.lpmdx_batch_progress.argtypes = [vp, l, vp, vp]
@@ -104,13 +105,23 @@ class Batch:
         _check(lib().lpmdx_batch_energies(self._h, sid, _p(out)))
         return float(out[0]), float(out[1]), float(out[2])
 
-    def autotune(self, trial_steps: int = 10):
+    def autotune(self, trial_steps: int = 10, cells: bool = False):
+        """Lanes per atom (and cells per list radius) of each potential kind,
+        timed on a snapshot of the batch: {kind: lanes}, or with cells=True
+        {kind: (lanes, cells per list radius)}."""
         lanes = np.zeros(NKINDS + 1, dtype=np.int32)
-        _check(lib().lpmdx_batch_autotune(self._h, trial_steps, _p(lanes)))
+        div = np.zeros(NKINDS + 1, dtype=np.int32)
+        _check(lib().lpmdx_batch_autotune(self._h, trial_steps, _p(lanes), _p(div)))
+        if cells:
+            return {name: (int(lanes[k]), int(div[k])) for name, k in KINDS.items()}
         return {name: int(lanes[k]) for name, k in KINDS.items()}
 
     def set_lanes(self, kind: str, lanes: int):
         _check(lib().lpmdx_batch_set_lanes(self._h, KINDS[kind], lanes))
+
+    def set_cells(self, kind: str, per_radius: int):
+        """Cells per list radius (1 or 2) of the grids of one kind's systems."""
+        _check(lib().lpmdx_batch_set_cells(self._h, KINDS[kind], per_radius))
 
     def progress(self, nslots: int):
         """(left, flags) per slot: graph steps still to run; bit 0 live, bit 1

@@ -68,12 +68,70 @@ struct Arena {
 struct HostSys {
   bool live = false;
   long n = 0, tiles = 0, atom0 = 0, list0 = 0, listw = 0, cell0 = 0, cells = 0;
+  double rl = 0.0; // list radius (cutoff + skin)
   std::vector<std::unique_ptr<PotParams>> pots;
   int kind = 0;
 };
 
 constexpr long kBytesPerAtom = 264; // per atom slot over all per-atom arenas
 constexpr int kMaxCycle = 16;       // longest cycle of one full and cyc - 1 lite steps
+
+// Cells per list radius of a system's exact grid for div (1 or 2): 2 only
+// where five half-radius cells fit along every axis within the sort's cell
+// limit and the FP32 one-shift-per-cell path applies.
+int grid_div(const double box[3], double rl, int div) {
+  if (div != 2)
+    return 1;
+  const double lmax = std::max(box[0], std::max(box[1], box[2]));
+  const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
+  const double hmin = 0.5 * std::min(box[0], std::min(box[1], box[2]));
+  if (!(hmin > rl + 4.0 * pos_err + 1e-9 * hmin))
+    return 1;
+  long cells = 1;
+  for (int q = 0; q < 3; ++q) {
+    const double m = std::floor(box[q] / (0.5 * rl * (1.0 + 1e-9)));
+    if (m < 5.0 || m > 512.0)
+      return 1;
+    cells *= (long)m;
+  }
+  return cells <= kMaxSortCells ? 2 : 1;
+}
+
+// The exact grid of system S (box S.b, list radius rl) with cells of at
+// least rl / div per axis (div as grid_div resolves it), its sort passes and
+// build path.
+void set_grid(BSys& S, double rl, int div) {
+  const double box[3] = {S.b.L[0], S.b.L[1], S.b.L[2]};
+  div = grid_div(box, rl, div);
+  int nq[3];
+  for (int q = 0; q < 3; ++q)
+    nq[q] = (int)std::max(1.0, std::min(std::floor(box[q] / (rl / div * (1.0 + 1e-9))), 512.0));
+  // at most kMaxSortCells cells (two digit passes of the sort): coarser
+  // cells only add candidates, the list is the same
+  while ((long)nq[0] * nq[1] * nq[2] > kMaxSortCells) {
+    const int q = (nq[0] >= nq[1] && nq[0] >= nq[2]) ? 0 : (nq[1] >= nq[2] ? 1 : 2);
+    nq[q] -= 1;
+  }
+  S.g.nx = nq[0];
+  S.g.ny = nq[1];
+  S.g.nz = nq[2];
+  S.g.side = 1;
+  S.g.mode = kExact;
+  S.g.div = div;
+  S.g.ncell = nq[0] * nq[1] * nq[2];
+  {
+    int bits = 0;
+    while (bits < 31 && ((uint32_t)(S.g.ncell - 1) >> bits))
+      ++bits;
+    S.passes = std::max(1, (bits + kRadixBits - 1) / kRadixBits);
+  }
+  const double lmax = std::max(box[0], std::max(box[1], box[2]));
+  const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
+  const double hmin = std::min(S.b.half[0], std::min(S.b.half[1], S.b.half[2]));
+  S.path = kPathExact64;
+  if (hmin > rl + 4.0 * pos_err + 1e-9 * hmin)
+    S.path = (nq[0] >= 3 && nq[1] >= 3 && nq[2] >= 3) ? kPathShift32 : kPathMin32;
+}
 
 } // namespace
 
@@ -114,6 +172,10 @@ struct lpmdx_batch {
   // (served Gupta / Buckingham / Finnis-Sinclair jobs measured 18 / 31 / 3 %
   // faster than at one lane; the tile kernels keep few atoms per workgroup)
   int lpa_of_kind[kNumKinds + 1] = {0, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+  // cells per list radius of each kind's grids until autotuned or set: 1
+  // (measured on the served Gupta / Finnis-Sinclair / Buckingham mix at
+  // full load: 2 was 2 % faster for Gupta, 11 % and 3 % slower for the others)
+  int div_of_kind[kNumKinds + 1] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
   // graphs of one full step and of one cycle (a full step and cyc - 1 lite steps)
   hipGraph_t graph = nullptr, cgraph = nullptr;
   hipGraphExec_t gexec = nullptr, cexec = nullptr;
@@ -272,33 +334,18 @@ struct lpmdx_batch {
       S.Lf[q] = (float)box[q];
       S.halff[q] = (float)S.b.half[q];
     }
-    int nq[3];
-    for (int q = 0; q < 3; ++q)
-      nq[q] = (int)std::max(1.0, std::min(std::floor(box[q] / (rl * (1.0 + 1e-9))), 512.0));
-    // at most kMaxSortCells cells (two digit passes of the sort): coarser
-    // cells only add candidates, the list is the same
-    while ((long)nq[0] * nq[1] * nq[2] > kMaxSortCells) {
-      const int q = (nq[0] >= nq[1] && nq[0] >= nq[2]) ? 0 : (nq[1] >= nq[2] ? 1 : 2);
-      nq[q] -= 1;
-    }
-    S.g.nx = nq[0];
-    S.g.ny = nq[1];
-    S.g.nz = nq[2];
-    S.g.side = 1;
-    S.g.mode = kExact;
-    S.g.ncell = nq[0] * nq[1] * nq[2];
-    {
-      int bits = 0;
-      while (bits < 31 && ((uint32_t)(S.g.ncell - 1) >> bits))
-        ++bits;
-      S.passes = std::max(1, (bits + kRadixBits - 1) / kRadixBits);
-    }
+    // the cell slice holds either grid (the finer one is the larger) when the
+    // arena has room, else the whole-radius grid only
+    set_grid(S, rl, 1);
+    const long ncell1 = S.g.ncell;
+    set_grid(S, rl, 2);
+    long ncell_max = std::max(ncell1, (long)S.g.ncell);
+    if (cells.largest() < ncell_max + 2)
+      ncell_max = ncell1;
+    set_grid(S, rl, ncell_max >= S.g.ncell ? div_of_kind[H.kind] : 1);
     const double lmax = std::max(box[0], std::max(box[1], box[2]));
     const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
     const double hmin = std::min(S.b.half[0], std::min(S.b.half[1], S.b.half[2]));
-    S.path = kPathExact64;
-    if (hmin > rl + 4.0 * pos_err + 1e-9 * hmin)
-      S.path = (nq[0] >= 3 && nq[1] >= 3 && nq[2] >= 3) ? kPathShift32 : kPathMin32;
     S.rl2 = rl * rl;
     S.rl2f = (float)((rl + 2.0 * pos_err) * (rl + 2.0 * pos_err) * (1.0 + 1e-6));
     S.budget = 0.5 * skin;
@@ -347,7 +394,7 @@ struct lpmdx_batch {
       atoms.release(t0, tiles);
       fail(LPMDX_ENOMEM, "list arena full");
     }
-    const long c0 = cells.alloc(S.g.ncell + 2);
+    const long c0 = cells.alloc(ncell_max + 2);
     if (c0 < 0) {
       atoms.release(t0, tiles);
       lists.release(l0, (lw + kTile - 1) / kTile);
@@ -368,7 +415,8 @@ struct lpmdx_batch {
     H.list0 = l0;
     H.listw = (lw + kTile - 1) / kTile;
     H.cell0 = c0;
-    H.cells = S.g.ncell + 2;
+    H.cells = ncell_max + 2;
+    H.rl = rl;
     // upload the atoms
     std::vector<double> ff(n), ms(n);
     std::vector<int> pm(n), zz2(n);
@@ -469,6 +517,30 @@ struct lpmdx_batch {
     sync();
     release_graph();
     // system table and sort chunks
+    // systems whose kind changed its cells per list radius get the new grid
+    // and build their lists again at their next step
+    std::vector<size_t> regrid;
+    for (size_t s = 0; s < hsys.size(); ++s) {
+      if (!meta[s].live)
+        continue;
+      BSys& S = hsys[s];
+      const double box[3] = {S.b.L[0], S.b.L[1], S.b.L[2]};
+      if (grid_div(box, meta[s].rl, div_of_kind[S.kind]) != S.g.div) {
+        BSys T = S;
+        set_grid(T, meta[s].rl, div_of_kind[S.kind]);
+        if (T.g.ncell + 2 > meta[s].cells)
+          continue; // its cell slice holds the whole-radius grid only
+        S = T;
+        regrid.push_back(s);
+      }
+    }
+    if (!regrid.empty()) {
+      pull_state();
+      for (size_t s : regrid)
+        if (hst[s].init != 1)
+          hst[s].built = 0;
+      hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+    }
     std::vector<int2> ch;
     passes = 1;
     for (size_t s = 0; s < hsys.size(); ++s) {
@@ -850,10 +922,13 @@ struct lpmdx_batch {
   }
 
   // ---------------------------------------------------------------- autotuner
-  // Lanes per atom of each potential kind's force launch, chosen by timing a
-  // few steps of this batch per candidate on a snapshot of its state; the
-  // snapshot is restored afterwards, so tuning leaves the trajectories alone.
-  void autotune(long trial_steps, int out_lpa[kNumKinds + 1]) {
+  // Per potential kind, the lanes per atom of its force launches (1, 2, 4)
+  // and the cells per list radius of its grids (1, 2: how many candidates a
+  // list build visits against how many cell ranges it reads), chosen by
+  // timing a few steps of this batch per candidate pair on a snapshot of its
+  // state; the snapshot is restored afterwards, so tuning leaves the
+  // trajectories alone.
+  void autotune(long trial_steps, int out_lpa[kNumKinds + 1], int out_div[kNumKinds + 1]) {
     if (dirty)
       plan();
     pull_state();
@@ -903,26 +978,31 @@ struct lpmdx_batch {
       kinds.push_back(G->kind);
     for (int kind : kinds) {
       double best = 1e300;
-      int best_lpa = lpa_of_kind[kind];
-      for (int cand : {1, 2, 4}) {
-        lpa_of_kind[kind] = cand;
-        plan();
-        reset();
-        // warm up (includes the build), then time
-        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
-        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
-        sync();
-        auto t0 = std::chrono::steady_clock::now();
-        for (long k = 0; k < trial_steps; ++k)
+      int best_lpa = lpa_of_kind[kind], best_div = div_of_kind[kind];
+      for (int dv : {1, 2})
+        for (int cand : {1, 2, 4}) {
+          lpa_of_kind[kind] = cand;
+          div_of_kind[kind] = dv;
+          plan();
+          reset();
+          // warm up (includes the build), then time full steps, each with
+          // the rebuild chain a build would run
           hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
-        sync();
-        double dt_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (dt_s < best) {
-          best = dt_s;
-          best_lpa = cand;
+          hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+          sync();
+          auto t0 = std::chrono::steady_clock::now();
+          for (long k = 0; k < trial_steps; ++k)
+            hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+          sync();
+          double dt_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+          if (dt_s < best) {
+            best = dt_s;
+            best_lpa = cand;
+            best_div = dv;
+          }
         }
-      }
       lpa_of_kind[kind] = best_lpa;
+      div_of_kind[kind] = best_div;
     }
     plan();
     copy(false);
@@ -932,9 +1012,12 @@ struct lpmdx_batch {
     }
     hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
     sync();
-    if (out_lpa)
-      for (int q = 0; q <= kNumKinds; ++q)
+    for (int q = 0; q <= kNumKinds; ++q) {
+      if (out_lpa)
         out_lpa[q] = lpa_of_kind[q];
+      if (out_div)
+        out_div[q] = div_of_kind[q];
+    }
   }
 
   static void need(const void* p, const char* what) {
@@ -1018,13 +1101,25 @@ int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]) {
   });
 }
 
-int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1]) {
+int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1],
+                         int cells_out[LPMDX_NKINDS + 1]) {
   return guard([&] {
     if (!b)
       fail(LPMDX_EINVAL, "batch is NULL");
     if (trial_steps <= 0)
       fail(LPMDX_EINVAL, "trial steps must be positive");
-    b->autotune(trial_steps, lanes_out);
+    b->autotune(trial_steps, lanes_out, cells_out);
+  });
+}
+
+int lpmdx_batch_set_cells(lpmdx_batch* b, int kind, int per_radius) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    if (kind < 1 || kind > kNumKinds || !(per_radius == 1 || per_radius == 2))
+      fail(LPMDX_EINVAL, "kind 1.." + std::to_string(kNumKinds) + " and 1 or 2 cells per list radius");
+    b->div_of_kind[kind] = per_radius;
+    b->dirty = true; // the next plan gives the kind's systems their grids and new lists
   });
 }
 

@@ -33,6 +33,9 @@ struct Grid {
              // key == ncell marks an atom outside every cell (linkedcell)
   int brick; // cells per brick edge on a bricked exact grid, 0 otherwise
   int nbx, nby, nbz; // bricks per axis (bricked grids)
+  int div;   // exact grids: cells per list radius along each axis (0 or 1: cells
+             // of at least the radius, 27-cell stencil; 2: at least half of it,
+             // rows of a 5 x 5 x 5 stencil trimmed per atom; unbricked only)
 };
 
 // Cell id of cell (i, j, k) (0 <= i < nx, ...).  The plain order is x-fastest.
@@ -475,6 +478,80 @@ __device__ __forceinline__ void scan_cell(const BuildArgs& A, int p, long i, con
   }
 }
 
+// Distance along one axis from a coordinate u inside its cell [0, h) to the
+// cell d cells away (0 for its own cell).
+__device__ __forceinline__ float cell_gap(int d, float u, float h) {
+  return d > 0 ? (float)(d - 1) * h + (h - u) : (d < 0 ? (float)(-d - 1) * h + u : 0.0f);
+}
+
+// The stencil of a grid of half-radius cells (g.div == 2, kPathShift32): the
+// 25 rows (dz, dy) of five cells along x around atom i's cell, each trimmed
+// to the cells within reach of the atom (its offset inside its cell fixes
+// how far the row's y and z gaps leave to go along x), and each scanned as
+// one contiguous range of sorted atoms (cells are numbered x-fastest), split
+// where the row wraps around the x faces.  The reach is the admission
+// radius widened by a relative 1e-5 and 1e-5 of a cell edge, so the FP32
+// trimming never drops a cell the admission test could take.  Entries come
+// row by row, in sorted order along a row.
+__device__ __forceinline__ void scan_rows(const BuildArgs& A, long i, int ci, int cj, int ck, float4 fi, Pending& P) {
+  const Grid& g = A.g;
+  const int ii = (int)i;
+  const float hx = A.Lf[0] / (float)g.nx, hy = A.Lf[1] / (float)g.ny, hz = A.Lf[2] / (float)g.nz;
+  const float reach = sqrtf(A.rl2f) * (1.0f + 1e-5f) + 1e-5f * fmaxf(hx, fmaxf(hy, hz));
+  const float reach2 = reach * reach;
+  // the atom's offset inside its cell, held inside [0, h]
+  const float ux = fminf(fmaxf(fi.x - (float)ci * hx, 0.0f), hx);
+  const float uy = fminf(fmaxf(fi.y - (float)cj * hy, 0.0f), hy);
+  const float uz = fminf(fmaxf(fi.z - (float)ck * hz, 0.0f), hz);
+  for (int dz = -2; dz <= 2; ++dz) {
+    const float gz = cell_gap(dz, uz, hz);
+    const float rz2 = reach2 - gz * gz;
+    if (!(rz2 > 0.0f))
+      continue;
+    const int rz = ck + dz;
+    const float shz = rz < 0 ? -A.Lf[2] : (rz >= g.nz ? A.Lf[2] : 0.0f);
+    const int z2 = pmod(rz, g.nz);
+    for (int dy = -2; dy <= 2; ++dy) {
+      const float gy = cell_gap(dy, uy, hy);
+      const float ry2 = rz2 - gy * gy;
+      if (!(ry2 > 0.0f))
+        continue;
+      const float rx = sqrtf(ry2);
+      const int lo = cell_gap(-2, ux, hx) < rx ? -2 : (cell_gap(-1, ux, hx) < rx ? -1 : 0);
+      const int hi = cell_gap(2, ux, hx) < rx ? 2 : (cell_gap(1, ux, hx) < rx ? 1 : 0);
+      const int ry = cj + dy;
+      const float shy = ry < 0 ? -A.Lf[1] : (ry >= g.ny ? A.Lf[1] : 0.0f);
+      const int z1 = pmod(ry, g.ny);
+      const int row = (z2 * g.ny + z1) * g.nx;
+      // up to three runs: cells past the low face, inside, past the high face
+      for (int part = 0; part < 3; ++part) {
+        int x0 = ci + lo, x1 = ci + hi, wrap = 0;
+        float shx = 0.0f;
+        if (part == 0) {
+          x1 = min(x1, -1);
+          wrap = g.nx;
+          shx = -A.Lf[0];
+        } else if (part == 1) {
+          x0 = max(x0, 0);
+          x1 = min(x1, g.nx - 1);
+        } else {
+          x0 = max(x0, g.nx);
+          wrap = -g.nx;
+          shx = A.Lf[0];
+        }
+        if (x0 > x1)
+          continue;
+        const int b = A.cb[row + x0 + wrap], e = A.cb[row + x1 + wrap + 1];
+        const float sx = fi.x - shx, sy = fi.y - shy, sz = fi.z - shz;
+        if (b <= ii && ii < e)
+          scan_cell_f32<true>(A, b, e, ii, sx, sy, sz, P, 0, false);
+        else
+          scan_cell_f32<false>(A, b, e, ii, sx, sy, sz, P, 0, false);
+      }
+    }
+  }
+}
+
 // Where atom i's neighbours go in the list: on a bricked grid (A.bhalo) the
 // halo slot of its brick, slot = sorted index + shift of the neighbour cell,
 // else the sorted index itself.
@@ -528,6 +605,8 @@ __device__ __forceinline__ void build_atom(const BuildArgs& A, long i, bool acti
             }
           }
         }
+      } else if (A.path == kPathShift32 && g.div == 2) {
+        scan_rows(A, i, ci, cj, ck, fi, P);
       } else if (A.path == kPathShift32) {
         for (int dz = -1; dz <= 1; ++dz) {
           const int rz = ck + dz;
