@@ -287,7 +287,9 @@ def batched_leg(args):
     shape = b.autotune(trial_steps=5, cells=True)
     lanes = {k: v[0] for k, v in shape.items()}
     cells = {k: v[1] for k, v in shape.items()}
-    log(f"  autotune {time.perf_counter() - t0:.2f} s: lanes {lanes}, cells per list radius {cells}")
+    tiles = {k: v[2] for k, v in shape.items()}
+    log(f"  autotune {time.perf_counter() - t0:.2f} s: lanes {lanes}, cells per list radius {cells}, "
+        f"tiles per workgroup {tiles}")
     b.run(warm + 1)  # initial force evaluation + warmup steps
     t0 = time.perf_counter()
     b.run(steps)
@@ -307,6 +309,7 @@ def batched_leg(args):
         "ms_per_step": 1e3 * t_b / steps,
         "lanes": {k: v for k, v in lanes.items() if k in ("lennardjones", "suttonchen")},
         "cells_per_list_radius": {k: v for k, v in cells.items() if k in ("lennardjones", "suttonchen")},
+        "tiles_per_workgroup": {k: v for k, v in tiles.items() if k in ("lennardjones", "suttonchen")},
         "workload": "512 systems, N log-uniform [256, 16384], half LJ Ar rho* 0.8-1.0 (skin 1.0), "
                     "half Sutton-Chen Cu/Ni/Ag rc 2a (skin 0.3), 100-900 K, seed 3",
     }

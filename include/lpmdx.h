@@ -348,12 +348,17 @@ LPMDX_API int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]);
 /* No counterpart in the reference (its runs have no launch shape to tune).
  * Times trial_steps steps for each potential kind on a snapshot of the batch
  * per candidate pair of lanes per atom (1, 2, 4) of its force launches and
- * cells per list radius (1, 2) of its systems' cell grids, and keeps the
- * fastest pair; lanes_out[kind] and cells_out[kind] for kind
- * 0 .. LPMDX_NKINDS (LPMDX_NKINDS + 1 entries each; either may be NULL).
- * The snapshot is restored: trajectories are those of the chosen shape. */
+ * cells per list radius (1, 2) of its systems' cell grids and keeps the
+ * fastest pair, then the same for the tiles of 64 atoms per workgroup (1, 2,
+ * 4) of its force and rho launches; lanes_out[kind], cells_out[kind] and
+ * tiles_out[kind] for kind 0 .. LPMDX_NKINDS (LPMDX_NKINDS + 1 entries each;
+ * any may be NULL).  The snapshot is restored: trajectories are those of the
+ * chosen shape. */
 LPMDX_API int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1],
-                                   int cells_out[LPMDX_NKINDS + 1]);
+                                   int cells_out[LPMDX_NKINDS + 1], int tiles_out[LPMDX_NKINDS + 1]);
+/* Tiles of 64 atoms per workgroup (1, 2 or 4, default 1) of one kind's force
+ * and rho launches.  No counterpart in the reference. */
+LPMDX_API int lpmdx_batch_set_tiles(lpmdx_batch* b, int kind, int tiles_per_group);
 LPMDX_API int lpmdx_batch_set_lanes(lpmdx_batch* b, int kind, int lanes);
 /* Cells per list radius (1 or 2, default 1) of the grids of one kind's
  * systems: 2 applies where five half-radius cells fit along every axis (the

@@ -29,7 +29,8 @@ def _bind():
     L.lpmdx_batch_run.argtypes = [vp, l]
     L.lpmdx_batch_get.argtypes = [vp, l, vp, vp, vp, vp]
     L.lpmdx_batch_energies.argtypes = [vp, l, vp]
-    L.lpmdx_batch_autotune.argtypes = [vp, l, vp, vp]
+    L.lpmdx_batch_autotune.argtypes = [vp, l, vp, vp, vp]
+    L.lpmdx_batch_set_tiles.argtypes = [vp, i, i]
     L.lpmdx_batch_set_lanes.argtypes = [vp, i, i]
     L.lpmdx_batch_set_cells.argtypes = [vp, i, i]
     L.lpmdx_batch_info.argtypes = [vp, vp]
@@ -106,15 +107,20 @@ class Batch:
         return float(out[0]), float(out[1]), float(out[2])
 
     def autotune(self, trial_steps: int = 10, cells: bool = False):
-        """Lanes per atom (and cells per list radius) of each potential kind,
-        timed on a snapshot of the batch: {kind: lanes}, or with cells=True
-        {kind: (lanes, cells per list radius)}."""
+        """Lanes per atom, cells per list radius and tiles per workgroup of each
+        potential kind, timed on a snapshot of the batch: {kind: lanes}, or
+        with cells=True {kind: (lanes, cells per list radius, tiles per workgroup)}."""
         lanes = np.zeros(NKINDS + 1, dtype=np.int32)
         div = np.zeros(NKINDS + 1, dtype=np.int32)
-        _check(lib().lpmdx_batch_autotune(self._h, trial_steps, _p(lanes), _p(div)))
+        tpw = np.zeros(NKINDS + 1, dtype=np.int32)
+        _check(lib().lpmdx_batch_autotune(self._h, trial_steps, _p(lanes), _p(div), _p(tpw)))
         if cells:
-            return {name: (int(lanes[k]), int(div[k])) for name, k in KINDS.items()}
+            return {name: (int(lanes[k]), int(div[k]), int(tpw[k])) for name, k in KINDS.items()}
         return {name: int(lanes[k]) for name, k in KINDS.items()}
+
+    def set_tiles(self, kind: str, tiles_per_group: int):
+        """Tiles of 64 atoms per workgroup (1, 2 or 4) of one kind's force and rho launches."""
+        _check(lib().lpmdx_batch_set_tiles(self._h, KINDS[kind], tiles_per_group))
 
     def set_lanes(self, kind: str, lanes: int):
         _check(lib().lpmdx_batch_set_lanes(self._h, KINDS[kind], lanes))

@@ -162,7 +162,7 @@ struct lpmdx_batch {
   DBuf<uint32_t> hist;
   int nchunks = 0, passes = 1;
   struct Group {
-    int kind, lpa;
+    int kind, lpa, tpw;
     bool metal;
     DBuf<int> tiles;
     long n = 0;
@@ -176,6 +176,9 @@ struct lpmdx_batch {
   // (measured on the served Gupta / Finnis-Sinclair / Buckingham mix at
   // full load: 2 was 2 % faster for Gupta, 11 % and 3 % slower for the others)
   int div_of_kind[kNumKinds + 1] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  // tiles (of 64 atoms) per workgroup of each kind's force and rho launches
+  // until autotuned or set: 1
+  int tpw_of_kind[kNumKinds + 1] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
   // graphs of one full step and of one cycle (a full step and cyc - 1 lite steps)
   hipGraph_t graph = nullptr, cgraph = nullptr;
   hipGraphExec_t gexec = nullptr, cexec = nullptr;
@@ -585,6 +588,7 @@ struct lpmdx_batch {
       auto G = std::make_unique<Group>();
       G->kind = kv.first;
       G->lpa = lpa_of_kind[kv.first];
+      G->tpw = tpw_of_kind[kv.first];
       G->metal = metal_kind(kv.first);
       G->n = (long)kv.second.size();
       G->tiles.alloc(kv.second.size());
@@ -603,21 +607,23 @@ struct lpmdx_batch {
 
   template <int KIND> void launch_group(Group& G, hipStream_t s) {
     BArgs B = args(G.tiles.p, G.n);
+    const int T = G.tpw;
+    const long nb = (G.n + T - 1) / T;
     switch (G.lpa) {
     case 1:
       if (G.metal)
-        k_b_rho<KIND, 1><<<G.n, kWave, 0, s>>>(B);
-      k_b_force<KIND, 1><<<G.n, kWave, 0, s>>>(B);
+        k_b_rho<KIND, 1><<<nb, T * kWave, 0, s>>>(B);
+      k_b_force<KIND, 1><<<nb, T * kWave, 0, s>>>(B);
       break;
     case 2:
       if (G.metal)
-        k_b_rho<KIND, 2><<<G.n, 2 * kWave, 0, s>>>(B);
-      k_b_force<KIND, 2><<<G.n, 2 * kWave, 0, s>>>(B);
+        k_b_rho<KIND, 2><<<nb, T * 2 * kWave, 0, s>>>(B);
+      k_b_force<KIND, 2><<<nb, T * 2 * kWave, 0, s>>>(B);
       break;
     default:
       if (G.metal)
-        k_b_rho<KIND, 4><<<G.n, 4 * kWave, 0, s>>>(B);
-      k_b_force<KIND, 4><<<G.n, 4 * kWave, 0, s>>>(B);
+        k_b_rho<KIND, 4><<<nb, T * 4 * kWave, 0, s>>>(B);
+      k_b_force<KIND, 4><<<nb, T * 4 * kWave, 0, s>>>(B);
       break;
     }
   }
@@ -925,10 +931,12 @@ struct lpmdx_batch {
   // Per potential kind, the lanes per atom of its force launches (1, 2, 4)
   // and the cells per list radius of its grids (1, 2: how many candidates a
   // list build visits against how many cell ranges it reads), chosen by
-  // timing a few steps of this batch per candidate pair on a snapshot of its
-  // state; the snapshot is restored afterwards, so tuning leaves the
+  // timing a few steps of this batch per candidate pair, then the tiles per
+  // workgroup of its force and rho launches (1, 2, 4) with that pair; on a
+  // snapshot of the batch's state, restored afterwards, so tuning leaves the
   // trajectories alone.
-  void autotune(long trial_steps, int out_lpa[kNumKinds + 1], int out_div[kNumKinds + 1]) {
+  void autotune(long trial_steps, int out_lpa[kNumKinds + 1], int out_div[kNumKinds + 1],
+                int out_tpw[kNumKinds + 1]) {
     if (dirty)
       plan();
     pull_state();
@@ -1003,6 +1011,28 @@ struct lpmdx_batch {
         }
       lpa_of_kind[kind] = best_lpa;
       div_of_kind[kind] = best_div;
+      // then the tile shape with those lanes and cells
+      int best_tpw = tpw_of_kind[kind];
+      for (int T : {1, 2, 4}) {
+        if (T == best_tpw)
+          continue; // timed above
+        tpw_of_kind[kind] = T;
+        plan();
+        reset();
+        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+        hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+        sync();
+        auto t0 = std::chrono::steady_clock::now();
+        for (long k = 0; k < trial_steps; ++k)
+          hipchk(hipGraphLaunch(gexec, st), "hipGraphLaunch");
+        sync();
+        double dt_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt_s < best) {
+          best = dt_s;
+          best_tpw = T;
+        }
+      }
+      tpw_of_kind[kind] = best_tpw;
     }
     plan();
     copy(false);
@@ -1017,6 +1047,8 @@ struct lpmdx_batch {
         out_lpa[q] = lpa_of_kind[q];
       if (out_div)
         out_div[q] = div_of_kind[q];
+      if (out_tpw)
+        out_tpw[q] = tpw_of_kind[q];
     }
   }
 
@@ -1102,13 +1134,24 @@ int lpmdx_batch_energies(lpmdx_batch* b, long sid, double out[3]) {
 }
 
 int lpmdx_batch_autotune(lpmdx_batch* b, long trial_steps, int lanes_out[LPMDX_NKINDS + 1],
-                         int cells_out[LPMDX_NKINDS + 1]) {
+                         int cells_out[LPMDX_NKINDS + 1], int tiles_out[LPMDX_NKINDS + 1]) {
   return guard([&] {
     if (!b)
       fail(LPMDX_EINVAL, "batch is NULL");
     if (trial_steps <= 0)
       fail(LPMDX_EINVAL, "trial steps must be positive");
-    b->autotune(trial_steps, lanes_out, cells_out);
+    b->autotune(trial_steps, lanes_out, cells_out, tiles_out);
+  });
+}
+
+int lpmdx_batch_set_tiles(lpmdx_batch* b, int kind, int tiles_per_group) {
+  return guard([&] {
+    if (!b)
+      fail(LPMDX_EINVAL, "batch is NULL");
+    if (kind < 1 || kind > kNumKinds || !(tiles_per_group == 1 || tiles_per_group == 2 || tiles_per_group == 4))
+      fail(LPMDX_EINVAL, "kind 1.." + std::to_string(kNumKinds) + " and 1, 2 or 4 tiles per workgroup");
+    b->tpw_of_kind[kind] = tiles_per_group;
+    b->dirty = true; // launch shapes only: the lists stay
   });
 }
 

@@ -509,37 +509,52 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
 }
 
 // ---------------------------------------------------------------- forces
-// One workgroup of 64 * LPA threads per tile of the launch's tile list.  The
-// EAM passes load each list batch when it starts (no prefetch): measured
-// faster for these one-tile workgroups.
+// Workgroups of T tiles (T = blockDim.x / (64 * LPA): the tile shape the
+// autotuner picks, 64, 128 or 256 atoms a workgroup), 64 * LPA threads a
+// tile of the launch's tile list; every wavefront lies inside one tile, so
+// per-system branches are uniform over a wavefront.  The EAM passes load
+// each list batch when it starts (no prefetch): measured faster for these
+// small workgroups.
+constexpr int kMaxTilesPerGroup = 4;
+
+struct TileLane {
+  long tl; // entry of the launch's tile list
+  int r;   // thread within the tile
+};
+
+template <int LPA> __device__ __forceinline__ TileLane tile_lane() {
+  const int per = kWave * LPA;
+  return TileLane{xcd_block() * (long)(blockDim.x / per) + threadIdx.x / per, (int)(threadIdx.x % per)};
+}
+
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kWave * LPA) void k_b_rho(BArgs B) {
-  const long tl = xcd_block();
-  if (tl >= B.ntiles)
-    return;
-  const int tile = B.tiles[tl];
+static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_rho(BArgs B) {
+  const TileLane tw = tile_lane<LPA>();
+  if (tw.tl >= B.ntiles)
+    return; // uniform over the wavefront
+  const int tile = B.tiles[tw.tl];
   const int s = B.tsys[tile];
   if (!B.st[s].act)
-    return; // uniform over the workgroup
+    return; // uniform over the wavefront
   const ForceArgs A = force_view(B, s, nullptr);
-  const long i = (long)tile * kTile - B.sys[s].a0 + threadIdx.x / LPA;
-  const int l = (int)(threadIdx.x % LPA);
+  const long i = (long)tile * kTile - B.sys[s].a0 + tw.r / LPA;
+  const int l = tw.r % LPA;
   double emb = 0.0;
   eam_rho_atom<KIND, LPA, false, false, false>(A, Stage{}, i, l, i < A.n, B.R.w + B.sys[s].a0, nullptr, emb);
 }
 
 template <int KIND, int LPA>
-static __global__ __launch_bounds__(kWave * LPA) void k_b_force(BArgs B) {
-  const long tl = xcd_block();
-  if (tl >= B.ntiles)
-    return;
-  const int tile = B.tiles[tl];
+static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_force(BArgs B) {
+  const TileLane tw = tile_lane<LPA>();
+  if (tw.tl >= B.ntiles)
+    return; // uniform over the wavefront
+  const int tile = B.tiles[tw.tl];
   const int s = B.tsys[tile];
   if (!B.st[s].act)
-    return; // uniform over the workgroup
+    return; // uniform over the wavefront
   const ForceArgs A = force_view(B, s, nullptr);
-  const long i = (long)tile * kTile - B.sys[s].a0 + threadIdx.x / LPA;
-  const int l = (int)(threadIdx.x % LPA);
+  const long i = (long)tile * kTile - B.sys[s].a0 + tw.r / LPA;
+  const int l = tw.r % LPA;
   Tally tal;
   // species are read only by the systems whose potentials need them (the
   // branch is uniform: one workgroup, one system)

@@ -78,7 +78,7 @@ def test_batch_half_radius_cells():
 
 
 def test_autotune_chooses_cells_and_keeps_state():
-    say("autotune: lanes and cells per list radius")
+    say("autotune: lanes, cells per list radius, tiles per workgroup")
     specs = [batch_system(c[0], c[1], c[3], c[4], c[5], c[6], c[7], 60 + k, 600.0) for k, c in enumerate(CASES)]
     b = Batch(atom_capacity=8192)
     sids = [b.add(steps=30, **s) for s in specs]
@@ -91,8 +91,8 @@ def test_autotune_chooses_cells_and_keeps_state():
         np.testing.assert_array_equal(v0, v1)
     say(f"  {shape}")
     for c in CASES:
-        lanes, cells = shape[c[0]]
-        assert lanes in (1, 2, 4) and cells in (1, 2)
+        lanes, cells, tiles = shape[c[0]]
+        assert lanes in (1, 2, 4) and cells in (1, 2) and tiles in (1, 2, 4)
     b.run(40)
     for s in sids:
         assert b.get(s)[3]["done"] == 30
