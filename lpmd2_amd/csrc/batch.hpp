@@ -314,8 +314,8 @@ static __global__ __launch_bounds__(kBlock) void k_b_gather(BArgs B) {
   const long tl = t / kTile;
   if (tl >= B.ntiles)
     return;
-  const int tile = B.tiles[tl];
-  const int s = B.tsys[tile];
+  const int tile = wave_uniform(B.tiles[tl]); // one tile per wavefront
+  const int s = wave_uniform(B.tsys[tile]);
   const BSys& S = B.sys[s];
   const BState& st = B.st[s];
   if (!(st.act && st.flag))
@@ -362,8 +362,8 @@ static __global__ __launch_bounds__(kBlock) void k_b_commit(BArgs B) {
   const long tl = t / kTile;
   if (tl >= B.ntiles)
     return;
-  const int tile = B.tiles[tl];
-  const int s = B.tsys[tile];
+  const int tile = wave_uniform(B.tiles[tl]); // one tile per wavefront
+  const int s = wave_uniform(B.tsys[tile]);
   const BSys& S = B.sys[s];
   const BState& st = B.st[s];
   if (!(st.act && st.flag))
@@ -426,8 +426,8 @@ template <int LPA> __device__ __forceinline__ TileSlot tile_slot(const BArgs& B)
   TileSlot r{0, 0, 0, false};
   if (tl >= B.ntiles)
     return r;
-  const int tile = B.tiles[tl];
-  const int s = B.tsys[tile];
+  const int tile = wave_uniform(B.tiles[tl]); // one tile per wavefront
+  const int s = wave_uniform(B.tsys[tile]);
   const long r0 = t % per;
   r.s = s;
   r.i = (long)tile * kTile - B.sys[s].a0 + r0 / LPA;
@@ -502,6 +502,7 @@ static __global__ __launch_bounds__(kBlock) void k_b_build(BArgs B) {
   A.cap = S.cap;
   A.lpa = S.lpa;
   A.path = S.path;
+  A = wave_uniform_record(A); // one system per wavefront: its arguments in scalar registers
   build_atom(A, ts.i, ts.i < S.n, ring);
   // each lane checks the count it has just written itself
   if (ts.i < S.n && A.cnt[ts.i] > S.cap)
@@ -532,8 +533,8 @@ static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_rh
   const TileLane tw = tile_lane<LPA>();
   if (tw.tl >= B.ntiles)
     return; // uniform over the wavefront
-  const int tile = B.tiles[tw.tl];
-  const int s = B.tsys[tile];
+  const int tile = wave_uniform(B.tiles[tw.tl]);
+  const int s = wave_uniform(B.tsys[tile]);
   if (!B.st[s].act)
     return; // uniform over the wavefront
   const ForceArgs A = force_view(B, s, nullptr);
@@ -548,8 +549,8 @@ static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_fo
   const TileLane tw = tile_lane<LPA>();
   if (tw.tl >= B.ntiles)
     return; // uniform over the wavefront
-  const int tile = B.tiles[tw.tl];
-  const int s = B.tsys[tile];
+  const int tile = wave_uniform(B.tiles[tw.tl]);
+  const int s = wave_uniform(B.tsys[tile]);
   if (!B.st[s].act)
     return; // uniform over the wavefront
   const ForceArgs A = force_view(B, s, nullptr);

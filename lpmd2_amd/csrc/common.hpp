@@ -1,6 +1,8 @@
 // Shared device helpers for the LPMD engine (gfx950, wave64).
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -88,6 +90,24 @@ __device__ __forceinline__ long xcd_block() {
   const long q = nb / kNumXcd, r = nb % kNumXcd;
   const long x = b % kNumXcd;
   return x * q + (x < r ? x : r) + b / kNumXcd;
+}
+
+// A value every lane of the wavefront holds alike (a tile's index, its
+// system), made visibly uniform to the compiler: what is loaded through it
+// (a system's box, grid, potentials) then lives in scalar registers.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The same for a whole record of 32-bit words (a kernel's per-system
+// arguments assembled from memory inside the kernel).
+template <typename T> __device__ __forceinline__ T wave_uniform_record(const T& v) {
+  static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "a record of 32-bit words");
+  T r;
+  const int* a = reinterpret_cast<const int*>(&v);
+  int* b = reinterpret_cast<int*>(&r);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k)
+    b[k] = __builtin_amdgcn_readfirstlane(a[k]);
+  return r;
 }
 
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
