@@ -1020,6 +1020,14 @@ const lpmd::BasicParticleSet& DeviceSimulation::OriginalAtoms() const {
   return atoms_;
 }
 
+lpmd::BasicCell& DeviceSimulation::OriginalCell() {
+  if (!raw_view_) {
+    Refresh();
+    handed_out_ = true;
+  }
+  return cell_;
+}
+
 bool DeviceSimulation::HostChanged() const {
   // positions or velocities written since host and device last agreed
   const long n = atoms_.Size();
@@ -1210,7 +1218,14 @@ void DeviceSimulation::DoSteps(long n) {
       HostStep();
     return;
   }
-  if (first) {
+  bool send = false;
+  if (first && restored_) {
+    // a restored state: sent to the device before the run (an integrator with
+    // an old configuration initialises once per object, as the reference's
+    // do, and keeps its history)
+    send = true;
+    restored_ = false;
+  } else if (first) {
     // Initialize has just handed the host state to the integrator (the
     // schemes with an old configuration sent it to the device there;
     // velocity Verlet sends it with its first run): host and device agree
@@ -1229,7 +1244,7 @@ void DeviceSimulation::DoSteps(long n) {
   // the host state goes to the device when the host holds the current state
   // and it differs from what host and device last agreed on (or the device
   // never held this simulation's state)
-  const bool upload = !device_ahead_ && (dev_ != cm || (handed_out_ && HostChanged()));
+  const bool upload = send || (!device_ahead_ && (dev_ != cm || (handed_out_ && HostChanged())));
   Virial() = 0.0;
   for (int p = 0; p < 3; ++p)
     for (int q = 0; q < 3; ++q)
@@ -1358,6 +1373,7 @@ void DeviceSimulation::Restore(const std::string& path) {
   }
   velocities_set_ = true;
   initialized_ = false;
+  restored_ = true;
   step_ = initial_step_;
 }
 

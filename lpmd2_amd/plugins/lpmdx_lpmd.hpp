@@ -336,7 +336,9 @@ public:
 
   lpmd::BasicParticleSet& OriginalAtoms() override;
   const lpmd::BasicParticleSet& OriginalAtoms() const override;
-  lpmd::BasicCell& OriginalCell() override { return cell_; }
+  // the cell handed out writable: the atoms are refreshed first, so a device
+  // system made again for a changed cell starts from the current state
+  lpmd::BasicCell& OriginalCell() override;
   const lpmd::BasicCell& OriginalCell() const override { return cell_; }
 
   void SetTemperature(double temp, bool tag) override;
@@ -382,6 +384,9 @@ private:
   // during a device run: OriginalAtoms hands out atoms_ as they are (their
   // tags, sizes and masses are current; positions are not read)
   bool raw_view_ = false;
+  // Restore replaced the host state: the next device run sends it, whatever
+  // the integrator's Initialize did
+  bool restored_ = false;
   mutable DeviceCellManager* dev_ = nullptr;
   mutable std::vector<double> agreed_x_, agreed_v_; // the state host and device last agreed on
   mutable long refreshes_ = 0;

@@ -183,3 +183,28 @@ def test_reference_integrator_runs_the_host_loop():
     assert np.abs(xs - xn).max() <= 1e-10 and np.abs(vs - vn).max() <= 1e-12
     check_tags(sim, xs, L, vs)
     sim.close()
+
+
+def test_restore_sends_the_restored_state(tmp_path):
+    """Dump after a device run, run on, Restore: the next device run starts
+    from the restored state (Dump's six significant digits), as a native run
+    started from that state does."""
+    say("device simulation: Dump / Restore between device runs")
+    x, L, v = setup(seed=19)
+    dt = 2.0
+    sim = make_sim(x, L, v)
+    sim.do_steps(10)
+    path = str(tmp_path / "state.dump")
+    sim.dump(path)
+    sim.do_steps(7)
+    sim.restore(path)
+    assert sim.counters()["step"] == 10
+    xr, vr, _ = sim.get()
+    sim.do_steps(10)
+    assert sim.counters()["uploads"] == 1 and sim.counters()["step"] == 20
+    xs, vs, _ = sim.get()
+    xn, vn, _ = native(xr, L, vr, "velocityverlet", dt, 10).get_state()
+    dx, dv = np.abs(xs - xn).max(), np.abs(vs - vn).max()
+    say(f"  max |dx| {dx:.3e}  |dv| {dv:.3e}")
+    assert dx <= 1e-10 and dv <= 1e-12
+    sim.close()
