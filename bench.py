@@ -398,6 +398,37 @@ def served_leg(args):
     return out
 
 
+def served_batch_leg(args):
+    """The served jobs (all arrived at once) in one batch on the calling
+    thread, run until every job is done: the served mix's batched rate
+    without the serving layer (no admission rounds; finished jobs stay in the
+    batch and their tiles return at once)."""
+    from lpmd2_amd import workloads as W
+    from lpmd2_amd.batch import Batch
+
+    jobs = [j for _, j in W.served_jobs(seed=5, n_jobs=args.serve_jobs, rate_hz=1e6)]
+    work = sum(j["x"].shape[0] * j["steps"] for j in jobs)
+    padded = sum(-(-j["x"].shape[0] // 64) * 64 for j in jobs)
+    b = Batch(atom_capacity=padded + 64, list_words=padded * 400, cell_slots=max(4096, padded))
+    sids = [b.add(x=j["x"], box=j["box"], z=j["z"], potentials=j["potentials"], cm=j["cm"], v=j["v"], dt=j["dt"],
+                  steps=j["steps"]) for j in jobs]
+    t0 = time.perf_counter()
+    graph_steps = 0
+    while True:
+        left, _ = b.progress(max(sids) + 1)
+        if not np.any(left > 0):
+            break
+        b.run(64)
+        graph_steps += 64
+    t = time.perf_counter() - t0
+    b.close()
+    out = {"value": work / t, "unit": "atom-steps/s", "jobs": len(jobs), "atom_steps": work, "seconds": t,
+           "graph_steps": graph_steps,
+           "workload": "the served jobs (seed 5) all resident in one batch on the calling thread"}
+    log(f"served jobs in one batch: {work:.3e} atom-steps in {t:.3f} s, {work / t:.4e} atom-steps/s")
+    return out
+
+
 def force_check(w, s, n_sample=20000, seed=99):
     """The relative force RMS ||F_gpu - F_64|| / ||F_64|| of the plain path at
     its current positions against the FP64 oracle (the checker): the
@@ -544,6 +575,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--plain-steps", type=int, default=100, help="steps per system of the plain comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
+    ap.add_argument("--served-batch", action="store_true",
+                    help="also run the served jobs all at once in one batch on this thread")
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[25.0, 100.0, 1e6])
     ap.add_argument("--serve-budget", type=int, default=16, help="server HBM budget in GiB")
@@ -599,6 +632,8 @@ def main():
         out["batched"] = batched_leg(args)
     if not args.no_serve:
         out["served"] = served_leg(args)
+    if args.served_batch:
+        out["served_in_one_batch"] = served_batch_leg(args)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
