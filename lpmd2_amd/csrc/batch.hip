@@ -552,7 +552,7 @@ struct lpmdx_batch {
       BSys& S = hsys[s];
       S.lpa = lpa_of_kind[S.kind];
       S.ch0 = (int)ch.size();
-      S.nch = (S.n + kSortChunk - 1) / kSortChunk;
+      S.nch = (S.n + kBSortChunk - 1) / kBSortChunk;
       for (int j = 0; j < S.nch; ++j)
         ch.push_back(make_int2((int)s, j));
       passes = std::max(passes, S.passes);

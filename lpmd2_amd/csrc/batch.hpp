@@ -4,7 +4,7 @@
 //   k_b_control  one thread per system: step bookkeeping, skin check
 //   k_b_count / k_b_scan / k_b_scatter (one triple per 8-bit digit pass)
 //                stable radix sort of the atoms of every system that needs a
-//                new list by cell: 4096-atom chunks, one workgroup each, so a
+//                new list by cell: 1024-atom chunks, one workgroup each, so a
 //                large system is sorted by many workgroups at once
 //   k_b_gather / k_b_commit / k_b_sorted
 //                cell starts and the reorder of the per-atom state
@@ -31,6 +31,11 @@ constexpr int kMaxTerms = 4;
 constexpr int kTile = kWave; // atoms per tile
 constexpr int kMaxSortCells = 1 << (2 * kRadixBits); // cells per system: at most two digit passes
 constexpr int kScanThreads = 1024;
+// Sort chunks of the batch: 1024 atoms (4 per thread), a quarter of the plain
+// path's, so that the few systems rebuilding in one step still give the
+// count and scatter passes many workgroups
+constexpr int kBSortItems = 4;
+constexpr int kBSortChunk = kBlock * kBSortItems;
 
 // Read-only description of one system for a round.
 struct BSys {
@@ -225,8 +230,8 @@ static __global__ __launch_bounds__(kBlock) void k_b_count(BArgs B, int pass) {
   uint32_t* kin = B.R.keys[pass & 1] + a0;
   uint32_t* vin = B.R.vals[pass & 1] + a0;
   const double* xc = B.R.x[st.par] + 3 * a0;
-  for (int u = 0; u < kSortItems; ++u) {
-    const int i = j * kSortChunk + u * kBlock + t;
+  for (int u = 0; u < kBSortItems; ++u) {
+    const int i = j * kBSortChunk + u * kBlock + t;
     if (i < n) {
       uint32_t key;
       if (pass == 0) {
@@ -291,8 +296,8 @@ static __global__ __launch_bounds__(kBlock) void k_b_scatter(BArgs B, int pass) 
   const uint32_t* vin = B.R.vals[pass & 1] + a0;
   uint32_t* kout = B.R.keys[(pass + 1) & 1] + a0;
   uint32_t* vout = B.R.vals[(pass + 1) & 1] + a0;
-  for (int u = 0; u < kSortItems; ++u) {
-    const int base = j * kSortChunk + u * kBlock;
+  for (int u = 0; u < kBSortItems; ++u) {
+    const int base = j * kBSortChunk + u * kBlock;
     if (base >= n)
       break; // uniform over the workgroup
     const int i = base + t;
