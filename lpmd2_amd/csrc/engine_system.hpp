@@ -924,6 +924,8 @@ struct lpmdx_system {
           k_build_list_lc_wave<true><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
         else
           k_build_list_lc_wave<false><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
+      } else if (grid.mode == kExact && path == kPathShift32) {
+        k_build_list_shift32<<<blocks_for(n), kBlock, 0, st>>>(B);
       } else {
         k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
       }
