@@ -164,6 +164,11 @@ int sim_get(void* h, double* x, double* v, double* a) {
   });
 }
 
+// Atoms() called once, as a loop that reads or tags the atoms each step does
+int sim_touch(void* h) {
+  return Guard([&] { (void)static_cast<Driver*>(h)->sim->Atoms().Size(); });
+}
+
 // the velocity of atom i written through Atoms()
 int sim_set_velocity(void* h, long i, const double* v) {
   return Guard([&] {

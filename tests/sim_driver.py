@@ -35,6 +35,7 @@ def lib():
         L.sim_do_steps.argtypes = [vp, l]
         L.sim_do_step.argtypes = [vp]
         L.sim_get.argtypes = [vp, vp, vp, vp]
+        L.sim_touch.argtypes = [vp]
         L.sim_set_velocity.argtypes = [vp, l, vp]
         L.sim_set_atom_tag.argtypes = [vp, l, s, s]
         L.sim_set_temperature.argtypes = [vp, d, i]
@@ -104,6 +105,10 @@ class Sim:
         x, v, a = (np.zeros((self.n, 3)) for _ in range(3))
         _check(lib().sim_get(self.h, _p(x), _p(v), _p(a)))
         return x, v, a
+
+    def touch(self):
+        """Atoms() once (a loop that reads or tags the atoms each step)."""
+        _check(lib().sim_touch(self.h))
 
     def set_velocity(self, i: int, v):
         vv = np.ascontiguousarray(v, dtype=np.float64).reshape(3)
