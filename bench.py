@@ -60,6 +60,8 @@ FP32_PER_CUT_PAIR = {"lennardjones": 9, "suttonchen": 13,
                      # finnissinclair-ext: r, the rho-derivative term (the pair polynomial is FP64)
                      "harmonic": 4, "lennardjonesMod": 14, "finnissinclair-ext": 9}
 
+PROF_EVERY = 4  # step sampling period of the per-kernel HIP-event timing
+
 WORKLOADS = {
     # the reference's own CPU-runnable case: 2048 argon atoms on the linkedcell
     # stencil of `mode auto` (12^3 cells, two a side), 1000 steps of 1 fs
@@ -481,7 +483,10 @@ def plain_leg(name: str, steps: int, warmup: int, lanes: int):
     ep0, ek0 = advance(w["dt"], max(warmup, 1), obs_every=max(warmup, 1))
     log(f"warmup {warmup} steps: {time.perf_counter() - t0:.3f} s; stats {s.stats()}")
 
-    s.profile(True)
+    # HIP events bracket the launches of every PROF_EVERY-th step (and every
+    # list build): each event costs the stream about 2.5 us, so bracketing
+    # every launch would add about 10 us to each EAM step
+    s.profile(True, every=PROF_EVERY)
     st0 = s.stats()
     t0 = time.perf_counter()
     advance(w["dt"], steps)
@@ -518,6 +523,7 @@ def plain_leg(name: str, steps: int, warmup: int, lanes: int):
         "traffic": measured_traffic(name, kname),
         "avg_launch_us": round(avg_force_s * 1e6, 2),
         "launches": f_n,
+        "events_every_steps": PROF_EVERY,
         "bytes_per_launch": int(bytes_force),
         "mean_list_len": round(nbar, 2),
     }

@@ -304,9 +304,11 @@ LPMDX_API int lpmdx_observables_noalloc(lpmdx_system* sys, double out[LPMDX_NOBS
 LPMDX_API int lpmdx_system_stats(const lpmdx_system* sys, long out[9]);
 
 /* Per-kernel timing with HIP events recorded on the system's own stream:
- * enable (1) or disable (0); enabling clears the totals.  read returns the
- * summed milliseconds and launch counts of [0] force pass (pair or EAM),
- * [1] EAM rho pass, [2] list build (sort + list), [3] other. */
+ * enable (k >= 1) or disable (0); enabling clears the totals.  With k > 1
+ * only the launches of every k-th integrator step are bracketed (list builds
+ * always are), which keeps the events' own cost out of the other steps.
+ * read returns the summed milliseconds and launch counts of [0] force pass
+ * (pair or EAM), [1] EAM rho pass, [2] list build (sort + list), [3] other. */
 LPMDX_API int lpmdx_system_profile(lpmdx_system* sys, int enable);
 LPMDX_API int lpmdx_system_profile_read(lpmdx_system* sys, double ms[4], long launches[4]);
 

@@ -294,8 +294,10 @@ class System:
                 "cut_chunks")
         return dict(zip(keys, (int(v) for v in out)))
 
-    def profile(self, enable: bool = True):
-        _check(lib().lpmdx_system_profile(self._h, int(enable)))
+    def profile(self, enable: bool = True, every: int = 1):
+        """Per-kernel HIP-event timing on (the launches of every `every`-th
+        integrator step, and every list build) or off; clears the totals."""
+        _check(lib().lpmdx_system_profile(self._h, int(every) if enable else 0))
 
     def profile_read(self):
         """{slot: (ms, launches)} for force, rho, build, other."""

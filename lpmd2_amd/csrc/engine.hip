@@ -671,8 +671,11 @@ int lpmdx_system_stats(const lpmdx_system* S, long out[9]) {
 int lpmdx_system_profile(lpmdx_system* S, int enable) {
   return guard([&] {
     need(S, "system");
+    if (enable < 0)
+      fail(LPMDX_EINVAL, "profile: enable must be 0 (off) or the step sampling period (>= 1)");
     S->sync();
     S->profiling = enable != 0;
+    S->prof_every = enable > 0 ? enable : 1;
     for (int k = 0; k < kProfSlots; ++k) {
       S->prof_ms[k] = 0.0;
       S->prof_n[k] = 0;

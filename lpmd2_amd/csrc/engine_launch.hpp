@@ -73,15 +73,14 @@ template <int KIND, bool SPC, int EXP> bool lpmdx_system::launch_eam_fixed(Force
     case 2: BrickKernels<KIND, 2, false>::template rho<EXP>(Lr, R, w_rho.p, rho.p); break;
     default: BrickKernels<KIND, 4, false>::template rho<EXP>(Lr, R, w_rho.p, rho.p); break;
     }
-    prof_end(kProfRho, e0);
+    hipEvent_t e1 = prof_next(kProfRho, e0);
     A.w_rho = w_rho.p;
-    hipEvent_t e1 = prof_begin();
     switch (lpa) {
     case 1: BrickKernels<KIND, 1, false>::template eam<SPC, EXP>(Lf, A, parts); break;
     case 2: BrickKernels<KIND, 2, false>::template eam<SPC, EXP>(Lf, A, parts); break;
     default: BrickKernels<KIND, 4, false>::template eam<SPC, EXP>(Lf, A, parts); break;
     }
-    prof_end(kProfForce, e1);
+    prof_end(kProfForce, e1, false);
     return true;
   }
   switch (lpa) {
@@ -89,15 +88,14 @@ template <int KIND, bool SPC, int EXP> bool lpmdx_system::launch_eam_fixed(Force
   case 2: k_eam_rho<KIND, 2, false, EXP><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
   default: k_eam_rho<KIND, 4, false, EXP><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
   }
-  prof_end(kProfRho, e0);
+  hipEvent_t e1 = prof_next(kProfRho, e0);
   A.w_rho = w_rho.p;
-  hipEvent_t e1 = prof_begin();
   switch (lpa) {
   case 1: k_eam_force<KIND, 1, false, SPC, EXP><<<nb, kBlock, 0, st>>>(A); break;
   case 2: k_eam_force<KIND, 2, false, SPC, EXP><<<nb, kBlock, 0, st>>>(A); break;
   default: k_eam_force<KIND, 4, false, SPC, EXP><<<nb, kBlock, 0, st>>>(A); break;
   }
-  prof_end(kProfForce, e1);
+  prof_end(kProfForce, e1, false);
   return true;
 }
 template <int KIND, bool EVAL, bool SPC> void lpmdx_system::launch_eam_s(ForceArgs& A, long nb, double rc2_rho) {
@@ -116,15 +114,14 @@ template <int KIND, bool EVAL, bool SPC> void lpmdx_system::launch_eam_s(ForceAr
     case 2: BrickKernels<KIND, 2, EVAL>::rho(Lr, R, w_rho.p, rho.p); break;
     default: BrickKernels<KIND, 4, EVAL>::rho(Lr, R, w_rho.p, rho.p); break;
     }
-    prof_end(kProfRho, e0);
+    hipEvent_t e1 = prof_next(kProfRho, e0);
     A.w_rho = w_rho.p;
-    hipEvent_t e1 = prof_begin();
     switch (lpa) {
     case 1: BrickKernels<KIND, 1, EVAL>::template eam<SPC>(Lf, A, parts); break;
     case 2: BrickKernels<KIND, 2, EVAL>::template eam<SPC>(Lf, A, parts); break;
     default: BrickKernels<KIND, 4, EVAL>::template eam<SPC>(Lf, A, parts); break;
     }
-    prof_end(kProfForce, e1);
+    prof_end(kProfForce, e1, false);
     return;
   }
   switch (lpa) {
@@ -134,9 +131,8 @@ template <int KIND, bool EVAL, bool SPC> void lpmdx_system::launch_eam_s(ForceAr
   case 8: k_eam_rho<KIND, 8, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
   default: k_eam_rho<KIND, 16, EVAL><<<nb, kBlock, 0, st>>>(R, w_rho.p, rho.p); break;
   }
-  prof_end(kProfRho, e0);
+  hipEvent_t e1 = prof_next(kProfRho, e0);
   A.w_rho = w_rho.p;
-  hipEvent_t e1 = prof_begin();
   switch (lpa) {
   case 1: k_eam_force<KIND, 1, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
   case 2: k_eam_force<KIND, 2, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
@@ -144,7 +140,7 @@ template <int KIND, bool EVAL, bool SPC> void lpmdx_system::launch_eam_s(ForceAr
   case 8: k_eam_force<KIND, 8, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
   default: k_eam_force<KIND, 16, EVAL, SPC><<<nb, kBlock, 0, st>>>(A); break;
   }
-  prof_end(kProfForce, e1);
+  prof_end(kProfForce, e1, false);
 }
 
 // one potential kind's launchers, both with and without energies

@@ -369,12 +369,16 @@ struct lpmdx_system {
       return false;
     }
   } bsched;
-  // optional per-kernel timing with HIP events on this system's stream
+  // optional per-kernel timing with HIP events on this system's stream:
+  // every launch outside the integrator steps (list builds), and the
+  // launches of every prof_every-th step
   bool profiling = false;
+  long prof_every = 1;
   struct Pending {
     int slot;
     hipEvent_t a, b;
-    long step; // the integrator step the launch belongs to, or -1
+    long step;  // the integrator step the launch belongs to, or -1
+    bool own_a; // false: a is the previous entry's b (one event between two launches)
   };
   std::vector<Pending> pending;
   long prof_step = -1; // tag of the launches being recorded
@@ -385,7 +389,8 @@ struct lpmdx_system {
     size_t k = 0;
     for (auto& p : pending) {
       if (p.step >= first) {
-        event_pool.push_back(p.a);
+        if (p.own_a)
+          event_pool.push_back(p.a);
         event_pool.push_back(p.b);
       } else {
         pending[k++] = p;
@@ -403,24 +408,39 @@ struct lpmdx_system {
       event_pool.pop_back();
       return e;
     }
+    // Timing-only events: the host reads them after a stream synchronisation,
+    // so they need no system-scope release, whose L2 writeback and
+    // invalidation would add several microseconds to every bracketed launch.
     hipEvent_t e;
-    hipchk(hipEventCreate(&e), "hipEventCreate");
+    hipchk(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
     return e;
   }
   // brackets work enqueued between begin() and end() on the stream
   hipEvent_t prof_begin() {
-    if (!profiling)
+    if (!profiling || (prof_step >= 0 && prof_step % prof_every != 0))
       return nullptr;
     hipEvent_t e = take_event();
     hipchk(hipEventRecord(e, st), "hipEventRecord");
     return e;
   }
-  void prof_end(int slot, hipEvent_t a) {
+  // own_a: a came from prof_begin (false: from prof_next, owned by the
+  // entry whose end it is)
+  void prof_end(int slot, hipEvent_t a, bool own_a = true) {
     if (!profiling || !a)
       return;
     hipEvent_t e = take_event();
     hipchk(hipEventRecord(e, st), "hipEventRecord");
-    pending.push_back({slot, a, e, prof_step});
+    pending.push_back({slot, a, e, prof_step, own_a});
+  }
+  // End of a bracketed launch whose successor is enqueued right after it:
+  // the end event is returned as the successor's begin event (one event
+  // between two launches instead of two); close the successor with
+  // prof_end(slot, event, false).
+  hipEvent_t prof_next(int slot, hipEvent_t a, bool own_a = true) {
+    if (!profiling || !a)
+      return nullptr;
+    prof_end(slot, a, own_a);
+    return pending.back().b;
   }
   void prof_collect() {
     for (auto& p : pending) {
@@ -428,7 +448,8 @@ struct lpmdx_system {
       hipchk(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
       prof_ms[p.slot] += ms;
       prof_n[p.slot] += 1;
-      event_pool.push_back(p.a);
+      if (p.own_a)
+        event_pool.push_back(p.a);
       event_pool.push_back(p.b);
     }
     pending.clear();
@@ -438,7 +459,8 @@ struct lpmdx_system {
     if (st)
       (void)hipStreamSynchronize(st);
     for (auto& p : pending) {
-      event_pool.push_back(p.a);
+      if (p.own_a)
+        event_pool.push_back(p.a);
       event_pool.push_back(p.b);
     }
     for (auto e : event_pool)

@@ -626,6 +626,38 @@ def test_nve_energy_matches_oracle_suttonchen():
     assert abs((et.max() - et.min()) - (et0.max() - et0.min())) < 0.2 * (et0.max() - et0.min())
 
 
+def test_event_profile_samples_steps_and_leaves_results_alone():
+    """Per-kernel HIP-event timing: every launch of every k-th step is
+    bracketed (the EAM rho and force passes share the event between them),
+    the counts follow the sampling period, and the trajectory is the same bit
+    for bit with the events on or off."""
+    say("event profile")
+    x, L = jittered(5, 3.61, 0.01, 4)
+    m = np.full(x.shape[0], 63.546)
+    v = E.maxwell_boltzmann(m, 300.0, seed=4)
+    out = []
+    for every in (0, 1, 4):
+        s = E.System(x, L, 29, v=v)
+        s.set_cellmanager("verletlist", "cutoff 7.22 maxneighbors 400 each 5 skin 0.3")
+        s.append_potential("suttonchen", "e 1.2382e-2 a 3.61 n 9 m 6 c 39.432 cutoff 7.22")
+        s.velocityverlet(2.0, 3)
+        s.profile(every > 0, every=max(every, 1))
+        s.velocityverlet(2.0, 16)
+        p = s.profile_read()
+        out.append(s.get_state())
+        if every:
+            assert p["force"][1] == 16 // every and p["rho"][1] == 16 // every, p
+            assert p["force"][0] > 0.0 and p["rho"][0] > 0.0
+        else:
+            assert p["force"][1] == 0 and p["rho"][1] == 0
+    with pytest.raises(E.LpmdxError) as e:
+        E._check(E.lib().lpmdx_system_profile(s._h, -1))
+    assert e.value.code == E.EINVAL
+    for st in out[1:]:
+        for a, b in zip(out[0], st):
+            np.testing.assert_array_equal(a, b)
+
+
 # --------------------------------------------------------------- error behaviour
 def test_error_paths():
     say("errors")
