@@ -59,6 +59,33 @@ __global__ __launch_bounds__(kBlock) void k_chunk_begin(unsigned int* __restrict
     md[t] = 0u;
 }
 
+// End of a chunk of m speculative steps whose list is rebuilt before the host
+// has read the displacement slots: the steps that ran (a step whose gate was
+// shut closed its own slot, and every later step's, so the closed slots are a
+// suffix of md[0, m); step 0 always runs) wrote their positions alternately
+// into the two buffers the host swapped after every launch.  When an odd
+// number of launches returned at once, the last positions written are in the
+// buffer the host now calls xn: they are copied into x, so that the host's x
+// holds them whatever the count.
+__global__ __launch_bounds__(kBlock) void k_settle_positions(const unsigned int* __restrict__ md, long m,
+                                                              double* __restrict__ x,
+                                                              const double* __restrict__ xn, long n3) {
+  // ran = the first closed slot (binary search over the closed suffix), or m
+  long lo = 1, hi = m;
+  while (lo < hi) {
+    const long mid = (lo + hi) / 2;
+    if (md[mid] == kGateClosed)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  if (((m - lo) & 1) == 0)
+    return;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (t < n3)
+    x[t] = xn[t];
+}
+
 // Integrator::GoBack (liblpmd/simulation/integrator.cc:31-42) for the leapfrog
 // and Beeman set-ups: x and a saved, vh = v - a dt (when vh is given),
 // x = FittedInside(x - v dt).
@@ -307,6 +334,7 @@ struct lpmdx_system {
   long cell_version = -1; // pos_version at the last UpdateCell
   double disp_cur = 0.0; // largest |x - xb| of the positions of the next force evaluation
   long chunk = 8;        // steps of the next velocity-Verlet chunk
+  bool chunk_by_growth = false; // chunk was sized by the displacement growth (expected to end past the budget)
   long n_cut_chunks = 0; // chunks cut short by the skin test
   double last_e[12] = {0};
   bool last_e_valid = false;
@@ -864,7 +892,9 @@ struct lpmdx_system {
     }
   }
 
-  void build_list() {
+  // Set-up of a list build: lanes, the staging choice, room for the partial
+  // sums, the brick halo tables and a first capacity estimate.
+  void build_prepare() {
     // list storage: cap entries per atom, rounded to the lane count
     lpa = auto_lanes();
     stride = ((n + kWave - 1) / kWave) * kWave; // whole tiles for every lane count
@@ -890,92 +920,111 @@ struct lpmdx_system {
         est = std::max(est, (double)n / vol * pow((2 * grid.side + 1) * box.L[0] / grid.nx, 3));
       cap = (int)std::min<double>(n * 8.0 + 64, est * 1.3 + 32);
     }
+  }
+  // One build with the current capacity, ending with the copy of its control
+  // words (max_count, pad0, sum_count, halo_max, brick_max) to the host; no
+  // host synchronisation.
+  void build_attempt() {
+    cap = ((cap + kChunk - 1) / kChunk) * kChunk; // whole 8-entry chunks
+    nbr.alloc((size_t)cap * stride);
+    cnt.alloc(n);
+    // max_count, pad0, sum_count: 16 contiguous bytes
+    hipchk(hipMemsetAsync(&ctrl.p->max_count, 0, 16, st), "memset");
+    sumslots.alloc((size_t)kSumSlots * kSlotStride);
+    hipchk(hipMemsetAsync(sumslots.p, 0, (size_t)kSumSlots * kSlotStride * sizeof(unsigned long long), st), "memset");
+    const double rl2 = rl * rl;
+    // FP32 positions are off by at most ulp(Lmax)/2; two of them in a distance
+    const double lmax = std::max(box.L[0], std::max(box.L[1], box.L[2]));
+    const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
+    const float rl2f = std::isfinite(rl2) ? (float)((rl + 2.0 * pos_err) * (rl + 2.0 * pos_err) * (1.0 + 1e-6))
+                                          : INFINITY;
+    // FP32 paths need the minimum image to be unambiguous within the list radius
+    const double hmin = std::min(box.half[0], std::min(box.half[1], box.half[2]));
+    int path = kPathExact64;
+    if (std::isfinite(rl2) && hmin > rl + 4.0 * pos_err + 1e-9 * hmin) {
+      if (grid.mode == kExact && grid.nx >= 3 && grid.ny >= 3 && grid.nz >= 3)
+        path = kPathShift32;
+      else
+        path = kPathMin32;
+    }
+    xf.alloc(n + kGroup); // padding read by the candidate groups
+    k_to_f32<<<blocks_for(n), kBlock, 0, st>>>(x.p, n, box, xf.p);
+    BuildArgs B{};
+    B.x = x.p;
+    B.xf = xf.p;
+    B.skeys = keys.p;
+    B.cb = cb.p;
+    B.bhalo = staged ? bhalo.p : nullptr;
+    B.nbr = nbr.p;
+    B.cnt = cnt.p;
+    B.max_count = &ctrl.p->max_count;
+    B.sum_count = sumslots.p;
+    B.sum_spread = 1;
+    B.g = grid;
+    B.b = box;
+    for (int q = 0; q < 3; ++q) {
+      B.Lf[q] = (float)box.L[q];
+      B.halff[q] = (float)box.half[q];
+    }
+    B.rl2 = rl2;
+    B.rl2f = rl2f;
+    B.n = n;
+    B.stride = stride;
+    B.cap = cap;
+    B.lpa = lpa;
+    B.path = path;
+    hipEvent_t e0 = prof_begin();
+    if (grid.mode == kLinkedCell && !staged && n <= kWaveBuildMaxAtoms) {
+      // few atoms on a linkedcell stencil: a wavefront per atom
+      if (path == kPathExact64)
+        k_build_list_lc_wave<true><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
+      else
+        k_build_list_lc_wave<false><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
+    } else if (grid.mode == kExact && path == kPathShift32) {
+      k_build_list_shift32<<<blocks_for(n), kBlock, 0, st>>>(B);
+    } else {
+      k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
+    }
+    prof_end(kProfBuild, e0);
+    k_sum_slots<<<1, kWave, 0, st>>>(sumslots.p, &ctrl.p->sum_count);
+    launch_check();
+    // max_count, pad0, sum_count, halo_max, brick_max: 24 contiguous bytes
+    hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 24, hipMemcpyDeviceToHost, st),
+           "memcpy");
+  }
+  // After the stream synchronised past build_attempt(): true when the list
+  // fits its capacity (and the staged kernels' halo fits LDS), with halo_cap,
+  // brick_threads, max_list and sum_list set; false after the capacity or the
+  // staging choice was changed for another attempt.
+  bool build_check() {
+    if (staged) {
+      halo_cap = (int)(((hctrl->halo_max + 63) / 64) * 64);
+      const int lanes_atoms = (int)hctrl->brick_max * lpa;
+      brick_threads = std::max(kWave, std::min(1024, ((lanes_atoms + kWave - 1) / kWave) * kWave));
+      if (stage_bytes(halo_cap, stage_parts()) > (size_t)kStageLdsLimit) {
+        // a halo too large for LDS (a dense spot): global indices this time
+        // (the grid keeps its brick order; the unstaged kernels take any order)
+        staged = false;
+        halo_cap = 0;
+        return false;
+      }
+    }
+    const long mc = hctrl->max_count;
+    max_list = mc;
+    sum_list = (long)hctrl->sum_count;
+    if (mc <= cap)
+      return true;
+    cap = (int)(mc * 1.2 + 16);
+    return false;
+  }
+
+  void build_list() {
+    build_prepare();
     for (int attempt = 0; attempt < 4; ++attempt) {
-      cap = ((cap + kChunk - 1) / kChunk) * kChunk; // whole 8-entry chunks
-      nbr.alloc((size_t)cap * stride);
-      cnt.alloc(n);
-      // max_count, pad0, sum_count: 16 contiguous bytes
-      hipchk(hipMemsetAsync(&ctrl.p->max_count, 0, 16, st), "memset");
-      sumslots.alloc((size_t)kSumSlots * kSlotStride);
-      hipchk(hipMemsetAsync(sumslots.p, 0, (size_t)kSumSlots * kSlotStride * sizeof(unsigned long long), st), "memset");
-      const double rl2 = rl * rl;
-      // FP32 positions are off by at most ulp(Lmax)/2; two of them in a distance
-      const double lmax = std::max(box.L[0], std::max(box.L[1], box.L[2]));
-      const double pos_err = 2.0 * std::ldexp(lmax, -24) + 1e-30;
-      const float rl2f = std::isfinite(rl2) ? (float)((rl + 2.0 * pos_err) * (rl + 2.0 * pos_err) * (1.0 + 1e-6))
-                                            : INFINITY;
-      // FP32 paths need the minimum image to be unambiguous within the list radius
-      const double hmin = std::min(box.half[0], std::min(box.half[1], box.half[2]));
-      int path = kPathExact64;
-      if (std::isfinite(rl2) && hmin > rl + 4.0 * pos_err + 1e-9 * hmin) {
-        if (grid.mode == kExact && grid.nx >= 3 && grid.ny >= 3 && grid.nz >= 3)
-          path = kPathShift32;
-        else
-          path = kPathMin32;
-      }
-      xf.alloc(n + kGroup); // padding read by the candidate groups
-      k_to_f32<<<blocks_for(n), kBlock, 0, st>>>(x.p, n, box, xf.p);
-      BuildArgs B{};
-      B.x = x.p;
-      B.xf = xf.p;
-      B.skeys = keys.p;
-      B.cb = cb.p;
-      B.bhalo = staged ? bhalo.p : nullptr;
-      B.nbr = nbr.p;
-      B.cnt = cnt.p;
-      B.max_count = &ctrl.p->max_count;
-      B.sum_count = sumslots.p;
-      B.sum_spread = 1;
-      B.g = grid;
-      B.b = box;
-      for (int q = 0; q < 3; ++q) {
-        B.Lf[q] = (float)box.L[q];
-        B.halff[q] = (float)box.half[q];
-      }
-      B.rl2 = rl2;
-      B.rl2f = rl2f;
-      B.n = n;
-      B.stride = stride;
-      B.cap = cap;
-      B.lpa = lpa;
-      B.path = path;
-      hipEvent_t e0 = prof_begin();
-      if (grid.mode == kLinkedCell && !staged && n <= kWaveBuildMaxAtoms) {
-        // few atoms on a linkedcell stencil: a wavefront per atom
-        if (path == kPathExact64)
-          k_build_list_lc_wave<true><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
-        else
-          k_build_list_lc_wave<false><<<blocks_for(n * kWave), kBlock, 0, st>>>(B);
-      } else if (grid.mode == kExact && path == kPathShift32) {
-        k_build_list_shift32<<<blocks_for(n), kBlock, 0, st>>>(B);
-      } else {
-        k_build_list<<<blocks_for(n), kBlock, 0, st>>>(B);
-      }
-      prof_end(kProfBuild, e0);
-      k_sum_slots<<<1, kWave, 0, st>>>(sumslots.p, &ctrl.p->sum_count);
-      launch_check();
-      // max_count, pad0, sum_count, halo_max, brick_max: 24 contiguous bytes
-      hipchk(hipMemcpyAsync(&hctrl->max_count, &ctrl.p->max_count, 24, hipMemcpyDeviceToHost, st),
-             "memcpy");
+      build_attempt();
       sync();
-      if (staged) {
-        halo_cap = (int)(((hctrl->halo_max + 63) / 64) * 64);
-        const int lanes_atoms = (int)hctrl->brick_max * lpa;
-        brick_threads = std::max(kWave, std::min(1024, ((lanes_atoms + kWave - 1) / kWave) * kWave));
-        if (stage_bytes(halo_cap, stage_parts()) > (size_t)kStageLdsLimit) {
-          // a halo too large for LDS (a dense spot): global indices this time
-          // (the grid keeps its brick order; the unstaged kernels take any order)
-          staged = false;
-          halo_cap = 0;
-          continue;
-        }
-      }
-      const long mc = hctrl->max_count;
-      max_list = mc;
-      sum_list = (long)hctrl->sum_count;
-      if (mc <= cap)
+      if (build_check())
         return;
-      cap = (int)(mc * 1.2 + 16);
     }
     fail(LPMDX_ECAPACITY, "neighbour list capacity could not be established");
   }
@@ -1005,6 +1054,36 @@ struct lpmdx_system {
     if (cm.kind == kCmVerlet && max_list >= cm.maxneighbors)
       fail(LPMDX_ECAPACITY, "verletlist: maxneighbors is too small!");
     hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    list_built = true;
+    n_builds++;
+  }
+
+  // rebuild() enqueued behind work still running, with no host
+  // synchronisation: the sort, one build attempt at the current capacity and
+  // the copy of the build positions.  rebuild_settle() finishes it once the
+  // stream has synchronised.  Not for lcbinary, whose grid growth needs the
+  // host between sorts.
+  bool can_rebuild_ahead() const { return n > 0 && cm.kind != kCmLcBinary && list_built && cap > 0; }
+  void rebuild_ahead() {
+    rl = list_radius();
+    setup_grid();
+    hipEvent_t e0 = prof_begin();
+    sort_atoms();
+    prof_end(kProfBuild, e0);
+    build_prepare();
+    build_attempt();
+    hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+  }
+  // After the synchronisation: the list of rebuild_ahead() is kept when it fit
+  // its capacity; otherwise the build runs again at the same positions with
+  // the larger capacity (rebuild(), which synchronises).
+  void rebuild_settle() {
+    if (!build_check()) {
+      rebuild();
+      return;
+    }
+    if (cm.kind == kCmVerlet && max_list >= cm.maxneighbors)
+      fail(LPMDX_ECAPACITY, "verletlist: maxneighbors is too small!");
     list_built = true;
     n_builds++;
   }
@@ -1501,11 +1580,6 @@ struct lpmdx_system {
       sync();
       disp_cur = read_disp(0);
     }
-    // Steps run in chunks without a host round trip; after a chunk the exact
-    // displacement of every step's positions from the build positions says
-    // whether the chunk's lists stayed valid (every atom within half the skin
-    // of its build position).  A chunk that went past it is rolled back and
-    // re-run up to its last valid step; the list is rebuilt there.
     // Steps run in chunks without a host round trip.  Every step after a
     // chunk's first is speculative: its kernels run only while the positions
     // its forces read passed the skin test (every atom within half the skin
@@ -1513,7 +1587,14 @@ struct lpmdx_system {
     // on the device), and return at once otherwise.  After the chunk the host
     // reads the slots, keeps the steps that ran, and rebuilds the list there;
     // nothing has to be rolled back.
+    // A chunk sized to end past the budget (and every step of a manager that
+    // rebuilds on every update) has its rebuild enqueued behind it before the
+    // host reads anything: the positions of the last step that ran are
+    // settled into x on the device, the list is rebuilt there, and one
+    // synchronisation serves both.  When the chunk ended inside the budget
+    // after all, the list was rebuilt a little early, which is still exact.
     long done = 0;
+    bool prebuilt = false; // the list for the next chunk was built ahead
     while (done < steps) {
       prof_step = -1; // rebuilds are no step's launches
       long m = steps - done;
@@ -1529,12 +1610,21 @@ struct lpmdx_system {
         need = !list_built || list_radius() != rl || disp_cur > budget();
         m = std::min(m, chunk);
       }
+      if (prebuilt) {
+        need = false;
+        prebuilt = false;
+      }
       // positions x are where the next force evaluation happens
       drifted = false;
       if (need) {
         rebuild();
         disp_cur = 0.0;
       }
+      // the next chunk's list is built ahead when this chunk is expected to
+      // end past the budget (or the manager rebuilds on every update), and
+      // steps follow it in this call without an observation in between
+      const bool ahead = can_rebuild_ahead() && (every || (m == chunk && chunk_by_growth)) && done + m < steps &&
+                         (obs_every <= 0 || (done + m) % obs_every != 0);
       // cleared per-step maxima (m <= kMaxChunk)
       k_chunk_begin<<<blocks_for(m), kBlock, 0, st>>>(md, m);
       launch_check();
@@ -1556,6 +1646,13 @@ struct lpmdx_system {
         }
       }
       prof_step = -1;
+      if (ahead) {
+        if (m > 1) {
+          k_settle_positions<<<blocks_for(3 * n), kBlock, 0, st>>>(md, m, x.p, xn.p, 3 * n);
+          launch_check();
+        }
+        rebuild_ahead();
+      }
       hipchk(hipMemcpyAsync(hctrl->maxd2, md, m * sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy");
       // no profile collection yet: the timings of steps that did not run go first
       hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1574,7 +1671,8 @@ struct lpmdx_system {
         d = dn;
       }
       // each launch swapped the position buffers; the skipped ones wrote none
-      if ((m - ran) % 2 == 1)
+      // (a list built ahead found the positions settled into x already)
+      if (!ahead && (m - ran) % 2 == 1)
         std::swap(x.p, xn.p);
       prof_discard_from(n_steps + ran);
       if (ran < m)
@@ -1584,14 +1682,24 @@ struct lpmdx_system {
       if (ran < m)
         n_cut_chunks++;
       disp_cur = d;
+      if (ahead) {
+        rebuild_settle();
+        disp_cur = 0.0;
+        prebuilt = true;
+      }
       pos_version++;
       // next chunk: double while nothing moves, else the steps the growth
       // rate leaves before the budget (from a fresh list after a cut), plus
       // two so that the chunk usually ends past the limit in one round trip
       long next = std::min<long>(2 * chunk, kMaxChunk);
+      chunk_by_growth = false;
       if (growth > 0.0) {
-        const double room = disp_cur > b ? b : b - disp_cur;
-        next = std::min<long>(next, (long)(room / growth) + 2);
+        const double room = (disp_cur > b || prebuilt) ? b : b - disp_cur;
+        const long by_growth = (long)(room / growth) + 2;
+        if (by_growth < next) {
+          next = by_growth;
+          chunk_by_growth = true;
+        }
       }
       chunk = std::max<long>(1, next);
       drifted = true;
