@@ -230,10 +230,17 @@ template <int KIND, int LPA, bool EVAL> struct BrickKernels {
     k_eam_rho_bk<KIND, LPA, EVAL, EXP><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick, L.cap, w, rho_out);
   }
   template <bool SPC, int EXP = 0> static void eam(const BrickLaunch& L, const ForceArgs& A, int parts) {
-    static const bool ok = allow_stage_lds((const void*)k_eam_force_bk<KIND, LPA, EVAL, SPC, EXP>);
-    (void)ok;
-    k_eam_force_bk<KIND, LPA, EVAL, SPC, EXP><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick, L.cap,
-                                                                                      parts);
+    if constexpr (eam_force_capped<KIND, EVAL, EXP>()) {
+      static const bool ok = allow_stage_lds((const void*)k_eam_force_bk<KIND, LPA, EVAL, SPC, EXP>);
+      (void)ok;
+      k_eam_force_bk<KIND, LPA, EVAL, SPC, EXP><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick, L.cap,
+                                                                                        parts);
+    } else {
+      static const bool ok = allow_stage_lds((const void*)k_eam_force_bk_wide<KIND, LPA, EVAL, SPC, EXP>);
+      (void)ok;
+      k_eam_force_bk_wide<KIND, LPA, EVAL, SPC, EXP><<<L.bricks, L.threads, L.lds, L.st>>>(A, L.bhalo, L.brick,
+                                                                                             L.cap, parts);
+    }
   }
 };
 
