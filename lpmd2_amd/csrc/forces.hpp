@@ -1024,14 +1024,15 @@ __host__ __device__ inline size_t stage_bytes(int cap, int parts) {
 // atom count.
 // Every thread takes slots threadIdx.x + u * blockDim.x, finds each slot's
 // halo cell by binary search over the cells' first slots, and issues the
-// loads of SB slots before storing any.  Slots past the halo's end load the
-// last slot's atom (no branch around a load, which would make the compiler
-// wait for each load before the next) and store nothing.  SB slots hold about
-// 9 SB registers while in flight: the EAM kernels take 4, which keeps their
-// staging inside the register budget of two workgroups per CU.
-constexpr int kStageBatch = 8;
+// loads of kStageBatch slots before storing any.  Slots past the halo's end
+// load the last slot's atom (no branch around a load, which would make the
+// compiler wait for each load before the next) and store nothing.  The slots
+// in flight hold about 9 registers each: 4 of them keep the staging inside
+// the register budget of the force kernels' occupancy (8 spilled there;
+// measured with 4: LJ 4M force pass 597 -> 558 us, 2 no better, 3 and 6
+// slower).
+constexpr int kStageBatch = 4;
 
-template <int SB = kStageBatch>
 __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __restrict__ row, int brick,
                                             const StageLds& L, const float* __restrict__ wsrc, int& a0,
                                             int& na) {
@@ -1041,10 +1042,10 @@ __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __re
     sr[h] = row[h];
   __syncthreads();
   const int total = sr[H].x;
-  for (int base = threadIdx.x; base < total; base += SB * blockDim.x) {
-    int k[SB];
+  for (int base = threadIdx.x; base < total; base += kStageBatch * blockDim.x) {
+    int k[kStageBatch];
 #pragma unroll
-    for (int u = 0; u < SB; ++u) {
+    for (int u = 0; u < kStageBatch; ++u) {
       const int slot = min(base + u * (int)blockDim.x, total - 1);
       // the last halo cell whose first slot is <= slot (sr[0].x == 0)
       int lo = 0, hi = H - 1;
@@ -1057,28 +1058,28 @@ __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __re
       }
       k[u] = sr[lo].y + (slot - sr[lo].x);
     }
-    double vx[SB], vy[SB], vz[SB];
+    double vx[kStageBatch], vy[kStageBatch], vz[kStageBatch];
 #pragma unroll
-    for (int u = 0; u < SB; ++u) {
+    for (int u = 0; u < kStageBatch; ++u) {
       const double* p = A.x + 3 * (long)k[u];
       vx[u] = p[0];
       vy[u] = p[1];
       vz[u] = p[2];
     }
-    float vw[SB];
-    int vs[SB];
+    float vw[kStageBatch];
+    int vs[kStageBatch];
     if (L.W) {
 #pragma unroll
-      for (int u = 0; u < SB; ++u)
+      for (int u = 0; u < kStageBatch; ++u)
         vw[u] = wsrc[k[u]];
     }
     if (L.S) {
 #pragma unroll
-      for (int u = 0; u < SB; ++u)
+      for (int u = 0; u < kStageBatch; ++u)
         vs[u] = A.z[k[u]];
     }
 #pragma unroll
-    for (int u = 0; u < SB; ++u) {
+    for (int u = 0; u < kStageBatch; ++u) {
       const int slot = base + u * blockDim.x;
       if (slot < total) {
         L.P[3 * slot] = vx[u];
@@ -1088,7 +1089,7 @@ __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __re
     }
     if (L.W) {
 #pragma unroll
-      for (int u = 0; u < SB; ++u) {
+      for (int u = 0; u < kStageBatch; ++u) {
         const int slot = base + u * blockDim.x;
         if (slot < total)
           L.W[slot] = vw[u];
@@ -1096,7 +1097,7 @@ __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __re
     }
     if (L.S) {
 #pragma unroll
-      for (int u = 0; u < SB; ++u) {
+      for (int u = 0; u < kStageBatch; ++u) {
         const int slot = base + u * blockDim.x;
         if (slot < total)
           L.S[slot] = vs[u];
@@ -1109,14 +1110,13 @@ __device__ __forceinline__ void stage_brick(const ForceArgs& A, const int2* __re
   __syncthreads();
 }
 
-// Runs body(i, l, active) over the brick's atoms after staging its halo
-// (SB slots in flight per thread, see stage_brick).
-template <int LPA, int SB = kStageBatch, typename Body>
+// Runs body(i, l, active) over the brick's atoms after staging its halo.
+template <int LPA, typename Body>
 __device__ __forceinline__ void for_brick_atoms(const ForceArgs& A, const int2* bhalo, int brick, int cap,
                                                 int parts, const float* wsrc, Stage& S, Body&& body) {
   const StageLds L = stage_lds(cap, parts);
   int a0, na;
-  stage_brick<SB>(A, bhalo + xcd_block() * (halo_cells(brick) + 1), brick, L, wsrc, a0, na);
+  stage_brick(A, bhalo + xcd_block() * (halo_cells(brick) + 1), brick, L, wsrc, a0, na);
   S = Stage{L.P, L.W, L.S};
   const int per = blockDim.x / LPA;
   const int l = (int)(threadIdx.x % LPA);
@@ -1142,18 +1142,15 @@ static __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))
     write_partials(A, tal, 0.0);
 }
 
-// The staged EAM passes stage 4 halo slots per thread at a time.  The force
-// pass of the kinds whose pair arithmetic fits (Sutton-Chen with compile-time
+// The force pass of the EAM kinds whose pair arithmetic fits (Sutton-Chen with compile-time
 // exponents, Gupta; not the evaluating variants) is held to six waves per SIMD
 // (80 VGPRs, k_eam_force_bk): at its unconstrained 107 VGPRs only one
 // 768-thread workgroup fits a CU, and the 512 bricks of the SC 131k box run in
-// two rounds.  Measured there: force pass 57.7 -> 44.0 us, rho pass 37.6 ->
-// 34.8 us, step 116 -> 99 us; Gupta Cu 131k step 124 -> 110 us.  The other
-// kinds spill in the neighbour loop at 80 VGPRs (Sutton-Chen with run-time
-// exponents: force pass 70 -> 94 us) and keep the unconstrained kernel
-// (k_eam_force_bk_wide).
-constexpr int kEamStageBatch = 4;
-
+// two rounds.  Measured there (with the 4-slot staging): force pass 57.7 ->
+// 44.0 us, rho pass 37.6 -> 34.8 us, step 116 -> 99 us; Gupta Cu 131k step
+// 124 -> 110 us.  The other kinds spill in the neighbour loop at 80 VGPRs
+// (Sutton-Chen with run-time exponents: force pass 70 -> 94 us) and keep the
+// unconstrained kernel (k_eam_force_bk_wide).
 template <int KIND, int EVAL, int EXP> constexpr bool eam_force_capped() {
   return !EVAL && ((KIND == kSuttonChen && EXP != 0) || KIND == kGupta);
 }
@@ -1166,7 +1163,7 @@ static __global__ __launch_bounds__(1024) void k_eam_rho_bk(ForceArgs A, const i
     return;
   double emb = 0.0;
   Stage S;
-  for_brick_atoms<LPA, kEamStageBatch>(A, bhalo, brick, cap, 0, nullptr, S, [&](long i, int l, bool active) {
+  for_brick_atoms<LPA>(A, bhalo, brick, cap, 0, nullptr, S, [&](long i, int l, bool active) {
     eam_rho_atom<KIND, LPA, EVAL, true, true, EXP>(A, S, i, l, active, w_out, rho_out, emb);
   });
   if constexpr (EVAL) {
@@ -1182,7 +1179,7 @@ __device__ __forceinline__ void eam_force_brick(const ForceArgs& A, const int2* 
     return;
   Tally tal;
   Stage S;
-  for_brick_atoms<LPA, kEamStageBatch>(A, bhalo, brick, cap, parts, A.w_rho, S, [&](long i, int l, bool active) {
+  for_brick_atoms<LPA>(A, bhalo, brick, cap, parts, A.w_rho, S, [&](long i, int l, bool active) {
     eam_force_atom<KIND, LPA, EVAL, true, SPC, true, EXP>(A, S, i, l, active, tal);
   });
   if constexpr (EVAL)
