@@ -463,8 +463,10 @@ __device__ __forceinline__ ForceArgs force_view(const BArgs& B, int s, double* x
   A.nterm = S.nterm;
   A.one_term = S.one_term;
   A.b = S.b;
-  for (int q = 0; q < 3; ++q)
+  for (int q = 0; q < 3; ++q) {
     A.ll[q] = S.ll[q];
+    A.margin_hi[q] = S.b.L[q] - S.margin;
+  }
   A.dt = S.dt;
   A.n = S.n;
   A.stride = (long)S.ntile * kTile;

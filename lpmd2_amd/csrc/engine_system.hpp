@@ -1245,6 +1245,8 @@ struct lpmdx_system {
       A.nn = nn.p;
       A.fix = fix_arg();
       A.margin = wrap_margin();
+      for (int q = 0; q < 3; ++q)
+        A.margin_hi[q] = box.L[q] - A.margin;
       if (staged && !(A.margin > 0.0))
         fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
       A.P = device_view(P, rcf, rcr);
@@ -1358,6 +1360,8 @@ struct lpmdx_system {
     A.maxd2 = ctrl.p->maxd2;
     A.xb = xb.p;
     A.margin = wrap_margin();
+    for (int q = 0; q < 3; ++q)
+      A.margin_hi[q] = box.L[q] - A.margin;
     if (staged && !(A.margin > 0.0))
       fail(LPMDX_EDEVICE, "staged force kernels without a wrap margin (internal error)");
     A.P = device_view(P, rcf, rcr);

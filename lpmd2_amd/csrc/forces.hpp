@@ -311,6 +311,9 @@ struct ForceArgs {
   // twice the displacement allowed since the build), or 0.  An atom at least
   // this far from every face has no listed neighbour across the boundary.
   double margin;
+  // L - margin per axis (the interior's upper bounds), set with margin: kernel
+  // arguments, so the interior test holds no per-thread copies
+  double margin_hi[3];
   // per-atom fixed-atom flags (kFixVel, kFixPos; internal order) of a
   // system where some atom carries the fixedvel or fixedpos tag, else null
   const int* fix;
@@ -743,8 +746,8 @@ template <int LPA, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PA
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& H, long i, int l, double xi,
                                                double yi, double zi, Body&& body) {
   const double m = A.margin;
-  const bool inner = m > 0.0 && xi >= m && xi <= A.b.L[0] - m && yi >= m && yi <= A.b.L[1] - m && zi >= m &&
-                     zi <= A.b.L[2] - m;
+  const bool inner = m > 0.0 && xi >= m && xi <= A.margin_hi[0] && yi >= m && yi <= A.margin_hi[1] && zi >= m &&
+                     zi <= A.margin_hi[2];
   if (__all(inner))
     neighbour_loop<LPA, kWrapNone, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
   else if (STAGED)
