@@ -220,6 +220,19 @@ static __global__ __launch_bounds__(kHaloThreads) void k_brick_halo(const int* _
   }
 }
 
+// FP32 copy of a position folded into [0, L) (in FP64 first), the frame the
+// periodic binning uses.
+__device__ __forceinline__ float4 fold_f32(const double* x, const Box& b) {
+  float w[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double v = x[q];
+    v -= b.L[q] * floor(v / b.L[q]);
+    w[q] = (float)v;
+  }
+  return make_float4(w[0], w[1], w[2], 0.0f);
+}
+
 // Reorders the per-atom state into the new (cell-sorted) order.  order[k] is
 // the current slot of the atom placed at k; perm maps slots to caller indices.
 constexpr int kPermuteExtra = 6;
@@ -248,6 +261,11 @@ struct PermuteArgs {
   // per-atom fixed-atom flags moved along (null: none)
   const int* fix;
   int* fix2;
+  // the list build's inputs written on the way, or null: the build positions
+  // (a copy of x2) and the FP32 copy of x2 folded into the box (k_to_f32's)
+  double* xb;
+  float4* xf;
+  Box b;
 };
 
 // Every load is issued before the first store (the output arrays are distinct
@@ -279,6 +297,13 @@ static __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs A) {
     A.v2[3 * k + q] = v[q];
     A.a2[3 * k + q] = a[q];
   }
+  if (A.xb) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      A.xb[3 * k + q] = x[q];
+  }
+  if (A.xf)
+    A.xf[k] = fold_f32(x, A.b);
   A.z2[k] = z;
   A.ffm2[k] = ffm;
   A.mass2[k] = mass;
@@ -343,14 +368,20 @@ static __global__ __launch_bounds__(kBlock) void k_to_f32(const double* __restri
   long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
     return;
-  float w[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    double v = x[3 * i + q];
-    v -= b.L[q] * floor(v / b.L[q]);
-    w[q] = (float)v;
-  }
-  xf[i] = make_float4(w[0], w[1], w[2], 0.0f);
+  const double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+  xf[i] = fold_f32(p, b);
+}
+
+// Zeroes a build's counters: max_count, pad0 and sum_count (16 bytes from
+// max_count), with halo_max and brick_max behind them when `halo`, and the
+// spread sum slots (one launch in place of three memsets).
+static __global__ __launch_bounds__(kBlock) void k_build_reset(unsigned int* __restrict__ ctrl_words, int halo,
+                                                               unsigned long long* __restrict__ slots, int m) {
+  const int t = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (t < (halo ? 6 : 4))
+    ctrl_words[t] = 0u;
+  if (t < m)
+    slots[t] = 0ull;
 }
 
 // One atom's list being written.  Accepted entries wait in a ring of kRing

@@ -879,6 +879,11 @@ struct lpmdx_system {
       P.ex[e] = ex[e].first->p;
       P.ex2[e] = ex[e].second->p;
     }
+    // the list build's inputs: the build positions and their FP32 copy
+    xf.alloc(n + kGroup); // padding read by the candidate groups
+    P.xb = xb.p;
+    P.xf = xf.p;
+    P.b = box;
     k_permute<<<nb, kBlock, 0, st>>>(P);
     launch_check();
     for (auto& e : ex)
@@ -911,8 +916,10 @@ struct lpmdx_system {
     const size_t np = (size_t)std::max<long>(blocks_for(n * 16), nbrick) * 12;
     partials.alloc(np);
     partials2.alloc(np);
-    // halo_max, brick_max
-    hipchk(hipMemsetAsync(&ctrl.p->halo_max, 0, 8, st), "memset");
+    // the build's counters, halo_max and brick_max among them (k_brick_halo's)
+    sumslots.alloc((size_t)kSumSlots * kSlotStride);
+    k_build_reset<<<blocks_for(kSumSlots * kSlotStride), kBlock, 0, st>>>(&ctrl.p->max_count, 1, sumslots.p,
+                                                                          kSumSlots * kSlotStride);
     if (staged) {
       bhalo.alloc((size_t)nbrick * (halo_cells(grid.brick) + 1));
       k_brick_halo<<<nbrick, kHaloThreads, 0, st>>>(cb.p, grid, bhalo.p, &ctrl.p->halo_max, &ctrl.p->brick_max);
@@ -930,15 +937,16 @@ struct lpmdx_system {
   }
   // One build with the current capacity, ending with the copy of its control
   // words (max_count, pad0, sum_count, halo_max, brick_max) to the host; no
-  // host synchronisation.
-  void build_attempt() {
+  // host synchronisation.  reset: clear the counters again (a repeated
+  // attempt; build_prepare cleared them for the first).  fresh_xf: the sort
+  // just wrote the FP32 copy of the positions (k_permute).
+  void build_attempt(bool reset, bool fresh_xf) {
     cap = ((cap + kChunk - 1) / kChunk) * kChunk; // whole 8-entry chunks
     nbr.alloc((size_t)cap * stride);
     cnt.alloc(n);
-    // max_count, pad0, sum_count: 16 contiguous bytes
-    hipchk(hipMemsetAsync(&ctrl.p->max_count, 0, 16, st), "memset");
-    sumslots.alloc((size_t)kSumSlots * kSlotStride);
-    hipchk(hipMemsetAsync(sumslots.p, 0, (size_t)kSumSlots * kSlotStride * sizeof(unsigned long long), st), "memset");
+    if (reset)
+      k_build_reset<<<blocks_for(kSumSlots * kSlotStride), kBlock, 0, st>>>(&ctrl.p->max_count, 0, sumslots.p,
+                                                                            kSumSlots * kSlotStride);
     const double rl2 = rl * rl;
     // FP32 positions are off by at most ulp(Lmax)/2; two of them in a distance
     const double lmax = std::max(box.L[0], std::max(box.L[1], box.L[2]));
@@ -954,8 +962,10 @@ struct lpmdx_system {
       else
         path = kPathMin32;
     }
-    xf.alloc(n + kGroup); // padding read by the candidate groups
-    k_to_f32<<<blocks_for(n), kBlock, 0, st>>>(x.p, n, box, xf.p);
+    if (!fresh_xf) {
+      xf.alloc(n + kGroup); // padding read by the candidate groups
+      k_to_f32<<<blocks_for(n), kBlock, 0, st>>>(x.p, n, box, xf.p);
+    }
     BuildArgs B{};
     B.x = x.p;
     B.xf = xf.p;
@@ -1025,10 +1035,11 @@ struct lpmdx_system {
     return false;
   }
 
-  void build_list() {
+  // fresh_xf: see build_attempt
+  void build_list(bool fresh_xf = false) {
     build_prepare();
     for (int attempt = 0; attempt < 4; ++attempt) {
-      build_attempt();
+      build_attempt(attempt > 0, fresh_xf);
       sync();
       if (build_check())
         return;
@@ -1056,18 +1067,18 @@ struct lpmdx_system {
       sort_atoms();
     }
     prof_end(kProfBuild, e0);
-    build_list();
+    // the sort wrote the build positions xb and the FP32 copy the build reads
+    build_list(true);
     // verletlist.cc:96-97: a list may hold at most maxneighbors-1 entries
     if (cm.kind == kCmVerlet && max_list >= cm.maxneighbors)
       fail(LPMDX_ECAPACITY, "verletlist: maxneighbors is too small!");
-    hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
     list_built = true;
     n_builds++;
   }
 
   // rebuild() enqueued behind work still running, with no host
-  // synchronisation: the sort, one build attempt at the current capacity and
-  // the copy of the build positions.  rebuild_settle() finishes it once the
+  // synchronisation: the sort (which writes the build positions) and one
+  // build attempt at the current capacity.  rebuild_settle() finishes it once the
   // stream has synchronised.  Not for lcbinary, whose grid growth needs the
   // host between sorts.
   bool can_rebuild_ahead() const { return n > 0 && cm.kind != kCmLcBinary && list_built && cap > 0; }
@@ -1078,8 +1089,7 @@ struct lpmdx_system {
     sort_atoms();
     prof_end(kProfBuild, e0);
     build_prepare();
-    build_attempt();
-    hipchk(hipMemcpyAsync(xb.p, x.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st), "copy");
+    build_attempt(false, true);
   }
   // After the synchronisation: the list of rebuild_ahead() is kept when it fit
   // its capacity; otherwise the build runs again at the same positions with
