@@ -3,8 +3,9 @@
 // HIP graph that replays one step of fixed launches (see batch.hpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
-
+#include <map>
 #include <set>
 
 #include "batch.hpp"
@@ -162,7 +163,7 @@ struct lpmdx_batch {
   DBuf<uint32_t> hist;
   int nchunks = 0, passes = 1;
   struct Group {
-    int kind, lpa, tpw;
+    int kind, exp, lpa, tpw; // exp: batch_exp of its systems
     bool metal;
     DBuf<int> tiles;
     long n = 0;
@@ -567,7 +568,7 @@ struct lpmdx_batch {
       hipchk(hipMemcpyAsync(dsys.p, hsys.data(), hsys.size() * sizeof(BSys), hipMemcpyHostToDevice, st), "memcpy");
     // tiles
     std::vector<int> ts(acap / kTile, 0), all;
-    std::map<int, std::vector<int>> by_kind;
+    std::map<std::pair<int, int>, std::vector<int>> by_kind; // (kind, batch_exp)
     for (size_t s = 0; s < hsys.size(); ++s) {
       if (!meta[s].live)
         continue;
@@ -575,7 +576,7 @@ struct lpmdx_batch {
       for (long t = 0; t < meta[s].tiles; ++t) {
         ts[t0 + t] = (int)s;
         all.push_back((int)(t0 + t));
-        by_kind[hsys[s].kind].push_back((int)(t0 + t));
+        by_kind[{hsys[s].kind, batch_exp(hsys[s])}].push_back((int)(t0 + t));
       }
     }
     hipchk(hipMemcpyAsync(tsys.p, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, st), "memcpy");
@@ -586,10 +587,11 @@ struct lpmdx_batch {
     groups.clear();
     for (auto& kv : by_kind) {
       auto G = std::make_unique<Group>();
-      G->kind = kv.first;
-      G->lpa = lpa_of_kind[kv.first];
-      G->tpw = tpw_of_kind[kv.first];
-      G->metal = metal_kind(kv.first);
+      G->kind = kv.first.first;
+      G->exp = kv.first.second;
+      G->lpa = lpa_of_kind[G->kind];
+      G->tpw = tpw_of_kind[G->kind];
+      G->metal = metal_kind(G->kind);
       G->n = (long)kv.second.size();
       G->tiles.alloc(kv.second.size());
       hipchk(hipMemcpyAsync(G->tiles.p, kv.second.data(), kv.second.size() * sizeof(int), hipMemcpyHostToDevice, st),
@@ -605,25 +607,25 @@ struct lpmdx_batch {
     dirty = false;
   }
 
-  template <int KIND> void launch_group(Group& G, hipStream_t s) {
+  template <int KIND, int EXP = 0> void launch_group(Group& G, hipStream_t s) {
     BArgs B = args(G.tiles.p, G.n);
     const int T = G.tpw;
     const long nb = (G.n + T - 1) / T;
     switch (G.lpa) {
     case 1:
       if (G.metal)
-        k_b_rho<KIND, 1><<<nb, T * kWave, 0, s>>>(B);
-      k_b_force<KIND, 1><<<nb, T * kWave, 0, s>>>(B);
+        k_b_rho<KIND, 1, EXP><<<nb, T * kWave, 0, s>>>(B);
+      k_b_force<KIND, 1, EXP><<<nb, T * kWave, 0, s>>>(B);
       break;
     case 2:
       if (G.metal)
-        k_b_rho<KIND, 2><<<nb, T * 2 * kWave, 0, s>>>(B);
-      k_b_force<KIND, 2><<<nb, T * 2 * kWave, 0, s>>>(B);
+        k_b_rho<KIND, 2, EXP><<<nb, T * 2 * kWave, 0, s>>>(B);
+      k_b_force<KIND, 2, EXP><<<nb, T * 2 * kWave, 0, s>>>(B);
       break;
     default:
       if (G.metal)
-        k_b_rho<KIND, 4><<<nb, T * 4 * kWave, 0, s>>>(B);
-      k_b_force<KIND, 4><<<nb, T * 4 * kWave, 0, s>>>(B);
+        k_b_rho<KIND, 4, EXP><<<nb, T * 4 * kWave, 0, s>>>(B);
+      k_b_force<KIND, 4, EXP><<<nb, T * 4 * kWave, 0, s>>>(B);
       break;
     }
   }
@@ -649,7 +651,14 @@ struct lpmdx_batch {
     case kFastLJ: launch_group<kFastLJ>(G, s); break;
     case kMorse: launch_group<kMorse>(G, s); break;
     case kBuckingham: launch_group<kBuckingham>(G, s); break;
-    case kSuttonChen: launch_group<kSuttonChen>(G, s); break;
+    case kSuttonChen:
+      if (G.exp == sc_exp(9, 6))
+        launch_group<kSuttonChen, sc_exp(9, 6)>(G, s);
+      else if (G.exp == sc_exp(12, 6))
+        launch_group<kSuttonChen, sc_exp(12, 6)>(G, s);
+      else
+        launch_group<kSuttonChen>(G, s);
+      break;
     case kFinnisSinclair: launch_group<kFinnisSinclair>(G, s); break;
     case kGupta: launch_group<kGupta>(G, s); break;
     case kHarmonic: launch_group<kHarmonic>(G, s); break;
@@ -661,7 +670,7 @@ struct lpmdx_batch {
   void enqueue_step() {
     const int ns = (int)hsys.size();
     BArgs B = args(tiles_all.p, n_all);
-    k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B, 0, -1);
+    k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B, 0, -1, 0);
     if (nchunks) {
       for (int p = 0; p < passes; ++p) {
         k_b_count<<<nchunks, kBlock, 0, st>>>(B, p);
@@ -702,7 +711,7 @@ struct lpmdx_batch {
     BArgs B = args(tiles_all.p, n_all);
     if (groups.size() == 1) {
       for (int k = 0; k < nlite; ++k) {
-        k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B, 1, groups[0]->kind);
+        k_b_control<<<blocks_for(ns), kBlock, 0, st>>>(B, 1, groups[0]->kind, groups[0]->exp);
         launch_kind(*groups[0], st);
       }
     } else if (groups.size() > 1) {
@@ -712,7 +721,7 @@ struct lpmdx_batch {
       for (size_t g = 0; g < groups.size(); ++g) {
         hipchk(hipStreamWaitEvent(bst[g], ev_fork, 0), "hipStreamWaitEvent");
         for (int k = 0; k < nlite; ++k) {
-          k_b_control<<<blocks_for(ns), kBlock, 0, bst[g]>>>(B, 1, groups[g]->kind);
+          k_b_control<<<blocks_for(ns), kBlock, 0, bst[g]>>>(B, 1, groups[g]->kind, groups[g]->exp);
           launch_kind(*groups[g], bst[g]);
         }
         hipchk(hipEventRecord(ev_join[g], bst[g]), "hipEventRecord");
@@ -981,9 +990,10 @@ struct lpmdx_batch {
       hipchk(hipMemcpyAsync(dst.p, hst, hsys.size() * sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
       sync();
     };
-    std::vector<int> kinds;
+    std::vector<int> kinds; // each kind once (its exponent groups share its shapes)
     for (auto& G : groups)
-      kinds.push_back(G->kind);
+      if (std::find(kinds.begin(), kinds.end(), G->kind) == kinds.end())
+        kinds.push_back(G->kind);
     for (int kind : kinds) {
       double best = 1e300;
       int best_lpa = lpa_of_kind[kind], best_div = div_of_kind[kind];

@@ -64,6 +64,18 @@ struct BSys {
   DevPot P[kMaxTerms];
 };
 
+// Sutton-Chen systems with the common integer exponent pairs, (9, 6) and
+// (12, 6), run force and rho kernels with the exponents fixed at compile time,
+// as the single-system engine does (bit-identical results, about 20 fewer
+// instructions per neighbour): the systems of a kind are grouped by this
+// key, sc_exp(n, m) for those and 0 for every other system.
+__host__ __device__ inline int batch_exp(const BSys& S) {
+  if (S.kind != kSuttonChen || S.P[0].ni <= 0 || S.P[0].mi <= 0)
+    return 0;
+  const int ex = sc_exp(S.P[0].ni, S.P[0].mi);
+  return (ex == sc_exp(9, 6) || ex == sc_exp(12, 6)) ? ex : 0;
+}
+
 // Mutable per-system state on the device.
 struct BState {
   double disp;        // largest displacement from the build positions
@@ -128,14 +140,15 @@ struct BArgs {
 // list has run out (or that still waits for its first list) holds: it does
 // not advance and keeps its displacement maximum, so the next full step
 // reaches the same rebuild decision.  kind >= 0: only the systems of that
-// potential kind (the lite steps run on one branch per kind).  A system
-// advances only while its quota of steps for the current run lasts.
-static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B, int lite, int kind) {
+// potential kind and exponent key (batch_exp; the lite steps run on one
+// branch per group).  A system advances only while its quota of steps for the
+// current run lasts.
+static __global__ __launch_bounds__(kBlock) void k_b_control(BArgs B, int lite, int kind, int exp) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= B.nsys)
     return;
   const BSys& S = B.sys[s];
-  if (kind >= 0 && S.kind != kind)
+  if (kind >= 0 && (S.kind != kind || batch_exp(S) != exp))
     return;
   BState& st = B.st[s];
   // the last step's drift moved the positions into the other buffer
@@ -535,7 +548,7 @@ template <int LPA> __device__ __forceinline__ TileLane tile_lane() {
   return TileLane{xcd_block() * (long)(blockDim.x / per) + threadIdx.x / per, (int)(threadIdx.x % per)};
 }
 
-template <int KIND, int LPA>
+template <int KIND, int LPA, int EXP = 0>
 static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_rho(BArgs B) {
   const TileLane tw = tile_lane<LPA>();
   if (tw.tl >= B.ntiles)
@@ -548,10 +561,10 @@ static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_rh
   const long i = (long)tile * kTile - B.sys[s].a0 + tw.r / LPA;
   const int l = tw.r % LPA;
   double emb = 0.0;
-  eam_rho_atom<KIND, LPA, false, false, false>(A, Stage{}, i, l, i < A.n, B.R.w + B.sys[s].a0, nullptr, emb);
+  eam_rho_atom<KIND, LPA, false, false, false, EXP>(A, Stage{}, i, l, i < A.n, B.R.w + B.sys[s].a0, nullptr, emb);
 }
 
-template <int KIND, int LPA>
+template <int KIND, int LPA, int EXP = 0>
 static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_force(BArgs B) {
   const TileLane tw = tile_lane<LPA>();
   if (tw.tl >= B.ntiles)
@@ -568,9 +581,9 @@ static __global__ __launch_bounds__(kWave * LPA * kMaxTilesPerGroup) void k_b_fo
   // branch is uniform: one workgroup, one system)
   if constexpr (metal_kind(KIND)) {
     if (A.P.spc_sum >= 0)
-      eam_force_atom<KIND, LPA, false, false, true, false>(A, Stage{}, i, l, i < A.n, tal);
+      eam_force_atom<KIND, LPA, false, false, true, false, EXP>(A, Stage{}, i, l, i < A.n, tal);
     else
-      eam_force_atom<KIND, LPA, false, false, false, false>(A, Stage{}, i, l, i < A.n, tal);
+      eam_force_atom<KIND, LPA, false, false, false, false, EXP>(A, Stage{}, i, l, i < A.n, tal);
   } else {
     if (A.nterm > 1 || A.P.spc_sum >= 0)
       pair_atom<KIND, LPA, false, false, true>(A, Stage{}, i, l, i < A.n, tal);
