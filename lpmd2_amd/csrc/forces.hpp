@@ -619,10 +619,11 @@ template <int LPA> __device__ __forceinline__ int4 list_batch(const int* __restr
   return *reinterpret_cast<const int4*>(nb + (long)(k0 / kChunk) * (T * kChunk) + (k0 % kChunk));
 }
 
-// Minimum image of the neighbour loop: none (no listed pair crosses a face),
-// the rint form for displacements clear of half an edge, or the reference's
-// compare-and-select form.
-enum WrapMode { kWrapNone = 0, kWrapNear = 1, kWrapSelect = 2 };
+// Minimum image of the neighbour loop: the rint form on the axes of a bit
+// mask (kWrapX | kWrapY | kWrapZ; none: no listed pair crosses a face), for
+// displacements clear of half an edge, or the reference's compare-and-select
+// form on every axis.
+enum WrapMode { kWrapNone = 0, kWrapX = 1, kWrapY = 2, kWrapZ = 4, kWrapNear = 7, kWrapSelect = 8 };
 
 template <int LPA, int WRAP, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED,
           typename Body>
@@ -710,10 +711,16 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
       if constexpr (NEED_W)
         wj = STAGED ? H.W[j] : (PACKED ? pw[u] : A.w_rho[j]);
       double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
-      if constexpr (WRAP == kWrapNear)
-        min_image_near(A.b, dx, dy, dz);
-      else if constexpr (WRAP == kWrapSelect)
+      if constexpr (WRAP == kWrapSelect) {
         min_image(A.b, dx, dy, dz);
+      } else {
+        if constexpr ((WRAP & kWrapX) != 0)
+          dx = wrap1_near(dx, A.b.L[0], A.b.inv[0]);
+        if constexpr ((WRAP & kWrapY) != 0)
+          dy = wrap1_near(dy, A.b.L[1], A.b.inv[1]);
+        if constexpr ((WRAP & kWrapZ) != 0)
+          dz = wrap1_near(dz, A.b.L[2], A.b.inv[2]);
+      }
       body(FULL || k0 + u < c, sj, wj, dx, dy, dz, sqmod(dx, dy, dz));
     }
   };
@@ -736,24 +743,39 @@ __device__ __forceinline__ void neighbour_loop(const ForceArgs& A, const Stage& 
 // bounds every listed pair's separation below half an edge: where no atom of
 // the wavefront is within it of a face, no displacement can reach half an
 // edge and the minimum image is the identity (OrthogonalCell::Displacement
-// would not wrap).  Elsewhere the staged kernels, which run only with a
-// margin, take the rint wrap (every raw displacement lies within the margin
-// of 0 or +-L, where it gives the reference's result; measured 16 % faster
-// for the staged LJ kernel, whose VGPR count it also lowers) and the others
-// the compare-and-select wrap (measured faster there: 7 % for the unstaged
-// EAM rho pass).  Each branch is uniform over the wavefront.
+// would not wrap).  The same holds axis by axis: the staged kernels, which
+// run only with a margin, take the rint wrap (every raw displacement lies
+// within the margin of 0 or +-L, where it gives the reference's result;
+// measured 16 % faster for the staged LJ kernel, whose VGPR count it also
+// lowers) on the one axis whose faces some atom of the wavefront is near, or
+// on all three; the others take the compare-and-select wrap on every axis
+// (measured faster there: 7 % for the unstaged EAM rho pass).  Each branch is
+// uniform over the wavefront.
 template <int LPA, bool STAGED, bool NEED_S, bool NEED_W, bool PREFETCH, bool PACKED = false, typename Body>
 __device__ __forceinline__ void for_neighbours(const ForceArgs& A, const Stage& H, long i, int l, double xi,
                                                double yi, double zi, Body&& body) {
   const double m = A.margin;
-  const bool inner = m > 0.0 && xi >= m && xi <= A.margin_hi[0] && yi >= m && yi <= A.margin_hi[1] && zi >= m &&
-                     zi <= A.margin_hi[2];
-  if (__all(inner))
-    neighbour_loop<LPA, kWrapNone, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
-  else if (STAGED)
-    neighbour_loop<LPA, kWrapNear, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
-  else
-    neighbour_loop<LPA, kWrapSelect, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+  const bool in_x = m > 0.0 && xi >= m && xi <= A.margin_hi[0];
+  const bool in_y = m > 0.0 && yi >= m && yi <= A.margin_hi[1];
+  const bool in_z = m > 0.0 && zi >= m && zi <= A.margin_hi[2];
+  if constexpr (STAGED) {
+    const int near = (__all(in_x) ? 0 : kWrapX) | (__all(in_y) ? 0 : kWrapY) | (__all(in_z) ? 0 : kWrapZ);
+    if (near == kWrapNone)
+      neighbour_loop<LPA, kWrapNone, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    else if (near == kWrapX)
+      neighbour_loop<LPA, kWrapX, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    else if (near == kWrapY)
+      neighbour_loop<LPA, kWrapY, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    else if (near == kWrapZ)
+      neighbour_loop<LPA, kWrapZ, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    else
+      neighbour_loop<LPA, kWrapNear, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+  } else {
+    if (__all(in_x && in_y && in_z))
+      neighbour_loop<LPA, kWrapNone, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+    else
+      neighbour_loop<LPA, kWrapSelect, STAGED, NEED_S, NEED_W, PREFETCH, PACKED>(A, H, i, l, xi, yi, zi, body);
+  }
 }
 
 // Per-atom force: FP32 pair scalar times the FP64 displacement, summed in FP64
