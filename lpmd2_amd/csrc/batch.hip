@@ -185,6 +185,7 @@ struct lpmdx_batch {
   hipGraphExec_t gexec = nullptr, cexec = nullptr;
   int cyc = 4;
   bool dirty = true;
+  long dead_tiles = 0; // tiles of systems removed since the last plan
   // evaluation scratch
   DBuf<double> evala, partials, red;
   double* hred = nullptr;
@@ -496,6 +497,7 @@ struct lpmdx_batch {
   void remove(long sid) {
     check_sid(sid);
     HostSys& H = meta[sid];
+    const long gone_tiles = H.tiles;
     atoms.release(H.atom0 / kTile, H.tiles);
     lists.release(H.list0, H.listw);
     cells.release(H.cell0, H.cells);
@@ -503,8 +505,25 @@ struct lpmdx_batch {
     hsys[sid].live = 0;
     hst[sid] = BState{};
     hipchk(hipMemcpyAsync(dst.p + sid, &hst[sid], sizeof(BState), hipMemcpyHostToDevice, st), "memcpy");
+    // A system of the current plan may leave it without a new plan: its
+    // device table entry says it is gone, so the control kernel keeps it
+    // inactive and every kernel of the captured steps skips its tiles, sort
+    // chunks and cells (each checks live, or act, first).  Its slices are free
+    // for the next add, which plans again before any step runs.  That saves
+    // the re-plan and the two graph instantiations (about 1.5 ms), but every
+    // step still launches the gone systems' empty workgroups, so the batch
+    // plans again once they pass an eighth of the live tiles (at once when
+    // few systems are left).
+    long live_tiles = 0;
+    for (const HostSys& m : meta)
+      if (m.live)
+        live_tiles += m.tiles;
+    dead_tiles += gone_tiles;
+    if (!dirty && (size_t)sid < dsys.n && dead_tiles * 8 <= live_tiles)
+      hipchk(hipMemcpyAsync(dsys.p + sid, &hsys[sid], sizeof(BSys), hipMemcpyHostToDevice, st), "memcpy");
+    else
+      dirty = true;
     sync();
-    dirty = true;
   }
 
   void check_sid(long sid) const {
@@ -520,6 +539,7 @@ struct lpmdx_batch {
     // graph can still be in flight when it is destroyed.
     sync();
     release_graph();
+    dead_tiles = 0;
     // system table and sort chunks
     // systems whose kind changed its cells per list radius get the new grid
     // and build their lists again at their next step

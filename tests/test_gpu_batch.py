@@ -84,6 +84,40 @@ def test_alone_and_mixed_are_identical():
     np.testing.assert_array_equal(a1, a2)
 
 
+def test_remove_mid_run_keeps_the_others():
+    """A system removed between runs leaves the captured steps in place (no
+    new plan): the systems still running end exactly as they do alone, and a
+    system added afterwards into the freed slices runs as it does alone."""
+    say("remove mid-run")
+    # 2048 atoms stay, 256 leave: the gone tiles (4) are at most an eighth of
+    # the live ones (32), so the removal keeps the captured plan
+    a_spec = system("sc", 8, 31, 500.0)
+    gone = system("gupta", 4, 32, 900.0)
+    c_spec = system("lj", 4, 33, 150.0)
+    alone = {}
+    for name, spec in (("a", a_spec), ("c", c_spec)):
+        b = Batch(atom_capacity=4096)
+        s = b.add(steps=50, **spec)
+        b.run(60)
+        alone[name] = b.get(s)
+    b = Batch(atom_capacity=4096)
+    sa = b.add(steps=50, **a_spec)
+    sg = b.add(steps=200, **gone)
+    b.run(12)
+    b.remove(sg)
+    b.run(13)
+    sc = b.add(steps=50, **c_spec)
+    b.run(60)
+    for name, sid in (("a", sa), ("c", sc)):
+        x, v, a, info = b.get(sid)
+        x0, v0, a0, i0 = alone[name]
+        say(f"  {name}: done {info['done']} alone {i0['done']}, builds {info['builds']} alone {i0['builds']}")
+        assert info["done"] == i0["done"] == 50
+        np.testing.assert_array_equal(x, x0)
+        np.testing.assert_array_equal(v, v0)
+        np.testing.assert_array_equal(a, a0)
+
+
 def test_batch_matches_plain_path():
     say("batch vs plain")
     s = system("lj", 4, 11, 100.0)
