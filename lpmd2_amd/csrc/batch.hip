@@ -825,8 +825,17 @@ struct lpmdx_batch {
   // cyc - 1 lite steps; systems that held in a lite step are owed steps, and
   // more cycles run until every quota is spent.
   void run(long steps) {
-    if (dirty)
+    if (dirty) {
+      // the cycle length first, so that the plan's capture is the only one
+      // of this run
+      if (!hsys.empty()) {
+        pull_state();
+        const int want = cycle_length();
+        if (want < cyc || want >= cyc + 2)
+          cyc = want;
+      }
       plan();
+    }
     if (hsys.empty())
       return;
     pull_state();
